@@ -1,0 +1,149 @@
+/*
+ * hops_mi355x.h — C ABI of the MI355X HOPS propagation engine.
+ *
+ * All pointers named *_d are device pointers (HBM) owned by the caller; the
+ * library never frees caller memory.  Host pointers are copied during the call.
+ * Every entry point returns 0 on success and a negative status on failure;
+ * hops_last_error() returns a thread-local message for the last failure.
+ *
+ * Reference interfaces replaced (MesoHOPS v1.6.0, src/mesohops/...):
+ *   hops_build_super       eom/eom_hops_ksuper.py:575-668 calculate_ksuper (and
+ *                          :387-572 update_ksuper: the device rebuilds the
+ *                          operator of the new basis instead of permuting it)
+ *   hops_propagate         trajectory/hops_trajectory.py:456-533 (time loop),
+ *                          integrator/integrator_rk.py:10-134 (RK4 + noise
+ *                          lookup), eom/hops_eom.py:196-542 (dsystem_dt),
+ *                          eom/eom_functions.py:11-309, and
+ *                          trajectory/hops_trajectory.py:616-634 (normalize)
+ *   hops_noise_filter      noise/hops_noise.py:630-716 _construct_correlated_noise
+ *   hops_noise_embedding   noise/hops_noise.py:644-667 (circulant embedding
+ *                          spectrum of the correlation function)
+ */
+#ifndef HOPS_MI355X_H
+#define HOPS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision modes of hops_propagate */
+#define HOPS_PREC_FP64 0  /* complex128 stages, complex128 state          */
+#define HOPS_PREC_MIXED 1 /* complex64 stages/SpMV, complex128 state carry */
+
+/* equation-of-motion kinds (eom/hops_eom.py:95-110) */
+#define HOPS_EOM_LINEAR 0
+#define HOPS_EOM_NONLINEAR 1
+#define HOPS_EOM_NORMALIZED_NONLINEAR 2
+
+/*
+ * Operator of one basis (one state list x one auxiliary list).  Built by the
+ * host from the auxiliary multi-indices; the ELL super-operator arrays are
+ * filled on the device by hops_build_super.  Complex values are interleaved
+ * (re, im) doubles unless noted.  Row r of the hierarchy vector is
+ * r = i_aux * n_state + i_state (eom/hops_eom.py:386-391).
+ */
+typedef struct hops_op {
+  int32_t n_state, n_hier, n_mode, n_lop, n_act, n_first, ell_w, n_pat;
+  int32_t n_mode_abs; /* number of absolute hierarchy modes (z_mem length) */
+  int32_t pad0;
+  /* auxiliary multi-indices, CSR over auxiliaries: absolute mode, depth */
+  const int32_t *aux_ptr_d, *aux_mode_d, *aux_val_d;
+  /* connections: up (k -> k+e_m) and down (k -> k-e_m, with k_m of aux k) */
+  const int32_t *up_ptr_d, *up_aux_d, *up_mode_d;
+  const int32_t *dn_ptr_d, *dn_aux_d, *dn_mode_d, *dn_val_d;
+  /* relative L-operators: row CSR and column CSR, [n_lop*(n_state+1)] ptrs */
+  const int32_t *lr_ptr_d, *lr_col_d;
+  const double *lr_val_d;
+  const int32_t *lc_ptr_d, *lc_row_d;
+  const double *lc_val_d;
+  /* relative modes: L index (relative), active L index or -1, absolute mode,
+     g and w (complex) */
+  const int32_t *mode_lrel_d, *mode_lact_d, *mode_abs_d;
+  const double *mode_g_d, *mode_w_d;
+  /* absolute modes: g, w and the active L index of the mode (-2: mode not
+     in the basis, -1: in the basis with an inactive L-operator) */
+  const double *abs_g_d, *abs_w_d;
+  const int32_t *abs_lact_d;
+  /* active L-operators: relative index, relative modes sharing the operator
+     (CSR over active L), COO of L and of L^2, LTC coefficient */
+  const int32_t *act_lrel_d;
+  const int32_t *am_ptr_d, *am_mode_d;
+  const int32_t *la_ptr_d, *la_row_d, *la_col_d;
+  const double *la_val_d;
+  const int32_t *lq_ptr_d, *lq_row_d, *lq_col_d;
+  const double *lq_val_d;
+  const double *lt_d;
+  /* first-level auxiliaries e_m: aux index, active L index, relative mode */
+  const int32_t *first_aux_d, *first_lact_d, *first_mode_d;
+  /* system-block pattern (union of H, L and L^2) over the state rows */
+  const int32_t *pat_ptr_d, *pat_col_d;
+  const double *pat_h_d; /* -iH at each pattern entry */
+  const int32_t *pc_ptr_d, *pc_act_d, *pc_kind_d;
+  const double *pc_val_d;
+  /* outputs of hops_build_super (caller-allocated) */
+  int32_t *ell_col_d; /* [ell_w * n_rows], column-major            */
+  void *ell_val_d;    /* [ell_w * n_rows] complex, fp64 or fp32     */
+  double *k0_d;       /* [n_hier] complex: -sum_m k_m w_m           */
+} hops_op_t;
+
+/*
+ * Batch of B trajectories sharing one operator (one basis).  Pools are laid
+ * out trajectory-major: trajectory b owns rows [b*n_rows, (b+1)*n_rows).
+ */
+typedef struct hops_batch {
+  int32_t B, precision, eom, n_t;  /* n_t = noise time points           */
+  int32_t coef_stride, ws_stride;  /* per-trajectory doubles            */
+  int32_t store_stride, pad0;      /* psi slots per trajectory          */
+  double dt_over_hbar;             /* tau / hbar                        */
+  int32_t noise_step, pad1;        /* noise points per tau/2            */
+  void *phi_d;   /* complex128 [B*n_rows] state carry                   */
+  void *y0_d;    /* complex64 [B*n_rows] (mixed only)                   */
+  void *ya_d;    /* stage input ping   (complex R)                      */
+  void *yb_d;    /* stage input pong   (complex R)                      */
+  void *acc_d;   /* RK4 accumulator    (complex R)                      */
+  const void *z1_d; /* complex64 [B][n_act][n_t] noise 1 (active rows)  */
+  const void *z2_d; /* complex64 [B][n_act][n_t] noise 2, may be NULL   */
+  double *zmem_d;   /* complex128 [B][n_mode_abs] noise memory          */
+  double *ws_d;     /* per-trajectory workspace, ws_stride doubles      */
+  double *coef_d;   /* per-trajectory stage coefficients                */
+  double *psi_store_d; /* complex128 [B][store_stride][n_state] or NULL */
+  const int32_t *tile_d; /* [n_tiles] trajectory*65536+tile, or -1 idle */
+  int32_t n_tiles, tile_rows;
+} hops_batch_t;
+
+const char *hops_last_error(void);
+
+/* ABI version of this library (bumped on any change of the structs). */
+int hops_version(void);
+
+/* Fill op->ell_col_d/ell_val_d/k0_d from the connection tables.
+   precision selects the ELL value type (complex128 or complex64). */
+int hops_build_super(const hops_op_t *op, int precision, void *stream);
+
+/* Advance every trajectory of the batch by n_steps RK4 steps of size tau
+   starting at noise index it0 (time t0 = it0 * TAU), storing psi after each
+   step into psi_store_d slot (store0 + step) when psi_store_d != NULL.
+   On return phi_d holds the (normalised, for NORMALIZED NONLINEAR) state. */
+int hops_propagate(const hops_op_t *op, const hops_batch_t *batch, int it0,
+                   int n_steps, int store0, void *stream);
+
+/* Circulant-embedding spectrum s_w[l][j] = max(0, Re FFT(embed(tilde c_l)))
+   for n_lop correlation functions of n_t points (FFT length 2(n_t-1)).
+   c_d: complex64 [n_lop][n_t]; s_w_d: double [n_lop][2(n_t-1)];
+   nu_d: double [n_lop] rotation frequency of each row. */
+int hops_noise_embedding(const void *c_d, int n_lop, int n_t, double *s_w_d,
+                         double *nu_d, void *stream);
+
+/* Correlated noise for R rows: out[r] = c64( unrotate( IFFT( FFT(z[r]) *
+   sqrt(s_w[lop(r)]/2) )[:n_t] ) ).  z_d: complex64 [R][2(n_t-1)];
+   row_lop_d: int32 [R] row -> embedding row; out_d: complex64 [R][n_t]. */
+int hops_noise_filter(const void *z_d, int R, int n_t, const double *s_w_d,
+                      const double *nu_d, const int32_t *row_lop_d, void *out_d,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

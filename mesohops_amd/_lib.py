@@ -1,0 +1,85 @@
+"""ctypes binding of libhops_mi355x.so (include/hops_mi355x.h).
+
+The library is built in-tree by ``mesohops_amd/build.py``.  Loading fails
+loudly when it is missing: there is no CPU fallback for the propagation path.
+"""
+import ctypes
+import os
+
+from .errors import EngineError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhops_mi355x.so")
+
+HOPS_PREC_FP64 = 0
+HOPS_PREC_MIXED = 1
+HOPS_EOM_LINEAR = 0
+HOPS_EOM_NONLINEAR = 1
+HOPS_EOM_NORMALIZED_NONLINEAR = 2
+ABI_VERSION = 1
+
+_i32 = ctypes.c_int32
+_vp = ctypes.c_void_p
+
+# field order must match struct hops_op in include/hops_mi355x.h
+_OP_INTS = ["n_state", "n_hier", "n_mode", "n_lop", "n_act", "n_first", "ell_w", "n_pat",
+            "n_mode_abs", "pad0"]
+_OP_PTRS = ["aux_ptr", "aux_mode", "aux_val",
+            "up_ptr", "up_aux", "up_mode",
+            "dn_ptr", "dn_aux", "dn_mode", "dn_val",
+            "lr_ptr", "lr_col", "lr_val", "lc_ptr", "lc_row", "lc_val",
+            "mode_lrel", "mode_lact", "mode_abs", "mode_g", "mode_w",
+            "abs_g", "abs_w", "abs_lact",
+            "act_lrel", "am_ptr", "am_mode",
+            "la_ptr", "la_row", "la_col", "la_val",
+            "lq_ptr", "lq_row", "lq_col", "lq_val", "lt",
+            "first_aux", "first_lact", "first_mode",
+            "pat_ptr", "pat_col", "pat_h", "pc_ptr", "pc_act", "pc_kind", "pc_val",
+            "ell_col", "ell_val", "k0"]
+
+
+class HopsOp(ctypes.Structure):
+    _fields_ = [(n, _i32) for n in _OP_INTS] + [(n + "_d", _vp) for n in _OP_PTRS]
+
+
+class HopsBatch(ctypes.Structure):
+    _fields_ = [("B", _i32), ("precision", _i32), ("eom", _i32), ("n_t", _i32),
+                ("coef_stride", _i32), ("ws_stride", _i32), ("store_stride", _i32), ("pad0", _i32),
+                ("dt_over_hbar", ctypes.c_double), ("noise_step", _i32), ("pad1", _i32),
+                ("phi_d", _vp), ("y0_d", _vp), ("ya_d", _vp), ("yb_d", _vp), ("acc_d", _vp),
+                ("z1_d", _vp), ("z2_d", _vp), ("zmem_d", _vp), ("ws_d", _vp), ("coef_d", _vp),
+                ("psi_store_d", _vp), ("tile_d", _vp), ("n_tiles", _i32), ("tile_rows", _i32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine library once; raise EngineError if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EngineError("HIP engine %s is not built; run `python -m mesohops_amd.build` "
+                          "(no CPU fallback exists for the propagation path)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.hops_last_error.restype = ctypes.c_char_p
+    L.hops_version.restype = ctypes.c_int
+    L.hops_build_super.argtypes = [ctypes.POINTER(HopsOp), ctypes.c_int, _vp]
+    L.hops_propagate.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, _vp]
+    L.hops_noise_embedding.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+    if L.hops_version() != ABI_VERSION:
+        raise EngineError("libhops_mi355x.so ABI %d != %d; rebuild" % (L.hops_version(), ABI_VERSION))
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise EngineError(lib().hops_last_error().decode())
+
+
+EXPORTED = ["hops_last_error", "hops_version", "hops_build_super", "hops_propagate",
+            "hops_noise_embedding", "hops_noise_filter"]

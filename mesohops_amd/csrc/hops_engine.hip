@@ -1,0 +1,696 @@
+// hops_engine.hip — HOPS propagation on MI355X (gfx950).
+//
+// One trajectory's hierarchy vector Phi is a set of rows r = k*n_state + s
+// (auxiliary k, state s).  A batch of B trajectories that share one basis
+// keeps its pools trajectory-major, so the neighbour gathers of one
+// trajectory stay inside one ~0.1-2 MB window; tiles of a trajectory are
+// placed on one XCD (workgroup i runs on XCD i % 8) so that window lives in
+// that XCD's L2.
+//
+// Per RK4 stage two launches run:
+//   k_prologue  one workgroup per trajectory: <L>, z-memory stage value and
+//               derivative, z-hat, the normalisation correction, the LTC
+//               terms and the per-trajectory coefficients of the system block
+//               (-iH + sum_j c_j L_j ...) — all in FP64.
+//   k_stage     one thread per row: the derivative (aux-diagonal K0, system
+//               block, ELL super-operator K2_kp1+K2_km1, Z2 term, -norm_corr)
+//               fused with the RK4 combination.  Stages run in complex R
+//               (double or float); the state carry Phi stays complex128.
+//
+// Normalisation (NORMALIZED NONLINEAR) is lazy: the scale sigma = 1/|psi| of
+// the state produced by the previous step is computed in FP64 by the stage-0
+// prologue and folded into every read of Phi during the next step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hops_mi355x.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail("%s failed: %s", #expr, hipGetErrorString(e_));            \
+  } while (0)
+
+template <typename R> struct alignas(2 * sizeof(R)) cx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> mk(R a, R b) {
+  cx<R> c;
+  c.x = a;
+  c.y = b;
+  return c;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) {
+  return mk<R>(a.x + b.x, a.y + b.y);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) {
+  return mk<R>(a.x - b.x, a.y - b.y);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+  return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a += b * c
+template <typename R>
+__device__ __forceinline__ void cfma(cx<R> &a, cx<R> b, cx<R> c) {
+  a.x += b.x * c.x - b.y * c.y;
+  a.y += b.x * c.y + b.y * c.x;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cscale(cx<R> a, R s) {
+  return mk<R>(a.x * s, a.y * s);
+}
+template <typename R> __device__ __forceinline__ cx<R> cconj(cx<R> a) {
+  return mk<R>(a.x, -a.y);
+}
+// Complex division by Smith's (1962) scaled algorithm with a reciprocal scale,
+// the same operation sequence NumPy uses for complex128 division, so that
+// g/w reproduces the reference's super-operator values bit for bit.
+template <typename R>
+__device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) {
+#pragma clang fp contract(off)
+  if (fabs(b.x) >= fabs(b.y)) {
+    const R rat = b.y / b.x;
+    const R scl = (R)1 / (b.x + b.y * rat);
+    return mk<R>((a.x + a.y * rat) * scl, (a.y - a.x * rat) * scl);
+  }
+  const R rat = b.x / b.y;
+  const R scl = (R)1 / (b.y + b.x * rat);
+  return mk<R>((a.x * rat + a.y) * scl, (a.y * rat - a.x) * scl);
+}
+template <typename R, typename S>
+__device__ __forceinline__ cx<R> ccast(cx<S> a) {
+  return mk<R>((R)a.x, (R)a.y);
+}
+__device__ __forceinline__ cx<double> ld2(const double *p, int i) {
+  return mk<double>(p[2 * i], p[2 * i + 1]);
+}
+
+typedef cx<double> zd;
+
+// Block-wide sum of a complex double (blockDim.x == 256, 4 wavefronts of 64).
+__device__ zd block_sum(zd v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * wid] = v.x;
+    red[2 * wid + 1] = v.y;
+  }
+  __syncthreads();
+  zd t = mk<double>(0.0, 0.0);
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) {
+    t.x += red[2 * w];
+    t.y += red[2 * w + 1];
+  }
+  __syncthreads();
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Super-operator build (eom/eom_hops_ksuper.py:36-230, :575-668).
+// Row r = (k, s).  Up connection (k -> k+e_m):  Kp1[(k,s),(k+e_m,s')] =
+// -(g_m/w_m) L_m[s,s'].  Down connection (k -> k-e_m, depth k_m of aux k):
+// Km1[(k,s),(k-e_m,s_r)] = (k_m w_m) conj(L_m[s_r,s]).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_build_ell(const hops_op_t op) {
+#pragma clang fp contract(off)
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int k = (int)(r / n), s = (int)(r - (long)k * n);
+  cx<R> *val = reinterpret_cast<cx<R> *>(op.ell_val_d);
+  int e = 0;
+  for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u) {
+    const int m = op.up_mode_d[u], k2 = op.up_aux_d[u];
+    const int j = op.mode_lrel_d[m];
+    const zd gw = cdiv(ld2(op.mode_g_d, m), ld2(op.mode_w_d, m));
+    const zd ngw = mk<double>(-gw.x, -gw.y);
+    const int q0 = op.lr_ptr_d[j * (n + 1) + s], q1 = op.lr_ptr_d[j * (n + 1) + s + 1];
+    for (int q = q0; q < q1; ++q) {
+      if (e >= op.ell_w) return;  // host sized ell_w; never taken
+      const zd v = cmul(ngw, ld2(op.lr_val_d, q));
+      op.ell_col_d[(long)e * nr + r] = k2 * n + op.lr_col_d[q];
+      val[(long)e * nr + r] = ccast<R>(v);
+      ++e;
+    }
+  }
+  for (int u = op.dn_ptr_d[k]; u < op.dn_ptr_d[k + 1]; ++u) {
+    const int m = op.dn_mode_d[u], k2 = op.dn_aux_d[u];
+    const int j = op.mode_lrel_d[m];
+    const zd w = ld2(op.mode_w_d, m);
+    const double kv = (double)op.dn_val_d[u];
+    const zd kw = mk<double>(kv * w.x, kv * w.y);
+    const int q0 = op.lc_ptr_d[j * (n + 1) + s], q1 = op.lc_ptr_d[j * (n + 1) + s + 1];
+    for (int q = q0; q < q1; ++q) {
+      if (e >= op.ell_w) return;
+      const zd v = cmul(kw, cconj(ld2(op.lc_val_d, q)));
+      op.ell_col_d[(long)e * nr + r] = k2 * n + op.lc_row_d[q];
+      val[(long)e * nr + r] = ccast<R>(v);
+      ++e;
+    }
+  }
+  for (; e < op.ell_w; ++e) {
+    op.ell_col_d[(long)e * nr + r] = -1;
+    val[(long)e * nr + r] = mk<R>((R)0, (R)0);
+  }
+}
+
+// K0[k] = -sum_m k_m W_m over the absolute modes of aux k
+// (eom/eom_hops_ksuper.py:36-90).
+__global__ __launch_bounds__(256) void k_build_k0(const hops_op_t op) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= op.n_hier) return;
+  double sx = 0.0, sy = 0.0;
+  for (int q = op.aux_ptr_d[k]; q < op.aux_ptr_d[k + 1]; ++q) {
+    const int m = op.aux_mode_d[q];
+    const double v = (double)op.aux_val_d[q];
+    sx += v * op.abs_w_d[2 * m];
+    sy += v * op.abs_w_d[2 * m + 1];
+  }
+  op.k0_d[2 * k] = -sx;
+  op.k0_d[2 * k + 1] = -sy;
+}
+
+// ---------------------------------------------------------------------------
+// Per-trajectory coefficient layout (doubles):
+//   A  [n_pat] complex   system block for every auxiliary
+//   A0 [n_pat] complex   system block for the physical wave function
+//   zc [n_mode] complex  (g_m/w_m) conj(<L_m>) for the Z2 term
+//   nc complex           normalisation correction
+//   sigma                scale of Phi for this step
+// Workspace (doubles): kz [n_mode_abs] complex, zacc [n_mode_abs] complex.
+// ---------------------------------------------------------------------------
+struct CoefLayout {
+  int A, A0, zc, nc, sig;
+};
+__host__ __device__ inline CoefLayout coef_layout(const hops_op_t &op) {
+  CoefLayout c;
+  c.A = 0;
+  c.A0 = 2 * op.n_pat;
+  c.zc = 4 * op.n_pat;
+  c.nc = 4 * op.n_pat + 2 * op.n_mode;
+  c.sig = c.nc + 2;
+  return c;
+}
+
+__constant__ double c_rk[4] = {0.0, 0.5, 0.5, 1.0};
+
+// Stage prologue: eom/hops_eom.py:329-441 (noise terms, <L>, norm_corr, LTC),
+// eom/eom_functions.py:11-309, integrator/integrator_rk.py:53-64 (z_mem stage
+// values, noise index map [0,1,1,2]).
+template <typename R>
+__global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops_batch_t bt,
+                                                  int stage, int tidx, const cx<R> *__restrict__ yin,
+                                                  int store_slot) {
+  extern __shared__ double sm[];
+  const int n = op.n_state, na = op.n_act, Ma = op.n_mode_abs;
+  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const long nr = (long)n * op.n_hier;
+  const long off = (long)b * nr;
+  const CoefLayout L = coef_layout(op);
+  double *coef = bt.coef_d + (long)b * bt.coef_stride;
+  double *ws = bt.ws_d + (long)b * bt.ws_stride;
+  double *zmem = bt.zmem_d + (long)b * 2 * Ma;
+  const bool nonlin = bt.eom != HOPS_EOM_LINEAR;
+  const bool normed = bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR;
+
+  double *psi = sm;                 // 2n
+  double *avg = psi + 2 * n;        // 2na
+  double *avq = avg + 2 * na;       // 2na
+  double *cj = avq + 2 * na;        // 2na  noise coefficient
+  double *zh = cj + 2 * na;         // 2na  z-hat
+  double *zt = zh + 2 * na;         // 2Ma
+  double *red = zt + 2 * Ma;        // 16
+
+  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d) + off;
+  // 1. physical wave function of the stage input (FP64), sigma at stage 0
+  double sigma;
+  if (stage == 0) {
+    zd acc = mk<double>(0.0, 0.0);
+    for (int s = tid; s < n; s += nt) {
+      const double2 v = phi[s];
+      acc.x += v.x * v.x + v.y * v.y;
+    }
+    acc = block_sum(acc, red);
+    sigma = normed ? 1.0 / sqrt(acc.x) : 1.0;
+    for (int s = tid; s < n; s += nt) {
+      const double2 v = phi[s];
+      psi[2 * s] = sigma * v.x;
+      psi[2 * s + 1] = sigma * v.y;
+      if (store_slot >= 0 && bt.psi_store_d) {
+        double *dst = bt.psi_store_d + 2 * ((long)b * bt.store_stride + store_slot) * n;
+        dst[2 * s] = sigma * v.x;
+        dst[2 * s + 1] = sigma * v.y;
+      }
+    }
+    if (tid == 0) coef[L.sig] = sigma;
+  } else {
+    sigma = coef[L.sig];
+    const cx<R> *y = yin + off;
+    for (int s = tid; s < n; s += nt) {
+      psi[2 * s] = (double)y[s].x;
+      psi[2 * s + 1] = (double)y[s].y;
+    }
+  }
+  __syncthreads();
+  zd nrm = mk<double>(0.0, 0.0);
+  for (int s = tid; s < n; s += nt) nrm.x += psi[2 * s] * psi[2 * s] + psi[2 * s + 1] * psi[2 * s + 1];
+  nrm = block_sum(nrm, red);
+  const bool has_lt = op.lt_d != nullptr;
+
+  // 2. <L_j> (and <L_j^2> for the LTC) over active L-operators
+  for (int j = tid; j < na; j += nt) {
+    zd a = mk<double>(0.0, 0.0);
+    for (int q = op.la_ptr_d[j]; q < op.la_ptr_d[j + 1]; ++q) {
+      const int r0 = op.la_row_d[q], c0 = op.la_col_d[q];
+      const zd v = cmul(cconj(ld2(psi, r0)), cmul(ld2(op.la_val_d, q), ld2(psi, c0)));
+      a = cadd(a, v);
+    }
+    avg[2 * j] = a.x / nrm.x;
+    avg[2 * j + 1] = a.y / nrm.x;
+    zd a2 = mk<double>(0.0, 0.0);
+    if (has_lt) {
+      for (int q = op.lq_ptr_d[j]; q < op.lq_ptr_d[j + 1]; ++q) {
+        const int r0 = op.lq_row_d[q], c0 = op.lq_col_d[q];
+        a2 = cadd(a2, cmul(cconj(ld2(psi, r0)), cmul(ld2(op.lq_val_d, q), ld2(psi, c0))));
+      }
+    }
+    avq[2 * j] = a2.x / nrm.x;
+    avq[2 * j + 1] = a2.y / nrm.x;
+  }
+  __syncthreads();
+
+  // 3. noise memory: stage value, derivative, RK4 accumulation
+  //    (eom/eom_functions.py:65-160, integrator/integrator_rk.py:59-73)
+  const double h = bt.dt_over_hbar;
+  double *kz = ws;
+  double *zacc = ws + 2 * Ma;
+  for (int m = tid; m < Ma; m += nt) {
+    zd z = ld2(zmem, m);
+    if (nonlin && stage > 0) {
+      const double c = c_rk[stage] * h;
+      z.x += c * kz[2 * m];
+      z.y += c * kz[2 * m + 1];
+    }
+    zt[2 * m] = z.x;
+    zt[2 * m + 1] = z.y;
+    if (!nonlin) continue;
+    const int la = op.abs_lact_d[m];
+    const zd g = ld2(op.abs_g_d, m), w = ld2(op.abs_w_d, m);
+    zd d = mk<double>(0.0, 0.0);
+    if (la >= -1) {
+      const zd lav = la >= 0 ? ld2(avg, la) : mk<double>(0.0, 0.0);
+      d = csub(cmul(lav, cconj(g)), cmul(cconj(w), z));
+    } else if (z.x > 1e-8 || (z.x == 1e-8 && z.y > 0.0)) {
+      d = mk<double>(0.0, 0.0);
+      d = csub(d, cmul(cconj(w), z));
+    }
+    kz[2 * m] = d.x;
+    kz[2 * m + 1] = d.y;
+    if (stage == 0) {
+      zacc[2 * m] = d.x;
+      zacc[2 * m + 1] = d.y;
+    } else if (stage < 3) {
+      zacc[2 * m] += 2.0 * d.x;
+      zacc[2 * m + 1] += 2.0 * d.y;
+    } else {
+      const double c = h / 6.0;
+      zmem[2 * m] += c * (zacc[2 * m] + d.x);
+      zmem[2 * m + 1] += c * (zacc[2 * m + 1] + d.y);
+    }
+  }
+  __syncthreads();
+
+  // 4. z-hat and the noise coefficient of each active L-operator
+  const float2 *z1 = reinterpret_cast<const float2 *>(bt.z1_d) + (long)b * na * bt.n_t;
+  const float2 *z2 = bt.z2_d ? reinterpret_cast<const float2 *>(bt.z2_d) + (long)b * na * bt.n_t : nullptr;
+  for (int j = tid; j < na; j += nt) {
+    const float2 v1 = z1[(long)j * bt.n_t + tidx];
+    zd zz = mk<double>((double)v1.x, -(double)v1.y);
+    if (nonlin) {
+      for (int q = op.am_ptr_d[j]; q < op.am_ptr_d[j + 1]; ++q) {
+        const int ma = op.mode_abs_d[op.am_mode_d[q]];
+        zz.x += zt[2 * ma];
+        zz.y += zt[2 * ma + 1];
+      }
+    }
+    zh[2 * j] = zz.x;
+    zh[2 * j + 1] = zz.y;
+    zd c = zz;
+    if (z2) {
+      const float2 v2 = z2[(long)j * bt.n_t + tidx];
+      c.x += (double)v2.y;
+      c.y -= (double)v2.x;
+    }
+    cj[2 * j] = c.x;
+    cj[2 * j + 1] = c.y;
+  }
+  __syncthreads();
+
+  // 5. normalisation correction (eom/eom_functions.py:163-213, :258-284)
+  zd nc = mk<double>(0.0, 0.0);
+  if (normed) {
+    zd part = mk<double>(0.0, 0.0);
+    for (int j = tid; j < na; j += nt) part = cadd(part, cmul(ld2(zh, j), ld2(avg, j)));
+    for (int f = tid; f < op.n_first; f += nt) {
+      const int ia = op.first_aux_d[f], l = op.first_lact_d[f], m = op.first_mode_d[f];
+      const zd gw = cdiv(ld2(op.mode_g_d, m), ld2(op.mode_w_d, m));
+      // rows of the first-level auxiliary in the stage input
+      zd lterm = mk<double>(0.0, 0.0), oterm = mk<double>(0.0, 0.0);
+      for (int q = op.la_ptr_d[l]; q < op.la_ptr_d[l + 1]; ++q) {
+        const int r0 = op.la_row_d[q], c0 = op.la_col_d[q];
+        zd pe;
+        if (stage == 0) {
+          const double2 v = phi[(long)ia * n + c0];
+          pe = mk<double>(sigma * v.x, sigma * v.y);
+        } else {
+          pe = ccast<double>(yin[off + (long)ia * n + c0]);
+        }
+        lterm = cadd(lterm, cmul(cconj(ld2(psi, r0)), cmul(ld2(op.la_val_d, q), cmul(gw, pe))));
+      }
+      for (int s = 0; s < n; ++s) {
+        zd pe;
+        if (stage == 0) {
+          const double2 v = phi[(long)ia * n + s];
+          pe = mk<double>(sigma * v.x, sigma * v.y);
+        } else {
+          pe = ccast<double>(yin[off + (long)ia * n + s]);
+        }
+        oterm = cadd(oterm, cmul(cconj(ld2(psi, s)), cmul(gw, pe)));
+      }
+      part = csub(part, lterm);
+      part = cadd(part, cmul(oterm, ld2(avg, l)));
+    }
+    part = block_sum(part, red);
+    nc = mk<double>(part.x, 0.0);
+    if (has_lt) {
+      zd q = mk<double>(0.0, 0.0);
+      for (int j = tid; j < na; j += nt) {
+        const zd a = ld2(avg, j);
+        const zd t = csub(cscale(cmul(a, a), 2.0), ld2(avq, j));
+        q = cadd(q, cscale(t, op.lt_d[2 * j]));
+      }
+      q = block_sum(q, red);
+      nc = cadd(nc, q);
+    }
+  }
+
+  // 6. system-block coefficients, Z2 coefficients
+  for (int p = tid; p < op.n_pat; p += nt) {
+    zd a = ld2(op.pat_h_d, p), a0 = a;
+    for (int e = op.pc_ptr_d[p]; e < op.pc_ptr_d[p + 1]; ++e) {
+      const int j = op.pc_act_d[e];
+      const zd v = ld2(op.pc_val_d, e);
+      if (op.pc_kind_d[e] == 0) {
+        zd c = ld2(cj, j);
+        zd c0 = c;
+        if (nonlin && has_lt) {
+          const zd lt = ld2(op.lt_d, j), av = ld2(avg, j);
+          c = cadd(c, cmul(cconj(lt), av));
+          c0 = cadd(c, cmul(lt, av));
+        }
+        a = cadd(a, cmul(c, v));
+        a0 = cadd(a0, cmul(c0, v));
+      } else {
+        a0 = csub(a0, cmul(ld2(op.lt_d, j), v));
+      }
+    }
+    coef[L.A + 2 * p] = a.x;
+    coef[L.A + 2 * p + 1] = a.y;
+    coef[L.A0 + 2 * p] = a0.x;
+    coef[L.A0 + 2 * p + 1] = a0.y;
+  }
+  if (nonlin) {
+    for (int m = tid; m < op.n_mode; m += nt) {
+      const int la = op.mode_lact_d[m];
+      zd z = mk<double>(0.0, 0.0);
+      if (la >= 0) z = cmul(cdiv(ld2(op.mode_g_d, m), ld2(op.mode_w_d, m)), cconj(ld2(avg, la)));
+      coef[L.zc + 2 * m] = z.x;
+      coef[L.zc + 2 * m + 1] = z.y;
+    }
+  }
+  if (tid == 0) {
+    coef[L.nc] = nc.x;
+    coef[L.nc + 1] = nc.y;
+  }
+}
+
+// One RK4 stage for every row of every trajectory
+// (eom/hops_eom.py:386-426, integrator/integrator_rk.py:53-73).
+template <typename R>
+__global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_batch_t bt, int stage,
+                                               const cx<R> *__restrict__ yin,
+                                               cx<R> *__restrict__ yout) {
+  const int t = bt.tile_d[blockIdx.x];
+  if (t < 0) return;
+  const int b = t >> 16, tile = t & 0xffff;
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)tile * bt.tile_rows + threadIdx.x;
+  if (r >= nr) return;
+  const long off = (long)b * nr;
+  const int k = (int)(r / n), s = (int)(r - (long)k * n);
+  const cx<R> *Y = yin + off;
+  const CoefLayout L = coef_layout(op);
+  const double *coef = bt.coef_d + (long)b * bt.coef_stride;
+  const double sigma = coef[L.sig];
+
+  const cx<R> yr = Y[r];
+  cx<R> d = cmul(ccast<R>(ld2(op.k0_d, k)), yr);
+  // system block: -iH + noise / LTC terms
+  const double *A = coef + (k == 0 ? L.A0 : L.A);
+  const long rowk = (long)k * n;
+  for (int p = op.pat_ptr_d[s]; p < op.pat_ptr_d[s + 1]; ++p)
+    cfma(d, ccast<R>(ld2(A, p)), Y[rowk + op.pat_col_d[p]]);
+  // hierarchy super-operator K2_kp1 + K2_km1 (ELL, column-major)
+  const cx<R> *ev = reinterpret_cast<const cx<R> *>(op.ell_val_d);
+  for (int e = 0; e < op.ell_w; ++e) {
+    const int c = op.ell_col_d[(long)e * nr + r];
+    if (c < 0) break;
+    cfma(d, ev[(long)e * nr + r], Y[c]);
+  }
+  if (bt.eom != HOPS_EOM_LINEAR) {
+    const double *zc = coef + L.zc;
+    for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
+      cfma(d, ccast<R>(ld2(zc, op.up_mode_d[u])), Y[(long)op.up_aux_d[u] * n + s]);
+    if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
+      const cx<R> nc = mk<R>((R)coef[L.nc], (R)coef[L.nc + 1]);
+      d = csub(d, cmul(nc, yr));
+    }
+  }
+  if (stage == 0) d = cscale(d, (R)sigma);
+
+  const long gi = off + r;
+  const double h = bt.dt_over_hbar;
+  cx<R> *acc = reinterpret_cast<cx<R> *>(bt.acc_d);
+  double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
+  if (stage < 3) {
+    cx<R> base;
+    if (bt.precision == HOPS_PREC_MIXED) {
+      base = cscale(reinterpret_cast<const cx<R> *>(bt.y0_d)[gi], (R)sigma);
+    } else {
+      const double2 p = phi[gi];
+      base = mk<R>((R)(sigma * p.x), (R)(sigma * p.y));
+    }
+    const R c = (R)((stage == 2 ? 1.0 : 0.5) * h);
+    yout[gi] = mk<R>(base.x + c * d.x, base.y + c * d.y);
+    if (stage == 0) {
+      acc[gi] = d;
+    } else {
+      cx<R> a = acc[gi];
+      a.x += (R)2 * d.x;
+      a.y += (R)2 * d.y;
+      acc[gi] = a;
+    }
+  } else {
+    const cx<R> a = acc[gi];
+    const double2 p = phi[gi];
+    const double c = h / 6.0;
+    double2 q;
+    q.x = sigma * p.x + c * ((double)a.x + (double)d.x);
+    q.y = sigma * p.y + c * ((double)a.y + (double)d.y);
+    phi[gi] = q;
+    if (bt.precision == HOPS_PREC_MIXED)
+      reinterpret_cast<cx<float> *>(bt.y0_d)[gi] = mk<float>((float)q.x, (float)q.y);
+  }
+}
+
+// y0 = complex64(Phi) at the start of a mixed-precision propagate call.
+__global__ __launch_bounds__(256) void k_prime_y0(const double2 *__restrict__ phi,
+                                                  float2 *__restrict__ y0, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 p = phi[i];
+  y0[i] = make_float2((float)p.x, (float)p.y);
+}
+
+// End of a propagate call: sigma of the last step's state and psi storage.
+__global__ __launch_bounds__(256) void k_final_sigma(const hops_op_t op, const hops_batch_t bt,
+                                                     int store_slot) {
+  __shared__ double red[16];
+  const int n = op.n_state, b = blockIdx.x, tid = threadIdx.x;
+  const long off = (long)b * n * op.n_hier;
+  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d) + off;
+  zd acc = mk<double>(0.0, 0.0);
+  for (int s = tid; s < n; s += blockDim.x) acc.x += phi[s].x * phi[s].x + phi[s].y * phi[s].y;
+  acc = block_sum(acc, red);
+  const double sigma = bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR ? 1.0 / sqrt(acc.x) : 1.0;
+  if (store_slot >= 0 && bt.psi_store_d) {
+    double *dst = bt.psi_store_d + 2 * ((long)b * bt.store_stride + store_slot) * n;
+    for (int s = tid; s < n; s += blockDim.x) {
+      dst[2 * s] = sigma * phi[s].x;
+      dst[2 * s + 1] = sigma * phi[s].y;
+    }
+  }
+  if (tid == 0) bt.ws_d[(long)b * bt.ws_stride + 4 * op.n_mode_abs] = sigma;
+}
+
+__global__ __launch_bounds__(256) void k_final_scale(const hops_op_t op, const hops_batch_t bt) {
+  const int t = bt.tile_d[blockIdx.x];
+  if (t < 0) return;
+  const int b = t >> 16, tile = t & 0xffff;
+  const long nr = (long)op.n_state * op.n_hier;
+  const long r = (long)tile * bt.tile_rows + threadIdx.x;
+  if (r >= nr) return;
+  const double sigma = bt.ws_d[(long)b * bt.ws_stride + 4 * op.n_mode_abs];
+  double2 *phi = reinterpret_cast<double2 *>(bt.phi_d) + (long)b * nr;
+  double2 p = phi[r];
+  p.x *= sigma;
+  p.y *= sigma;
+  phi[r] = p;
+}
+
+int check_op(const hops_op_t *op) {
+  if (!op) return fail("op is NULL");
+  if (op->n_state <= 0 || op->n_hier <= 0) return fail("empty basis (n_state=%d n_hier=%d)", op->n_state, op->n_hier);
+  if ((long)op->n_state * op->n_hier >= (1L << 31)) return fail("basis too large for int32 rows");
+  if (op->ell_w < 0 || op->n_pat < 0 || op->n_act < 0) return fail("negative sizes");
+  return 0;
+}
+
+template <typename R>
+int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
+                hipStream_t st) {
+  const int n = op->n_state;
+  const long nr = (long)n * op->n_hier;
+  const size_t shm = sizeof(double) * (2 * (size_t)n + 8 * (size_t)op->n_act + 2 * (size_t)op->n_mode_abs + 16);
+  if (shm > 160 * 1024) return fail("prologue LDS %zu B exceeds 160 KiB", shm);
+  if (shm > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_prologue<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  const int ns = bt->noise_step;
+  if (it0 < 0 || it0 + (long)n_steps * 2 * ns >= bt->n_t)
+    return fail("noise index range [%d, %ld] outside [0, %d)", it0, it0 + (long)n_steps * 2 * ns, bt->n_t);
+  cx<R> *Y0 = reinterpret_cast<cx<R> *>(bt->precision == HOPS_PREC_MIXED ? bt->y0_d : bt->phi_d);
+  cx<R> *ya = reinterpret_cast<cx<R> *>(bt->ya_d);
+  cx<R> *yb = reinterpret_cast<cx<R> *>(bt->yb_d);
+  if (bt->precision == HOPS_PREC_MIXED) {
+    const long tot = nr * bt->B;
+    k_prime_y0<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(reinterpret_cast<const double2 *>(bt->phi_d),
+                                                            reinterpret_cast<float2 *>(bt->y0_d), tot);
+    HIP_TRY(hipGetLastError());
+  }
+  const int toff[4] = {0, ns, ns, 2 * ns};
+  for (int step = 0; step < n_steps; ++step) {
+    const int itb = it0 + step * 2 * ns;
+    for (int i = 0; i < 4; ++i) {
+      const cx<R> *yin = i == 0 ? Y0 : (i == 2 ? yb : ya);
+      cx<R> *yout = i == 1 ? yb : ya;
+      const int slot = (i == 0 && step > 0 && bt->psi_store_d) ? store0 + step - 1 : -1;
+      k_prologue<R><<<bt->B, 256, shm, st>>>(*op, *bt, i, itb + toff[i], yin, slot);
+      HIP_TRY(hipGetLastError());
+      k_stage<R><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, i, yin, yout);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  const int slot = (n_steps > 0 && bt->psi_store_d) ? store0 + n_steps - 1 : -1;
+  k_final_sigma<<<bt->B, 256, 0, st>>>(*op, *bt, slot);
+  HIP_TRY(hipGetLastError());
+  if (bt->eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
+    k_final_scale<<<bt->n_tiles, 256, 0, st>>>(*op, *bt);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hops_last_error(void) { return g_err.c_str(); }
+
+// Shared by the other translation units of the library (noise_fft.hip).
+int hops_internal_set_error(const char *msg) {
+  g_err = msg;
+  return -1;
+}
+
+int hops_version(void) { return 1; }
+
+int hops_build_super(const hops_op_t *op, int precision, void *stream) {
+  if (int rc = check_op(op)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const long nr = (long)op->n_state * op->n_hier;
+  const unsigned g = (unsigned)((nr + 255) / 256);
+  if (op->ell_w > 0) {
+    if (precision == HOPS_PREC_FP64)
+      k_build_ell<double><<<g, 256, 0, st>>>(*op);
+    else if (precision == HOPS_PREC_MIXED)
+      k_build_ell<float><<<g, 256, 0, st>>>(*op);
+    else
+      return fail("unknown precision %d", precision);
+    HIP_TRY(hipGetLastError());
+  }
+  k_build_k0<<<(unsigned)((op->n_hier + 255) / 256), 256, 0, st>>>(*op);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
+                   void *stream) {
+  if (int rc = check_op(op)) return rc;
+  if (!bt) return fail("batch is NULL");
+  if (bt->B <= 0 || bt->tile_rows != 256 || bt->n_tiles <= 0) return fail("bad batch geometry");
+  if (bt->eom < 0 || bt->eom > 2) return fail("unknown eom %d", bt->eom);
+  const CoefLayout L = coef_layout(*op);
+  if (bt->coef_stride < L.sig + 1) return fail("coef_stride %d < %d", bt->coef_stride, L.sig + 1);
+  if (bt->ws_stride < 4 * op->n_mode_abs + 1) return fail("ws_stride too small");
+  if (n_steps < 0) return fail("n_steps < 0");
+  hipStream_t st = (hipStream_t)stream;
+  if (bt->precision == HOPS_PREC_FP64) return propagate_t<double>(op, bt, it0, n_steps, store0, st);
+  if (bt->precision == HOPS_PREC_MIXED) return propagate_t<float>(op, bt, it0, n_steps, store0, st);
+  return fail("unknown precision %d", bt->precision);
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// noise_fft.hip — FFT-filtered correlated noise on MI355X (gfx950).
+//
+// Implements the circulant-embedding filter of Percival (2006) the way the
+// reference applies it (noise/hops_noise.py:630-716):
+//   tilde c_j = c_j exp(-2 pi i nu j)          (rotate so c[n_t-1] is real)
+//   s_w = Re FFT([tilde c, conj(reverse(tilde c[1:-1]))]), negatives -> 0
+//   eta = unrotate( IFFT( FFT(z) * sqrt(s_w / 2) )[:n_t] )
+// One workgroup owns one row and keeps it in LDS.  The transform length is
+// N = 2(n_t - 1).  Powers of two run a radix-2 decimation-in-frequency
+// forward pass (natural -> bit-reversed order) and a decimation-in-time
+// inverse pass (bit-reversed -> natural), so the spectral multiply happens in
+// bit-reversed order and no permutation pass is needed.  Other lengths use
+// Bluestein's chirp-z algorithm on a power-of-two length M >= 2N-1 with the
+// same two passes.  All arithmetic is FP64; the output is complex64 like the
+// reference's noise.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/hops_mi355x.h"
+
+extern "C" int hops_internal_set_error(const char *msg);
+
+namespace {
+
+constexpr int kMaxLds = 8192;  // complex doubles held in LDS (128 KiB)
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 twiddle(double sgn, long j, long len) {
+  double s, c;
+  sincospi(sgn * 2.0 * (double)j / (double)len, &s, &c);
+  return make_double2(c, s);
+}
+__device__ __forceinline__ unsigned bitrev(unsigned x, int logm) {
+  return __brev(x) >> (32 - logm);
+}
+
+// In-place radix-2 DIF: natural order in, bit-reversed order out.
+__device__ void fft_dif(double2 *a, int M, double sgn) {
+  for (int len = M; len >= 2; len >>= 1) {
+    const int half = len >> 1;
+    for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
+      const int blk = t / half, j = t - blk * half;
+      const int i0 = blk * len + j, i1 = i0 + half;
+      const double2 u = a[i0], v = a[i1];
+      a[i0] = make_double2(u.x + v.x, u.y + v.y);
+      a[i1] = cmul2(make_double2(u.x - v.x, u.y - v.y), twiddle(sgn, j, len));
+    }
+    __syncthreads();
+  }
+}
+
+// In-place radix-2 DIT: bit-reversed order in, natural order out.
+__device__ void fft_dit(double2 *a, int M, double sgn) {
+  for (int len = 2; len <= M; len <<= 1) {
+    const int half = len >> 1;
+    for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
+      const int blk = t / half, j = t - blk * half;
+      const int i0 = blk * len + j, i1 = i0 + half;
+      const double2 u = a[i0];
+      const double2 v = cmul2(a[i1], twiddle(sgn, j, len));
+      a[i0] = make_double2(u.x + v.x, u.y + v.y);
+      a[i1] = make_double2(u.x - v.x, u.y - v.y);
+    }
+    __syncthreads();
+  }
+}
+
+// Bluestein chirp w_j = exp(i pi j^2 / N)  (j^2 reduced mod 2N).
+__device__ __forceinline__ double2 chirp(long j, long N) {
+  const long q = (j * j) % (2 * N);
+  double s, c;
+  sincospi((double)q / (double)N, &s, &c);
+  return make_double2(c, s);
+}
+
+// Forward DFT of length N held in a[0..N) (natural order); M = 2^logm.
+// pow2 (M == N): result left in BIT-REVERSED order.  Bluestein (M > N): result
+// in natural order in a[0..N).  Bhat_d: FFT_M of the chirp filter in
+// bit-reversed order (Bluestein only).
+__device__ void dft_forward(double2 *a, int N, int M, int logm, const double2 *Bhat_d) {
+  if (M == N) {
+    fft_dif(a, M, -1.0);
+    return;
+  }
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    if (j < N) {
+      const double2 w = chirp(j, N);
+      a[j] = cmul2(a[j], make_double2(w.x, -w.y));
+    } else {
+      a[j] = make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  fft_dif(a, M, -1.0);
+  for (int j = threadIdx.x; j < M; j += blockDim.x) a[j] = cmul2(a[j], Bhat_d[j]);
+  __syncthreads();
+  fft_dit(a, M, 1.0);
+  const double inv = 1.0 / (double)M;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    const double2 w = chirp(k, N);
+    const double2 v = cmul2(a[k], make_double2(w.x, -w.y));
+    a[k] = make_double2(v.x * inv, v.y * inv);
+  }
+  __syncthreads();
+}
+
+// FFT_M of the Bluestein filter b_j = w_j (j < N), b_{M-j} = w_j (0 < j < N),
+// stored in bit-reversed order.
+__global__ __launch_bounds__(kThreads) void k_bluestein_filter(double2 *Bhat, int N, int M) {
+  extern __shared__ double2 lds[];
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    double2 v = make_double2(0.0, 0.0);
+    if (j < N) v = chirp(j, N);
+    else if (j > M - N) v = chirp(M - j, N);
+    lds[j] = v;
+  }
+  __syncthreads();
+  fft_dif(lds, M, -1.0);
+  for (int j = threadIdx.x; j < M; j += blockDim.x) Bhat[j] = lds[j];
+}
+
+// Embedding spectrum of each correlation function row.
+__global__ __launch_bounds__(kThreads) void k_embedding(const float2 *__restrict__ c, int n_t, int N, int M,
+                                                        int logm, const double2 *Bhat, double *s_w,
+                                                        double *nu) {
+  extern __shared__ double2 lds[];
+  const int row = blockIdx.x;
+  const float2 *cr = c + (long)row * n_t;
+  // rotation so that the last point is real; angles in float like the
+  // complex64 input of the reference (np.angle of complex64 is float32)
+  const float2 cl = cr[n_t - 1];
+  const double ang_last = (double)(float)atan2((double)cl.y, (double)cl.x);
+  const double nur = ang_last / (((double)n_t - 1.0) * 2.0 * M_PI);
+  for (int j = threadIdx.x; j < n_t; j += blockDim.x) {
+    const float2 v = cr[j];
+    const double ang = (double)(float)atan2((double)v.y, (double)v.x);
+    const double mag = (double)(float)sqrt((double)v.x * v.x + (double)v.y * v.y);
+    double s1, c1, s2, c2;
+    sincos(ang, &s1, &c1);
+    sincospi(-2.0 * nur * (double)j, &s2, &c2);
+    const double2 t = cmul2(make_double2(mag * c1, mag * s1), make_double2(c2, s2));
+    lds[j] = t;
+    if (j >= 1 && j <= n_t - 2) lds[N - j] = make_double2(t.x, -t.y);
+  }
+  __syncthreads();
+  dft_forward(lds, N, M, logm, Bhat);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    const int src = (M == N) ? (int)bitrev((unsigned)k, logm) : k;
+    const double v = lds[src].x;
+    s_w[(long)row * N + k] = v < 0.0 ? 0.0 : v;
+  }
+  if (threadIdx.x == 0) nu[row] = nur;
+}
+
+// Correlated noise of each raw row.
+__global__ __launch_bounds__(kThreads) void k_filter(const float2 *__restrict__ z, int n_t, int N, int M,
+                                                     int logm, const double2 *Bhat,
+                                                     const double *__restrict__ s_w,
+                                                     const double *__restrict__ nu,
+                                                     const int32_t *__restrict__ row_lop,
+                                                     float2 *__restrict__ out) {
+  extern __shared__ double2 lds[];
+  const int row = blockIdx.x;
+  const int l = row_lop[row];
+  const float2 *zr = z + (long)row * N;
+  const double *sw = s_w + (long)l * N;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) lds[j] = make_double2((double)zr[j].x, (double)zr[j].y);
+  __syncthreads();
+  dft_forward(lds, N, M, logm, Bhat);
+  if (M == N) {
+    // spectrum in bit-reversed order: multiply, then DIT inverse -> natural
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      const double f = sqrt(sw[bitrev((unsigned)i, logm)] / 2.0);
+      lds[i] = make_double2(lds[i].x * f, lds[i].y * f);
+    }
+    __syncthreads();
+    fft_dit(lds, M, 1.0);
+  } else {
+    // inverse DFT = conj(DFT(conj(X))) / N
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+      const double f = sqrt(sw[k] / 2.0);
+      lds[k] = make_double2(lds[k].x * f, -lds[k].y * f);
+    }
+    __syncthreads();
+    dft_forward(lds, N, M, logm, Bhat);
+    for (int k = threadIdx.x; k < N; k += blockDim.x) lds[k].y = -lds[k].y;
+    __syncthreads();
+  }
+  const double inv = 1.0 / (double)N;
+  const double nur = nu[l];
+  for (int j = threadIdx.x; j < n_t; j += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * nur * (double)j, &s, &c);  // conj(E2)
+    const double2 y = make_double2(lds[j].x * inv, lds[j].y * inv);
+    const double2 v = cmul2(y, make_double2(c, s));
+    out[(long)row * n_t + j] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+struct Plan {
+  int N, M, logm;
+  double2 *Bhat;  // library-owned, one per length, freed at process exit
+};
+
+std::mutex g_plan_mu;
+std::map<int, Plan> g_plans;
+
+int make_plan(int n_t, hipStream_t st, Plan *out) {
+  if (n_t < 3) return hops_internal_set_error("noise time axis needs at least 3 points");
+  const long N = 2L * (n_t - 1);
+  long M = 1;
+  int logm = 0;
+  while (M < N) {
+    M <<= 1;
+    ++logm;
+  }
+  Plan p{(int)N, (int)M, logm, nullptr};
+  if (M != N) {
+    M = 1;
+    logm = 0;
+    while (M < 2 * N - 1) {
+      M <<= 1;
+      ++logm;
+    }
+    p.M = (int)M;
+    p.logm = logm;
+  }
+  if (p.M > kMaxLds)
+    return hops_internal_set_error("noise FFT length exceeds the LDS-resident limit (8192 points)");
+  if (p.M != p.N) {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(p.N);
+    if (it != g_plans.end()) {
+      *out = it->second;
+      return 0;
+    }
+    if (hipMalloc(&p.Bhat, sizeof(double2) * p.M) != hipSuccess)
+      return hops_internal_set_error("hipMalloc of the Bluestein filter failed");
+    k_bluestein_filter<<<1, kThreads, sizeof(double2) * p.M, st>>>(p.Bhat, p.N, p.M);
+    if (hipGetLastError() != hipSuccess) return hops_internal_set_error("Bluestein filter launch failed");
+    g_plans[p.N] = p;
+  }
+  *out = p;
+  return 0;
+}
+
+int set_lds(const void *fn, size_t bytes) {
+  if (bytes > 64 * 1024 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return hops_internal_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int hops_noise_embedding(const void *c_d, int n_lop, int n_t, double *s_w_d, double *nu_d,
+                                    void *stream) {
+  if (n_lop <= 0) return 0;
+  if (!c_d || !s_w_d || !nu_d) return hops_internal_set_error("hops_noise_embedding: NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  Plan p;
+  if (int rc = make_plan(n_t, st, &p)) return rc;
+  const size_t lds = sizeof(double2) * p.M;
+  if (int rc = set_lds((const void *)k_embedding, lds)) return rc;
+  k_embedding<<<n_lop, kThreads, lds, st>>>((const float2 *)c_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d);
+  if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_embedding launch failed");
+  return 0;
+}
+
+extern "C" int hops_noise_filter(const void *z_d, int R, int n_t, const double *s_w_d, const double *nu_d,
+                                 const int32_t *row_lop_d, void *out_d, void *stream) {
+  if (R <= 0) return 0;
+  if (!z_d || !s_w_d || !nu_d || !row_lop_d || !out_d)
+    return hops_internal_set_error("hops_noise_filter: NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  Plan p;
+  if (int rc = make_plan(n_t, st, &p)) return rc;
+  const size_t lds = sizeof(double2) * p.M;
+  if (int rc = set_lds((const void *)k_filter, lds)) return rc;
+  k_filter<<<R, kThreads, lds, st>>>((const float2 *)z_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d,
+                                      row_lop_d, (float2 *)out_d);
+  if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_filter launch failed");
+  return 0;
+}

@@ -1,0 +1,192 @@
+"""Device-side runtime: operator upload, batch buffers and the propagate call.
+
+PyTorch is used only to allocate HBM buffers and to name the current stream;
+all arithmetic runs in libhops_mi355x.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import HopsBatch, HopsOp, check, lib
+from .basis import BasisTables
+from .errors import EngineError
+from .params import hbar
+
+TILE_ROWS = 256
+N_XCD = 8
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise EngineError("no ROCm GPU visible: the HOPS engine runs only on the device")
+
+
+def _dev(arr, device, dtype=None):
+    """Upload a host array (at least one element, so the pointer is valid)."""
+    a = np.ascontiguousarray(arr)
+    if a.size == 0:
+        a = np.zeros(1, dtype=a.dtype)
+    t = torch.from_numpy(a.copy())
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device)
+
+
+def _stream_ptr(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class DeviceOperator:
+    """hops_op_t of one basis with its tables resident in HBM; the hierarchy
+    super-operator (ELL K2_kp1 + K2_km1) and K0 are generated on the device."""
+
+    def __init__(self, tables: BasisTables, precision: int, device="cuda:0"):
+        _require_gpu()
+        self.device = torch.device(device)
+        self.tables = tables
+        self.precision = precision
+        self._keep = {}
+        op = HopsOp()
+        for name in ("n_state", "n_hier", "n_mode", "n_lop", "n_act", "ell_w", "n_pat", "n_mode_abs"):
+            setattr(op, name, int(getattr(tables, name)))
+        op.n_first = len(tables.first)
+        for name, arr in tables.tables.items():
+            if arr is None:
+                setattr(op, name + "_d", None)
+                continue
+            t = _dev(arr, self.device)
+            self._keep[name] = t
+            setattr(op, name + "_d", t.data_ptr())
+        nr = tables.n_state * tables.n_hier
+        w = max(tables.ell_w, 1)
+        self.ell_col = torch.empty(w * nr, dtype=torch.int32, device=self.device)
+        vdt = torch.complex128 if precision == _lib.HOPS_PREC_FP64 else torch.complex64
+        self.ell_val = torch.empty(w * nr, dtype=vdt, device=self.device)
+        self.k0 = torch.empty(tables.n_hier, dtype=torch.complex128, device=self.device)
+        op.ell_col_d = self.ell_col.data_ptr()
+        op.ell_val_d = self.ell_val.data_ptr()
+        op.k0_d = self.k0.data_ptr()
+        self.op = op
+        with torch.cuda.device(self.device):
+            check(lib().hops_build_super(ctypes.byref(op), precision, _stream_ptr(self.device)))
+
+    @property
+    def n_rows(self):
+        return self.tables.n_state * self.tables.n_hier
+
+    def ell_dense(self):
+        """Download K2_kp1 + K2_km1 as (row, col, value) triplets (for tests)."""
+        nr = self.n_rows
+        w = self.tables.ell_w
+        col = self.ell_col.cpu().numpy()[: w * nr].reshape(w, nr)
+        val = self.ell_val.cpu().numpy()[: w * nr].reshape(w, nr)
+        rows = np.broadcast_to(np.arange(nr), (w, nr))
+        m = col >= 0
+        return rows[m], col[m], val[m]
+
+
+def tile_map(B, n_rows):
+    """Workgroup -> (trajectory, tile) list.  With at least 8 trajectories each
+    trajectory's tiles are pinned to one XCD (workgroup i runs on XCD i % 8),
+    so its neighbour gathers hit that XCD's L2; smaller batches spread every
+    trajectory over all XCDs."""
+    tpt = (n_rows + TILE_ROWS - 1) // TILE_ROWS
+    if tpt >= 65536 or B >= 32768:
+        raise EngineError("batch too large for the tile encoding")
+    if B < N_XCD:
+        return np.array([b * 65536 + t for b in range(B) for t in range(tpt)], dtype=np.int32)
+    queues = [[b * 65536 + t for b in range(x, B, N_XCD) for t in range(tpt)] for x in range(N_XCD)]
+    depth = max(len(q) for q in queues)
+    out = np.full(depth * N_XCD, -1, dtype=np.int32)
+    for x, q in enumerate(queues):
+        out[x: x + N_XCD * len(q): N_XCD] = q
+    return out
+
+
+class BatchState:
+    """Device buffers of B trajectories that share one DeviceOperator."""
+
+    def __init__(self, dop: DeviceOperator, B: int, eom_kind: int, noise1, noise2, n_t: int,
+                 store_slots: int = 0):
+        self.dop = dop
+        self.device = dop.device
+        t = dop.tables
+        self.B = B
+        self.eom_kind = eom_kind
+        nr = dop.n_rows
+        prec = dop.precision
+        cR = torch.complex128 if prec == _lib.HOPS_PREC_FP64 else torch.complex64
+        dev = self.device
+        self.phi = torch.zeros(B * nr, dtype=torch.complex128, device=dev)
+        self.y0 = torch.empty(B * nr if prec == _lib.HOPS_PREC_MIXED else 1, dtype=torch.complex64, device=dev)
+        self.ya = torch.empty(B * nr, dtype=cR, device=dev)
+        self.yb = torch.empty(B * nr, dtype=cR, device=dev)
+        self.acc = torch.empty(B * nr, dtype=cR, device=dev)
+        if noise1.shape != (B, t.n_act, n_t):
+            raise EngineError("noise1 shape %s != %s" % (tuple(noise1.shape), (B, t.n_act, n_t)))
+        self.z1 = noise1.contiguous()
+        self.z2 = None if noise2 is None else noise2.contiguous()
+        self.zmem = torch.zeros(B * t.n_mode_abs + 1, dtype=torch.complex128, device=dev)
+        self.ws_stride = 4 * t.n_mode_abs + 2
+        self.coef_stride = 4 * t.n_pat + 2 * t.n_mode + 4
+        self.ws = torch.zeros(B * self.ws_stride, dtype=torch.float64, device=dev)
+        self.coef = torch.zeros(B * self.coef_stride, dtype=torch.float64, device=dev)
+        self.store_slots = store_slots
+        self.psi_store = (torch.zeros(B * store_slots * t.n_state, dtype=torch.complex128, device=dev)
+                          if store_slots > 0 else None)
+        self.tiles = torch.from_numpy(tile_map(B, nr)).to(dev)
+        bt = HopsBatch()
+        bt.B = B
+        bt.precision = prec
+        bt.eom = eom_kind
+        bt.n_t = n_t
+        bt.coef_stride = self.coef_stride
+        bt.ws_stride = self.ws_stride
+        bt.store_stride = store_slots
+        bt.phi_d = self.phi.data_ptr()
+        bt.y0_d = self.y0.data_ptr()
+        bt.ya_d = self.ya.data_ptr()
+        bt.yb_d = self.yb.data_ptr()
+        bt.acc_d = self.acc.data_ptr()
+        bt.z1_d = self.z1.data_ptr()
+        bt.z2_d = None if self.z2 is None else self.z2.data_ptr()
+        bt.zmem_d = self.zmem.data_ptr()
+        bt.ws_d = self.ws.data_ptr()
+        bt.coef_d = self.coef.data_ptr()
+        bt.psi_store_d = None if self.psi_store is None else self.psi_store.data_ptr()
+        bt.tile_d = self.tiles.data_ptr()
+        bt.n_tiles = int(self.tiles.numel())
+        bt.tile_rows = TILE_ROWS
+        self.bt = bt
+
+    def set_phi(self, phi_host):
+        """phi_host: complex [B, n_rows] (or [n_rows], broadcast)."""
+        nr = self.dop.n_rows
+        a = np.asarray(phi_host, dtype=np.complex128)
+        a = np.broadcast_to(a.reshape(-1, nr), (self.B, nr))
+        self.phi.copy_(torch.from_numpy(np.ascontiguousarray(a).reshape(-1)).to(self.device))
+
+    def phi_host(self):
+        return self.phi.cpu().numpy().reshape(self.B, -1)
+
+    def zmem_host(self):
+        return self.zmem.cpu().numpy()[: self.B * self.dop.tables.n_mode_abs].reshape(self.B, -1)
+
+    def psi_store_host(self, n_slots):
+        n = self.dop.tables.n_state
+        a = self.psi_store.cpu().numpy().reshape(self.B, self.store_slots, n)
+        return a[:, :n_slots]
+
+    def propagate(self, it0: int, n_steps: int, tau: float, noise_step: int, store0: int = 0):
+        """n_steps RK4 steps of size tau from noise index it0."""
+        self.bt.dt_over_hbar = float(tau / hbar)
+        self.bt.noise_step = int(noise_step)
+        if self.bt.psi_store_d and store0 + n_steps > self.store_slots:
+            raise EngineError("psi store too small (%d + %d > %d)" % (store0, n_steps, self.store_slots))
+        with torch.cuda.device(self.device):
+            check(lib().hops_propagate(ctypes.byref(self.dop.op), ctypes.byref(self.bt), int(it0),
+                                       int(n_steps), int(store0), _stream_ptr(self.device)))

@@ -1,0 +1,167 @@
+"""Noise of a HOPS trajectory (noise/hops_noise.py, noise/prepare_functions.py).
+
+Raw white noise is drawn on the host with NumPy, because the reference defines
+its values by NumPy's PCG64 streams (``PCG64(seed).jumped(lop)``) and
+``normal``/``random`` draws; the correlation filter (circulant embedding and
+the two FFTs per row) runs on the device in noise_fft.hip.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .basis import SystemInfo, operator_signature
+from .errors import UnsupportedRequest
+from .params import NOISE_DICT_DEFAULT, NOISE_TYPE_DEFAULT, initialize_dict, precision
+
+
+def noise_time_axis(tlen, tau):
+    """T_AXIS of the noise (noise/hops_noise.py:161-163)."""
+    return np.arange(int(np.ceil(tlen / tau)) + 1) * tau
+
+
+def raw_slot_index(ntaus):
+    """Slots of the real/imaginary (or radius/phase) draws so that a prefix of
+    the noise does not depend on the axis length (noise/hops_noise.py:592-628):
+    slot pairs (0,1), (2,3), ... walk up from the start and the last pairs walk
+    back down from the end."""
+    m1 = ntaus - 1
+    a = np.concatenate([[0], np.arange(2, 4 * (ntaus - 2), 4), [4 * m1 - 2],
+                        np.arange(4 * (ntaus - 2), 3, -4)]).astype(np.int64)
+    b = np.concatenate([[1], np.arange(3, 4 * (ntaus - 1) - 1, 4), [4 * m1 - 1],
+                        np.arange(4 * (m1 - 1) + 1, 4, -4)]).astype(np.int64)
+    return a, b
+
+
+class HopsNoise:
+    """Parameters and host-side raw draws of one noise trajectory."""
+
+    def __init__(self, noise_param, info: SystemInfo, system_param, noise_type=1):
+        p = dict(noise_param)
+        tag = "1"
+        if noise_type == 1 or p.get("MODEL") == "ZERO":
+            if noise_type == 1 and p.get("FLAG_REAL", False):
+                p["FLAG_REAL"] = False
+        else:
+            tag = "2"
+            if "FLAG_REAL" not in p:
+                p["FLAG_REAL"] = True
+        self.param = initialize_dict(p, NOISE_DICT_DEFAULT, NOISE_TYPE_DEFAULT, "HopsNoise")
+        if self.param["MODEL"] not in ("FFT_FILTER", "ZERO", "PRE_CALCULATED"):
+            raise UnsupportedRequest("Noise.param[MODEL] {}".format(self.param["MODEL"]), "Noise.prepare_noise")
+        for key in ("INTERPOLATE",):
+            if self.param[key]:
+                raise UnsupportedRequest("noise {} = True".format(key), "HopsNoise")
+        if self.param["NOISE_WINDOW"] is not None:
+            raise UnsupportedRequest("NOISE_WINDOW", "HopsNoise")
+        self.t_axis = noise_time_axis(self.param["TLEN"], self.param["TAU"])
+        self.n_t = len(self.t_axis)
+        key = "L_NOISE" + tag
+        lops = system_param.get(key, system_param["L_HIER"]) if tag == "1" else system_param[key]
+        self.n_l2 = len(set(operator_signature(L) for L in lops))
+        self.corr = system_param.get("ALPHA_NOISE" + tag)
+        self.corr_param = system_param.get("PARAM_NOISE" + tag)
+        self.modes_by_l = info.noise_modes_by_l.get(tag, [])
+
+    @property
+    def model(self):
+        return self.param["MODEL"]
+
+    def alpha(self):
+        """Correlation function of each L-operator, complex64 [n_l2, n_t]
+        (noise/hops_noise.py:167-192, cast at :236)."""
+        a = np.zeros((self.n_l2, self.n_t), dtype=np.complex128)
+        for l in range(self.n_l2):
+            for i in self.modes_by_l[l]:
+                a[l] += self.corr(self.t_axis, *self.corr_param[i])
+        return np.complex64(a)
+
+    def raw(self):
+        """Uncorrelated complex64 noise [n_l2, 2(n_t-1)] (noise/hops_noise.py:453-590)."""
+        seed = self.param["SEED"]
+        nt = self.n_t
+        if isinstance(seed, str):
+            if not os.path.isfile(seed):
+                raise UnsupportedRequest("Noise.param[SEED] {} is not the address of a valid file ".format(seed),
+                                         "Noise._prepare_rand", True)
+            seed = np.load(seed, allow_pickle=False)
+        if isinstance(seed, np.ndarray):
+            if seed.shape != (self.n_l2, 2 * (nt - 1)):
+                raise UnsupportedRequest("Noise.param[SEED] is an array of the wrong length",
+                                         "Noise._prepare_rand", True)
+            return np.complex64(seed)
+        draws = np.zeros((self.n_l2, 4 * (nt - 1)))
+        for l in range(self.n_l2):
+            rng = np.random.default_rng(np.random.PCG64(seed=seed).jumped(l))
+            if self.param["RAND_MODEL"] == "BOX_MULLER":
+                draws[l] = rng.random(size=4 * (nt - 1))
+            elif self.param["RAND_MODEL"] == "SUM_GAUSSIAN":
+                draws[l] = rng.normal(loc=0, size=4 * (nt - 1))
+            else:
+                raise UnsupportedRequest("Noise.param[RAND_MODEL] {}".format(self.param["RAND_MODEL"]),
+                                         "Noise._prepare_rand")
+        ia, ib = raw_slot_index(nt)
+        if self.param["RAND_MODEL"] == "BOX_MULLER":
+            return np.complex64(np.sqrt(-2.0 * np.log(draws[:, ia])) * np.exp(2.0j * np.pi * draws[:, ib]))
+        return np.complex64(draws[:, ia] + 1j * draws[:, ib])
+
+    def precalculated(self):
+        seed = self.param["SEED"]
+        if isinstance(seed, str):
+            if not os.path.isfile(seed) or not seed.endswith(".npy"):
+                raise UnsupportedRequest("Noise.param[SEED] {} is not the address of a valid file".format(seed),
+                                         "Noise.prepare_noise", True)
+            seed = np.load(seed, allow_pickle=False)
+        arr = np.asarray(seed)
+        if arr.shape != (self.n_l2, self.n_t):
+            raise UnsupportedRequest("Noise.param[SEED] is an array of the wrong length",
+                                     "Noise.prepare_noise", True)
+        return np.complex64(arr)
+
+    def time_index(self, t):
+        """Index of t on the noise grid; off-grid times are errors
+        (noise/hops_noise.py:430-447)."""
+        hit = np.abs(self.t_axis - t) < precision
+        if np.sum(hit) != 1:
+            raise UnsupportedRequest("Off axis t-samples when INTERPOLATE = False", "NoiseModel.get_noise()")
+        return int(np.where(hit)[0][0])
+
+
+def device_noise(noises, rows, device, stream_ptr):
+    """Correlated noise of B trajectories for the hierarchy L-operators
+    ``rows`` (absolute indices), complex64 [B, len(rows), n_t] in HBM.
+    All noises must share the system (same correlation functions)."""
+    B = len(noises)
+    nz = noises[0]
+    nt = nz.n_t
+    rows = [int(r) for r in rows]
+    out = torch.zeros((B, len(rows), nt), dtype=torch.complex64, device=device)
+    if nz.model == "ZERO":
+        return out
+    if nz.model == "PRE_CALCULATED":
+        host = np.stack([n.precalculated()[rows] for n in noises])
+        out.copy_(torch.from_numpy(host).to(device))
+    else:
+        alpha = nz.alpha()
+        N = 2 * (nt - 1)
+        c_d = torch.from_numpy(np.ascontiguousarray(alpha)).to(device)
+        s_w = torch.empty((nz.n_l2, N), dtype=torch.float64, device=device)
+        nu = torch.empty(nz.n_l2, dtype=torch.float64, device=device)
+        check(lib().hops_noise_embedding(ctypes.c_void_p(c_d.data_ptr()), nz.n_l2, nt,
+                                         ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
+                                         stream_ptr))
+        raw = np.stack([n.raw()[rows] for n in noises]).reshape(B * len(rows), N)
+        z_d = torch.from_numpy(np.ascontiguousarray(raw)).to(device)
+        row_lop = torch.from_numpy(np.tile(np.array(rows, dtype=np.int32), B)).to(device)
+        check(lib().hops_noise_filter(ctypes.c_void_p(z_d.data_ptr()), B * len(rows), nt,
+                                      ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
+                                      ctypes.c_void_p(row_lop.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                      stream_ptr))
+        torch.cuda.current_stream(device).synchronize()
+    if nz.param["FLAG_REAL"]:
+        torch.view_as_real(out)[..., 1] = 0
+    return out

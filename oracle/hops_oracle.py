@@ -1,0 +1,647 @@
+"""FP64 NumPy/SciPy restatement of the reference HOPS propagation path.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  The functions below
+follow the reference step by step so that their outputs can be compared with
+the HIP engine; where the reference's arithmetic order fixes the last bits of
+a golden fixture (the super-operator values, the noise filter), the same NumPy
+operations are used in the same order and dtype.
+
+Citations are to MesoHOPS v1.6.0 ``src/mesohops/...``.
+"""
+from __future__ import annotations
+
+import copy
+import itertools
+from collections import Counter
+
+import numpy as np
+import scipy.sparse as sp
+
+# util/physical_constants.py:21-28
+H_PLANCK = 33357.179
+HBAR = H_PLANCK / (2 * np.pi)
+KB = 0.69503
+PRECISION = 1e-8
+
+
+# ---------------------------------------------------------------------------
+# Bath correlation functions
+# ---------------------------------------------------------------------------
+def bcf_exp(t_axis, g, w):
+    """g * exp(-w t / hbar)  (trajectory/exp_noise.py:5-25)."""
+    return g * np.exp(-w * t_axis / HBAR)
+
+
+def bcf_convert_dl_to_exp(lambda_dl, gamma_dl, temp, k_matsubara=0):
+    """High-temperature Drude-Lorentz mode plus Matsubara modes
+    (util/bath_corr_functions.py:70-112)."""
+    beta = 1 / (KB * temp)
+    g_exp = 2 * lambda_dl / beta - 1j * lambda_dl * gamma_dl
+    w_exp = gamma_dl
+    nu = 2 * np.pi / beta
+
+    def spectral(w):
+        return 2 * lambda_dl * gamma_dl * w / (w ** 2 + gamma_dl ** 2)
+
+    extra = []
+    for k in np.arange(k_matsubara) + 1:
+        w_k = k * nu
+        g_k = 2j * spectral(1j * w_k) / beta
+        g_exp += 2 * lambda_dl * (1 / beta) * (2 * gamma_dl ** 2) / (gamma_dl ** 2 - w_k ** 2)
+        extra += [g_k, w_k]
+    return [g_exp, w_exp] + extra
+
+
+# ---------------------------------------------------------------------------
+# System dictionary (basis/system_functions.py:10-171)
+# ---------------------------------------------------------------------------
+def l_signature(op):
+    """Nonzero pattern of an operator as a hashable tuple
+    (util/helper_functions.py:8-32)."""
+    if sp.issparse(op):
+        if op.getnnz() == 0:
+            return tuple()
+        return tuple(tuple(x) for x in op.nonzero())
+    arr = np.asarray(op)
+    if len(arr) == 0:
+        return tuple()
+    return tuple(tuple(x) for x in np.nonzero(arr))
+
+
+def _states_of(op):
+    """States touched by an L-operator (util/helper_functions.py:50-66)."""
+    if sp.issparse(op):
+        r, c = op.nonzero()
+    else:
+        r, c = np.nonzero(np.asarray(op))
+    return tuple(sorted(set(r.tolist()) | set(c.tolist())))
+
+
+def derive_system(system_param):
+    """Derived system dictionary; restates initialize_system_dict
+    (basis/system_functions.py:10-171) for the keys the propagation uses."""
+    p = dict(system_param)
+    ham = p["HAMILTONIAN"]
+    H = ham.toarray() if sp.issparse(ham) else np.asarray(ham)
+    p["NSTATES"] = H.shape[0]
+    p["H_DENSE"] = np.array(H, dtype=np.complex128)
+    p["N_HMODES"] = len(p["GW_SYSBATH"])
+    p["G"] = np.array([g for (g, w) in p["GW_SYSBATH"]])
+    p["W"] = np.array([w for (g, w) in p["GW_SYSBATH"]])
+    p["LIST_STATE_INDICES_BY_HMODE"] = [_states_of(L) for L in p["L_HIER"]]
+    p["LIST_HMODE_INDICES_BY_STATE"] = [[] for _ in range(p["NSTATES"])]
+    for hm, sts in enumerate(p["LIST_STATE_INDICES_BY_HMODE"]):
+        for s in sts:
+            p["LIST_HMODE_INDICES_BY_STATE"][s].append(hm)
+    if "L_LT_CORR" not in p:
+        p["L_LT_CORR"] = []
+        p["PARAM_LT_CORR"] = []
+    sigs = [l_signature(L) for L in p["L_HIER"]]
+    uniq = {}
+    p["LIST_INDEX_L2_BY_HMODE"] = []
+    p["LIST_L2_COO"] = []
+    p["LIST_STATE_INDICES_BY_INDEX_L2"] = []
+    for i, s in enumerate(sigs):
+        if s not in uniq:
+            uniq[s] = len(uniq)
+            tmp = sp.coo_matrix(p["L_HIER"][i])
+            tmp.eliminate_zeros()
+            p["LIST_L2_COO"].append(tmp)
+            p["LIST_STATE_INDICES_BY_INDEX_L2"].append(p["LIST_STATE_INDICES_BY_HMODE"][i])
+        p["LIST_INDEX_L2_BY_HMODE"].append(uniq[s])
+    p["N_L2"] = len(uniq)
+    p["LIST_INDEX_L2_BY_STATE_INDICES"] = [[] for _ in range(p["NSTATES"])]
+    for il, sts in enumerate(p["LIST_STATE_INDICES_BY_INDEX_L2"]):
+        for s in sts:
+            p["LIST_INDEX_L2_BY_STATE_INDICES"][s].append(il)
+    p["LIST_LT_PARAM"] = [0] * p["N_L2"]
+    for i, L in enumerate(p["L_LT_CORR"]):
+        s = l_signature(L)
+        if s in uniq:
+            p["LIST_LT_PARAM"][uniq[s]] += p["PARAM_LT_CORR"][i]
+    for tag in ("1", "2"):
+        key = "L_NOISE" + tag
+        if key not in p:
+            continue
+        p["LIST_NMODE%s_BY_INDEX_L2" % tag] = [[] for _ in range(len(sigs))]
+        p["LIST_INDEX_L2_BY_NMODE%s" % tag] = [None] * len(p["PARAM_NOISE" + tag])
+        for i, L in enumerate(p[key]):
+            s = l_signature(L)
+            if s in uniq:
+                p["LIST_INDEX_L2_BY_NMODE%s" % tag][i] = uniq[s]
+                p["LIST_NMODE%s_BY_INDEX_L2" % tag][uniq[s]].append(i)
+        p["N_L2_NOISE" + tag] = len(set(l_signature(L) for L in p[key]))
+    # Hamiltonian couplings used by the adaptive state basis
+    # (basis/system_functions.py:41-46)
+    hc = sp.coo_matrix(H)
+    p["COUPLED_STATES"] = [[] for _ in range(p["NSTATES"])]
+    for r, c in zip(hc.row, hc.col):
+        p["COUPLED_STATES"][r].append(c)
+    p["COUPLED_STATES"] = [sorted(x) for x in p["COUPLED_STATES"]]
+    dest = [[] for _ in range(p["NSTATES"])]
+    for s in uniq:
+        if len(s) == 0:
+            continue
+        for a, b in zip(s[0], s[1]):
+            dest[a].append(b)
+    p["LIST_DESTINATION_STATES_BY_STATE_INDEX"] = [sorted(set(x)) for x in dest]
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Hierarchy (basis/hops_hierarchy.py:340-368, basis/hops_aux.py:171-201,460-480)
+# ---------------------------------------------------------------------------
+def aux_sort_key(aux):
+    """Reference ordering: by ID-string length (= depth), then by the ID string,
+    which for fixed-width mode fields is the lexicographic order of the
+    expanded (sorted) mode sequence (basis/hops_aux.py:171-201,460-480)."""
+    expanded = tuple(m for (m, v) in aux for _ in range(v))
+    return (len(expanded), expanded)
+
+
+def triangular_hierarchy(n_modes, maxhier):
+    """All multi-indices of depth <= maxhier as sparse (mode, value) tuples,
+    sorted as the reference sorts them (hops_hierarchy.py:340-368,582)."""
+    out = []
+    for k in range(maxhier + 1):
+        for raw in itertools.combinations_with_replacement(range(n_modes), k):
+            c = Counter(raw)
+            out.append(tuple((m, c[m]) for m in sorted(c)))
+    out.sort(key=aux_sort_key)
+    return out
+
+
+def aux_dict(aux):
+    return {m: v for (m, v) in aux}
+
+
+def aux_step(aux, mode, step):
+    d = aux_dict(aux)
+    d[mode] = d.get(mode, 0) + step
+    if d[mode] < 0:
+        return None
+    if d[mode] == 0:
+        del d[mode]
+    return tuple((m, d[m]) for m in sorted(d))
+
+
+# ---------------------------------------------------------------------------
+# Basis bookkeeping (basis/hops_system.py:206-286, basis/hops_modes.py:111-171,
+# basis/hops_basis.py:90-115)
+# ---------------------------------------------------------------------------
+class Basis:
+    """State list + auxiliary list + derived relative indexing."""
+
+    def __init__(self, sysd, state_list, aux_list):
+        self.sysd = sysd
+        self.state_list = np.array(sorted(int(s) for s in state_list), dtype=int)
+        self.aux_list = sorted(aux_list, key=aux_sort_key)
+        assert len(self.aux_list[0]) == 0, "zero auxiliary must be present"
+        self.aux_index = {a: i for i, a in enumerate(self.aux_list)}
+        st = self.state_list
+        n = len(st)
+        self.n_state = n
+        self.n_hier = len(self.aux_list)
+        modes_in_use = sorted({m for a in self.aux_list for (m, v) in a})
+        state_modes = sorted({m for s in st for m in sysd["LIST_HMODE_INDICES_BY_STATE"][s]})
+        self.state_modes = np.array(state_modes, dtype=int)
+        self.modes = np.array(sorted(set(modes_in_use) | set(state_modes)), dtype=int)
+        self.rel_mode = {int(m): i for i, m in enumerate(self.modes)}
+        l_by_hmode = sysd["LIST_INDEX_L2_BY_HMODE"]
+        self.absL = np.array(sorted({l_by_hmode[m] for m in self.modes}), dtype=int)
+        self.relL_by_mode = np.array([list(self.absL).index(l_by_hmode[m]) for m in self.modes], dtype=int)
+        self.absL_active = np.array(
+            sorted({l for s in st for l in sysd["LIST_INDEX_L2_BY_STATE_INDICES"][s]}), dtype=int)
+        self.relL_active = np.array([list(self.absL).index(l) for l in self.absL_active], dtype=int)
+        self.g = np.array([sysd["G"][m] for m in self.modes])
+        self.w = np.array([sysd["W"][m] for m in self.modes])
+        # reduced L-operators (basis/hops_system.py:318-354)
+        pos = {int(s): i for i, s in enumerate(st)}
+        self.L_coo = []
+        for l in self.absL:
+            c = sysd["LIST_L2_COO"][l]
+            keep = [(pos[int(i)], pos[int(j)], d) for (i, j, d) in zip(c.row, c.col, c.data)
+                    if int(i) in pos and int(j) in pos]
+            if keep:
+                r = [k[0] for k in keep]
+                cc = [k[1] for k in keep]
+                dd = np.array([k[2] for k in keep])
+                self.L_coo.append(sp.coo_matrix((dd, (r, cc)), shape=(n, n)))
+            else:
+                self.L_coo.append(sp.coo_matrix((n, n)))
+        self.L_csr = [sp.csr_array(L) for L in self.L_coo]
+        self.L_sq_csr = [L @ L for L in self.L_csr]
+        self.H = sysd["H_DENSE"][np.ix_(st, st)]
+        lt = np.array(sysd["LIST_LT_PARAM"], dtype=np.complex128)
+        self.lt_active = lt[self.absL_active] if len(self.absL_active) else np.zeros(0, complex)
+        self._build_super()
+
+    # -- hierarchy super-operators: eom/eom_hops_ksuper.py:575-668, :36-230 --
+    def _build_super(self):
+        n, nh = self.n_state, self.n_hier
+        W = self.sysd["W"]
+        K0 = np.zeros(nh, dtype=np.complex128)
+        for i, a in enumerate(self.aux_list):
+            if a:
+                arr = np.array(a, dtype=int)
+                K0[i] = -np.dot(arr[:, 1], W[arr[:, 0]])
+        self.K0 = K0
+        kr, kc, kd = [], [], []
+        mr, md = [], []
+        zr = [[] for _ in self.absL]
+        zc = [[] for _ in self.absL]
+        zd = [[] for _ in self.absL]
+        # connections (lower aux, upper aux, depth of upper in the mode)
+        self.conn = []
+        for lm, m in enumerate(self.modes):
+            lows, ups, vals = [], [], []
+            for i, a in enumerate(self.aux_list):
+                up = aux_step(a, int(m), 1)
+                j = self.aux_index.get(up)
+                if j is not None:
+                    lows.append(i)
+                    ups.append(j)
+                    vals.append(aux_dict(up)[int(m)])
+                    self.conn.append((i, j, lm, vals[-1]))
+            if not lows:
+                continue
+            il = self.relL_by_mode[lm]
+            L = self.L_coo[il]
+            gw = self.g[lm] / self.w[lm]
+            lows = np.array(lows)
+            ups = np.array(ups)
+            nd = len(L.data)
+            kd.extend(np.repeat(np.ones(len(lows)) * -gw, nd) * np.tile(L.data, len(lows)))
+            kr.extend(np.repeat(lows, nd) * n + np.tile(L.row, len(lows)))
+            kc.extend(np.repeat(ups, nd) * n + np.tile(L.col, len(lows)))
+            md.extend(np.repeat(np.array(vals) * self.w[lm], nd) * np.tile(np.conj(L.data), len(lows)))
+            zd[il].extend(np.ones(len(lows)) * gw)
+            zr[il].extend(lows)
+            zc[il].extend(ups)
+        N = n * nh
+        self.K2_k = sp.coo_matrix((K0, (np.arange(nh), np.arange(nh))), shape=(nh, nh),
+                                  dtype=np.complex128).tocsr()
+        self.K2_k.eliminate_zeros()
+        self.K2_kp1 = sp.coo_matrix((kd, (kr, kc)), shape=(N, N), dtype=np.complex128).tocsr()
+        self.K2_km1 = sp.coo_matrix((md, (kc, kr)), shape=(N, N), dtype=np.complex128).tocsr()
+        self.Z2_kp1 = [sp.coo_matrix((zd[i], (zr[i], zc[i])), shape=(nh, nh),
+                                     dtype=np.complex128).tocsr() for i in range(len(self.absL))]
+        self.K2_stable = self.K2_kp1 + self.K2_km1
+        # first-level auxiliaries e_m used by the normalisation correction
+        # (eom/hops_eom.py:178-193)
+        act = list(self.relL_active)
+        self.first_level = []
+        for lm, m in enumerate(self.modes):
+            j = self.aux_index.get(((int(m), 1),))
+            if j is None:
+                continue
+            il = self.relL_by_mode[lm]
+            if il in act:
+                self.first_level.append((j, act.index(il), lm))
+
+
+# ---------------------------------------------------------------------------
+# Derivative (eom/hops_eom.py:118-552, eom/eom_functions.py:11-309)
+# ---------------------------------------------------------------------------
+def expectation(op, vec):
+    """<vec|op|vec>/<vec|vec>  (eom/eom_functions.py:11-29)."""
+    return (np.conj(vec) @ (op @ vec)) / (np.conj(vec) @ vec)
+
+
+def make_derivative(basis, eom_type):
+    """Returns dsystem_dt(phi, z_mem, z1, z2) -> (hbar dphi/dt, hbar dz_mem/dt).
+
+    z_mem is a dense complex vector over the absolute hierarchy modes; z1 and
+    z2 are the noise values of the L-operators of the basis in relative order
+    (mode.list_absindex_L2).  Restates the closures at eom/hops_eom.py:196-441
+    (NONLINEAR / NORMALIZED NONLINEAR) and :445-542 (LINEAR)."""
+    b = basis
+    n, nh = b.n_state, b.n_hier
+    H = b.H
+    act = list(b.relL_active)
+    normalized = eom_type == "NORMALIZED NONLINEAR"
+    sysd = b.sysd
+
+    def rows_apply(op, phi):
+        # op (n x n) applied to every auxiliary block of phi
+        return (op @ phi.reshape(nh, n).T).T.reshape(-1)
+
+    if eom_type == "LINEAR":
+        def dsystem_dt(phi, z_mem, z1, z2):
+            d = b.K2_stable @ phi
+            d += (b.K0[:, None] * phi.reshape(nh, n)).reshape(-1)
+            d += rows_apply(-1j * H, phi)
+            if np.any(b.lt_active):
+                corr = -1 * np.sum(np.array(b.lt_active) * np.array(b.L_sq_csr, dtype=object), axis=0)
+                d[:n] += corr @ phi[:n]
+            zhat = np.conj(z1)
+            for j in range(len(b.L_csr)):
+                d += (zhat[j] - 1j * z2[j]) * rows_apply(b.L_csr[j], phi)
+            return d, 0 * z_mem
+        return dsystem_dt
+
+    if eom_type not in ("NONLINEAR", "NORMALIZED NONLINEAR"):
+        raise ValueError(eom_type)
+
+    def dsystem_dt(phi, z_mem, z1, z2):
+        # z-hat: conj(noise) + memory compressed by L-operator (eom_functions.py:32-62)
+        comp = np.zeros(len(b.absL), dtype=np.complex128)
+        for i, m in enumerate(b.modes):
+            comp[b.relL_by_mode[i]] += z_mem[m]
+        zhat = (np.conj(z1) + comp)[act]
+        ztmp2 = np.asarray(z2)[act]
+        psi = phi[:n]
+        avg = [expectation(b.L_csr[j], psi) for j in act]
+        norm_corr = 0
+        if normalized:
+            # eom_functions.py:163-213
+            delta = np.dot(zhat, avg)
+            for (i_aux, l_ind, lm) in b.first_level:
+                phi1 = (b.g[lm] / b.w[lm]) * phi[n * i_aux:n * (i_aux + 1)]
+                delta -= np.conj(psi) @ (b.L_csr[act[l_ind]] @ phi1)
+                delta += (np.conj(psi) @ phi1) * avg[l_ind]
+            norm_corr = np.real(delta)
+        lt = b.lt_active
+        has_lt = bool(np.any(lt))
+        if has_lt:
+            avg_sq = [expectation(b.L_sq_csr[j], psi) for j in act]
+            L_act = [b.L_csr[j] for j in act]
+            Lsq_act = [b.L_sq_csr[j] for j in act]
+            c_phys = sum((lt[k] * avg[k]) * L_act[k] - lt[k] * Lsq_act[k] for k in range(len(act)))
+            c_hier = sum((np.conj(lt[k]) * avg[k]) * L_act[k] for k in range(len(act)))
+            lt_norm = np.sum(np.real(lt) * (2 * np.array(avg) ** 2 - np.array(avg_sq)))
+        d = b.K2_stable @ phi
+        d += (b.K0[:, None] * phi.reshape(nh, n)).reshape(-1)
+        d += rows_apply(-1j * H, phi)
+        if has_lt:
+            d += rows_apply(c_hier, phi)
+            d[:n] += c_phys @ phi[:n]
+            norm_corr += lt_norm
+        if normalized:
+            d -= norm_corr * phi
+        dv = d.reshape(nh, n)
+        pv = phi.reshape(nh, n)
+        for k, j in enumerate(act):
+            dv += ((zhat[k] - 1j * ztmp2[k]) * (b.L_csr[j] @ pv.T)).T
+            dv += np.conj(avg[k]) * (b.Z2_kp1[j] @ pv)
+        # memory drift (eom_functions.py:65-160)
+        dz = np.zeros(len(z_mem), dtype=np.complex128)
+        absL_by_hmode = sysd["LIST_INDEX_L2_BY_HMODE"]
+        act_abs = list(b.absL_active)
+        for m in b.modes:
+            la = absL_by_hmode[m]
+            l_avg = avg[act_abs.index(la)] if la in act_abs else 0
+            dz[m] = l_avg * np.conj(sysd["G"][m]) - np.conj(sysd["W"][m]) * z_mem[m]
+        in_basis = set(int(m) for m in b.modes)
+        for m in np.where(z_mem > PRECISION)[0]:
+            if int(m) not in in_basis:
+                dz[m] -= np.conj(sysd["W"][m]) * z_mem[m]
+        return d, dz
+
+    return dsystem_dt
+
+
+# ---------------------------------------------------------------------------
+# RK4 (integrator/integrator_rk.py:10-75)
+# ---------------------------------------------------------------------------
+def rk4_step(dsystem_dt, phi, z_mem, z1, z2, tau):
+    """z1, z2: noise at [t, t+tau/2, t+tau] (columns); stage noise map [0,1,1,2]."""
+    c = [0.0, 0.5, 0.5, 1.0]
+    iz = [0, 1, 1, 2]
+    k = [None] * 4
+    kz = [None] * 4
+    for i in range(4):
+        if i == 0:
+            pt, zt = phi.copy(), z_mem.copy()
+        else:
+            zt = z_mem + c[i] * kz[i - 1] * tau / HBAR
+            pt = phi + c[i] * k[i - 1] * tau / HBAR
+        k[i], kz[i] = dsystem_dt(pt, zt, z1[:, iz[i]], z2[:, iz[i]])
+    phi = phi + tau / HBAR * (k[0] + 2.0 * k[1] + 2.0 * k[2] + k[3]) / 6.0
+    z_mem = z_mem + tau / HBAR * (kz[0] + 2.0 * kz[1] + 2.0 * kz[2] + kz[3]) / 6.0
+    return phi, z_mem
+
+
+# ---------------------------------------------------------------------------
+# Noise (noise/hops_noise.py, noise/prepare_functions.py)
+# ---------------------------------------------------------------------------
+NOISE_DEFAULTS = {
+    "SEED": None, "MODEL": "FFT_FILTER", "TLEN": 1000.0, "TAU": 1.0,
+    "INTERPOLATE": False, "RAND_MODEL": "SUM_GAUSSIAN", "STORE_RAW_NOISE": False,
+    "NOISE_WINDOW": None, "ADAPTIVE": False, "FLAG_REAL": False,
+}
+
+
+def noise_time_axis(tlen, tau):
+    """hops_noise.py:161-163."""
+    nstep = int(np.ceil(tlen / tau)) + 1
+    return np.arange(nstep) * tau
+
+
+def raw_index(ntaus):
+    """Assignment of raw draws to re/im (or radius/phase) slots so that a
+    prefix is independent of the time-axis length (hops_noise.py:592-628)."""
+    m1 = ntaus - 1
+    a = [0]
+    a.extend(np.arange(2, 4 * (ntaus - 2), 4))
+    a.append(4 * m1 - 2)
+    a.extend(np.arange(4 * (ntaus - 2), 3, -4))
+    b = [1]
+    b.extend(np.arange(3, 4 * (ntaus - 1) - 1, 4))
+    b.append(4 * m1 - 1)
+    b.extend(np.arange(4 * (m1 - 1) + 1, 4, -4))
+    return a, b
+
+
+def raw_draws(seed, lops, ntaus, rand_model):
+    """Per-L stream PCG64(seed).jumped(lop) (hops_noise.py:558-590)."""
+    out = np.zeros((len(lops), 4 * (ntaus - 1)))
+    for i, lop in enumerate(lops):
+        rng = np.random.default_rng(np.random.PCG64(seed=seed).jumped(int(lop)))
+        if rand_model == "BOX_MULLER":
+            out[i, :] = rng.random(size=4 * (ntaus - 1))
+        elif rand_model == "SUM_GAUSSIAN":
+            out[i, :] = rng.normal(loc=0, size=4 * (ntaus - 1))
+        else:
+            raise ValueError(rand_model)
+    return out
+
+
+def uncorrelated_noise(seed, lops, ntaus, rand_model):
+    """Complex64 white noise of shape [n_lop, 2(ntaus-1)] (hops_noise.py:453-556)."""
+    r = raw_draws(seed, lops, ntaus, rand_model)
+    ia, ib = raw_index(ntaus)
+    if rand_model == "BOX_MULLER":
+        g, ph = r[:, np.array(ia)], r[:, np.array(ib)]
+        return np.complex64(np.sqrt(-2.0 * np.log(g)) * np.exp(2.0j * np.pi * ph))
+    return np.complex64(r[:, np.array(ia)] + 1j * r[:, np.array(ib)])
+
+
+def correlated_noise(c_t, z_t):
+    """Circulant-embedding FFT filter (Percival 2006), in the dtypes the
+    reference uses (hops_noise.py:630-716)."""
+    n = len(c_t[0, :])
+    ang = np.array(np.angle(c_t), dtype=np.float64)
+    nu = ang[:, -1] / ((n - 1.0) * 2.0 * np.pi)
+    E2 = np.exp(-1.0j * 2.0 * np.pi * np.outer(nu, np.arange(n)))
+    tc = np.abs(c_t) * np.exp(1.0j * ang) * E2
+    s_w = np.zeros([len(tc[:, 0]), 2 * n - 2])
+    for l in range(len(tc[:, 0])):
+        s_w[l, :] = np.real(np.fft.fft(np.concatenate([tc[l, :], np.conj(np.flip(tc[l, 1:-1]))])))
+    if np.min(s_w) < 0:
+        s_w[np.where(s_w < 0)] = 0
+    z_w = np.zeros([len(z_t[:, 0]), len(z_t[0, :])], dtype=np.complex128)
+    for l in range(len(z_t[:, 0])):
+        z_w[l, :] = np.fft.fft(z_t[l, :])
+    ty = np.zeros_like(E2)
+    for l in range(len(z_w[:, 0])):
+        ty[l, :] = np.fft.ifft(np.abs(z_w[l, :]) * np.exp(1.0j * np.angle(z_w[l, :]))
+                               * np.sqrt(s_w[l, :] / 2.0))[:n]
+    return np.abs(ty) * np.exp(1.0j * np.angle(ty)) * np.conj(E2)
+
+
+class Noise:
+    """Dense (non-windowed) restatement of HopsNoise for the FFT_FILTER, ZERO
+    and PRE_CALCULATED models (hops_noise.py:85-451)."""
+
+    def __init__(self, noise_param, sysd, noise_type=1):
+        p = copy.deepcopy(NOISE_DEFAULTS)
+        p.update(noise_param)
+        tag = str(noise_type)
+        if noise_type == 1 or p["MODEL"] == "ZERO":
+            tag = "1"
+            if noise_type == 1:
+                p["FLAG_REAL"] = False
+        elif "FLAG_REAL" not in noise_param:
+            p["FLAG_REAL"] = True
+        self.p = p
+        self.t_axis = noise_time_axis(p["TLEN"], p["TAU"])
+        self.n_l2 = len(set(l_signature(L) for L in sysd["L_NOISE" + tag]))
+        self.corr = sysd["ALPHA_NOISE" + tag]
+        self.corr_param = sysd["PARAM_NOISE" + tag]
+        self.nmode_by_lind = sysd["LIST_NMODE%s_BY_INDEX_L2" % tag]
+        self._noise = None
+
+    def alpha(self, lops):
+        """hops_noise.py:167-192 (complex128; the caller casts)."""
+        a = np.zeros([len(lops), len(self.t_axis)], dtype=np.complex128)
+        for r, l in enumerate(sorted(set(lops))):
+            for i in self.nmode_by_lind[l]:
+                a[r, :] += self.corr(self.t_axis, *self.corr_param[i])
+        return a
+
+    def prepare(self):
+        p = self.p
+        nt = len(self.t_axis)
+        lops = list(range(self.n_l2))
+        if p["MODEL"] == "ZERO":
+            self._noise = np.zeros((self.n_l2, nt), dtype=np.complex64)
+        elif p["MODEL"] == "FFT_FILTER":
+            seed = p["SEED"]
+            if isinstance(seed, np.ndarray):
+                z_unc = seed
+            else:
+                z_unc = uncorrelated_noise(seed, lops, nt, p["RAND_MODEL"])
+            self.z_unc = z_unc
+            a = np.complex64(self.alpha(lops))
+            self._noise = np.complex64(correlated_noise(a, z_unc))
+        elif p["MODEL"] == "PRE_CALCULATED":
+            self._noise = np.asarray(p["SEED"])
+        else:
+            raise ValueError(p["MODEL"])
+
+    @property
+    def noise(self):
+        if self._noise is None:
+            self.prepare()
+        return self._noise
+
+    def get_noise(self, t_list, lops):
+        """hops_noise.py:339-451 (grid lookup, no interpolation)."""
+        if self.p["MODEL"] == "ZERO":
+            return np.zeros([len(lops), len(t_list)], dtype=np.complex64)
+        Z = self.noise
+        it = []
+        for t in t_list:
+            hit = np.abs(self.t_axis - t) < PRECISION
+            if np.sum(hit) != 1:
+                raise ValueError("off-grid noise time %r" % t)
+            it.append(int(np.where(hit)[0][0]))
+        out = np.zeros((len(lops), len(t_list)), dtype=np.complex64)
+        for i, l in enumerate(lops):
+            for j, t in enumerate(it):
+                out[i, j] = Z[l, t]
+        if self.p["FLAG_REAL"]:
+            return np.real(out)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Trajectory driver, non-adaptive (trajectory/hops_trajectory.py:253-538,616-634)
+# ---------------------------------------------------------------------------
+class Trajectory:
+    """Plain (non-adaptive) HOPS trajectory restated from HopsTrajectory."""
+
+    def __init__(self, system_param, eom_param=None, noise_param=None, noise2_param=None,
+                 hierarchy_param=None):
+        eom_param = dict(eom_param or {})
+        hierarchy_param = dict(hierarchy_param or {})
+        self.eom_type = eom_param.get("EQUATION_OF_MOTION", "NORMALIZED NONLINEAR")
+        self.maxhier = min(int(hierarchy_param.get("MAXHIER", 3)), 255)
+        self.sysd = derive_system(system_param)
+        self.noise_param = dict(noise_param)
+        if noise2_param is None:
+            noise2_param = {"TLEN": noise_param["TLEN"], "TAU": noise_param["TAU"],
+                            "MODEL": "ZERO", "SEED": None}
+        self.noise1 = Noise(noise_param, self.sysd, 1)
+        self.noise2 = Noise(noise2_param, self.sysd, 2)
+        self.t = 0.0
+        self.psi_traj = []
+        self.t_axis = []
+        self.z_mem_traj = []
+
+    def initialize(self, psi_0):
+        sysd = self.sysd
+        aux = triangular_hierarchy(sysd["N_HMODES"], self.maxhier)
+        self.basis = Basis(sysd, np.arange(sysd["NSTATES"]), aux)
+        self.dsystem_dt = make_derivative(self.basis, self.eom_type)
+        b = self.basis
+        phi = np.zeros(b.n_hier * b.n_state, dtype=np.complex128)
+        phi[:b.n_state] = np.asarray(psi_0, dtype=np.complex128)[b.state_list]
+        self.phi = phi
+        self.z_mem = np.zeros(len(sysd["L_NOISE1"]), dtype=np.complex128)
+        self._store(0.0)
+        self.t = 0.0
+
+    def _store(self, t):
+        self.psi_traj.append(self.phi[:self.basis.n_state].copy())
+        self.t_axis.append(t)
+        self.z_mem_traj.append(self.z_mem.copy())
+
+    def noise_block(self, t, tau):
+        lops = list(self.basis.absL)
+        z1 = self.noise1.get_noise([t, t + tau * 0.5, t + tau], lops)
+        z2 = self.noise2.get_noise([t, t + tau * 0.5, t + tau], lops)
+        return z1, z2
+
+    def propagate(self, t_advance, tau):
+        t0 = self.t
+        t_axis = t0 + np.arange(1, 1 + int(np.ceil(t_advance / tau))) * tau
+        if (t0 + t_advance + tau) > self.noise1.p["TLEN"]:
+            raise ValueError("trajectory longer than TLEN")
+        for t in t_axis:
+            z1, z2 = self.noise_block(self.t, tau)
+            phi, z_mem = rk4_step(self.dsystem_dt, self.phi, self.z_mem, z1, z2, tau)
+            if self.eom_type == "NORMALIZED NONLINEAR":
+                phi = phi / np.sqrt(np.sum(np.abs(phi[:self.basis.n_state]) ** 2))
+            self.phi = phi
+            # the stored z_mem lags one step (hops_trajectory.py:526-532)
+            self.psi_traj.append(self.phi[:self.basis.n_state].copy())
+            self.t_axis.append(float(t))
+            self.z_mem_traj.append(self.z_mem.copy())
+            self.z_mem = z_mem
+            self.t = float(t)
+
+    @property
+    def psi(self):
+        return self.phi[:self.basis.n_state]

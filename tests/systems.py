@@ -1,0 +1,83 @@
+"""Parameter dictionaries of the test systems (shared by CPU and GPU tests)."""
+import numpy as np
+import scipy.sparse as sp
+
+from oracle import hops_oracle as O
+
+
+def dimer_of_dimers(altered=False):
+    """The system of the reference's tests/test_dimer_of_dimers.py:19-86:
+    4 sites, 2 modes per site (one L given as sparse, one as dense)."""
+    g0, w0 = O.bcf_convert_dl_to_exp(20.0, 50.0, 140.0)
+    scale = np.sqrt(2) if altered else 1.0
+    gdiv = 2.0 if altered else 1.0
+    lop = np.zeros([4, 4, 4])
+    gw, ll = [], []
+    for i in range(4):
+        lop[i, i, i] = scale
+        gw.append([g0 / gdiv, w0])
+        ll.append(sp.coo_matrix(lop[i]))
+        gw.append([-1j * np.imag(g0 / gdiv), 500.0])
+        ll.append(lop[i])
+    hs = np.zeros([4, 4])
+    hs[0, 1] = hs[1, 0] = 40
+    hs[1, 2] = hs[2, 1] = 10
+    hs[2, 3] = hs[3, 2] = 40
+    return dict(HAMILTONIAN=np.array(hs, dtype=np.complex128), GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll,
+                ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+
+
+def chain(nsite, V=50.0, sigma=100.0, seed_h=0, lam=35.0, gam=50.0, temp=295.0, k_mats=1,
+          ltc=None):
+    """Nearest-neighbour chain with E ~ N(0, sigma) and DL modes per site."""
+    rng = np.random.default_rng(seed_h)
+    H = np.diag(rng.normal(0.0, sigma, nsite)).astype(np.complex128)
+    for i in range(nsite - 1):
+        H[i, i + 1] = H[i + 1, i] = V
+    modes = O.bcf_convert_dl_to_exp(lam, gam, temp, k_matsubara=k_mats)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        for j in range(0, len(modes), 2):
+            gw.append([modes[j], modes[j + 1]])
+            ll.append(L)
+    p = dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+    if ltc is not None:
+        p["L_LT_CORR"] = [sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite)) for i in range(nsite)]
+        p["PARAM_LT_CORR"] = [ltc] * nsite
+    return p
+
+
+def three_site(general=False):
+    """The 3-site, 6-mode systems of tests/test_hops_eom.py:231-316 (diagonal
+    L-operators, or generalised ones with complex off-diagonal entries)."""
+    lop = np.zeros([3, 3, 3], dtype=np.complex128)
+    if general:
+        lop[0, 0, 1] = 1j
+        lop[0, 1, 0] = -1j
+        lop[1, 1, 2] = 1
+        lop[1, 2, 1] = 1
+        lop[2, 0, 0] = 0.5
+        lop[2, 1, 1] = 1
+        lop[2, 2, 2] = -1
+    else:
+        lop[0, 0, 0] = 1.0
+        lop[0, 1, 1] = -1.0
+        lop[1, 1, 1] = 1.0
+        lop[1, 2, 2] = -1.0
+        lop[2, 2, 2] = 1.0
+        lop[2, 0, 0] = -1.0
+    H = np.array([[1.0, 14.0, 8.0], [14.0, 0, 10.0], [8.0, 10.0, -1.0]])
+    by_mode = [0, 0, 1, 1, 2, 2]
+    gw = [[12.0, 18.0], [7.0, 3.0], [11.0, 9.0], [6.0, 4.0], [10.0, 10.0], [5.0, 5.0]]
+    ll = [lop[i] for i in by_mode]
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+
+
+def dimer_c1():
+    """2-site LINEAR dimer: E = 0, V = 50, one DL mode per site (50, 50, 295 K)."""
+    g, w = O.bcf_convert_dl_to_exp(50.0, 50.0, 295.0)
+    H = np.array([[0.0, 50.0], [50.0, 0.0]], dtype=np.complex128)
+    ll = [sp.coo_matrix(([1.0], ([i], [i])), shape=(2, 2)) for i in range(2)]
+    gw = [[g, w], [g, w]]
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
