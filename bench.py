@@ -1,0 +1,228 @@
+"""Benchmark of the HOPS propagation hot path on one MI355X.
+
+Workload (the batched non-adaptive configuration of the headline metric):
+10-site chain, V = 50 cm^-1 nearest neighbour, E_n ~ N(0, 100 cm^-1) (seed 0),
+2 Drude-Lorentz modes per site (lambda 35, gamma 50 cm^-1, 295 K, high-T mode +
+1 Matsubara mode), LINEAR EOM, depth 4 (10626 auxiliaries x 10 states),
+256 trajectories (noise seeds 0..255, FFT_FILTER, TAU 1 fs, TLEN 2048 fs),
+dt = 2 fs.  A step is one RK4 step of all 256 trajectories; psi of every
+trajectory is recorded on the device after every step.
+
+Prints one JSON line.  value = trajectories x RK4 steps / s over the timed
+region (inputs, operator and noise already in HBM).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "traj·RK4-steps/s at ≤1e-4 pop. error vs FP64; HBM GB/s; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_system():
+    from mesohops_amd import bcf_convert_dl_to_exp, bcf_exp
+    import scipy.sparse as sp
+    nsite = 10
+    rng = np.random.default_rng(0)
+    H = np.diag(rng.normal(0.0, 100.0, nsite)).astype(np.complex128)
+    for i in range(nsite - 1):
+        H[i, i + 1] = H[i + 1, i] = 50.0
+    modes = bcf_convert_dl_to_exp(35.0, 50.0, 295.0, k_matsubara=1)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        for j in range(0, len(modes), 2):
+            gw.append([modes[j], modes[j + 1]])
+            ll.append(L)
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=bcf_exp, PARAM_NOISE1=gw)
+
+
+NOISE = {"MODEL": "FFT_FILTER", "TLEN": 2048.0, "TAU": 1.0}
+EOM = {"EQUATION_OF_MOTION": "LINEAR"}
+HIER = {"MAXHIER": 4}
+DT = 2.0
+
+
+def psi0():
+    p = np.zeros(10, dtype=np.complex128)
+    p[0] = 1.0
+    return p
+
+
+def _cpu_worker(args):
+    """One trajectory of the workload on the FP64 oracle (1 thread)."""
+    seed, steps = args
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle import hops_oracle as O
+    sysp = c2_system()
+    sysp["ALPHA_NOISE1"] = O.bcf_exp
+    tr = O.Trajectory(sysp, EOM, dict(NOISE, SEED=seed), hierarchy_param=HIER)
+    tr.initialize(psi0())
+    _ = tr.noise1.noise
+    t0 = time.perf_counter()
+    tr.propagate(steps * DT, DT)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(steps_per_proc, procs):
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        per = pool.map(_cpu_worker, [(s, steps_per_proc) for s in range(procs)])
+    wall = time.perf_counter() - t0
+    # throughput over the propagation parts, all processes in parallel
+    value = procs * steps_per_proc / max(per)
+    return {"value": value, "unit": "traj·RK4-steps/s", "cores": procs, "kind": "port",
+            "sample": "%d trajectories x %d RK4 steps of the workload on the FP64 NumPy/SciPy oracle, one "
+                      "process per core (OMP_NUM_THREADS=1); slowest process %.1f s, pool wall %.1f s"
+                      % (procs, steps_per_proc, max(per), wall)}
+
+
+def algorithmic_bytes_per_traj_step(tables, B):
+    """96 B per hierarchy element (complex128 state read + write, 4 stages x
+    complex64 stage input read + derivative write), noise reads, and the
+    shared operator amortised over the batch."""
+    n_phi = tables.n_state * tables.n_hier
+    s_op = tables.ell_w * n_phi * 12 + tables.n_hier * 16
+    return 96 * n_phi + 3 * tables.n_act * 8 * 2 + s_op / B
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--ref-traj", type=int, default=8, help="trajectories re-run in FP64 for the population error")
+    ap.add_argument("--single-steps", type=int, default=200, help="steps of the plain B=1 FP64 GPU path")
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (trajectories would shard "
+              "one batch per GPU, see DESIGN.md); got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if (args.steps + 1) * DT > NOISE["TLEN"] or (args.warmup + 1) * DT > NOISE["TLEN"]:
+        sys.exit("steps x dt exceeds the noise length %.0f fs" % NOISE["TLEN"])
+
+    import torch
+    from mesohops_amd import HopsBatch, _lib
+    from mesohops_amd.engine import _stream_ptr
+
+    dev = torch.device("cuda:0")
+    sysp = c2_system()
+    B = args.batch
+    t_init = time.perf_counter()
+    batch = HopsBatch(sysp, EOM, dict(NOISE, SEED=0), None, HIER, seeds=list(range(B)),
+                      precision_mode=args.precision, device="cuda:0", store_psi=True)
+    batch.initialize(psi0())
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t_init
+    st = batch.state
+    tb = batch.tables
+    ns = int(round(DT * 0.5 / NOISE["TAU"]))
+
+    def run(steps):
+        if st.psi_store is None or st.store_slots < steps:
+            st.store_slots = steps
+            st.psi_store = torch.zeros(B * steps * tb.n_state, dtype=torch.complex128, device=dev)
+            st.bt.store_stride = steps
+        st.bt.psi_store_d = st.psi_store.data_ptr()
+        st.propagate(0, steps, DT, ns, 0)
+
+    # warmup, then rewind to t = 0 with the psi store sized for the timed region
+    run(args.warmup)
+    torch.cuda.synchronize()
+    batch.rewind(psi0())
+    st.psi_store = None
+    st.psi_store = torch.zeros(B * args.steps * tb.n_state, dtype=torch.complex128, device=dev)
+    st.store_slots = args.steps
+    st.bt.store_stride = args.steps
+    torch.cuda.synchronize()
+
+    _lib.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms, n = _lib.profile_read()
+    _lib.profile_enable(False)
+    value = B * args.steps / elapsed
+    psi_mixed = st.psi_store_host(args.steps)
+
+    # FP64 reference run on identical noise for the population error
+    nref = min(args.ref_traj, B)
+    ref = HopsBatch(sysp, EOM, dict(NOISE, SEED=0), None, HIER, seeds=list(range(nref)),
+                    precision_mode="fp64", device="cuda:0", store_psi=True)
+    ref.initialize(psi0())
+    ref.propagate(args.steps * DT, DT)
+    pref = np.array([np.array(ref.psi_traj[b][1:]) for b in range(nref)])
+    pop_err = float(np.max(np.abs(np.abs(psi_mixed[:nref]) ** 2 - np.abs(pref) ** 2)))
+    del ref
+
+    # plain per-trajectory FP64 GPU path (B = 1, no batching)
+    single = HopsBatch(sysp, EOM, dict(NOISE, SEED=0), None, HIER, seeds=[0], precision_mode="fp64",
+                       device="cuda:0", store_psi=True)
+    single.initialize(psi0())
+    ss = min(args.single_steps, args.steps)
+    single.propagate(2 * DT, DT)
+    single.rewind(psi0())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    single.propagate(ss * DT, DT)
+    torch.cuda.synchronize()
+    single_rate = ss / (time.perf_counter() - t1)
+    del single
+
+    stage_ms = ms[1] / max(n[1], 1)
+    bytes_step = algorithmic_bytes_per_traj_step(tb, B)
+    bytes_launch = bytes_step * B / 4.0
+    achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "traj·RK4-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" if args.precision == "mixed" else "c128",
+        "data": "synthetic",
+        "config": {"workload": "batched non-adaptive LINEAR HOPS: 10-site chain, 2 DL modes/site, depth 4 "
+                               "(10626 aux x 10 states), %d trajectories, dt 2 fs; complex64 stages with a "
+                               "complex128 state carry" % B if args.precision == "mixed" else
+                               "same workload in complex128",
+                   "trajectories": B, "n_state": tb.n_state, "n_hier": tb.n_hier, "dt_fs": DT,
+                   "precision": args.precision},
+        "pop_err_vs_fp64": pop_err,
+        "pop_err_trajectories": nref,
+        "gpu_fp64_single_traj_steps_per_s": single_rate,
+        "speedup_vs_gpu_fp64_single": value / single_rate,
+        "init_s": t_init,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_stage<float>", "kernel_avg_ms": stage_ms,
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "prologue_avg_ms": ms[0] / max(n[0], 1)},
+    }
+    if not args.no_cpu:
+        procs = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, procs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

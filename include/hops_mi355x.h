@@ -118,6 +118,14 @@ const char *hops_last_error(void);
 /* ABI version of this library (bumped on any change of the structs). */
 int hops_version(void);
 
+/* Per-launch timing of hops_propagate (no counterpart in the reference; used
+   by the benchmark).  enable(1) clears the counters and records a HIP event
+   on the launch stream after every kernel; read() returns the summed
+   milliseconds and launch counts for [0] the stage prologue, [1] the fused
+   derivative/RK4 stage kernel, [2] other kernels. */
+int hops_profile_enable(int on);
+int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
+
 /* Fill op->ell_col_d/ell_val_d/k0_d from the connection tables.
    precision selects the ELL value type (complex128 or complex64). */
 int hops_build_super(const hops_op_t *op, int precision, void *stream);
@@ -131,10 +139,13 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *batch, int it0,
 
 /* Circulant-embedding spectrum s_w[l][j] = max(0, Re FFT(embed(tilde c_l)))
    for n_lop correlation functions of n_t points (FFT length 2(n_t-1)).
-   c_d: complex64 [n_lop][n_t]; s_w_d: double [n_lop][2(n_t-1)];
+   ang_d, mag_d: float32 [n_lop][n_t] phase and modulus of the complex64
+   correlation function as NumPy's np.angle/np.abs round them (the embedding
+   is ill-conditioned in its near-zero bins, so these roundings are inputs,
+   noise/hops_noise.py:653-656); s_w_d: double [n_lop][2(n_t-1)];
    nu_d: double [n_lop] rotation frequency of each row. */
-int hops_noise_embedding(const void *c_d, int n_lop, int n_t, double *s_w_d,
-                         double *nu_d, void *stream);
+int hops_noise_embedding(const float *ang_d, const float *mag_d, int n_lop, int n_t,
+                         double *s_w_d, double *nu_d, void *stream);
 
 /* Correlated noise for R rows: out[r] = c64( unrotate( IFFT( FFT(z[r]) *
    sqrt(s_w[lop(r)]/2) )[:n_t] ) ).  z_d: complex64 [R][2(n_t-1)];

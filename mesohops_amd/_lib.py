@@ -68,8 +68,10 @@ def lib():
     L.hops_build_super.argtypes = [ctypes.POINTER(HopsOp), ctypes.c_int, _vp]
     L.hops_propagate.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, _vp]
-    L.hops_noise_embedding.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    L.hops_noise_embedding.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+    L.hops_profile_enable.argtypes = [ctypes.c_int]
+    L.hops_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
     if L.hops_version() != ABI_VERSION:
         raise EngineError("libhops_mi355x.so ABI %d != %d; rebuild" % (L.hops_version(), ABI_VERSION))
     _lib = L
@@ -81,5 +83,17 @@ def check(rc):
         raise EngineError(lib().hops_last_error().decode())
 
 
-EXPORTED = ["hops_last_error", "hops_version", "hops_build_super", "hops_propagate",
-            "hops_noise_embedding", "hops_noise_filter"]
+EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read",
+            "hops_build_super", "hops_propagate", "hops_noise_embedding", "hops_noise_filter"]
+
+
+def profile_enable(on=True):
+    check(lib().hops_profile_enable(1 if on else 0))
+
+
+def profile_read():
+    """(ms, launches) for [prologue, stage, other] since profile_enable."""
+    ms = (ctypes.c_double * 3)()
+    n = (ctypes.c_longlong * 3)()
+    check(lib().hops_profile_read(ms, n))
+    return list(ms), list(n)

@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/hops_mi355x.h"
 
@@ -592,6 +593,43 @@ __global__ __launch_bounds__(256) void k_final_scale(const hops_op_t op, const h
   phi[r] = p;
 }
 
+// Optional per-launch timing: an event is recorded on the launch stream after
+// every kernel; the gap between consecutive events is that kernel's time.
+struct Profile {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;  // 0 prologue, 1 stage, 2 other
+  double ms[3] = {0, 0, 0};
+  long long n[3] = {0, 0, 0};
+};
+Profile g_prof;
+
+int prof_mark(hipStream_t st, int kind, size_t *used) {
+  if (!g_prof.on) return 0;
+  if (*used >= g_prof.ev.size()) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    g_prof.ev.push_back(e);
+    g_prof.kind.push_back(0);
+  }
+  g_prof.kind[*used] = kind;
+  HIP_TRY(hipEventRecord(g_prof.ev[*used], st));
+  ++*used;
+  return 0;
+}
+
+int prof_collect(size_t used) {
+  if (!g_prof.on || used < 2) return 0;
+  HIP_TRY(hipEventSynchronize(g_prof.ev[used - 1]));
+  for (size_t i = 1; i < used; ++i) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g_prof.ev[i - 1], g_prof.ev[i]));
+    g_prof.ms[g_prof.kind[i]] += ms;
+    g_prof.n[g_prof.kind[i]] += 1;
+  }
+  return 0;
+}
+
 int check_op(const hops_op_t *op) {
   if (!op) return fail("op is NULL");
   if (op->n_state <= 0 || op->n_hier <= 0) return fail("empty basis (n_state=%d n_hier=%d)", op->n_state, op->n_hier);
@@ -622,6 +660,8 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     HIP_TRY(hipGetLastError());
   }
   const int toff[4] = {0, ns, ns, 2 * ns};
+  size_t used = 0;
+  if (int rc = prof_mark(st, 2, &used)) return rc;
   for (int step = 0; step < n_steps; ++step) {
     const int itb = it0 + step * 2 * ns;
     for (int i = 0; i < 4; ++i) {
@@ -630,10 +670,13 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       const int slot = (i == 0 && step > 0 && bt->psi_store_d) ? store0 + step - 1 : -1;
       k_prologue<R><<<bt->B, 256, shm, st>>>(*op, *bt, i, itb + toff[i], yin, slot);
       HIP_TRY(hipGetLastError());
+      if (int rc = prof_mark(st, 0, &used)) return rc;
       k_stage<R><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, i, yin, yout);
       HIP_TRY(hipGetLastError());
+      if (int rc = prof_mark(st, 1, &used)) return rc;
     }
   }
+  if (int rc = prof_collect(used)) return rc;
   const int slot = (n_steps > 0 && bt->psi_store_d) ? store0 + n_steps - 1 : -1;
   k_final_sigma<<<bt->B, 256, 0, st>>>(*op, *bt, slot);
   HIP_TRY(hipGetLastError());
@@ -657,6 +700,24 @@ int hops_internal_set_error(const char *msg) {
 }
 
 int hops_version(void) { return 1; }
+
+int hops_profile_enable(int on) {
+  g_prof.on = on != 0;
+  for (int k = 0; k < 3; ++k) {
+    g_prof.ms[k] = 0.0;
+    g_prof.n[k] = 0;
+  }
+  return 0;
+}
+
+int hops_profile_read(double *ms_out, long long *n_out) {
+  if (!ms_out || !n_out) return fail("hops_profile_read: NULL pointer");
+  for (int k = 0; k < 3; ++k) {
+    ms_out[k] = g_prof.ms[k];
+    n_out[k] = g_prof.n[k];
+  }
+  return 0;
+}
 
 int hops_build_super(const hops_op_t *op, int precision, void *stream) {
   if (int rc = check_op(op)) return rc;

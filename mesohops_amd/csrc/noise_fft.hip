@@ -128,25 +128,21 @@ __global__ __launch_bounds__(kThreads) void k_bluestein_filter(double2 *Bhat, in
 }
 
 // Embedding spectrum of each correlation function row.
-__global__ __launch_bounds__(kThreads) void k_embedding(const float2 *__restrict__ c, int n_t, int N, int M,
-                                                        int logm, const double2 *Bhat, double *s_w,
-                                                        double *nu) {
+__global__ __launch_bounds__(kThreads) void k_embedding(const float *__restrict__ ang, const float *__restrict__ mag,
+                                                        int n_t, int N, int M, int logm, const double2 *Bhat,
+                                                        double *s_w, double *nu) {
   extern __shared__ double2 lds[];
   const int row = blockIdx.x;
-  const float2 *cr = c + (long)row * n_t;
-  // rotation so that the last point is real; angles in float like the
-  // complex64 input of the reference (np.angle of complex64 is float32)
-  const float2 cl = cr[n_t - 1];
-  const double ang_last = (double)(float)atan2((double)cl.y, (double)cl.x);
-  const double nur = ang_last / (((double)n_t - 1.0) * 2.0 * M_PI);
+  const float *ar = ang + (long)row * n_t;
+  const float *mr = mag + (long)row * n_t;
+  // rotation so that the last point is real
+  const double nur = (double)ar[n_t - 1] / (((double)n_t - 1.0) * 2.0 * M_PI);
   for (int j = threadIdx.x; j < n_t; j += blockDim.x) {
-    const float2 v = cr[j];
-    const double ang = (double)(float)atan2((double)v.y, (double)v.x);
-    const double mag = (double)(float)sqrt((double)v.x * v.x + (double)v.y * v.y);
+    const double a = (double)ar[j], m = (double)mr[j];
     double s1, c1, s2, c2;
-    sincos(ang, &s1, &c1);
+    sincos(a, &s1, &c1);
     sincospi(-2.0 * nur * (double)j, &s2, &c2);
-    const double2 t = cmul2(make_double2(mag * c1, mag * s1), make_double2(c2, s2));
+    const double2 t = cmul2(make_double2(m * c1, m * s1), make_double2(c2, s2));
     lds[j] = t;
     if (j >= 1 && j <= n_t - 2) lds[N - j] = make_double2(t.x, -t.y);
   }
@@ -261,16 +257,16 @@ int set_lds(const void *fn, size_t bytes) {
 
 }  // namespace
 
-extern "C" int hops_noise_embedding(const void *c_d, int n_lop, int n_t, double *s_w_d, double *nu_d,
-                                    void *stream) {
+extern "C" int hops_noise_embedding(const float *ang_d, const float *mag_d, int n_lop, int n_t, double *s_w_d,
+                                    double *nu_d, void *stream) {
   if (n_lop <= 0) return 0;
-  if (!c_d || !s_w_d || !nu_d) return hops_internal_set_error("hops_noise_embedding: NULL pointer");
+  if (!ang_d || !mag_d || !s_w_d || !nu_d) return hops_internal_set_error("hops_noise_embedding: NULL pointer");
   hipStream_t st = (hipStream_t)stream;
   Plan p;
   if (int rc = make_plan(n_t, st, &p)) return rc;
   const size_t lds = sizeof(double2) * p.M;
   if (int rc = set_lds((const void *)k_embedding, lds)) return rc;
-  k_embedding<<<n_lop, kThreads, lds, st>>>((const float2 *)c_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d);
+  k_embedding<<<n_lop, kThreads, lds, st>>>(ang_d, mag_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d);
   if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_embedding launch failed");
   return 0;
 }

@@ -148,10 +148,13 @@ def device_noise(noises, rows, device, stream_ptr):
     else:
         alpha = nz.alpha()
         N = 2 * (nt - 1)
-        c_d = torch.from_numpy(np.ascontiguousarray(alpha)).to(device)
+        # phase and modulus rounded as NumPy rounds them for complex64 input
+        ang_d = torch.from_numpy(np.ascontiguousarray(np.angle(alpha))).to(device)
+        mag_d = torch.from_numpy(np.ascontiguousarray(np.abs(alpha))).to(device)
         s_w = torch.empty((nz.n_l2, N), dtype=torch.float64, device=device)
         nu = torch.empty(nz.n_l2, dtype=torch.float64, device=device)
-        check(lib().hops_noise_embedding(ctypes.c_void_p(c_d.data_ptr()), nz.n_l2, nt,
+        check(lib().hops_noise_embedding(ctypes.c_void_p(ang_d.data_ptr()), ctypes.c_void_p(mag_d.data_ptr()),
+                                         nz.n_l2, nt,
                                          ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
                                          stream_ptr))
         raw = np.stack([n.raw()[rows] for n in noises]).reshape(B * len(rows), N)

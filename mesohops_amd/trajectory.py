@@ -215,6 +215,18 @@ class HopsBatch:
             self.t_axis.extend([float(t) for t in t_axis])
         self.t = float(t_axis[-1]) if n_steps else self.t
 
+    def rewind(self, psi_0):
+        """Reset every trajectory to t = 0 with the initial state psi_0 and
+        zero noise memory (the device operator and noise are kept)."""
+        tb = self.tables
+        phi = np.zeros(tb.n_hier * tb.n_state, dtype=np.complex128)
+        phi[:tb.n_state] = np.asarray(psi_0, dtype=np.complex128)[tb.state_list]
+        self.state.set_phi(phi)
+        self.state.zmem.zero_()
+        self.psi_traj = [[phi[:tb.n_state].copy()] for _ in range(self.B)]
+        self.t_axis = [0.0]
+        self.t = 0.0
+
     def synchronize(self):
         torch.cuda.synchronize(self.device)
 

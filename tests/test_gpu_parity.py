@@ -191,7 +191,9 @@ def test_batch_equals_alone():
 @pytest.mark.parametrize("tlen,tau", [(1024.0, 2.0), (1000.0, 1.0), (60.0, 0.5)])
 def test_device_noise_filter(tlen, tau):
     """Device circulant-embedding filter vs the oracle (power-of-two and
-    Bluestein lengths); both start from the same NumPy raw draws."""
+    Bluestein lengths); both start from the same NumPy raw draws.  The
+    reference transforms the complex64 raw noise in single precision, which
+    bounds the agreement at ~1e-7 of the noise scale; the bar is 1e-6."""
     _need_gpu()
     from mesohops_amd.basis import SystemInfo
     from mesohops_amd.engine import _stream_ptr
@@ -207,4 +209,4 @@ def test_device_noise_filter(tlen, tau):
         scale = np.max(np.abs(want))
         err = np.max(np.abs(got - want)) / scale
         print("noise", tlen, tau, rand, "max rel err", err)
-        assert err < 2e-6
+        assert err < 1e-6
