@@ -190,6 +190,12 @@ class HopsBatch:
         n_steps = int(np.ceil(t_advance / tau))
         it0 = nz1.time_index(self.t)
         ns = int(round(tau * 0.5 / nz1.param["TAU"]))
+        # the signed check above lets a half step between grid points through;
+        # the reference then fails on the off-grid noise lookup
+        # (noise/hops_noise.py:439-447), and so does this engine
+        for nz in (nz1, nz2):
+            if abs(tau * 0.5 - ns * nz.param["TAU"]) >= precision or nz.param["TAU"] != nz1.param["TAU"]:
+                raise UnsupportedRequest("Off axis t-samples when INTERPOLATE = False", "NoiseModel.get_noise()")
         return n_steps, it0, ns
 
     def propagate(self, t_advance, tau, store=None):

@@ -1,0 +1,223 @@
+"""Pins of the FP64 oracle: the reference's golden fixtures, an independent
+brute-force restatement of the EOM, closed forms and invariants.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import systems
+from oracle import hops_oracle as O
+
+FX = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_fixtures.npz"))
+
+
+def _dense(prefix, shape):
+    a = np.zeros(shape, complex)
+    idx = FX[prefix + "_idx"]
+    a[idx[0], idx[1]] = FX[prefix + "_val"]
+    return a
+
+
+@pytest.fixture(scope="module")
+def dod():
+    tr = O.Trajectory(systems.dimer_of_dimers(), {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"},
+                      {"SEED": 0, "MODEL": "FFT_FILTER", "TLEN": 25000.0, "TAU": 1.0, "RAND_MODEL": "BOX_MULLER"},
+                      hierarchy_param={"MAXHIER": 2})
+    psi0 = np.zeros(4, complex)
+    psi0[2] = 1.0
+    tr.initialize(psi0)
+    return tr
+
+
+def test_super_operators_equal_golden(dod):
+    """tests/test_dimer_of_dimers.py:163-187 (exact equality)."""
+    b = dod.basis
+    assert np.array_equal(b.K2_k.toarray(), _dense("K2_k", tuple(FX["K2_k_shape"])))
+    assert np.array_equal(b.K2_kp1.toarray(), _dense("K2_kp1", tuple(FX["K2_kp1_shape"])))
+    assert np.array_equal(b.K2_km1.toarray(), _dense("K2_km1", tuple(FX["K2_km1_shape"])))
+    shp = tuple(FX["Z2_shape"])
+    for i in range(shp[0]):
+        assert np.array_equal(b.Z2_kp1[i].toarray(), _dense("Z2_%d" % i, shp[1:]))
+
+
+def test_alpha_equals_golden(dod):
+    """tests/test_dimer_of_dimers.py:213-220."""
+    a = dod.noise1.alpha(list(range(4)))
+    np.testing.assert_allclose(a[:, ::50], FX["alpha_stride50"])
+
+
+def test_eta_equals_golden(dod):
+    """tests/test_dimer_of_dimers.py:223-239 (rtol 1e-10)."""
+    a = dod.noise1.alpha(list(range(4)))
+    z = O.uncorrelated_noise(0, list(range(4)), len(dod.noise1.t_axis), "BOX_MULLER")
+    eta = O.correlated_noise(a, z)[0]
+    np.testing.assert_allclose(eta[::5], FX["eta_stride5"], rtol=1e-10)
+
+
+def test_dimer_of_dimers_trajectory(dod):
+    """tests/test_dimer_of_dimers.py:242-260 (index 25, rtol 1e-5) and the
+    whole recorded trajectory."""
+    import copy
+    tr = copy.deepcopy(dod)
+    tr.propagate(200.0, 4.0)
+    traj = FX["traj_dimer_of_dimers"]
+    assert np.allclose(tr.psi_traj[25], traj[25], rtol=1e-5)
+    assert np.max(np.abs(np.array(tr.psi_traj) - traj)) < 1e-5
+
+
+def test_raw_noise_equals_golden():
+    """Seed-0 SUM_GAUSSIAN raw noise, TLEN 10, TAU 1 (tests/test_noise_fft.py:86-120)."""
+    z = O.uncorrelated_noise(0, [0, 1], 11, "SUM_GAUSSIAN")
+    np.testing.assert_allclose(z, FX["pre_calculated_uncorrelated_noise_test"])
+
+
+def test_raw_index_prefix_invariant():
+    """A prefix of the raw-slot assignment does not depend on the axis length
+    (tests/test_noise_fft.py:376-400); the slots of one axis are a permutation
+    of all 4(n-1) draws."""
+    big_a, big_b = O.raw_index(1000)
+    for n in range(2, 1000):
+        a, b = O.raw_index(n)
+        assert list(big_a[:n]) == list(a[:n])
+        assert list(big_b[:n]) == list(b[:n])
+    for n in (3, 4, 17):
+        a, b = O.raw_index(n)
+        assert len(a) == len(b) == 2 * (n - 1)
+        assert len(set(list(a) + list(b))) == 4 * (n - 1) or n < 3
+
+
+def test_known_bath_parameters():
+    """G and W derived for the system of tests/test_hops_system.py:25-55 (the
+    dimer of dimers) equal the golden known_param_g/w (:88-100)."""
+    sysd = O.derive_system(systems.dimer_of_dimers())
+    assert np.allclose(sysd["G"], FX["known_param_g"])
+    assert np.allclose(sysd["W"], FX["known_param_w"])
+    lam, gam, T = 20.0, 50.0, 140.0
+    g0, w0 = O.bcf_convert_dl_to_exp(lam, gam, T)
+    assert np.isclose(g0, 2 * lam * O.KB * T - 1j * lam * gam) and w0 == gam
+
+
+# ---------------------------------------------------------------------------
+# Brute force: dense per-auxiliary loops over k +- e_m (an independent
+# formulation of the EOM, after tests/test_hops_eom.py:14-229)
+# ---------------------------------------------------------------------------
+def brute_force(phi, basis, eom, z1, z_mem):
+    n, nh = basis.n_state, basis.n_hier
+    P = phi.reshape(nh, n)
+    D = np.zeros_like(P)
+    modes = basis.modes
+    Lm = [basis.L_csr[basis.relL_by_mode[i]].toarray() for i in range(len(modes))]
+    g, w = basis.g, basis.w
+    kvec = np.zeros((nh, basis.sysd["N_HMODES"]), int)
+    for i, a in enumerate(basis.aux_list):
+        for (m, v) in a:
+            kvec[i, m] = v
+    # noise per unique L in relative order, conj; plus memory for nonlinear
+    zc = np.conj(z1).astype(complex)
+    if eom != "LINEAR":
+        for i, m in enumerate(modes):
+            zc[basis.relL_by_mode[i]] += z_mem[m]
+    Lrel = [basis.L_csr[j].toarray() for j in range(len(basis.L_csr))]
+    noise_mat = sum(zc[j] * Lrel[j] for j in range(len(Lrel)))
+    psi0 = P[0]
+    lexp = [np.conj(psi0) @ L @ psi0 / (np.conj(psi0) @ psi0) for L in Lrel]
+    I = np.eye(n)
+    norm_corr = 0.0
+    if eom == "NORMALIZED NONLINEAR":
+        norm_corr = np.real(np.sum(np.array(lexp) * zc))
+        for i, m in enumerate(modes):
+            e = np.zeros(basis.sysd["N_HMODES"], int)
+            e[m] = 1
+            hit = np.where((kvec == e).all(1))[0]
+            if len(hit):
+                pm = P[hit[0]]
+                r = g[i] / w[i]
+                norm_corr -= np.real(r * np.conj(psi0) @ Lm[i] @ pm)
+                norm_corr += np.real(lexp[basis.relL_by_mode[i]] * r * np.conj(psi0) @ pm)
+    for k in range(nh):
+        D[k] += (-1j * basis.H - (kvec[k] @ basis.sysd["W"]) * I - norm_corr * I + noise_mat) @ P[k]
+        for i, m in enumerate(modes):
+            e = np.zeros(basis.sysd["N_HMODES"], int)
+            e[m] = 1
+            dn = np.where((kvec == kvec[k] - e).all(1))[0]
+            if len(dn):
+                D[k] += kvec[k][m] * w[i] * Lm[i] @ P[dn[0]]
+            up = np.where((kvec == kvec[k] + e).all(1))[0]
+            if len(up):
+                op = Lm[i] if eom == "LINEAR" else Lm[i] - lexp[basis.relL_by_mode[i]] * I
+                D[k] -= (g[i] / w[i]) * op @ P[up[0]]
+    return D.reshape(-1)
+
+
+@pytest.mark.parametrize("general", [False, True])
+@pytest.mark.parametrize("eom", ["LINEAR", "NONLINEAR", "NORMALIZED NONLINEAR"])
+def test_derivative_equals_brute_force(general, eom):
+    sysp = systems.three_site(general)
+    sysd = O.derive_system(sysp)
+    basis = O.Basis(sysd, np.arange(3), O.triangular_hierarchy(6, 3))
+    f = O.make_derivative(basis, eom)
+    rng = np.random.default_rng(11)
+    phi = rng.normal(size=basis.n_state * basis.n_hier) + 1j * rng.normal(size=basis.n_state * basis.n_hier)
+    z1 = rng.normal(size=len(basis.absL)) + 1j * rng.normal(size=len(basis.absL))
+    zmem = (rng.normal(size=6) + 1j * rng.normal(size=6)) * (0 if eom == "LINEAR" else 1)
+    d, _ = f(phi, zmem, z1, np.zeros(len(basis.absL)))
+    bf = brute_force(phi, basis, eom, z1, zmem)
+    assert np.max(np.abs(d - bf)) < 1e-10 * np.max(np.abs(bf))
+
+
+def test_zmem_derivative_closed_form():
+    """dz_m = <L_m> conj(g_m) - conj(w_m) z_m (eom/eom_functions.py:65-160)."""
+    sysp = systems.three_site(False)
+    sysd = O.derive_system(sysp)
+    basis = O.Basis(sysd, np.arange(3), O.triangular_hierarchy(6, 2))
+    f = O.make_derivative(basis, "NONLINEAR")
+    rng = np.random.default_rng(2)
+    phi = rng.normal(size=basis.n_state * basis.n_hier) + 0j
+    zmem = rng.normal(size=6) + 1j * rng.normal(size=6)
+    _, dz = f(phi, zmem, np.zeros(3), np.zeros(3))
+    psi = phi[:3]
+    for m in range(6):
+        L = basis.L_csr[basis.relL_by_mode[m]].toarray()
+        lexp = np.conj(psi) @ L @ psi / (np.conj(psi) @ psi)
+        assert np.isclose(dz[m], lexp * np.conj(sysd["G"][m]) - np.conj(sysd["W"][m]) * zmem[m])
+
+
+def test_rk4_exact_for_uncoupled_decay():
+    """No coupling, no noise, depth 0: psi(t) = exp(-i E t / hbar) psi(0); RK4
+    error after 50 steps of h = 1 fs is O(h^5) per step."""
+    H = np.diag([100.0, -50.0]).astype(complex)
+    L = [sp.coo_matrix(([1.0], ([i], [i])), shape=(2, 2)) for i in range(2)]
+    gw = [[10.0, 10.0], [10.0, 10.0]]
+    sysp = dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=L, L_NOISE1=L, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+    tr = O.Trajectory(sysp, {"EQUATION_OF_MOTION": "LINEAR"},
+                      {"SEED": 0, "MODEL": "ZERO", "TLEN": 100.0, "TAU": 0.5}, hierarchy_param={"MAXHIER": 0})
+    psi0 = np.array([1.0, 1.0], complex) / np.sqrt(2)
+    tr.initialize(psi0)
+    tr.propagate(50.0, 1.0)
+    exact = np.exp(-1j * np.diag(H) * 50.0 / O.HBAR) * psi0
+    assert np.max(np.abs(tr.psi - exact)) < 1e-9
+
+
+def test_noise_correlation_reconstructs_alpha():
+    """Time-averaged autocorrelation of the correlated noise, averaged over an
+    ensemble, reproduces alpha(t) on its first lags within 2 %
+    (tests/integrated_tests/test_rand_models.py:13-28,117-170)."""
+    g1, w1, g2, w2 = 10000.0 - 1000.0j, 100.0, 1000.0j, 1000.0
+    L = [np.diag([1.0, 0.0]), np.diag([1.0, 0.0]), np.diag([0.0, 1.0]), np.diag([0.0, 1.0])]
+    gw = [[g1, w1], [g2, w2], [g1, w1], [g2, w2]]
+    sysp = dict(HAMILTONIAN=np.array([[0, 10.0], [10.0, 0]]), GW_SYSBATH=gw, L_HIER=L, L_NOISE1=L,
+                ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+    sysd = O.derive_system(sysp)
+    n_traj = 400
+    t_axis = np.arange(0.0, 2500.0, 1.0)
+    ct = np.zeros(len(t_axis), complex)
+    for s in range(n_traj):
+        nz = O.Noise({"SEED": 1000 + s, "MODEL": "FFT_FILTER", "TLEN": 2500.0, "TAU": 1.0,
+                      "RAND_MODEL": "SUM_GAUSSIAN"}, sysd, 1)
+        x = nz.get_noise(t_axis, [0])[0].astype(complex)
+        r = np.correlate(x, x, mode="full")
+        ct += r[r.size // 2:]
+    ct /= n_traj * len(t_axis)
+    cf = g1 * np.exp(-t_axis * w1 / O.HBAR) + g2 * np.exp(-t_axis * w2 / O.HBAR)
+    assert np.allclose(ct[:10], cf[:10], rtol=2e-2)
