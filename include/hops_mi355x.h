@@ -126,6 +126,11 @@ int hops_version(void);
 int hops_profile_enable(int on);
 int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
 
+/* Engine options (no counterpart in the reference).  "stationary_stage":
+   1 (default) runs the operator-stationary stage kernel for batches of at
+   least 8 trajectories with ell_w <= 16; 0 forces the streaming kernel. */
+int hops_set_option(const char *name, int value);
+
 /* Fill op->ell_col_d/ell_val_d/k0_d from the connection tables.
    precision selects the ELL value type (complex128 or complex64). */
 int hops_build_super(const hops_op_t *op, int precision, void *stream);

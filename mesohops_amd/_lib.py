@@ -71,6 +71,7 @@ def lib():
     L.hops_noise_embedding.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
     L.hops_profile_enable.argtypes = [ctypes.c_int]
+    L.hops_set_option.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.hops_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
     if L.hops_version() != ABI_VERSION:
         raise EngineError("libhops_mi355x.so ABI %d != %d; rebuild" % (L.hops_version(), ABI_VERSION))
@@ -83,8 +84,12 @@ def check(rc):
         raise EngineError(lib().hops_last_error().decode())
 
 
-EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read",
+EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
             "hops_build_super", "hops_propagate", "hops_noise_embedding", "hops_noise_filter"]
+
+
+def set_option(name, value):
+    check(lib().hops_set_option(name.encode(), int(value)))
 
 
 def profile_enable(on=True):

@@ -468,39 +468,41 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
   }
 }
 
-// One RK4 stage for every row of every trajectory
-// (eom/hops_eom.py:386-426, integrator/integrator_rk.py:53-73).
-template <typename R>
-__global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_batch_t bt, int stage,
-                                               const cx<R> *__restrict__ yin,
-                                               cx<R> *__restrict__ yout) {
-  const int t = bt.tile_d[blockIdx.x];
-  if (t < 0) return;
-  const int b = t >> 16, tile = t & 0xffff;
+// Derivative of row r = (k, s) of trajectory b fused with its RK4 stage
+// update (eom/hops_eom.py:386-426, integrator/integrator_rk.py:53-73).  The
+// ELL row of the super-operator comes in ecol/ev: registers when WR > 0,
+// otherwise the column-major ELL arrays in HBM.
+template <typename R, int WR>
+__device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_t &bt, int stage,
+                                          const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout, int b,
+                                          long r, int k, int s, cx<R> k0, const int *ecol, const cx<R> *ev) {
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
-  const long r = (long)tile * bt.tile_rows + threadIdx.x;
-  if (r >= nr) return;
   const long off = (long)b * nr;
-  const int k = (int)(r / n), s = (int)(r - (long)k * n);
   const cx<R> *Y = yin + off;
   const CoefLayout L = coef_layout(op);
   const double *coef = bt.coef_d + (long)b * bt.coef_stride;
   const double sigma = coef[L.sig];
 
   const cx<R> yr = Y[r];
-  cx<R> d = cmul(ccast<R>(ld2(op.k0_d, k)), yr);
+  cx<R> d = cmul(k0, yr);
   // system block: -iH + noise / LTC terms
   const double *A = coef + (k == 0 ? L.A0 : L.A);
   const long rowk = (long)k * n;
   for (int p = op.pat_ptr_d[s]; p < op.pat_ptr_d[s + 1]; ++p)
     cfma(d, ccast<R>(ld2(A, p)), Y[rowk + op.pat_col_d[p]]);
-  // hierarchy super-operator K2_kp1 + K2_km1 (ELL, column-major)
-  const cx<R> *ev = reinterpret_cast<const cx<R> *>(op.ell_val_d);
-  for (int e = 0; e < op.ell_w; ++e) {
-    const int c = op.ell_col_d[(long)e * nr + r];
-    if (c < 0) break;
-    cfma(d, ev[(long)e * nr + r], Y[c]);
+  // hierarchy super-operator K2_kp1 + K2_km1
+  if (WR > 0) {
+#pragma unroll
+    for (int e = 0; e < WR; ++e)
+      if (ecol[e] >= 0) cfma(d, ev[e], Y[ecol[e]]);
+  } else {
+    const cx<R> *evm = reinterpret_cast<const cx<R> *>(op.ell_val_d);
+    for (int e = 0; e < op.ell_w; ++e) {
+      const int c = op.ell_col_d[(long)e * nr + r];
+      if (c < 0) break;
+      cfma(d, evm[(long)e * nr + r], Y[c]);
+    }
   }
   if (bt.eom != HOPS_EOM_LINEAR) {
     const double *zc = coef + L.zc;
@@ -546,6 +548,56 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
     if (bt.precision == HOPS_PREC_MIXED)
       reinterpret_cast<cx<float> *>(bt.y0_d)[gi] = mk<float>((float)q.x, (float)q.y);
   }
+}
+
+// Streaming form: one thread per row, workgroups from the tile list.
+template <typename R>
+__global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_batch_t bt, int stage,
+                                               const cx<R> *__restrict__ yin,
+                                               cx<R> *__restrict__ yout) {
+  const int t = bt.tile_d[blockIdx.x];
+  if (t < 0) return;
+  const int b = t >> 16, tile = t & 0xffff;
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)tile * bt.tile_rows + threadIdx.x;
+  if (r >= nr) return;
+  const int k = (int)(r / n), s = (int)(r - (long)k * n);
+  stage_row<R, 0>(op, bt, stage, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
+}
+
+// Operator-stationary form for a batch that shares one operator: workgroup i
+// owns a 512-row tile, keeps its K0 and ELL rows in registers and walks the
+// trajectories b = x + 8 (q + Q j) of XCD x = i % 8, so the operator is read
+// once per tile and stage instead of once per trajectory, and every XCD works
+// on the same few trajectories at a time (their rows stay in its L2).
+template <typename R, int WR>
+__global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const hops_batch_t bt, int stage,
+                                                      const cx<R> *__restrict__ yin,
+                                                      cx<R> *__restrict__ yout, int n_tiles, int Q) {
+  const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int tile = slot % n_tiles, q = slot / n_tiles;
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)tile * 512 + threadIdx.x;
+  if (r >= nr) return;
+  const int k = (int)(r / n), s = (int)(r - (long)k * n);
+  int ecol[WR];
+  cx<R> ev[WR];
+  const cx<R> *evm = reinterpret_cast<const cx<R> *>(op.ell_val_d);
+#pragma unroll
+  for (int e = 0; e < WR; ++e) {
+    if (e < op.ell_w) {
+      ecol[e] = op.ell_col_d[(long)e * nr + r];
+      ev[e] = evm[(long)e * nr + r];
+    } else {
+      ecol[e] = -1;
+      ev[e] = mk<R>((R)0, (R)0);
+    }
+  }
+  const cx<R> k0 = ccast<R>(ld2(op.k0_d, k));
+  for (int b = x + 8 * q; b < bt.B; b += 8 * Q)
+    stage_row<R, WR>(op, bt, stage, yin, yout, b, r, k, s, k0, ecol, ev);
 }
 
 // y0 = complex64(Phi) at the start of a mixed-precision propagate call.
@@ -603,6 +655,7 @@ struct Profile {
   long long n[3] = {0, 0, 0};
 };
 Profile g_prof;
+bool g_stationary = true;
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
   if (!g_prof.on) return 0;
@@ -660,6 +713,16 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     HIP_TRY(hipGetLastError());
   }
   const int toff[4] = {0, ns, ns, 2 * ns};
+  // operator-stationary stage kernel when every XCD has trajectories to walk
+  int sh_w = 0, sh_tiles = 0, sh_q = 1;
+  unsigned sh_grid = 0;
+  if (g_stationary && bt->B >= 8 && op->ell_w <= 16) {
+    sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
+    sh_tiles = (int)((nr + 511) / 512);
+    const int per_xcd = (bt->B + 7) / 8;
+    sh_q = (per_xcd + 31) / 32;
+    sh_grid = (unsigned)(8L * sh_q * sh_tiles);
+  }
   size_t used = 0;
   if (int rc = prof_mark(st, 2, &used)) return rc;
   for (int step = 0; step < n_steps; ++step) {
@@ -671,7 +734,14 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       k_prologue<R><<<bt->B, 256, shm, st>>>(*op, *bt, i, itb + toff[i], yin, slot);
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 0, &used)) return rc;
-      k_stage<R><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, i, yin, yout);
+      if (sh_w == 4)
+        k_stage_shared<R, 4><<<sh_grid, 512, 0, st>>>(*op, *bt, i, yin, yout, sh_tiles, sh_q);
+      else if (sh_w == 8)
+        k_stage_shared<R, 8><<<sh_grid, 512, 0, st>>>(*op, *bt, i, yin, yout, sh_tiles, sh_q);
+      else if (sh_w == 16)
+        k_stage_shared<R, 16><<<sh_grid, 512, 0, st>>>(*op, *bt, i, yin, yout, sh_tiles, sh_q);
+      else
+        k_stage<R><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, i, yin, yout);
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 1, &used)) return rc;
     }
@@ -708,6 +778,15 @@ int hops_profile_enable(int on) {
     g_prof.n[k] = 0;
   }
   return 0;
+}
+
+int hops_set_option(const char *name, int value) {
+  if (!name) return fail("hops_set_option: NULL name");
+  if (std::string(name) == "stationary_stage") {
+    g_stationary = value != 0;
+    return 0;
+  }
+  return fail("unknown option '%s'", name);
 }
 
 int hops_profile_read(double *ms_out, long long *n_out) {

@@ -210,3 +210,26 @@ def test_device_noise_filter(tlen, tau):
         err = np.max(np.abs(got - want)) / scale
         print("noise", tlen, tau, rand, "max rel err", err)
         assert err < 1e-6
+
+
+@pytest.mark.parametrize("sysname", ["chain", "three_general"])
+def test_stationary_kernel_equals_streaming(sysname):
+    """The operator-stationary stage kernel (batches of >= 8) and the streaming
+    kernel give bit-identical trajectories."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(5, k_mats=1) if sysname == "chain" else systems.three_site(True)
+    tlen, tau = 70.0, 0.5
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(12)]
+    psi0 = np.zeros(len(sysp["HAMILTONIAN"]), complex)
+    psi0[1] = 1
+    for prec in ("fp64", "mixed"):
+        for eom in ("LINEAR", "NORMALIZED NONLINEAR"):
+            _lib.set_option("stationary_stage", 1)
+            a = engine_run(sysp, eom, zs, tlen, tau, 3, 50.0, 1.0, psi0, prec=prec)
+            _lib.set_option("stationary_stage", 0)
+            b = engine_run(sysp, eom, zs, tlen, tau, 3, 50.0, 1.0, psi0, prec=prec)
+            _lib.set_option("stationary_stage", 1)
+            d = max(np.max(np.abs(np.array(a.psi_traj[i]) - np.array(b.psi_traj[i]))) for i in range(12))
+            print(sysname, prec, eom, "stationary vs streaming", d)
+            assert d == 0.0
