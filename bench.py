@@ -106,6 +106,7 @@ def main():
     ap.add_argument("--single-steps", type=int, default=200, help="steps of the plain B=1 FP64 GPU path")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="trajectories interleaved per row (0: autotune)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -129,9 +130,15 @@ def main():
     batch.initialize(psi0())
     torch.cuda.synchronize()
     t_init = time.perf_counter() - t_init
-    st = batch.state
     tb = batch.tables
     ns = int(round(DT * 0.5 / NOISE["TAU"]))
+    from mesohops_amd import autotune
+    if args.lanes == 0:
+        choice, tune_t = autotune.tune(batch, psi0(), DT, verbose=True)
+    else:
+        batch.relayout(args.lanes)
+        choice, tune_t = (args.lanes, True), {}
+    st = batch.state
 
     def run(steps):
         if st.psi_store is None or st.store_slots < steps:
@@ -232,6 +239,8 @@ def main():
         "gpu_fp64_single_traj_steps_per_s": single_rate,
         "speedup_vs_gpu_fp64_single": value / single_rate,
         "init_s": t_init,
+        "layout": {"lanes": choice[0], "stationary_stage": bool(choice[1]),
+                   "autotune_ms_per_step": {"%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_stage<float>", "kernel_avg_ms": stage_ms,

@@ -89,17 +89,21 @@ typedef struct hops_op {
 } hops_op_t;
 
 /*
- * Batch of B trajectories sharing one operator (one basis).  Pools are laid
- * out trajectory-major: trajectory b owns rows [b*n_rows, (b+1)*n_rows).
+ * Batch of B trajectories sharing one operator (one basis).  The state pools
+ * (phi, y0, ya, yb, acc) hold G = ceil(B / T) groups of T = 2^lane_shift
+ * trajectories interleaved row by row: element (b, r) is at index
+ * ((b >> lane_shift) * n_rows + r) * T + (b & (T - 1)); each pool has
+ * G * T * n_rows elements.  The tile list names (group, tile) pairs, a tile
+ * being 256 / T rows of one group.
  */
 typedef struct hops_batch {
   int32_t B, precision, eom, n_t;  /* n_t = noise time points           */
   int32_t coef_stride, ws_stride;  /* per-trajectory doubles            */
-  int32_t store_stride, pad0;      /* psi slots per trajectory          */
+  int32_t store_stride, lane_shift; /* psi slots per trajectory; log2 T */
   double dt_over_hbar;             /* tau / hbar                        */
   int32_t noise_step, pad1;        /* noise points per tau/2            */
-  void *phi_d;   /* complex128 [B*n_rows] state carry                   */
-  void *y0_d;    /* complex64 [B*n_rows] (mixed only)                   */
+  void *phi_d;   /* complex128 [G*T*n_rows] state carry                 */
+  void *y0_d;    /* complex64 [G*T*n_rows] (mixed only)                 */
   void *ya_d;    /* stage input ping   (complex R)                      */
   void *yb_d;    /* stage input pong   (complex R)                      */
   void *acc_d;   /* RK4 accumulator    (complex R)                      */
@@ -109,7 +113,7 @@ typedef struct hops_batch {
   double *ws_d;     /* per-trajectory workspace, ws_stride doubles      */
   double *coef_d;   /* per-trajectory stage coefficients                */
   double *psi_store_d; /* complex128 [B][store_stride][n_state] or NULL */
-  const int32_t *tile_d; /* [n_tiles] trajectory*65536+tile, or -1 idle */
+  const int32_t *tile_d; /* [n_tiles] group*65536+tile, or -1 idle      */
   int32_t n_tiles, tile_rows;
 } hops_batch_t;
 

@@ -1,0 +1,46 @@
+"""Launch-shape autotuner for the RK4 stage kernels.
+
+For a given batch it times a few RK4 steps for each candidate layout (number
+of trajectories interleaved per row) and stage kernel (operator-stationary or
+streaming) and keeps the fastest.  The batch is rewound to its initial state
+afterwards, so tuning does not change results (every candidate computes the
+same numbers; only the memory layout and the launch shape differ).
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+CANDIDATES = [(1, True), (2, True), (4, True), (8, True), (16, True), (4, False), (1, False)]
+
+
+def tune(batch, psi_0, tau, steps=6, candidates=None, verbose=False):
+    """Returns ((lanes, stationary), {candidate: seconds per step})."""
+    candidates = candidates or CANDIDATES
+    nz = batch.noise1[0]
+    ns = int(round(tau * 0.5 / nz.param["TAU"]))
+    timings = {}
+    for lanes, stationary in candidates:
+        if lanes > 1 and batch.B < 8 * lanes and stationary:
+            continue
+        batch.relayout(lanes)
+        batch.rewind(psi_0)
+        _lib.set_option("stationary_stage", 1 if stationary else 0)
+        st = batch.state
+        st.bt.psi_store_d = None
+        st.propagate(0, 1, tau, ns, 0)
+        torch.cuda.synchronize(batch.device)
+        t0 = time.perf_counter()
+        st.propagate(0, steps, tau, ns, 0)
+        torch.cuda.synchronize(batch.device)
+        timings[(lanes, stationary)] = (time.perf_counter() - t0) / steps
+        if verbose:
+            print("autotune lanes=%d stationary=%d: %.3f ms/step" % (lanes, stationary,
+                                                                     timings[(lanes, stationary)] * 1e3))
+    best = min(timings, key=timings.get)
+    batch.relayout(best[0])
+    _lib.set_option("stationary_stage", 1 if best[1] else 0)
+    batch.rewind(psi_0)
+    return best, timings

@@ -136,6 +136,20 @@ __device__ zd block_sum(zd v, double *red) {
   return t;
 }
 
+// Pools (phi, y0, ya, yb, acc) hold groups of T = 2^lane_shift trajectories
+// interleaved row by row: element (b, r) lives at ((b >> lane_shift) * nr + r)
+// * T + (b & (T - 1)).  A gathered cache line then carries T trajectories.
+struct Lane {
+  long base;
+  int T;
+};
+__host__ __device__ __forceinline__ Lane lane_of(const hops_batch_t &bt, long nr, int b) {
+  Lane l;
+  l.T = 1 << bt.lane_shift;
+  l.base = (long)(b >> bt.lane_shift) * nr * l.T + (b & (l.T - 1));
+  return l;
+}
+
 // ---------------------------------------------------------------------------
 // Super-operator build (eom/eom_hops_ksuper.py:36-230, :575-668).
 // Row r = (k, s).  Up connection (k -> k+e_m):  Kp1[(k,s),(k+e_m,s')] =
@@ -239,7 +253,7 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
   const int n = op.n_state, na = op.n_act, Ma = op.n_mode_abs;
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const long nr = (long)n * op.n_hier;
-  const long off = (long)b * nr;
+  const Lane ln = lane_of(bt, nr, b);
   const CoefLayout L = coef_layout(op);
   double *coef = bt.coef_d + (long)b * bt.coef_stride;
   double *ws = bt.ws_d + (long)b * bt.ws_stride;
@@ -255,19 +269,20 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
   double *zt = zh + 2 * na;         // 2Ma
   double *red = zt + 2 * Ma;        // 16
 
-  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d) + off;
+  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d) + ln.base;
+  const int T = ln.T;
   // 1. physical wave function of the stage input (FP64), sigma at stage 0
   double sigma;
   if (stage == 0) {
     zd acc = mk<double>(0.0, 0.0);
     for (int s = tid; s < n; s += nt) {
-      const double2 v = phi[s];
+      const double2 v = phi[(long)s * T];
       acc.x += v.x * v.x + v.y * v.y;
     }
     acc = block_sum(acc, red);
     sigma = normed ? 1.0 / sqrt(acc.x) : 1.0;
     for (int s = tid; s < n; s += nt) {
-      const double2 v = phi[s];
+      const double2 v = phi[(long)s * T];
       psi[2 * s] = sigma * v.x;
       psi[2 * s + 1] = sigma * v.y;
       if (store_slot >= 0 && bt.psi_store_d) {
@@ -279,10 +294,10 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
     if (tid == 0) coef[L.sig] = sigma;
   } else {
     sigma = coef[L.sig];
-    const cx<R> *y = yin + off;
+    const cx<R> *y = yin + ln.base;
     for (int s = tid; s < n; s += nt) {
-      psi[2 * s] = (double)y[s].x;
-      psi[2 * s + 1] = (double)y[s].y;
+      psi[2 * s] = (double)y[(long)s * T].x;
+      psi[2 * s + 1] = (double)y[(long)s * T].y;
     }
   }
   __syncthreads();
@@ -394,20 +409,20 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
         const int r0 = op.la_row_d[q], c0 = op.la_col_d[q];
         zd pe;
         if (stage == 0) {
-          const double2 v = phi[(long)ia * n + c0];
+          const double2 v = phi[((long)ia * n + c0) * T];
           pe = mk<double>(sigma * v.x, sigma * v.y);
         } else {
-          pe = ccast<double>(yin[off + (long)ia * n + c0]);
+          pe = ccast<double>(yin[ln.base + ((long)ia * n + c0) * T]);
         }
         lterm = cadd(lterm, cmul(cconj(ld2(psi, r0)), cmul(ld2(op.la_val_d, q), cmul(gw, pe))));
       }
       for (int s = 0; s < n; ++s) {
         zd pe;
         if (stage == 0) {
-          const double2 v = phi[(long)ia * n + s];
+          const double2 v = phi[((long)ia * n + s) * T];
           pe = mk<double>(sigma * v.x, sigma * v.y);
         } else {
-          pe = ccast<double>(yin[off + (long)ia * n + s]);
+          pe = ccast<double>(yin[ln.base + ((long)ia * n + s) * T]);
         }
         oterm = cadd(oterm, cmul(cconj(ld2(psi, s)), cmul(gw, pe)));
       }
@@ -478,36 +493,37 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
                                           long r, int k, int s, cx<R> k0, const int *ecol, const cx<R> *ev) {
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
-  const long off = (long)b * nr;
-  const cx<R> *Y = yin + off;
+  const Lane ln = lane_of(bt, nr, b);
+  const int T = ln.T;
+  const cx<R> *Y = yin + ln.base;
   const CoefLayout L = coef_layout(op);
   const double *coef = bt.coef_d + (long)b * bt.coef_stride;
   const double sigma = coef[L.sig];
 
-  const cx<R> yr = Y[r];
+  const cx<R> yr = Y[r * T];
   cx<R> d = cmul(k0, yr);
   // system block: -iH + noise / LTC terms
   const double *A = coef + (k == 0 ? L.A0 : L.A);
   const long rowk = (long)k * n;
   for (int p = op.pat_ptr_d[s]; p < op.pat_ptr_d[s + 1]; ++p)
-    cfma(d, ccast<R>(ld2(A, p)), Y[rowk + op.pat_col_d[p]]);
+    cfma(d, ccast<R>(ld2(A, p)), Y[(rowk + op.pat_col_d[p]) * T]);
   // hierarchy super-operator K2_kp1 + K2_km1
   if (WR > 0) {
 #pragma unroll
     for (int e = 0; e < WR; ++e)
-      if (ecol[e] >= 0) cfma(d, ev[e], Y[ecol[e]]);
+      if (ecol[e] >= 0) cfma(d, ev[e], Y[(long)ecol[e] * T]);
   } else {
     const cx<R> *evm = reinterpret_cast<const cx<R> *>(op.ell_val_d);
     for (int e = 0; e < op.ell_w; ++e) {
       const int c = op.ell_col_d[(long)e * nr + r];
       if (c < 0) break;
-      cfma(d, evm[(long)e * nr + r], Y[c]);
+      cfma(d, evm[(long)e * nr + r], Y[(long)c * T]);
     }
   }
   if (bt.eom != HOPS_EOM_LINEAR) {
     const double *zc = coef + L.zc;
     for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
-      cfma(d, ccast<R>(ld2(zc, op.up_mode_d[u])), Y[(long)op.up_aux_d[u] * n + s]);
+      cfma(d, ccast<R>(ld2(zc, op.up_mode_d[u])), Y[((long)op.up_aux_d[u] * n + s) * T]);
     if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
       const cx<R> nc = mk<R>((R)coef[L.nc], (R)coef[L.nc + 1]);
       d = csub(d, cmul(nc, yr));
@@ -515,7 +531,7 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
   }
   if (stage == 0) d = cscale(d, (R)sigma);
 
-  const long gi = off + r;
+  const long gi = ln.base + r * T;
   const double h = bt.dt_over_hbar;
   cx<R> *acc = reinterpret_cast<cx<R> *>(bt.acc_d);
   double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
@@ -550,27 +566,32 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
   }
 }
 
-// Streaming form: one thread per row, workgroups from the tile list.
+// Streaming form: one thread per (row, lane); workgroups from the tile list,
+// a tile being 256 / T rows of one trajectory group.
 template <typename R>
 __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_batch_t bt, int stage,
                                                const cx<R> *__restrict__ yin,
                                                cx<R> *__restrict__ yout) {
   const int t = bt.tile_d[blockIdx.x];
   if (t < 0) return;
-  const int b = t >> 16, tile = t & 0xffff;
+  const int g = t >> 16, tile = t & 0xffff;
+  const int lane = threadIdx.x & ((1 << bt.lane_shift) - 1);
+  const int b = (g << bt.lane_shift) + lane;
+  if (b >= bt.B) return;
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
-  const long r = (long)tile * bt.tile_rows + threadIdx.x;
+  const long r = (long)tile * (bt.tile_rows >> bt.lane_shift) + (threadIdx.x >> bt.lane_shift);
   if (r >= nr) return;
   const int k = (int)(r / n), s = (int)(r - (long)k * n);
   stage_row<R, 0>(op, bt, stage, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
 }
 
 // Operator-stationary form for a batch that shares one operator: workgroup i
-// owns a 512-row tile, keeps its K0 and ELL rows in registers and walks the
-// trajectories b = x + 8 (q + Q j) of XCD x = i % 8, so the operator is read
-// once per tile and stage instead of once per trajectory, and every XCD works
-// on the same few trajectories at a time (their rows stay in its L2).
+// owns a tile of 512 / T rows, keeps its K0 and ELL rows in registers and
+// walks the trajectory groups g = x + 8 (q + Q j) of XCD x = i % 8, so the
+// operator is read once per tile and stage instead of once per trajectory,
+// and every XCD works on the same few groups at a time (their rows stay in
+// its L2).
 template <typename R, int WR>
 __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const hops_batch_t bt, int stage,
                                                       const cx<R> *__restrict__ yin,
@@ -579,7 +600,8 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
   const int tile = slot % n_tiles, q = slot / n_tiles;
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
-  const long r = (long)tile * 512 + threadIdx.x;
+  const int lane = threadIdx.x & ((1 << bt.lane_shift) - 1);
+  const long r = (long)tile * (512 >> bt.lane_shift) + (threadIdx.x >> bt.lane_shift);
   if (r >= nr) return;
   const int k = (int)(r / n), s = (int)(r - (long)k * n);
   int ecol[WR];
@@ -596,8 +618,11 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
     }
   }
   const cx<R> k0 = ccast<R>(ld2(op.k0_d, k));
-  for (int b = x + 8 * q; b < bt.B; b += 8 * Q)
-    stage_row<R, WR>(op, bt, stage, yin, yout, b, r, k, s, k0, ecol, ev);
+  const int G = (bt.B + (1 << bt.lane_shift) - 1) >> bt.lane_shift;
+  for (int g = x + 8 * q; g < G; g += 8 * Q) {
+    const int b = (g << bt.lane_shift) + lane;
+    if (b < bt.B) stage_row<R, WR>(op, bt, stage, yin, yout, b, r, k, s, k0, ecol, ev);
+  }
 }
 
 // y0 = complex64(Phi) at the start of a mixed-precision propagate call.
@@ -614,17 +639,22 @@ __global__ __launch_bounds__(256) void k_final_sigma(const hops_op_t op, const h
                                                      int store_slot) {
   __shared__ double red[16];
   const int n = op.n_state, b = blockIdx.x, tid = threadIdx.x;
-  const long off = (long)b * n * op.n_hier;
-  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d) + off;
+  const Lane ln = lane_of(bt, (long)n * op.n_hier, b);
+  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d) + ln.base;
+  const int T = ln.T;
   zd acc = mk<double>(0.0, 0.0);
-  for (int s = tid; s < n; s += blockDim.x) acc.x += phi[s].x * phi[s].x + phi[s].y * phi[s].y;
+  for (int s = tid; s < n; s += blockDim.x) {
+    const double2 v = phi[(long)s * T];
+    acc.x += v.x * v.x + v.y * v.y;
+  }
   acc = block_sum(acc, red);
   const double sigma = bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR ? 1.0 / sqrt(acc.x) : 1.0;
   if (store_slot >= 0 && bt.psi_store_d) {
     double *dst = bt.psi_store_d + 2 * ((long)b * bt.store_stride + store_slot) * n;
     for (int s = tid; s < n; s += blockDim.x) {
-      dst[2 * s] = sigma * phi[s].x;
-      dst[2 * s + 1] = sigma * phi[s].y;
+      const double2 v = phi[(long)s * T];
+      dst[2 * s] = sigma * v.x;
+      dst[2 * s + 1] = sigma * v.y;
     }
   }
   if (tid == 0) bt.ws_d[(long)b * bt.ws_stride + 4 * op.n_mode_abs] = sigma;
@@ -633,16 +663,20 @@ __global__ __launch_bounds__(256) void k_final_sigma(const hops_op_t op, const h
 __global__ __launch_bounds__(256) void k_final_scale(const hops_op_t op, const hops_batch_t bt) {
   const int t = bt.tile_d[blockIdx.x];
   if (t < 0) return;
-  const int b = t >> 16, tile = t & 0xffff;
+  const int g = t >> 16, tile = t & 0xffff;
+  const int lane = threadIdx.x & ((1 << bt.lane_shift) - 1);
+  const int b = (g << bt.lane_shift) + lane;
+  if (b >= bt.B) return;
   const long nr = (long)op.n_state * op.n_hier;
-  const long r = (long)tile * bt.tile_rows + threadIdx.x;
+  const long r = (long)tile * (bt.tile_rows >> bt.lane_shift) + (threadIdx.x >> bt.lane_shift);
   if (r >= nr) return;
   const double sigma = bt.ws_d[(long)b * bt.ws_stride + 4 * op.n_mode_abs];
-  double2 *phi = reinterpret_cast<double2 *>(bt.phi_d) + (long)b * nr;
-  double2 p = phi[r];
+  const Lane ln = lane_of(bt, nr, b);
+  double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
+  double2 p = phi[ln.base + r * ln.T];
   p.x *= sigma;
   p.y *= sigma;
-  phi[r] = p;
+  phi[ln.base + r * ln.T] = p;
 }
 
 // Optional per-launch timing: an event is recorded on the launch stream after
@@ -703,11 +737,12 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const int ns = bt->noise_step;
   if (it0 < 0 || it0 + (long)n_steps * 2 * ns >= bt->n_t)
     return fail("noise index range [%d, %ld] outside [0, %d)", it0, it0 + (long)n_steps * 2 * ns, bt->n_t);
+  const int G = (bt->B + (1 << bt->lane_shift) - 1) >> bt->lane_shift;
   cx<R> *Y0 = reinterpret_cast<cx<R> *>(bt->precision == HOPS_PREC_MIXED ? bt->y0_d : bt->phi_d);
   cx<R> *ya = reinterpret_cast<cx<R> *>(bt->ya_d);
   cx<R> *yb = reinterpret_cast<cx<R> *>(bt->yb_d);
   if (bt->precision == HOPS_PREC_MIXED) {
-    const long tot = nr * bt->B;
+    const long tot = nr * ((long)G << bt->lane_shift);
     k_prime_y0<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(reinterpret_cast<const double2 *>(bt->phi_d),
                                                             reinterpret_cast<float2 *>(bt->y0_d), tot);
     HIP_TRY(hipGetLastError());
@@ -716,10 +751,11 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   // operator-stationary stage kernel when every XCD has trajectories to walk
   int sh_w = 0, sh_tiles = 0, sh_q = 1;
   unsigned sh_grid = 0;
-  if (g_stationary && bt->B >= 8 && op->ell_w <= 16) {
+  if (g_stationary && G >= 8 && op->ell_w <= 16) {
     sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
-    sh_tiles = (int)((nr + 511) / 512);
-    const int per_xcd = (bt->B + 7) / 8;
+    const long rows_per_tile = 512 >> bt->lane_shift;
+    sh_tiles = (int)((nr + rows_per_tile - 1) / rows_per_tile);
+    const int per_xcd = (G + 7) / 8;
     sh_q = (per_xcd + 31) / 32;
     sh_grid = (unsigned)(8L * sh_q * sh_tiles);
   }
@@ -822,6 +858,7 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_s
   if (int rc = check_op(op)) return rc;
   if (!bt) return fail("batch is NULL");
   if (bt->B <= 0 || bt->tile_rows != 256 || bt->n_tiles <= 0) return fail("bad batch geometry");
+  if (bt->lane_shift < 0 || bt->lane_shift > 4) return fail("lane_shift %d outside [0, 4]", bt->lane_shift);
   if (bt->eom < 0 || bt->eom > 2) return fail("unknown eom %d", bt->eom);
   const CoefLayout L = coef_layout(*op);
   if (bt->coef_stride < L.sig + 1) return fail("coef_stride %d < %d", bt->coef_stride, L.sig + 1);

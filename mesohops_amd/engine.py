@@ -89,17 +89,18 @@ class DeviceOperator:
         return rows[m], col[m], val[m]
 
 
-def tile_map(B, n_rows):
-    """Workgroup -> (trajectory, tile) list.  With at least 8 trajectories each
-    trajectory's tiles are pinned to one XCD (workgroup i runs on XCD i % 8),
-    so its neighbour gathers hit that XCD's L2; smaller batches spread every
-    trajectory over all XCDs."""
-    tpt = (n_rows + TILE_ROWS - 1) // TILE_ROWS
-    if tpt >= 65536 or B >= 32768:
+def tile_map(G, n_rows, lanes=1):
+    """Workgroup -> (group, tile) list; a tile is 256 / lanes rows of one group
+    of ``lanes`` interleaved trajectories.  With at least 8 groups each group's
+    tiles are pinned to one XCD (workgroup i runs on XCD i % 8), so its
+    neighbour gathers hit that XCD's L2; fewer groups spread over all XCDs."""
+    rows = TILE_ROWS // lanes
+    tpt = (n_rows + rows - 1) // rows
+    if tpt >= 65536 or G >= 32768:
         raise EngineError("batch too large for the tile encoding")
-    if B < N_XCD:
-        return np.array([b * 65536 + t for b in range(B) for t in range(tpt)], dtype=np.int32)
-    queues = [[b * 65536 + t for b in range(x, B, N_XCD) for t in range(tpt)] for x in range(N_XCD)]
+    if G < N_XCD:
+        return np.array([g * 65536 + t for g in range(G) for t in range(tpt)], dtype=np.int32)
+    queues = [[g * 65536 + t for g in range(x, G, N_XCD) for t in range(tpt)] for x in range(N_XCD)]
     depth = max(len(q) for q in queues)
     out = np.full(depth * N_XCD, -1, dtype=np.int32)
     for x, q in enumerate(queues):
@@ -107,11 +108,17 @@ def tile_map(B, n_rows):
     return out
 
 
+def default_lanes(B):
+    """Trajectories interleaved per row: 4 once there are enough trajectories
+    to give every XCD at least one group of 4, else 1."""
+    return 4 if B >= 32 else 1
+
+
 class BatchState:
     """Device buffers of B trajectories that share one DeviceOperator."""
 
     def __init__(self, dop: DeviceOperator, B: int, eom_kind: int, noise1, noise2, n_t: int,
-                 store_slots: int = 0):
+                 store_slots: int = 0, lanes: int = None):
         self.dop = dop
         self.device = dop.device
         t = dop.tables
@@ -119,13 +126,19 @@ class BatchState:
         self.eom_kind = eom_kind
         nr = dop.n_rows
         prec = dop.precision
+        lanes = default_lanes(B) if lanes is None else int(lanes)
+        if lanes not in (1, 2, 4, 8, 16):
+            raise EngineError("lanes must be a power of two <= 16")
+        self.lanes = lanes
+        self.G = (B + lanes - 1) // lanes
+        P = self.G * lanes * nr
         cR = torch.complex128 if prec == _lib.HOPS_PREC_FP64 else torch.complex64
         dev = self.device
-        self.phi = torch.zeros(B * nr, dtype=torch.complex128, device=dev)
-        self.y0 = torch.empty(B * nr if prec == _lib.HOPS_PREC_MIXED else 1, dtype=torch.complex64, device=dev)
-        self.ya = torch.empty(B * nr, dtype=cR, device=dev)
-        self.yb = torch.empty(B * nr, dtype=cR, device=dev)
-        self.acc = torch.empty(B * nr, dtype=cR, device=dev)
+        self.phi = torch.zeros(P, dtype=torch.complex128, device=dev)
+        self.y0 = torch.empty(P if prec == _lib.HOPS_PREC_MIXED else 1, dtype=torch.complex64, device=dev)
+        self.ya = torch.empty(P, dtype=cR, device=dev)
+        self.yb = torch.empty(P, dtype=cR, device=dev)
+        self.acc = torch.empty(P, dtype=cR, device=dev)
         if noise1.shape != (B, t.n_act, n_t):
             raise EngineError("noise1 shape %s != %s" % (tuple(noise1.shape), (B, t.n_act, n_t)))
         self.z1 = noise1.contiguous()
@@ -138,7 +151,7 @@ class BatchState:
         self.store_slots = store_slots
         self.psi_store = (torch.zeros(B * store_slots * t.n_state, dtype=torch.complex128, device=dev)
                           if store_slots > 0 else None)
-        self.tiles = torch.from_numpy(tile_map(B, nr)).to(dev)
+        self.tiles = torch.from_numpy(tile_map(self.G, nr, lanes)).to(dev)
         bt = HopsBatch()
         bt.B = B
         bt.precision = prec
@@ -147,6 +160,7 @@ class BatchState:
         bt.coef_stride = self.coef_stride
         bt.ws_stride = self.ws_stride
         bt.store_stride = store_slots
+        bt.lane_shift = int(lanes).bit_length() - 1
         bt.phi_d = self.phi.data_ptr()
         bt.y0_d = self.y0.data_ptr()
         bt.ya_d = self.ya.data_ptr()
@@ -163,15 +177,26 @@ class BatchState:
         bt.tile_rows = TILE_ROWS
         self.bt = bt
 
+    def _to_pool(self, a):
+        """[B, n_rows] -> interleaved pool order [G, n_rows, T]."""
+        nr, T = self.dop.n_rows, self.lanes
+        full = np.zeros((self.G * T, nr), dtype=a.dtype)
+        full[: self.B] = a
+        return np.ascontiguousarray(full.reshape(self.G, T, nr).transpose(0, 2, 1)).reshape(-1)
+
+    def _from_pool(self, flat):
+        nr, T = self.dop.n_rows, self.lanes
+        return flat.reshape(self.G, nr, T).transpose(0, 2, 1).reshape(self.G * T, nr)[: self.B]
+
     def set_phi(self, phi_host):
         """phi_host: complex [B, n_rows] (or [n_rows], broadcast)."""
         nr = self.dop.n_rows
         a = np.asarray(phi_host, dtype=np.complex128)
         a = np.broadcast_to(a.reshape(-1, nr), (self.B, nr))
-        self.phi.copy_(torch.from_numpy(np.ascontiguousarray(a).reshape(-1)).to(self.device))
+        self.phi.copy_(torch.from_numpy(self._to_pool(a)).to(self.device))
 
     def phi_host(self):
-        return self.phi.cpu().numpy().reshape(self.B, -1)
+        return np.ascontiguousarray(self._from_pool(self.phi.cpu().numpy()))
 
     def zmem_host(self):
         return self.zmem.cpu().numpy()[: self.B * self.dop.tables.n_mode_abs].reshape(self.B, -1)

@@ -134,7 +134,8 @@ class HopsBatch:
 
     def __init__(self, system_param, eom_param=None, noise_param=None, noise2_param=None,
                  hierarchy_param=None, integration_param=None, seeds=None, noise_params=None,
-                 precision_mode="fp64", device="cuda:0", store_psi=True):
+                 precision_mode="fp64", device="cuda:0", store_psi=True, lanes=None):
+        self.lanes = lanes
         self.setup = _Setup(system_param, eom_param, noise_param, noise2_param, hierarchy_param,
                             integration_param)
         if noise_params is None:
@@ -169,7 +170,7 @@ class HopsBatch:
         if self.noise2[0].model != "ZERO":
             z2 = device_noise(self.noise2, tb.absL_active, self.device, sp)
         self.state = BatchState(self.dop, self.B, self.setup.eom_kind, z1, z2, self.noise1[0].n_t,
-                                store_slots=0)
+                                store_slots=0, lanes=self.lanes)
         psi0 = np.asarray(psi_0, dtype=np.complex128)
         phi = np.zeros(tb.n_hier * tb.n_state, dtype=np.complex128)
         phi[:tb.n_state] = psi0[tb.state_list]
@@ -232,6 +233,21 @@ class HopsBatch:
         self.psi_traj = [[phi[:tb.n_state].copy()] for _ in range(self.B)]
         self.t_axis = [0.0]
         self.t = 0.0
+
+    def relayout(self, lanes):
+        """Rebuild the device pools with ``lanes`` interleaved trajectories per
+        row, keeping the state, the noise memory and the noise."""
+        old = self.state
+        phi = old.phi_host()
+        zmem = old.zmem.clone()
+        self.state = None
+        del old.phi, old.y0, old.ya, old.yb, old.acc
+        st = BatchState(self.dop, self.B, self.setup.eom_kind, old.z1, old.z2, self.noise1[0].n_t,
+                        store_slots=0, lanes=lanes)
+        st.set_phi(phi)
+        st.zmem.copy_(zmem)
+        self.state = st
+        self.lanes = lanes
 
     def synchronize(self):
         torch.cuda.synchronize(self.device)

@@ -36,7 +36,7 @@ def oracle_run(sysp, eom, noise_param, maxhier, t_max, dt, noise2_param=None):
 
 
 def engine_run(sysp, eom, noise_arrays, tlen, tau_noise, maxhier, t_max, dt, psi0, prec="fp64",
-               noise2_arrays=None):
+               noise2_arrays=None, lanes=None):
     from mesohops_amd import HopsBatch
     nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau_noise} for z in noise_arrays]
     n2 = None
@@ -44,7 +44,7 @@ def engine_run(sysp, eom, noise_arrays, tlen, tau_noise, maxhier, t_max, dt, psi
         n2 = {"SEED": noise2_arrays[0], "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau_noise,
               "FLAG_REAL": False}
     b = HopsBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], n2, {"MAXHIER": maxhier},
-                  noise_params=nps, precision_mode=prec)
+                  noise_params=nps, precision_mode=prec, lanes=lanes)
     b.initialize(psi0)
     b.propagate(t_max, dt)
     return b
@@ -233,3 +233,27 @@ def test_stationary_kernel_equals_streaming(sysname):
             d = max(np.max(np.abs(np.array(a.psi_traj[i]) - np.array(b.psi_traj[i]))) for i in range(12))
             print(sysname, prec, eom, "stationary vs streaming", d)
             assert d == 0.0
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_interleaved_lanes_equal_plain_layout(lanes):
+    """Interleaving T trajectories per row changes only the memory layout:
+    trajectories are bit-identical to the one-per-row layout (both kernels)."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1)
+    tlen, tau = 60.0, 0.5
+    B = 70
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[0] = 1
+    for eom in ("LINEAR", "NORMALIZED NONLINEAR"):
+        ref = engine_run(sysp, eom, zs, tlen, tau, 3, 40.0, 1.0, psi0, prec="mixed", lanes=1)
+        for stationary in (1, 0):
+            _lib.set_option("stationary_stage", stationary)
+            got = engine_run(sysp, eom, zs, tlen, tau, 3, 40.0, 1.0, psi0, prec="mixed", lanes=lanes)
+            _lib.set_option("stationary_stage", 1)
+            d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+            dphi = np.max(np.abs(got.phi - ref.phi))
+            print("lanes", lanes, eom, "stationary", stationary, d, dphi)
+            assert d == 0.0 and dphi == 0.0

@@ -126,6 +126,27 @@ def test_tile_map_is_xcd_aware():
             assert (v >> 16) % N_XCD == i % N_XCD
     small = tile_map(2, 300)
     assert small.tolist() == [0, 1, 65536, 65537]
+    # 4 interleaved lanes: a tile is 64 rows of one group
+    t4 = tile_map(8, 130, 4)
+    assert sorted(t4[t4 >= 0].tolist()) == sorted(g * 65536 + k for g in range(8) for k in range(3))
+
+
+def test_pool_interleave_roundtrip():
+    """[B, n_rows] <-> interleaved pool order is a bijection with element
+    (b, r) at ((b >> s) n_rows + r) T + (b & (T-1))."""
+    from mesohops_amd.engine import BatchState
+    st = BatchState.__new__(BatchState)
+
+    class D:
+        n_rows = 7
+    st.dop, st.B, st.lanes = D(), 10, 4
+    st.G = 3
+    a = (np.arange(70) + 1j * np.arange(70)).reshape(10, 7)
+    flat = st._to_pool(a)
+    for b in range(10):
+        for r in range(7):
+            assert flat[((b >> 2) * 7 + r) * 4 + (b & 3)] == a[b, r]
+    assert np.array_equal(st._from_pool(flat), a)
 
 
 def test_parameter_validation():
