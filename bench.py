@@ -107,6 +107,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="trajectories interleaved per row (0: autotune)")
+    ap.add_argument("--chunk", type=int, default=0, help="trajectories per cache-resident chunk with --lanes")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -137,7 +138,8 @@ def main():
         choice, tune_t = autotune.tune(batch, psi0(), DT, verbose=True)
     else:
         batch.relayout(args.lanes)
-        choice, tune_t = (args.lanes, True), {}
+        _lib.set_option("chunk_traj", args.chunk)
+        choice, tune_t = (args.lanes, True, args.chunk), {}
     st = batch.state
 
     def run(steps):
@@ -239,8 +241,8 @@ def main():
         "gpu_fp64_single_traj_steps_per_s": single_rate,
         "speedup_vs_gpu_fp64_single": value / single_rate,
         "init_s": t_init,
-        "layout": {"lanes": choice[0], "stationary_stage": bool(choice[1]),
-                   "autotune_ms_per_step": {"%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
+        "layout": {"lanes": choice[0], "stationary_stage": bool(choice[1]), "chunk_traj": choice[2],
+                   "autotune_ms_per_step": {"%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_stage<float>", "kernel_avg_ms": stage_ms,

@@ -47,7 +47,7 @@ extern "C" {
 typedef struct hops_op {
   int32_t n_state, n_hier, n_mode, n_lop, n_act, n_first, ell_w, n_pat;
   int32_t n_mode_abs; /* number of absolute hierarchy modes (z_mem length) */
-  int32_t pad0;
+  int32_t pat_w;      /* most pattern entries in one state row            */
   /* auxiliary multi-indices, CSR over auxiliaries: absolute mode, depth */
   const int32_t *aux_ptr_d, *aux_mode_d, *aux_val_d;
   /* connections: up (k -> k+e_m) and down (k -> k-e_m, with k_m of aux k) */

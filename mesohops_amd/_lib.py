@@ -23,7 +23,7 @@ _vp = ctypes.c_void_p
 
 # field order must match struct hops_op in include/hops_mi355x.h
 _OP_INTS = ["n_state", "n_hier", "n_mode", "n_lop", "n_act", "n_first", "ell_w", "n_pat",
-            "n_mode_abs", "pad0"]
+            "n_mode_abs", "pat_w"]
 _OP_PTRS = ["aux_ptr", "aux_mode", "aux_val",
             "up_ptr", "up_aux", "up_mode",
             "dn_ptr", "dn_aux", "dn_mode", "dn_val",

@@ -13,21 +13,26 @@ import torch
 
 from . import _lib
 
-CANDIDATES = [(1, True), (2, True), (4, True), (8, True), (16, True), (4, False), (1, False)]
+# (lanes, stationary stage kernel, trajectories per cache-resident chunk; 0 = whole batch)
+CANDIDATES = [(1, True, 0), (4, True, 0), (8, True, 0), (16, True, 0), (4, False, 0),
+              (4, True, 32), (4, True, 64), (8, True, 64), (8, True, 128), (2, True, 32)]
 
 
-def tune(batch, psi_0, tau, steps=6, candidates=None, verbose=False):
-    """Returns ((lanes, stationary), {candidate: seconds per step})."""
+def tune(batch, psi_0, tau, steps=12, candidates=None, verbose=False):
+    """Returns ((lanes, stationary, chunk), {candidate: seconds per step})."""
     candidates = candidates or CANDIDATES
     nz = batch.noise1[0]
     ns = int(round(tau * 0.5 / nz.param["TAU"]))
     timings = {}
-    for lanes, stationary in candidates:
+    for lanes, stationary, chunk in candidates:
         if lanes > 1 and batch.B < 8 * lanes and stationary:
+            continue
+        if chunk and (chunk >= batch.B or chunk < 8 * lanes):
             continue
         batch.relayout(lanes)
         batch.rewind(psi_0)
         _lib.set_option("stationary_stage", 1 if stationary else 0)
+        _lib.set_option("chunk_traj", chunk)
         st = batch.state
         st.bt.psi_store_d = None
         st.propagate(0, 1, tau, ns, 0)
@@ -35,12 +40,13 @@ def tune(batch, psi_0, tau, steps=6, candidates=None, verbose=False):
         t0 = time.perf_counter()
         st.propagate(0, steps, tau, ns, 0)
         torch.cuda.synchronize(batch.device)
-        timings[(lanes, stationary)] = (time.perf_counter() - t0) / steps
+        timings[(lanes, stationary, chunk)] = (time.perf_counter() - t0) / steps
         if verbose:
-            print("autotune lanes=%d stationary=%d: %.3f ms/step" % (lanes, stationary,
-                                                                     timings[(lanes, stationary)] * 1e3))
+            print("autotune lanes=%d stationary=%d chunk=%d: %.3f ms/step"
+                  % (lanes, stationary, chunk, timings[(lanes, stationary, chunk)] * 1e3))
     best = min(timings, key=timings.get)
     batch.relayout(best[0])
     _lib.set_option("stationary_stage", 1 if best[1] else 0)
+    _lib.set_option("chunk_traj", best[2])
     batch.rewind(psi_0)
     return best, timings

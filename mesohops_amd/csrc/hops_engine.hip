@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/hops_mi355x.h"
 
@@ -586,13 +587,103 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
   stage_row<R, 0>(op, bt, stage, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
 }
 
+// Row update with the row's pattern and ELL entries in registers and every
+// load issued before the arithmetic (more bytes in flight per wave).
+// pidx[p] < 0 and ecol[e] < 0 mark unused slots.
+template <typename R, int WR, int PW>
+__device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_batch_t &bt, int stage,
+                                              const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
+                                              int b, long r, int k, cx<R> k0, const int *ecol,
+                                              const cx<R> *ev, const int *pidx, const int *pcol) {
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const Lane ln = lane_of(bt, nr, b);
+  const int T = ln.T;
+  const cx<R> *Y = yin + ln.base;
+  const CoefLayout L = coef_layout(op);
+  const double *coef = bt.coef_d + (long)b * bt.coef_stride;
+  const long rowk = (long)k * n;
+  const long gi = ln.base + r * T;
+  cx<R> *acc = reinterpret_cast<cx<R> *>(bt.acc_d);
+  double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
+  const cx<R> zero = mk<R>((R)0, (R)0);
+  // loads
+  const cx<R> yr = Y[r * T];
+  cx<R> yp[PW], ye[WR];
+  double2 ap[PW];
+  const double2 *A = reinterpret_cast<const double2 *>(coef + (k == 0 ? L.A0 : L.A));
+#pragma unroll
+  for (int p = 0; p < PW; ++p) {
+    yp[p] = pidx[p] >= 0 ? Y[(rowk + pcol[p]) * T] : zero;
+    ap[p] = pidx[p] >= 0 ? A[pidx[p]] : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? Y[(long)ecol[e] * T] : zero;
+  const double sigma = coef[L.sig];
+  cx<R> base = zero, a_in = zero;
+  double2 ph = make_double2(0.0, 0.0);
+  if (stage < 3) {
+    if (bt.precision == HOPS_PREC_MIXED) {
+      base = reinterpret_cast<const cx<R> *>(bt.y0_d)[gi];
+    } else {
+      const double2 p = phi[gi];
+      base = mk<R>((R)p.x, (R)p.y);
+    }
+  } else {
+    ph = phi[gi];
+  }
+  if (stage > 0) a_in = acc[gi];
+  // arithmetic
+  cx<R> d = cmul(k0, yr);
+#pragma unroll
+  for (int p = 0; p < PW; ++p) cfma(d, mk<R>((R)ap[p].x, (R)ap[p].y), yp[p]);
+#pragma unroll
+  for (int e = 0; e < WR; ++e) cfma(d, ev[e], ye[e]);
+  if (bt.eom != HOPS_EOM_LINEAR) {
+    const double *zc = coef + L.zc;
+    for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
+      cfma(d, ccast<R>(ld2(zc, op.up_mode_d[u])), Y[((long)op.up_aux_d[u] * n + (r - rowk)) * T]);
+    if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
+      const cx<R> nc = mk<R>((R)coef[L.nc], (R)coef[L.nc + 1]);
+      d = csub(d, cmul(nc, yr));
+    }
+  }
+  if (stage == 0) d = cscale(d, (R)sigma);
+  const double h = bt.dt_over_hbar;
+  if (stage < 3) {
+    if (bt.precision == HOPS_PREC_MIXED) {
+      base = cscale(base, (R)sigma);
+    } else {
+      const double2 p = phi[gi];
+      base = mk<R>((R)(sigma * p.x), (R)(sigma * p.y));
+    }
+    const R c = (R)((stage == 2 ? 1.0 : 0.5) * h);
+    yout[gi] = mk<R>(base.x + c * d.x, base.y + c * d.y);
+    if (stage == 0) {
+      acc[gi] = d;
+    } else {
+      a_in.x += (R)2 * d.x;
+      a_in.y += (R)2 * d.y;
+      acc[gi] = a_in;
+    }
+  } else {
+    const double c = h / 6.0;
+    double2 q;
+    q.x = sigma * ph.x + c * ((double)a_in.x + (double)d.x);
+    q.y = sigma * ph.y + c * ((double)a_in.y + (double)d.y);
+    phi[gi] = q;
+    if (bt.precision == HOPS_PREC_MIXED)
+      reinterpret_cast<cx<float> *>(bt.y0_d)[gi] = mk<float>((float)q.x, (float)q.y);
+  }
+}
+
 // Operator-stationary form for a batch that shares one operator: workgroup i
 // owns a tile of 512 / T rows, keeps its K0 and ELL rows in registers and
 // walks the trajectory groups g = x + 8 (q + Q j) of XCD x = i % 8, so the
 // operator is read once per tile and stage instead of once per trajectory,
 // and every XCD works on the same few groups at a time (their rows stay in
 // its L2).
-template <typename R, int WR>
+template <typename R, int WR, int PW>
 __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const hops_batch_t bt, int stage,
                                                       const cx<R> *__restrict__ yin,
                                                       cx<R> *__restrict__ yout, int n_tiles, int Q) {
@@ -617,11 +708,18 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
       ev[e] = mk<R>((R)0, (R)0);
     }
   }
+  int pidx[PW], pcol[PW];
+  const int p0 = op.pat_ptr_d[s], np = op.pat_ptr_d[s + 1] - p0;
+#pragma unroll
+  for (int p = 0; p < PW; ++p) {
+    pidx[p] = p < np ? p0 + p : -1;
+    pcol[p] = p < np ? op.pat_col_d[p0 + p] : 0;
+  }
   const cx<R> k0 = ccast<R>(ld2(op.k0_d, k));
   const int G = (bt.B + (1 << bt.lane_shift) - 1) >> bt.lane_shift;
   for (int g = x + 8 * q; g < G; g += 8 * Q) {
     const int b = (g << bt.lane_shift) + lane;
-    if (b < bt.B) stage_row<R, WR>(op, bt, stage, yin, yout, b, r, k, s, k0, ecol, ev);
+    if (b < bt.B) stage_row_reg<R, WR, PW>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
   }
 }
 
@@ -690,6 +788,7 @@ struct Profile {
 };
 Profile g_prof;
 bool g_stationary = true;
+int g_chunk_traj = 0;
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
   if (!g_prof.on) return 0;
@@ -748,38 +847,82 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     HIP_TRY(hipGetLastError());
   }
   const int toff[4] = {0, ns, ns, 2 * ns};
-  // operator-stationary stage kernel when every XCD has trajectories to walk
-  int sh_w = 0, sh_tiles = 0, sh_q = 1;
+  // operator-stationary stage kernel when every XCD has a trajectory group and
+  // the row's ELL and pattern entries fit in registers
+  const int max_pat = op->pat_w;
+  const int T = 1 << bt->lane_shift;
+  int chunk = g_chunk_traj;
+  if (chunk > 0) chunk = (chunk / (8 * T)) * 8 * T;  // whole groups, one per XCD at least
+  if (chunk <= 0 || chunk >= bt->B) chunk = bt->B;
+  const int Gc = (chunk + T - 1) / T;
+  int sh_w = 0, sh_p = 0, sh_tiles = 0, sh_q = 1;
   unsigned sh_grid = 0;
-  if (g_stationary && G >= 8 && op->ell_w <= 16) {
+  if (g_stationary && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
     sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
+    sh_p = max_pat <= 4 ? 4 : 8;
     const long rows_per_tile = 512 >> bt->lane_shift;
     sh_tiles = (int)((nr + rows_per_tile - 1) / rows_per_tile);
-    const int per_xcd = (G + 7) / 8;
+    const int per_xcd = (Gc + 7) / 8;
     sh_q = (per_xcd + 31) / 32;
     sh_grid = (unsigned)(8L * sh_q * sh_tiles);
+  } else {
+    chunk = bt->B;  // chunks need the stationary kernel (no tile list per chunk)
   }
   size_t used = 0;
   if (int rc = prof_mark(st, 2, &used)) return rc;
-  for (int step = 0; step < n_steps; ++step) {
+  // one RK4 step of a (sub-)batch; slot: psi store slot written at stage 0
+  auto one_step = [&](const hops_batch_t &sb, int step) -> int {
     const int itb = it0 + step * 2 * ns;
     for (int i = 0; i < 4; ++i) {
-      const cx<R> *yin = i == 0 ? Y0 : (i == 2 ? yb : ya);
-      cx<R> *yout = i == 1 ? yb : ya;
-      const int slot = (i == 0 && step > 0 && bt->psi_store_d) ? store0 + step - 1 : -1;
-      k_prologue<R><<<bt->B, 256, shm, st>>>(*op, *bt, i, itb + toff[i], yin, slot);
+      const cx<R> *yin = i == 0 ? reinterpret_cast<const cx<R> *>(sb.precision == HOPS_PREC_MIXED ? sb.y0_d : sb.phi_d)
+                                : reinterpret_cast<const cx<R> *>(i == 2 ? sb.yb_d : sb.ya_d);
+      cx<R> *yout = reinterpret_cast<cx<R> *>(i == 1 ? sb.yb_d : sb.ya_d);
+      const int slot = (i == 0 && step > 0 && sb.psi_store_d) ? store0 + step - 1 : -1;
+      k_prologue<R><<<sb.B, 256, shm, st>>>(*op, sb, i, itb + toff[i], yin, slot);
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 0, &used)) return rc;
-      if (sh_w == 4)
-        k_stage_shared<R, 4><<<sh_grid, 512, 0, st>>>(*op, *bt, i, yin, yout, sh_tiles, sh_q);
-      else if (sh_w == 8)
-        k_stage_shared<R, 8><<<sh_grid, 512, 0, st>>>(*op, *bt, i, yin, yout, sh_tiles, sh_q);
-      else if (sh_w == 16)
-        k_stage_shared<R, 16><<<sh_grid, 512, 0, st>>>(*op, *bt, i, yin, yout, sh_tiles, sh_q);
-      else
-        k_stage<R><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, i, yin, yout);
+#define HOPS_SH(W, P) k_stage_shared<R, W, P><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q)
+      if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
+      else if (sh_w == 4 && sh_p == 8) HOPS_SH(4, 8);
+      else if (sh_w == 8 && sh_p == 4) HOPS_SH(8, 4);
+      else if (sh_w == 8 && sh_p == 8) HOPS_SH(8, 8);
+      else if (sh_w == 16 && sh_p == 4) HOPS_SH(16, 4);
+      else if (sh_w == 16 && sh_p == 8) HOPS_SH(16, 8);
+      else k_stage<R><<<sb.n_tiles, 256, 0, st>>>(*op, sb, i, yin, yout);
+#undef HOPS_SH
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 1, &used)) return rc;
+    }
+    return 0;
+  };
+  // sub-batch of trajectories [b0, b0 + nb): every pointer offset to it
+  auto sub_batch = [&](int b0, int nb) {
+    hops_batch_t sb = *bt;
+    const size_t es = sizeof(cx<R>), pe = (size_t)b0 * nr;  // b0 is a multiple of T
+    sb.B = nb;
+    sb.phi_d = (char *)bt->phi_d + pe * 16;
+    sb.y0_d = bt->precision == HOPS_PREC_MIXED ? (char *)bt->y0_d + pe * 8 : bt->y0_d;
+    sb.ya_d = (char *)bt->ya_d + pe * es;
+    sb.yb_d = (char *)bt->yb_d + pe * es;
+    sb.acc_d = (char *)bt->acc_d + pe * es;
+    sb.z1_d = (const char *)bt->z1_d + (size_t)b0 * op->n_act * bt->n_t * 8;
+    if (bt->z2_d) sb.z2_d = (const char *)bt->z2_d + (size_t)b0 * op->n_act * bt->n_t * 8;
+    sb.zmem_d = bt->zmem_d + (size_t)b0 * 2 * op->n_mode_abs;
+    sb.ws_d = bt->ws_d + (size_t)b0 * bt->ws_stride;
+    sb.coef_d = bt->coef_d + (size_t)b0 * bt->coef_stride;
+    if (bt->psi_store_d) sb.psi_store_d = bt->psi_store_d + (size_t)b0 * bt->store_stride * 2 * op->n_state;
+    return sb;
+  };
+  if (chunk >= bt->B) {
+    for (int step = 0; step < n_steps; ++step)
+      if (int rc = one_step(*bt, step)) return rc;
+  } else {
+    // all steps of one chunk before the next: the chunk's pools stay in the
+    // last-level cache across the whole propagation
+    for (int b0 = 0; b0 < bt->B; b0 += chunk) {
+      const hops_batch_t sb = sub_batch(b0, std::min(chunk, bt->B - b0));
+      for (int step = 0; step < n_steps; ++step)
+        if (int rc = one_step(sb, step)) return rc;
     }
   }
   if (int rc = prof_collect(used)) return rc;
@@ -820,6 +963,11 @@ int hops_set_option(const char *name, int value) {
   if (!name) return fail("hops_set_option: NULL name");
   if (std::string(name) == "stationary_stage") {
     g_stationary = value != 0;
+    return 0;
+  }
+  if (std::string(name) == "chunk_traj") {
+    if (value < 0) return fail("chunk_traj must be >= 0");
+    g_chunk_traj = value;
     return 0;
   }
   return fail("unknown option '%s'", name);

@@ -54,6 +54,8 @@ class DeviceOperator:
         for name in ("n_state", "n_hier", "n_mode", "n_lop", "n_act", "ell_w", "n_pat", "n_mode_abs"):
             setattr(op, name, int(getattr(tables, name)))
         op.n_first = len(tables.first)
+        pp = tables.tables["pat_ptr"]
+        op.pat_w = int(np.max(np.diff(pp))) if len(pp) > 1 else 0
         for name, arr in tables.tables.items():
             if arr is None:
                 setattr(op, name + "_d", None)

@@ -257,3 +257,28 @@ def test_interleaved_lanes_equal_plain_layout(lanes):
             dphi = np.max(np.abs(got.phi - ref.phi))
             print("lanes", lanes, eom, "stationary", stationary, d, dphi)
             assert d == 0.0 and dphi == 0.0
+
+
+def test_chunked_propagation_equals_whole_batch():
+    """Propagating cache-sized chunks of the batch one after another (all
+    steps of a chunk before the next) gives bit-identical trajectories."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1)
+    tlen, tau = 60.0, 0.5
+    B = 70
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[0] = 1
+    for eom in ("LINEAR", "NORMALIZED NONLINEAR"):
+        ref = engine_run(sysp, eom, zs, tlen, tau, 3, 40.0, 1.0, psi0, prec="mixed", lanes=4)
+        _lib.set_option("chunk_traj", 32)
+        try:
+            got = engine_run(sysp, eom, zs, tlen, tau, 3, 40.0, 1.0, psi0, prec="mixed", lanes=4)
+        finally:
+            _lib.set_option("chunk_traj", 0)
+        d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+        dphi = np.max(np.abs(got.phi - ref.phi))
+        dz = np.max(np.abs(got.state.zmem_host() - ref.state.zmem_host()))
+        print("chunked", eom, d, dphi, dz)
+        assert d == 0.0 and dphi == 0.0 and dz == 0.0
