@@ -76,11 +76,16 @@ template <typename R>
 __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
   return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-// a += b * c
+// a += b * c as four explicit fused multiply-adds: every kernel variant
+// performs exactly this sequence, so they agree bit for bit.
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return fma(a, b, c); }
 template <typename R>
 __device__ __forceinline__ void cfma(cx<R> &a, cx<R> b, cx<R> c) {
-  a.x += b.x * c.x - b.y * c.y;
-  a.y += b.x * c.y + b.y * c.x;
+  a.x = fmaR(b.x, c.x, a.x);
+  a.x = fmaR(-b.y, c.y, a.x);
+  a.y = fmaR(b.x, c.y, a.y);
+  a.y = fmaR(b.y, c.x, a.y);
 }
 template <typename R>
 __device__ __forceinline__ cx<R> cscale(cx<R> a, R s) {
@@ -492,6 +497,7 @@ template <typename R, int WR>
 __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_t &bt, int stage,
                                           const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout, int b,
                                           long r, int k, int s, cx<R> k0, const int *ecol, const cx<R> *ev) {
+#pragma clang fp contract(off)
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
   const Lane ln = lane_of(bt, nr, b);
@@ -595,6 +601,7 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
                                               const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
                                               int b, long r, int k, cx<R> k0, const int *ecol,
                                               const cx<R> *ev, const int *pidx, const int *pcol) {
+#pragma clang fp contract(off)
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
   const Lane ln = lane_of(bt, nr, b);
