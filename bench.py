@@ -211,7 +211,9 @@ def main():
 
     stage_ms = ms[1] / max(n[1], 1)
     bytes_step = algorithmic_bytes_per_traj_step(tb, B)
-    bytes_launch = bytes_step * B / 4.0
+    # launches may cover a chunk of the batch: spread the timed region's
+    # algorithmic bytes over the stage launches that moved them
+    bytes_launch = bytes_step * B * args.steps / max(n[1], 1)
     achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,

@@ -115,6 +115,12 @@ typedef struct hops_batch {
   double *psi_store_d; /* complex128 [B][store_stride][n_state] or NULL */
   const int32_t *tile_d; /* [n_tiles] group*65536+tile, or -1 idle      */
   int32_t n_tiles, tile_rows;
+  /* stage coefficients in stage precision, trajectory-fast: complex R
+     [(2 n_pat + n_mode + 1) rows][coefT_stride >= B]: rows A, A0, zc, nc;
+     sigT_d: double [coefT_stride] scale of Phi in the current step */
+  void *coefT_d;
+  double *sigT_d;
+  int32_t coefT_stride, pad2;
 } hops_batch_t;
 
 const char *hops_last_error(void);

@@ -297,7 +297,10 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
         dst[2 * s + 1] = sigma * v.y;
       }
     }
-    if (tid == 0) coef[L.sig] = sigma;
+    if (tid == 0) {
+      coef[L.sig] = sigma;
+      bt.sigT_d[b] = sigma;
+    }
   } else {
     sigma = coef[L.sig];
     const cx<R> *y = yin + ln.base;
@@ -449,7 +452,10 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
     }
   }
 
-  // 6. system-block coefficients, Z2 coefficients
+  // 6. system-block coefficients, Z2 coefficients (FP64 per trajectory for
+  //    this kernel, and in stage precision trajectory-fast for k_stage)
+  cx<R> *cT = reinterpret_cast<cx<R> *>(bt.coefT_d);
+  const long cs = bt.coefT_stride;
   for (int p = tid; p < op.n_pat; p += nt) {
     zd a = ld2(op.pat_h_d, p), a0 = a;
     for (int e = op.pc_ptr_d[p]; e < op.pc_ptr_d[p + 1]; ++e) {
@@ -473,6 +479,8 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
     coef[L.A + 2 * p + 1] = a.y;
     coef[L.A0 + 2 * p] = a0.x;
     coef[L.A0 + 2 * p + 1] = a0.y;
+    cT[(long)p * cs + b] = ccast<R>(a);
+    cT[(long)(op.n_pat + p) * cs + b] = ccast<R>(a0);
   }
   if (nonlin) {
     for (int m = tid; m < op.n_mode; m += nt) {
@@ -481,11 +489,13 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
       if (la >= 0) z = cmul(cdiv(ld2(op.mode_g_d, m), ld2(op.mode_w_d, m)), cconj(ld2(avg, la)));
       coef[L.zc + 2 * m] = z.x;
       coef[L.zc + 2 * m + 1] = z.y;
+      cT[(long)(2 * op.n_pat + m) * cs + b] = ccast<R>(z);
     }
   }
   if (tid == 0) {
     coef[L.nc] = nc.x;
     coef[L.nc + 1] = nc.y;
+    cT[(long)(2 * op.n_pat + op.n_mode) * cs + b] = ccast<R>(nc);
   }
 }
 
@@ -503,17 +513,17 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
   const Lane ln = lane_of(bt, nr, b);
   const int T = ln.T;
   const cx<R> *Y = yin + ln.base;
-  const CoefLayout L = coef_layout(op);
-  const double *coef = bt.coef_d + (long)b * bt.coef_stride;
-  const double sigma = coef[L.sig];
+  const cx<R> *cT = reinterpret_cast<const cx<R> *>(bt.coefT_d) + b;
+  const long cs = bt.coefT_stride;
+  const double sigma = bt.sigT_d[b];
 
   const cx<R> yr = Y[r * T];
   cx<R> d = cmul(k0, yr);
   // system block: -iH + noise / LTC terms
-  const double *A = coef + (k == 0 ? L.A0 : L.A);
+  const cx<R> *A = cT + (k == 0 ? (long)op.n_pat * cs : 0);
   const long rowk = (long)k * n;
   for (int p = op.pat_ptr_d[s]; p < op.pat_ptr_d[s + 1]; ++p)
-    cfma(d, ccast<R>(ld2(A, p)), Y[(rowk + op.pat_col_d[p]) * T]);
+    cfma(d, A[p * cs], Y[(rowk + op.pat_col_d[p]) * T]);
   // hierarchy super-operator K2_kp1 + K2_km1
   if (WR > 0) {
 #pragma unroll
@@ -528,11 +538,11 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
     }
   }
   if (bt.eom != HOPS_EOM_LINEAR) {
-    const double *zc = coef + L.zc;
+    const cx<R> *zc = cT + 2L * op.n_pat * cs;
     for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
-      cfma(d, ccast<R>(ld2(zc, op.up_mode_d[u])), Y[((long)op.up_aux_d[u] * n + s) * T]);
+      cfma(d, zc[(long)op.up_mode_d[u] * cs], Y[((long)op.up_aux_d[u] * n + s) * T]);
     if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
-      const cx<R> nc = mk<R>((R)coef[L.nc], (R)coef[L.nc + 1]);
+      const cx<R> nc = cT[(2L * op.n_pat + op.n_mode) * cs];
       d = csub(d, cmul(nc, yr));
     }
   }
@@ -607,8 +617,8 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
   const Lane ln = lane_of(bt, nr, b);
   const int T = ln.T;
   const cx<R> *Y = yin + ln.base;
-  const CoefLayout L = coef_layout(op);
-  const double *coef = bt.coef_d + (long)b * bt.coef_stride;
+  const cx<R> *cT = reinterpret_cast<const cx<R> *>(bt.coefT_d) + b;
+  const long cs = bt.coefT_stride;
   const long rowk = (long)k * n;
   const long gi = ln.base + r * T;
   cx<R> *acc = reinterpret_cast<cx<R> *>(bt.acc_d);
@@ -616,17 +626,16 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
   const cx<R> zero = mk<R>((R)0, (R)0);
   // loads
   const cx<R> yr = Y[r * T];
-  cx<R> yp[PW], ye[WR];
-  double2 ap[PW];
-  const double2 *A = reinterpret_cast<const double2 *>(coef + (k == 0 ? L.A0 : L.A));
+  cx<R> yp[PW], ye[WR], ap[PW];
+  const cx<R> *A = cT + (k == 0 ? (long)op.n_pat * cs : 0);
 #pragma unroll
   for (int p = 0; p < PW; ++p) {
     yp[p] = pidx[p] >= 0 ? Y[(rowk + pcol[p]) * T] : zero;
-    ap[p] = pidx[p] >= 0 ? A[pidx[p]] : make_double2(0.0, 0.0);
+    ap[p] = pidx[p] >= 0 ? A[(long)pidx[p] * cs] : zero;
   }
 #pragma unroll
   for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? Y[(long)ecol[e] * T] : zero;
-  const double sigma = coef[L.sig];
+  const double sigma = bt.sigT_d[b];
   cx<R> base = zero, a_in = zero;
   double2 ph = make_double2(0.0, 0.0);
   if (stage < 3) {
@@ -643,15 +652,15 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
   // arithmetic
   cx<R> d = cmul(k0, yr);
 #pragma unroll
-  for (int p = 0; p < PW; ++p) cfma(d, mk<R>((R)ap[p].x, (R)ap[p].y), yp[p]);
+  for (int p = 0; p < PW; ++p) cfma(d, ap[p], yp[p]);
 #pragma unroll
   for (int e = 0; e < WR; ++e) cfma(d, ev[e], ye[e]);
   if (bt.eom != HOPS_EOM_LINEAR) {
-    const double *zc = coef + L.zc;
+    const cx<R> *zc = cT + 2L * op.n_pat * cs;
     for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
-      cfma(d, ccast<R>(ld2(zc, op.up_mode_d[u])), Y[((long)op.up_aux_d[u] * n + (r - rowk)) * T]);
+      cfma(d, zc[(long)op.up_mode_d[u] * cs], Y[((long)op.up_aux_d[u] * n + (r - rowk)) * T]);
     if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
-      const cx<R> nc = mk<R>((R)coef[L.nc], (R)coef[L.nc + 1]);
+      const cx<R> nc = cT[(2L * op.n_pat + op.n_mode) * cs];
       d = csub(d, cmul(nc, yr));
     }
   }
@@ -917,6 +926,8 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     sb.zmem_d = bt->zmem_d + (size_t)b0 * 2 * op->n_mode_abs;
     sb.ws_d = bt->ws_d + (size_t)b0 * bt->ws_stride;
     sb.coef_d = bt->coef_d + (size_t)b0 * bt->coef_stride;
+    sb.coefT_d = (char *)bt->coefT_d + (size_t)b0 * es * 1;
+    sb.sigT_d = bt->sigT_d + b0;
     if (bt->psi_store_d) sb.psi_store_d = bt->psi_store_d + (size_t)b0 * bt->store_stride * 2 * op->n_state;
     return sb;
   };
@@ -1018,6 +1029,7 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_s
   const CoefLayout L = coef_layout(*op);
   if (bt->coef_stride < L.sig + 1) return fail("coef_stride %d < %d", bt->coef_stride, L.sig + 1);
   if (bt->ws_stride < 4 * op->n_mode_abs + 1) return fail("ws_stride too small");
+  if (!bt->coefT_d || !bt->sigT_d || bt->coefT_stride < bt->B) return fail("stage coefficient table missing or too small");
   if (n_steps < 0) return fail("n_steps < 0");
   hipStream_t st = (hipStream_t)stream;
   if (bt->precision == HOPS_PREC_FP64) return propagate_t<double>(op, bt, it0, n_steps, store0, st);

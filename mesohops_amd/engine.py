@@ -150,6 +150,10 @@ class BatchState:
         self.coef_stride = 4 * t.n_pat + 2 * t.n_mode + 4
         self.ws = torch.zeros(B * self.ws_stride, dtype=torch.float64, device=dev)
         self.coef = torch.zeros(B * self.coef_stride, dtype=torch.float64, device=dev)
+        self.coefT_stride = ((B + 63) // 64) * 64
+        rowsT = 2 * t.n_pat + t.n_mode + 1
+        self.coefT = torch.zeros(rowsT * self.coefT_stride, dtype=cR, device=dev)
+        self.sigT = torch.zeros(self.coefT_stride, dtype=torch.float64, device=dev)
         self.store_slots = store_slots
         self.psi_store = (torch.zeros(B * store_slots * t.n_state, dtype=torch.complex128, device=dev)
                           if store_slots > 0 else None)
@@ -177,6 +181,9 @@ class BatchState:
         bt.tile_d = self.tiles.data_ptr()
         bt.n_tiles = int(self.tiles.numel())
         bt.tile_rows = TILE_ROWS
+        bt.coefT_d = self.coefT.data_ptr()
+        bt.sigT_d = self.sigT.data_ptr()
+        bt.coefT_stride = self.coefT_stride
         self.bt = bt
 
     def _to_pool(self, a):
