@@ -645,3 +645,399 @@ class Trajectory:
     @property
     def psi(self):
         return self.phi[:self.basis.n_state]
+
+
+# ---------------------------------------------------------------------------
+# Adaptive basis (basis/hops_basis.py:117-292, :294-998, :1000-1151;
+# basis/basis_functions_adaptive.py; basis/basis_functions.py;
+# basis/hops_fluxfilters.py).  Restated for L-operators without off-diagonal
+# entries (site projectors), where the noise/LTC state-flux terms vanish
+# (hops_basis.py:914-998 return empty / None) and no static filters.
+# ---------------------------------------------------------------------------
+def determine_error_thresh(sorted_error, max_error, offset=0.0):
+    """basis/basis_functions.py:8-41."""
+    thresh = 0.0
+    if len(sorted_error) > 0:
+        idx = np.argmax(np.cumsum(sorted_error) + offset >= max_error)
+        if idx > 0:
+            thresh = sorted_error[idx - 1]
+        elif sorted_error[0] + offset < max_error:
+            thresh = np.inf
+    return thresh
+
+
+def basis_from_list(error, max_error, members):
+    """hops_basis.py:1000-1028."""
+    thresh = determine_error_thresh(np.sort(error), max_error)
+    idx = np.where(error > thresh)[0]
+    return idx, [members[i] for i in idx]
+
+
+def _aux_depth(aux):
+    return sum(v for (m, v) in aux)
+
+
+class AdaptiveState:
+    """The pieces of HopsBasis used by define_basis/update_basis for one
+    trajectory, built on Basis (which is rebuilt for every new basis, the
+    invariant of eom/eom_hops_ksuper.py:387-572)."""
+
+    def __init__(self, sysd, eom_type, maxhier, delta_a, delta_s, f_discard, markov=None):
+        self.markov = None if markov is None else np.array(markov, bool)
+        self.sysd = sysd
+        self.eom_type = eom_type
+        self.maxhier = maxhier
+        self.delta_a = delta_a
+        self.delta_s = delta_s
+        self.f_discard = f_discard
+        self.adaptive_h = delta_a > 0
+        self.adaptive_s = delta_s > 0
+        for L in sysd["LIST_L2_COO"]:
+            if np.any(L.row != L.col):
+                raise NotImplementedError("adaptive basis with off-diagonal L-operators")
+
+    # -- per-basis arrays (hops_basis.py:1030-1151) --
+    def tables(self, b):
+        nm, n, nh = len(b.modes), b.n_state, b.n_hier
+        M2 = np.zeros((nm, n), complex)            # L_m[s, s]
+        for i in range(nm):
+            L = b.L_coo[b.relL_by_mode[i]]
+            for r, c, d in zip(L.row, L.col, L.data):
+                if r == c:
+                    M2[i, c] += d
+        psi = None
+        return M2
+
+    def expectations(self, b, psi):
+        return np.array([expectation(L, psi) for L in b.L_csr])
+
+    def K2_aux_by_mode(self, b):
+        K = np.zeros((len(b.modes), b.n_hier), dtype=np.int64)
+        for j, a in enumerate(b.aux_list):
+            for (m, v) in a:
+                if m in b.rel_mode:
+                    K[b.rel_mode[m], j] = v
+        return K
+
+    def markov_up(self, b):
+        """No flux up along Markovian-filtered modes except from the physical
+        wave function, and none out of their first-level auxiliaries
+        (hops_fluxfilters.py:354-397)."""
+        nm, nh = len(b.modes), b.n_hier
+        F = np.ones((nm, nh), bool)
+        if self.markov is None:
+            return F
+        mask = self.markov[b.modes]
+        F[mask, 1:] = False
+        for lm, m in enumerate(b.modes):
+            if mask[lm]:
+                j = b.aux_index.get(((int(m), 1),))
+                if j is not None:
+                    F[:, j] = False
+        return F
+
+    def list_sc(self, b):
+        """States outside the basis reachable by L or H (hops_system.py:191-204)."""
+        st = set(int(s) for s in b.state_list)
+        dest = set()
+        for s in st:
+            dest |= set(self.sysd["LIST_DESTINATION_STATES_BY_STATE_INDEX"][s])
+        coupled = set()
+        for s in st:
+            coupled |= set(self.sysd["COUPLED_STATES"][s])
+        return sorted((dest - st) | (coupled - st))
+
+    def error_deriv(self, dsys, b, Phi, z_step, dt, stable_idx=None):
+        """basis_functions_adaptive.py:10-74 (returns |.|^2 as [n, n_hier])."""
+        n, nh = b.n_state, b.n_hier
+        if stable_idx is not None:
+            P = np.zeros(n * nh, complex)
+            Pv = P.reshape(nh, n)
+            Pv[stable_idx] = Phi.reshape(nh, n)[stable_idx]
+        else:
+            P = Phi
+        d = dsys(P, z_step[2], z_step[0], z_step[1])[0] / HBAR
+        d = d + P / dt
+        D = d.reshape(nh, n).T
+        if stable_idx is not None:
+            D = D[:, stable_idx]
+        return np.abs(D) ** 2
+
+    def hier_stable_error(self, dsys, b, Phi, dt, z_step):
+        """hops_basis.py:560-665."""
+        n, nh, nm = b.n_state, b.n_hier, len(b.modes)
+        C2 = Phi.reshape(nh, n).T          # [n, n_hier]
+        E1 = np.sum(self.error_deriv(dsys, b, Phi, z_step, dt), axis=0)
+        # state flux out of the basis through H (basis_functions_adaptive.py:76-163)
+        sc = self.list_sc(b)
+        if len(sc):
+            Hsc = -1j * self.sysd["H_DENSE"][np.ix_(sc, b.state_list)]
+            E1 = E1 + np.sum(np.abs(Hsc @ C2 / HBAR) ** 2, axis=0)
+        M2 = self.tables(b)
+        K2 = self.K2_aux_by_mode(b)
+        w = np.abs(b.w)
+        KW = w.reshape(nm, 1) * (1 + K2)
+        E_up = np.abs(KW) ** 2 * ((np.abs(M2) ** 2) @ (np.abs(C2) ** 2)) / HBAR ** 2
+        F_up = self.markov_up(b)
+        for j, a in enumerate(b.aux_list):
+            if _aux_depth(a) == self.maxhier:
+                F_up[:, j] = False
+        E1 = E1 + np.sum(F_up * E_up, axis=0)
+        avg = self.expectations(b, Phi[:n])
+        X2 = np.repeat(avg[b.relL_by_mode].reshape(nm, 1), n, axis=1)
+        gw2 = np.square(np.abs(b.g / b.w)).reshape(nm, 1)
+        E_down = gw2 * ((np.abs(M2 - X2) ** 2) @ (np.abs(C2) ** 2)) / HBAR ** 2
+        F_down = K2 > 0
+        E1 = E1 + np.sum(F_down * E_down, axis=0)
+        return E1, [E_up, E_down]
+
+    def boundary_hier(self, b, E_updown, stable_idx, dbound_sq):
+        """hops_basis.py:667-793."""
+        nm, nh = len(b.modes), b.n_hier
+        E_up, E_down = E_updown[0].copy(), E_updown[1].copy()
+        F_up = np.ones((nm, nh), bool)
+        F_down = np.ones((nm, nh), bool)
+        for j, a in enumerate(b.aux_list):
+            d = dict(a)
+            if _aux_depth(a) < self.maxhier:
+                for lm, m in enumerate(b.modes):
+                    if aux_step(a, int(m), 1) in b.aux_index:
+                        F_up[lm, j] = False
+            else:
+                F_up[:, j] = False
+            for lm, m in enumerate(b.modes):
+                m = int(m)
+                if m not in d or aux_step(a, m, -1) in b.aux_index:
+                    F_down[lm, j] = False
+        F_up &= self.markov_up(b)
+        E_up = (E_up * F_up)[:, stable_idx]
+        E_down = (E_down * F_down)[:, stable_idx]
+        srt = np.sort(np.append(E_up[E_up != 0], E_down[E_down != 0]))
+        thr = determine_error_thresh(srt, dbound_sq * self.f_discard * self.f_discard)
+        discarded = np.sum(E_up[E_up <= thr]) + np.sum(E_down[E_down <= thr])
+        E_up[E_up <= thr] = 0
+        E_down[E_down <= thr] = 0
+        acc = {}
+        for col, j in enumerate(stable_idx):
+            a = b.aux_list[j]
+            for lm in np.nonzero(E_up[:, col])[0]:
+                key = aux_step(a, int(b.modes[lm]), 1)
+                acc[key] = acc.get(key, 0.0) + E_up[lm, col]
+            d = dict(a)
+            for lm in np.nonzero(E_down[:, col])[0]:
+                m = int(b.modes[lm])
+                if m in d:
+                    key = aux_step(a, m, -1)
+                    acc[key] = acc.get(key, 0.0) + E_down[lm, col]
+        thr = determine_error_thresh(np.sort(np.array(list(acc.values()))), dbound_sq, offset=discarded)
+        return [k for k, v in acc.items() if v > thr]
+
+    def state_stable_error(self, dsys, b, Phi, dt, z_step, stable_idx, aux_bound):
+        """hops_basis.py:795-912."""
+        n, nh, nm = b.n_state, b.n_hier, len(b.modes)
+        E1 = np.sum(self.error_deriv(dsys, b, Phi, z_step, dt, stable_idx), axis=1)
+        C2 = Phi.reshape(nh, n).T
+        # state flux through H couplings (basis_functions_adaptive.py:893-995)
+        H = -1j * self.sysd["H_DENSE"]
+        Hc = H - np.diag(np.diag(H))
+        V1 = np.sum(np.abs(Hc[:, b.state_list]) ** 2, axis=0)
+        C1 = np.sum(np.abs(C2[:, stable_idx]) ** 2, axis=1)
+        E1 = E1 + V1 * C1 / HBAR ** 2
+        M2 = self.tables(b)
+        avg = self.expectations(b, Phi[:n])
+        X2 = np.repeat(avg[b.relL_by_mode].reshape(nm, 1), n, axis=1)
+        # filters from the boundary auxiliaries (hops_fluxfilters.py:292-352)
+        F_dn = np.zeros((nm, nh), bool)
+        F_up = np.zeros((nm, nh), bool)
+        for a in aux_bound:
+            for lm, m in enumerate(b.modes):
+                j = b.aux_index.get(aux_step(a, int(m), 1))
+                if j is not None:
+                    F_dn[lm, j] = True
+            for (m, v) in a:
+                j = b.aux_index.get(aux_step(a, m, -1))
+                if j is not None:
+                    F_up[b.rel_mode[m], j] = True
+        gw2 = np.square(np.abs(b.g / b.w)).reshape(nm, 1)
+        E_down = ((np.abs(M2 - X2) ** 2).T @ (F_dn * gw2)) * np.abs(C2) ** 2 / HBAR ** 2
+        E1 = E1 + np.sum(E_down[:, stable_idx], axis=1)
+        K2 = self.K2_aux_by_mode(b)
+        KW = F_up * (np.abs(b.w).reshape(nm, 1) * (1 + K2))
+        E_up = ((np.abs(M2) ** 2).T @ (np.abs(KW) ** 2)) * np.abs(C2) ** 2 / HBAR ** 2
+        E1 = E1 + np.sum(E_up[:, stable_idx], axis=1)
+        return E1
+
+    def state_basis(self, dsys, b, Phi, dt, z_step, stable_idx, aux_bound):
+        """hops_basis.py:294-482 (no off-diagonal couplings)."""
+        n, nh = b.n_state, b.n_hier
+        dsq = self.delta_s ** 2
+        err = self.state_stable_error(dsys, b, Phi, dt, z_step, stable_idx, aux_bound)
+        rel_stable, st_stable = basis_from_list(err, dsq / 2.0, list(b.state_list))
+        unstable = np.setdiff1d(np.arange(n), rel_stable)
+        dbound = dsq - np.sum(err[unstable])
+        sc = self.list_sc(b)
+        bound = []
+        if len(rel_stable) < self.sysd["NSTATES"] and len(sc):
+            C2 = Phi.reshape(nh, n).T[np.ix_(rel_stable, stable_idx)]
+            Hsc = -1j * self.sysd["H_DENSE"][np.ix_(sc, st_stable)]
+            E = np.sum(np.abs(Hsc @ (C2 / HBAR)) ** 2, axis=1)
+            nz = np.nonzero(E)[0]
+            if len(nz):
+                _, bound = basis_from_list(E[nz], dbound, list(np.array(sc)[nz]))
+        bound = sorted(set(bound) - set(int(s) for s in b.state_list))
+        return [int(s) for s in st_stable], bound
+
+    def define_basis(self, dsys, b, Phi, dt, z_step):
+        """hops_basis.py:117-182."""
+        n = b.n_state
+        Pn = Phi / np.linalg.norm(Phi[:n])
+        if self.adaptive_h:
+            dsq = self.delta_a ** 2
+            err, E_updown = self.hier_stable_error(dsys, b, Pn, dt, z_step)
+            idx_stable, aux_stable = basis_from_list(err, dsq / 2.0, b.aux_list)
+            unstable = np.setdiff1d(np.arange(b.n_hier), idx_stable)
+            dbound = dsq - np.sum(err[unstable])
+            aux_bound = self.boundary_hier(b, E_updown, idx_stable, dbound)
+            aux_new = list(set(aux_stable) | set(aux_bound))
+            stable_idx = sorted(b.aux_index[a] for a in aux_stable)
+        else:
+            aux_new = list(b.aux_list)
+            aux_bound = []
+            stable_idx = list(range(b.n_hier))
+        if self.adaptive_s:
+            st, sb = self.state_basis(dsys, b, Pn, dt, z_step, stable_idx, aux_bound)
+            states_new = sorted(set(st) | set(sb))
+        else:
+            states_new = [int(s) for s in b.state_list]
+        return states_new, aux_new
+
+
+def permute_phi(b_old, b_new, Phi):
+    """Copy the entries of the stable (aux, state) pairs into the new basis and
+    rescale to the old physical norm (hops_basis.py:214-274)."""
+    n_old, n_new = b_old.n_state, b_new.n_state
+    old_pos = {int(s): i for i, s in enumerate(b_old.state_list)}
+    new_pos = {int(s): i for i, s in enumerate(b_new.state_list)}
+    stable_states = sorted(set(old_pos) & set(new_pos))
+    out = np.zeros(b_new.n_hier * n_new, complex)
+    for a in b_old.aux_list:
+        j = b_new.aux_index.get(a)
+        if j is None:
+            continue
+        i = b_old.aux_index[a]
+        for s in stable_states:
+            out[j * n_new + new_pos[s]] = Phi[i * n_old + old_pos[s]]
+    norm_old = np.linalg.norm(Phi[:n_old])
+    return norm_old * out / np.linalg.norm(out[:n_new])
+
+
+class AdaptiveTrajectory(Trajectory):
+    """adHOPS trajectory (trajectory/hops_trajectory.py:253-336, :338-399,
+    :456-533, :636-714) for site-projector L-operators."""
+
+    def __init__(self, *args, delta_a=1e-4, delta_s=1e-4, update_step=1, f_discard=0.01,
+                 early_steps=5, inchworm_cap=5, markov=None, **kw):
+        super().__init__(*args, **kw)
+        self.ad = AdaptiveState(self.sysd, self.eom_type, self.maxhier, delta_a, delta_s, f_discard,
+                                markov=markov)
+        self.update_step = update_step
+        self.early_steps = early_steps
+        self.inchworm_cap = inchworm_cap
+        self.early_counter = 0
+        self.state_traj = []
+        self.aux_traj = []
+
+    def _basis(self, states, aux):
+        b = Basis(self.sysd, states, aux)
+        return b, make_derivative(b, self.eom_type)
+
+    def z_step(self, z_mem):
+        lops = list(self.basis.absL)
+        z1 = self.noise1.get_noise([self.t], lops)[:, 0]
+        z2 = self.noise2.get_noise([self.t], lops)[:, 0]
+        return [z1, z2, z_mem]
+
+    def update_basis(self, phi, states, aux):
+        b_old = self.basis
+        if set(states) == set(int(s) for s in b_old.state_list) and set(aux) == set(b_old.aux_list):
+            return phi
+        b_new, d = self._basis(states, aux)
+        phi = permute_phi(b_old, b_new, phi)
+        self.basis, self.dsystem_dt = b_new, d
+        return phi
+
+    def initialize(self, psi_0):
+        psi_0 = np.asarray(psi_0, dtype=np.complex128)
+        states = [int(s) for s in np.where(np.abs(psi_0) > 0)[0]]
+        if not self.ad.adaptive_s:
+            states = list(range(self.sysd["NSTATES"]))
+        if self.ad.adaptive_h:
+            aux = [()]
+        else:
+            aux = triangular_hierarchy(self.sysd["N_HMODES"], self.maxhier)
+            if self.ad.markov is not None:
+                # Markovian static filter (basis/hierarchy_functions.py:95-140)
+                mk = self.ad.markov
+                aux = [a for a in aux if sum(v for _, v in a) <= 1 or not any(mk[m] for m, _ in a)]
+        self.basis, self.dsystem_dt = self._basis(states, aux)
+        b = self.basis
+        phi = np.zeros(b.n_hier * b.n_state, complex)
+        phi[:b.n_state] = psi_0[b.state_list]
+        self.z_mem = np.zeros(len(self.sysd["L_NOISE1"]), complex)
+        self.t = 0.0
+        st, ax = self.ad.define_basis(self.dsystem_dt, b, phi, 1, self.z_step(self.z_mem))
+        self.phi = self.update_basis(phi, st, ax)
+        self._store(0.0)
+
+    def _store(self, t):
+        b = self.basis
+        full = np.zeros(self.sysd["NSTATES"], complex)
+        full[b.state_list] = self.phi[:b.n_state]
+        self.psi_traj.append(full)
+        self.t_axis.append(t)
+        self.z_mem_traj.append(self.z_mem.copy())
+        self.state_traj.append(list(int(s) for s in b.state_list))
+        self.aux_traj.append(list(b.aux_list))
+
+    def _rk(self, phi, z_mem, tau):
+        z1, z2 = self.noise_block(self.t, tau)
+        phi, z_mem = rk4_step(self.dsystem_dt, phi, z_mem, z1, z2, tau)
+        if self.eom_type == "NORMALIZED NONLINEAR":
+            phi = phi / np.sqrt(np.sum(np.abs(phi[:self.basis.n_state]) ** 2))
+        return phi, z_mem
+
+    def propagate(self, t_advance, tau):
+        t0 = self.t
+        t_axis = t0 + np.arange(1, 1 + int(np.ceil(t_advance / tau))) * tau
+        for index_t, t in enumerate(t_axis):
+            phi, z_mem = self._rk(self.phi, self.z_mem, tau)
+            if self.early_counter < self.early_steps:
+                st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
+                k = 0
+                while (set(st) != set(int(s) for s in self.basis.state_list)
+                       or set(ax) != set(self.basis.aux_list)):
+                    # inchworm: union basis, redo the step from the old phi
+                    st_u = sorted(set(st) | set(int(s) for s in self.basis.state_list))
+                    ax_u = list(set(ax) | set(self.basis.aux_list))
+                    self.phi = self.update_basis(self.phi, st_u, ax_u)
+                    phi, z_mem = self._rk(self.phi, self.z_mem, tau)
+                    st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
+                    k += 1
+                    if k >= self.inchworm_cap:
+                        break
+                phi = self.update_basis(phi, st, ax)
+                self.early_counter += 1
+            elif (index_t + 1) % self.update_step == 0:
+                st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
+                phi = self.update_basis(phi, st, ax)
+            self.phi = phi
+            b = self.basis
+            full = np.zeros(self.sysd["NSTATES"], complex)
+            full[b.state_list] = phi[:b.n_state]
+            self.psi_traj.append(full)
+            self.t_axis.append(float(t))
+            self.z_mem_traj.append(self.z_mem.copy())
+            self.state_traj.append(list(int(s) for s in b.state_list))
+            self.aux_traj.append(list(b.aux_list))
+            self.z_mem = z_mem
+            self.t = float(t)

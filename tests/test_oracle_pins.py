@@ -221,3 +221,81 @@ def test_noise_correlation_reconstructs_alpha():
     ct /= n_traj * len(t_axis)
     cf = g1 * np.exp(-t_axis * w1 / O.HBAR) + g2 * np.exp(-t_axis * w2 / O.HBAR)
     assert np.allclose(ct[:10], cf[:10], rtol=2e-2)
+
+
+# ---------------------------------------------------------------------------
+# Adaptive basis pins (tests/test_dimer_of_dimers.py:263-305,
+# tests/test_adap_basis.py:110-361)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("da,ds", [(1e-15, 0), (0, 1e-100), (1e-15, 1e-100)])
+def test_adaptive_delta_to_zero_equals_full(da, ds):
+    """With vanishing thresholds the adaptive trajectory equals the plain one."""
+    sysp = systems.dimer_of_dimers()
+    noise = dict(SEED=0, MODEL="FFT_FILTER", TLEN=400.0, TAU=1.0, RAND_MODEL="BOX_MULLER")
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(4, complex)
+    psi0[2] = 1
+    ref = O.Trajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2})
+    ref.initialize(psi0)
+    ref.propagate(200.0, 4.0)
+    tr = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2}, delta_a=da, delta_s=ds)
+    tr.initialize(psi0)
+    tr.propagate(200.0, 4.0)
+    np.testing.assert_allclose(np.array(tr.psi_traj)[33], np.array(ref.psi_traj)[33])
+    assert np.max(np.abs(np.array(tr.psi_traj) - np.array(ref.psi_traj))) < 1e-12
+
+
+def _embed(b_small, b_full, phi):
+    out = np.zeros(b_full.n_hier * b_full.n_state, complex)
+    pos = {int(s): i for i, s in enumerate(b_full.state_list)}
+    for i, a in enumerate(b_small.aux_list):
+        j = b_full.aux_index[a]
+        for si, s in enumerate(b_small.state_list):
+            out[j * b_full.n_state + pos[int(s)]] = phi[i * b_small.n_state + si]
+    return out
+
+
+@pytest.mark.parametrize("mode", ["hier", "state"])
+def test_adaptive_derivative_error_below_delta(mode):
+    """|D_full(phi_adaptive) - D_adaptive(phi_adaptive)| <= delta at every step,
+    for the chains of tests/test_adap_basis.py:110-361 (Markovian filter on
+    the fast mode of every site, seed 100100)."""
+    if mode == "hier":
+        nsite, V, lam, maxhier, t_max = 3, 40.0, 40.0, 4, 200.0
+        da, ds = 5e-4, 0
+    else:
+        nsite, V, lam, maxhier, t_max = 10, 20.0, 60.0, 2, 100.0
+        da, ds = 0, 5e-4
+    g0, w0 = O.bcf_convert_dl_to_exp(lam, 50.0, 140.0)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        gw += [[g0, w0], [-1j * np.imag(g0), 500.0]]
+        ll += [L, L]
+    markov = [False, True] * nsite
+    H = (np.eye(nsite, k=1) + np.eye(nsite, k=-1)) * V
+    sysp = dict(HAMILTONIAN=np.array(H, complex), GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp,
+                PARAM_NOISE1=gw)
+    noise = dict(SEED=100100, MODEL="FFT_FILTER", TLEN=2100.0, TAU=1.0)
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(nsite, complex)
+    psi0[int(np.round(nsite / 2))] = 1.0
+    aux_full = [a for a in O.triangular_hierarchy(2 * nsite, maxhier)
+                if sum(v for _, v in a) <= 1 or not any(markov[m] for m, _ in a)]
+    full = O.Basis(O.derive_system(sysp), np.arange(nsite), aux_full)
+    f_full = O.make_derivative(full, "NORMALIZED NONLINEAR")
+    tr = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": maxhier}, delta_a=da, delta_s=ds,
+                              markov=markov)
+    tr.initialize(psi0)
+    errs = []
+    for t in np.arange(0, t_max, 4.0):
+        tr.propagate(4.0, 4.0)
+        b = tr.basis
+        lops = list(b.absL)
+        z1 = tr.noise1.get_noise([t], lops)[:, 0]
+        d_ad = tr.dsystem_dt(tr.phi, tr.z_mem, z1, np.zeros(len(lops)))[0] / O.HBAR
+        z1f = tr.noise1.get_noise([t], list(full.absL))[:, 0]
+        d_full = f_full(_embed(b, full, tr.phi), tr.z_mem, z1f, np.zeros(len(full.absL)))[0] / O.HBAR
+        errs.append(np.linalg.norm(d_full - _embed(b, full, d_ad)))
+    assert b.n_hier < full.n_hier or b.n_state < full.n_state
+    assert max(errs) <= max(da, ds), max(errs)
