@@ -152,6 +152,20 @@ int hops_build_super(const hops_op_t *op, int precision, void *stream);
 int hops_propagate(const hops_op_t *op, const hops_batch_t *batch, int it0,
                    int n_steps, int store0, void *stream);
 
+/* hbar dY/dt for every trajectory of the batch at the input pool y_d
+   (complex128, same layout as phi_d), with the batch's current z-memory and
+   the noise at index tidx; output d_d (complex128, same layout).  FP64
+   batches only.  Replaces the extra dsystem_dt calls of the adaptive error
+   estimate (basis/basis_functions_adaptive.py:10-74). */
+int hops_derivative(const hops_op_t *op, const hops_batch_t *bt, const void *y_d, void *d_d, int tidx,
+                    void *stream);
+
+/* dst[dst_idx[i]] = scale * src[src_idx[i]] for i < n (complex128): moves
+   the stable entries of the hierarchy vector into a new basis and rescales
+   them (basis/hops_basis.py:214-274). */
+int hops_permute_c128(const void *src_d, void *dst_d, const int32_t *src_idx_d, const int32_t *dst_idx_d,
+                      int n, double scale, void *stream);
+
 /* Circulant-embedding spectrum s_w[l][j] = max(0, Re FFT(embed(tilde c_l)))
    for n_lop correlation functions of n_t points (FFT length 2(n_t-1)).
    ang_d, mag_d: float32 [n_lop][n_t] phase and modulus of the complex64

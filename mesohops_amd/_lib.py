@@ -69,6 +69,8 @@ def lib():
     L.hops_build_super.argtypes = [ctypes.POINTER(HopsOp), ctypes.c_int, _vp]
     L.hops_propagate.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, _vp]
+    L.hops_derivative.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), _vp, _vp, ctypes.c_int, _vp]
+    L.hops_permute_c128.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp]
     L.hops_noise_embedding.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
     L.hops_profile_enable.argtypes = [ctypes.c_int]
@@ -86,7 +88,8 @@ def check(rc):
 
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
-            "hops_build_super", "hops_propagate", "hops_noise_embedding", "hops_noise_filter"]
+            "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128",
+            "hops_noise_embedding", "hops_noise_filter"]
 
 
 def set_option(name, value):

@@ -255,6 +255,9 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops_batch_t bt,
                                                   int stage, int tidx, const cx<R> *__restrict__ yin,
                                                   int store_slot) {
+  // stage < 0: evaluate the derivative at yin with the current z-memory and
+  // no RK bookkeeping (used by the adaptive error estimate)
+  const bool eval = stage < 0;
   extern __shared__ double sm[];
   const int n = op.n_state, na = op.n_act, Ma = op.n_mode_abs;
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -302,7 +305,11 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
       bt.sigT_d[b] = sigma;
     }
   } else {
-    sigma = coef[L.sig];
+    sigma = eval ? 1.0 : coef[L.sig];
+    if (eval && tid == 0) {
+      coef[L.sig] = 1.0;
+      bt.sigT_d[b] = 1.0;
+    }
     const cx<R> *y = yin + ln.base;
     for (int s = tid; s < n; s += nt) {
       psi[2 * s] = (double)y[(long)s * T].x;
@@ -362,6 +369,7 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
       d = mk<double>(0.0, 0.0);
       d = csub(d, cmul(cconj(w), z));
     }
+    if (eval) continue;
     kz[2 * m] = d.x;
     kz[2 * m + 1] = d.y;
     if (stage == 0) {
@@ -549,6 +557,10 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
   if (stage == 0) d = cscale(d, (R)sigma);
 
   const long gi = ln.base + r * T;
+  if (stage < 0) {
+    yout[gi] = d;
+    return;
+  }
   const double h = bt.dt_over_hbar;
   cx<R> *acc = reinterpret_cast<cx<R> *>(bt.acc_d);
   double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
@@ -737,6 +749,17 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
     const int b = (g << bt.lane_shift) + lane;
     if (b < bt.B) stage_row_reg<R, WR, PW>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
   }
+}
+
+// dst[dst_idx[i]] = scale * src[src_idx[i]]: moves the stable (auxiliary,
+// state) entries of Phi into a new basis (basis/hops_basis.py:214-274).
+__global__ __launch_bounds__(256) void k_permute(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                 const int32_t *__restrict__ si, const int32_t *__restrict__ di,
+                                                 int n, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 v = src[si[i]];
+  dst[di[i]] = make_double2(scale * v.x, scale * v.y);
 }
 
 // y0 = complex64(Phi) at the start of a mixed-precision propagate call.
@@ -1015,6 +1038,38 @@ int hops_build_super(const hops_op_t *op, int precision, void *stream) {
     HIP_TRY(hipGetLastError());
   }
   k_build_k0<<<(unsigned)((op->n_hier + 255) / 256), 256, 0, st>>>(*op);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_derivative(const hops_op_t *op, const hops_batch_t *bt, const void *y_d, void *d_d, int tidx,
+                    void *stream) {
+  if (int rc = check_op(op)) return rc;
+  if (!bt || !y_d || !d_d) return fail("hops_derivative: NULL argument");
+  if (bt->precision != HOPS_PREC_FP64) return fail("hops_derivative needs an FP64 batch");
+  if (tidx < 0 || tidx >= bt->n_t) return fail("noise index %d outside [0, %d)", tidx, bt->n_t);
+  if (!bt->coefT_d || !bt->sigT_d || bt->coefT_stride < bt->B) return fail("stage coefficient table missing");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = op->n_state;
+  const size_t shm = sizeof(double) * (2 * (size_t)n + 8 * (size_t)op->n_act + 2 * (size_t)op->n_mode_abs + 16);
+  if (shm > 160 * 1024) return fail("prologue LDS %zu B exceeds 160 KiB", shm);
+  if (shm > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_prologue<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  const cx<double> *y = reinterpret_cast<const cx<double> *>(y_d);
+  k_prologue<double><<<bt->B, 256, shm, st>>>(*op, *bt, -1, tidx, y, -1);
+  HIP_TRY(hipGetLastError());
+  k_stage<double><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, -1, y, reinterpret_cast<cx<double> *>(d_d));
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_permute_c128(const void *src_d, void *dst_d, const int32_t *src_idx_d, const int32_t *dst_idx_d,
+                      int n, double scale, void *stream) {
+  if (n <= 0) return 0;
+  if (!src_d || !dst_d || !src_idx_d || !dst_idx_d) return fail("hops_permute_c128: NULL argument");
+  k_permute<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      reinterpret_cast<const double2 *>(src_d), reinterpret_cast<double2 *>(dst_d), src_idx_d, dst_idx_d, n,
+      scale);
   HIP_TRY(hipGetLastError());
   return 0;
 }
