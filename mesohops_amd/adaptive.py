@@ -1,0 +1,433 @@
+"""Adaptive HOPS (adHOPS) on the device engine.
+
+Per RK4 step of one trajectory: the step runs on the device in the current
+basis; the error bound of every auxiliary and state is then assembled from two
+device derivative evaluations (hops_derivative: the full hierarchy, and the
+hierarchy restricted to the stable auxiliaries) and the flux terms; the host
+selects the stable and boundary sets (sorting and cumulative thresholds, the
+reference's set logic); the new basis gets a device-built super-operator and
+the hierarchy vector is moved into it on the device (hops_permute_c128).
+
+Follows basis/hops_basis.py:117-292 (define_basis / update_basis), :294-998
+(error bounds), basis/basis_functions_adaptive.py (error terms),
+basis/basis_functions.py:8-65 (thresholds), basis/hops_fluxfilters.py (flux
+filters) and trajectory/hops_trajectory.py:456-533, :636-714 (adaptive time
+loop and INCH_WORM early integrator).  L-operators with off-diagonal entries
+are not supported in adaptive mode (UnsupportedRequest).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .basis import BasisTables
+from .engine import BatchState, DeviceOperator, _stream_ptr
+from .errors import UnsupportedRequest
+from .params import hbar
+
+
+def error_threshold(sorted_err, max_err, offset=0.0):
+    """Largest error value that can be dropped with the cumulative dropped
+    error staying below max_err (basis/basis_functions.py:8-41)."""
+    if len(sorted_err) == 0:
+        return 0.0
+    cum = np.cumsum(sorted_err) + offset
+    i = int(np.argmax(cum >= max_err))
+    if i > 0:
+        return sorted_err[i - 1]
+    return np.inf if sorted_err[0] + offset < max_err else 0.0
+
+
+def keep_above(err, max_err, members):
+    """hops_basis.py:1000-1028."""
+    thr = error_threshold(np.sort(err), max_err)
+    idx = np.where(err > thr)[0]
+    return idx, [members[i] for i in idx]
+
+
+def _step_aux(aux, mode, step):
+    d = dict(aux)
+    v = d.get(mode, 0) + step
+    if v < 0:
+        return None
+    if v == 0:
+        d.pop(mode, None)
+    else:
+        d[mode] = v
+    return tuple((m, d[m]) for m in sorted(d))
+
+
+def _depth(aux):
+    return sum(v for _, v in aux)
+
+
+class BasisSelector:
+    """Host half of the adaptive basis update: from the current basis tables,
+    the normalised hierarchy vector and two derivative evaluations (a callable
+    returning hbar dPhi/dt for a vector in this basis), the new state and
+    auxiliary lists (basis/hops_basis.py:117-182, :294-998).  It touches no
+    device memory; the trajectory engine supplies the derivative."""
+
+    def __init__(self, info, delta_a, delta_s, maxhier, f_discard, markov=None):
+        for L in info.l_ops:
+            if np.any(L.row != L.col):
+                raise UnsupportedRequest("adaptive basis with off-diagonal L-operators", "make_adaptive")
+        self.info = info
+        self.delta_a, self.delta_s = float(delta_a), float(delta_s)
+        self.maxhier = int(maxhier)
+        self.f_discard = float(f_discard)
+        self.markov = None if markov is None else np.array(markov, bool)
+
+    def _bind(self, tb):
+        self.tb = tb
+        self.aux_index = {a: i for i, a in enumerate(tb.aux_list)}
+        self.rel_mode = {int(m): i for i, m in enumerate(tb.modes)}
+
+    # -- error terms ---------------------------------------------------------
+    def _mode_tables(self):
+        tb = self.tb
+        nm, n = tb.n_mode, tb.n_state
+        M2 = np.zeros((nm, n), complex)
+        for i in range(nm):
+            L = tb.L[tb.mode_lrel[i]].tocoo()
+            for r, c, v in zip(L.row, L.col, L.data):
+                if r == c:
+                    M2[i, c] += v
+        K2 = np.zeros((nm, tb.n_hier), dtype=np.int64)
+        for j, a in enumerate(tb.aux_list):
+            for (m, v) in a:
+                K2[self.rel_mode[m], j] = v
+        g = self.info.G[tb.modes]
+        w = self.info.W[tb.modes]
+        psi = self._pn[: n]
+        nrm = np.vdot(psi, psi)
+        avg = np.array([np.vdot(psi, tb.L[j] @ psi) / nrm for j in range(tb.n_lop)])
+        X2 = np.repeat(avg[tb.mode_lrel].reshape(nm, 1), n, axis=1)
+        return M2, K2, g, w, X2
+
+    def _markov_up(self):
+        tb = self.tb
+        F = np.ones((tb.n_mode, tb.n_hier), bool)
+        if self.markov is None:
+            return F
+        mask = self.markov[tb.modes]
+        F[mask, 1:] = False
+        for lm in np.nonzero(mask)[0]:
+            j = self.aux_index.get(((int(tb.modes[lm]), 1),))
+            if j is not None:
+                F[:, j] = False
+        return F
+
+    def _list_sc(self):
+        info, st = self.info, set(int(s) for s in self.tb.state_list)
+        out = set()
+        H = info.H
+        for s in st:
+            out |= set(int(x) for x in np.nonzero(H[s])[0])
+            for l in info.l_by_state[s]:
+                c = info.l_ops[l]
+                out |= set(int(x) for x in c.col[c.row == s])
+        return sorted(out - st)
+
+    def define_basis(self, tb, phi, dt, derivative):
+        """hops_basis.py:117-182: new state list and auxiliary list."""
+        self._bind(tb)
+        self._deriv = derivative
+        n, nh = tb.n_state, tb.n_hier
+        self._pn = phi / np.linalg.norm(phi[:n])
+        Pn = self._pn
+        C2 = Pn.reshape(nh, n).T
+        absC2 = np.abs(C2) ** 2
+        M2, K2, g, w, X2 = self._mode_tables()
+        sc = self._list_sc()
+        H = self.info.H
+        if self.delta_a > 0:
+            # hier_stable_error (hops_basis.py:560-665)
+            D = self._deriv(Pn) / hbar + Pn / dt
+            err = np.sum(np.abs(D.reshape(nh, n).T) ** 2, axis=0)
+            if sc:
+                Hsc = -1j * H[np.ix_(sc, tb.state_list)]
+                err = err + np.sum(np.abs(Hsc @ C2 / hbar) ** 2, axis=0)
+            KW = np.abs(w).reshape(-1, 1) * (1 + K2)
+            E_up = np.abs(KW) ** 2 * ((np.abs(M2) ** 2) @ absC2) / hbar ** 2
+            F_up = self._markov_up()
+            depth = np.array([_depth(a) for a in tb.aux_list])
+            F_up[:, depth == self.maxhier] = False
+            err = err + np.sum(F_up * E_up, axis=0)
+            gw2 = np.square(np.abs(g / w)).reshape(-1, 1)
+            E_dn = gw2 * ((np.abs(M2 - X2) ** 2) @ absC2) / hbar ** 2
+            err = err + np.sum((K2 > 0) * E_dn, axis=0)
+            dsq = self.delta_a ** 2
+            idx_st, aux_st = keep_above(err, dsq / 2.0, tb.aux_list)
+            unstable = np.setdiff1d(np.arange(nh), idx_st)
+            dbound = dsq - np.sum(err[unstable])
+            aux_b = self._boundary_aux(E_up, E_dn, idx_st, dbound, depth)
+            aux_new = list(set(aux_st) | set(aux_b))
+            stable_idx = sorted(self.aux_index[a] for a in aux_st)
+        else:
+            aux_new, aux_b, stable_idx = list(tb.aux_list), [], list(range(nh))
+        if self.delta_s > 0:
+            st_stable, st_bound = self._state_basis(Pn, C2, absC2, M2, K2, g, w, X2, sc, dt, stable_idx, aux_b)
+            states_new = sorted(set(st_stable) | set(st_bound))
+        else:
+            states_new = [int(s) for s in tb.state_list]
+        return states_new, aux_new
+
+    def _boundary_aux(self, E_up, E_dn, idx_st, dbound, depth):
+        """hops_basis.py:667-793."""
+        tb = self.tb
+        nm, nh = tb.n_mode, tb.n_hier
+        F_up = np.ones((nm, nh), bool)
+        F_dn = np.ones((nm, nh), bool)
+        for j, a in enumerate(tb.aux_list):
+            d = dict(a)
+            if depth[j] < self.maxhier:
+                for lm, m in enumerate(tb.modes):
+                    if _step_aux(a, int(m), 1) in self.aux_index:
+                        F_up[lm, j] = False
+            else:
+                F_up[:, j] = False
+            for lm, m in enumerate(tb.modes):
+                m = int(m)
+                if m not in d or _step_aux(a, m, -1) in self.aux_index:
+                    F_dn[lm, j] = False
+        F_up &= self._markov_up()
+        Eu = (E_up * F_up)[:, idx_st]
+        Ed = (E_dn * F_dn)[:, idx_st]
+        srt = np.sort(np.append(Eu[Eu != 0], Ed[Ed != 0]))
+        thr = error_threshold(srt, dbound * self.f_discard * self.f_discard)
+        discarded = np.sum(Eu[Eu <= thr]) + np.sum(Ed[Ed <= thr])
+        Eu[Eu <= thr] = 0
+        Ed[Ed <= thr] = 0
+        acc = {}
+        for col, j in enumerate(idx_st):
+            a = tb.aux_list[j]
+            for lm in np.nonzero(Eu[:, col])[0]:
+                key = _step_aux(a, int(tb.modes[lm]), 1)
+                acc[key] = acc.get(key, 0.0) + Eu[lm, col]
+            d = dict(a)
+            for lm in np.nonzero(Ed[:, col])[0]:
+                m = int(tb.modes[lm])
+                if m in d:
+                    key = _step_aux(a, m, -1)
+                    acc[key] = acc.get(key, 0.0) + Ed[lm, col]
+        thr = error_threshold(np.sort(np.array(list(acc.values()))), dbound, offset=discarded)
+        return [k for k, v in acc.items() if v > thr]
+
+    def _state_basis(self, Pn, C2, absC2, M2, K2, g, w, X2, sc, dt, stable_idx, aux_b):
+        """hops_basis.py:294-482 and :795-912."""
+        tb = self.tb
+        n, nh, nm = tb.n_state, tb.n_hier, tb.n_mode
+        P_st = np.zeros(nh * n, complex)
+        P_st.reshape(nh, n)[stable_idx] = Pn.reshape(nh, n)[stable_idx]
+        D = self._deriv(P_st) / hbar + P_st / dt
+        err = np.sum(np.abs(D.reshape(nh, n).T[:, stable_idx]) ** 2, axis=1)
+        Hm = -1j * self.info.H
+        Hc = Hm - np.diag(np.diag(Hm))
+        V1 = np.sum(np.abs(Hc[:, tb.state_list]) ** 2, axis=0)
+        err = err + V1 * np.sum(absC2[:, stable_idx], axis=1) / hbar ** 2
+        F_dn = np.zeros((nm, nh), bool)
+        F_up = np.zeros((nm, nh), bool)
+        for a in aux_b:
+            for lm, m in enumerate(tb.modes):
+                j = self.aux_index.get(_step_aux(a, int(m), 1))
+                if j is not None:
+                    F_dn[lm, j] = True
+            for (m, v) in a:
+                j = self.aux_index.get(_step_aux(a, m, -1))
+                if j is not None:
+                    F_up[self.rel_mode[m], j] = True
+        gw2 = np.square(np.abs(g / w)).reshape(-1, 1)
+        E_dn = ((np.abs(M2 - X2) ** 2).T @ (F_dn * gw2)) * absC2 / hbar ** 2
+        err = err + np.sum(E_dn[:, stable_idx], axis=1)
+        KW = F_up * (np.abs(w).reshape(-1, 1) * (1 + K2))
+        E_up = ((np.abs(M2) ** 2).T @ (np.abs(KW) ** 2)) * absC2 / hbar ** 2
+        err = err + np.sum(E_up[:, stable_idx], axis=1)
+        dsq = self.delta_s ** 2
+        rel_st, st_stable = keep_above(err, dsq / 2.0, [int(s) for s in tb.state_list])
+        dbound = dsq - np.sum(err[np.setdiff1d(np.arange(n), rel_st)])
+        bound = []
+        if len(rel_st) < self.info.nstates and sc:
+            C2s = C2[np.ix_(rel_st, stable_idx)]
+            Hsc = -1j * self.info.H[np.ix_(sc, st_stable)]
+            E = np.sum(np.abs(Hsc @ (C2s / hbar)) ** 2, axis=1)
+            nz = np.nonzero(E)[0]
+            if len(nz):
+                _, bound = keep_above(E[nz], dbound, list(np.array(sc)[nz]))
+        bound = sorted(set(int(b) for b in bound) - set(int(s) for s in tb.state_list))
+        return [int(s) for s in st_stable], bound
+
+class AdaptiveTrajectoryEngine:
+    """One adaptive trajectory: device state + basis bookkeeping."""
+
+    def __init__(self, setup, noise1, noise2, precision_mode, device, delta_a, delta_s, update_step,
+                 f_discard, early_integrator="INCH_WORM", early_steps=5, inchworm_cap=5):
+        self.setup = setup
+        self.info = setup.info
+        self.noise1, self.noise2 = noise1, noise2
+        self.prec = {"fp64": _lib.HOPS_PREC_FP64, "mixed": _lib.HOPS_PREC_MIXED}[precision_mode]
+        self.device = torch.device(device)
+        self.delta_a, self.delta_s = float(delta_a), float(delta_s)
+        self.update_step = int(update_step) if update_step else 1
+        self.f_discard = float(f_discard)
+        if early_integrator not in ("INCH_WORM", "STATIC"):
+            raise UnsupportedRequest(early_integrator, "early time integrator clause of the propagate")
+        self.early_integrator = early_integrator
+        self.early_steps, self.inchworm_cap = int(early_steps), int(inchworm_cap)
+        self.maxhier = setup.hier["MAXHIER"]
+        self.markov = None
+        for name, prm in setup.hier["STATIC_FILTERS"]:
+            if name != "Markovian":
+                raise UnsupportedRequest("static filter %s with an adaptive hierarchy" % name, "make_adaptive")
+            mk = np.array(prm, bool)
+            self.markov = mk if self.markov is None else (self.markov | mk)
+        self.selector = BasisSelector(self.info, self.delta_a, self.delta_s, self.maxhier, self.f_discard,
+                                      self.markov)
+        self.early_counter = 0
+        self.sp = _stream_ptr(self.device)
+        # noise of every hierarchy L-operator; a basis takes its active rows
+        from .noise import device_noise
+        nl = self.info.n_l2
+        self.z1_all = device_noise([noise1], list(range(nl)), self.device, self.sp)[0]
+        self.z2_all = None if noise2.model == "ZERO" else device_noise([noise2], list(range(nl)), self.device,
+                                                                       self.sp)[0]
+        self.zmem = torch.zeros(self.info.n_hmodes + 1, dtype=torch.complex128, device=self.device)
+
+    # -- basis objects -------------------------------------------------------
+    def _make(self, states, aux):
+        tb = BasisTables(self.info, states, aux, self.setup.eom_kind)
+        dop = DeviceOperator(tb, self.prec, self.device)
+        act = torch.as_tensor(tb.absL_active, dtype=torch.long, device=self.device)
+        z1 = self.z1_all.index_select(0, act).unsqueeze(0).contiguous()
+        z2 = None if self.z2_all is None else self.z2_all.index_select(0, act).unsqueeze(0).contiguous()
+        st = BatchState(dop, 1, self.setup.eom_kind, z1, z2, self.noise1.n_t, store_slots=0, lanes=1)
+        st.zmem[: self.info.n_hmodes].copy_(self.zmem[: self.info.n_hmodes])
+        self.tb, self.dop, self.st = tb, dop, st
+        # derivative evaluation runs in FP64 on its own operator when the
+        # propagation is mixed precision
+        self.dop64 = dop if self.prec == _lib.HOPS_PREC_FP64 else DeviceOperator(tb, _lib.HOPS_PREC_FP64, self.device)
+        self.st64 = st if self.prec == _lib.HOPS_PREC_FP64 else BatchState(
+            self.dop64, 1, self.setup.eom_kind, z1, z2, self.noise1.n_t, store_slots=0, lanes=1)
+
+    def phi_host(self):
+        return self.st.phi_host()[0]
+
+    def _sync_zmem_from_state(self):
+        self.zmem[: self.info.n_hmodes].copy_(self.st.zmem[: self.info.n_hmodes])
+
+    # -- device derivative ---------------------------------------------------
+    def _derivative(self, y_host, tidx):
+        st = self.st64
+        if st is not self.st:
+            st.zmem[: self.info.n_hmodes].copy_(self.st.zmem[: self.info.n_hmodes])
+        y = torch.from_numpy(np.ascontiguousarray(y_host, dtype=np.complex128)).to(self.device)
+        d = torch.empty_like(y)
+        check(lib().hops_derivative(ctypes.byref(self.dop64.op), ctypes.byref(st.bt), ctypes.c_void_p(y.data_ptr()),
+                                    ctypes.c_void_p(d.data_ptr()), int(tidx), self.sp))
+        return d.cpu().numpy()
+
+    def define_basis(self, phi, dt, tidx):
+        return self.selector.define_basis(self.tb, phi, dt, lambda y: self._derivative(y, tidx))
+
+    # -- basis change ----------------------------------------------------------
+    def update_basis(self, phi_host_vec, states, aux):
+        """hops_basis.py:184-292: new basis, Phi moved on the device and
+        rescaled to the old physical norm."""
+        old = self.tb
+        if set(states) == set(int(s) for s in old.state_list) and set(aux) == set(old.aux_list):
+            return False
+        self._sync_zmem_from_state()
+        old_st = self.st
+        n_old = old.n_state
+        old_pos = {int(s): i for i, s in enumerate(old.state_list)}
+        old_aux = {a: i for i, a in enumerate(old.aux_list)}
+        self._make(states, aux)
+        tb = self.tb
+        new_pos = {int(s): i for i, s in enumerate(tb.state_list)}
+        stable_states = sorted(set(old_pos) & set(new_pos))
+        src, dst = [], []
+        for j, a in enumerate(tb.aux_list):
+            i = old_aux.get(a)
+            if i is None:
+                continue
+            for s in stable_states:
+                src.append(i * n_old + old_pos[s])
+                dst.append(j * tb.n_state + new_pos[s])
+        norm_old = np.linalg.norm(phi_host_vec[:n_old])
+        norm_new = np.linalg.norm(phi_host_vec[[old_pos[s] for s in stable_states]])
+        scale = norm_old / norm_new
+        si = torch.as_tensor(np.array(src, dtype=np.int32), device=self.device)
+        di = torch.as_tensor(np.array(dst, dtype=np.int32), device=self.device)
+        check(lib().hops_permute_c128(ctypes.c_void_p(old_st.phi.data_ptr()), ctypes.c_void_p(self.st.phi.data_ptr()),
+                                      ctypes.c_void_p(si.data_ptr()), ctypes.c_void_p(di.data_ptr()), len(src),
+                                      float(scale), self.sp))
+        return True
+
+    # -- time loop ---------------------------------------------------------------
+    def initialize(self, psi_0, t0_index):
+        """Initial basis and its first update (trajectory/hops_trajectory.py:279-317)."""
+        psi_0 = np.asarray(psi_0, dtype=np.complex128)
+        if self.delta_s > 0:
+            states = [int(s) for s in np.where(np.abs(psi_0) > 0)[0]]
+        else:
+            states = list(range(self.info.nstates))
+        if self.delta_a > 0:
+            aux = [()]
+        else:
+            from .basis import apply_static_filters, triangular_aux_list
+            aux = triangular_aux_list(self.info.n_hmodes, self.maxhier)
+            aux = apply_static_filters(aux, self.setup.hier["STATIC_FILTERS"], self.info.n_hmodes)
+        self._make(states, aux)
+        tb = self.tb
+        phi = np.zeros(tb.n_hier * tb.n_state, complex)
+        phi[: tb.n_state] = psi_0[tb.state_list]
+        self.st.set_phi(phi)
+        st, ax = self.define_basis(phi, 1.0, t0_index)
+        self.update_basis(phi, st, ax)
+
+    def _rk(self, it, tau, ns):
+        self.st.propagate(it, 1, tau, ns, 0)
+
+    def _same_basis(self, states, aux):
+        return (set(states) == set(int(s) for s in self.tb.state_list)
+                and set(aux) == set(self.tb.aux_list))
+
+    def step(self, index_t, it, tau, ns):
+        """One adaptive RK4 step from noise index ``it``
+        (trajectory/hops_trajectory.py:456-533, :636-714)."""
+        early = self.early_counter < self.early_steps
+        if early and self.early_integrator == "INCH_WORM":
+            phi_old = self.phi_host()
+            zmem_old = self.st.zmem.clone()
+        self._rk(it, tau, ns)
+        if early:
+            if self.early_integrator == "INCH_WORM":
+                phi = self.phi_host()
+                s_new, a_new = self.define_basis(phi, tau, it)
+                k = 0
+                while not self._same_basis(s_new, a_new):
+                    # union basis, then the step again from the state before it
+                    s_u = sorted(set(s_new) | set(int(s) for s in self.tb.state_list))
+                    a_u = list(set(a_new) | set(self.tb.aux_list))
+                    self.st.set_phi(phi_old)
+                    self.st.zmem.copy_(zmem_old)
+                    self.update_basis(phi_old, s_u, a_u)
+                    phi_old = self.phi_host()
+                    zmem_old = self.st.zmem.clone()
+                    self._rk(it, tau, ns)
+                    phi = self.phi_host()
+                    s_new, a_new = self.define_basis(phi, tau, it)
+                    k += 1
+                    if k >= self.inchworm_cap:
+                        break
+                self.update_basis(phi, s_new, a_new)
+            self.early_counter += 1
+        elif (index_t + 1) % self.update_step == 0:
+            phi = self.phi_host()
+            s_new, a_new = self.define_basis(phi, tau, it)
+            self.update_basis(phi, s_new, a_new)
+        self._sync_zmem_from_state()

@@ -404,8 +404,15 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
     zd c = zz;
     if (z2) {
       const float2 v2 = z2[(long)j * bt.n_t + tidx];
-      c.x += (double)v2.y;
-      c.y -= (double)v2.x;
+      if (nonlin) {
+        c.x += (double)v2.y;
+        c.y -= (double)v2.x;
+      } else {
+        // LINEAR: conj(z1) - i z2 of two complex64 noise values is itself
+        // complex64, rounded in single precision (eom/hops_eom.py:519,534-536)
+        c.x = (double)(v1.x + v2.y);
+        c.y = (double)(-v1.y - v2.x);
+      }
     }
     cj[2 * j] = c.x;
     cj[2 * j + 1] = c.y;

@@ -27,20 +27,81 @@ _EOM_KIND = {"LINEAR": _lib.HOPS_EOM_LINEAR, "NONLINEAR": _lib.HOPS_EOM_NONLINEA
 _PREC = {"fp64": _lib.HOPS_PREC_FP64, "mixed": _lib.HOPS_PREC_MIXED}
 
 
-class HopsStorage:
-    """Per-step record of psi and t (storage/hops_storage.py:33-209); psi is
-    downloaded from the device at the end of each propagate call."""
+def _save_psi_traj(phi_new, state_list, **kw):
+    return np.array(phi_new[: len(state_list)], dtype=np.complex128)
 
-    def __init__(self, storage_param):
-        p = dict(storage_param or {})
-        self.storage_time = p.pop("STORAGE_TIME", True)
-        self.storage_dic = {"phi_traj": False, "psi_traj": True, "t_axis": True, "z_mem": False}
-        for k, v in p.items():
-            if k not in ("phi_traj", "psi_traj", "t_axis", "z_mem"):
-                raise UnsupportedRequest("storage key {}".format(k), "HopsStorage")
-            self.storage_dic[k] = v
-        self.data = {k: [] for k, v in self.storage_dic.items() if v}
+
+def _save_aux_norm(phi_new, aux_list, **kw):
+    n_aux = len(aux_list)
+    return np.sqrt(np.sum(np.abs(np.asarray(phi_new)).reshape(n_aux, -1) ** 2, axis=1))
+
+
+# storage/storage_functions.py:8-196
+STORAGE_DEFAULT_FUNC = {
+    "psi_traj": _save_psi_traj,
+    "phi_traj": lambda phi_new, **kw: np.array(phi_new),
+    "phi_norm": lambda phi_new, **kw: np.linalg.norm(phi_new),
+    "t_axis": lambda t_new, **kw: t_new,
+    "aux_list": lambda aux_list, **kw: [np.array([[m, v] for (m, v) in a], dtype=int).reshape(-1, 2)
+                                        if len(a) else np.array([], dtype=int) for a in aux_list],
+    "state_list": lambda state_list, **kw: list(state_list),
+    "list_nstate": lambda state_list, **kw: len(state_list),
+    "list_nhier": lambda aux_list, **kw: len(aux_list),
+    "list_aux_norm": _save_aux_norm,
+    "z_mem": lambda z_mem_new, **kw: z_mem_new,
+}
+
+
+class HopsStorage:
+    """Per-step record of a trajectory (storage/hops_storage.py:13-209): the
+    same keys, defaults (adaptive runs add aux_list, state_list, list_nhier,
+    list_nstate, list_aux_norm), callables as custom savers, STORAGE_TIME, and
+    the dense reconstruction of psi_traj for adaptive runs."""
+
+    def __init__(self, adaptive, storage_dic):
+        self.storage_dic = dict(storage_dic or {})
+        self.storage_time = self.storage_dic.pop("STORAGE_TIME", True)
+        self._adaptive = False
+        self.n_dim = 0
         self.metadata = {"INITIALIZATION_TIME": 0, "LIST_PROPAGATION_TIME": []}
+        self.adaptive = adaptive
+
+    @property
+    def adaptive(self):
+        return self._adaptive
+
+    @adaptive.setter
+    def adaptive(self, new):
+        self._adaptive = new
+        self.dic_save = {}
+        self.data = {}
+        for k, v in (("phi_traj", False), ("psi_traj", True), ("t_axis", True), ("z_mem", False)):
+            self.storage_dic.setdefault(k, v)
+        if new:
+            for k in ("aux_list", "state_list", "list_nhier", "list_nstate", "list_aux_norm"):
+                self.storage_dic.setdefault(k, True)
+        for key, value in self.storage_dic.items():
+            if value is not False:
+                if isinstance(value, bool):
+                    if key not in STORAGE_DEFAULT_FUNC:
+                        raise UnsupportedRequest("this value", "Storage.adaptive")
+                    self.dic_save[key] = STORAGE_DEFAULT_FUNC[key]
+                elif callable(value):
+                    self.dic_save[key] = value
+                else:
+                    raise UnsupportedRequest("this value", "Storage.adaptive")
+        for key, value in self.storage_dic.items():
+            if value:
+                self.data[key] = []
+
+    @property
+    def needs_full_state(self):
+        """True when a saver needs more than psi and t of each step."""
+        return any(k not in ("psi_traj", "t_axis", "state_list", "list_nstate") for k in self.dic_save)
+
+    def store_step(self, **kwargs):
+        for key, fn in self.dic_save.items():
+            self.data[key].append(fn(**kwargs))
 
     def check_storage_time(self, t):
         st = self.storage_time
@@ -50,21 +111,27 @@ class HopsStorage:
             return t % st == 0
         if isinstance(st, bool):
             return st
-        raise UnsupportedRequest("Type {} is not allowed in the STORAGE_TIME key".format(type(st)),
+        raise UnsupportedRequest("Type {} is not allowed in the STORAGE_TIME key of storage parameters for "
+                                 "checking whether the current time should be saved".format(type(st)),
                                  "check_storage_time", override=True)
 
-    def store(self, t, psi, phi=None, z_mem=None):
-        if "psi_traj" in self.data:
-            self.data["psi_traj"].append(psi)
-        if "t_axis" in self.data:
-            self.data["t_axis"].append(t)
-        if "phi_traj" in self.data and phi is not None:
-            self.data["phi_traj"].append(phi)
-        if "z_mem" in self.data and z_mem is not None:
-            self.data["z_mem"].append(z_mem)
-
-    def __getitem__(self, key):
-        return self.data[key]
+    def __getitem__(self, item):
+        if self._adaptive and item == "psi_traj":
+            full = np.zeros([len(self["t_axis"]), self.n_dim], dtype=np.complex128)
+            for i, psi in enumerate(self.data["psi_traj"]):
+                full[i, np.array(self.data["state_list"][i])] = psi
+            return full
+        if self._adaptive and item == "psi_traj_sparse":
+            import scipy.sparse as sps
+            dat, row, col = [], [], []
+            for i, psi in enumerate(self.data["psi_traj"]):
+                dat.extend(list(psi))
+                row.extend([i] * len(psi))
+                col.extend(self.data["state_list"][i])
+            return sps.csr_array((dat, (row, col)), shape=(len(self["t_axis"]), self.n_dim))
+        if item in self.data:
+            return self.data[item]
+        print("{} not a key of storage.data".format(item))
 
 
 class _Setup:
@@ -263,8 +330,9 @@ class HopsBatch:
 
 class HopsTrajectory:
     """One HOPS trajectory with the reference's constructor and methods
-    (trajectory/hops_trajectory.py:84-538).  Extra keyword arguments select
-    the precision mode ("fp64" by default) and the device."""
+    (trajectory/hops_trajectory.py:84-538, :1071-1125).  make_adaptive()
+    switches to the adaptive basis (mesohops_amd/adaptive.py).  Extra keyword
+    arguments select the precision mode ("fp64" by default) and the device."""
 
     def __init__(self, system_param, eom_param=None, noise_param=None, noise2_param=None,
                  hierarchy_param=None, storage_param=None, integration_param=None,
@@ -275,50 +343,112 @@ class HopsTrajectory:
                                 integration_param, noise_params=[dict(noise_param)],
                                 precision_mode=precision_mode, device=device)
         self.noise_param = noise_param
-        self.storage = HopsStorage(storage_param)
+        self.noise1 = self._batch.noise1[0]
+        self.noise2 = self._batch.noise2[0]
+        self.integration_param = self._batch.setup.integ
+        if self.integration_param["STATIC_BASIS"] is not None:
+            raise UnsupportedRequest("STATIC_BASIS", "HopsTrajectory")
+        self.storage = HopsStorage(self._batch.setup.eom["ADAPTIVE"], storage_param or {})
+        self._adaptive = None
         self.__initialized__ = False
 
     # -- reference API ----------------------------------------------------
     def make_adaptive(self, delta_a=1e-4, delta_s=1e-4, update_step=1, f_discard=0.01, adaptive_noise=True):
+        """trajectory/hops_trajectory.py:338-399.  adaptive_noise changes no
+        value: the noise of an L-operator does not depend on when it is first
+        generated (tests/test_hops_noise.py:297-362)."""
         if float(f_discard) < 0 or float(f_discard) > 1:
-            raise UnsupportedRequest("f_discard", "f_discard not in acceptable range of [0,1]")
+            raise UnsupportedRequest("make_adaptive", "f_discard not in acceptable range of [0,1]")
         if self.__initialized__:
             raise TrajectoryError("Calling make_adaptive on an initialized trajectory")
         eom = self._batch.setup.eom
         if delta_a > 0:
             eom.update(ADAPTIVE=True, ADAPTIVE_H=True, DELTA_A=delta_a, UPDATE_STEP=update_step,
                        F_DISCARD=float(f_discard))
+            self.storage.adaptive = True
         if delta_s > 0:
             eom.update(ADAPTIVE=True, ADAPTIVE_S=True, DELTA_S=delta_s, UPDATE_STEP=update_step,
                        F_DISCARD=float(f_discard))
+            self.storage.adaptive = True
+
+    @property
+    def adaptive(self):
+        return bool(self._batch.setup.eom["ADAPTIVE"])
+
+    def _store(self, t, phi, aux_list, state_list, z_mem):
+        self.storage.store_step(phi_new=phi, aux_list=aux_list, state_list=state_list, t_new=t,
+                                z_mem_new=z_mem)
 
     def initialize(self, psi_0, timer_checkpoint=None):
         if timer_checkpoint is None:
             timer_checkpoint = time.time()
         if self.__initialized__:
             raise LockedException("HopsTrajectory.initialize()")
-        self._batch.initialize(psi_0)
         b = self._batch
-        self.storage.store(0, b.psi_traj[0][0])
+        self.storage.n_dim = b.setup.info.nstates
+        if self.adaptive:
+            from .adaptive import AdaptiveTrajectoryEngine
+            e = b.setup.eom
+            ip = self.integration_param
+            self._adaptive = AdaptiveTrajectoryEngine(
+                b.setup, self.noise1, self.noise2, b.precision_mode, b.device,
+                e["DELTA_A"] if e["ADAPTIVE_H"] else 0.0, e["DELTA_S"] if e["ADAPTIVE_S"] else 0.0,
+                e["UPDATE_STEP"], e["F_DISCARD"], ip["EARLY_ADAPTIVE_INTEGRATOR"],
+                ip["EARLY_INTEGRATOR_STEPS"], ip["INCHWORM_CAP"])
+            self._adaptive.initialize(psi_0, self.noise1.time_index(0.0))
+            self._t = 0.0
+        else:
+            b.initialize(psi_0)
+        self._store(0, self.phi, self.auxiliary_list, list(self.state_list), self.z_mem)
         self.storage.metadata["INITIALIZATION_TIME"] = time.time() - timer_checkpoint
         self.__initialized__ = True
 
     def propagate(self, t_advance, tau, timer_checkpoint=None):
         if timer_checkpoint is None:
             timer_checkpoint = time.time()
+        if not self.__initialized__:
+            raise TrajectoryError("propagate called before initialize")
         b = self._batch
-        n0 = len(b.t_axis)
-        t_end = b.t + np.arange(1, 1 + int(np.ceil(t_advance / tau))) * tau
-        print("Integration from ", b.t, " to ", np.max(t_end) if len(t_end) else b.t)
-        b.propagate(t_advance, tau, store=True)
-        for t, psi in zip(b.t_axis[n0:], b.psi_traj[0][n0:]):
-            if self.storage.check_storage_time(t):
-                self.storage.store(t, psi)
+        n_steps, it0, ns = b._plan(t_advance, tau) if not self.adaptive else self._plan_adaptive(t_advance, tau)
+        t_axis = self.t + np.arange(1, 1 + n_steps) * tau
+        print("Integration from ", self.t, " to ", np.max(t_axis) if len(t_axis) else self.t)
+        if self.adaptive:
+            ad = self._adaptive
+            for index_t, t in enumerate(t_axis):
+                z_lag = self.z_mem
+                ad.step(index_t, it0 + 2 * ns * index_t, tau, ns)
+                if self.storage.check_storage_time(t):
+                    self._store(float(t), ad.phi_host(), self.auxiliary_list, list(self.state_list), z_lag)
+                self._t = float(t)
+        elif self.storage.needs_full_state:
+            # one step per call: every saver sees the whole state
+            for t in t_axis:
+                z_lag = self.z_mem
+                b.propagate(tau, tau, store=False)
+                if self.storage.check_storage_time(t):
+                    self._store(float(t), self.phi, self.auxiliary_list, list(self.state_list), z_lag)
+        else:
+            n0 = len(b.t_axis)
+            b.propagate(t_advance, tau, store=True)
+            for t, psi in zip(b.t_axis[n0:], b.psi_traj[0][n0:]):
+                if self.storage.check_storage_time(t):
+                    self._store(t, psi, self.auxiliary_list, list(self.state_list), None)
         self.storage.metadata["LIST_PROPAGATION_TIME"].append(time.time() - timer_checkpoint)
+
+    def _plan_adaptive(self, t_advance, tau):
+        b = self._batch
+        saved = b.t
+        b.t = self._t
+        try:
+            return b._plan(t_advance, tau)
+        finally:
+            b.t = saved
 
     # -- readbacks (trajectory/hops_trajectory.py:1071-1125) -------------
     @property
     def phi(self):
+        if self._adaptive is not None:
+            return self._adaptive.phi_host()
         return self._batch.phi[0]
 
     @property
@@ -327,24 +457,50 @@ class HopsTrajectory:
 
     @property
     def t(self):
-        return self._batch.t
+        return self._t if self._adaptive is not None else self._batch.t
 
     @property
     def z_mem(self):
-        return self._batch.state.zmem_host()[0]
+        """Noise memory over the absolute modes as a (len(L_NOISE1), 1) CSR
+        array, as the reference keeps it."""
+        import scipy.sparse as sps
+        if self._adaptive is not None:
+            zm = self._adaptive.st.zmem.cpu().numpy()[: self._batch.setup.info.n_hmodes]
+        else:
+            zm = self._batch.state.zmem_host()[0]
+        n = max(self._batch.setup.info.n_noise_modes, len(zm))
+        full = np.zeros((n, 1), dtype=np.complex128)
+        full[: len(zm), 0] = zm
+        return sps.csr_array(full)
+
+    @property
+    def _tables(self):
+        return self._adaptive.tb if self._adaptive is not None else self._batch.tables
 
     @property
     def n_hier(self):
-        return self._batch.tables.n_hier
+        return self._tables.n_hier
 
     @property
     def n_state(self):
-        return self._batch.tables.n_state
+        return self._tables.n_state
 
     @property
     def state_list(self):
-        return self._batch.tables.state_list
+        return self._tables.state_list
 
     @property
     def auxiliary_list(self):
-        return self._batch.tables.aux_list
+        return self._tables.aux_list
+
+    @property
+    def update_step(self):
+        return self._batch.setup.eom["UPDATE_STEP"]
+
+    @property
+    def early_steps(self):
+        return self.integration_param["EARLY_INTEGRATOR_STEPS"]
+
+    @property
+    def inchworm_cap(self):
+        return self.integration_param["INCHWORM_CAP"]

@@ -1,0 +1,142 @@
+"""Adaptive HOPS on the device against the FP64 oracle.
+
+hops_derivative (the derivative the adaptive error bound needs) equals the
+oracle's dsystem_dt to 1e-10 relative; whole adaptive trajectories reproduce
+the oracle's basis sequence (state_list and aux_list at every step) exactly
+and its psi(t) to 1e-10.  Noise arrays are shared via PRE_CALCULATED.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import systems
+from oracle import hops_oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _noise(sysp, seed, tlen, tau):
+    nz = O.Noise({"SEED": seed, "MODEL": "FFT_FILTER", "TLEN": tlen, "TAU": tau}, O.derive_system(sysp), 1)
+    return nz.noise
+
+
+@pytest.mark.parametrize("eom", ["NORMALIZED NONLINEAR", "NONLINEAR", "LINEAR"])
+@pytest.mark.parametrize("sys_name", ["dod", "chain_ltc", "three_general"])
+def test_device_derivative_matches_oracle(eom, sys_name):
+    _need_gpu()
+    from mesohops_amd import _lib
+    from mesohops_amd.basis import BasisTables, SystemInfo
+    from mesohops_amd.engine import BatchState, DeviceOperator, _stream_ptr
+    from mesohops_amd.trajectory import _EOM_KIND
+    sysp = {"dod": systems.dimer_of_dimers(), "chain_ltc": systems.chain(5, ltc=20.0 - 3.0j),
+            "three_general": systems.three_site(True)}[sys_name]
+    info = SystemInfo(sysp)
+    sysd = O.derive_system(sysp)
+    rng = np.random.default_rng(3)
+    # a ragged aux set; for the dimer of dimers also a state sub-basis whose
+    # aux list keeps modes of the dropped state (L-operators without an
+    # active state).  The LTC chain keeps every state: with an inactive
+    # L-operator the reference's LINEAR LTC term has mismatched shapes
+    # (eom/hops_eom.py:527-530 multiplies the active-L coefficients with all
+    # basis L^2) and raises.
+    full = O.triangular_hierarchy(sysd["N_HMODES"], 2)
+    aux = [full[0]] + [a for i, a in enumerate(full[1:]) if rng.random() < 0.7]
+    states = list(range(info.nstates - 1)) if sys_name == "dod" else list(range(info.nstates))
+    tb = BasisTables(info, states, aux, _EOM_KIND[eom])
+    b = O.Basis(sysd, states, tb.aux_list)
+    dsys = O.make_derivative(b, eom)
+    dop = DeviceOperator(tb, _lib.HOPS_PREC_FP64)
+    n_t = 8
+    z1h = (rng.normal(size=(tb.n_act, n_t)) + 1j * rng.normal(size=(tb.n_act, n_t))).astype(np.complex64)
+    z2h = (rng.normal(size=(tb.n_act, n_t)) + 1j * rng.normal(size=(tb.n_act, n_t))).astype(np.complex64)
+    z1 = torch.from_numpy(z1h[None]).cuda()
+    z2 = torch.from_numpy(z2h[None]).cuda()
+    st = BatchState(dop, 1, _EOM_KIND[eom], z1, z2, n_t, lanes=1)
+    zmem = (rng.normal(size=info.n_hmodes) + 1j * rng.normal(size=info.n_hmodes)) * 30.0
+    st.zmem[: info.n_hmodes].copy_(torch.from_numpy(zmem))
+    phi = rng.normal(size=tb.n_hier * tb.n_state) + 1j * rng.normal(size=tb.n_hier * tb.n_state)
+    y = torch.from_numpy(phi).cuda()
+    d = torch.empty_like(y)
+    tidx = 5
+    _lib.check(_lib.lib().hops_derivative(ctypes.byref(dop.op), ctypes.byref(st.bt), ctypes.c_void_p(y.data_ptr()),
+                                          ctypes.c_void_p(d.data_ptr()), tidx, _stream_ptr(dop.device)))
+    got = d.cpu().numpy()
+    # oracle noise is indexed by the basis L-operators (relative order of
+    # absL) and, like the reference's get_noise output, is complex64
+    act = {int(l): j for j, l in enumerate(tb.absL_active)}
+    zo1 = np.array([z1h[act[int(l)], tidx] if int(l) in act else 0 for l in b.absL], dtype=np.complex64)
+    zo2 = np.array([z2h[act[int(l)], tidx] if int(l) in act else 0 for l in b.absL], dtype=np.complex64)
+    ref = dsys(phi, zmem, zo1, zo2)[0]
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print("derivative", sys_name, eom, "rel err %.3e" % err)
+    assert err < FP64_TOL
+
+
+def _markov_chain(nsite):
+    import scipy.sparse as sp
+    g0, w0 = O.bcf_convert_dl_to_exp(50.0, 50.0, 295.0)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        gw += [[g0, w0], [-1j * np.imag(g0), 500.0]]
+        ll += [L, L]
+    H = (np.eye(nsite, k=1) + np.eye(nsite, k=-1)) * 50.0
+    return dict(HAMILTONIAN=np.array(H, complex), GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll,
+                ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw), [False, True] * nsite
+
+
+ADAPTIVE_CASES = {
+    "dod_both": (systems.dimer_of_dimers(), None, 1e-3, 1e-3, 3, "NORMALIZED NONLINEAR", 80.0, 2.0),
+    "chain_hier_only": (systems.chain(6), None, 1e-3, 0.0, 4, "NORMALIZED NONLINEAR", 60.0, 2.0),
+    "chain_states_only": (systems.chain(8), None, 0.0, 2e-3, 2, "NONLINEAR", 60.0, 2.0),
+    "markov_chain": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR", 60.0, 2.0),
+    "chain10_update3": (systems.chain(10), None, 3e-3, 3e-3, 3, "NORMALIZED NONLINEAR", 60.0, 1.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ADAPTIVE_CASES))
+def test_adaptive_trajectory_matches_oracle(name):
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp, markov, da, ds, K, eom, t_max, dt = ADAPTIVE_CASES[name]
+    update = 3 if name.endswith("update3") else 1
+    nst = sysp["HAMILTONIAN"].shape[0]
+    tlen, tau = t_max + 20.0, 0.5
+    z = _noise(sysp, 11, tlen, tau)
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    psi0 = np.zeros(nst, complex)
+    psi0[nst // 2] = 1.0
+    ref = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
+                               delta_a=da, delta_s=ds, update_step=update, markov=markov)
+    ref.initialize(psi0)
+    ref.propagate(t_max, dt)
+    hp = {"MAXHIER": K}
+    if markov is not None:
+        hp["STATIC_FILTERS"] = [["Markovian", markov]]
+    tr = HopsTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, dict(noise), hierarchy_param=hp)
+    tr.make_adaptive(da, ds, update_step=update)
+    tr.initialize(psi0)
+    tr.propagate(t_max, dt)
+    got_states = tr.storage["state_list"]
+    got_aux = tr.storage["list_nhier"]
+    assert len(got_states) == len(ref.state_traj)
+    for i, (s_got, s_ref) in enumerate(zip(got_states, ref.state_traj)):
+        assert list(s_got) == list(s_ref), (i, s_got, s_ref)
+        assert got_aux[i] == len(ref.aux_traj[i]), i
+    assert [tuple(a) for a in tr.auxiliary_list] == list(ref.basis.aux_list)
+    psi = tr.storage["psi_traj"]
+    err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+    sizes = sorted(set(zip(tr.storage["list_nstate"], tr.storage["list_nhier"])))
+    print("adaptive %s: max |dpsi| = %.3e, (n_state, n_hier) from %s to %s" % (name, err, sizes[0], sizes[-1]))
+    assert err < FP64_TOL
+    assert np.max(np.abs(tr.phi - ref.phi)) < FP64_TOL
+    assert len(sizes) > 1   # the basis moved during the run

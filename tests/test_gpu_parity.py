@@ -282,3 +282,29 @@ def test_chunked_propagation_equals_whole_batch():
         dz = np.max(np.abs(got.state.zmem_host() - ref.state.zmem_host()))
         print("chunked", eom, d, dphi, dz)
         assert d == 0.0 and dphi == 0.0 and dz == 0.0
+
+
+@pytest.mark.parametrize("flag_real", [False, True])
+@pytest.mark.parametrize("eom", ["NORMALIZED NONLINEAR", "NONLINEAR", "LINEAR"])
+def test_fp64_second_noise(eom, flag_real):
+    """A nonzero noise 2 (PRE_CALCULATED, complex or FLAG_REAL): the
+    coefficient conj(z1) - i z2 of LINEAR HOPS is formed in complex64, as the
+    reference forms it from its complex64 noise arrays."""
+    _need_gpu()
+    from mesohops_amd import HopsBatch
+    sysp = systems.three_site(False)
+    sysp.update(L_NOISE2=sysp["L_NOISE1"], ALPHA_NOISE2=O.bcf_exp, PARAM_NOISE2=sysp["PARAM_NOISE1"])
+    tlen, tau = 60.0, 0.5
+    z1 = _oracle_noise(sysp, 3, tlen, tau)
+    z2 = _oracle_noise(sysp, 4, tlen, tau)
+    n1 = {"SEED": z1, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    n2 = {"SEED": z2, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau, "FLAG_REAL": flag_real}
+    tr, psi0 = oracle_run(sysp, eom, n1, 3, 40.0, 1.0, noise2_param=n2)
+    b = HopsBatch(sysp, {"EQUATION_OF_MOTION": eom}, n1, dict(n2), {"MAXHIER": 3}, noise_params=[n1],
+                  precision_mode="fp64")
+    b.initialize(psi0)
+    b.propagate(40.0, 1.0)
+    ref = np.array(tr.psi_traj)
+    err = np.max(np.abs(np.array(b.psi_traj[0]) - ref)) / max(1.0, np.max(np.abs(ref)))
+    print("fp64 noise2 %s flag_real=%s max |dpsi| = %.3e" % (eom, flag_real, err))
+    assert err < FP64_TOL
