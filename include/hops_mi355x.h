@@ -69,6 +69,9 @@ typedef struct hops_op {
   /* active L-operators: relative index, relative modes sharing the operator
      (CSR over active L), COO of L and of L^2, LTC coefficient */
   const int32_t *act_lrel_d;
+  /* noise row of each active L-operator in the z1/z2 arrays of one
+     trajectory; NULL: row j holds active L-operator j */
+  const int32_t *act_zrow_d;
   const int32_t *am_ptr_d, *am_mode_d;
   const int32_t *la_ptr_d, *la_row_d, *la_col_d;
   const double *la_val_d;
@@ -182,6 +185,65 @@ int hops_noise_embedding(const float *ang_d, const float *mag_d, int n_lop, int 
 int hops_noise_filter(const void *z_d, int R, int n_t, const double *s_w_d,
                       const double *nu_d, const int32_t *row_lop_d, void *out_d,
                       void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Ragged batches ("slots"): trajectories with different bases, and different
+ * systems, advanced by one launch per stage.  Slot i has its own operator
+ * ops_d[i] and a B = 1, lane_shift = 0 batch descriptor bts_d[i] (both arrays
+ * in device memory); list_d names the slots of a call and tiles_d their
+ * 256-row tiles (slot << 20 | tile, every tile of every listed slot once).
+ * it_d[i]: noise index of slot i at the start of its current step.
+ * shm_doubles: max over the listed slots of 2 n_state + 8 n_act + 2 n_mode_abs
+ * + 16.  Together these replace the per-trajectory Python loop
+ * (trajectory/hops_trajectory.py:456-533) for a batch of adaptive
+ * trajectories whose bases differ.
+ * ------------------------------------------------------------------------- */
+
+/* Phi move of one slot into its new basis (basis/hops_basis.py:214-274):
+   new row (j, s) = scale * old row (inv_aux[j], inv_state[s]), or 0 when
+   either index is -1. */
+typedef struct hops_slot_perm {
+  const void *old_phi_d; /* complex128 [nh_old * n_old] */
+  void *new_phi_d;       /* complex128 [nh_new * n_new] */
+  const int32_t *inv_aux_d, *inv_state_d;
+  int32_t n_old, n_new, nh_new, pad;
+  double scale;
+} hops_slot_perm_t;
+
+/* One RK4 step (integrator/integrator_rk.py:10-75, eom/hops_eom.py:196-542)
+   of every listed slot, then the normalisation of NORMALIZED NONLINEAR
+   slots (trajectory/hops_trajectory.py:616-634).  One precision per call. */
+int hops_slots_step(const hops_op_t *ops_d, const hops_batch_t *bts_d, const int32_t *list_d, int n_list,
+                    const uint32_t *tiles_d, int n_tiles, const int32_t *it_d, int precision,
+                    int shm_doubles, void *stream);
+
+/* Pn = Phi / |psi| of every listed slot (basis/hops_basis.py:157): complex128
+   into out_d + off_d[slot] (elements), stage precision into the slot's ya
+   pool, |psi| into nrm_d[slot].  psi_only != 0: copy psi = Phi[:n_state] to
+   out_d + off_d[slot] and nothing else. */
+int hops_slots_pn(const hops_op_t *ops_d, const hops_batch_t *bts_d, const int32_t *list_d, int n_list,
+                  const int64_t *off_d, void *out_d, double *nrm_d, int precision, int psi_only,
+                  void *stream);
+
+/* Derivative part of the adaptive error bound (basis_functions_adaptive.py:
+   10-74; hops_basis.py:560-665, :795-912).  With mask_d == NULL: D at Pn
+   (from hops_slots_pn, pn_d + off_d[slot]) and e_d + eoff_d[slot] gets, per
+   auxiliary k, sum_s |D[k,s]/hbar + Pn[k,s]/dt[slot]|^2.  With mask_d:
+   Pn restricted to the auxiliaries k with mask_d[moff_d[slot] + k] != 0,
+   and per state s the same sum over those auxiliaries. */
+int hops_slots_errors(const hops_op_t *ops_d, const hops_batch_t *bts_d, const int32_t *list_d, int n_list,
+                      const uint32_t *tiles_d, int n_tiles, const int32_t *it_d, int precision,
+                      int shm_doubles, const int64_t *off_d, const void *pn_d, const int64_t *moff_d,
+                      const uint8_t *mask_d, const int64_t *eoff_d, double *e_d, const double *dt_d,
+                      double hbar, void *stream);
+
+/* hops_build_super for every listed slot (eom/eom_hops_ksuper.py:387-668). */
+int hops_slots_build(const hops_op_t *ops_d, const int32_t *list_d, int n_list, const uint32_t *tiles_d,
+                     int n_tiles, int precision, void *stream);
+
+/* Phi of the new bases; tiles run over the new rows of each slot and index
+   perm_d by slot. */
+int hops_slots_permute(const hops_slot_perm_t *perm_d, const uint32_t *tiles_d, int n_tiles, void *stream);
 
 #ifdef __cplusplus
 }

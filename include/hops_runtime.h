@@ -1,0 +1,117 @@
+/*
+ * hops_runtime.h — native host runtime of the MI355X HOPS engine: ragged
+ * batches of adaptive trajectories (each with its own basis and, if wanted,
+ * its own system), advanced together on one GPU.
+ *
+ * The runtime owns the basis bookkeeping the reference does in Python per
+ * trajectory, and drives the slot kernels of hops_mi355x.h:
+ *   hops_rt_batch_step   trajectory/hops_trajectory.py:401-533 (time loop with
+ *                        basis updates), :636-714 (INCH_WORM early steps)
+ *   (inside)             basis/hops_basis.py:117-292 define_basis /
+ *                        update_basis, :294-998 error bounds,
+ *                        basis/basis_functions.py:8-41 thresholds,
+ *                        basis/hops_fluxfilters.py flux filters,
+ *                        basis/hops_hierarchy.py, hops_aux.py, hops_system.py,
+ *                        hops_modes.py (auxiliary order, connections, relative
+ *                        indexing), eom/eom_hops_ksuper.py:387-668 (operator of
+ *                        each new basis, rebuilt on the device)
+ *
+ * Device memory comes from one caller-allocated arena; the runtime never
+ * calls hipMalloc for trajectory data, so admission by HBM footprint is a
+ * question of arena space (hops_rt_arena_free).  All work runs on the stream
+ * given at creation.  Functions return 0 (or a non-negative value) on
+ * success and a negative status on failure; hops_last_error() has the text.
+ */
+#ifndef HOPS_RUNTIME_H
+#define HOPS_RUNTIME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A system (basis/system_functions.py:10-171): Hamiltonian, hierarchy modes
+   and their unique L-operators (COO per operator, zeros removed), and the
+   low-temperature correction coefficient of each L-operator. */
+typedef struct hops_rt_system_desc {
+  int32_t n_state, n_hmode, n_lop, pad;
+  const double *H;          /* complex [n_state][n_state] row-major          */
+  const double *g, *w;      /* complex [n_hmode]                             */
+  const int32_t *l_of_mode; /* [n_hmode] L-operator index of each mode       */
+  const int32_t *l_ptr;     /* [n_lop + 1] CSR over operators of COO entries */
+  const int32_t *l_row, *l_col;
+  const double *l_val;      /* complex                                       */
+  const double *lt;         /* complex [n_lop] LTC coefficients, or NULL     */
+} hops_rt_system_desc_t;
+
+/* Per-trajectory parameters (eom/hops_eom.py:20-41, hops_trajectory.py:34-50,
+   :338-399).  delta_a / delta_s = 0 turns that half of the adaptivity off;
+   with delta_a = 0 the hierarchy is aux0 (CSR over auxiliaries of absolute
+   mode / depth entries, first entry the zero auxiliary). */
+typedef struct hops_rt_traj_param {
+  int32_t eom, maxhier, update_step, early_integrator; /* 0 INCH_WORM, 1 STATIC */
+  int32_t early_steps, inchworm_cap, noise_step, n_t;
+  int32_t it0, record, n_aux0, pad;
+  double delta_a, delta_s, f_discard, dt;
+  const uint8_t *markov;   /* [n_hmode] Markovian filter flags, or NULL */
+  const int32_t *aux0_ptr, *aux0_mode, *aux0_val;
+} hops_rt_traj_param_t;
+
+void *hops_rt_system_create(const hops_rt_system_desc_t *desc);
+int hops_rt_system_destroy(void *sys);
+
+/* A ragged batch over arena_d[0, arena_bytes) (caller-owned device memory,
+   256-byte aligned).  precision: HOPS_PREC_FP64 or HOPS_PREC_MIXED.
+   n_threads: host worker threads for the basis bookkeeping (<= 0: 16). */
+void *hops_rt_batch_create(int precision, void *arena_d, size_t arena_bytes, int max_slots, int n_threads,
+                           void *stream);
+int hops_rt_batch_destroy(void *batch);
+
+/* Admit one trajectory: initial state psi0 (complex [n_state] host), its
+   noise z1_d / z2_d (complex64 [n_lop][n_t] device, caller-owned, z2_d may be
+   NULL), parameters prm.  Builds the initial basis and its first update
+   (trajectory/hops_trajectory.py:253-336).  Returns the slot id. */
+int hops_rt_batch_add(void *batch, void *sys, const hops_rt_traj_param_t *prm, const double *psi0,
+                      const void *z1_d, const void *z2_d);
+/* Release a slot and its arena memory. */
+int hops_rt_batch_remove(void *batch, int slot);
+/* Device bytes a trajectory would take at the given basis size, for
+   admission by HBM footprint. */
+long long hops_rt_slot_estimate(void *batch, void *sys, int n_state, int n_hier);
+long long hops_rt_arena_free(void *batch);
+
+/* Advance every active slot (slots_or_null == NULL) or the listed slots by
+   n_steps steps of their own dt. */
+int hops_rt_batch_step(void *batch, int n_steps, const int32_t *slots_or_null, int n_slots);
+
+/* Read-backs.  info[0..7] = n_state, n_hier, steps taken, noise index,
+   recorded steps, device bytes, early steps left, active. */
+int hops_rt_slot_info(void *batch, int slot, long long *info);
+int hops_rt_slot_states(void *batch, int slot, int32_t *states_out);
+int hops_rt_slot_aux(void *batch, int slot, int32_t *ptr_out, int32_t *mode_out, int32_t *val_out);
+int hops_rt_slot_phi(void *batch, int slot, double *phi_out);     /* complex [n_state*n_hier] */
+int hops_rt_slot_zmem(void *batch, int slot, double *zmem_out);   /* complex [n_hmode]        */
+/* Recorded step idx (0 = initial state): time index, psi over the basis
+   states, the states, n_hier. */
+int hops_rt_slot_record(void *batch, int slot, int idx, double *psi_out, int32_t *states_out,
+                        int32_t *n_state_out, int32_t *n_hier_out);
+int hops_rt_slot_clear_records(void *batch, int slot);
+
+/* Host-only hooks used by the tests (no GPU): build the tables of one basis,
+   read one table, run the hierarchy / state selection on given inputs. */
+void *hops_rt_debug_basis(void *sys, int eom, int n_states, const int32_t *states, int n_aux,
+                          const int32_t *aux_ptr, const int32_t *aux_mode, const int32_t *aux_val);
+long long hops_rt_debug_table(void *basis, const char *name, void *out, long long cap_bytes);
+int hops_rt_debug_select(void *basis, const hops_rt_traj_param_t *prm, const double *pn, const double *e_aux,
+                         double dt, uint8_t *stable_out);
+int hops_rt_debug_select_states(void *basis, const double *e_state);
+int hops_rt_debug_result(void *basis, int32_t *n_states, int32_t *states_out, int32_t *n_aux, int32_t *aux_ptr,
+                         int32_t *aux_mode, int32_t *aux_val, long long cap);
+int hops_rt_debug_free(void *basis);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

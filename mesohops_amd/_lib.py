@@ -16,7 +16,7 @@ HOPS_PREC_MIXED = 1
 HOPS_EOM_LINEAR = 0
 HOPS_EOM_NONLINEAR = 1
 HOPS_EOM_NORMALIZED_NONLINEAR = 2
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _i32 = ctypes.c_int32
 _vp = ctypes.c_void_p
@@ -30,7 +30,7 @@ _OP_PTRS = ["aux_ptr", "aux_mode", "aux_val",
             "lr_ptr", "lr_col", "lr_val", "lc_ptr", "lc_row", "lc_val",
             "mode_lrel", "mode_lact", "mode_abs", "mode_g", "mode_w",
             "abs_g", "abs_w", "abs_lact",
-            "act_lrel", "am_ptr", "am_mode",
+            "act_lrel", "act_zrow", "am_ptr", "am_mode",
             "la_ptr", "la_row", "la_col", "la_val",
             "lq_ptr", "lq_row", "lq_col", "lq_val", "lt",
             "first_aux", "first_lact", "first_mode",
@@ -50,6 +50,23 @@ class HopsBatch(ctypes.Structure):
                 ("z1_d", _vp), ("z2_d", _vp), ("zmem_d", _vp), ("ws_d", _vp), ("coef_d", _vp),
                 ("psi_store_d", _vp), ("tile_d", _vp), ("n_tiles", _i32), ("tile_rows", _i32),
                 ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("pad2", _i32)]
+
+
+class RtSystemDesc(ctypes.Structure):
+    """hops_rt_system_desc_t (include/hops_runtime.h)."""
+    _fields_ = [("n_state", _i32), ("n_hmode", _i32), ("n_lop", _i32), ("pad", _i32),
+                ("H", _vp), ("g", _vp), ("w", _vp), ("l_of_mode", _vp), ("l_ptr", _vp),
+                ("l_row", _vp), ("l_col", _vp), ("l_val", _vp), ("lt", _vp)]
+
+
+class RtTrajParam(ctypes.Structure):
+    """hops_rt_traj_param_t (include/hops_runtime.h)."""
+    _fields_ = [("eom", _i32), ("maxhier", _i32), ("update_step", _i32), ("early_integrator", _i32),
+                ("early_steps", _i32), ("inchworm_cap", _i32), ("noise_step", _i32), ("n_t", _i32),
+                ("it0", _i32), ("record", _i32), ("n_aux0", _i32), ("pad", _i32),
+                ("delta_a", ctypes.c_double), ("delta_s", ctypes.c_double), ("f_discard", ctypes.c_double),
+                ("dt", ctypes.c_double), ("markov", _vp), ("aux0_ptr", _vp), ("aux0_mode", _vp),
+                ("aux0_val", _vp)]
 
 
 _lib = None
@@ -76,6 +93,43 @@ def lib():
     L.hops_profile_enable.argtypes = [ctypes.c_int]
     L.hops_set_option.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.hops_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
+    _ll = ctypes.c_longlong
+    for name in ("hops_slots_step",):
+        getattr(L, name).argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int,
+                                     ctypes.c_int, _vp]
+    L.hops_slots_pn.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
+    L.hops_slots_errors.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int,
+                                    ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp]
+    L.hops_slots_build.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]
+    L.hops_slots_permute.argtypes = [_vp, _vp, ctypes.c_int, _vp]
+    L.hops_rt_system_create.restype = _vp
+    L.hops_rt_system_create.argtypes = [ctypes.POINTER(RtSystemDesc)]
+    L.hops_rt_system_destroy.argtypes = [_vp]
+    L.hops_rt_batch_create.restype = _vp
+    L.hops_rt_batch_create.argtypes = [ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _vp]
+    L.hops_rt_batch_destroy.argtypes = [_vp]
+    L.hops_rt_batch_add.argtypes = [_vp, _vp, ctypes.POINTER(RtTrajParam), _vp, _vp, _vp]
+    L.hops_rt_batch_remove.argtypes = [_vp, ctypes.c_int]
+    L.hops_rt_slot_estimate.restype = _ll
+    L.hops_rt_slot_estimate.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
+    L.hops_rt_arena_free.restype = _ll
+    L.hops_rt_arena_free.argtypes = [_vp]
+    L.hops_rt_batch_step.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int]
+    L.hops_rt_slot_info.argtypes = [_vp, ctypes.c_int, _vp]
+    L.hops_rt_slot_states.argtypes = [_vp, ctypes.c_int, _vp]
+    L.hops_rt_slot_aux.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
+    L.hops_rt_slot_phi.argtypes = [_vp, ctypes.c_int, _vp]
+    L.hops_rt_slot_zmem.argtypes = [_vp, ctypes.c_int, _vp]
+    L.hops_rt_slot_record.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]
+    L.hops_rt_slot_clear_records.argtypes = [_vp, ctypes.c_int]
+    L.hops_rt_debug_basis.restype = _vp
+    L.hops_rt_debug_basis.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]
+    L.hops_rt_debug_table.restype = _ll
+    L.hops_rt_debug_table.argtypes = [_vp, ctypes.c_char_p, _vp, _ll]
+    L.hops_rt_debug_select.argtypes = [_vp, ctypes.POINTER(RtTrajParam), _vp, _vp, ctypes.c_double, _vp]
+    L.hops_rt_debug_select_states.argtypes = [_vp, _vp]
+    L.hops_rt_debug_result.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ll]
+    L.hops_rt_debug_free.argtypes = [_vp]
     if L.hops_version() != ABI_VERSION:
         raise EngineError("libhops_mi355x.so ABI %d != %d; rebuild" % (L.hops_version(), ABI_VERSION))
     _lib = L
@@ -89,7 +143,14 @@ def check(rc):
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
             "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128",
-            "hops_noise_embedding", "hops_noise_filter"]
+            "hops_noise_embedding", "hops_noise_filter",
+            "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute",
+            "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
+            "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_estimate", "hops_rt_arena_free",
+            "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
+            "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
+            "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",
+            "hops_rt_debug_result", "hops_rt_debug_free"]
 
 
 def set_option(name, value):

@@ -325,7 +325,7 @@ class BasisTables:
             "mode_lrel": i32(mode_lrel), "mode_lact": i32(mode_lact), "mode_abs": i32(modes),
             "mode_g": cplx(info.G[modes]), "mode_w": cplx(info.W[modes]),
             "abs_g": cplx(info.G), "abs_w": cplx(info.W), "abs_lact": i32(abs_lact),
-            "act_lrel": i32(act_lrel), "am_ptr": i32(am_ptr), "am_mode": i32(am_mode),
+            "act_lrel": i32(act_lrel), "act_zrow": None, "am_ptr": i32(am_ptr), "am_mode": i32(am_mode),
             "la_ptr": i32(la_ptr), "la_row": i32(la_row), "la_col": i32(la_col), "la_val": cplx(la_val),
             "lq_ptr": i32(lq_ptr), "lq_row": i32(lq_row), "lq_col": i32(lq_col), "lq_val": cplx(lq_val),
             "lt": cplx(lt) if self.has_lt else None,

@@ -7,7 +7,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("hops_engine.hip", "noise_fft.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("hops_engine.hip", "noise_fft.hip", "hops_runtime.cpp")]
 OUT = os.path.join(HERE, "libhops_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",

@@ -163,11 +163,10 @@ __host__ __device__ __forceinline__ Lane lane_of(const hops_batch_t &bt, long nr
 // Km1[(k,s),(k-e_m,s_r)] = (k_m w_m) conj(L_m[s_r,s]).
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_build_ell(const hops_op_t op) {
+__device__ __forceinline__ void build_ell_row(const hops_op_t &op, const long r) {
 #pragma clang fp contract(off)
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
   const int k = (int)(r / n), s = (int)(r - (long)k * n);
   cx<R> *val = reinterpret_cast<cx<R> *>(op.ell_val_d);
@@ -209,9 +208,8 @@ __global__ __launch_bounds__(256) void k_build_ell(const hops_op_t op) {
 
 // K0[k] = -sum_m k_m W_m over the absolute modes of aux k
 // (eom/eom_hops_ksuper.py:36-90).
-__global__ __launch_bounds__(256) void k_build_k0(const hops_op_t op) {
+__device__ __forceinline__ void build_k0_aux(const hops_op_t &op, const int k) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= op.n_hier) return;
   double sx = 0.0, sy = 0.0;
   for (int q = op.aux_ptr_d[k]; q < op.aux_ptr_d[k + 1]; ++q) {
@@ -222,6 +220,15 @@ __global__ __launch_bounds__(256) void k_build_k0(const hops_op_t op) {
   }
   op.k0_d[2 * k] = -sx;
   op.k0_d[2 * k + 1] = -sy;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_build_ell(const hops_op_t op) {
+  build_ell_row<R>(op, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_build_k0(const hops_op_t op) {
+  build_k0_aux(op, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,16 +258,16 @@ __constant__ double c_rk[4] = {0.0, 0.5, 0.5, 1.0};
 // Stage prologue: eom/hops_eom.py:329-441 (noise terms, <L>, norm_corr, LTC),
 // eom/eom_functions.py:11-309, integrator/integrator_rk.py:53-64 (z_mem stage
 // values, noise index map [0,1,1,2]).
+// One workgroup per trajectory b; stage < 0 evaluates the derivative at yin
+// with the current z-memory and no RK bookkeeping (adaptive error estimate).
 template <typename R>
-__global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops_batch_t bt,
-                                                  int stage, int tidx, const cx<R> *__restrict__ yin,
-                                                  int store_slot) {
-  // stage < 0: evaluate the derivative at yin with the current z-memory and
-  // no RK bookkeeping (used by the adaptive error estimate)
+__device__ __forceinline__ void prologue_body(const hops_op_t &op, const hops_batch_t &bt, const int b,
+                                              int stage, int tidx, const cx<R> *__restrict__ yin,
+                                              int store_slot) {
   const bool eval = stage < 0;
   extern __shared__ double sm[];
   const int n = op.n_state, na = op.n_act, Ma = op.n_mode_abs;
-  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const long nr = (long)n * op.n_hier;
   const Lane ln = lane_of(bt, nr, b);
   const CoefLayout L = coef_layout(op);
@@ -390,7 +397,8 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
   const float2 *z1 = reinterpret_cast<const float2 *>(bt.z1_d) + (long)b * na * bt.n_t;
   const float2 *z2 = bt.z2_d ? reinterpret_cast<const float2 *>(bt.z2_d) + (long)b * na * bt.n_t : nullptr;
   for (int j = tid; j < na; j += nt) {
-    const float2 v1 = z1[(long)j * bt.n_t + tidx];
+    const long zr = op.act_zrow_d ? op.act_zrow_d[j] : j;
+    const float2 v1 = z1[zr * bt.n_t + tidx];
     zd zz = mk<double>((double)v1.x, -(double)v1.y);
     if (nonlin) {
       for (int q = op.am_ptr_d[j]; q < op.am_ptr_d[j + 1]; ++q) {
@@ -403,7 +411,7 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
     zh[2 * j + 1] = zz.y;
     zd c = zz;
     if (z2) {
-      const float2 v2 = z2[(long)j * bt.n_t + tidx];
+      const float2 v2 = z2[zr * bt.n_t + tidx];
       if (nonlin) {
         c.x += (double)v2.y;
         c.y -= (double)v2.x;
@@ -512,6 +520,13 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
     coef[L.nc + 1] = nc.y;
     cT[(long)(2 * op.n_pat + op.n_mode) * cs + b] = ccast<R>(nc);
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops_batch_t bt,
+                                                  int stage, int tidx, const cx<R> *__restrict__ yin,
+                                                  int store_slot) {
+  prologue_body<R>(op, bt, blockIdx.x, stage, tidx, yin, store_slot);
 }
 
 // Derivative of row r = (k, s) of trajectory b fused with its RK4 stage
@@ -984,6 +999,288 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Ragged batch ("slots").  Every slot is one trajectory with its own basis:
+// its own hops_op_t and a B = 1, lane_shift = 0 hops_batch_t, both in device
+// arrays indexed by slot.  One launch covers every listed slot; a tile is 256
+// rows of one slot, encoded slot << kSlotShift | tile.
+// ---------------------------------------------------------------------------
+constexpr int kSlotShift = 20;
+constexpr uint32_t kTileMask = (1u << kSlotShift) - 1;
+
+// stage input / output pools (stage < 0: derivative evaluation ya -> yb)
+template <typename R>
+__device__ __forceinline__ const cx<R> *slot_in(const hops_batch_t &bt, int stage) {
+  if (stage < 0) return reinterpret_cast<const cx<R> *>(bt.ya_d);
+  if (stage == 0)
+    return reinterpret_cast<const cx<R> *>(bt.precision == HOPS_PREC_MIXED ? bt.y0_d : bt.phi_d);
+  return reinterpret_cast<const cx<R> *>(stage == 2 ? bt.yb_d : bt.ya_d);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> *slot_out(const hops_batch_t &bt, int stage) {
+  if (stage < 0) return reinterpret_cast<cx<R> *>(bt.yb_d);
+  return reinterpret_cast<cx<R> *>(stage == 1 ? bt.yb_d : bt.ya_d);
+}
+
+__constant__ int c_toff[4] = {0, 1, 1, 2};
+
+// it[slot]: noise index of the slot's current step start
+template <typename R>
+__global__ __launch_bounds__(256) void k_prologue_slots(const hops_op_t *__restrict__ ops,
+                                                        const hops_batch_t *__restrict__ bts,
+                                                        const int32_t *__restrict__ list,
+                                                        const int32_t *__restrict__ it, int stage) {
+  const int slot = list[blockIdx.x];
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  const int tidx = it[slot] + (stage < 0 ? 0 : c_toff[stage] * bt.noise_step);
+  prologue_body<R>(op, bt, 0, stage, tidx, slot_in<R>(bt, stage), -1);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_stage_slots(const hops_op_t *__restrict__ ops,
+                                                     const hops_batch_t *__restrict__ bts,
+                                                     const uint32_t *__restrict__ tiles, int stage) {
+  const uint32_t t = tiles[blockIdx.x];
+  const int slot = (int)(t >> kSlotShift);
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)(t & kTileMask) * 256 + threadIdx.x;
+  if (r >= nr) return;
+  const int k = (int)(r / n), s = (int)(r - (long)k * n);
+  stage_row<R, 0>(op, bt, stage, slot_in<R>(bt, stage), slot_out<R>(bt, stage), 0, r, k, s,
+                  ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
+}
+
+// complex64 copy of Phi: the stage-0 input of a mixed-precision step
+__global__ __launch_bounds__(256) void k_prime_slots(const hops_op_t *__restrict__ ops,
+                                                     const hops_batch_t *__restrict__ bts,
+                                                     const uint32_t *__restrict__ tiles) {
+  const uint32_t t = tiles[blockIdx.x];
+  const int slot = (int)(t >> kSlotShift);
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  const long nr = (long)op.n_state * op.n_hier;
+  const long r = (long)(t & kTileMask) * 256 + threadIdx.x;
+  if (r >= nr) return;
+  const double2 p = reinterpret_cast<const double2 *>(bt.phi_d)[r];
+  reinterpret_cast<float2 *>(bt.y0_d)[r] = make_float2((float)p.x, (float)p.y);
+}
+
+// end of a step: Phi <- Phi / |psi| for NORMALIZED NONLINEAR
+// (trajectory/hops_trajectory.py:616-634); one workgroup per slot
+__global__ __launch_bounds__(256) void k_final_slots(const hops_op_t *__restrict__ ops,
+                                                     const hops_batch_t *__restrict__ bts,
+                                                     const int32_t *__restrict__ list) {
+  __shared__ double red[16];
+  const int slot = list[blockIdx.x];
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  if (bt.eom != HOPS_EOM_NORMALIZED_NONLINEAR) return;  // uniform over the block
+  const int n = op.n_state;
+  double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
+  zd acc = mk<double>(0.0, 0.0);
+  for (int s = threadIdx.x; s < n; s += blockDim.x) acc.x += phi[s].x * phi[s].x + phi[s].y * phi[s].y;
+  acc = block_sum(acc, red);
+  const double sigma = 1.0 / sqrt(acc.x);
+  const long nr = (long)n * op.n_hier;
+  for (long i = threadIdx.x; i < nr; i += blockDim.x) {
+    double2 p = phi[i];
+    p.x *= sigma;
+    p.y *= sigma;
+    phi[i] = p;
+  }
+}
+
+// Pn = Phi / |psi|: into ya (stage precision, the derivative input) and in
+// FP64 into out + off[slot]; |psi| into nrm[slot] (basis/hops_basis.py:157).
+// psi_only: copy psi = Phi[:n_state] to out + off[slot] and nothing else.
+template <typename R>
+__global__ __launch_bounds__(256) void k_pn_slots(const hops_op_t *__restrict__ ops,
+                                                  const hops_batch_t *__restrict__ bts,
+                                                  const int32_t *__restrict__ list,
+                                                  const int64_t *__restrict__ off, double2 *__restrict__ out,
+                                                  double *__restrict__ nrm, int psi_only) {
+  __shared__ double red[16];
+  const int slot = list[blockIdx.x];
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  const int n = op.n_state;
+  const double2 *phi = reinterpret_cast<const double2 *>(bt.phi_d);
+  double2 *o = out + off[slot];
+  if (psi_only) {
+    for (int s = threadIdx.x; s < n; s += blockDim.x) o[s] = phi[s];
+    return;
+  }
+  zd acc = mk<double>(0.0, 0.0);
+  for (int s = threadIdx.x; s < n; s += blockDim.x) acc.x += phi[s].x * phi[s].x + phi[s].y * phi[s].y;
+  acc = block_sum(acc, red);
+  const double norm = sqrt(acc.x);
+  if (threadIdx.x == 0) nrm[slot] = norm;
+  cx<R> *ya = reinterpret_cast<cx<R> *>(bt.ya_d);
+  const long nr = (long)n * op.n_hier;
+  for (long i = threadIdx.x; i < nr; i += blockDim.x) {
+    const double2 p = phi[i];
+    double2 q;
+    q.x = p.x / norm;
+    q.y = p.y / norm;
+    o[i] = q;
+    ya[i] = mk<R>((R)q.x, (R)q.y);
+  }
+}
+
+// derivative input restricted to the stable auxiliaries: ya[k, s] = Pn[k, s]
+// if mask[k] else 0 (basis_functions_adaptive.py:10-74 with list_index_aux_stable)
+template <typename R>
+__global__ __launch_bounds__(256) void k_mask_slots(const hops_op_t *__restrict__ ops,
+                                                    const hops_batch_t *__restrict__ bts,
+                                                    const uint32_t *__restrict__ tiles,
+                                                    const int64_t *__restrict__ off,
+                                                    const double2 *__restrict__ pn,
+                                                    const int64_t *__restrict__ moff,
+                                                    const uint8_t *__restrict__ mask) {
+  const uint32_t t = tiles[blockIdx.x];
+  const int slot = (int)(t >> kSlotShift);
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  const int n = op.n_state;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)(t & kTileMask) * 256 + threadIdx.x;
+  if (r >= nr) return;
+  const int k = (int)(r / n);
+  const double2 q = pn[off[slot] + r];
+  const bool keep = mask[moff[slot] + k] != 0;
+  reinterpret_cast<cx<R> *>(bt.ya_d)[r] = keep ? mk<R>((R)q.x, (R)q.y) : mk<R>((R)0, (R)0);
+}
+
+// |D/hbar + P/dt|^2 summed over the states of each auxiliary (mode 0) or over
+// the masked auxiliaries of each state (mode 1); D = yb, P = Pn (masked)
+// (basis/hops_basis.py:560-665, :795-912, basis_functions_adaptive.py:10-74)
+template <typename R>
+__global__ __launch_bounds__(256) void k_err_slots(const hops_op_t *__restrict__ ops,
+                                                   const hops_batch_t *__restrict__ bts,
+                                                   const int32_t *__restrict__ list,
+                                                   const int64_t *__restrict__ off,
+                                                   const double2 *__restrict__ pn,
+                                                   const int64_t *__restrict__ eoff, double *__restrict__ eout,
+                                                   const double *__restrict__ dt, double hbar, int mode,
+                                                   const int64_t *__restrict__ moff,
+                                                   const uint8_t *__restrict__ mask) {
+#pragma clang fp contract(off)
+  const int slot = list[blockIdx.x];
+  const hops_op_t &op = ops[slot];
+  const hops_batch_t &bt = bts[slot];
+  const int n = op.n_state, nh = op.n_hier;
+  const cx<R> *d = reinterpret_cast<const cx<R> *>(bt.yb_d);
+  const double2 *p = pn + off[slot];
+  const double tau = dt[slot];
+  double *e = eout + eoff[slot];
+  if (mode == 0) {
+    for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+      double acc = 0.0;
+      for (int s = 0; s < n; ++s) {
+        const long r = (long)k * n + s;
+        const double vx = (double)d[r].x / hbar + p[r].x / tau;
+        const double vy = (double)d[r].y / hbar + p[r].y / tau;
+        acc += vx * vx + vy * vy;
+      }
+      e[k] = acc;
+    }
+  } else {
+    const uint8_t *m = mask + moff[slot];
+    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+      double acc = 0.0;
+      for (int k = 0; k < nh; ++k) {
+        if (!m[k]) continue;
+        const long r = (long)k * n + s;
+        const double vx = (double)d[r].x / hbar + p[r].x / tau;
+        const double vy = (double)d[r].y / hbar + p[r].y / tau;
+        acc += vx * vx + vy * vy;
+      }
+      e[s] = acc;
+    }
+  }
+}
+
+// super-operator of listed slots (tiles over their rows / one block per slot)
+template <typename R>
+__global__ __launch_bounds__(256) void k_build_ell_slots(const hops_op_t *__restrict__ ops,
+                                                         const uint32_t *__restrict__ tiles) {
+  const uint32_t t = tiles[blockIdx.x];
+  build_ell_row<R>(ops[t >> kSlotShift], (long)(t & kTileMask) * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(256) void k_build_k0_slots(const hops_op_t *__restrict__ ops,
+                                                        const int32_t *__restrict__ list) {
+  const hops_op_t &op = ops[list[blockIdx.x]];
+  for (int k = threadIdx.x; k < op.n_hier; k += blockDim.x) build_k0_aux(op, k);
+}
+
+// Phi of a new basis (hops_basis.py:214-274): new row (j, s') takes
+// scale * old[(inv_aux[j], inv_state[s'])] when both exist, else 0; tiles run
+// over the new rows of each listed slot.
+__global__ __launch_bounds__(256) void k_permute_slots(const hops_slot_perm_t *__restrict__ pd,
+                                                       const uint32_t *__restrict__ tiles) {
+  const uint32_t t = tiles[blockIdx.x];
+  const hops_slot_perm_t &p = pd[t >> kSlotShift];
+  const long nr = (long)p.n_new * p.nh_new;
+  const long r = (long)(t & kTileMask) * 256 + threadIdx.x;
+  if (r >= nr) return;
+  const int j = (int)(r / p.n_new), s = (int)(r - (long)j * p.n_new);
+  const int i = p.inv_aux_d[j], so = p.inv_state_d[s];
+  double2 v = make_double2(0.0, 0.0);
+  if (i >= 0 && so >= 0) {
+    const double2 q = reinterpret_cast<const double2 *>(p.old_phi_d)[(long)i * p.n_old + so];
+    v.x = p.scale * q.x;
+    v.y = p.scale * q.y;
+  }
+  reinterpret_cast<double2 *>(p.new_phi_d)[r] = v;
+}
+
+template <typename R>
+int slots_step_t(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *list, int n_list,
+                 const uint32_t *tiles, int n_tiles, const int32_t *it, size_t shm, bool mixed,
+                 hipStream_t st) {
+  if (shm > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_prologue_slots<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+  if (mixed) {
+    k_prime_slots<<<n_tiles, 256, 0, st>>>(ops, bts, tiles);
+    HIP_TRY(hipGetLastError());
+  }
+  for (int i = 0; i < 4; ++i) {
+    k_prologue_slots<R><<<n_list, 256, shm, st>>>(ops, bts, list, it, i);
+    HIP_TRY(hipGetLastError());
+    k_stage_slots<R><<<n_tiles, 256, 0, st>>>(ops, bts, tiles, i);
+    HIP_TRY(hipGetLastError());
+  }
+  k_final_slots<<<n_list, 256, 0, st>>>(ops, bts, list);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int slots_deriv_t(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *list, int n_list,
+                  const uint32_t *tiles, int n_tiles, const int32_t *it, size_t shm, hipStream_t st) {
+  if (shm > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_prologue_slots<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+  k_prologue_slots<R><<<n_list, 256, shm, st>>>(ops, bts, list, it, -1);
+  HIP_TRY(hipGetLastError());
+  k_stage_slots<R><<<n_tiles, 256, 0, st>>>(ops, bts, tiles, -1);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int slots_args_ok(const void *ops, const void *bts, const void *list, int n_list, const void *tiles,
+                  int n_tiles, size_t shm_doubles) {
+  if (!ops || !bts || !list || !tiles) return fail("slots: NULL argument");
+  if (n_list <= 0 || n_tiles <= 0) return fail("slots: empty slot or tile list");
+  if (shm_doubles * sizeof(double) > 160 * 1024) return fail("slots: prologue LDS exceeds 160 KiB");
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -996,7 +1293,7 @@ int hops_internal_set_error(const char *msg) {
   return -1;
 }
 
-int hops_version(void) { return 1; }
+int hops_version(void) { return 2; }
 
 int hops_profile_enable(int on) {
   g_prof.on = on != 0;
@@ -1097,6 +1394,101 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_s
   if (bt->precision == HOPS_PREC_FP64) return propagate_t<double>(op, bt, it0, n_steps, store0, st);
   if (bt->precision == HOPS_PREC_MIXED) return propagate_t<float>(op, bt, it0, n_steps, store0, st);
   return fail("unknown precision %d", bt->precision);
+}
+
+
+int hops_slots_step(const hops_op_t *ops_d, const hops_batch_t *bts_d, const int32_t *list_d, int n_list,
+                    const uint32_t *tiles_d, int n_tiles, const int32_t *it_d, int precision,
+                    int shm_doubles, void *stream) {
+  if (int rc = slots_args_ok(ops_d, bts_d, list_d, n_list, tiles_d, n_tiles, (size_t)shm_doubles)) return rc;
+  if (!it_d) return fail("hops_slots_step: NULL noise index array");
+  const size_t shm = sizeof(double) * (size_t)shm_doubles;
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == HOPS_PREC_FP64)
+    return slots_step_t<double>(ops_d, bts_d, list_d, n_list, tiles_d, n_tiles, it_d, shm, false, st);
+  if (precision == HOPS_PREC_MIXED)
+    return slots_step_t<float>(ops_d, bts_d, list_d, n_list, tiles_d, n_tiles, it_d, shm, true, st);
+  return fail("unknown precision %d", precision);
+}
+
+int hops_slots_pn(const hops_op_t *ops_d, const hops_batch_t *bts_d, const int32_t *list_d, int n_list,
+                  const int64_t *off_d, void *out_d, double *nrm_d, int precision, int psi_only,
+                  void *stream) {
+  if (!ops_d || !bts_d || !list_d || !off_d || !out_d || (!psi_only && !nrm_d))
+    return fail("hops_slots_pn: NULL argument");
+  if (n_list <= 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  double2 *out = reinterpret_cast<double2 *>(out_d);
+  if (precision == HOPS_PREC_FP64)
+    k_pn_slots<double><<<n_list, 256, 0, st>>>(ops_d, bts_d, list_d, off_d, out, nrm_d, psi_only);
+  else if (precision == HOPS_PREC_MIXED)
+    k_pn_slots<float><<<n_list, 256, 0, st>>>(ops_d, bts_d, list_d, off_d, out, nrm_d, psi_only);
+  else
+    return fail("unknown precision %d", precision);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_slots_errors(const hops_op_t *ops_d, const hops_batch_t *bts_d, const int32_t *list_d, int n_list,
+                      const uint32_t *tiles_d, int n_tiles, const int32_t *it_d, int precision,
+                      int shm_doubles, const int64_t *off_d, const void *pn_d, const int64_t *moff_d,
+                      const uint8_t *mask_d, const int64_t *eoff_d, double *e_d, const double *dt_d,
+                      double hbar, void *stream) {
+  if (int rc = slots_args_ok(ops_d, bts_d, list_d, n_list, tiles_d, n_tiles, (size_t)shm_doubles)) return rc;
+  if (!it_d || !off_d || !pn_d || !eoff_d || !e_d || !dt_d) return fail("hops_slots_errors: NULL argument");
+  if ((moff_d == nullptr) != (mask_d == nullptr)) return fail("hops_slots_errors: mask without offsets");
+  const size_t shm = sizeof(double) * (size_t)shm_doubles;
+  hipStream_t st = (hipStream_t)stream;
+  const double2 *pn = reinterpret_cast<const double2 *>(pn_d);
+  const int mode = mask_d ? 1 : 0;
+  if (precision == HOPS_PREC_FP64) {
+    if (mask_d) {
+      k_mask_slots<double><<<n_tiles, 256, 0, st>>>(ops_d, bts_d, tiles_d, off_d, pn, moff_d, mask_d);
+      HIP_TRY(hipGetLastError());
+    }
+    if (int rc = slots_deriv_t<double>(ops_d, bts_d, list_d, n_list, tiles_d, n_tiles, it_d, shm, st)) return rc;
+    k_err_slots<double><<<n_list, 256, 0, st>>>(ops_d, bts_d, list_d, off_d, pn, eoff_d, e_d, dt_d, hbar, mode,
+                                                moff_d, mask_d);
+  } else if (precision == HOPS_PREC_MIXED) {
+    if (mask_d) {
+      k_mask_slots<float><<<n_tiles, 256, 0, st>>>(ops_d, bts_d, tiles_d, off_d, pn, moff_d, mask_d);
+      HIP_TRY(hipGetLastError());
+    }
+    if (int rc = slots_deriv_t<float>(ops_d, bts_d, list_d, n_list, tiles_d, n_tiles, it_d, shm, st)) return rc;
+    k_err_slots<float><<<n_list, 256, 0, st>>>(ops_d, bts_d, list_d, off_d, pn, eoff_d, e_d, dt_d, hbar, mode,
+                                               moff_d, mask_d);
+  } else {
+    return fail("unknown precision %d", precision);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_slots_build(const hops_op_t *ops_d, const int32_t *list_d, int n_list, const uint32_t *tiles_d,
+                     int n_tiles, int precision, void *stream) {
+  if (!ops_d || !list_d || !tiles_d) return fail("hops_slots_build: NULL argument");
+  if (n_list <= 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_tiles > 0) {
+    if (precision == HOPS_PREC_FP64)
+      k_build_ell_slots<double><<<n_tiles, 256, 0, st>>>(ops_d, tiles_d);
+    else if (precision == HOPS_PREC_MIXED)
+      k_build_ell_slots<float><<<n_tiles, 256, 0, st>>>(ops_d, tiles_d);
+    else
+      return fail("unknown precision %d", precision);
+    HIP_TRY(hipGetLastError());
+  }
+  k_build_k0_slots<<<n_list, 256, 0, st>>>(ops_d, list_d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_slots_permute(const hops_slot_perm_t *perm_d, const uint32_t *tiles_d, int n_tiles, void *stream) {
+  if (n_tiles <= 0) return 0;
+  if (!perm_d || !tiles_d) return fail("hops_slots_permute: NULL argument");
+  k_permute_slots<<<n_tiles, 256, 0, (hipStream_t)stream>>>(perm_d, tiles_d);
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 }  // extern "C"

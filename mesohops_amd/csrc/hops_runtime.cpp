@@ -1,0 +1,2214 @@
+// hops_runtime.cpp — native host runtime for ragged batches of adaptive HOPS
+// trajectories on MI355X (include/hops_runtime.h).
+//
+// Per batch step the GPU runs the RK4 step of every slot, the derivative of
+// the error bound and its per-auxiliary / per-state sums (slot kernels of
+// hops_engine.hip); the host worker threads run the set logic of the basis
+// update for every slot in parallel (thresholds, boundary auxiliaries, state
+// basis, connection tables of the new basis); the GPU then builds the new
+// operators and moves every Phi into its new basis in one launch each.
+//
+// Restated from the reference (MesoHOPS v1.6.0, src/mesohops/):
+//   basis/hops_basis.py:117-292      define_basis / update_basis
+//   basis/hops_basis.py:294-998      hierarchy and state error bounds
+//   basis/hops_basis.py:1000-1151    basis from error list, M2/X2/K2 arrays
+//   basis/basis_functions.py:8-41    error threshold
+//   basis/hops_fluxfilters.py        flux filters (incl. Markovian)
+//   basis/hops_hierarchy.py, hops_aux.py, hops_system.py, hops_modes.py
+//                                    auxiliary order, connections, relative
+//                                    indexing of states, modes, L-operators
+//   trajectory/hops_trajectory.py:253-336, :401-533, :636-714
+//                                    initial basis, time loop, INCH_WORM
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <complex>
+#include <condition_variable>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <type_traits>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hops_mi355x.h"
+#include "../../include/hops_runtime.h"
+
+extern "C" int hops_internal_set_error(const char *msg);
+
+namespace {
+
+using cplx = std::complex<double>;
+constexpr double kHbar = 33357.179 / (2.0 * 3.14159265358979323846);  // util/physical_constants.py:21-28
+constexpr int kSlotShift = 20;
+
+int rt_fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return hops_internal_set_error(buf);
+}
+
+#define RT_HIP(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) return rt_fail("%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+#define RT_TRY(expr)           \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ < 0) return rc_;   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Auxiliary multi-index: packed (mode << 8 | depth) entries sorted by mode.
+// ---------------------------------------------------------------------------
+typedef std::u32string Aux;
+
+inline int aux_mode(char32_t e) { return (int)(e >> 8); }
+inline int aux_val(char32_t e) { return (int)(e & 0xff); }
+inline char32_t aux_pack(int m, int v) { return (char32_t)(((uint32_t)m << 8) | (uint32_t)v); }
+
+int aux_depth(const Aux &a) {
+  int d = 0;
+  for (char32_t e : a) d += aux_val(e);
+  return d;
+}
+int aux_get(const Aux &a, int m) {
+  for (char32_t e : a)
+    if (aux_mode(e) == m) return aux_val(e);
+  return 0;
+}
+// a + step e_m (step = +-1); returns false if the depth would go negative
+bool aux_step(const Aux &a, int m, int step, Aux &out) {
+  out.clear();
+  bool done = false;
+  for (char32_t e : a) {
+    const int mm = aux_mode(e);
+    if (!done && mm > m) {
+      if (step < 0) return false;
+      out.push_back(aux_pack(m, 1));
+      done = true;
+    }
+    if (mm == m) {
+      const int v = aux_val(e) + step;
+      if (v > 0) out.push_back(aux_pack(m, v));
+      done = true;
+    } else {
+      out.push_back(e);
+    }
+  }
+  if (!done) {
+    if (step < 0) return false;
+    out.push_back(aux_pack(m, 1));
+  }
+  return true;
+}
+// Reference order (basis/hops_aux.py:171-201, :460-480): depth, then the
+// sorted mode sequence (mode m repeated k_m times) lexicographically.
+bool aux_less(const Aux &a, const Aux &b) {
+  const int da = aux_depth(a), db = aux_depth(b);
+  if (da != db) return da < db;
+  size_t ia = 0, ib = 0;
+  int ra = a.empty() ? 0 : aux_val(a[0]), rb = b.empty() ? 0 : aux_val(b[0]);
+  while (ia < a.size() && ib < b.size()) {
+    const int ma = aux_mode(a[ia]), mb = aux_mode(b[ib]);
+    if (ma != mb) return ma < mb;
+    if (--ra == 0 && ++ia < a.size()) ra = aux_val(a[ia]);
+    if (--rb == 0 && ++ib < b.size()) rb = aux_val(b[ib]);
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// System (basis/system_functions.py:10-171)
+// ---------------------------------------------------------------------------
+struct LOp {
+  std::vector<int> row, col;
+  std::vector<cplx> val;
+};
+
+struct System {
+  int n = 0, n_hm = 0, n_l = 0;
+  std::vector<cplx> H;  // n x n
+  std::vector<cplx> G, W;
+  std::vector<int> l_of_mode;
+  std::vector<LOp> L;
+  std::vector<cplx> lt;
+  std::vector<std::vector<int>> modes_by_state, l_by_state, coupled, dest;
+  std::vector<int> site_of_mode;  // the single state of a site-projector mode, or -1
+  bool diag_L = true;
+  std::vector<uint64_t> zob;      // Zobrist keys [n_hm * 256]
+};
+
+// ---------------------------------------------------------------------------
+// Tables of one basis (hops_op_t, include/hops_mi355x.h) — the same arrays,
+// values and order as mesohops_amd/basis.py::BasisTables.
+// ---------------------------------------------------------------------------
+struct RedL {  // an L-operator reduced to the state list: CSR and CSC
+  std::vector<int32_t> rp, ci, cp, ri;
+  std::vector<cplx> rv, cv;
+};
+
+struct Basis {
+  const System *sys = nullptr;
+  int eom = 2;
+  std::vector<int> states;
+  std::vector<Aux> aux;
+  std::unordered_map<Aux, int> index;
+  int n = 0, nh = 0, nm = 0, nl = 0, na = 0, n_pat = 0, pat_w = 0, ell_w = 0;
+  bool has_lt = false;
+  std::vector<int> modes, rel_mode, absL, mode_lrel, mode_lact, act_abs, act_lrel, pos;
+  std::vector<RedL> L;
+  std::vector<uint64_t> aux_hash;
+  std::vector<int> depth;
+  // device tables
+  std::vector<int32_t> aux_ptr, aux_mode_t, aux_val_t, up_ptr, up_aux, up_mode, dn_ptr, dn_aux, dn_mode, dn_val,
+      lr_ptr, lr_col, lc_ptr, lc_row, mode_lrel32, mode_lact32, mode_abs32, abs_lact, act_lrel32, act_zrow,
+      am_ptr, am_mode, la_ptr, la_row, la_col, lq_ptr, lq_row, lq_col, first_aux, first_lact, first_mode,
+      pat_ptr, pat_col, pc_ptr, pc_act, pc_kind;
+  std::vector<cplx> lr_val, lc_val, mode_g, mode_w, abs_g, abs_w, la_val, lq_val, lt, pat_h, pc_val;
+};
+
+uint64_t aux_hash_of(const System &S, const Aux &a) {
+  uint64_t h = 0;
+  for (char32_t e : a) h ^= S.zob[(size_t)aux_mode(e) * 256 + aux_val(e)];
+  return h;
+}
+
+int build_basis(const System &S, int eom, std::vector<int> states, std::vector<Aux> aux, Basis &B) {
+  B = Basis();
+  B.sys = &S;
+  B.eom = eom;
+  std::sort(states.begin(), states.end());
+  states.erase(std::unique(states.begin(), states.end()), states.end());
+  std::sort(aux.begin(), aux.end(), aux_less);
+  aux.erase(std::unique(aux.begin(), aux.end()), aux.end());
+  if (aux.empty() || !aux[0].empty()) return rt_fail("Zero Vector not contained in list_aux!");
+  if (states.empty()) return rt_fail("empty state basis");
+  B.states = states;
+  B.aux = aux;
+  const int n = (int)states.size(), nh = (int)aux.size();
+  B.n = n;
+  B.nh = nh;
+  B.index.reserve((size_t)nh * 2);
+  for (int i = 0; i < nh; ++i) B.index.emplace(aux[i], i);
+  B.aux_hash.resize(nh);
+  B.depth.resize(nh);
+  for (int i = 0; i < nh; ++i) {
+    B.aux_hash[i] = aux_hash_of(S, aux[i]);
+    B.depth[i] = aux_depth(aux[i]);
+  }
+  // modes: in use by the aux list, or of the states (hops_modes.py:111-171)
+  std::vector<char> mflag(S.n_hm, 0);
+  for (const Aux &a : aux)
+    for (char32_t e : a) {
+      if (aux_mode(e) >= S.n_hm) return rt_fail("auxiliary mode %d out of range", aux_mode(e));
+      mflag[aux_mode(e)] = 1;
+    }
+  std::vector<char> in_use = mflag;
+  for (int s : states) {
+    if (s < 0 || s >= S.n) return rt_fail("state %d out of range", s);
+    for (int m : S.modes_by_state[s]) mflag[m] = 1;
+  }
+  B.rel_mode.assign(S.n_hm, -1);
+  for (int m = 0; m < S.n_hm; ++m)
+    if (mflag[m]) {
+      B.rel_mode[m] = (int)B.modes.size();
+      B.modes.push_back(m);
+    }
+  const int nm = (int)B.modes.size();
+  B.nm = nm;
+  std::vector<int> relL(S.n_l, -1);
+  {
+    std::vector<char> lf(S.n_l, 0);
+    for (int m : B.modes) lf[S.l_of_mode[m]] = 1;
+    for (int l = 0; l < S.n_l; ++l)
+      if (lf[l]) {
+        relL[l] = (int)B.absL.size();
+        B.absL.push_back(l);
+      }
+  }
+  const int nl = (int)B.absL.size();
+  B.nl = nl;
+  for (int m : B.modes) B.mode_lrel.push_back(relL[S.l_of_mode[m]]);
+  {
+    std::vector<char> af(S.n_l, 0);
+    for (int s : states)
+      for (int l : S.l_by_state[s]) af[l] = 1;
+    for (int l = 0; l < S.n_l; ++l)
+      if (af[l]) {
+        if (relL[l] < 0) return rt_fail("an active L-operator without a hierarchy mode");
+        B.act_abs.push_back(l);
+      }
+  }
+  const int na = (int)B.act_abs.size();
+  B.na = na;
+  std::vector<int> lact_of_lrel(nl, -1);
+  for (int j = 0; j < na; ++j) {
+    B.act_lrel.push_back(relL[B.act_abs[j]]);
+    lact_of_lrel[relL[B.act_abs[j]]] = j;
+  }
+  for (int i = 0; i < nm; ++i) B.mode_lact.push_back(lact_of_lrel[B.mode_lrel[i]]);
+
+  // reduced L-operators (hops_system.py:318-354): duplicates summed, sorted
+  B.pos.assign(S.n, -1);
+  for (int i = 0; i < n; ++i) B.pos[states[i]] = i;
+  B.L.resize(nl);
+  for (int j = 0; j < nl; ++j) {
+    const LOp &op = S.L[B.absL[j]];
+    std::map<std::pair<int, int>, cplx> ent;
+    for (size_t q = 0; q < op.row.size(); ++q) {
+      const int r = B.pos[op.row[q]], c = B.pos[op.col[q]];
+      if (r >= 0 && c >= 0) ent[{r, c}] += op.val[q];
+    }
+    RedL &R = B.L[j];
+    R.rp.assign(n + 1, 0);
+    R.cp.assign(n + 1, 0);
+    for (auto &kv : ent) {
+      R.rp[kv.first.first + 1]++;
+      R.cp[kv.first.second + 1]++;
+    }
+    for (int i = 0; i < n; ++i) {
+      R.rp[i + 1] += R.rp[i];
+      R.cp[i + 1] += R.cp[i];
+    }
+    R.ci.resize(ent.size());
+    R.rv.resize(ent.size());
+    R.ri.resize(ent.size());
+    R.cv.resize(ent.size());
+    std::vector<int> rf(R.rp.begin(), R.rp.end() - 1), cf(R.cp.begin(), R.cp.end() - 1);
+    for (auto &kv : ent) {  // map order: row-major, columns ascending
+      const int r = kv.first.first, c = kv.first.second;
+      R.ci[rf[r]] = c;
+      R.rv[rf[r]++] = kv.second;
+    }
+    std::vector<std::pair<std::pair<int, int>, cplx>> bycol(ent.begin(), ent.end());
+    std::stable_sort(bycol.begin(), bycol.end(), [](const auto &x, const auto &y) {
+      return x.first.second != y.first.second ? x.first.second < y.first.second : x.first.first < y.first.first;
+    });
+    for (auto &kv : bycol) {
+      const int r = kv.first.first, c = kv.first.second;
+      R.ri[cf[c]] = r;
+      R.cv[cf[c]++] = kv.second;
+    }
+  }
+  // aux CSR
+  B.aux_ptr.assign(nh + 1, 0);
+  for (int i = 0; i < nh; ++i) {
+    B.aux_ptr[i + 1] = B.aux_ptr[i] + (int)aux[i].size();
+    for (char32_t e : aux[i]) {
+      B.aux_mode_t.push_back(aux_mode(e));
+      B.aux_val_t.push_back(aux_val(e));
+    }
+  }
+  // connections (hops_hierarchy.py:470-523): j = i + e_m
+  std::vector<std::vector<std::pair<int, int>>> up(nh);
+  std::vector<std::vector<std::array<int, 3>>> dn(nh);
+  {
+    Aux tmp;
+    for (int j = 0; j < nh; ++j)
+      for (char32_t e : aux[j]) {
+        const int m = aux_mode(e), v = aux_val(e);
+        aux_step(aux[j], m, -1, tmp);
+        auto it = B.index.find(tmp);
+        if (it == B.index.end()) continue;
+        up[it->second].push_back({j, B.rel_mode[m]});
+        dn[j].push_back({it->second, B.rel_mode[m], v});
+      }
+  }
+  B.up_ptr.assign(nh + 1, 0);
+  B.dn_ptr.assign(nh + 1, 0);
+  for (int i = 0; i < nh; ++i) {
+    std::sort(up[i].begin(), up[i].end(), [](const auto &x, const auto &y) { return x.second < y.second; });
+    std::sort(dn[i].begin(), dn[i].end(), [](const auto &x, const auto &y) { return x[1] < y[1]; });
+    B.up_ptr[i + 1] = B.up_ptr[i] + (int)up[i].size();
+    B.dn_ptr[i + 1] = B.dn_ptr[i] + (int)dn[i].size();
+    for (auto &u : up[i]) {
+      B.up_aux.push_back(u.first);
+      B.up_mode.push_back(u.second);
+    }
+    for (auto &d : dn[i]) {
+      B.dn_aux.push_back(d[0]);
+      B.dn_mode.push_back(d[1]);
+      B.dn_val.push_back(d[2]);
+    }
+  }
+  // per relative L: row and column CSR tables with running bases
+  B.lr_ptr.assign((size_t)nl * (n + 1), 0);
+  B.lc_ptr.assign((size_t)nl * (n + 1), 0);
+  int base_r = 0, base_c = 0;
+  for (int j = 0; j < nl; ++j) {
+    const RedL &R = B.L[j];
+    for (int i = 0; i <= n; ++i) {
+      B.lr_ptr[(size_t)j * (n + 1) + i] = R.rp[i] + base_r;
+      B.lc_ptr[(size_t)j * (n + 1) + i] = R.cp[i] + base_c;
+    }
+    base_r += (int)R.ci.size();
+    base_c += (int)R.ri.size();
+    B.lr_col.insert(B.lr_col.end(), R.ci.begin(), R.ci.end());
+    B.lr_val.insert(B.lr_val.end(), R.rv.begin(), R.rv.end());
+    B.lc_row.insert(B.lc_row.end(), R.ri.begin(), R.ri.end());
+    B.lc_val.insert(B.lc_val.end(), R.cv.begin(), R.cv.end());
+  }
+  // ELL width: densest row of K2_kp1 + K2_km1
+  {
+    int w = 0;
+    for (int i = 0; i < nh; ++i)
+      for (int s = 0; s < n; ++s) {
+        int c = 0;
+        for (auto &u : up[i]) {
+          const RedL &R = B.L[B.mode_lrel[u.second]];
+          c += R.rp[s + 1] - R.rp[s];
+        }
+        for (auto &d : dn[i]) {
+          const RedL &R = B.L[B.mode_lrel[d[1]]];
+          c += R.cp[s + 1] - R.cp[s];
+        }
+        w = std::max(w, c);
+      }
+    B.ell_w = w;
+  }
+  // active L-operators: COO of L and L^2, LTC
+  for (int j = 0; j < na; ++j) {
+    const cplx c = S.lt.empty() ? cplx(0, 0) : S.lt[B.act_abs[j]];
+    B.lt.push_back(c);
+    if (c != cplx(0, 0)) B.has_lt = true;
+  }
+  std::vector<std::vector<std::array<int, 2>>> sq_idx(na);
+  std::vector<std::vector<cplx>> sq_val(na);
+  B.la_ptr.push_back(0);
+  B.lq_ptr.push_back(0);
+  for (int j = 0; j < na; ++j) {
+    const RedL &R = B.L[B.act_lrel[j]];
+    for (int r = 0; r < n; ++r)
+      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q) {
+        B.la_row.push_back(r);
+        B.la_col.push_back(R.ci[q]);
+        B.la_val.push_back(R.rv[q]);
+      }
+    B.la_ptr.push_back((int)B.la_row.size());
+    // L @ L, zero results dropped, row-major with sorted columns
+    for (int r = 0; r < n; ++r) {
+      std::map<int, cplx> acc;
+      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q) {
+        const int c = R.ci[q];
+        for (int q2 = R.rp[c]; q2 < R.rp[c + 1]; ++q2) acc[R.ci[q2]] += R.rv[q] * R.rv[q2];
+      }
+      for (auto &kv : acc)
+        if (kv.second != cplx(0, 0)) {
+          sq_idx[j].push_back({r, kv.first});
+          sq_val[j].push_back(kv.second);
+          B.lq_row.push_back(r);
+          B.lq_col.push_back(kv.first);
+          B.lq_val.push_back(kv.second);
+        }
+    }
+    B.lq_ptr.push_back((int)B.lq_row.size());
+  }
+  // modes per active L
+  {
+    std::vector<std::vector<int>> am(na);
+    for (int i = 0; i < nm; ++i)
+      if (B.mode_lact[i] >= 0) am[B.mode_lact[i]].push_back(i);
+    B.am_ptr.push_back(0);
+    for (int j = 0; j < na; ++j) {
+      B.am_mode.insert(B.am_mode.end(), am[j].begin(), am[j].end());
+      B.am_ptr.push_back((int)B.am_mode.size());
+    }
+  }
+  B.abs_lact.assign(S.n_hm, -2);
+  for (int i = 0; i < nm; ++i) B.abs_lact[B.modes[i]] = B.mode_lact[i];
+  for (int i = 0; i < nm; ++i) {
+    auto it = B.index.find(Aux(1, aux_pack(B.modes[i], 1)));
+    if (it != B.index.end() && B.mode_lact[i] >= 0) {
+      B.first_aux.push_back(it->second);
+      B.first_lact.push_back(B.mode_lact[i]);
+      B.first_mode.push_back(i);
+    }
+  }
+  // system-block pattern: H, active L, (LTC) L^2
+  std::vector<std::vector<int>> prow(n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c)
+      if (std::abs(S.H[(size_t)states[r] * S.n + states[c]]) > 0) prow[r].push_back(c);
+  for (int j = 0; j < na; ++j) {
+    const RedL &R = B.L[B.act_lrel[j]];
+    for (int r = 0; r < n; ++r)
+      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q)
+        if (std::abs(R.rv[q]) > 0) prow[r].push_back(R.ci[q]);
+    if (B.has_lt)
+      for (size_t q = 0; q < sq_idx[j].size(); ++q)
+        if (std::abs(sq_val[j][q]) > 0) prow[sq_idx[j][q][0]].push_back(sq_idx[j][q][1]);
+  }
+  B.pat_ptr.assign(n + 1, 0);
+  for (int r = 0; r < n; ++r) {
+    std::sort(prow[r].begin(), prow[r].end());
+    prow[r].erase(std::unique(prow[r].begin(), prow[r].end()), prow[r].end());
+    B.pat_ptr[r + 1] = B.pat_ptr[r] + (int)prow[r].size();
+    B.pat_w = std::max(B.pat_w, (int)prow[r].size());
+    for (int c : prow[r]) {
+      B.pat_col.push_back(c);
+      B.pat_h.push_back(cplx(0, -1) * S.H[(size_t)states[r] * S.n + states[c]]);
+    }
+  }
+  B.n_pat = (int)B.pat_col.size();
+  auto pidx = [&](int r, int c) {
+    const int *b = B.pat_col.data() + B.pat_ptr[r], *e = B.pat_col.data() + B.pat_ptr[r + 1];
+    return (int)(std::lower_bound(b, e, c) - B.pat_col.data());
+  };
+  std::vector<std::vector<std::tuple<int, int, cplx>>> contrib(B.n_pat);
+  for (int j = 0; j < na; ++j) {
+    const RedL &R = B.L[B.act_lrel[j]];
+    for (int r = 0; r < n; ++r)
+      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q) contrib[pidx(r, R.ci[q])].emplace_back(j, 0, R.rv[q]);
+    if (B.has_lt)
+      for (size_t q = 0; q < sq_idx[j].size(); ++q)
+        contrib[pidx(sq_idx[j][q][0], sq_idx[j][q][1])].emplace_back(j, 1, sq_val[j][q]);
+  }
+  B.pc_ptr.push_back(0);
+  for (auto &c : contrib) {
+    for (auto &t : c) {
+      B.pc_act.push_back(std::get<0>(t));
+      B.pc_kind.push_back(std::get<1>(t));
+      B.pc_val.push_back(std::get<2>(t));
+    }
+    B.pc_ptr.push_back((int)B.pc_act.size());
+  }
+  for (int i = 0; i < nm; ++i) {
+    B.mode_lrel32.push_back(B.mode_lrel[i]);
+    B.mode_lact32.push_back(B.mode_lact[i]);
+    B.mode_abs32.push_back(B.modes[i]);
+    B.mode_g.push_back(S.G[B.modes[i]]);
+    B.mode_w.push_back(S.W[B.modes[i]]);
+  }
+  B.abs_g = S.G;
+  B.abs_w = S.W;
+  for (int j = 0; j < na; ++j) {
+    B.act_lrel32.push_back(B.act_lrel[j]);
+    B.act_zrow.push_back(B.act_abs[j]);
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive selection (basis/hops_basis.py:117-182, :294-998)
+// ---------------------------------------------------------------------------
+struct Params {
+  int eom = 2, maxhier = 3, update_step = 1, early_integrator = 0, early_steps = 5, inchworm_cap = 5;
+  int noise_step = 1, n_t = 0, it0 = 0, record = 1;
+  double delta_a = 0, delta_s = 0, f_discard = 0, dt = 1;
+  std::vector<uint8_t> markov;
+  std::vector<Aux> aux0;
+};
+
+double error_threshold(const std::vector<double> &sorted, double max_err, double offset = 0.0) {
+  // basis/basis_functions.py:8-41
+  if (sorted.empty()) return 0.0;
+  double cum = offset;
+  for (size_t i = 0; i < sorted.size(); ++i) {
+    cum += sorted[i];
+    if (cum >= max_err) return i > 0 ? sorted[i - 1] : 0.0;
+  }
+  return sorted[0] + offset < max_err ? INFINITY : 0.0;
+}
+
+struct Selection {
+  std::vector<int> stable_idx;        // stable auxiliaries (sorted indices)
+  std::vector<uint8_t> stable_mask;   // [n_hier]
+  std::vector<Aux> aux_new, aux_bound;
+  std::vector<int> states_new;
+  // kept between the two halves
+  std::vector<double> absC2;          // [nh * n] |Pn|^2, aux-major
+  std::vector<cplx> X2;               // [nm]
+  std::vector<std::vector<std::pair<int, cplx>>> M2;  // per rel mode: (state, L[s,s])
+  std::vector<int> sc;
+};
+
+std::vector<int> list_sc(const System &S, const Basis &B) {
+  std::vector<char> in(S.n, 0), out(S.n, 0);
+  for (int s : B.states) in[s] = 1;
+  for (int s : B.states) {
+    for (int c : S.coupled[s]) out[c] = 1;
+    for (int c : S.dest[s]) out[c] = 1;
+  }
+  std::vector<int> r;
+  for (int s = 0; s < S.n; ++s)
+    if (out[s] && !in[s]) r.push_back(s);
+  return r;
+}
+
+void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
+  // hops_fluxfilters.py:354-397
+  F.assign((size_t)B.nm * B.nh, 1);
+  if (P.markov.empty()) return;
+  for (int lm = 0; lm < B.nm; ++lm) {
+    if (!P.markov[B.modes[lm]]) continue;
+    for (int k = 1; k < B.nh; ++k) F[(size_t)lm * B.nh + k] = 0;
+    auto it = B.index.find(Aux(1, aux_pack(B.modes[lm], 1)));
+    if (it != B.index.end())
+      for (int m2 = 0; m2 < B.nm; ++m2) F[(size_t)m2 * B.nh + it->second] = 0;
+  }
+}
+
+// hierarchy half: stable auxiliaries, boundary auxiliaries, new aux list
+void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *e_aux, Selection &out) {
+  const System &S = *B.sys;
+  const int n = B.n, nh = B.nh, nm = B.nm;
+  out = Selection();
+  out.absC2.resize((size_t)nh * n);
+  for (size_t i = 0; i < (size_t)nh * n; ++i) out.absC2[i] = std::norm(pn[i]);
+  // M2: diagonal entries of each mode's reduced L (basis/hops_basis.py:1030-1151)
+  out.M2.resize(nm);
+  for (int lm = 0; lm < nm; ++lm) {
+    const RedL &R = B.L[B.mode_lrel[lm]];
+    for (int s = 0; s < n; ++s)
+      for (int q = R.rp[s]; q < R.rp[s + 1]; ++q)
+        if (R.ci[q] == s) out.M2[lm].push_back({s, R.rv[q]});
+  }
+  // <L> over psi = Pn[:n]
+  cplx nrm = 0;
+  for (int s = 0; s < n; ++s) nrm += std::conj(pn[s]) * pn[s];
+  std::vector<cplx> avg(B.nl);
+  for (int j = 0; j < B.nl; ++j) {
+    const RedL &R = B.L[j];
+    cplx a = 0;
+    for (int r = 0; r < n; ++r) {
+      cplx lv = 0;
+      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q) lv += R.rv[q] * pn[R.ci[q]];
+      a += std::conj(pn[r]) * lv;
+    }
+    avg[j] = a / nrm;
+  }
+  out.X2.resize(nm);
+  for (int lm = 0; lm < nm; ++lm) out.X2[lm] = avg[B.mode_lrel[lm]];
+  out.sc = list_sc(S, B);
+  if (P.delta_a <= 0) {
+    out.aux_new = B.aux;
+    out.stable_idx.resize(nh);
+    for (int k = 0; k < nh; ++k) out.stable_idx[k] = k;
+    out.stable_mask.assign(nh, 1);
+    return;
+  }
+  const double h2 = kHbar * kHbar;
+  std::vector<double> err(e_aux, e_aux + nh);
+  // flux out of the state basis through H (basis_functions_adaptive.py:76-163)
+  if (!out.sc.empty()) {
+    for (int k = 0; k < nh; ++k) {
+      double acc = 0;
+      for (int sp : out.sc) {
+        cplx z = 0;
+        for (int s = 0; s < n; ++s) z += cplx(0, -1) * S.H[(size_t)sp * S.n + B.states[s]] * pn[(size_t)k * n + s];
+        z /= kHbar;
+        acc += std::norm(z);
+      }
+      err[k] += acc;
+    }
+  }
+  // flux up / down (basis_functions_adaptive.py:165-390)
+  std::vector<double> E_up((size_t)nm * nh, 0.0), E_dn((size_t)nm * nh, 0.0);
+  std::vector<uint8_t> F_up;
+  markov_up(P, B, F_up);
+  for (int lm = 0; lm < nm; ++lm) {
+    const int m = B.modes[lm];
+    const double aw = std::abs(S.W[m]);
+    const double gw2 = std::norm(S.G[m] / S.W[m]);
+    const cplx X = out.X2[lm];
+    for (int k = 0; k < nh; ++k) {
+      double p_up = 0, p_dn = 0;
+      const double *c2 = &out.absC2[(size_t)k * n];
+      // sum_s |M2[m,s]|^2 |C2[s,k]|^2 and sum_s |M2[m,s] - X|^2 |C2[s,k]|^2
+      for (auto &e : out.M2[lm]) {
+        p_up += std::norm(e.second) * c2[e.first];
+        p_dn += std::norm(e.second - X) * c2[e.first];
+      }
+      // states with M2 = 0 contribute |X|^2 |C2|^2
+      double zx = 0;
+      if (std::norm(X) != 0) {
+        zx = 0;
+        size_t q = 0;
+        for (int s = 0; s < n; ++s) {
+          if (q < out.M2[lm].size() && out.M2[lm][q].first == s) {
+            ++q;
+            continue;
+          }
+          zx += c2[s];
+        }
+        p_dn += std::norm(X) * zx;
+      }
+      const double kw = aw * (1.0 + aux_get(B.aux[k], m));
+      E_up[(size_t)lm * nh + k] = (kw * kw * p_up) / h2;
+      E_dn[(size_t)lm * nh + k] = (gw2 * p_dn) / h2;
+    }
+  }
+  for (int k = 0; k < nh; ++k) {
+    const bool top = B.depth[k] == P.maxhier;
+    double acc = 0;
+    for (int lm = 0; lm < nm; ++lm)
+      if (!top && F_up[(size_t)lm * nh + k]) acc += E_up[(size_t)lm * nh + k];
+    err[k] += acc;
+    acc = 0;
+    for (int lm = 0; lm < nm; ++lm)
+      if (aux_get(B.aux[k], B.modes[lm]) > 0) acc += E_dn[(size_t)lm * nh + k];
+    err[k] += acc;
+  }
+  // stable auxiliaries (hops_basis.py:1000-1028)
+  const double dsq = P.delta_a * P.delta_a;
+  std::vector<double> srt(err);
+  std::sort(srt.begin(), srt.end());
+  const double thr = error_threshold(srt, dsq / 2.0);
+  out.stable_mask.assign(nh, 0);
+  double unstable = 0;
+  std::vector<double> un;
+  for (int k = 0; k < nh; ++k) {
+    if (err[k] > thr) {
+      out.stable_idx.push_back(k);
+      out.stable_mask[k] = 1;
+    } else {
+      un.push_back(err[k]);
+    }
+  }
+  for (double v : un) unstable += v;
+  const double dbound = dsq - unstable;
+  // boundary auxiliaries (hops_basis.py:667-793)
+  std::vector<uint8_t> has_up((size_t)nm * nh, 0);
+  for (int k = 0; k < nh; ++k)
+    for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) has_up[(size_t)B.up_mode[u] * nh + k] = 1;
+  std::vector<uint8_t> has_dn((size_t)nm * nh, 0);
+  for (int k = 0; k < nh; ++k)
+    for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) has_dn[(size_t)B.dn_mode[u] * nh + k] = 1;
+  struct Cand {
+    int col, lm, dir;
+    double v;
+  };
+  std::vector<Cand> cand;
+  std::vector<double> nz;
+  for (size_t c = 0; c < out.stable_idx.size(); ++c) {
+    const int k = out.stable_idx[c];
+    for (int lm = 0; lm < nm; ++lm) {
+      const bool fu = B.depth[k] < P.maxhier && !has_up[(size_t)lm * nh + k] && F_up[(size_t)lm * nh + k];
+      const double vu = fu ? E_up[(size_t)lm * nh + k] : 0.0;
+      const bool fd = aux_get(B.aux[k], B.modes[lm]) > 0 && !has_dn[(size_t)lm * nh + k];
+      const double vd = fd ? E_dn[(size_t)lm * nh + k] : 0.0;
+      if (vu != 0) {
+        cand.push_back({(int)c, lm, +1, vu});
+        nz.push_back(vu);
+      }
+      if (vd != 0) {
+        cand.push_back({(int)c, lm, -1, vd});
+        nz.push_back(vd);
+      }
+    }
+  }
+  std::sort(nz.begin(), nz.end());
+  const double thr1 = error_threshold(nz, dbound * P.f_discard * P.f_discard);
+  double discarded = 0;
+  for (double v : nz)
+    if (v <= thr1) discarded += v;
+  // accumulate by target auxiliary in the reference's insertion order: stable
+  // column, then flux up by mode, then flux down by mode
+  std::unordered_map<uint64_t, std::vector<int>> by_hash;
+  std::vector<Aux> keys;
+  std::vector<double> vals;
+  Aux tgt;
+  std::stable_sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) {
+    if (a.col != b.col) return a.col < b.col;
+    if (a.dir != b.dir) return a.dir > b.dir;
+    return a.lm < b.lm;
+  });
+  for (const Cand &cd : cand) {
+    if (cd.v <= thr1) continue;
+    const int k = out.stable_idx[cd.col], m = B.modes[cd.lm];
+    if (!aux_step(B.aux[k], m, cd.dir, tgt)) continue;
+    const uint64_t hk = B.aux_hash[k] ^ S.zob[(size_t)m * 256 + aux_get(B.aux[k], m)] ^
+                        S.zob[(size_t)m * 256 + aux_get(B.aux[k], m) + cd.dir];
+    auto &lst = by_hash[hk];
+    int found = -1;
+    for (int id : lst)
+      if (keys[id] == tgt) {
+        found = id;
+        break;
+      }
+    if (found < 0) {
+      found = (int)keys.size();
+      keys.push_back(tgt);
+      vals.push_back(0.0);
+      lst.push_back(found);
+    }
+    vals[found] += cd.v;
+  }
+  std::vector<double> sv(vals);
+  std::sort(sv.begin(), sv.end());
+  const double thr2 = error_threshold(sv, dbound, discarded);
+  for (size_t i = 0; i < keys.size(); ++i)
+    if (vals[i] > thr2) out.aux_bound.push_back(keys[i]);
+  out.aux_new.reserve(out.stable_idx.size() + out.aux_bound.size());
+  for (int k : out.stable_idx) out.aux_new.push_back(B.aux[k]);
+  for (const Aux &a : out.aux_bound) out.aux_new.push_back(a);
+  std::sort(out.aux_new.begin(), out.aux_new.end(), aux_less);
+  out.aux_new.erase(std::unique(out.aux_new.begin(), out.aux_new.end()), out.aux_new.end());
+}
+
+// state half (hops_basis.py:294-482, :795-912); e_state[s] = sum over stable
+// auxiliaries of |D_stable/hbar + P_stable/dt|^2
+void select_states(const Params &P, const Basis &B, const cplx *pn, const double *e_state, Selection &sel) {
+  const System &S = *B.sys;
+  const int n = B.n, nh = B.nh, nm = B.nm;
+  if (P.delta_s <= 0) {
+    sel.states_new = B.states;
+    return;
+  }
+  const double h2 = kHbar * kHbar;
+  std::vector<double> err(e_state, e_state + n);
+  // flux through the off-diagonal Hamiltonian
+  for (int s = 0; s < n; ++s) {
+    double v1 = 0;
+    for (int sp = 0; sp < S.n; ++sp)
+      if (sp != B.states[s]) v1 += std::norm(S.H[(size_t)sp * S.n + B.states[s]]);
+    double c1 = 0;
+    for (int k : sel.stable_idx) c1 += sel.absC2[(size_t)k * n + s];
+    err[s] += v1 * c1 / h2;
+  }
+  // filters from the boundary auxiliaries (hops_fluxfilters.py:292-352)
+  std::vector<uint8_t> F_dn((size_t)nm * nh, 0), F_up((size_t)nm * nh, 0);
+  Aux tmp;
+  for (const Aux &a : sel.aux_bound) {
+    for (int lm = 0; lm < nm; ++lm) {
+      aux_step(a, B.modes[lm], +1, tmp);
+      auto it = B.index.find(tmp);
+      if (it != B.index.end()) F_dn[(size_t)lm * nh + it->second] = 1;
+    }
+    for (char32_t e : a) {
+      if (!aux_step(a, aux_mode(e), -1, tmp)) continue;
+      auto it = B.index.find(tmp);
+      if (it != B.index.end()) F_up[(size_t)B.rel_mode[aux_mode(e)] * nh + it->second] = 1;
+    }
+  }
+  for (int lm = 0; lm < nm; ++lm) {
+    const int m = B.modes[lm];
+    const double gw2 = std::norm(S.G[m] / S.W[m]);
+    const double aw = std::abs(S.W[m]);
+    const cplx X = sel.X2[lm];
+    // |M2[m,s] - X|^2 and |M2[m,s]|^2 for every state
+    std::vector<double> a_dn(n, std::norm(X)), a_up(n, 0.0);
+    for (auto &e : sel.M2[lm]) {
+      a_dn[e.first] = std::norm(e.second - X);
+      a_up[e.first] = std::norm(e.second);
+    }
+    for (int k : sel.stable_idx) {
+      const bool fd = F_dn[(size_t)lm * nh + k], fu = F_up[(size_t)lm * nh + k];
+      if (!fd && !fu) continue;
+      const double kw = aw * (1.0 + aux_get(B.aux[k], m));
+      for (int s = 0; s < n; ++s) {
+        const double c2 = sel.absC2[(size_t)k * n + s];
+        if (fd) err[s] += (a_dn[s] * gw2) * c2 / h2;
+        if (fu) err[s] += (a_up[s] * kw * kw) * c2 / h2;
+      }
+    }
+  }
+  const double dsq = P.delta_s * P.delta_s;
+  std::vector<double> srt(err);
+  std::sort(srt.begin(), srt.end());
+  const double thr = error_threshold(srt, dsq / 2.0);
+  std::vector<int> rel_st, st_stable;
+  double unstable = 0;
+  for (int s = 0; s < n; ++s) {
+    if (err[s] > thr) {
+      rel_st.push_back(s);
+      st_stable.push_back(B.states[s]);
+    } else {
+      unstable += err[s];
+    }
+  }
+  const double dbound = dsq - unstable;
+  std::vector<int> bound;
+  if ((int)rel_st.size() < S.n && !sel.sc.empty()) {
+    std::vector<double> E(sel.sc.size(), 0.0);
+    for (size_t i = 0; i < sel.sc.size(); ++i) {
+      const int sp = sel.sc[i];
+      for (int k : sel.stable_idx) {
+        cplx z = 0;
+        for (size_t q = 0; q < rel_st.size(); ++q)
+          z += cplx(0, -1) * S.H[(size_t)sp * S.n + st_stable[q]] * (pn[(size_t)k * n + rel_st[q]] / kHbar);
+        E[i] += std::norm(z);
+      }
+    }
+    std::vector<double> ev;
+    std::vector<int> es;
+    for (size_t i = 0; i < E.size(); ++i)
+      if (E[i] != 0) {
+        ev.push_back(E[i]);
+        es.push_back(sel.sc[i]);
+      }
+    if (!ev.empty()) {
+      std::vector<double> s2(ev);
+      std::sort(s2.begin(), s2.end());
+      const double t2 = error_threshold(s2, dbound);
+      for (size_t i = 0; i < ev.size(); ++i)
+        if (ev[i] > t2) bound.push_back(es[i]);
+    }
+  }
+  std::vector<int> res(st_stable);
+  for (int b : bound)
+    if (std::find(B.states.begin(), B.states.end(), b) == B.states.end()) res.push_back(b);
+  std::sort(res.begin(), res.end());
+  res.erase(std::unique(res.begin(), res.end()), res.end());
+  sel.states_new = res;
+}
+
+// ---------------------------------------------------------------------------
+// Thread pool for per-slot host work
+// ---------------------------------------------------------------------------
+class Pool {
+ public:
+  explicit Pool(int n) {
+    for (int i = 0; i < n; ++i) th_.emplace_back([this] { run(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  // fn(i) for i in [0, n); returns after all finished
+  void parallel_for(int n, const std::function<void(int)> &fn) {
+    if (n <= 0) return;
+    if (th_.empty() || n == 1) {
+      for (int i = 0; i < n; ++i) fn(i);
+      return;
+    }
+    std::unique_lock<std::mutex> g(mu_);
+    fn_ = &fn;
+    n_ = n;
+    next_.store(0);
+    done_ = 0;
+    ++gen_;
+    cv_.notify_all();
+    g.unlock();
+    work();
+    g.lock();
+    done_cv_.wait(g, [&] { return done_ == (int)th_.size(); });
+    fn_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      const int i = next_.fetch_add(1);
+      if (i >= n_) break;
+      (*fn_)(i);
+    }
+  }
+  void run() {
+    long seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      g.unlock();
+      work();
+      g.lock();
+      ++done_;
+      done_cv_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)> *fn_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, done_ = 0;
+  long gen_ = 0;
+  bool stop_ = false;
+};
+
+// ---------------------------------------------------------------------------
+// Arena: first-fit allocator over caller-owned device memory.  One stream,
+// so memory freed after its last use was enqueued can be handed out again.
+// ---------------------------------------------------------------------------
+class Arena {
+ public:
+  void init(char *base, size_t bytes) {
+    base_ = base;
+    size_ = bytes;
+    free_.clear();
+    free_[0] = bytes;
+  }
+  long long alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= bytes) {
+        const size_t off = it->first, sz = it->second;
+        free_.erase(it);
+        if (sz > bytes) free_[off + bytes] = sz - bytes;
+        used_[off] = bytes;
+        return (long long)off;
+      }
+    return -1;
+  }
+  void release(long long off) {
+    if (off < 0) return;
+    auto u = used_.find((size_t)off);
+    if (u == used_.end()) return;
+    size_t o = u->first, sz = u->second;
+    used_.erase(u);
+    auto nx = free_.lower_bound(o);
+    if (nx != free_.end() && o + sz == nx->first) {
+      sz += nx->second;
+      nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == o) {
+        pv->second += sz;
+        return;
+      }
+    }
+    free_[o] = sz;
+  }
+  size_t free_bytes() const {
+    size_t s = 0;
+    for (auto &kv : free_) s += kv.second;
+    return s;
+  }
+  char *ptr(long long off) const { return base_ + off; }
+
+ private:
+  char *base_ = nullptr;
+  size_t size_ = 0;
+  std::map<size_t, size_t> free_, used_;
+};
+
+// ---------------------------------------------------------------------------
+// Slots and the batch
+// ---------------------------------------------------------------------------
+struct Record {
+  std::vector<cplx> psi;
+  std::vector<int> states;
+  int n_hier;
+};
+
+struct Slot {
+  bool used = false, pending_init = false;
+  const System *sys = nullptr;
+  Params prm;
+  Basis B;
+  long long blk = -1, pers = -1;  // arena offsets: per-basis block, persistent block
+  size_t blk_bytes = 0;
+  hops_op_t op{};
+  hops_batch_t bt{};
+  const void *z1 = nullptr, *z2 = nullptr;
+  int it = 0, early_counter = 0;
+  long long steps = 0;
+  std::vector<cplx> psi0;
+  // device pointers inside the blocks
+  void *phi_prev = nullptr;
+  double *zmem_prev = nullptr;
+  std::vector<Record> rec;
+  // per-step scratch
+  Selection sel;
+  std::vector<cplx> pn;
+  double nrm = 1;
+  Basis Bnew;
+  bool changed = false;
+  double scale = 1;
+};
+
+struct Batch {
+  int precision = HOPS_PREC_FP64;
+  hipStream_t stream = nullptr;
+  Arena arena;
+  std::vector<Slot> slots;
+  std::unique_ptr<Pool> pool;
+  // device descriptor arrays
+  long long ops_off = -1, bts_off = -1, perm_off = -1, it_off = -1, dt_off = -1, nrm_off = -1;
+  // staging: device scratch region and pinned host buffers
+  long long scratch_off = -1;
+  size_t scratch_bytes = 0;
+  char *h_up = nullptr, *h_down = nullptr;
+  size_t h_up_bytes = 0, h_down_bytes = 0;
+  std::vector<hops_op_t> ops;
+  std::vector<hops_batch_t> bts;
+  std::vector<hops_slot_perm_t> perms;
+  std::vector<int32_t> its;
+  std::vector<double> dts, nrms;
+};
+
+size_t rsz(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// per-basis block layout
+struct Layout {
+  size_t total = 0;
+  std::vector<std::pair<const void *, size_t>> tabs;  // host source, bytes
+  std::vector<size_t> tab_off;
+  size_t ell_col, ell_val, k0, phi, phi_prev, ya, yb, acc, y0, coef, coefT;
+};
+
+template <typename T>
+void add_tab(Layout &L, const std::vector<T> &v) {
+  L.tab_off.push_back(L.total);
+  L.tabs.push_back({v.data(), v.size() * sizeof(T)});
+  L.total += rsz(std::max<size_t>(v.size() * sizeof(T), 16));
+}
+
+size_t stage_bytes(int precision) { return precision == HOPS_PREC_FP64 ? 16 : 8; }
+
+Layout layout_of(const Basis &B, int precision) {
+  Layout L;
+  // order = order of assignment in set_op_pointers
+  add_tab(L, B.aux_ptr); add_tab(L, B.aux_mode_t); add_tab(L, B.aux_val_t);
+  add_tab(L, B.up_ptr); add_tab(L, B.up_aux); add_tab(L, B.up_mode);
+  add_tab(L, B.dn_ptr); add_tab(L, B.dn_aux); add_tab(L, B.dn_mode); add_tab(L, B.dn_val);
+  add_tab(L, B.lr_ptr); add_tab(L, B.lr_col); add_tab(L, B.lr_val);
+  add_tab(L, B.lc_ptr); add_tab(L, B.lc_row); add_tab(L, B.lc_val);
+  add_tab(L, B.mode_lrel32); add_tab(L, B.mode_lact32); add_tab(L, B.mode_abs32);
+  add_tab(L, B.mode_g); add_tab(L, B.mode_w); add_tab(L, B.abs_g); add_tab(L, B.abs_w); add_tab(L, B.abs_lact);
+  add_tab(L, B.act_lrel32); add_tab(L, B.act_zrow); add_tab(L, B.am_ptr); add_tab(L, B.am_mode);
+  add_tab(L, B.la_ptr); add_tab(L, B.la_row); add_tab(L, B.la_col); add_tab(L, B.la_val);
+  add_tab(L, B.lq_ptr); add_tab(L, B.lq_row); add_tab(L, B.lq_col); add_tab(L, B.lq_val); add_tab(L, B.lt);
+  add_tab(L, B.first_aux); add_tab(L, B.first_lact); add_tab(L, B.first_mode);
+  add_tab(L, B.pat_ptr); add_tab(L, B.pat_col); add_tab(L, B.pat_h);
+  add_tab(L, B.pc_ptr); add_tab(L, B.pc_act); add_tab(L, B.pc_kind); add_tab(L, B.pc_val);
+  const size_t nr = (size_t)B.n * B.nh, es = stage_bytes(precision);
+  const size_t w = std::max(B.ell_w, 1);
+  auto put = [&](size_t bytes) {
+    const size_t o = L.total;
+    L.total += rsz(std::max<size_t>(bytes, 16));
+    return o;
+  };
+  L.ell_col = put(w * nr * 4);
+  L.ell_val = put(w * nr * es);
+  L.k0 = put((size_t)B.nh * 16);
+  L.phi = put(nr * 16);
+  L.phi_prev = put(nr * 16);
+  L.ya = put(nr * es);
+  L.yb = put(nr * es);
+  L.acc = put(nr * es);
+  L.y0 = put(precision == HOPS_PREC_MIXED ? nr * 8 : 16);
+  const size_t coef_stride = 4 * (size_t)B.n_pat + 2 * B.nm + 4;
+  L.coef = put(coef_stride * 8);
+  const size_t rowsT = 2 * (size_t)B.n_pat + B.nm + 1;
+  L.coefT = put(rowsT * es);
+  return L;
+}
+
+size_t pers_bytes(const System &S) {
+  const size_t Ma = S.n_hm;
+  return rsz(Ma * 16 + 16) * 2 + rsz((4 * Ma + 2) * 8) + rsz(16);
+}
+
+void set_op(const Basis &B, char *base, const Layout &L, int precision, hops_op_t &op) {
+  op = hops_op_t{};
+  op.n_state = B.n;
+  op.n_hier = B.nh;
+  op.n_mode = B.nm;
+  op.n_lop = B.nl;
+  op.n_act = B.na;
+  op.n_first = (int)B.first_aux.size();
+  op.ell_w = B.ell_w;
+  op.n_pat = B.n_pat;
+  op.n_mode_abs = B.sys->n_hm;
+  op.pat_w = B.pat_w;
+  size_t t = 0;
+  auto P = [&](auto *&field) {
+    using T = std::remove_reference_t<decltype(field)>;
+    field = reinterpret_cast<T>(base + L.tab_off[t++]);
+  };
+  P(op.aux_ptr_d); P(op.aux_mode_d); P(op.aux_val_d);
+  P(op.up_ptr_d); P(op.up_aux_d); P(op.up_mode_d);
+  P(op.dn_ptr_d); P(op.dn_aux_d); P(op.dn_mode_d); P(op.dn_val_d);
+  P(op.lr_ptr_d); P(op.lr_col_d); P(op.lr_val_d);
+  P(op.lc_ptr_d); P(op.lc_row_d); P(op.lc_val_d);
+  P(op.mode_lrel_d); P(op.mode_lact_d); P(op.mode_abs_d);
+  P(op.mode_g_d); P(op.mode_w_d); P(op.abs_g_d); P(op.abs_w_d); P(op.abs_lact_d);
+  P(op.act_lrel_d); P(op.act_zrow_d); P(op.am_ptr_d); P(op.am_mode_d);
+  P(op.la_ptr_d); P(op.la_row_d); P(op.la_col_d); P(op.la_val_d);
+  P(op.lq_ptr_d); P(op.lq_row_d); P(op.lq_col_d); P(op.lq_val_d); P(op.lt_d);
+  P(op.first_aux_d); P(op.first_lact_d); P(op.first_mode_d);
+  P(op.pat_ptr_d); P(op.pat_col_d); P(op.pat_h_d);
+  P(op.pc_ptr_d); P(op.pc_act_d); P(op.pc_kind_d); P(op.pc_val_d);
+  if (!B.has_lt) op.lt_d = nullptr;
+  op.ell_col_d = reinterpret_cast<int32_t *>(base + L.ell_col);
+  op.ell_val_d = base + L.ell_val;
+  op.k0_d = reinterpret_cast<double *>(base + L.k0);
+  (void)precision;
+}
+
+// ---------------------------------------------------------------------------
+// Staging: a host-pinned upload buffer mirrored by a device scratch region.
+// Uploads of one phase go to increasing offsets; the cursor resets at sync.
+// ---------------------------------------------------------------------------
+struct Stage {
+  Batch *b;
+  size_t cur = 0;
+  int push(const void *src, size_t bytes, void **dev) {
+    const size_t need = cur + rsz(bytes);
+    if (need > b->scratch_bytes || need > b->h_up_bytes) return rt_fail("staging area too small (%zu bytes)", need);
+    std::memcpy(b->h_up + cur, src, bytes);
+    *dev = b->arena.ptr(b->scratch_off) + cur;
+    RT_HIP(hipMemcpyAsync(*dev, b->h_up + cur, bytes, hipMemcpyHostToDevice, b->stream));
+    cur = need;
+    return 0;
+  }
+  // device-side output region at the current cursor
+  int reserve(size_t bytes, void **dev, size_t *off) {
+    const size_t need = cur + rsz(bytes);
+    if (need > b->scratch_bytes) return rt_fail("staging area too small (%zu bytes)", need);
+    *off = cur;
+    *dev = b->arena.ptr(b->scratch_off) + cur;
+    cur = need;
+    return 0;
+  }
+};
+
+int ensure_staging(Batch &b, size_t bytes) {
+  bytes = rsz(bytes) + (1 << 20);
+  if (bytes <= b.scratch_bytes && bytes <= b.h_up_bytes && bytes <= b.h_down_bytes) return 0;
+  RT_HIP(hipStreamSynchronize(b.stream));
+  if (b.scratch_off >= 0) b.arena.release(b.scratch_off);
+  const size_t nb = std::max(bytes, b.scratch_bytes * 3 / 2);
+  b.scratch_off = b.arena.alloc(nb);
+  if (b.scratch_off < 0) {
+    b.scratch_bytes = 0;
+    return rt_fail("arena full: %zu bytes of staging", nb);
+  }
+  b.scratch_bytes = nb;
+  if (b.h_up) RT_HIP(hipHostFree(b.h_up));
+  if (b.h_down) RT_HIP(hipHostFree(b.h_down));
+  b.h_up = b.h_down = nullptr;
+  RT_HIP(hipHostMalloc((void **)&b.h_up, nb, hipHostMallocDefault));
+  RT_HIP(hipHostMalloc((void **)&b.h_down, nb, hipHostMallocDefault));
+  b.h_up_bytes = b.h_down_bytes = nb;
+  return 0;
+}
+
+std::vector<uint32_t> tiles_of(const Batch &b, const std::vector<int> &list, bool new_basis = false) {
+  std::vector<uint32_t> t;
+  for (int s : list) {
+    const Basis &B = new_basis ? b.slots[s].Bnew : b.slots[s].B;
+    const long nr = (long)B.n * B.nh;
+    const long nt = (nr + 255) / 256;
+    for (long i = 0; i < nt; ++i) t.push_back(((uint32_t)s << kSlotShift) | (uint32_t)i);
+  }
+  return t;
+}
+
+int shm_of(const Batch &b, const std::vector<int> &list) {
+  int m = 0;
+  for (int s : list) {
+    const Basis &B = b.slots[s].B;
+    m = std::max(m, 2 * B.n + 8 * B.na + 2 * B.sys->n_hm + 16);
+  }
+  return m;
+}
+
+// upload descriptor arrays and per-slot scalars
+int upload_desc(Batch &b) {
+  const size_t S = b.slots.size();
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.ops_off), b.ops.data(), S * sizeof(hops_op_t), hipMemcpyHostToDevice, b.stream));
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.bts_off), b.bts.data(), S * sizeof(hops_batch_t), hipMemcpyHostToDevice, b.stream));
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.it_off), b.its.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, b.stream));
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.dt_off), b.dts.data(), S * sizeof(double), hipMemcpyHostToDevice, b.stream));
+  // pageable sources: make sure the copies have read them before they change
+  RT_HIP(hipStreamSynchronize(b.stream));
+  return 0;
+}
+
+hops_op_t *d_ops(Batch &b) { return reinterpret_cast<hops_op_t *>(b.arena.ptr(b.ops_off)); }
+hops_batch_t *d_bts(Batch &b) { return reinterpret_cast<hops_batch_t *>(b.arena.ptr(b.bts_off)); }
+
+// allocate the block of a new basis, upload its tables, point op/bt at it
+int place_basis(Batch &b, Slot &sl, Basis &B, long long *blk_out, hops_op_t &op, hops_batch_t &bt,
+                std::vector<char> &host_tables) {
+  Layout L = layout_of(B, b.precision);
+  const long long blk = b.arena.alloc(L.total);
+  if (blk < 0) return rt_fail("arena full: a basis of %d states x %d auxiliaries needs %zu bytes", B.n, B.nh, L.total);
+  char *base = b.arena.ptr(blk);
+  set_op(B, base, L, b.precision, op);
+  // tables are contiguous at the start of the block: one copy
+  const size_t tab_bytes = L.tab_off.empty() ? 0 : L.ell_col;
+  host_tables.assign(tab_bytes, 0);
+  for (size_t i = 0; i < L.tabs.size(); ++i)
+    if (L.tabs[i].second) std::memcpy(host_tables.data() + L.tab_off[i], L.tabs[i].first, L.tabs[i].second);
+  *blk_out = blk;
+  bt = hops_batch_t{};
+  bt.B = 1;
+  bt.precision = b.precision;
+  bt.eom = sl.prm.eom;
+  bt.n_t = sl.prm.n_t;
+  bt.coef_stride = 4 * B.n_pat + 2 * B.nm + 4;
+  bt.ws_stride = 4 * B.sys->n_hm + 2;
+  bt.store_stride = 0;
+  bt.lane_shift = 0;
+  bt.dt_over_hbar = sl.prm.dt / kHbar;
+  bt.noise_step = sl.prm.noise_step;
+  bt.phi_d = base + L.phi;
+  bt.y0_d = base + L.y0;
+  bt.ya_d = base + L.ya;
+  bt.yb_d = base + L.yb;
+  bt.acc_d = base + L.acc;
+  bt.z1_d = sl.z1;
+  bt.z2_d = sl.z2;
+  char *pb = b.arena.ptr(sl.pers);
+  const size_t Ma = B.sys->n_hm;
+  bt.zmem_d = reinterpret_cast<double *>(pb);
+  bt.ws_d = reinterpret_cast<double *>(pb + 2 * rsz(Ma * 16 + 16));
+  bt.coef_d = reinterpret_cast<double *>(base + L.coef);
+  bt.psi_store_d = nullptr;
+  bt.tile_d = nullptr;
+  bt.n_tiles = 1;
+  bt.tile_rows = 256;
+  bt.coefT_d = base + L.coefT;
+  bt.sigT_d = reinterpret_cast<double *>(pb + 2 * rsz(Ma * 16 + 16) + rsz((4 * Ma + 2) * 8));
+  bt.coefT_stride = 1;
+  sl.blk_bytes = L.total;
+  return 0;
+}
+
+// ---- the step machinery ----------------------------------------------------
+
+int run_step(Batch &b, const std::vector<int> &list) {
+  if (list.empty()) return 0;
+  for (int s : list) b.its[s] = b.slots[s].it;
+  RT_TRY(upload_desc(b));
+  std::vector<uint32_t> tiles = tiles_of(b, list);
+  Stage st{&b};
+  void *dl, *dt;
+  RT_TRY(st.push(list.data(), list.size() * 4, &dl));
+  RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dt));
+  RT_TRY(hops_slots_step(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const uint32_t *)dt,
+                         (int)tiles.size(), (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm_of(b, list),
+                         b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  return 0;
+}
+
+// copy psi (psi_only) of the listed slots to the host
+int fetch_psi(Batch &b, const std::vector<int> &list) {
+  if (list.empty()) return 0;
+  std::vector<int64_t> off(b.slots.size(), 0);
+  size_t tot = 0;
+  for (int s : list) {
+    off[s] = (int64_t)tot;
+    tot += b.slots[s].B.n;
+  }
+  RT_TRY(ensure_staging(b, tot * 16 + off.size() * 8 + list.size() * 4 + 4096));
+  Stage st{&b};
+  void *dl, *doff, *dout;
+  size_t oo;
+  RT_TRY(st.push(list.data(), list.size() * 4, &dl));
+  RT_TRY(st.push(off.data(), off.size() * 8, &doff));
+  RT_TRY(st.reserve(tot * 16, &dout, &oo));
+  RT_TRY(hops_slots_pn(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const int64_t *)doff, dout,
+                       nullptr, b.precision, 1, b.stream));
+  RT_HIP(hipMemcpyAsync(b.h_down, dout, tot * 16, hipMemcpyDeviceToHost, b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  const cplx *h = reinterpret_cast<const cplx *>(b.h_down);
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    sl.pn.assign(h + off[s], h + off[s] + sl.B.n);
+    sl.nrm = 0;  // marks psi-only content
+  }
+  return 0;
+}
+
+// define_basis for the listed slots; dt per slot in b.dts
+int define_bases(Batch &b, const std::vector<int> &list) {
+  if (list.empty()) return 0;
+  const size_t S = b.slots.size();
+  std::vector<int64_t> off(S, 0), eoff(S, 0);
+  size_t tot = 0, etot = 0;
+  for (int s : list) {
+    off[s] = (int64_t)tot;
+    tot += (size_t)b.slots[s].B.n * b.slots[s].B.nh;
+    eoff[s] = (int64_t)etot;
+    etot += std::max(b.slots[s].B.nh, b.slots[s].B.n);
+  }
+  const std::vector<uint32_t> tiles = tiles_of(b, list);
+  for (int s : list) b.its[s] = b.slots[s].it;
+  RT_TRY(upload_desc(b));
+  RT_TRY(ensure_staging(b, tot * 17 + etot * 16 + S * 8 * 6 + tiles.size() * 8 + list.size() * 8 + 65536));
+  Stage st{&b};
+  void *dl, *dtl, *doff, *deoff, *dpn, *de;
+  size_t opn, oe;
+  RT_TRY(st.push(list.data(), list.size() * 4, &dl));
+  RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dtl));
+  RT_TRY(st.push(off.data(), S * 8, &doff));
+  RT_TRY(st.push(eoff.data(), S * 8, &deoff));
+  RT_TRY(st.reserve(tot * 16, &dpn, &opn));
+  RT_TRY(st.reserve(etot * 8, &de, &oe));
+  double *dnrm = reinterpret_cast<double *>(b.arena.ptr(b.nrm_off));
+  const int shm = shm_of(b, list);
+  RT_TRY(hops_slots_pn(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const int64_t *)doff, dpn, dnrm,
+                       b.precision, 0, b.stream));
+  bool any_h = false, any_s = false;
+  for (int s : list) {
+    any_h |= b.slots[s].prm.delta_a > 0;
+    any_s |= b.slots[s].prm.delta_s > 0;
+  }
+  if (any_h)
+    RT_TRY(hops_slots_errors(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const uint32_t *)dtl,
+                             (int)tiles.size(), (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm,
+                             (const int64_t *)doff, dpn, nullptr, nullptr, (const int64_t *)deoff, (double *)de,
+                             (const double *)b.arena.ptr(b.dt_off), kHbar, b.stream));
+  RT_HIP(hipMemcpyAsync(b.h_down, dpn, tot * 16, hipMemcpyDeviceToHost, b.stream));
+  RT_HIP(hipMemcpyAsync(b.h_down + tot * 16, de, etot * 8, hipMemcpyDeviceToHost, b.stream));
+  RT_HIP(hipMemcpyAsync(b.nrms.data(), dnrm, S * 8, hipMemcpyDeviceToHost, b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  const cplx *hpn = reinterpret_cast<const cplx *>(b.h_down);
+  const double *he = reinterpret_cast<const double *>(b.h_down + tot * 16);
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    sl.pn.assign(hpn + off[s], hpn + off[s] + (size_t)sl.B.n * sl.B.nh);
+    sl.nrm = b.nrms[s];
+  }
+  std::vector<std::vector<double>> eaux(S);
+  for (int s : list) eaux[s].assign(he + eoff[s], he + eoff[s] + b.slots[s].B.nh);
+  b.pool->parallel_for((int)list.size(), [&](int i) {
+    Slot &sl = b.slots[list[i]];
+    select_hier(sl.prm, sl.B, sl.pn.data(), eaux[list[i]].data(), sl.sel);
+  });
+  if (any_s) {
+    std::vector<int> ls;
+    for (int s : list)
+      if (b.slots[s].prm.delta_s > 0) ls.push_back(s);
+    std::vector<int64_t> moff(S, 0), soff(S, 0);
+    size_t mtot = 0, stot = 0;
+    for (int s : ls) {
+      moff[s] = (int64_t)mtot;
+      mtot += b.slots[s].B.nh;
+      soff[s] = (int64_t)stot;
+      stot += b.slots[s].B.n;
+    }
+    std::vector<uint8_t> mask(mtot);
+    for (int s : ls) std::memcpy(mask.data() + moff[s], b.slots[s].sel.stable_mask.data(), b.slots[s].B.nh);
+    const std::vector<uint32_t> tl = tiles_of(b, ls);
+    void *dls, *dtls, *dmoff, *dmask, *dsoff, *des;
+    size_t oes;
+    RT_TRY(st.push(ls.data(), ls.size() * 4, &dls));
+    RT_TRY(st.push(tl.data(), tl.size() * 4, &dtls));
+    RT_TRY(st.push(moff.data(), S * 8, &dmoff));
+    RT_TRY(st.push(mask.data(), mtot, &dmask));
+    RT_TRY(st.push(soff.data(), S * 8, &dsoff));
+    RT_TRY(st.reserve(stot * 8, &des, &oes));
+    RT_TRY(hops_slots_errors(d_ops(b), d_bts(b), (const int32_t *)dls, (int)ls.size(), (const uint32_t *)dtls,
+                             (int)tl.size(), (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm_of(b, ls),
+                             (const int64_t *)doff, dpn, (const int64_t *)dmoff, (const uint8_t *)dmask,
+                             (const int64_t *)dsoff, (double *)des, (const double *)b.arena.ptr(b.dt_off), kHbar,
+                             b.stream));
+    RT_HIP(hipMemcpyAsync(b.h_down, des, stot * 8, hipMemcpyDeviceToHost, b.stream));
+    RT_HIP(hipStreamSynchronize(b.stream));
+    const double *hs = reinterpret_cast<const double *>(b.h_down);
+    std::vector<std::vector<double>> es(S);
+    for (int s : ls) es[s].assign(hs + soff[s], hs + soff[s] + b.slots[s].B.n);
+    b.pool->parallel_for((int)ls.size(), [&](int i) {
+      Slot &sl = b.slots[ls[i]];
+      select_states(sl.prm, sl.B, sl.pn.data(), es[ls[i]].data(), sl.sel);
+    });
+  }
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    if (sl.prm.delta_s <= 0) sl.sel.states_new = sl.B.states;
+  }
+  return 0;
+}
+
+bool same_basis(const Basis &B, const std::vector<int> &states, const std::vector<Aux> &aux) {
+  if (states.size() != B.states.size() || aux.size() != B.aux.size()) return false;
+  if (!std::equal(states.begin(), states.end(), B.states.begin())) return false;
+  for (const Aux &a : aux)
+    if (!B.index.count(a)) return false;
+  return true;
+}
+
+// move the listed slots to the bases (states, aux) given in tgt_states/tgt_aux;
+// scale: per slot scale of the moved Phi
+int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector<int>> &tgt_states,
+                 std::vector<std::vector<Aux>> &tgt_aux, const std::vector<double> &scale) {
+  std::vector<int> ch;
+  std::vector<int> idx_of(b.slots.size(), -1);
+  for (size_t i = 0; i < list.size(); ++i) {
+    Slot &sl = b.slots[list[i]];
+    std::sort(tgt_states[i].begin(), tgt_states[i].end());
+    std::sort(tgt_aux[i].begin(), tgt_aux[i].end(), aux_less);
+    sl.changed = !same_basis(sl.B, tgt_states[i], tgt_aux[i]);
+    if (sl.changed) {
+      idx_of[list[i]] = (int)i;
+      ch.push_back(list[i]);
+    }
+  }
+  if (ch.empty()) return 0;
+  std::vector<int> rcs(ch.size(), 0);
+  b.pool->parallel_for((int)ch.size(), [&](int i) {
+    Slot &sl = b.slots[ch[i]];
+    const int j = idx_of[ch[i]];
+    rcs[i] = build_basis(*sl.sys, sl.prm.eom, tgt_states[j], tgt_aux[j], sl.Bnew);
+  });
+  for (int rc : rcs)
+    if (rc < 0) return rc;
+  // blocks, tables, permutation maps
+  std::vector<long long> nblk(ch.size());
+  std::vector<hops_op_t> nop(ch.size());
+  std::vector<hops_batch_t> nbt(ch.size());
+  std::vector<std::vector<char>> tabs(ch.size());
+  std::vector<std::vector<int32_t>> inv(ch.size());
+  size_t up_bytes = 0;
+  for (size_t i = 0; i < ch.size(); ++i) {
+    Slot &sl = b.slots[ch[i]];
+    RT_TRY(place_basis(b, sl, sl.Bnew, &nblk[i], nop[i], nbt[i], tabs[i]));
+    const Basis &Bo = sl.B, &Bn = sl.Bnew;
+    inv[i].assign(Bn.nh + Bn.n, -1);
+    for (int j = 0; j < Bn.nh; ++j) {
+      auto it = Bo.index.find(Bn.aux[j]);
+      if (it != Bo.index.end()) inv[i][j] = it->second;
+    }
+    for (int s = 0; s < Bn.n; ++s) inv[i][Bn.nh + s] = Bo.pos[Bn.states[s]];
+    up_bytes += rsz(tabs[i].size()) + rsz(inv[i].size() * 4);
+  }
+  const std::vector<uint32_t> tiles = tiles_of(b, ch, true);
+  RT_TRY(ensure_staging(b, up_bytes + tiles.size() * 4 + ch.size() * 4 + 65536));
+  Stage st{&b};
+  for (size_t i = 0; i < ch.size(); ++i) {
+    Slot &sl = b.slots[ch[i]];
+    if (!tabs[i].empty())
+      RT_HIP(hipMemcpyAsync(b.arena.ptr(nblk[i]), tabs[i].data(), tabs[i].size(), hipMemcpyHostToDevice, b.stream));
+    void *dinv;
+    RT_TRY(st.push(inv[i].data(), inv[i].size() * 4, &dinv));
+    hops_slot_perm_t &p = b.perms[ch[i]];
+    p.old_phi_d = sl.bt.phi_d;
+    p.new_phi_d = nbt[i].phi_d;
+    p.inv_aux_d = (const int32_t *)dinv;
+    p.inv_state_d = (const int32_t *)dinv + sl.Bnew.nh;
+    p.n_old = sl.B.n;
+    p.n_new = sl.Bnew.n;
+    p.nh_new = sl.Bnew.nh;
+    p.scale = scale[idx_of[ch[i]]];
+    b.ops[ch[i]] = nop[i];
+  }
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.perm_off), b.perms.data(), b.slots.size() * sizeof(hops_slot_perm_t),
+                        hipMemcpyHostToDevice, b.stream));
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.ops_off), b.ops.data(), b.slots.size() * sizeof(hops_op_t),
+                        hipMemcpyHostToDevice, b.stream));
+  void *dl, *dt;
+  RT_TRY(st.push(ch.data(), ch.size() * 4, &dl));
+  RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dt));
+  RT_TRY(hops_slots_build(d_ops(b), (const int32_t *)dl, (int)ch.size(), (const uint32_t *)dt, (int)tiles.size(),
+                          b.precision, b.stream));
+  RT_TRY(hops_slots_permute((const hops_slot_perm_t *)b.arena.ptr(b.perm_off), (const uint32_t *)dt,
+                            (int)tiles.size(), b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  for (size_t i = 0; i < ch.size(); ++i) {
+    Slot &sl = b.slots[ch[i]];
+    b.arena.release(sl.blk);
+    sl.blk = nblk[i];
+    sl.B = std::move(sl.Bnew);
+    sl.Bnew = Basis();
+    sl.op = nop[i];
+    sl.bt = nbt[i];
+    b.bts[ch[i]] = nbt[i];
+    char *base = b.arena.ptr(sl.blk);
+    sl.phi_prev = base + layout_of(sl.B, b.precision).phi_prev;
+  }
+  return 0;
+}
+
+// psi of a slot after its basis update, from the Pn of its define step
+void record_step(Slot &sl, const std::vector<int> &old_states, const std::vector<cplx> &psi_old) {
+  if (!sl.prm.record) return;
+  Record r;
+  r.states = sl.B.states;
+  r.n_hier = sl.B.nh;
+  r.psi.assign(sl.B.n, cplx(0, 0));
+  for (int s = 0; s < sl.B.n; ++s) {
+    const auto it = std::lower_bound(old_states.begin(), old_states.end(), sl.B.states[s]);
+    if (it != old_states.end() && *it == sl.B.states[s]) r.psi[s] = sl.scale * psi_old[it - old_states.begin()];
+  }
+  sl.rec.push_back(std::move(r));
+}
+
+// scale of update_basis: |psi_old| / |psi_old on the states kept|
+double keep_scale(const Slot &sl, const std::vector<int> &new_states, const std::vector<cplx> &psi) {
+  double a = 0, k = 0;
+  for (int s = 0; s < sl.B.n; ++s) {
+    const double v = std::norm(psi[s]);
+    a += v;
+    if (std::binary_search(new_states.begin(), new_states.end(), sl.B.states[s])) k += v;
+  }
+  return std::sqrt(a) / std::sqrt(k);
+}
+
+int device_copy(Batch &b, void *dst, const void *src, size_t bytes) {
+  RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.stream));
+  return 0;
+}
+
+// initial basis update of newly added slots (trajectory/hops_trajectory.py:279-317)
+int init_pending(Batch &b) {
+  std::vector<int> list;
+  for (size_t s = 0; s < b.slots.size(); ++s)
+    if (b.slots[s].used && b.slots[s].pending_init) list.push_back((int)s);
+  if (list.empty()) return 0;
+  std::vector<int> adapt;
+  for (int s : list) {
+    b.dts[s] = 1.0;
+    Slot &sl = b.slots[s];
+    if (sl.prm.delta_a > 0 || sl.prm.delta_s > 0) adapt.push_back(s);
+  }
+  RT_TRY(define_bases(b, adapt));
+  std::vector<std::vector<int>> ts;
+  std::vector<std::vector<Aux>> ta;
+  std::vector<double> sc;
+  std::vector<std::vector<int>> old_states;
+  std::vector<std::vector<cplx>> old_psi;
+  for (int s : adapt) {
+    Slot &sl = b.slots[s];
+    ts.push_back(sl.sel.states_new);
+    ta.push_back(sl.sel.aux_new);
+    std::vector<cplx> psi(sl.pn.begin(), sl.pn.begin() + sl.B.n);
+    for (auto &v : psi) v *= sl.nrm;
+    std::vector<int> sn(sl.sel.states_new);
+    std::sort(sn.begin(), sn.end());
+    sc.push_back(keep_scale(sl, sn, psi));
+    old_states.push_back(sl.B.states);
+    old_psi.push_back(psi);
+  }
+  RT_TRY(update_bases(b, adapt, ts, ta, sc));
+  for (size_t i = 0; i < adapt.size(); ++i) {
+    Slot &sl = b.slots[adapt[i]];
+    if (!sl.changed) sl.scale = 1.0;
+    else sl.scale = sc[i];
+    record_step(sl, old_states[i], old_psi[i]);
+  }
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    if (!(sl.prm.delta_a > 0 || sl.prm.delta_s > 0) && sl.prm.record) {
+      Record r;
+      r.states = sl.B.states;
+      r.n_hier = sl.B.nh;
+      r.psi.assign(sl.B.n, 0);
+      for (int q = 0; q < sl.B.n; ++q) r.psi[q] = sl.psi0[sl.B.states[q]];
+      sl.rec.push_back(std::move(r));
+    }
+    sl.pending_init = false;
+  }
+  return 0;
+}
+
+// index_t: step index within this call (the update schedule restarts with every
+// propagate call, trajectory/hops_trajectory.py:456, :511)
+int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
+  // early INCH_WORM slots keep the state before the step
+  std::vector<int> early, upd, plain;
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    const bool adaptive = sl.prm.delta_a > 0 || sl.prm.delta_s > 0;
+    const bool is_early = adaptive && sl.early_counter < sl.prm.early_steps;
+    if (is_early && sl.prm.early_integrator == 0) {
+      early.push_back(s);
+      RT_TRY(device_copy(b, sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16));
+      RT_TRY(device_copy(b, sl.zmem_prev, sl.bt.zmem_d, (size_t)sl.sys->n_hm * 16));
+    }
+    const int us = sl.prm.update_step > 0 ? sl.prm.update_step : 1;
+    if (adaptive && (is_early || (index_t + 1) % us == 0)) {
+      if (!(is_early && sl.prm.early_integrator == 1)) upd.push_back(s);
+    } else {
+      plain.push_back(s);
+    }
+  }
+  RT_TRY(run_step(b, list));
+  for (int s : upd) b.dts[s] = b.slots[s].prm.dt;
+  RT_TRY(define_bases(b, upd));
+  // INCH_WORM (trajectory/hops_trajectory.py:469-510, :636-691)
+  std::vector<int> redo;
+  std::vector<char> is_early(b.slots.size(), 0);
+  for (int s : early) is_early[s] = 1;
+  for (int s : upd) {
+    Slot &sl = b.slots[s];
+    if (is_early[s] && !same_basis(sl.B, sl.sel.states_new, sl.sel.aux_new)) redo.push_back(s);
+  }
+  std::vector<int> tries(b.slots.size(), 0);
+  while (!redo.empty()) {
+    std::vector<std::vector<int>> ts;
+    std::vector<std::vector<Aux>> ta;
+    std::vector<double> sc;
+    for (int s : redo) {
+      Slot &sl = b.slots[s];
+      std::vector<int> su(sl.sel.states_new);
+      su.insert(su.end(), sl.B.states.begin(), sl.B.states.end());
+      std::sort(su.begin(), su.end());
+      su.erase(std::unique(su.begin(), su.end()), su.end());
+      std::vector<Aux> au(sl.sel.aux_new);
+      au.insert(au.end(), sl.B.aux.begin(), sl.B.aux.end());
+      ts.push_back(su);
+      ta.push_back(au);
+      sc.push_back(1.0);  // every old state is kept: |psi| unchanged
+      RT_TRY(device_copy(b, sl.bt.phi_d, sl.phi_prev, (size_t)sl.B.n * sl.B.nh * 16));
+      RT_TRY(device_copy(b, sl.bt.zmem_d, sl.zmem_prev, (size_t)sl.sys->n_hm * 16));
+    }
+    RT_TRY(update_bases(b, redo, ts, ta, sc));
+    for (int s : redo) {
+      Slot &sl = b.slots[s];
+      RT_TRY(device_copy(b, sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16));
+    }
+    RT_TRY(run_step(b, redo));
+    RT_TRY(define_bases(b, redo));
+    std::vector<int> nxt;
+    for (int s : redo) {
+      Slot &sl = b.slots[s];
+      if (++tries[s] >= sl.prm.inchworm_cap) continue;
+      if (!same_basis(sl.B, sl.sel.states_new, sl.sel.aux_new)) nxt.push_back(s);
+    }
+    redo.swap(nxt);
+  }
+  // final update of every defining slot
+  std::vector<std::vector<int>> ts;
+  std::vector<std::vector<Aux>> ta;
+  std::vector<double> sc;
+  std::vector<std::vector<int>> old_states;
+  std::vector<std::vector<cplx>> old_psi;
+  for (int s : upd) {
+    Slot &sl = b.slots[s];
+    ts.push_back(sl.sel.states_new);
+    ta.push_back(sl.sel.aux_new);
+    std::vector<cplx> psi(sl.pn.begin(), sl.pn.begin() + sl.B.n);
+    for (auto &v : psi) v *= sl.nrm;
+    std::vector<int> sn(sl.sel.states_new);
+    std::sort(sn.begin(), sn.end());
+    sc.push_back(keep_scale(sl, sn, psi));
+    old_states.push_back(sl.B.states);
+    old_psi.push_back(psi);
+  }
+  RT_TRY(update_bases(b, upd, ts, ta, sc));
+  for (size_t i = 0; i < upd.size(); ++i) {
+    Slot &sl = b.slots[upd[i]];
+    sl.scale = sl.changed ? sc[i] : 1.0;
+    record_step(sl, old_states[i], old_psi[i]);
+  }
+  // psi of the slots without a basis update this step
+  std::vector<int> rec_plain;
+  for (int s : plain)
+    if (b.slots[s].prm.record) rec_plain.push_back(s);
+  std::vector<int> early_static;
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    const bool adaptive = sl.prm.delta_a > 0 || sl.prm.delta_s > 0;
+    if (adaptive && sl.early_counter < sl.prm.early_steps && sl.prm.early_integrator == 1 && sl.prm.record)
+      rec_plain.push_back(s);
+  }
+  RT_TRY(fetch_psi(b, rec_plain));
+  for (int s : rec_plain) {
+    Slot &sl = b.slots[s];
+    Record r;
+    r.states = sl.B.states;
+    r.n_hier = sl.B.nh;
+    r.psi = sl.pn;
+    sl.rec.push_back(std::move(r));
+  }
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    const bool adaptive = sl.prm.delta_a > 0 || sl.prm.delta_s > 0;
+    if (adaptive && sl.early_counter < sl.prm.early_steps) sl.early_counter++;
+    sl.steps++;
+    sl.it += 2 * sl.prm.noise_step;
+  }
+  return 0;
+}
+
+int check_batch(void *p) {
+  if (!p) return rt_fail("batch is NULL");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *hops_rt_system_create(const hops_rt_system_desc_t *d) {
+  if (!d || d->n_state <= 0 || d->n_hmode <= 0 || d->n_lop <= 0 || !d->H || !d->g || !d->w || !d->l_of_mode ||
+      !d->l_ptr || !d->l_row || !d->l_col || !d->l_val) {
+    rt_fail("hops_rt_system_create: bad description");
+    return nullptr;
+  }
+  auto *S = new System();
+  S->n = d->n_state;
+  S->n_hm = d->n_hmode;
+  S->n_l = d->n_lop;
+  S->H.resize((size_t)S->n * S->n);
+  for (size_t i = 0; i < S->H.size(); ++i) S->H[i] = cplx(d->H[2 * i], d->H[2 * i + 1]);
+  for (int m = 0; m < S->n_hm; ++m) {
+    S->G.push_back(cplx(d->g[2 * m], d->g[2 * m + 1]));
+    S->W.push_back(cplx(d->w[2 * m], d->w[2 * m + 1]));
+    if (d->l_of_mode[m] < 0 || d->l_of_mode[m] >= S->n_l) {
+      delete S;
+      rt_fail("mode %d has L-operator index %d", m, d->l_of_mode[m]);
+      return nullptr;
+    }
+    S->l_of_mode.push_back(d->l_of_mode[m]);
+  }
+  S->L.resize(S->n_l);
+  for (int l = 0; l < S->n_l; ++l)
+    for (int q = d->l_ptr[l]; q < d->l_ptr[l + 1]; ++q) {
+      const int r = d->l_row[q], c = d->l_col[q];
+      if (r < 0 || r >= S->n || c < 0 || c >= S->n) {
+        delete S;
+        rt_fail("L-operator %d entry (%d, %d) out of range", l, r, c);
+        return nullptr;
+      }
+      S->L[l].row.push_back(r);
+      S->L[l].col.push_back(c);
+      S->L[l].val.push_back(cplx(d->l_val[2 * q], d->l_val[2 * q + 1]));
+      if (r != c) S->diag_L = false;
+    }
+  if (d->lt)
+    for (int l = 0; l < S->n_l; ++l) S->lt.push_back(cplx(d->lt[2 * l], d->lt[2 * l + 1]));
+  S->modes_by_state.resize(S->n);
+  S->l_by_state.resize(S->n);
+  S->coupled.resize(S->n);
+  S->dest.resize(S->n);
+  for (int l = 0; l < S->n_l; ++l) {
+    std::vector<int> touched(S->L[l].row);
+    touched.insert(touched.end(), S->L[l].col.begin(), S->L[l].col.end());
+    std::sort(touched.begin(), touched.end());
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    for (int s : touched) S->l_by_state[s].push_back(l);
+    for (size_t q = 0; q < S->L[l].row.size(); ++q) S->dest[S->L[l].row[q]].push_back(S->L[l].col[q]);
+  }
+  for (int m = 0; m < S->n_hm; ++m)
+    for (int s = 0; s < S->n; ++s)
+      if (std::find(S->l_by_state[s].begin(), S->l_by_state[s].end(), S->l_of_mode[m]) != S->l_by_state[s].end())
+        S->modes_by_state[s].push_back(m);
+  for (int r = 0; r < S->n; ++r)
+    for (int c = 0; c < S->n; ++c)
+      if (S->H[(size_t)r * S->n + c] != cplx(0, 0)) S->coupled[r].push_back(c);
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  S->zob.resize((size_t)S->n_hm * 256);
+  for (auto &z : S->zob) {  // splitmix64
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t v = x;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    z = v ^ (v >> 31);
+  }
+  for (int m = 0; m < S->n_hm; ++m) S->zob[(size_t)m * 256] = 0;  // depth 0 contributes nothing
+  return S;
+}
+
+int hops_rt_system_destroy(void *sys) {
+  delete static_cast<System *>(sys);
+  return 0;
+}
+
+void *hops_rt_batch_create(int precision, void *arena_d, size_t arena_bytes, int max_slots, int n_threads,
+                           void *stream) {
+  if (!arena_d || arena_bytes < (1 << 20) || max_slots <= 0 || max_slots > (1 << (32 - kSlotShift))) {
+    rt_fail("hops_rt_batch_create: bad arguments");
+    return nullptr;
+  }
+  if (precision != HOPS_PREC_FP64 && precision != HOPS_PREC_MIXED) {
+    rt_fail("unknown precision %d", precision);
+    return nullptr;
+  }
+  if (((uintptr_t)arena_d & 255) != 0) {
+    rt_fail("arena not 256-byte aligned");
+    return nullptr;
+  }
+  auto *b = new Batch();
+  b->precision = precision;
+  b->stream = (hipStream_t)stream;
+  b->arena.init((char *)arena_d, arena_bytes);
+  b->slots.resize(max_slots);
+  b->ops.assign(max_slots, hops_op_t{});
+  b->bts.assign(max_slots, hops_batch_t{});
+  b->perms.assign(max_slots, hops_slot_perm_t{});
+  b->its.assign(max_slots, 0);
+  b->dts.assign(max_slots, 1.0);
+  b->nrms.assign(max_slots, 1.0);
+  b->ops_off = b->arena.alloc(max_slots * sizeof(hops_op_t));
+  b->bts_off = b->arena.alloc(max_slots * sizeof(hops_batch_t));
+  b->perm_off = b->arena.alloc(max_slots * sizeof(hops_slot_perm_t));
+  b->it_off = b->arena.alloc(max_slots * 4);
+  b->dt_off = b->arena.alloc(max_slots * 8);
+  b->nrm_off = b->arena.alloc(max_slots * 8);
+  if (b->ops_off < 0 || b->bts_off < 0 || b->perm_off < 0 || b->it_off < 0 || b->dt_off < 0 || b->nrm_off < 0 ||
+      ensure_staging(*b, 4 << 20) < 0) {
+    rt_fail("arena too small for the batch descriptors");
+    delete b;
+    return nullptr;
+  }
+  const int nt = n_threads > 0 ? n_threads : 16;
+  b->pool.reset(new Pool(nt - 1));
+  return b;
+}
+
+int hops_rt_batch_destroy(void *p) {
+  auto *b = static_cast<Batch *>(p);
+  if (!b) return 0;
+  (void)hipStreamSynchronize(b->stream);
+  if (b->h_up) (void)hipHostFree(b->h_up);
+  if (b->h_down) (void)hipHostFree(b->h_down);
+  delete b;
+  return 0;
+}
+
+int hops_rt_batch_add(void *p, void *sys, const hops_rt_traj_param_t *prm, const double *psi0, const void *z1_d,
+                      const void *z2_d) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (!sys || !prm || !psi0 || !z1_d) return rt_fail("hops_rt_batch_add: NULL argument");
+  const System &S = *static_cast<System *>(sys);
+  int id = -1;
+  for (size_t s = 0; s < b.slots.size(); ++s)
+    if (!b.slots[s].used) {
+      id = (int)s;
+      break;
+    }
+  if (id < 0) return rt_fail("no free slot (max %zu)", b.slots.size());
+  Slot &sl = b.slots[id];
+  sl = Slot();
+  sl.sys = &S;
+  Params &P = sl.prm;
+  P.eom = prm->eom;
+  P.maxhier = std::min(prm->maxhier, 255);
+  P.update_step = prm->update_step;
+  P.early_integrator = prm->early_integrator;
+  P.early_steps = prm->early_steps;
+  P.inchworm_cap = prm->inchworm_cap;
+  P.noise_step = prm->noise_step;
+  P.n_t = prm->n_t;
+  P.it0 = prm->it0;
+  P.record = prm->record;
+  P.delta_a = prm->delta_a;
+  P.delta_s = prm->delta_s;
+  P.f_discard = prm->f_discard;
+  P.dt = prm->dt;
+  if (P.eom < 0 || P.eom > 2) return rt_fail("unknown eom %d", P.eom);
+  if (P.noise_step <= 0 || P.n_t <= 0 || P.it0 < 0 || P.it0 >= P.n_t) return rt_fail("bad noise grid");
+  if ((P.delta_a > 0 || P.delta_s > 0) && !S.diag_L)
+    return rt_fail("adaptive basis with off-diagonal L-operators is not supported");
+  if (prm->markov) P.markov.assign(prm->markov, prm->markov + S.n_hm);
+  if (P.delta_a <= 0) {
+    if (prm->n_aux0 <= 0 || !prm->aux0_ptr) return rt_fail("non-adaptive hierarchy needs aux0");
+    for (int i = 0; i < prm->n_aux0; ++i) {
+      Aux a;
+      for (int q = prm->aux0_ptr[i]; q < prm->aux0_ptr[i + 1]; ++q) a.push_back(aux_pack(prm->aux0_mode[q], prm->aux0_val[q]));
+      std::sort(a.begin(), a.end());
+      P.aux0.push_back(a);
+    }
+  }
+  std::vector<int> states;
+  sl.psi0.resize(S.n);
+  for (int s = 0; s < S.n; ++s) {
+    sl.psi0[s] = cplx(psi0[2 * s], psi0[2 * s + 1]);
+    if (P.delta_s <= 0 || std::abs(sl.psi0[s]) > 0) states.push_back(s);
+  }
+  std::vector<Aux> aux = P.delta_a > 0 ? std::vector<Aux>{Aux()} : P.aux0;
+  sl.z1 = z1_d;
+  sl.z2 = z2_d;
+  sl.it = P.it0;
+  RT_TRY(build_basis(S, P.eom, states, aux, sl.B));
+  sl.pers = b.arena.alloc(pers_bytes(S));
+  if (sl.pers < 0) return rt_fail("arena full");
+  std::vector<char> tabs;
+  hops_op_t op;
+  hops_batch_t bt;
+  if (int rc = place_basis(b, sl, sl.B, &sl.blk, op, bt, tabs); rc < 0) {
+    b.arena.release(sl.pers);
+    return rc;
+  }
+  sl.op = op;
+  sl.bt = bt;
+  b.ops[id] = op;
+  b.bts[id] = bt;
+  char *base = b.arena.ptr(sl.blk);
+  Layout L = layout_of(sl.B, b.precision);
+  sl.phi_prev = base + L.phi_prev;
+  sl.zmem_prev = reinterpret_cast<double *>(b.arena.ptr(sl.pers) + rsz(S.n_hm * 16 + 16));
+  RT_HIP(hipMemcpyAsync(base, tabs.data(), tabs.size(), hipMemcpyHostToDevice, b.stream));
+  std::vector<cplx> phi0((size_t)sl.B.n * sl.B.nh, cplx(0, 0));
+  for (int q = 0; q < sl.B.n; ++q) phi0[q] = sl.psi0[sl.B.states[q]];
+  RT_HIP(hipMemcpyAsync(bt.phi_d, phi0.data(), phi0.size() * 16, hipMemcpyHostToDevice, b.stream));
+  RT_HIP(hipMemsetAsync(b.arena.ptr(sl.pers), 0, pers_bytes(S), b.stream));
+  // super-operator of the initial basis
+  std::vector<int> one{id};
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.ops_off) + id * sizeof(hops_op_t), &b.ops[id], sizeof(hops_op_t),
+                        hipMemcpyHostToDevice, b.stream));
+  RT_TRY(hops_build_super(&b.ops[id], b.precision, b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  sl.used = true;
+  sl.pending_init = true;
+  return id;
+}
+
+int hops_rt_batch_remove(void *p, int slot) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  RT_HIP(hipStreamSynchronize(b.stream));
+  Slot &sl = b.slots[slot];
+  b.arena.release(sl.blk);
+  b.arena.release(sl.pers);
+  sl = Slot();
+  b.ops[slot] = hops_op_t{};
+  b.bts[slot] = hops_batch_t{};
+  return 0;
+}
+
+long long hops_rt_slot_estimate(void *p, void *sys, int n_state, int n_hier) {
+  if (!p || !sys) return rt_fail("NULL argument");
+  auto &b = *static_cast<Batch *>(p);
+  const System &S = *static_cast<System *>(sys);
+  const size_t nr = (size_t)n_state * n_hier, es = stage_bytes(b.precision);
+  const size_t ell_w = 2 * (size_t)std::max(1, std::min(S.n_hm, 2 * n_state));  // up + down per row
+  const size_t tables = (size_t)n_hier * (16 + 8 * ell_w) + (size_t)n_state * n_state * 48 + 4096;
+  return (long long)(pers_bytes(S) + tables + ell_w * nr * (4 + es) + nr * (16 * 2 + 3 * es + 8) + n_hier * 16);
+}
+
+long long hops_rt_arena_free(void *p) {
+  if (!p) return rt_fail("batch is NULL");
+  return (long long)static_cast<Batch *>(p)->arena.free_bytes();
+}
+
+int hops_rt_batch_step(void *p, int n_steps, const int32_t *slots_or_null, int n_slots) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  RT_TRY(init_pending(b));
+  std::vector<int> list;
+  if (slots_or_null) {
+    for (int i = 0; i < n_slots; ++i) {
+      const int s = slots_or_null[i];
+      if (s < 0 || s >= (int)b.slots.size() || !b.slots[s].used) return rt_fail("no slot %d", s);
+      list.push_back(s);
+    }
+  } else {
+    for (size_t s = 0; s < b.slots.size(); ++s)
+      if (b.slots[s].used) list.push_back((int)s);
+  }
+  for (int s : list) {
+    const Slot &sl = b.slots[s];
+    if (sl.it + (long)n_steps * 2 * sl.prm.noise_step >= sl.prm.n_t)
+      return rt_fail("slot %d: trajectory times longer than the noise (index %ld >= %d)", s,
+                     sl.it + (long)n_steps * 2 * sl.prm.noise_step, sl.prm.n_t);
+  }
+  for (int k = 0; k < n_steps; ++k) RT_TRY(batch_step_once(b, list, k));
+  return 0;
+}
+
+int hops_rt_slot_info(void *p, int slot, long long *info) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !info) return rt_fail("bad slot %d", slot);
+  const Slot &sl = b.slots[slot];
+  info[0] = sl.B.n;
+  info[1] = sl.B.nh;
+  info[2] = sl.steps;
+  info[3] = sl.it;
+  info[4] = (long long)sl.rec.size();
+  info[5] = (long long)(sl.blk_bytes + (sl.sys ? pers_bytes(*sl.sys) : 0));
+  info[6] = std::max(0, sl.prm.early_steps - sl.early_counter);
+  info[7] = sl.used;
+  return 0;
+}
+
+int hops_rt_slot_states(void *p, int slot, int32_t *out) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  for (int i = 0; i < b.slots[slot].B.n; ++i) out[i] = b.slots[slot].B.states[i];
+  return 0;
+}
+
+int hops_rt_slot_aux(void *p, int slot, int32_t *ptr, int32_t *mode, int32_t *val) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  const Basis &B = b.slots[slot].B;
+  std::copy(B.aux_ptr.begin(), B.aux_ptr.end(), ptr);
+  std::copy(B.aux_mode_t.begin(), B.aux_mode_t.end(), mode);
+  std::copy(B.aux_val_t.begin(), B.aux_val_t.end(), val);
+  return 0;
+}
+
+int hops_rt_slot_phi(void *p, int slot, double *out) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  const Slot &sl = b.slots[slot];
+  RT_HIP(hipMemcpyAsync(out, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16, hipMemcpyDeviceToHost, b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  return 0;
+}
+
+int hops_rt_slot_zmem(void *p, int slot, double *out) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  const Slot &sl = b.slots[slot];
+  RT_HIP(hipMemcpyAsync(out, sl.bt.zmem_d, (size_t)sl.sys->n_hm * 16, hipMemcpyDeviceToHost, b.stream));
+  RT_HIP(hipStreamSynchronize(b.stream));
+  return 0;
+}
+
+int hops_rt_slot_record(void *p, int slot, int idx, double *psi_out, int32_t *states_out, int32_t *n_state_out,
+                        int32_t *n_hier_out) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size()) return rt_fail("bad slot %d", slot);
+  const Slot &sl = b.slots[slot];
+  if (idx < 0 || idx >= (int)sl.rec.size()) return rt_fail("slot %d has no record %d", slot, idx);
+  const Record &r = sl.rec[idx];
+  if (n_state_out) *n_state_out = (int32_t)r.states.size();
+  if (n_hier_out) *n_hier_out = r.n_hier;
+  if (psi_out)
+    for (size_t i = 0; i < r.psi.size(); ++i) {
+      psi_out[2 * i] = r.psi[i].real();
+      psi_out[2 * i + 1] = r.psi[i].imag();
+    }
+  if (states_out) std::copy(r.states.begin(), r.states.end(), states_out);
+  return 0;
+}
+
+int hops_rt_slot_clear_records(void *p, int slot) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size()) return rt_fail("bad slot %d", slot);
+  b.slots[slot].rec.clear();
+  return 0;
+}
+
+// ---- host-only hooks -------------------------------------------------------
+struct DebugBasis {
+  Basis B;
+  Selection sel;
+  Params P;
+  std::vector<cplx> pn;
+};
+
+void *hops_rt_debug_basis(void *sys, int eom, int n_states, const int32_t *states, int n_aux, const int32_t *aux_ptr,
+                          const int32_t *amode, const int32_t *aval) {
+  if (!sys) return nullptr;
+  auto *d = new DebugBasis();
+  std::vector<int> st(states, states + n_states);
+  std::vector<Aux> aux;
+  for (int i = 0; i < n_aux; ++i) {
+    Aux a;
+    for (int q = aux_ptr[i]; q < aux_ptr[i + 1]; ++q) a.push_back(aux_pack(amode[q], aval[q]));
+    std::sort(a.begin(), a.end());
+    aux.push_back(a);
+  }
+  if (build_basis(*static_cast<System *>(sys), eom, st, aux, d->B) < 0) {
+    delete d;
+    return nullptr;
+  }
+  return d;
+}
+
+long long hops_rt_debug_table(void *p, const char *name, void *out, long long cap) {
+  if (!p || !name) return rt_fail("NULL argument");
+  const Basis &B = static_cast<DebugBasis *>(p)->B;
+  std::string nm(name);
+  auto give = [&](const void *src, size_t bytes) -> long long {
+    if (out && (long long)bytes <= cap) std::memcpy(out, src, bytes);
+    return (long long)bytes;
+  };
+#define TAB(key, vec) \
+  if (nm == key) return give(B.vec.data(), B.vec.size() * sizeof(B.vec[0]));
+  TAB("aux_ptr", aux_ptr) TAB("aux_mode", aux_mode_t) TAB("aux_val", aux_val_t) TAB("up_ptr", up_ptr)
+  TAB("up_aux", up_aux) TAB("up_mode", up_mode) TAB("dn_ptr", dn_ptr) TAB("dn_aux", dn_aux)
+  TAB("dn_mode", dn_mode) TAB("dn_val", dn_val) TAB("lr_ptr", lr_ptr) TAB("lr_col", lr_col)
+  TAB("lr_val", lr_val) TAB("lc_ptr", lc_ptr) TAB("lc_row", lc_row) TAB("lc_val", lc_val)
+  TAB("mode_lrel", mode_lrel32) TAB("mode_lact", mode_lact32) TAB("mode_abs", mode_abs32)
+  TAB("mode_g", mode_g) TAB("mode_w", mode_w) TAB("abs_g", abs_g) TAB("abs_w", abs_w)
+  TAB("abs_lact", abs_lact) TAB("act_lrel", act_lrel32) TAB("act_zrow", act_zrow) TAB("am_ptr", am_ptr)
+  TAB("am_mode", am_mode) TAB("la_ptr", la_ptr) TAB("la_row", la_row) TAB("la_col", la_col)
+  TAB("la_val", la_val) TAB("lq_ptr", lq_ptr) TAB("lq_row", lq_row) TAB("lq_col", lq_col)
+  TAB("lq_val", lq_val) TAB("lt", lt) TAB("first_aux", first_aux) TAB("first_lact", first_lact)
+  TAB("first_mode", first_mode) TAB("pat_ptr", pat_ptr) TAB("pat_col", pat_col) TAB("pat_h", pat_h)
+  TAB("pc_ptr", pc_ptr) TAB("pc_act", pc_act) TAB("pc_kind", pc_kind) TAB("pc_val", pc_val)
+#undef TAB
+  if (nm == "sizes") {
+    const int32_t v[10] = {B.n, B.nh, B.nm, B.nl, B.na, (int32_t)B.first_aux.size(), B.ell_w, B.n_pat,
+                           B.sys->n_hm, B.pat_w};
+    return give(v, sizeof(v));
+  }
+  if (nm == "has_lt") {
+    const int32_t v = B.has_lt;
+    return give(&v, sizeof(v));
+  }
+  return rt_fail("unknown table %s", name);
+}
+
+int hops_rt_debug_select(void *p, const hops_rt_traj_param_t *prm, const double *pn, const double *e_aux, double dt,
+                         uint8_t *stable_out) {
+  if (!p || !prm || !pn || !e_aux) return rt_fail("NULL argument");
+  auto *d = static_cast<DebugBasis *>(p);
+  d->P = Params();
+  d->P.eom = prm->eom;
+  d->P.maxhier = prm->maxhier;
+  d->P.delta_a = prm->delta_a;
+  d->P.delta_s = prm->delta_s;
+  d->P.f_discard = prm->f_discard;
+  if (prm->markov) d->P.markov.assign(prm->markov, prm->markov + d->B.sys->n_hm);
+  (void)dt;
+  const size_t N = (size_t)d->B.n * d->B.nh;
+  d->pn.resize(N);
+  for (size_t i = 0; i < N; ++i) d->pn[i] = cplx(pn[2 * i], pn[2 * i + 1]);
+  select_hier(d->P, d->B, d->pn.data(), e_aux, d->sel);
+  if (stable_out) std::copy(d->sel.stable_mask.begin(), d->sel.stable_mask.end(), stable_out);
+  return 0;
+}
+
+int hops_rt_debug_select_states(void *p, const double *e_state) {
+  if (!p) return rt_fail("NULL argument");
+  auto *d = static_cast<DebugBasis *>(p);
+  if (d->P.delta_s > 0 && !e_state) return rt_fail("state errors missing");
+  select_states(d->P, d->B, d->pn.data(), e_state, d->sel);
+  return 0;
+}
+
+int hops_rt_debug_result(void *p, int32_t *n_states, int32_t *states_out, int32_t *n_aux, int32_t *aux_ptr,
+                         int32_t *amode, int32_t *aval, long long cap) {
+  if (!p) return rt_fail("NULL argument");
+  auto *d = static_cast<DebugBasis *>(p);
+  const Selection &s = d->sel;
+  *n_states = (int32_t)s.states_new.size();
+  *n_aux = (int32_t)s.aux_new.size();
+  long long ent = 0;
+  for (const Aux &a : s.aux_new) ent += (long long)a.size();
+  if ((long long)s.states_new.size() > cap || (long long)s.aux_new.size() + 1 > cap || ent > cap) return 1;
+  std::copy(s.states_new.begin(), s.states_new.end(), states_out);
+  aux_ptr[0] = 0;
+  int q = 0;
+  for (size_t i = 0; i < s.aux_new.size(); ++i) {
+    for (char32_t e : s.aux_new[i]) {
+      amode[q] = aux_mode(e);
+      aval[q] = aux_val(e);
+      ++q;
+    }
+    aux_ptr[i + 1] = q;
+  }
+  return 0;
+}
+
+int hops_rt_debug_free(void *p) {
+  delete static_cast<DebugBasis *>(p);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+"""Ragged batches on the device (native runtime + slot kernels) against the
+FP64 oracle: every trajectory of a batch keeps its own adaptive basis; each
+must reproduce the oracle's basis sequence exactly and its psi(t) to 1e-10
+(FP64), be bitwise the same alone and in a mix, and keep the population
+error <= 1e-4 in mixed precision."""
+import numpy as np
+import pytest
+
+import systems
+from oracle import hops_oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+ARENA = 1 << 30
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _noise(sysp, seed, tlen, tau):
+    return O.Noise({"SEED": seed, "MODEL": "FFT_FILTER", "TLEN": tlen, "TAU": tau}, O.derive_system(sysp), 1).noise
+
+
+def _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0, update=1, markov=None):
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    tr = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
+                              delta_a=da, delta_s=ds, update_step=update, markov=markov)
+    tr.initialize(psi0)
+    tr.propagate(t_max, dt)
+    return tr
+
+
+CASES = {
+    "dod": (systems.dimer_of_dimers(), "NORMALIZED NONLINEAR", 3, 1e-3, 1e-3, 60.0, 2.0, 1),
+    "chain8_states": (systems.chain(8), "NONLINEAR", 2, 0.0, 2e-3, 40.0, 2.0, 1),
+    "chain10_update2": (systems.chain(10), "NORMALIZED NONLINEAR", 4, 2e-3, 2e-3, 40.0, 1.0, 2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_adaptive_batch_matches_oracle(name):
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp, eom, K, da, ds, t_max, dt, update = CASES[name]
+    nst = sysp["HAMILTONIAN"].shape[0]
+    tlen, tau = t_max + 20.0, 0.5
+    seeds = [3, 17, 29, 41]
+    zs = [_noise(sysp, s, tlen, tau) for s in seeds]
+    psi0 = np.zeros(nst, complex)
+    psi0[nst // 2] = 1.0
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+    ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], None, {"MAXHIER": K}, noise_params=nps,
+                       precision_mode="fp64", arena_bytes=ARENA)
+    ab.make_adaptive(da, ds, update_step=update)
+    ab.initialize(psi0)
+    ab.propagate(t_max, dt)
+    for b, z in enumerate(zs):
+        print("case", name, "trajectory", b, flush=True)
+        ref = _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0, update)
+        psi, states, nhier = ab.results(b)
+        assert len(states) == len(ref.state_traj)
+        for i, (s_got, s_ref) in enumerate(zip(states, ref.state_traj)):
+            assert list(s_got) == list(s_ref), (b, i, s_got, s_ref)
+            assert nhier[i] == len(ref.aux_traj[i]), (b, i)
+        assert set(ab.auxiliary_list(b)) == set(ref.basis.aux_list)
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        print("ragged %s traj %d: max |dpsi| = %.3e, n_hier %d..%d" % (name, b, err, min(nhier), max(nhier)))
+        assert err < FP64_TOL
+        assert np.max(np.abs(ab.phi(b) - ref.phi)) < FP64_TOL
+        np.testing.assert_allclose(ab.z_mem(b), ref.z_mem, rtol=0, atol=1e-9 * max(1, np.max(np.abs(ref.z_mem))))
+
+
+def test_alone_equals_in_mix():
+    """A trajectory's result does not depend on which others share its batch."""
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp = systems.chain(10)
+    tlen, tau = 80.0, 0.5
+    psi0 = np.zeros(10, complex)
+    psi0[5] = 1.0
+    zs = [_noise(sysp, s, tlen, tau) for s in (5, 6, 7)]
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+
+    def run(params):
+        ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}, params[0], None, {"MAXHIER": 4},
+                           noise_params=params, arena_bytes=ARENA)
+        ab.make_adaptive(2e-3, 2e-3)
+        ab.initialize(psi0)
+        ab.propagate(50.0, 1.0)
+        return ab
+
+    mix = run(nps)
+    alone = run([nps[1]])
+    p_mix, s_mix, h_mix = mix.results(1)
+    p_alone, s_alone, h_alone = alone.results(0)
+    assert h_mix == h_alone
+    assert np.array_equal(p_mix, p_alone)
+
+
+def test_mixed_precision_population_error():
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp = systems.chain(10)
+    tlen, tau = 80.0, 0.5
+    psi0 = np.zeros(10, complex)
+    psi0[5] = 1.0
+    zs = [_noise(sysp, s, tlen, tau) for s in (1, 2)]
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+    ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}, nps[0], None, {"MAXHIER": 4},
+                       noise_params=nps, precision_mode="mixed", arena_bytes=ARENA)
+    ab.make_adaptive(2e-3, 2e-3)
+    ab.initialize(psi0)
+    ab.propagate(50.0, 1.0)
+    for b, z in enumerate(zs):
+        ref = _oracle(sysp, "NORMALIZED NONLINEAR", z, tlen, tau, 4, 2e-3, 2e-3, 50.0, 1.0, psi0)
+        psi, _, _ = ab.results(b)
+        err = np.max(np.abs(np.abs(psi) ** 2 - np.abs(np.array(ref.psi_traj)) ** 2))
+        print("mixed ragged traj %d: max population error %.3e" % (b, err))
+        assert err < 1e-4
+
+
+def test_ragged_batch_of_different_systems():
+    """Slots with different systems and hierarchy depths in one batch."""
+    _need_gpu()
+    from mesohops_amd.basis import SystemInfo
+    from mesohops_amd.ragged import RaggedBatch, RtSystem, traj_param
+    tlen, tau = 60.0, 0.5
+    specs = [(systems.chain(6), 3, 2e-3), (systems.dimer_of_dimers(), 2, 1e-3), (systems.chain(9, seed_h=4), 4, 3e-3)]
+    rb = RaggedBatch("fp64", ARENA, max_slots=8)
+    keep = []
+    for i, (sysp, K, d) in enumerate(specs):
+        info = SystemInfo(sysp)
+        rs = RtSystem(info)
+        z = _noise(sysp, 100 + i, tlen, tau)
+        zt = torch.from_numpy(np.ascontiguousarray(z)).cuda()
+        psi0 = np.zeros(info.nstates, complex)
+        psi0[info.nstates // 2] = 1.0
+        prm = traj_param(2, K, d, d, 1, 0.01, 2.0, 2, z.shape[1], 0)
+        slot = rb.add(rs, prm, psi0, zt)
+        keep.append((slot, sysp, K, d, z, psi0, info, rs, zt, prm))
+    rb.step(20)
+    for slot, sysp, K, d, z, psi0, info, *_ in keep:
+        ref = _oracle(sysp, "NORMALIZED NONLINEAR", z, tlen, tau, K, d, d, 40.0, 2.0, psi0)
+        psi, states, nhier = rb.records(slot, info.nstates)
+        assert [list(s) for s in states] == [list(s) for s in ref.state_traj]
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        print("mixed-system slot %d: max |dpsi| = %.3e" % (slot, err))
+        assert err < FP64_TOL

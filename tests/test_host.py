@@ -17,9 +17,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "hops_mi355x.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(hops_[a-z_0-9]+)\s*\(", txt)))
+    out = set()
+    for h in ("hops_mi355x.h", "hops_runtime.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        out |= set(re.findall(r"\b(hops_[a-z_0-9]+)\s*\(", txt))
+    return sorted(out)
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,122 @@
+"""The native runtime's host half (mesohops_amd/csrc/hops_runtime.cpp), without
+a GPU: its basis tables equal mesohops_amd/basis.py::BasisTables (which the
+oracle pins), and its basis selection equals the oracle's define_basis on the
+bases and hierarchy vectors of oracle adaptive trajectories (derivative sums
+from the oracle's dsystem_dt)."""
+import numpy as np
+import pytest
+
+import systems
+from oracle import hops_oracle as O
+
+INT_TABLES = ["aux_ptr", "aux_mode", "aux_val", "up_ptr", "up_aux", "up_mode", "dn_ptr", "dn_aux", "dn_mode",
+              "dn_val", "lr_ptr", "lr_col", "lc_ptr", "lc_row", "mode_lrel", "mode_lact", "mode_abs", "abs_lact",
+              "act_lrel", "am_ptr", "am_mode", "la_ptr", "la_row", "la_col", "lq_ptr", "lq_row", "lq_col",
+              "first_aux", "first_lact", "first_mode", "pat_ptr", "pat_col", "pc_ptr", "pc_act", "pc_kind"]
+CPLX_TABLES = ["lr_val", "lc_val", "mode_g", "mode_w", "abs_g", "abs_w", "la_val", "lq_val", "pat_h", "pc_val"]
+
+
+def _markov_chain(nsite):
+    import scipy.sparse as sp
+    g0, w0 = O.bcf_convert_dl_to_exp(50.0, 50.0, 295.0)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        gw += [[g0, w0], [-1j * np.imag(g0), 500.0]]
+        ll += [L, L]
+    H = (np.eye(nsite, k=1) + np.eye(nsite, k=-1)) * 50.0
+    return dict(HAMILTONIAN=np.array(H, complex), GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll,
+                ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw), [False, True] * nsite
+
+
+def _bases(sysp, rng):
+    from mesohops_amd.basis import SystemInfo, triangular_aux_list
+    info = SystemInfo(sysp)
+    full = triangular_aux_list(info.n_hmodes, 3)
+    out = [(list(range(info.nstates)), full)]
+    for _ in range(3):
+        k = rng.integers(1, info.nstates + 1)
+        st = sorted(rng.choice(info.nstates, size=k, replace=False).tolist())
+        aux = [()] + [a for a in full[1:] if rng.random() < 0.5]
+        out.append((st, aux))
+    return info, out
+
+
+@pytest.mark.parametrize("name", ["dod", "chain_ltc", "three_diag", "three_general"])
+def test_runtime_tables_equal_host_tables(name):
+    from mesohops_amd.basis import BasisTables
+    from mesohops_amd.ragged import DebugBasis, RtSystem
+    sysp = {"dod": systems.dimer_of_dimers(), "chain_ltc": systems.chain(5, ltc=12.0 - 3.0j),
+            "three_diag": systems.three_site(False), "three_general": systems.three_site(True)}[name]
+    rng = np.random.default_rng(5)
+    info, bases = _bases(sysp, rng)
+    rs = RtSystem(info)
+    for states, aux in bases:
+        try:
+            tb = BasisTables(info, states, aux, 2)
+        except Exception:
+            continue
+        db = DebugBasis(rs, states, aux, 2)
+        sizes = db.table("sizes", np.int32)
+        assert list(sizes[:8]) == [tb.n_state, tb.n_hier, tb.n_mode, tb.n_lop, tb.n_act, len(tb.first), tb.ell_w,
+                                   tb.n_pat], (name, states)
+        for t in INT_TABLES:
+            np.testing.assert_array_equal(db.table(t, np.int32), np.asarray(tb.tables[t], np.int32).reshape(-1),
+                                          err_msg=t)
+        for t in CPLX_TABLES:
+            np.testing.assert_array_equal(db.table(t, np.complex128), tb.tables[t].view(np.complex128), err_msg=t)
+        assert bool(db.table("has_lt", np.int32)[0]) == tb.has_lt
+        if tb.has_lt:
+            np.testing.assert_array_equal(db.table("lt", np.complex128), tb.tables["lt"].view(np.complex128))
+        np.testing.assert_array_equal(db.table("act_zrow", np.int32), tb.absL_active.astype(np.int32))
+
+
+CASES = {
+    "dod_both": (systems.dimer_of_dimers(), None, 1e-3, 1e-3, 3, "NORMALIZED NONLINEAR"),
+    "chain_hier_only": (systems.chain(6), None, 1e-3, 0.0, 4, "NORMALIZED NONLINEAR"),
+    "chain_states_only": (systems.chain(8), None, 0.0, 2e-3, 2, "NONLINEAR"),
+    "markov_chain": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR"),
+    "chain12": (systems.chain(12), None, 2e-3, 2e-3, 5, "NORMALIZED NONLINEAR"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_runtime_selection_matches_oracle(name):
+    from mesohops_amd.basis import SystemInfo
+    from mesohops_amd.ragged import DebugBasis, RtSystem, traj_param
+    sysp, markov, da, ds, K, eom = CASES[name]
+    nst = sysp["HAMILTONIAN"].shape[0]
+    noise = {"SEED": 9, "MODEL": "FFT_FILTER", "TLEN": 200.0, "TAU": 1.0}
+    tr = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
+                              delta_a=da, delta_s=ds, markov=markov)
+    psi0 = np.zeros(nst, complex)
+    psi0[nst // 2] = 1.0
+    tr.initialize(psi0)
+    info = SystemInfo(sysp)
+    rs = RtSystem(info)
+    prm = traj_param(2, K, da, ds, 1, 0.01, 2.0, 1, 10, 0, markov=markov)
+    dt = 2.0
+    sizes = set()
+    for step in range(14):
+        tr.propagate(dt, dt)
+        b = tr.basis
+        z1, z2, zm = tr.z_step(tr.z_mem)
+        ref = tr.ad.define_basis(tr.dsystem_dt, b, tr.phi, dt, [z1, z2, zm])
+        n, nh = b.n_state, b.n_hier
+        pn = tr.phi / np.linalg.norm(tr.phi[:n])
+        D = tr.dsystem_dt(pn, zm, z1, z2)[0] / O.HBAR + pn / dt
+        e_aux = np.sum(np.abs(D.reshape(nh, n)) ** 2, axis=1)
+        db = DebugBasis(rs, b.state_list, b.aux_list, 2)
+        stable = db.select(prm, pn, e_aux, dt)
+        e_state = None
+        if ds > 0:
+            pst = np.zeros_like(pn)
+            pst.reshape(nh, n)[stable] = pn.reshape(nh, n)[stable]
+            D2 = tr.dsystem_dt(pst, zm, z1, z2)[0] / O.HBAR + pst / dt
+            e_state = np.sum(np.abs(D2.reshape(nh, n)[stable]) ** 2, axis=0)
+        db.select_states(e_state)
+        st_new, aux_new = db.result()
+        assert st_new == [int(s) for s in ref[0]], (step, st_new, ref[0])
+        assert set(aux_new) == set(ref[1]), step
+        sizes.add((n, nh))
+    assert len(sizes) > 1
