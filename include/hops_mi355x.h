@@ -241,6 +241,17 @@ int hops_slots_errors(const hops_op_t *ops_d, const hops_batch_t *bts_d, const i
 int hops_slots_build(const hops_op_t *ops_d, const int32_t *list_d, int n_list, const uint32_t *tiles_d,
                      int n_tiles, int precision, void *stream);
 
+/* Byte copies from a staging area into per-slot device blocks, one launch for
+   all segments (no counterpart in the reference: it uploads the tables of the
+   new bases of a whole batch at once).  bytes: multiple of 16, src and dst
+   16-byte aligned. */
+typedef struct hops_segment {
+  const void *src;
+  void *dst;
+  long long bytes;
+} hops_segment_t;
+int hops_copy_segments(const hops_segment_t *segs_d, int n, void *stream);
+
 /* Phi of the new bases; tiles run over the new rows of each slot and index
    perm_d by slot. */
 int hops_slots_permute(const hops_slot_perm_t *perm_d, const uint32_t *tiles_d, int n_tiles, void *stream);

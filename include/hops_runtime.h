@@ -81,6 +81,11 @@ int hops_rt_batch_remove(void *batch, int slot);
    admission by HBM footprint. */
 long long hops_rt_slot_estimate(void *batch, void *sys, int n_state, int n_hier);
 long long hops_rt_arena_free(void *batch);
+/* Accumulated wall seconds per phase of hops_rt_batch_step (no counterpart
+   in the reference): [0] RK4 steps, [1] error sums on the device, [2] host
+   hierarchy selection, [3] state errors and selection, [4] table build of
+   new bases, [5] device update (operators, Phi moves), [6] psi fetch. */
+int hops_rt_batch_timers(void *batch, double *out /*[8]*/);
 
 /* Advance every active slot (slots_or_null == NULL) or the listed slots by
    n_steps steps of their own dt. */

@@ -1238,6 +1238,16 @@ __global__ __launch_bounds__(256) void k_permute_slots(const hops_slot_perm_t *_
   reinterpret_cast<double2 *>(p.new_phi_d)[r] = v;
 }
 
+// byte segments (16-byte multiples, 16-byte aligned) from a staging area to
+// their destinations; one workgroup per segment
+__global__ __launch_bounds__(256) void k_copy_segments(const hops_segment_t *__restrict__ segs) {
+  const hops_segment_t sg = segs[blockIdx.x];
+  const long n = sg.bytes >> 4;
+  const uint4 *src = reinterpret_cast<const uint4 *>(sg.src);
+  uint4 *dst = reinterpret_cast<uint4 *>(sg.dst);
+  for (long i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 template <typename R>
 int slots_step_t(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *list, int n_list,
                  const uint32_t *tiles, int n_tiles, const int32_t *it, size_t shm, bool mixed,
@@ -1487,6 +1497,14 @@ int hops_slots_permute(const hops_slot_perm_t *perm_d, const uint32_t *tiles_d, 
   if (n_tiles <= 0) return 0;
   if (!perm_d || !tiles_d) return fail("hops_slots_permute: NULL argument");
   k_permute_slots<<<n_tiles, 256, 0, (hipStream_t)stream>>>(perm_d, tiles_d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_copy_segments(const hops_segment_t *segs_d, int n, void *stream) {
+  if (n <= 0) return 0;
+  if (!segs_d) return fail("hops_copy_segments: NULL argument");
+  k_copy_segments<<<n, 256, 0, (hipStream_t)stream>>>(segs_d);
   HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -25,6 +25,7 @@
 #include <array>
 #include <atomic>
 #include <complex>
+#include <chrono>
 #include <condition_variable>
 #include <cmath>
 #include <cstdarg>
@@ -77,6 +78,13 @@ int rt_fail(const char *fmt, ...) {
 // Auxiliary multi-index: packed (mode << 8 | depth) entries sorted by mode.
 // ---------------------------------------------------------------------------
 typedef std::u32string Aux;
+
+// complex products without the inf/nan recovery path of std::complex (the
+// values here are finite); same formula, same rounding
+inline cplx cm(cplx a, cplx b) {
+  return cplx(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+inline cplx cmi(cplx a) { return cplx(a.imag(), -a.real()); }  // -i a
 
 inline int aux_mode(char32_t e) { return (int)(e >> 8); }
 inline int aux_val(char32_t e) { return (int)(e & 0xff); }
@@ -151,7 +159,7 @@ struct System {
   std::vector<std::vector<int>> modes_by_state, l_by_state, coupled, dest;
   std::vector<int> site_of_mode;  // the single state of a site-projector mode, or -1
   bool diag_L = true;
-  std::vector<uint64_t> zob;      // Zobrist keys [n_hm * 256]
+  std::vector<uint64_t> zob, zob2;  // two independent Zobrist tables [n_hm * 256]
 };
 
 // ---------------------------------------------------------------------------
@@ -173,7 +181,7 @@ struct Basis {
   bool has_lt = false;
   std::vector<int> modes, rel_mode, absL, mode_lrel, mode_lact, act_abs, act_lrel, pos;
   std::vector<RedL> L;
-  std::vector<uint64_t> aux_hash;
+  std::vector<uint64_t> aux_hash, aux_hash2;
   std::vector<int> depth;
   // device tables
   std::vector<int32_t> aux_ptr, aux_mode_t, aux_val_t, up_ptr, up_aux, up_mode, dn_ptr, dn_aux, dn_mode, dn_val,
@@ -181,13 +189,12 @@ struct Basis {
       am_ptr, am_mode, la_ptr, la_row, la_col, lq_ptr, lq_row, lq_col, first_aux, first_lact, first_mode,
       pat_ptr, pat_col, pc_ptr, pc_act, pc_kind;
   std::vector<cplx> lr_val, lc_val, mode_g, mode_w, abs_g, abs_w, la_val, lq_val, lt, pat_h, pc_val;
+  // destroy the contents of o on this thread
+  void swap_out(Basis &o) {
+    Basis t(std::move(o));
+    o = Basis();
+  }
 };
-
-uint64_t aux_hash_of(const System &S, const Aux &a) {
-  uint64_t h = 0;
-  for (char32_t e : a) h ^= S.zob[(size_t)aux_mode(e) * 256 + aux_val(e)];
-  return h;
-}
 
 int build_basis(const System &S, int eom, std::vector<int> states, std::vector<Aux> aux, Basis &B) {
   B = Basis();
@@ -195,8 +202,20 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   B.eom = eom;
   std::sort(states.begin(), states.end());
   states.erase(std::unique(states.begin(), states.end()), states.end());
-  std::sort(aux.begin(), aux.end(), aux_less);
-  aux.erase(std::unique(aux.begin(), aux.end()), aux.end());
+  {
+    // reference order with the depth of every auxiliary computed once
+    std::vector<std::pair<int, int>> key(aux.size());
+    for (size_t i = 0; i < aux.size(); ++i) key[i] = {aux_depth(aux[i]), (int)i};
+    std::sort(key.begin(), key.end(), [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+      if (x.first != y.first) return x.first < y.first;
+      return aux_less(aux[x.second], aux[y.second]);
+    });
+    std::vector<Aux> sorted;
+    sorted.reserve(aux.size());
+    for (auto &k : key)
+      if (sorted.empty() || sorted.back() != aux[k.second]) sorted.push_back(std::move(aux[k.second]));
+    aux.swap(sorted);
+  }
   if (aux.empty() || !aux[0].empty()) return rt_fail("Zero Vector not contained in list_aux!");
   if (states.empty()) return rt_fail("empty state basis");
   B.states = states;
@@ -207,9 +226,16 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   B.index.reserve((size_t)nh * 2);
   for (int i = 0; i < nh; ++i) B.index.emplace(aux[i], i);
   B.aux_hash.resize(nh);
+  B.aux_hash2.resize(nh);
   B.depth.resize(nh);
   for (int i = 0; i < nh; ++i) {
-    B.aux_hash[i] = aux_hash_of(S, aux[i]);
+    uint64_t h1 = 0, h2 = 0;
+    for (char32_t e : aux[i]) {
+      h1 ^= S.zob[(size_t)aux_mode(e) * 256 + aux_val(e)];
+      h2 ^= S.zob2[(size_t)aux_mode(e) * 256 + aux_val(e)];
+    }
+    B.aux_hash[i] = h1;
+    B.aux_hash2[i] = h2;
     B.depth[i] = aux_depth(aux[i]);
   }
   // modes: in use by the aux list, or of the states (hops_modes.py:111-171)
@@ -315,37 +341,50 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
       B.aux_val_t.push_back(aux_val(e));
     }
   }
-  // connections (hops_hierarchy.py:470-523): j = i + e_m
-  std::vector<std::vector<std::pair<int, int>>> up(nh);
-  std::vector<std::vector<std::array<int, 3>>> dn(nh);
+  // connections (hops_hierarchy.py:470-523): j = i + e_m, as flat lists
+  // sorted by (aux, relative mode)
+  struct Conn {
+    int32_t key, lm, other, v;
+  };
+  std::vector<Conn> ups, dns;
   {
     Aux tmp;
+    ups.reserve((size_t)nh * 4);
+    dns.reserve((size_t)nh * 4);
     for (int j = 0; j < nh; ++j)
       for (char32_t e : aux[j]) {
         const int m = aux_mode(e), v = aux_val(e);
         aux_step(aux[j], m, -1, tmp);
         auto it = B.index.find(tmp);
         if (it == B.index.end()) continue;
-        up[it->second].push_back({j, B.rel_mode[m]});
-        dn[j].push_back({it->second, B.rel_mode[m], v});
+        ups.push_back({it->second, B.rel_mode[m], j, 0});
+        dns.push_back({j, B.rel_mode[m], it->second, v});
       }
   }
+  auto by_key = [](const Conn &x, const Conn &y) { return x.key != y.key ? x.key < y.key : x.lm < y.lm; };
+  std::sort(ups.begin(), ups.end(), by_key);
+  std::sort(dns.begin(), dns.end(), by_key);
   B.up_ptr.assign(nh + 1, 0);
   B.dn_ptr.assign(nh + 1, 0);
+  for (const Conn &c : ups) B.up_ptr[c.key + 1]++;
+  for (const Conn &c : dns) B.dn_ptr[c.key + 1]++;
   for (int i = 0; i < nh; ++i) {
-    std::sort(up[i].begin(), up[i].end(), [](const auto &x, const auto &y) { return x.second < y.second; });
-    std::sort(dn[i].begin(), dn[i].end(), [](const auto &x, const auto &y) { return x[1] < y[1]; });
-    B.up_ptr[i + 1] = B.up_ptr[i] + (int)up[i].size();
-    B.dn_ptr[i + 1] = B.dn_ptr[i] + (int)dn[i].size();
-    for (auto &u : up[i]) {
-      B.up_aux.push_back(u.first);
-      B.up_mode.push_back(u.second);
-    }
-    for (auto &d : dn[i]) {
-      B.dn_aux.push_back(d[0]);
-      B.dn_mode.push_back(d[1]);
-      B.dn_val.push_back(d[2]);
-    }
+    B.up_ptr[i + 1] += B.up_ptr[i];
+    B.dn_ptr[i + 1] += B.dn_ptr[i];
+  }
+  B.up_aux.reserve(ups.size());
+  B.up_mode.reserve(ups.size());
+  for (const Conn &c : ups) {
+    B.up_aux.push_back(c.other);
+    B.up_mode.push_back(c.lm);
+  }
+  B.dn_aux.reserve(dns.size());
+  B.dn_mode.reserve(dns.size());
+  B.dn_val.reserve(dns.size());
+  for (const Conn &c : dns) {
+    B.dn_aux.push_back(c.other);
+    B.dn_mode.push_back(c.lm);
+    B.dn_val.push_back(c.v);
   }
   // per relative L: row and column CSR tables with running bases
   B.lr_ptr.assign((size_t)nl * (n + 1), 0);
@@ -370,12 +409,12 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
     for (int i = 0; i < nh; ++i)
       for (int s = 0; s < n; ++s) {
         int c = 0;
-        for (auto &u : up[i]) {
-          const RedL &R = B.L[B.mode_lrel[u.second]];
+        for (int u = B.up_ptr[i]; u < B.up_ptr[i + 1]; ++u) {
+          const RedL &R = B.L[B.mode_lrel[B.up_mode[u]]];
           c += R.rp[s + 1] - R.rp[s];
         }
-        for (auto &d : dn[i]) {
-          const RedL &R = B.L[B.mode_lrel[d[1]]];
+        for (int u = B.dn_ptr[i]; u < B.dn_ptr[i + 1]; ++u) {
+          const RedL &R = B.L[B.mode_lrel[B.dn_mode[u]]];
           c += R.cp[s + 1] - R.cp[s];
         }
         w = std::max(w, c);
@@ -526,6 +565,45 @@ double error_threshold(const std::vector<double> &sorted, double max_err, double
   return sorted[0] + offset < max_err ? INFINITY : 0.0;
 }
 
+// ascending sort of non-negative doubles: LSD radix sort of their bit
+// patterns (monotonic for x >= 0), 11-bit digits
+void sort_nonneg(std::vector<double> &v) {
+  const size_t n = v.size();
+  if (n < 512) {
+    std::sort(v.begin(), v.end());
+    return;
+  }
+  std::vector<uint64_t> a(n), t(n);
+  for (size_t i = 0; i < n; ++i) std::memcpy(&a[i], &v[i], 8);
+  std::vector<uint32_t> cnt(2048);
+  for (int sh = 0; sh < 64; sh += 11) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (uint64_t x : a) cnt[(x >> sh) & 2047]++;
+    uint32_t sum = 0;
+    for (auto &c : cnt) {
+      const uint32_t c0 = c;
+      c = sum;
+      sum += c0;
+    }
+    for (uint64_t x : a) t[cnt[(x >> sh) & 2047]++] = x;
+    a.swap(t);
+  }
+  for (size_t i = 0; i < n; ++i) std::memcpy(&v[i], &a[i], 8);
+}
+
+// error_threshold(sort(v), max_err, offset) and the sum of the values <= the
+// threshold (v: non-negative error terms)
+double partial_threshold(std::vector<double> &v, double max_err, double offset, double *below) {
+  std::vector<double> w(v);
+  sort_nonneg(w);
+  const double thr = error_threshold(w, max_err, offset);
+  double sb = 0;
+  for (double x : v)
+    if (x <= thr) sb += x;
+  *below = sb;
+  return thr;
+}
+
 struct Selection {
   std::vector<int> stable_idx;        // stable auxiliaries (sorted indices)
   std::vector<uint8_t> stable_mask;   // [n_hier]
@@ -581,15 +659,15 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   }
   // <L> over psi = Pn[:n]
   cplx nrm = 0;
-  for (int s = 0; s < n; ++s) nrm += std::conj(pn[s]) * pn[s];
+  for (int s = 0; s < n; ++s) nrm += cm(std::conj(pn[s]), pn[s]);
   std::vector<cplx> avg(B.nl);
   for (int j = 0; j < B.nl; ++j) {
     const RedL &R = B.L[j];
     cplx a = 0;
     for (int r = 0; r < n; ++r) {
       cplx lv = 0;
-      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q) lv += R.rv[q] * pn[R.ci[q]];
-      a += std::conj(pn[r]) * lv;
+      for (int q = R.rp[r]; q < R.rp[r + 1]; ++q) lv += cm(R.rv[q], pn[R.ci[q]]);
+      a += cm(std::conj(pn[r]), lv);
     }
     avg[j] = a / nrm;
   }
@@ -611,7 +689,7 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
       double acc = 0;
       for (int sp : out.sc) {
         cplx z = 0;
-        for (int s = 0; s < n; ++s) z += cplx(0, -1) * S.H[(size_t)sp * S.n + B.states[s]] * pn[(size_t)k * n + s];
+        for (int s = 0; s < n; ++s) z += cm(cmi(S.H[(size_t)sp * S.n + B.states[s]]), pn[(size_t)k * n + s]);
         z /= kHbar;
         acc += std::norm(z);
       }
@@ -619,14 +697,21 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     }
   }
   // flux up / down (basis_functions_adaptive.py:165-390)
+  std::vector<uint8_t> kval((size_t)nm * nh, 0);  // K2_aux_by_mode
+  for (int k = 0; k < nh; ++k)
+    for (int q = B.aux_ptr[k]; q < B.aux_ptr[k + 1]; ++q) kval[(size_t)B.rel_mode[B.aux_mode_t[q]] * nh + k] = B.aux_val_t[q];
   std::vector<double> E_up((size_t)nm * nh, 0.0), E_dn((size_t)nm * nh, 0.0);
   std::vector<uint8_t> F_up;
   markov_up(P, B, F_up);
+  std::vector<uint8_t> not_m2(n);
   for (int lm = 0; lm < nm; ++lm) {
     const int m = B.modes[lm];
     const double aw = std::abs(S.W[m]);
     const double gw2 = std::norm(S.G[m] / S.W[m]);
     const cplx X = out.X2[lm];
+    const double nX = std::norm(X);
+    std::fill(not_m2.begin(), not_m2.end(), 1);
+    for (auto &e : out.M2[lm]) not_m2[e.first] = 0;
     for (int k = 0; k < nh; ++k) {
       double p_up = 0, p_dn = 0;
       const double *c2 = &out.absC2[(size_t)k * n];
@@ -635,21 +720,13 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
         p_up += std::norm(e.second) * c2[e.first];
         p_dn += std::norm(e.second - X) * c2[e.first];
       }
-      // states with M2 = 0 contribute |X|^2 |C2|^2
-      double zx = 0;
-      if (std::norm(X) != 0) {
-        zx = 0;
-        size_t q = 0;
-        for (int s = 0; s < n; ++s) {
-          if (q < out.M2[lm].size() && out.M2[lm][q].first == s) {
-            ++q;
-            continue;
-          }
-          zx += c2[s];
-        }
-        p_dn += std::norm(X) * zx;
+      if (nX != 0) {
+        double zx = 0;
+        for (int s2 = 0; s2 < n; ++s2)
+          if (not_m2[s2]) zx += c2[s2];
+        p_dn += nX * zx;
       }
-      const double kw = aw * (1.0 + aux_get(B.aux[k], m));
+      const double kw = aw * (1.0 + kval[(size_t)lm * nh + k]);
       E_up[(size_t)lm * nh + k] = (kw * kw * p_up) / h2;
       E_dn[(size_t)lm * nh + k] = (gw2 * p_dn) / h2;
     }
@@ -662,16 +739,14 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     err[k] += acc;
     acc = 0;
     for (int lm = 0; lm < nm; ++lm)
-      if (aux_get(B.aux[k], B.modes[lm]) > 0) acc += E_dn[(size_t)lm * nh + k];
+      if (kval[(size_t)lm * nh + k] > 0) acc += E_dn[(size_t)lm * nh + k];
     err[k] += acc;
   }
   // stable auxiliaries (hops_basis.py:1000-1028)
   const double dsq = P.delta_a * P.delta_a;
-  std::vector<double> srt(err);
-  std::sort(srt.begin(), srt.end());
-  const double thr = error_threshold(srt, dsq / 2.0);
+  double below = 0;
+  const double thr = partial_threshold(err, dsq / 2.0, 0.0, &below);
   out.stable_mask.assign(nh, 0);
-  double unstable = 0;
   std::vector<double> un;
   for (int k = 0; k < nh; ++k) {
     if (err[k] > thr) {
@@ -681,85 +756,85 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
       un.push_back(err[k]);
     }
   }
+  double unstable = 0;
   for (double v : un) unstable += v;
   const double dbound = dsq - unstable;
   // boundary auxiliaries (hops_basis.py:667-793)
-  std::vector<uint8_t> has_up((size_t)nm * nh, 0);
-  for (int k = 0; k < nh; ++k)
+  std::vector<uint8_t> has_up((size_t)nm * nh, 0), has_dn((size_t)nm * nh, 0);
+  for (int k = 0; k < nh; ++k) {
     for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) has_up[(size_t)B.up_mode[u] * nh + k] = 1;
-  std::vector<uint8_t> has_dn((size_t)nm * nh, 0);
-  for (int k = 0; k < nh; ++k)
     for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) has_dn[(size_t)B.dn_mode[u] * nh + k] = 1;
+  }
   struct Cand {
-    int col, lm, dir;
+    int k, lm, dir;
     double v;
   };
+  // insertion order of the reference: stable column, flux up by mode, then
+  // flux down by mode
   std::vector<Cand> cand;
   std::vector<double> nz;
-  for (size_t c = 0; c < out.stable_idx.size(); ++c) {
-    const int k = out.stable_idx[c];
+  for (int k : out.stable_idx) {
+    const bool room = B.depth[k] < P.maxhier;
     for (int lm = 0; lm < nm; ++lm) {
-      const bool fu = B.depth[k] < P.maxhier && !has_up[(size_t)lm * nh + k] && F_up[(size_t)lm * nh + k];
-      const double vu = fu ? E_up[(size_t)lm * nh + k] : 0.0;
-      const bool fd = aux_get(B.aux[k], B.modes[lm]) > 0 && !has_dn[(size_t)lm * nh + k];
-      const double vd = fd ? E_dn[(size_t)lm * nh + k] : 0.0;
-      if (vu != 0) {
-        cand.push_back({(int)c, lm, +1, vu});
-        nz.push_back(vu);
+      const size_t q = (size_t)lm * nh + k;
+      if (room && !has_up[q] && F_up[q] && E_up[q] != 0) {
+        cand.push_back({k, lm, +1, E_up[q]});
+        nz.push_back(E_up[q]);
       }
-      if (vd != 0) {
-        cand.push_back({(int)c, lm, -1, vd});
-        nz.push_back(vd);
+    }
+    for (int lm = 0; lm < nm; ++lm) {
+      const size_t q = (size_t)lm * nh + k;
+      if (kval[q] > 0 && !has_dn[q] && E_dn[q] != 0) {
+        cand.push_back({k, lm, -1, E_dn[q]});
+        nz.push_back(E_dn[q]);
       }
     }
   }
-  std::sort(nz.begin(), nz.end());
-  const double thr1 = error_threshold(nz, dbound * P.f_discard * P.f_discard);
   double discarded = 0;
-  for (double v : nz)
-    if (v <= thr1) discarded += v;
-  // accumulate by target auxiliary in the reference's insertion order: stable
-  // column, then flux up by mode, then flux down by mode
-  std::unordered_map<uint64_t, std::vector<int>> by_hash;
-  std::vector<Aux> keys;
+  const double thr1 = partial_threshold(nz, dbound * P.f_discard * P.f_discard, 0.0, &discarded);
+  // accumulate by target auxiliary, keyed by two independent Zobrist hashes
+  struct Key {
+    uint64_t a, b;
+    bool operator==(const Key &o) const { return a == o.a && b == o.b; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key &k) const { return (size_t)(k.a ^ (k.b * 0x9E3779B97F4A7C15ull)); }
+  };
+  size_t cap = 16;
+  while (cap < cand.size() * 2) cap <<= 1;
+  std::vector<Key> tkey(cap);
+  std::vector<int32_t> tval(cap, -1);
+  std::vector<int> rep;  // representative candidate of each target
   std::vector<double> vals;
-  Aux tgt;
-  std::stable_sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) {
-    if (a.col != b.col) return a.col < b.col;
-    if (a.dir != b.dir) return a.dir > b.dir;
-    return a.lm < b.lm;
-  });
-  for (const Cand &cd : cand) {
+  for (size_t c = 0; c < cand.size(); ++c) {
+    const Cand &cd = cand[c];
     if (cd.v <= thr1) continue;
-    const int k = out.stable_idx[cd.col], m = B.modes[cd.lm];
-    if (!aux_step(B.aux[k], m, cd.dir, tgt)) continue;
-    const uint64_t hk = B.aux_hash[k] ^ S.zob[(size_t)m * 256 + aux_get(B.aux[k], m)] ^
-                        S.zob[(size_t)m * 256 + aux_get(B.aux[k], m) + cd.dir];
-    auto &lst = by_hash[hk];
-    int found = -1;
-    for (int id : lst)
-      if (keys[id] == tgt) {
-        found = id;
-        break;
-      }
-    if (found < 0) {
-      found = (int)keys.size();
-      keys.push_back(tgt);
+    const int k = cd.k, m = B.modes[cd.lm];
+    const int v0 = kval[(size_t)cd.lm * nh + k], v1 = v0 + cd.dir;
+    const size_t z0 = (size_t)m * 256 + v0, z1 = (size_t)m * 256 + v1;
+    const Key key{B.aux_hash[k] ^ S.zob[z0] ^ S.zob[z1], B.aux_hash2[k] ^ S.zob2[z0] ^ S.zob2[z1]};
+    size_t h = KeyHash()(key) & (cap - 1);
+    while (tval[h] >= 0 && !(tkey[h] == key)) h = (h + 1) & (cap - 1);
+    if (tval[h] < 0) {
+      tkey[h] = key;
+      tval[h] = (int32_t)vals.size();
+      rep.push_back((int)c);
       vals.push_back(0.0);
-      lst.push_back(found);
     }
-    vals[found] += cd.v;
+    vals[tval[h]] += cd.v;
   }
-  std::vector<double> sv(vals);
-  std::sort(sv.begin(), sv.end());
-  const double thr2 = error_threshold(sv, dbound, discarded);
-  for (size_t i = 0; i < keys.size(); ++i)
-    if (vals[i] > thr2) out.aux_bound.push_back(keys[i]);
-  out.aux_new.reserve(out.stable_idx.size() + out.aux_bound.size());
+  double dummy = 0;
+  const double thr2 = partial_threshold(vals, dbound, discarded, &dummy);
+  out.aux_new.reserve(out.stable_idx.size() + vals.size());
   for (int k : out.stable_idx) out.aux_new.push_back(B.aux[k]);
-  for (const Aux &a : out.aux_bound) out.aux_new.push_back(a);
-  std::sort(out.aux_new.begin(), out.aux_new.end(), aux_less);
-  out.aux_new.erase(std::unique(out.aux_new.begin(), out.aux_new.end()), out.aux_new.end());
+  Aux tgt;
+  for (size_t i = 0; i < vals.size(); ++i)
+    if (vals[i] > thr2) {
+      const Cand &cd = cand[rep[i]];
+      aux_step(B.aux[cd.k], B.modes[cd.lm], cd.dir, tgt);
+      out.aux_bound.push_back(tgt);
+      out.aux_new.push_back(tgt);
+    }
 }
 
 // state half (hops_basis.py:294-482, :795-912); e_state[s] = sum over stable
@@ -842,7 +917,7 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
       for (int k : sel.stable_idx) {
         cplx z = 0;
         for (size_t q = 0; q < rel_st.size(); ++q)
-          z += cplx(0, -1) * S.H[(size_t)sp * S.n + st_stable[q]] * (pn[(size_t)k * n + rel_st[q]] / kHbar);
+          z += cm(cmi(S.H[(size_t)sp * S.n + st_stable[q]]), pn[(size_t)k * n + rel_st[q]] / kHbar);
         E[i] += std::norm(z);
       }
     }
@@ -1049,6 +1124,17 @@ struct Batch {
   std::vector<hops_slot_perm_t> perms;
   std::vector<int32_t> its;
   std::vector<double> dts, nrms;
+  // accumulated wall time (s) per phase: step, errors on the device, host
+  // hierarchy selection, state errors + selection, table build, device
+  // update, psi fetch
+  double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct Timer {
+  double *acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit Timer(double *a) : acc(a) {}
+  ~Timer() { *acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
 size_t rsz(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1157,13 +1243,20 @@ void set_op(const Basis &B, char *base, const Layout &L, int precision, hops_op_
 // ---------------------------------------------------------------------------
 struct Stage {
   Batch *b;
-  size_t cur = 0;
+  size_t cur = 0, flushed = 0;
+  // copy into the pinned buffer; the device copy is issued by flush()
   int push(const void *src, size_t bytes, void **dev) {
+    char *h;
+    RT_TRY(place(bytes, dev, &h));
+    if (bytes) std::memcpy(h, src, bytes);
+    return 0;
+  }
+  // room for bytes in the pinned buffer (filled by the caller) and its device twin
+  int place(size_t bytes, void **dev, char **host) {
     const size_t need = cur + rsz(bytes);
     if (need > b->scratch_bytes || need > b->h_up_bytes) return rt_fail("staging area too small (%zu bytes)", need);
-    std::memcpy(b->h_up + cur, src, bytes);
+    *host = b->h_up + cur;
     *dev = b->arena.ptr(b->scratch_off) + cur;
-    RT_HIP(hipMemcpyAsync(*dev, b->h_up + cur, bytes, hipMemcpyHostToDevice, b->stream));
     cur = need;
     return 0;
   }
@@ -1174,6 +1267,14 @@ struct Stage {
     *off = cur;
     *dev = b->arena.ptr(b->scratch_off) + cur;
     cur = need;
+    return 0;
+  }
+  // one host-to-device copy of everything placed since the last flush
+  int flush() {
+    if (cur > flushed)
+      RT_HIP(hipMemcpyAsync(b->arena.ptr(b->scratch_off) + flushed, b->h_up + flushed, cur - flushed,
+                            hipMemcpyHostToDevice, b->stream));
+    flushed = cur;
     return 0;
   }
 };
@@ -1234,20 +1335,7 @@ int upload_desc(Batch &b) {
 hops_op_t *d_ops(Batch &b) { return reinterpret_cast<hops_op_t *>(b.arena.ptr(b.ops_off)); }
 hops_batch_t *d_bts(Batch &b) { return reinterpret_cast<hops_batch_t *>(b.arena.ptr(b.bts_off)); }
 
-// allocate the block of a new basis, upload its tables, point op/bt at it
-int place_basis(Batch &b, Slot &sl, Basis &B, long long *blk_out, hops_op_t &op, hops_batch_t &bt,
-                std::vector<char> &host_tables) {
-  Layout L = layout_of(B, b.precision);
-  const long long blk = b.arena.alloc(L.total);
-  if (blk < 0) return rt_fail("arena full: a basis of %d states x %d auxiliaries needs %zu bytes", B.n, B.nh, L.total);
-  char *base = b.arena.ptr(blk);
-  set_op(B, base, L, b.precision, op);
-  // tables are contiguous at the start of the block: one copy
-  const size_t tab_bytes = L.tab_off.empty() ? 0 : L.ell_col;
-  host_tables.assign(tab_bytes, 0);
-  for (size_t i = 0; i < L.tabs.size(); ++i)
-    if (L.tabs[i].second) std::memcpy(host_tables.data() + L.tab_off[i], L.tabs[i].first, L.tabs[i].second);
-  *blk_out = blk;
+void fill_bt(const Batch &b, const Slot &sl, const Basis &B, const Layout &L, char *base, hops_batch_t &bt) {
   bt = hops_batch_t{};
   bt.B = 1;
   bt.precision = b.precision;
@@ -1278,6 +1366,22 @@ int place_basis(Batch &b, Slot &sl, Basis &B, long long *blk_out, hops_op_t &op,
   bt.coefT_d = base + L.coefT;
   bt.sigT_d = reinterpret_cast<double *>(pb + 2 * rsz(Ma * 16 + 16) + rsz((4 * Ma + 2) * 8));
   bt.coefT_stride = 1;
+}
+
+// allocate the block of a basis, stage its tables in host_tables, point op/bt at it
+int place_basis(Batch &b, Slot &sl, Basis &B, long long *blk_out, hops_op_t &op, hops_batch_t &bt,
+                std::vector<char> &host_tables) {
+  Layout L = layout_of(B, b.precision);
+  const long long blk = b.arena.alloc(L.total);
+  if (blk < 0) return rt_fail("arena full: a basis of %d states x %d auxiliaries needs %zu bytes", B.n, B.nh, L.total);
+  char *base = b.arena.ptr(blk);
+  set_op(B, base, L, b.precision, op);
+  const size_t tab_bytes = L.tab_off.empty() ? 0 : L.ell_col;
+  host_tables.assign(tab_bytes, 0);
+  for (size_t i = 0; i < L.tabs.size(); ++i)
+    if (L.tabs[i].second) std::memcpy(host_tables.data() + L.tab_off[i], L.tabs[i].first, L.tabs[i].second);
+  *blk_out = blk;
+  fill_bt(b, sl, B, L, base, bt);
   sl.blk_bytes = L.total;
   return 0;
 }
@@ -1286,13 +1390,16 @@ int place_basis(Batch &b, Slot &sl, Basis &B, long long *blk_out, hops_op_t &op,
 
 int run_step(Batch &b, const std::vector<int> &list) {
   if (list.empty()) return 0;
+  Timer tmr(&b.tm[0]);
   for (int s : list) b.its[s] = b.slots[s].it;
   RT_TRY(upload_desc(b));
   std::vector<uint32_t> tiles = tiles_of(b, list);
   Stage st{&b};
   void *dl, *dt;
+  RT_TRY(ensure_staging(b, tiles.size() * 4 + list.size() * 4 + 4096));
   RT_TRY(st.push(list.data(), list.size() * 4, &dl));
   RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dt));
+  RT_TRY(st.flush());
   RT_TRY(hops_slots_step(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const uint32_t *)dt,
                          (int)tiles.size(), (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm_of(b, list),
                          b.stream));
@@ -1303,6 +1410,7 @@ int run_step(Batch &b, const std::vector<int> &list) {
 // copy psi (psi_only) of the listed slots to the host
 int fetch_psi(Batch &b, const std::vector<int> &list) {
   if (list.empty()) return 0;
+  Timer tmr(&b.tm[6]);
   std::vector<int64_t> off(b.slots.size(), 0);
   size_t tot = 0;
   for (int s : list) {
@@ -1316,6 +1424,7 @@ int fetch_psi(Batch &b, const std::vector<int> &list) {
   RT_TRY(st.push(list.data(), list.size() * 4, &dl));
   RT_TRY(st.push(off.data(), off.size() * 8, &doff));
   RT_TRY(st.reserve(tot * 16, &dout, &oo));
+  RT_TRY(st.flush());
   RT_TRY(hops_slots_pn(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const int64_t *)doff, dout,
                        nullptr, b.precision, 1, b.stream));
   RT_HIP(hipMemcpyAsync(b.h_down, dout, tot * 16, hipMemcpyDeviceToHost, b.stream));
@@ -1341,6 +1450,7 @@ int define_bases(Batch &b, const std::vector<int> &list) {
     eoff[s] = (int64_t)etot;
     etot += std::max(b.slots[s].B.nh, b.slots[s].B.n);
   }
+  auto t_dev = std::chrono::steady_clock::now();
   const std::vector<uint32_t> tiles = tiles_of(b, list);
   for (int s : list) b.its[s] = b.slots[s].it;
   RT_TRY(upload_desc(b));
@@ -1354,6 +1464,7 @@ int define_bases(Batch &b, const std::vector<int> &list) {
   RT_TRY(st.push(eoff.data(), S * 8, &deoff));
   RT_TRY(st.reserve(tot * 16, &dpn, &opn));
   RT_TRY(st.reserve(etot * 8, &de, &oe));
+  RT_TRY(st.flush());
   double *dnrm = reinterpret_cast<double *>(b.arena.ptr(b.nrm_off));
   const int shm = shm_of(b, list);
   RT_TRY(hops_slots_pn(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const int64_t *)doff, dpn, dnrm,
@@ -1381,10 +1492,15 @@ int define_bases(Batch &b, const std::vector<int> &list) {
   }
   std::vector<std::vector<double>> eaux(S);
   for (int s : list) eaux[s].assign(he + eoff[s], he + eoff[s] + b.slots[s].B.nh);
+  auto t_h = std::chrono::steady_clock::now();
+  b.tm[1] += std::chrono::duration<double>(t_h - t_dev).count();
   b.pool->parallel_for((int)list.size(), [&](int i) {
     Slot &sl = b.slots[list[i]];
     select_hier(sl.prm, sl.B, sl.pn.data(), eaux[list[i]].data(), sl.sel);
   });
+  auto t_s = std::chrono::steady_clock::now();
+  b.tm[2] += std::chrono::duration<double>(t_s - t_h).count();
+  Timer tmr_s(&b.tm[3]);
   if (any_s) {
     std::vector<int> ls;
     for (int s : list)
@@ -1408,6 +1524,7 @@ int define_bases(Batch &b, const std::vector<int> &list) {
     RT_TRY(st.push(mask.data(), mtot, &dmask));
     RT_TRY(st.push(soff.data(), S * 8, &dsoff));
     RT_TRY(st.reserve(stot * 8, &des, &oes));
+    RT_TRY(st.flush());
     RT_TRY(hops_slots_errors(d_ops(b), d_bts(b), (const int32_t *)dls, (int)ls.size(), (const uint32_t *)dtls,
                              (int)tl.size(), (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm_of(b, ls),
                              (const int64_t *)doff, dpn, (const int64_t *)dmoff, (const uint8_t *)dmask,
@@ -1447,7 +1564,6 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
   for (size_t i = 0; i < list.size(); ++i) {
     Slot &sl = b.slots[list[i]];
     std::sort(tgt_states[i].begin(), tgt_states[i].end());
-    std::sort(tgt_aux[i].begin(), tgt_aux[i].end(), aux_less);
     sl.changed = !same_basis(sl.B, tgt_states[i], tgt_aux[i]);
     if (sl.changed) {
       idx_of[list[i]] = (int)i;
@@ -1456,76 +1572,110 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
   }
   if (ch.empty()) return 0;
   std::vector<int> rcs(ch.size(), 0);
-  b.pool->parallel_for((int)ch.size(), [&](int i) {
-    Slot &sl = b.slots[ch[i]];
-    const int j = idx_of[ch[i]];
-    rcs[i] = build_basis(*sl.sys, sl.prm.eom, tgt_states[j], tgt_aux[j], sl.Bnew);
-  });
+  {
+    Timer tmr(&b.tm[4]);
+    b.pool->parallel_for((int)ch.size(), [&](int i) {
+      Slot &sl = b.slots[ch[i]];
+      const int j = idx_of[ch[i]];
+      rcs[i] = build_basis(*sl.sys, sl.prm.eom, tgt_states[j], tgt_aux[j], sl.Bnew);
+    });
+  }
   for (int rc : rcs)
     if (rc < 0) return rc;
-  // blocks, tables, permutation maps
+  // blocks, descriptors, staging layout (serial: the arena is not shared)
+  Timer tmr_u(&b.tm[5]);
   std::vector<long long> nblk(ch.size());
   std::vector<hops_op_t> nop(ch.size());
   std::vector<hops_batch_t> nbt(ch.size());
-  std::vector<std::vector<char>> tabs(ch.size());
-  std::vector<std::vector<int32_t>> inv(ch.size());
-  size_t up_bytes = 0;
+  std::vector<size_t> tab_bytes(ch.size()), tab_off(ch.size()), inv_off(ch.size()), prev_off(ch.size());
+  std::vector<Layout> lays(ch.size());
+  size_t stage_bytes_needed = 0;
   for (size_t i = 0; i < ch.size(); ++i) {
     Slot &sl = b.slots[ch[i]];
-    RT_TRY(place_basis(b, sl, sl.Bnew, &nblk[i], nop[i], nbt[i], tabs[i]));
-    const Basis &Bo = sl.B, &Bn = sl.Bnew;
-    inv[i].assign(Bn.nh + Bn.n, -1);
-    for (int j = 0; j < Bn.nh; ++j) {
-      auto it = Bo.index.find(Bn.aux[j]);
-      if (it != Bo.index.end()) inv[i][j] = it->second;
+    lays[i] = layout_of(sl.Bnew, b.precision);
+    const Layout &L = lays[i];
+    nblk[i] = b.arena.alloc(L.total);
+    if (nblk[i] < 0) {
+      for (size_t j = 0; j < i; ++j) b.arena.release(nblk[j]);
+      return rt_fail("arena full: a basis of %d states x %d auxiliaries needs %zu bytes", sl.Bnew.n, sl.Bnew.nh,
+                     L.total);
     }
-    for (int s = 0; s < Bn.n; ++s) inv[i][Bn.nh + s] = Bo.pos[Bn.states[s]];
-    up_bytes += rsz(tabs[i].size()) + rsz(inv[i].size() * 4);
+    char *base = b.arena.ptr(nblk[i]);
+    set_op(sl.Bnew, base, L, b.precision, nop[i]);
+    fill_bt(b, sl, sl.Bnew, L, base, nbt[i]);
+    tab_bytes[i] = L.tab_off.empty() ? 0 : L.ell_col;
+    prev_off[i] = L.phi_prev;
+    stage_bytes_needed += rsz(tab_bytes[i]) + rsz((size_t)(sl.Bnew.nh + sl.Bnew.n) * 4);
   }
   const std::vector<uint32_t> tiles = tiles_of(b, ch, true);
-  RT_TRY(ensure_staging(b, up_bytes + tiles.size() * 4 + ch.size() * 4 + 65536));
+  RT_TRY(ensure_staging(b, stage_bytes_needed + tiles.size() * 4 + ch.size() * (4 + sizeof(hops_segment_t)) +
+                               b.slots.size() * sizeof(hops_slot_perm_t) + 65536));
   Stage st{&b};
+  std::vector<char *> htab(ch.size()), hinv(ch.size());
+  std::vector<void *> dtab(ch.size()), dinv(ch.size());
+  for (size_t i = 0; i < ch.size(); ++i) {
+    const Basis &Bn = b.slots[ch[i]].Bnew;
+    RT_TRY(st.place(tab_bytes[i], &dtab[i], &htab[i]));
+    RT_TRY(st.place((size_t)(Bn.nh + Bn.n) * 4, &dinv[i], &hinv[i]));
+  }
+  // fill tables and inverse maps in parallel, straight into pinned memory
+  b.pool->parallel_for((int)ch.size(), [&](int i) {
+    Slot &sl = b.slots[ch[i]];
+    const Layout &L = lays[i];
+    for (size_t t = 0; t < L.tabs.size(); ++t)
+      if (L.tabs[t].second) std::memcpy(htab[i] + L.tab_off[t], L.tabs[t].first, L.tabs[t].second);
+    const Basis &Bo = sl.B, &Bn = sl.Bnew;
+    int32_t *inv = reinterpret_cast<int32_t *>(hinv[i]);
+    for (int j = 0; j < Bn.nh; ++j) {
+      auto it = Bo.index.find(Bn.aux[j]);
+      inv[j] = it != Bo.index.end() ? it->second : -1;
+    }
+    for (int q = 0; q < Bn.n; ++q) inv[Bn.nh + q] = Bo.pos[Bn.states[q]];
+  });
+  std::vector<hops_segment_t> segs;
   for (size_t i = 0; i < ch.size(); ++i) {
     Slot &sl = b.slots[ch[i]];
-    if (!tabs[i].empty())
-      RT_HIP(hipMemcpyAsync(b.arena.ptr(nblk[i]), tabs[i].data(), tabs[i].size(), hipMemcpyHostToDevice, b.stream));
-    void *dinv;
-    RT_TRY(st.push(inv[i].data(), inv[i].size() * 4, &dinv));
+    if (tab_bytes[i]) segs.push_back({dtab[i], b.arena.ptr(nblk[i]), (long long)tab_bytes[i]});
     hops_slot_perm_t &p = b.perms[ch[i]];
     p.old_phi_d = sl.bt.phi_d;
     p.new_phi_d = nbt[i].phi_d;
-    p.inv_aux_d = (const int32_t *)dinv;
-    p.inv_state_d = (const int32_t *)dinv + sl.Bnew.nh;
+    p.inv_aux_d = (const int32_t *)dinv[i];
+    p.inv_state_d = (const int32_t *)dinv[i] + sl.Bnew.nh;
     p.n_old = sl.B.n;
     p.n_new = sl.Bnew.n;
     p.nh_new = sl.Bnew.nh;
     p.scale = scale[idx_of[ch[i]]];
     b.ops[ch[i]] = nop[i];
   }
-  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.perm_off), b.perms.data(), b.slots.size() * sizeof(hops_slot_perm_t),
-                        hipMemcpyHostToDevice, b.stream));
-  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.ops_off), b.ops.data(), b.slots.size() * sizeof(hops_op_t),
-                        hipMemcpyHostToDevice, b.stream));
-  void *dl, *dt;
+  void *dsegs, *dperm, *dl, *dt;
+  RT_TRY(st.push(segs.data(), segs.size() * sizeof(hops_segment_t), &dsegs));
+  RT_TRY(st.push(b.perms.data(), b.slots.size() * sizeof(hops_slot_perm_t), &dperm));
   RT_TRY(st.push(ch.data(), ch.size() * 4, &dl));
   RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dt));
+  RT_TRY(st.flush());
+  RT_TRY(hops_copy_segments((const hops_segment_t *)dsegs, (int)segs.size(), b.stream));
+  RT_HIP(hipMemcpyAsync(b.arena.ptr(b.ops_off), b.ops.data(), b.slots.size() * sizeof(hops_op_t),
+                        hipMemcpyHostToDevice, b.stream));
   RT_TRY(hops_slots_build(d_ops(b), (const int32_t *)dl, (int)ch.size(), (const uint32_t *)dt, (int)tiles.size(),
                           b.precision, b.stream));
-  RT_TRY(hops_slots_permute((const hops_slot_perm_t *)b.arena.ptr(b.perm_off), (const uint32_t *)dt,
-                            (int)tiles.size(), b.stream));
+  RT_TRY(hops_slots_permute((const hops_slot_perm_t *)dperm, (const uint32_t *)dt, (int)tiles.size(), b.stream));
   RT_HIP(hipStreamSynchronize(b.stream));
+  // retire the old bases on the worker threads (many small frees each)
+  std::vector<Basis> old(ch.size());
+  for (size_t i = 0; i < ch.size(); ++i) old[i] = std::move(b.slots[ch[i]].B);
   for (size_t i = 0; i < ch.size(); ++i) {
     Slot &sl = b.slots[ch[i]];
     b.arena.release(sl.blk);
     sl.blk = nblk[i];
+    sl.blk_bytes = lays[i].total;
     sl.B = std::move(sl.Bnew);
     sl.Bnew = Basis();
     sl.op = nop[i];
     sl.bt = nbt[i];
     b.bts[ch[i]] = nbt[i];
-    char *base = b.arena.ptr(sl.blk);
-    sl.phi_prev = base + layout_of(sl.B, b.precision).phi_prev;
+    sl.phi_prev = b.arena.ptr(sl.blk) + prev_off[i];
   }
+  b.pool->parallel_for((int)old.size(), [&](int i) { Basis().swap_out(old[i]); });
   return 0;
 }
 
@@ -1799,14 +1949,16 @@ void *hops_rt_system_create(const hops_rt_system_desc_t *d) {
       if (S->H[(size_t)r * S->n + c] != cplx(0, 0)) S->coupled[r].push_back(c);
   uint64_t x = 0x9E3779B97F4A7C15ull;
   S->zob.resize((size_t)S->n_hm * 256);
-  for (auto &z : S->zob) {  // splitmix64
-    x += 0x9E3779B97F4A7C15ull;
-    uint64_t v = x;
-    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
-    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
-    z = v ^ (v >> 31);
-  }
-  for (int m = 0; m < S->n_hm; ++m) S->zob[(size_t)m * 256] = 0;  // depth 0 contributes nothing
+  S->zob2.resize((size_t)S->n_hm * 256);
+  for (auto *tab : {&S->zob, &S->zob2})
+    for (auto &z : *tab) {  // splitmix64
+      x += 0x9E3779B97F4A7C15ull;
+      uint64_t v = x;
+      v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+      v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+      z = v ^ (v >> 31);
+    }
+  for (int m = 0; m < S->n_hm; ++m) S->zob[(size_t)m * 256] = S->zob2[(size_t)m * 256] = 0;  // depth 0: no key
   return S;
 }
 
@@ -1978,6 +2130,13 @@ long long hops_rt_slot_estimate(void *p, void *sys, int n_state, int n_hier) {
   const size_t ell_w = 2 * (size_t)std::max(1, std::min(S.n_hm, 2 * n_state));  // up + down per row
   const size_t tables = (size_t)n_hier * (16 + 8 * ell_w) + (size_t)n_state * n_state * 48 + 4096;
   return (long long)(pers_bytes(S) + tables + ell_w * nr * (4 + es) + nr * (16 * 2 + 3 * es + 8) + n_hier * 16);
+}
+
+int hops_rt_batch_timers(void *p, double *out) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  for (int i = 0; i < 8; ++i) out[i] = b.tm[i];
+  return 0;
 }
 
 long long hops_rt_arena_free(void *p) {

@@ -62,9 +62,12 @@ class RtSystem:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h:
-            lib().hops_rt_system_destroy(h)
-            self.handle = None
+        if h and _lib._lib is not None:
+            try:
+                _lib._lib.hops_rt_system_destroy(h)
+            except Exception:
+                pass
+        self.handle = None
 
 
 def _aux_csr(aux_list):
@@ -154,9 +157,12 @@ class DebugBasis:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
-            lib().hops_rt_debug_free(h)
-            self.h = None
+        if h and _lib._lib is not None:
+            try:
+                _lib._lib.hops_rt_debug_free(h)
+            except Exception:
+                pass
+        self.h = None
 
 
 class RaggedBatch:
@@ -260,10 +266,22 @@ class RaggedBatch:
     def synchronize(self):
         torch.cuda.synchronize(self.device)
 
+    def timers(self):
+        """Accumulated seconds per runtime phase (see hops_rt_batch_timers)."""
+        out = np.zeros(8, np.float64)
+        check(lib().hops_rt_batch_timers(self.h, _ptr(out)))
+        names = ["rk4_step", "errors_device", "select_hier", "select_states", "build_tables", "update_device",
+                 "fetch_psi"]
+        return dict(zip(names, out[:7].tolist()))
+
     def close(self):
-        if getattr(self, "h", None):
-            lib().hops_rt_batch_destroy(self.h)
-            self.h = None
+        h = getattr(self, "h", None)
+        if h and _lib._lib is not None:
+            try:
+                _lib._lib.hops_rt_batch_destroy(h)
+            except Exception:
+                pass
+        self.h = None
 
     def __del__(self):
         self.close()
