@@ -5,7 +5,9 @@
 //   tilde c_j = c_j exp(-2 pi i nu j)          (rotate so c[n_t-1] is real)
 //   s_w = Re FFT([tilde c, conj(reverse(tilde c[1:-1]))]), negatives -> 0
 //   eta = unrotate( IFFT( FFT(z) * sqrt(s_w / 2) )[:n_t] )
-// One workgroup owns one row and keeps it in LDS.  The transform length is
+// One workgroup owns one row and keeps it in LDS (rows longer than the LDS
+// holds work in a library-owned global scratch row instead, which the same
+// workgroup barriers order).  The transform length is
 // N = 2(n_t - 1).  Powers of two run a radix-2 decimation-in-frequency
 // forward pass (natural -> bit-reversed order) and a decimation-in-time
 // inverse pass (bit-reversed -> natural), so the spectral multiply happens in
@@ -27,7 +29,8 @@ extern "C" int hops_internal_set_error(const char *msg);
 
 namespace {
 
-constexpr int kMaxLds = 8192;  // complex doubles held in LDS (128 KiB)
+constexpr int kMaxLds = 8192;     // complex doubles held in LDS (128 KiB)
+constexpr int kMaxLen = 1 << 22;  // longest transform (global scratch rows)
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
@@ -114,25 +117,28 @@ __device__ void dft_forward(double2 *a, int N, int M, int logm, const double2 *B
 
 // FFT_M of the Bluestein filter b_j = w_j (j < N), b_{M-j} = w_j (0 < j < N),
 // stored in bit-reversed order.
-__global__ __launch_bounds__(kThreads) void k_bluestein_filter(double2 *Bhat, int N, int M) {
+__global__ __launch_bounds__(kThreads) void k_bluestein_filter(double2 *Bhat, int N, int M, int in_place) {
   extern __shared__ double2 lds[];
+  double2 *a = in_place ? Bhat : lds;
   for (int j = threadIdx.x; j < M; j += blockDim.x) {
     double2 v = make_double2(0.0, 0.0);
     if (j < N) v = chirp(j, N);
     else if (j > M - N) v = chirp(M - j, N);
-    lds[j] = v;
+    a[j] = v;
   }
   __syncthreads();
-  fft_dif(lds, M, -1.0);
-  for (int j = threadIdx.x; j < M; j += blockDim.x) Bhat[j] = lds[j];
+  fft_dif(a, M, -1.0);
+  if (!in_place)
+    for (int j = threadIdx.x; j < M; j += blockDim.x) Bhat[j] = lds[j];
 }
 
 // Embedding spectrum of each correlation function row.
 __global__ __launch_bounds__(kThreads) void k_embedding(const float *__restrict__ ang, const float *__restrict__ mag,
                                                         int n_t, int N, int M, int logm, const double2 *Bhat,
-                                                        double *s_w, double *nu) {
-  extern __shared__ double2 lds[];
+                                                        double *s_w, double *nu, double2 *gscratch) {
+  extern __shared__ double2 lds_[];
   const int row = blockIdx.x;
+  double2 *lds = gscratch ? gscratch + (long)row * M : lds_;
   const float *ar = ang + (long)row * n_t;
   const float *mr = mag + (long)row * n_t;
   // rotation so that the last point is real
@@ -162,9 +168,10 @@ __global__ __launch_bounds__(kThreads) void k_filter(const float2 *__restrict__ 
                                                      const double *__restrict__ s_w,
                                                      const double *__restrict__ nu,
                                                      const int32_t *__restrict__ row_lop,
-                                                     float2 *__restrict__ out) {
-  extern __shared__ double2 lds[];
+                                                     float2 *__restrict__ out, double2 *gscratch) {
+  extern __shared__ double2 lds_[];
   const int row = blockIdx.x;
+  double2 *lds = gscratch ? gscratch + (long)row * M : lds_;
   const int l = row_lop[row];
   const float2 *zr = z + (long)row * N;
   const double *sw = s_w + (long)l * N;
@@ -209,6 +216,37 @@ struct Plan {
 std::mutex g_plan_mu;
 std::map<int, Plan> g_plans;
 
+int set_lds(const void *fn, size_t bytes) {
+  if (bytes > 64 * 1024 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return hops_internal_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+  return 0;
+}
+
+// global scratch rows for transforms longer than the LDS holds (library-owned,
+// grown on demand, kept until the process ends)
+std::mutex g_scratch_mu;
+double2 *g_scratch = nullptr;
+size_t g_scratch_bytes = 0;
+
+int scratch_rows(size_t bytes, hipStream_t st, double2 **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (bytes > g_scratch_bytes) {
+    if (g_scratch) {
+      if (hipStreamSynchronize(st) != hipSuccess || hipFree(g_scratch) != hipSuccess)
+        return hops_internal_set_error("releasing the noise FFT scratch failed");
+      g_scratch = nullptr;
+    }
+    if (hipMalloc(&g_scratch, bytes) != hipSuccess) {
+      g_scratch_bytes = 0;
+      return hops_internal_set_error("hipMalloc of the noise FFT scratch failed");
+    }
+    g_scratch_bytes = bytes;
+  }
+  *out = g_scratch;
+  return 0;
+}
+
 int make_plan(int n_t, hipStream_t st, Plan *out) {
   if (n_t < 3) return hops_internal_set_error("noise time axis needs at least 3 points");
   const long N = 2L * (n_t - 1);
@@ -229,8 +267,7 @@ int make_plan(int n_t, hipStream_t st, Plan *out) {
     p.M = (int)M;
     p.logm = logm;
   }
-  if (p.M > kMaxLds)
-    return hops_internal_set_error("noise FFT length exceeds the LDS-resident limit (8192 points)");
+  if (p.M > kMaxLen) return hops_internal_set_error("noise FFT length exceeds 2^22 points");
   if (p.M != p.N) {
     std::lock_guard<std::mutex> lk(g_plan_mu);
     auto it = g_plans.find(p.N);
@@ -240,7 +277,9 @@ int make_plan(int n_t, hipStream_t st, Plan *out) {
     }
     if (hipMalloc(&p.Bhat, sizeof(double2) * p.M) != hipSuccess)
       return hops_internal_set_error("hipMalloc of the Bluestein filter failed");
-    k_bluestein_filter<<<1, kThreads, sizeof(double2) * p.M, st>>>(p.Bhat, p.N, p.M);
+    const bool big = p.M > kMaxLds;
+    if (!big && set_lds((const void *)k_bluestein_filter, sizeof(double2) * p.M)) return -1;
+    k_bluestein_filter<<<1, kThreads, big ? 0 : sizeof(double2) * p.M, st>>>(p.Bhat, p.N, p.M, big ? 1 : 0);
     if (hipGetLastError() != hipSuccess) return hops_internal_set_error("Bluestein filter launch failed");
     g_plans[p.N] = p;
   }
@@ -248,12 +287,6 @@ int make_plan(int n_t, hipStream_t st, Plan *out) {
   return 0;
 }
 
-int set_lds(const void *fn, size_t bytes) {
-  if (bytes > 64 * 1024 &&
-      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
-    return hops_internal_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
-  return 0;
-}
 
 }  // namespace
 
@@ -264,9 +297,13 @@ extern "C" int hops_noise_embedding(const float *ang_d, const float *mag_d, int 
   hipStream_t st = (hipStream_t)stream;
   Plan p;
   if (int rc = make_plan(n_t, st, &p)) return rc;
-  const size_t lds = sizeof(double2) * p.M;
+  double2 *gs = nullptr;
+  if (p.M > kMaxLds) {
+    if (int rc = scratch_rows(sizeof(double2) * p.M * (size_t)n_lop, st, &gs)) return rc;
+  }
+  const size_t lds = gs ? 0 : sizeof(double2) * p.M;
   if (int rc = set_lds((const void *)k_embedding, lds)) return rc;
-  k_embedding<<<n_lop, kThreads, lds, st>>>(ang_d, mag_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d);
+  k_embedding<<<n_lop, kThreads, lds, st>>>(ang_d, mag_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d, gs);
   if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_embedding launch failed");
   return 0;
 }
@@ -279,10 +316,14 @@ extern "C" int hops_noise_filter(const void *z_d, int R, int n_t, const double *
   hipStream_t st = (hipStream_t)stream;
   Plan p;
   if (int rc = make_plan(n_t, st, &p)) return rc;
-  const size_t lds = sizeof(double2) * p.M;
+  double2 *gs = nullptr;
+  if (p.M > kMaxLds) {
+    if (int rc = scratch_rows(sizeof(double2) * p.M * (size_t)R, st, &gs)) return rc;
+  }
+  const size_t lds = gs ? 0 : sizeof(double2) * p.M;
   if (int rc = set_lds((const void *)k_filter, lds)) return rc;
   k_filter<<<R, kThreads, lds, st>>>((const float2 *)z_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d,
-                                      row_lop_d, (float2 *)out_d);
+                                      row_lop_d, (float2 *)out_d, gs);
   if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_filter launch failed");
   return 0;
 }

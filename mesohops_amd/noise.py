@@ -131,10 +131,11 @@ class HopsNoise:
         return int(np.where(hit)[0][0])
 
 
-def device_noise(noises, rows, device, stream_ptr):
+def device_noise(noises, rows, device, stream_ptr, raws=None):
     """Correlated noise of B trajectories for the hierarchy L-operators
     ``rows`` (absolute indices), complex64 [B, len(rows), n_t] in HBM.
-    All noises must share the system (same correlation functions)."""
+    All noises must share the system (same correlation functions).  raws:
+    the host raw draws of each noise (HopsNoise.raw()), when already made."""
     B = len(noises)
     nz = noises[0]
     nt = nz.n_t
@@ -157,7 +158,9 @@ def device_noise(noises, rows, device, stream_ptr):
                                          nz.n_l2, nt,
                                          ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
                                          stream_ptr))
-        raw = np.stack([n.raw()[rows] for n in noises]).reshape(B * len(rows), N)
+        if raws is None:
+            raws = [n.raw() for n in noises]
+        raw = np.stack([r[rows] for r in raws]).reshape(B * len(rows), N)
         z_d = torch.from_numpy(np.ascontiguousarray(raw)).to(device)
         row_lop = torch.from_numpy(np.tile(np.array(rows, dtype=np.int32), B)).to(device)
         check(lib().hops_noise_filter(ctypes.c_void_p(z_d.data_ptr()), B * len(rows), nt,

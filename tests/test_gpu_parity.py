@@ -188,10 +188,12 @@ def test_batch_equals_alone():
             assert d == 0.0
 
 
-@pytest.mark.parametrize("tlen,tau", [(1024.0, 2.0), (1000.0, 1.0), (60.0, 0.5)])
+@pytest.mark.parametrize("tlen,tau", [(1024.0, 2.0), (1000.0, 1.0), (60.0, 0.5), (2048.0, 0.5), (1536.0, 0.5)])
 def test_device_noise_filter(tlen, tau):
     """Device circulant-embedding filter vs the oracle (power-of-two and
-    Bluestein lengths); both start from the same NumPy raw draws.  The
+    Bluestein lengths, LDS-resident rows and, for TLEN 1536 at TAU 0.5 with a
+    16384-point Bluestein transform, global scratch rows); both start from
+    the same NumPy raw draws.  The
     reference transforms the complex64 raw noise in single precision, which
     bounds the agreement at ~1e-7 of the noise scale; the bar is 1e-6."""
     _need_gpu()

@@ -1,0 +1,50 @@
+"""Serving loop (continuous batching) on the device: every request's result is
+the same alone and in the mix — bitwise, since a slot's kernels and basis
+bookkeeping touch only that slot — while requests of different sizes arrive,
+join and leave the batch at different times."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _requests():
+    from mesohops_amd.workloads import Request
+    return [Request(rid=11, arrival=0.0, nsite=6, maxhier=3, delta=2e-3, t_max=40.0),
+            Request(rid=12, arrival=0.02, nsite=9, maxhier=4, delta=5e-3, t_max=25.0),
+            Request(rid=13, arrival=0.04, nsite=4, maxhier=2, delta=1e-3, t_max=60.0),
+            Request(rid=14, arrival=0.05, nsite=12, maxhier=3, delta=8e-3, t_max=15.0)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_alone_equals_in_mix(prec):
+    _need_gpu()
+    from mesohops_amd.serving import ServingLoop
+    reqs = _requests()
+    loop = ServingLoop(precision_mode=prec, arena_bytes=1 << 30, max_slots=8, record=True)
+    mix, stats = loop.run(reqs)
+    loop.close()
+    print(prec, "mix stats:", stats)
+    assert stats["completed"] == len(reqs)
+    assert stats["p50_latency_s"] is not None and stats["p99_latency_s"] >= stats["p50_latency_s"]
+    # the requests overlapped on the device
+    spans = sorted((mix[r.rid]["admitted"], mix[r.rid]["finished"]) for r in reqs)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    for r in reqs:
+        r.arrival = 0.0
+        one = ServingLoop(precision_mode=prec, arena_bytes=1 << 30, max_slots=2, record=True)
+        alone, _ = one.run([r])
+        one.close()
+        a, m = alone[r.rid], mix[r.rid]
+        assert a["nhier"] == m["nhier"], r.rid
+        assert all(np.array_equal(x, y) for x, y in zip(a["states"], m["states"]))
+        assert np.array_equal(a["psi"], m["psi"]), r.rid
+        assert a["psi"].shape[0] == r.n_steps + 1
+        print("request", r.rid, "alone == in mix over", a["psi"].shape[0], "records; n_hier", min(a["nhier"]),
+              "..", max(a["nhier"]))
