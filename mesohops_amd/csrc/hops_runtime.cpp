@@ -162,6 +162,58 @@ struct System {
   std::vector<uint64_t> zob, zob2;  // two independent Zobrist tables [n_hm * 256]
 };
 
+// Identity of an auxiliary: two independent 64-bit Zobrist hashes of its
+// (mode, depth) entries; a +-e_m step updates them in O(1).
+struct AKey {
+  uint64_t a = 0, b = 0;
+  bool operator==(const AKey &o) const { return a == o.a && b == o.b; }
+};
+inline AKey key_of(const System &S, const Aux &x) {
+  AKey k;
+  for (char32_t e : x) {
+    k.a ^= S.zob[(size_t)aux_mode(e) * 256 + aux_val(e)];
+    k.b ^= S.zob2[(size_t)aux_mode(e) * 256 + aux_val(e)];
+  }
+  return k;
+}
+// key of the auxiliary whose depth in mode m changes from v0 to v1
+inline AKey key_step(const System &S, AKey k, int m, int v0, int v1) {
+  const size_t z0 = (size_t)m * 256 + v0, z1 = (size_t)m * 256 + v1;
+  k.a ^= S.zob[z0] ^ S.zob[z1];
+  k.b ^= S.zob2[z0] ^ S.zob2[z1];
+  return k;
+}
+
+// open-addressing map AKey -> auxiliary index
+struct KeyTable {
+  std::vector<AKey> keys;
+  std::vector<int32_t> vals;
+  size_t mask = 0;
+  static size_t hh(const AKey &k) { return (size_t)(k.a ^ (k.b * 0x9E3779B97F4A7C15ull)); }
+  void reset(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    keys.assign(cap, AKey());
+    vals.assign(cap, -1);
+    mask = cap - 1;
+  }
+  void insert(const AKey &k, int v) {
+    size_t h = hh(k) & mask;
+    while (vals[h] >= 0 && !(keys[h] == k)) h = (h + 1) & mask;
+    keys[h] = k;
+    vals[h] = v;
+  }
+  int find(const AKey &k) const {
+    if (vals.empty()) return -1;
+    size_t h = hh(k) & mask;
+    while (vals[h] >= 0) {
+      if (keys[h] == k) return vals[h];
+      h = (h + 1) & mask;
+    }
+    return -1;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Tables of one basis (hops_op_t, include/hops_mi355x.h) — the same arrays,
 // values and order as mesohops_amd/basis.py::BasisTables.
@@ -176,7 +228,7 @@ struct Basis {
   int eom = 2;
   std::vector<int> states;
   std::vector<Aux> aux;
-  std::unordered_map<Aux, int> index;
+  KeyTable index;  // auxiliary -> position
   int n = 0, nh = 0, nm = 0, nl = 0, na = 0, n_pat = 0, pat_w = 0, ell_w = 0;
   bool has_lt = false;
   std::vector<int> modes, rel_mode, absL, mode_lrel, mode_lact, act_abs, act_lrel, pos;
@@ -223,20 +275,16 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   const int n = (int)states.size(), nh = (int)aux.size();
   B.n = n;
   B.nh = nh;
-  B.index.reserve((size_t)nh * 2);
-  for (int i = 0; i < nh; ++i) B.index.emplace(aux[i], i);
+  B.index.reset(nh);
   B.aux_hash.resize(nh);
   B.aux_hash2.resize(nh);
   B.depth.resize(nh);
   for (int i = 0; i < nh; ++i) {
-    uint64_t h1 = 0, h2 = 0;
-    for (char32_t e : aux[i]) {
-      h1 ^= S.zob[(size_t)aux_mode(e) * 256 + aux_val(e)];
-      h2 ^= S.zob2[(size_t)aux_mode(e) * 256 + aux_val(e)];
-    }
-    B.aux_hash[i] = h1;
-    B.aux_hash2[i] = h2;
+    const AKey k = key_of(S, aux[i]);
+    B.aux_hash[i] = k.a;
+    B.aux_hash2[i] = k.b;
     B.depth[i] = aux_depth(aux[i]);
+    B.index.insert(k, i);
   }
   // modes: in use by the aux list, or of the states (hops_modes.py:111-171)
   std::vector<char> mflag(S.n_hm, 0);
@@ -348,18 +396,18 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   };
   std::vector<Conn> ups, dns;
   {
-    Aux tmp;
     ups.reserve((size_t)nh * 4);
     dns.reserve((size_t)nh * 4);
-    for (int j = 0; j < nh; ++j)
+    for (int j = 0; j < nh; ++j) {
+      const AKey kj{B.aux_hash[j], B.aux_hash2[j]};
       for (char32_t e : aux[j]) {
         const int m = aux_mode(e), v = aux_val(e);
-        aux_step(aux[j], m, -1, tmp);
-        auto it = B.index.find(tmp);
-        if (it == B.index.end()) continue;
-        ups.push_back({it->second, B.rel_mode[m], j, 0});
-        dns.push_back({j, B.rel_mode[m], it->second, v});
+        const int i = B.index.find(key_step(S, kj, m, v, v - 1));
+        if (i < 0) continue;
+        ups.push_back({i, B.rel_mode[m], j, 0});
+        dns.push_back({j, B.rel_mode[m], i, v});
       }
+    }
   }
   auto by_key = [](const Conn &x, const Conn &y) { return x.key != y.key ? x.key < y.key : x.lm < y.lm; };
   std::sort(ups.begin(), ups.end(), by_key);
@@ -472,9 +520,9 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   B.abs_lact.assign(S.n_hm, -2);
   for (int i = 0; i < nm; ++i) B.abs_lact[B.modes[i]] = B.mode_lact[i];
   for (int i = 0; i < nm; ++i) {
-    auto it = B.index.find(Aux(1, aux_pack(B.modes[i], 1)));
-    if (it != B.index.end() && B.mode_lact[i] >= 0) {
-      B.first_aux.push_back(it->second);
+    const int fi = B.index.find(key_step(S, AKey(), B.modes[i], 0, 1));
+    if (fi >= 0 && B.mode_lact[i] >= 0) {
+      B.first_aux.push_back(fi);
       B.first_lact.push_back(B.mode_lact[i]);
       B.first_mode.push_back(i);
     }
@@ -636,9 +684,9 @@ void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
   for (int lm = 0; lm < B.nm; ++lm) {
     if (!P.markov[B.modes[lm]]) continue;
     for (int k = 1; k < B.nh; ++k) F[(size_t)lm * B.nh + k] = 0;
-    auto it = B.index.find(Aux(1, aux_pack(B.modes[lm], 1)));
-    if (it != B.index.end())
-      for (int m2 = 0; m2 < B.nm; ++m2) F[(size_t)m2 * B.nh + it->second] = 0;
+    const int j = B.index.find(key_step(*B.sys, AKey(), B.modes[lm], 0, 1));
+    if (j >= 0)
+      for (int m2 = 0; m2 < B.nm; ++m2) F[(size_t)m2 * B.nh + j] = 0;
   }
 }
 
@@ -859,17 +907,17 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
   }
   // filters from the boundary auxiliaries (hops_fluxfilters.py:292-352)
   std::vector<uint8_t> F_dn((size_t)nm * nh, 0), F_up((size_t)nm * nh, 0);
-  Aux tmp;
   for (const Aux &a : sel.aux_bound) {
+    const AKey ka = key_of(S, a);
     for (int lm = 0; lm < nm; ++lm) {
-      aux_step(a, B.modes[lm], +1, tmp);
-      auto it = B.index.find(tmp);
-      if (it != B.index.end()) F_dn[(size_t)lm * nh + it->second] = 1;
+      const int v = aux_get(a, B.modes[lm]);
+      if (v >= 255) continue;
+      const int j = B.index.find(key_step(S, ka, B.modes[lm], v, v + 1));
+      if (j >= 0) F_dn[(size_t)lm * nh + j] = 1;
     }
     for (char32_t e : a) {
-      if (!aux_step(a, aux_mode(e), -1, tmp)) continue;
-      auto it = B.index.find(tmp);
-      if (it != B.index.end()) F_up[(size_t)B.rel_mode[aux_mode(e)] * nh + it->second] = 1;
+      const int j = B.index.find(key_step(S, ka, aux_mode(e), aux_val(e), aux_val(e) - 1));
+      if (j >= 0) F_up[(size_t)B.rel_mode[aux_mode(e)] * nh + j] = 1;
     }
   }
   for (int lm = 0; lm < nm; ++lm) {
@@ -1551,7 +1599,7 @@ bool same_basis(const Basis &B, const std::vector<int> &states, const std::vecto
   if (states.size() != B.states.size() || aux.size() != B.aux.size()) return false;
   if (!std::equal(states.begin(), states.end(), B.states.begin())) return false;
   for (const Aux &a : aux)
-    if (!B.index.count(a)) return false;
+    if (B.index.find(key_of(*B.sys, a)) < 0) return false;
   return true;
 }
 
@@ -1626,10 +1674,7 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
       if (L.tabs[t].second) std::memcpy(htab[i] + L.tab_off[t], L.tabs[t].first, L.tabs[t].second);
     const Basis &Bo = sl.B, &Bn = sl.Bnew;
     int32_t *inv = reinterpret_cast<int32_t *>(hinv[i]);
-    for (int j = 0; j < Bn.nh; ++j) {
-      auto it = Bo.index.find(Bn.aux[j]);
-      inv[j] = it != Bo.index.end() ? it->second : -1;
-    }
+    for (int j = 0; j < Bn.nh; ++j) inv[j] = Bo.index.find(AKey{Bn.aux_hash[j], Bn.aux_hash2[j]});
     for (int q = 0; q < Bn.n; ++q) inv[Bn.nh + q] = Bo.pos[Bn.states[q]];
   });
   std::vector<hops_segment_t> segs;
