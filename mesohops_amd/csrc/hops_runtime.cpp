@@ -751,32 +751,37 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   std::vector<double> E_up((size_t)nm * nh, 0.0), E_dn((size_t)nm * nh, 0.0);
   std::vector<uint8_t> F_up;
   markov_up(P, B, F_up);
-  std::vector<uint8_t> not_m2(n);
+  std::vector<double> colsum(nh, 0.0);
+  for (int k = 0; k < nh; ++k) {
+    const double *c2 = &out.absC2[(size_t)k * n];
+    double a = 0;
+    for (int s2 = 0; s2 < n; ++s2) a += c2[s2];
+    colsum[k] = a;
+  }
   for (int lm = 0; lm < nm; ++lm) {
     const int m = B.modes[lm];
     const double aw = std::abs(S.W[m]);
     const double gw2 = std::norm(S.G[m] / S.W[m]);
     const cplx X = out.X2[lm];
     const double nX = std::norm(X);
-    std::fill(not_m2.begin(), not_m2.end(), 1);
-    for (auto &e : out.M2[lm]) not_m2[e.first] = 0;
+    const auto &M2 = out.M2[lm];
+    double *eu = &E_up[(size_t)lm * nh], *ed = &E_dn[(size_t)lm * nh];
+    const uint8_t *kv = &kval[(size_t)lm * nh];
     for (int k = 0; k < nh; ++k) {
-      double p_up = 0, p_dn = 0;
       const double *c2 = &out.absC2[(size_t)k * n];
-      // sum_s |M2[m,s]|^2 |C2[s,k]|^2 and sum_s |M2[m,s] - X|^2 |C2[s,k]|^2
-      for (auto &e : out.M2[lm]) {
-        p_up += std::norm(e.second) * c2[e.first];
-        p_dn += std::norm(e.second - X) * c2[e.first];
+      // sum_s |M2[m,s]|^2 |C2[s,k]|^2 and sum_s |M2[m,s] - X|^2 |C2[s,k]|^2;
+      // the states without an M2 entry contribute |X|^2 |C2[s,k]|^2
+      double p_up = 0, p_dn = 0, on = 0;
+      for (auto &e : M2) {
+        const double c = c2[e.first];
+        p_up += std::norm(e.second) * c;
+        p_dn += std::norm(e.second - X) * c;
+        on += c;
       }
-      if (nX != 0) {
-        double zx = 0;
-        for (int s2 = 0; s2 < n; ++s2)
-          if (not_m2[s2]) zx += c2[s2];
-        p_dn += nX * zx;
-      }
-      const double kw = aw * (1.0 + kval[(size_t)lm * nh + k]);
-      E_up[(size_t)lm * nh + k] = (kw * kw * p_up) / h2;
-      E_dn[(size_t)lm * nh + k] = (gw2 * p_dn) / h2;
+      if (nX != 0) p_dn += nX * (colsum[k] - on);
+      const double kw = aw * (1.0 + kv[k]);
+      eu[k] = (kw * kw * p_up) / h2;
+      ed[k] = (gw2 * p_dn) / h2;
     }
   }
   for (int k = 0; k < nh; ++k) {
@@ -813,63 +818,56 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) has_up[(size_t)B.up_mode[u] * nh + k] = 1;
     for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) has_dn[(size_t)B.dn_mode[u] * nh + k] = 1;
   }
-  struct Cand {
-    int k, lm, dir;
-    double v;
+  // flux out of the basis: up (depth < maxhier, k+e_m not in the basis) and
+  // down (k-e_m not in the basis), in the reference's insertion order:
+  // stable column, flux up by mode, then flux down by mode
+  auto up_ok = [&](int k, int lm) {
+    const size_t q = (size_t)lm * nh + k;
+    return B.depth[k] < P.maxhier && !has_up[q] && F_up[q] && E_up[q] != 0;
   };
-  // insertion order of the reference: stable column, flux up by mode, then
-  // flux down by mode
-  std::vector<Cand> cand;
+  auto dn_ok = [&](int k, int lm) {
+    const size_t q = (size_t)lm * nh + k;
+    return kval[q] > 0 && !has_dn[q] && E_dn[q] != 0;
+  };
   std::vector<double> nz;
-  for (int k : out.stable_idx) {
-    const bool room = B.depth[k] < P.maxhier;
+  for (int k : out.stable_idx)
     for (int lm = 0; lm < nm; ++lm) {
-      const size_t q = (size_t)lm * nh + k;
-      if (room && !has_up[q] && F_up[q] && E_up[q] != 0) {
-        cand.push_back({k, lm, +1, E_up[q]});
-        nz.push_back(E_up[q]);
-      }
+      if (up_ok(k, lm)) nz.push_back(E_up[(size_t)lm * nh + k]);
+      if (dn_ok(k, lm)) nz.push_back(E_dn[(size_t)lm * nh + k]);
     }
-    for (int lm = 0; lm < nm; ++lm) {
-      const size_t q = (size_t)lm * nh + k;
-      if (kval[q] > 0 && !has_dn[q] && E_dn[q] != 0) {
-        cand.push_back({k, lm, -1, E_dn[q]});
-        nz.push_back(E_dn[q]);
-      }
-    }
-  }
   double discarded = 0;
   const double thr1 = partial_threshold(nz, dbound * P.f_discard * P.f_discard, 0.0, &discarded);
-  // accumulate by target auxiliary, keyed by two independent Zobrist hashes
-  struct Key {
-    uint64_t a, b;
-    bool operator==(const Key &o) const { return a == o.a && b == o.b; }
-  };
-  struct KeyHash {
-    size_t operator()(const Key &k) const { return (size_t)(k.a ^ (k.b * 0x9E3779B97F4A7C15ull)); }
+  // accumulate the survivors by target auxiliary (Zobrist key)
+  struct Ent {
+    AKey key;
+    int32_t id;
   };
   size_t cap = 16;
-  while (cap < cand.size() * 2) cap <<= 1;
-  std::vector<Key> tkey(cap);
-  std::vector<int32_t> tval(cap, -1);
-  std::vector<int> rep;  // representative candidate of each target
+  while (cap < nz.size() * 2) cap <<= 1;
+  std::vector<Ent> tab(cap, Ent{AKey(), -1});
+  struct Rep {
+    int k, lm, dir;
+  };
+  std::vector<Rep> rep;
   std::vector<double> vals;
-  for (size_t c = 0; c < cand.size(); ++c) {
-    const Cand &cd = cand[c];
-    if (cd.v <= thr1) continue;
-    const int k = cd.k, m = B.modes[cd.lm];
-    const int v0 = kval[(size_t)cd.lm * nh + k], v1 = v0 + cd.dir;
-    const size_t z0 = (size_t)m * 256 + v0, z1 = (size_t)m * 256 + v1;
-    const Key key{B.aux_hash[k] ^ S.zob[z0] ^ S.zob[z1], B.aux_hash2[k] ^ S.zob2[z0] ^ S.zob2[z1]};
-    size_t h = KeyHash()(key) & (cap - 1);
-    while (tval[h] >= 0 && !(tkey[h] == key)) h = (h + 1) & (cap - 1);
-    if (tval[h] < 0) {
-      tkey[h] = key;
-      tval[h] = (int32_t)vals.size();
-      rep.push_back((int)c);
+  auto add = [&](int k, int lm, int dir, double v) {
+    const int m = B.modes[lm];
+    const int v0 = kval[(size_t)lm * nh + k];
+    const AKey key = key_step(S, AKey{B.aux_hash[k], B.aux_hash2[k]}, m, v0, v0 + dir);
+    size_t h = KeyTable::hh(key) & (cap - 1);
+    while (tab[h].id >= 0 && !(tab[h].key == key)) h = (h + 1) & (cap - 1);
+    if (tab[h].id < 0) {
+      tab[h] = Ent{key, (int32_t)vals.size()};
+      rep.push_back({k, lm, dir});
       vals.push_back(0.0);
     }
-    vals[tval[h]] += cd.v;
+    vals[tab[h].id] += v;
+  };
+  for (int k : out.stable_idx) {
+    for (int lm = 0; lm < nm; ++lm)
+      if (up_ok(k, lm) && E_up[(size_t)lm * nh + k] > thr1) add(k, lm, +1, E_up[(size_t)lm * nh + k]);
+    for (int lm = 0; lm < nm; ++lm)
+      if (dn_ok(k, lm) && E_dn[(size_t)lm * nh + k] > thr1) add(k, lm, -1, E_dn[(size_t)lm * nh + k]);
   }
   double dummy = 0;
   const double thr2 = partial_threshold(vals, dbound, discarded, &dummy);
@@ -878,7 +876,7 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   Aux tgt;
   for (size_t i = 0; i < vals.size(); ++i)
     if (vals[i] > thr2) {
-      const Cand &cd = cand[rep[i]];
+      const Rep &cd = rep[i];
       aux_step(B.aux[cd.k], B.modes[cd.lm], cd.dir, tgt);
       out.aux_bound.push_back(tgt);
       out.aux_new.push_back(tgt);
