@@ -9,7 +9,16 @@ dt = 2 fs.  A step is one RK4 step of all 256 trajectories; psi of every
 trajectory is recorded on the device after every step.
 
 Prints one JSON line.  value = trajectories x RK4 steps / s over the timed
-region (inputs, operator and noise already in HBM).
+region (inputs, operator and noise already in HBM).  The line also carries a
+short measurement of the adaptive batch ("adaptive_c3", below).
+
+--workload c3: adaptive NORMALIZED NONLINEAR batch — 100-site chain (same
+bath), depth 10, delta_a = delta_s = 1e-3, psi0 on site 50, dt 1 fs, TAU
+0.5 fs, TLEN 512 fs, noise seeds 0..B-1; every trajectory keeps its own basis
+(ragged batch); complex64 stages, FP64 state carry, norms and error bounds.
+--workload c4: serving stream — Poisson arrivals (200/s, seed 0) of adaptive
+requests with chain length U{2..200}, depth U{2..10}, delta U[1e-4, 1e-2],
+t_max U[200, 2000] fs, dt 1 fs; continuous batching; p50/p99 latency.
 """
 import argparse
 import json
@@ -95,6 +104,137 @@ def algorithmic_bytes_per_traj_step(tables, B):
     return 96 * n_phi + 3 * tables.n_act * 8 * 2 + s_op / B
 
 
+def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_bytes=None, log=None):
+    """The adaptive batch: B trajectories, each with its own basis."""
+    import torch
+    from mesohops_amd import HopsTrajectory, _lib
+    from mesohops_amd.ragged import AdaptiveBatch
+    from mesohops_amd.workloads import chain_system
+    sysp = chain_system(100)
+    psi = np.zeros(100, np.complex128)
+    psi[50] = 1.0
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    noise = {"MODEL": "FFT_FILTER", "TLEN": 512.0, "TAU": 0.5}
+    hier = {"MAXHIER": 10}
+
+    def make(nb, p):
+        ab = AdaptiveBatch(sysp, eom, dict(noise, SEED=0), None, hier, seeds=list(range(nb)), precision_mode=p,
+                           arena_bytes=arena_bytes)
+        ab.make_adaptive(1e-3, 1e-3)
+        ab.initialize(psi, tau=1.0)
+        return ab
+
+    t0 = time.perf_counter()
+    ab = make(B, prec)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    _lib.profile_enable(True)
+    chunks = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        c = min(chunk, steps - done)
+        t1 = time.perf_counter()
+        ab.propagate(float(c), 1.0)
+        torch.cuda.synchronize()
+        infos = [ab.rt.info(sl) for sl in ab.slots]
+        chunks.append({"t_fs": done + c, "s": time.perf_counter() - t1,
+                       "mean_n_phi": float(np.mean([i["n_state"] * i["n_hier"] for i in infos])),
+                       "max_n_hier": int(max(i["n_hier"] for i in infos))})
+        done += c
+        if log:
+            log("c3 t=%d fs: %.0f traj*steps/s, mean N_phi %.0f" % (done, B * c / chunks[-1]["s"],
+                                                                  chunks[-1]["mean_n_phi"]))
+    elapsed = time.perf_counter() - t0
+    ms, n = _lib.profile_read()
+    _lib.profile_enable(False)
+    timers = ab.rt.timers()
+    es = 16 if prec == "fp64" else 8
+    stage_bytes = timers["rows_stepped"] * (8 * es + 32)
+    achieved = stage_bytes / (ms[1] * 1e-3) / 1e9 if ms[1] > 0 else 0.0
+    # plain per-trajectory FP64 GPU path over the first single_steps steps
+    tr = HopsTrajectory(sysp, eom, dict(noise, SEED=0), hierarchy_param=hier)
+    tr.make_adaptive(1e-3, 1e-3)
+    tr.initialize(psi)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):  # the reference's progress print stays off the JSON line
+        tr.propagate(float(single_steps), 1.0)
+    torch.cuda.synchronize()
+    single_rate = single_steps / (time.perf_counter() - t1)
+    # the batch's rate over the same first steps (whole chunks)
+    win, prev = [], 0
+    for c in chunks:
+        if c["t_fs"] > single_steps and win:
+            break
+        win.append((c["t_fs"] - prev, c["s"]))
+        prev = c["t_fs"]
+    batch_win = B * sum(w[0] for w in win) / sum(w[1] for w in win)
+    out = {"value": B * steps / elapsed, "trajectories": B, "steps": steps, "elapsed_s": elapsed, "init_s": t_init,
+           "ms_per_step": elapsed / steps * 1e3, "chunks": chunks,
+           "gpu_fp64_single_traj_steps_per_s": single_rate, "single_window_steps": single_steps,
+           "batch_rate_same_window": batch_win, "speedup_vs_gpu_fp64_single_same_window": batch_win / single_rate,
+           "host_phase_s": {k: v for k, v in timers.items() if k != "rows_stepped"},
+           "stage_kernel": {"avg_ms": ms[1] / max(n[1], 1), "launches": n[1],
+                            "algorithmic_bytes": stage_bytes, "achieved_GBs": achieved,
+                            "frac_of_8TBs": achieved / HBM_PEAK_GBS}}
+    if ref_traj > 0:
+        ref = make(ref_traj, "fp64")
+        ref.propagate(float(steps), 1.0)
+        err = 0.0
+        for b in range(ref_traj):
+            pa = np.abs(ab.results(b)[0]) ** 2
+            pb = np.abs(ref.results(b)[0]) ** 2
+            err = max(err, float(np.max(np.abs(pa - pb))))
+        out["pop_err_vs_fp64"] = err
+        out["pop_err_trajectories"] = ref_traj
+        ref.rt.close()
+    ab.rt.close()
+    return out
+
+
+def run_c4(n_req, prec="mixed", time_limit=600.0, log=None):
+    from mesohops_amd.serving import ServingLoop
+    from mesohops_amd.workloads import c4_requests
+    reqs = c4_requests(n_req)
+    loop = ServingLoop(precision_mode=prec, max_slots=512, record=False)
+    done, stats = loop.run(reqs, time_limit=time_limit, log=log)
+    stats["host_phase_s"] = {k: v for k, v in loop.rt.timers().items() if k != "rows_stepped"}
+    loop.close()
+    return done, stats
+
+
+def main_c3(args):
+    out3 = run_c3(args.batch if args.batch != 256 else 256, args.steps, args.precision, args.single_steps,
+                  args.ref_traj, log=lambda m: print(m, file=sys.stderr, flush=True))
+    sk = out3["stage_kernel"]
+    out = {"metric": METRIC, "value": out3["value"], "unit": "traj·RK4-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 0, "ms_per_step": out3["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "c64" if args.precision == "mixed" else "c128",
+           "data": "synthetic",
+           "config": {"workload": "adaptive NORMALIZED NONLINEAR batch: 100-site chain, 2 DL modes/site, depth 10, "
+                                  "delta_a = delta_s = 1e-3, %d trajectories with their own bases, dt 1 fs, "
+                                  "t = 0..%d fs" % (out3["trajectories"], args.steps)},
+           "roofline": {"bound": "hbm", "achieved": sk["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": sk["frac_of_8TBs"], "traffic": None, "kernel": "k_stage_slots",
+                        "kernel_avg_ms": sk["avg_ms"]},
+           "adaptive": out3}
+    print(json.dumps(out), flush=True)
+
+
+def main_c4(args):
+    done, stats = run_c4(args.requests, args.precision, log=lambda m: print(m, file=sys.stderr, flush=True))
+    out = {"metric": METRIC, "value": stats["traj_steps_per_s"], "unit": "traj·RK4-steps/s", "n_gpus": 1,
+           "steps": stats["iterations"], "warmup": 0, "ms_per_step": stats["wall_s"] / max(stats["iterations"], 1) * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64" if args.precision == "mixed" else "c128", "data": "synthetic",
+           "config": {"workload": "serving stream: %d adaptive requests, Poisson arrivals 200/s, chain 2-200 sites, "
+                                  "depth 2-10, delta 1e-4..1e-2, t_max 200-2000 fs, dt 1 fs" % args.requests},
+           "p50_latency_s": stats["p50_latency_s"], "p99_latency_s": stats["p99_latency_s"], "serving": stats}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,11 +250,20 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="trajectories per cache-resident chunk with --lanes")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--requests", type=int, default=64, help="c4: requests in the stream")
+    ap.add_argument("--c3-steps", type=int, default=40, help="steps of the adaptive summary in the c2 run (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (trajectories would shard "
               "one batch per GPU, see DESIGN.md); got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        if args.steps > 500:
+            sys.exit("c3 runs at most 500 steps of 1 fs (TLEN 512 fs)")
+        return main_c3(args)
+    if args.workload == "c4":
+        return main_c4(args)
     if (args.steps + 1) * DT > NOISE["TLEN"] or (args.warmup + 1) * DT > NOISE["TLEN"]:
         sys.exit("steps x dt exceeds the noise length %.0f fs" % NOISE["TLEN"])
 
@@ -251,6 +400,14 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "prologue_avg_ms": ms[0] / max(n[0], 1)},
     }
+    if args.c3_steps > 0:
+        del batch, st
+        torch.cuda.empty_cache()
+        c3 = run_c3(256, args.c3_steps, "mixed", min(20, args.c3_steps), 4)
+        c3.pop("chunks", None)
+        out["adaptive_c3"] = dict(c3, config="adaptive NORMALIZED NONLINEAR batch: 100-site chain, depth 10, "
+                                             "delta 1e-3, 256 trajectories with their own bases, t = 0..%d fs"
+                                             % args.c3_steps)
     if not args.no_cpu:
         procs = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, procs)

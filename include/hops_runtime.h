@@ -84,7 +84,8 @@ long long hops_rt_arena_free(void *batch);
 /* Accumulated wall seconds per phase of hops_rt_batch_step (no counterpart
    in the reference): [0] RK4 steps, [1] error sums on the device, [2] host
    hierarchy selection, [3] state errors and selection, [4] table build of
-   new bases, [5] device update (operators, Phi moves), [6] psi fetch. */
+   new bases, [5] device update (operators, Phi moves), [6] psi fetch;
+   [7] (a count, not seconds) hierarchy rows advanced by RK4 steps. */
 int hops_rt_batch_timers(void *batch, double *out /*[8]*/);
 
 /* Advance every active slot (slots_or_null == NULL) or the listed slots by

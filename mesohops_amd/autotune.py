@@ -6,6 +6,7 @@ streaming) and keeps the fastest.  The batch is rewound to its initial state
 afterwards, so tuning does not change results (every candidate computes the
 same numbers; only the memory layout and the launch shape differ).
 """
+import sys
 import time
 
 import numpy as np
@@ -43,7 +44,7 @@ def tune(batch, psi_0, tau, steps=12, candidates=None, verbose=False):
         timings[(lanes, stationary, chunk)] = (time.perf_counter() - t0) / steps
         if verbose:
             print("autotune lanes=%d stationary=%d chunk=%d: %.3f ms/step"
-                  % (lanes, stationary, chunk, timings[(lanes, stationary, chunk)] * 1e3))
+                  % (lanes, stationary, chunk, timings[(lanes, stationary, chunk)] * 1e3), file=sys.stderr)
     best = min(timings, key=timings.get)
     batch.relayout(best[0])
     _lib.set_option("stationary_stage", 1 if best[1] else 0)

@@ -1255,19 +1255,25 @@ int slots_step_t(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *l
   if (shm > 64 * 1024)
     HIP_TRY(hipFuncSetAttribute((const void *)k_prologue_slots<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
+  size_t used = 0;
+  if (int rc = prof_mark(st, 2, &used)) return rc;
   if (mixed) {
     k_prime_slots<<<n_tiles, 256, 0, st>>>(ops, bts, tiles);
     HIP_TRY(hipGetLastError());
+    if (int rc = prof_mark(st, 2, &used)) return rc;
   }
   for (int i = 0; i < 4; ++i) {
     k_prologue_slots<R><<<n_list, 256, shm, st>>>(ops, bts, list, it, i);
     HIP_TRY(hipGetLastError());
+    if (int rc = prof_mark(st, 0, &used)) return rc;
     k_stage_slots<R><<<n_tiles, 256, 0, st>>>(ops, bts, tiles, i);
     HIP_TRY(hipGetLastError());
+    if (int rc = prof_mark(st, 1, &used)) return rc;
   }
   k_final_slots<<<n_list, 256, 0, st>>>(ops, bts, list);
   HIP_TRY(hipGetLastError());
-  return 0;
+  if (int rc = prof_mark(st, 2, &used)) return rc;
+  return prof_collect(used);
 }
 
 template <typename R>

@@ -1172,7 +1172,7 @@ struct Batch {
   std::vector<double> dts, nrms;
   // accumulated wall time (s) per phase: step, errors on the device, host
   // hierarchy selection, state errors + selection, table build, device
-  // update, psi fetch
+  // update, psi fetch; [7]: hierarchy rows advanced by RK4 steps
   double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -1446,6 +1446,7 @@ int run_step(Batch &b, const std::vector<int> &list) {
   RT_TRY(st.push(list.data(), list.size() * 4, &dl));
   RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dt));
   RT_TRY(st.flush());
+  for (int s : list) b.tm[7] += (double)b.slots[s].B.n * b.slots[s].B.nh;
   RT_TRY(hops_slots_step(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const uint32_t *)dt,
                          (int)tiles.size(), (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm_of(b, list),
                          b.stream));

@@ -271,8 +271,8 @@ class RaggedBatch:
         out = np.zeros(8, np.float64)
         check(lib().hops_rt_batch_timers(self.h, _ptr(out)))
         names = ["rk4_step", "errors_device", "select_hier", "select_states", "build_tables", "update_device",
-                 "fetch_psi"]
-        return dict(zip(names, out[:7].tolist()))
+                 "fetch_psi", "rows_stepped"]
+        return dict(zip(names, out.tolist()))
 
     def close(self):
         h = getattr(self, "h", None)
