@@ -1189,17 +1189,31 @@ __global__ __launch_bounds__(256) void k_err_slots(const hops_op_t *__restrict__
       e[k] = acc;
     }
   } else {
+    // per state: g = blockDim / n threads split the auxiliaries (k = j, j+g,
+    // ...), then one thread adds the g partial sums in a fixed order
+    __shared__ double part[256];
     const uint8_t *m = mask + moff[slot];
-    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    for (int s0 = 0; s0 < n; s0 += blockDim.x) {
+      const int ns = min(n - s0, (int)blockDim.x);
+      const int g = blockDim.x / ns;
+      const int s = s0 + threadIdx.x / g, j = threadIdx.x % g;
       double acc = 0.0;
-      for (int k = 0; k < nh; ++k) {
-        if (!m[k]) continue;
-        const long r = (long)k * n + s;
-        const double vx = (double)d[r].x / hbar + p[r].x / tau;
-        const double vy = (double)d[r].y / hbar + p[r].y / tau;
-        acc += vx * vx + vy * vy;
+      if ((int)threadIdx.x < g * ns)
+        for (int k = j; k < nh; k += g) {
+          if (!m[k]) continue;
+          const long r = (long)k * n + s;
+          const double vx = (double)d[r].x / hbar + p[r].x / tau;
+          const double vy = (double)d[r].y / hbar + p[r].y / tau;
+          acc += vx * vx + vy * vy;
+        }
+      part[threadIdx.x] = acc;
+      __syncthreads();
+      if ((int)threadIdx.x < ns) {
+        double t = 0.0;
+        for (int q = 0; q < g; ++q) t += part[threadIdx.x * g + q];
+        e[s0 + threadIdx.x] = t;
       }
-      e[s] = acc;
+      __syncthreads();
     }
   }
 }

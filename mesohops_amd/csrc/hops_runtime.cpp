@@ -1748,10 +1748,26 @@ double keep_scale(const Slot &sl, const std::vector<int> &new_states, const std:
   return std::sqrt(a) / std::sqrt(k);
 }
 
-int device_copy(Batch &b, void *dst, const void *src, size_t bytes) {
-  RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.stream));
-  return 0;
-}
+// device-to-device copies of one phase, run as one segment-copy launch
+// (every region is a 256-byte aligned block, sizes rounded to 16 bytes fit)
+struct Copies {
+  std::vector<hops_segment_t> segs;
+  void add(void *dst, const void *src, size_t bytes) {
+    segs.push_back({src, dst, (long long)((bytes + 15) & ~(size_t)15)});
+  }
+  int run(Batch &b) {
+    if (segs.empty()) return 0;
+    RT_TRY(ensure_staging(b, segs.size() * sizeof(hops_segment_t) + 4096));
+    Stage st{&b};
+    void *d;
+    RT_TRY(st.push(segs.data(), segs.size() * sizeof(hops_segment_t), &d));
+    RT_TRY(st.flush());
+    RT_TRY(hops_copy_segments((const hops_segment_t *)d, (int)segs.size(), b.stream));
+    RT_HIP(hipStreamSynchronize(b.stream));
+    segs.clear();
+    return 0;
+  }
+};
 
 // initial basis update of newly added slots (trajectory/hops_trajectory.py:279-317)
 int init_pending(Batch &b) {
@@ -1810,14 +1826,15 @@ int init_pending(Batch &b) {
 int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
   // early INCH_WORM slots keep the state before the step
   std::vector<int> early, upd, plain;
+  Copies cp;
   for (int s : list) {
     Slot &sl = b.slots[s];
     const bool adaptive = sl.prm.delta_a > 0 || sl.prm.delta_s > 0;
     const bool is_early = adaptive && sl.early_counter < sl.prm.early_steps;
     if (is_early && sl.prm.early_integrator == 0) {
       early.push_back(s);
-      RT_TRY(device_copy(b, sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16));
-      RT_TRY(device_copy(b, sl.zmem_prev, sl.bt.zmem_d, (size_t)sl.sys->n_hm * 16));
+      cp.add(sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16);
+      cp.add(sl.zmem_prev, sl.bt.zmem_d, (size_t)sl.sys->n_hm * 16);
     }
     const int us = sl.prm.update_step > 0 ? sl.prm.update_step : 1;
     if (adaptive && (is_early || (index_t + 1) % us == 0)) {
@@ -1826,6 +1843,7 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
       plain.push_back(s);
     }
   }
+  RT_TRY(cp.run(b));
   RT_TRY(run_step(b, list));
   for (int s : upd) b.dts[s] = b.slots[s].prm.dt;
   RT_TRY(define_bases(b, upd));
@@ -1853,14 +1871,16 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
       ts.push_back(su);
       ta.push_back(au);
       sc.push_back(1.0);  // every old state is kept: |psi| unchanged
-      RT_TRY(device_copy(b, sl.bt.phi_d, sl.phi_prev, (size_t)sl.B.n * sl.B.nh * 16));
-      RT_TRY(device_copy(b, sl.bt.zmem_d, sl.zmem_prev, (size_t)sl.sys->n_hm * 16));
+      cp.add(sl.bt.phi_d, sl.phi_prev, (size_t)sl.B.n * sl.B.nh * 16);
+      cp.add(sl.bt.zmem_d, sl.zmem_prev, (size_t)sl.sys->n_hm * 16);
     }
+    RT_TRY(cp.run(b));
     RT_TRY(update_bases(b, redo, ts, ta, sc));
     for (int s : redo) {
       Slot &sl = b.slots[s];
-      RT_TRY(device_copy(b, sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16));
+      cp.add(sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16);
     }
+    RT_TRY(cp.run(b));
     RT_TRY(run_step(b, redo));
     RT_TRY(define_bases(b, redo));
     std::vector<int> nxt;
