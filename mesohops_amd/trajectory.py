@@ -15,11 +15,11 @@ import numpy as np
 import torch
 
 from . import _lib
-from .basis import BasisTables, SystemInfo, apply_static_filters, triangular_aux_list
+from .basis import BasisTables, SystemInfo, apply_static_filters, aux_order_key, triangular_aux_list
 from .engine import BatchState, DeviceOperator, _stream_ptr
 from .errors import LockedException, TrajectoryError, UnsupportedRequest
 from .noise import HopsNoise, device_noise
-from .params import (EOM_DICT_DEFAULT, EOM_DICT_TYPES, HIERARCHY_DICT_DEFAULT, HIERARCHY_DICT_TYPES,
+from .params import (EOM_DICT_DEFAULT, EOM_DICT_TYPES, HIERARCHY_DICT_DEFAULT, HIERARCHY_DICT_TYPES, NOISE_DICT_DEFAULT,
                      INTEGRATION_DICT_DEFAULT, INTEGRATION_DICT_TYPES, initialize_dict, precision)
 
 _EOM_KIND = {"LINEAR": _lib.HOPS_EOM_LINEAR, "NONLINEAR": _lib.HOPS_EOM_NONLINEAR,
@@ -443,6 +443,106 @@ class HopsTrajectory:
             return b._plan(t_advance, tau)
         finally:
             b.t = saved
+
+    # -- checkpoints (trajectory/hops_trajectory.py:853-1034) ------------
+    _SYSTEM_KEYS = ["HAMILTONIAN", "GW_SYSBATH", "L_HIER", "L_NOISE1", "ALPHA_NOISE1", "PARAM_NOISE1",
+                    "L_NOISE2", "ALPHA_NOISE2", "PARAM_NOISE2", "L_LT_CORR", "PARAM_LT_CORR"]
+
+    def save_checkpoint(self, filepath, compress=True, drop_seed=False):
+        """Parameters, the current hierarchy vector, state and auxiliary
+        lists, noise memory, time, early-step counter and the storage, in the
+        reference's .npz layout (keys phi, z_mem, t, state_list, aux_list,
+        early_counter, storage_data, storage_meta, params)."""
+        if not self.__initialized__:
+            raise TrajectoryError("save_checkpoint before initialize")
+        b = self._batch
+        setup = b.setup
+        eom_param = {k: v for k, v in setup.eom.items() if k != "ADAPTIVE"}
+        for key in ("DELTA_A", "DELTA_S", "F_DISCARD"):
+            if key in eom_param:
+                eom_param[key] = float(eom_param[key])
+        if eom_param.get("UPDATE_STEP") is None:
+            eom_param.pop("UPDATE_STEP", None)
+
+        def noise_dict(nz):
+            return {k: nz.param[k] for k in NOISE_DICT_DEFAULT if k in nz.param and not (k == "SEED" and drop_seed)}
+
+        sysp = setup.system_param
+        params = {
+            "system_param": {k: sysp[k] for k in self._SYSTEM_KEYS if k in sysp},
+            "hierarchy_param": {k: setup.hier[k] for k in HIERARCHY_DICT_DEFAULT if k in setup.hier},
+            "eom_param": eom_param,
+            "noise1_param": noise_dict(self.noise1),
+            "noise2_param": noise_dict(self.noise2),
+            "integration_param": dict(self.integration_param),
+            "storage_param": dict(self.storage.storage_dic),
+        }
+        if not drop_seed and self.noise1.param["SEED"] is None:
+            warnings.warn("Noise1 seed is None but drop_seed is False; this may cause unphysical behavior.",
+                          UserWarning)
+        aux = np.empty(len(self.auxiliary_list), dtype=object)
+        for i, a in enumerate(self.auxiliary_list):
+            aux[i] = np.array([[m, v] for (m, v) in a], dtype=int).reshape(-1, 2)
+        early = self._adaptive.early_counter if self._adaptive is not None else 0
+        ck = {"phi": self.phi, "z_mem": self.z_mem, "t": self.t,
+              "state_list": np.array(self.state_list, dtype=int), "aux_list": aux, "early_counter": early,
+              "storage_data": self.storage.data, "storage_meta": self.storage.metadata, "params": params}
+        (np.savez_compressed if compress else np.savez)(filepath, **ck)
+
+    @classmethod
+    def load_checkpoint(cls, filename, add_seed1=None, add_seed2=None, add_system_param=None,
+                        precision_mode="fp64", device="cuda:0"):
+        """Rebuild a trajectory from a file written by save_checkpoint: the
+        same parameters (seeds and system optionally replaced), then the
+        stored basis, hierarchy vector, noise memory, time and storage, so
+        that propagation continues exactly where it stopped.  Reads pickled
+        parameter objects (functions, sparse matrices), as the reference's
+        checkpoints do: load only files you trust."""
+        data = np.load(filename, allow_pickle=True)
+        params = data["params"].item()
+        if add_seed1 is not None:
+            params["noise1_param"]["SEED"] = add_seed1
+        if add_seed2 is not None:
+            params["noise2_param"]["SEED"] = add_seed2
+        if add_system_param is not None:
+            params["system_param"] = add_system_param
+        traj = cls(params["system_param"], eom_param=params["eom_param"], noise_param=params["noise1_param"],
+                   noise2_param=params["noise2_param"], hierarchy_param=params["hierarchy_param"],
+                   storage_param=params["storage_param"], integration_param=params["integration_param"],
+                   precision_mode=precision_mode, device=device)
+        states = [int(s) for s in data["state_list"]]
+        phi = np.asarray(data["phi"], dtype=np.complex128)
+        psi_0 = np.zeros(traj._batch.setup.info.nstates, dtype=np.complex128)
+        psi_0[states] = phi[:len(states)]
+        traj.initialize(psi_0)
+        aux = [tuple((int(m), int(v)) for m, v in np.asarray(a).reshape(-1, 2)) for a in data["aux_list"]]
+        zm = data["z_mem"].item()
+        zm = np.asarray(zm.toarray() if hasattr(zm, "toarray") else zm, dtype=np.complex128).reshape(-1)
+        info = traj._batch.setup.info
+        zdev = torch.zeros(info.n_hmodes + 1, dtype=torch.complex128)
+        zdev[: info.n_hmodes] = torch.from_numpy(zm[: info.n_hmodes])
+        if traj._adaptive is not None:
+            ad = traj._adaptive
+            ad.update_basis(ad.phi_host(), states, aux)
+            if [int(s) for s in ad.tb.state_list] != sorted(states) or set(ad.tb.aux_list) != set(aux):
+                raise TrajectoryError("checkpoint basis could not be restored")
+            ad.st.set_phi(phi)
+            ad.st.zmem.copy_(zdev.to(ad.device))
+            ad._sync_zmem_from_state()
+            ad.early_counter = int(data["early_counter"])
+            traj._t = float(data["t"])
+        else:
+            b = traj._batch
+            if [tuple(a) for a in b.tables.aux_list] != sorted(aux, key=aux_order_key):
+                raise TrajectoryError("checkpoint hierarchy does not match the parameters")
+            b.state.set_phi(phi)
+            b.state.zmem[: info.n_hmodes + 1].copy_(zdev.to(b.device)[: info.n_hmodes + 1])
+            b.t = float(data["t"])
+            b.t_axis = [b.t]
+            b.psi_traj = [[phi[: b.tables.n_state].copy()]]
+        traj.storage.data = data["storage_data"].item()
+        traj.storage.metadata = data["storage_meta"].item()
+        return traj
 
     # -- readbacks (trajectory/hops_trajectory.py:1071-1125) -------------
     @property
