@@ -186,6 +186,13 @@ int hops_noise_filter(const void *z_d, int R, int n_t, const double *s_w_d,
                       const double *nu_d, const int32_t *row_lop_d, void *out_d,
                       void *stream);
 
+/* Moving average of the noise for EFFECTIVE_NOISE_INTEGRATION
+   (integrator/integrator_rk.py:115-128): out[r][i] = mean(z[r][i .. i+w)),
+   complex64 [R][n_t] in and out, w = (tau/2)/TAU samples (1..64), summed in
+   NumPy's order for complex64 means; 0 where the window passes the end.  The
+   RK4 stages then read out[] at the usual indices it + {0, w, w, 2w}. */
+int hops_noise_window_mean(const void *z_d, void *out_d, int R, int n_t, int w, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Ragged batches ("slots"): trajectories with different bases, and different
  * systems, advanced by one launch per stage.  Slot i has its own operator

@@ -90,6 +90,7 @@ def lib():
     L.hops_permute_c128.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp]
     L.hops_noise_embedding.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+    L.hops_noise_window_mean.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
     L.hops_profile_enable.argtypes = [ctypes.c_int]
     L.hops_set_option.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.hops_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
@@ -145,7 +146,7 @@ def check(rc):
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
             "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128",
-            "hops_noise_embedding", "hops_noise_filter",
+            "hops_noise_embedding", "hops_noise_filter", "hops_noise_window_mean",
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
             "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",

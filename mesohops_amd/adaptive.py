@@ -296,14 +296,44 @@ class AdaptiveTrajectoryEngine:
         self.z2_all = None if noise2.model == "ZERO" else device_noise([noise2], list(range(nl)), self.device,
                                                                        self.sp)[0]
         self.zmem = torch.zeros(self.info.n_hmodes + 1, dtype=torch.complex128, device=self.device)
+        # EFFECTIVE_NOISE_INTEGRATION: the RK stages read moving averages of
+        # the noise (integrator/integrator_rk.py:115-128); the error bound
+        # keeps the point samples (hops_trajectory.py:693-714)
+        self.effective = bool(setup.integ["EFFECTIVE_NOISE_INTEGRATION"])
+        self.z1_eff = self.z2_eff = None
+        self._eff_w = None
+
+    def _rows(self, z, act):
+        return None if z is None else z.index_select(0, act).unsqueeze(0).contiguous()
+
+    def _use_effective_noise(self, w):
+        if not self.effective or self._eff_w == w:
+            return
+        from .noise import window_mean
+        self.z1_eff = window_mean(self.z1_all, w, self.sp)
+        self.z2_eff = None if self.z2_all is None else window_mean(self.z2_all, w, self.sp)
+        self._eff_w = w
+        self._attach_noise()
+
+    def _attach_noise(self):
+        """Point the RK state at the (effective) noise rows of its basis."""
+        if not self.effective or self.z1_eff is None:
+            return
+        act = torch.as_tensor(self.tb.absL_active, dtype=torch.long, device=self.device)
+        st = self.st
+        st.z1 = self._rows(self.z1_eff, act)
+        st.bt.z1_d = st.z1.data_ptr()
+        if self.z2_eff is not None:
+            st.z2 = self._rows(self.z2_eff, act)
+            st.bt.z2_d = st.z2.data_ptr()
 
     # -- basis objects -------------------------------------------------------
     def _make(self, states, aux):
         tb = BasisTables(self.info, states, aux, self.setup.eom_kind)
         dop = DeviceOperator(tb, self.prec, self.device)
         act = torch.as_tensor(tb.absL_active, dtype=torch.long, device=self.device)
-        z1 = self.z1_all.index_select(0, act).unsqueeze(0).contiguous()
-        z2 = None if self.z2_all is None else self.z2_all.index_select(0, act).unsqueeze(0).contiguous()
+        z1 = self._rows(self.z1_all, act)
+        z2 = self._rows(self.z2_all, act)
         st = BatchState(dop, 1, self.setup.eom_kind, z1, z2, self.noise1.n_t, store_slots=0, lanes=1)
         st.zmem[: self.info.n_hmodes].copy_(self.zmem[: self.info.n_hmodes])
         self.tb, self.dop, self.st = tb, dop, st
@@ -312,6 +342,8 @@ class AdaptiveTrajectoryEngine:
         self.dop64 = dop if self.prec == _lib.HOPS_PREC_FP64 else DeviceOperator(tb, _lib.HOPS_PREC_FP64, self.device)
         self.st64 = st if self.prec == _lib.HOPS_PREC_FP64 else BatchState(
             self.dop64, 1, self.setup.eom_kind, z1, z2, self.noise1.n_t, store_slots=0, lanes=1)
+        self._z_point = (z1, z2)
+        self._attach_noise()
 
     def phi_host(self):
         return self.st.phi_host()[0]
@@ -326,8 +358,16 @@ class AdaptiveTrajectoryEngine:
             st.zmem[: self.info.n_hmodes].copy_(self.st.zmem[: self.info.n_hmodes])
         y = torch.from_numpy(np.ascontiguousarray(y_host, dtype=np.complex128)).to(self.device)
         d = torch.empty_like(y)
-        check(lib().hops_derivative(ctypes.byref(self.dop64.op), ctypes.byref(st.bt), ctypes.c_void_p(y.data_ptr()),
-                                    ctypes.c_void_p(d.data_ptr()), int(tidx), self.sp))
+        saved = (st.bt.z1_d, st.bt.z2_d)
+        z1, z2 = self._z_point  # point samples for the error bound
+        st.bt.z1_d = z1.data_ptr()
+        st.bt.z2_d = None if z2 is None else z2.data_ptr()
+        try:
+            check(lib().hops_derivative(ctypes.byref(self.dop64.op), ctypes.byref(st.bt),
+                                        ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(d.data_ptr()), int(tidx),
+                                        self.sp))
+        finally:
+            st.bt.z1_d, st.bt.z2_d = saved
         return d.cpu().numpy()
 
     def define_basis(self, phi, dt, tidx):
@@ -390,6 +430,7 @@ class AdaptiveTrajectoryEngine:
         self.update_basis(phi, st, ax)
 
     def _rk(self, it, tau, ns):
+        self._use_effective_noise(ns)
         self.st.propagate(it, 1, tau, ns, 0)
 
     def _same_basis(self, states, aux):

@@ -208,6 +208,47 @@ __global__ __launch_bounds__(kThreads) void k_filter(const float2 *__restrict__ 
   }
 }
 
+// Sum of n float32 values a[0], a[stride], ... in NumPy's pairwise order for
+// complex64 reductions (numpy/_core/src/umath/loops_utils.h.src): fewer than
+// 4 values one after another starting from -0; 4..64 values in four running
+// sums combined as (r0 + r1) + (r2 + r3), then the remainder one by one.
+__device__ __forceinline__ float np_pairwise_sum(const float *a, int stride, int n) {
+  if (n < 4) {
+    float r = -0.0f;
+    for (int i = 0; i < n; ++i) r += a[(long)i * stride];
+    return r;
+  }
+  float r0 = a[0], r1 = a[stride], r2 = a[2L * stride], r3 = a[3L * stride];
+  int i = 4;
+  for (; i < n - (n % 4); i += 4) {
+    r0 += a[(long)i * stride];
+    r1 += a[(long)(i + 1) * stride];
+    r2 += a[(long)(i + 2) * stride];
+    r3 += a[(long)(i + 3) * stride];
+  }
+  float res = (r0 + r1) + (r2 + r3);
+  for (; i < n; ++i) res += a[(long)i * stride];
+  return res;
+}
+
+// out[row][i] = mean of z[row][i .. i+w) (complex64, NumPy's summation order
+// and a float32 division by w); 0 where the window passes the end
+__global__ __launch_bounds__(kThreads) void k_window_mean(const float2 *__restrict__ z, float2 *__restrict__ out,
+                                                         int n_t, int w) {
+#pragma clang fp contract(off)
+  const int row = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_t) return;
+  const float2 *zr = z + (long)row * n_t;
+  float2 v = make_float2(0.f, 0.f);
+  if (i + w <= n_t) {
+    const float *base = reinterpret_cast<const float *>(zr + i);
+    v.x = np_pairwise_sum(base, 2, w) / (float)w;
+    v.y = np_pairwise_sum(base + 1, 2, w) / (float)w;
+  }
+  out[(long)row * n_t + i] = v;
+}
+
 struct Plan {
   int N, M, logm;
   double2 *Bhat;  // library-owned, one per length, freed at process exit
@@ -325,5 +366,16 @@ extern "C" int hops_noise_filter(const void *z_d, int R, int n_t, const double *
   k_filter<<<R, kThreads, lds, st>>>((const float2 *)z_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d,
                                       row_lop_d, (float2 *)out_d, gs);
   if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_filter launch failed");
+  return 0;
+}
+
+extern "C" int hops_noise_window_mean(const void *z_d, void *out_d, int R, int n_t, int w, void *stream) {
+  if (R <= 0) return 0;
+  if (!z_d || !out_d) return hops_internal_set_error("hops_noise_window_mean: NULL pointer");
+  if (w < 1 || w > 64) return hops_internal_set_error("noise averaging window must hold 1..64 samples");
+  if (n_t < 1 || R > 65535) return hops_internal_set_error("hops_noise_window_mean: bad shape");
+  dim3 grid((unsigned)((n_t + kThreads - 1) / kThreads), (unsigned)R);
+  k_window_mean<<<grid, kThreads, 0, (hipStream_t)stream>>>((const float2 *)z_d, (float2 *)out_d, n_t, w);
+  if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_window_mean launch failed");
   return 0;
 }

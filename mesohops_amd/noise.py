@@ -171,3 +171,14 @@ def device_noise(noises, rows, device, stream_ptr, raws=None):
     if nz.param["FLAG_REAL"]:
         torch.view_as_real(out)[..., 1] = 0
     return out
+
+
+def window_mean(z, w, stream_ptr):
+    """Moving average over w samples of device noise z (complex64 [..., n_t]):
+    the noise of EFFECTIVE_NOISE_INTEGRATION (integrator/integrator_rk.py:115-128)."""
+    out = torch.empty_like(z)
+    n_t = z.shape[-1]
+    R = z.numel() // n_t if n_t else 0
+    check(lib().hops_noise_window_mean(ctypes.c_void_p(z.data_ptr()), ctypes.c_void_p(out.data_ptr()), R, n_t,
+                                       int(w), stream_ptr))
+    return out

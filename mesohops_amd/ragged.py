@@ -304,6 +304,9 @@ class AdaptiveBatch:
                             hierarchy_param, integration_param)
         if self.setup.integ["STATIC_BASIS"] is not None:
             raise UnsupportedRequest("STATIC_BASIS", "AdaptiveBatch")
+        if self.setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
+            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION in a ragged batch (HopsTrajectory supports it)",
+                                     "AdaptiveBatch")
         if noise_params is None:
             if seeds is None:
                 raise TrajectoryError("AdaptiveBatch needs seeds or noise_params")

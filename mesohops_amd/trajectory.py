@@ -156,8 +156,6 @@ class _Setup:
                                      "integration_param in the HopsTrajectory initialization")
         if self.integ["INTEGRATOR"] != "RUNGE_KUTTA":
             raise UnsupportedRequest("Integrator of type " + str(self.integ["INTEGRATOR"]), "HopsTrajectory")
-        if self.integ["EFFECTIVE_NOISE_INTEGRATION"]:
-            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION", "HopsTrajectory")
         self.info = SystemInfo(system_param)
         self.noise_param = dict(noise_param or {})
         if noise2_param is None:
@@ -236,6 +234,8 @@ class HopsBatch:
         z2 = None
         if self.noise2[0].model != "ZERO":
             z2 = device_noise(self.noise2, tb.absL_active, self.device, sp)
+        self._z_raw = (z1, z2)
+        self._eff_w = None
         self.state = BatchState(self.dop, self.B, self.setup.eom_kind, z1, z2, self.noise1[0].n_t,
                                 store_slots=0, lanes=self.lanes)
         psi0 = np.asarray(psi_0, dtype=np.complex128)
@@ -274,6 +274,8 @@ class HopsBatch:
         n_steps, it0, ns = self._plan(t_advance, tau)
         store = self.store_psi if store is None else store
         st = self.state
+        if self.setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
+            self._use_effective_noise(ns)
         if store and (st.psi_store is None or st.store_slots < n_steps):
             st.store_slots = n_steps
             st.psi_store = torch.zeros(self.B * n_steps * self.tables.n_state, dtype=torch.complex128,
@@ -288,6 +290,21 @@ class HopsBatch:
                 self.psi_traj[b].extend(list(psi[b]))
             self.t_axis.extend([float(t) for t in t_axis])
         self.t = float(t_axis[-1]) if n_steps else self.t
+
+    def _use_effective_noise(self, w):
+        """Stage noise = moving averages over w samples (integrator/integrator_rk.py:115-128)."""
+        if self._eff_w == w:
+            return
+        from .noise import window_mean
+        sp = _stream_ptr(self.device)
+        z1, z2 = self._z_raw
+        st = self.state
+        st.z1 = window_mean(z1, w, sp)
+        st.bt.z1_d = st.z1.data_ptr()
+        if z2 is not None:
+            st.z2 = window_mean(z2, w, sp)
+            st.bt.z2_d = st.z2.data_ptr()
+        self._eff_w = w
 
     def rewind(self, psi_0):
         """Reset every trajectory to t = 0 with the initial state psi_0 and

@@ -576,6 +576,16 @@ class Noise:
         return out
 
 
+def effective_noise(noise, t, tau, lops):
+    """Moving averages of the noise over the three RK sub-intervals
+    [t, t+tau/2), [t+tau/2, t+tau), [t+tau, t+3tau/2)
+    (integrator/integrator_rk.py:115-128)."""
+    r = round(tau / noise.p["TAU"])
+    raw = noise.get_noise([t + (i / r) * tau for i in range(round(r * 1.5))], lops)
+    return np.array([np.mean(raw[:, :round(r / 2)], axis=1), np.mean(raw[:, round(r / 2):r], axis=1),
+                     np.mean(raw[:, r:], axis=1)]).T
+
+
 # ---------------------------------------------------------------------------
 # Trajectory driver, non-adaptive (trajectory/hops_trajectory.py:253-538,616-634)
 # ---------------------------------------------------------------------------
@@ -583,9 +593,10 @@ class Trajectory:
     """Plain (non-adaptive) HOPS trajectory restated from HopsTrajectory."""
 
     def __init__(self, system_param, eom_param=None, noise_param=None, noise2_param=None,
-                 hierarchy_param=None):
+                 hierarchy_param=None, effective_noise_integration=False):
         eom_param = dict(eom_param or {})
         hierarchy_param = dict(hierarchy_param or {})
+        self.effective = bool(effective_noise_integration)
         self.eom_type = eom_param.get("EQUATION_OF_MOTION", "NORMALIZED NONLINEAR")
         self.maxhier = min(int(hierarchy_param.get("MAXHIER", 3)), 255)
         self.sysd = derive_system(system_param)
@@ -620,6 +631,8 @@ class Trajectory:
 
     def noise_block(self, t, tau):
         lops = list(self.basis.absL)
+        if self.effective:
+            return (effective_noise(self.noise1, t, tau, lops), effective_noise(self.noise2, t, tau, lops))
         z1 = self.noise1.get_noise([t, t + tau * 0.5, t + tau], lops)
         z2 = self.noise2.get_noise([t, t + tau * 0.5, t + tau], lops)
         return z1, z2

@@ -310,3 +310,61 @@ def test_fp64_second_noise(eom, flag_real):
     err = np.max(np.abs(np.array(b.psi_traj[0]) - ref)) / max(1.0, np.max(np.abs(ref)))
     print("fp64 noise2 %s flag_real=%s max |dpsi| = %.3e" % (eom, flag_real, err))
     assert err < FP64_TOL
+
+
+@pytest.mark.parametrize("ratio", [2, 6, 10])
+@pytest.mark.parametrize("eom", ["NORMALIZED NONLINEAR", "LINEAR"])
+def test_fp64_effective_noise_integration(eom, ratio):
+    """EFFECTIVE_NOISE_INTEGRATION (integrator/integrator_rk.py:115-128): the
+    device moving average reproduces NumPy's complex64 means (sequential below
+    4 samples, blocked pairwise from 4), so psi(t) matches the oracle to 1e-10;
+    ratio = tau / TAU gives windows of 1, 3 and 5 samples."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.three_site(False)
+    tau_n = 0.25
+    dt = ratio * tau_n
+    tlen = 40.0
+    z = _oracle_noise(sysp, 9, tlen, tau_n)
+    nparam = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau_n}
+    ref = O.Trajectory(sysp, {"EQUATION_OF_MOTION": eom}, nparam, hierarchy_param={"MAXHIER": 3},
+                       effective_noise_integration=True)
+    psi0 = np.array([0, 1, 0], complex)
+    ref.initialize(psi0)
+    t_max = dt * int(20.0 / dt)
+    ref.propagate(t_max, dt)
+    tr = HopsTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, dict(nparam), hierarchy_param={"MAXHIER": 3},
+                        integration_param={"EFFECTIVE_NOISE_INTEGRATION": True})
+    tr.initialize(psi0)
+    tr.propagate(t_max, dt)
+    got = np.array(tr.storage["psi_traj"])
+    err = np.max(np.abs(got - np.array(ref.psi_traj))) / max(1.0, np.max(np.abs(ref.psi_traj)))
+    print("effective noise", eom, "ratio", ratio, "max |dpsi| = %.3e" % err)
+    assert err < FP64_TOL
+
+
+def test_adaptive_effective_noise_integration():
+    """Adaptive trajectory with EFFECTIVE_NOISE_INTEGRATION: RK stages on the
+    averaged noise, the error bound on point samples, as the reference."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.chain(6)
+    tau_n = 0.25
+    z = _oracle_noise(sysp, 4, 60.0, tau_n)
+    nparam = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": 60.0, "TAU": tau_n}
+    psi0 = np.zeros(6, complex)
+    psi0[3] = 1.0
+    ref = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}, nparam,
+                               hierarchy_param={"MAXHIER": 3}, delta_a=2e-3, delta_s=2e-3,
+                               effective_noise_integration=True)
+    ref.initialize(psi0)
+    ref.propagate(30.0, 1.5)
+    tr = HopsTrajectory(sysp, {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}, dict(nparam),
+                        hierarchy_param={"MAXHIER": 3}, integration_param={"EFFECTIVE_NOISE_INTEGRATION": True})
+    tr.make_adaptive(2e-3, 2e-3)
+    tr.initialize(psi0)
+    tr.propagate(30.0, 1.5)
+    assert [list(s) for s in tr.storage["state_list"]] == [list(s) for s in ref.state_traj]
+    err = np.max(np.abs(tr.storage["psi_traj"] - np.array(ref.psi_traj)))
+    print("adaptive effective noise max |dpsi| = %.3e" % err)
+    assert err < FP64_TOL

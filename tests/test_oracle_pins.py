@@ -299,3 +299,21 @@ def test_adaptive_derivative_error_below_delta(mode):
         errs.append(np.linalg.norm(d_full - _embed(b, full, d_ad)))
     assert b.n_hier < full.n_hier or b.n_state < full.n_state
     assert max(errs) <= max(da, ds), max(errs)
+
+
+def test_effective_noise_matches_reference_hand_values():
+    """tests/test_integrator_rk.py:45-73: PRE_CALCULATED noise arange(-10,
+    10.5, 0.5) (and its negative), TAU 0.25, step 1.5 at t = 5: the effective
+    noise is the mean of samples [0:3], [3:6], [6:9] of the window from t."""
+    sysp = dict(HAMILTONIAN=np.array([[0, 10.0], [10.0, 0]]), GW_SYSBATH=[[10.0, 10.0], [5.0, 5.0]],
+                L_HIER=[np.diag([1.0, 0.0]), np.diag([0.0, 1.0])], L_NOISE1=[np.diag([1.0, 0.0]), np.diag([0.0, 1.0])],
+                ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=[[10.0, 10.0], [5.0, 5.0]])
+    seed = np.array([np.arange(-10, 10.5, 0.5), -1 * np.arange(-10, 10.5, 0.5)])
+    nz = O.Noise({"SEED": seed, "MODEL": "PRE_CALCULATED", "TLEN": 10.0, "TAU": 0.25}, O.derive_system(sysp), 1)
+    got = O.effective_noise(nz, 5.0, 1.5, [0, 1])
+    known = seed[:, 5 * 4:8 * 4]
+    want = np.array([np.mean(known[:, :3], axis=1), np.mean(known[:, 3:6], axis=1),
+                     np.mean(known[:, 6:9], axis=1)]).T
+    assert np.allclose(got, want)
+    # the point samples of the plain integrator are columns 0, 3, 6 of the same window
+    assert np.allclose(nz.get_noise([5.0, 5.75, 6.5], [0, 1]), known[:, [0, 3, 6]])
