@@ -56,8 +56,9 @@ class HopsNoise:
         for key in ("INTERPOLATE",):
             if self.param[key]:
                 raise UnsupportedRequest("noise {} = True".format(key), "HopsNoise")
-        if self.param["NOISE_WINDOW"] is not None:
-            raise UnsupportedRequest("NOISE_WINDOW", "HopsNoise")
+        # NOISE_WINDOW only bounds the slice of the noise the reference keeps in
+        # a working copy (noise/hops_noise.py:385-433); the values are those of
+        # the full noise, which stays resident in HBM here, so it changes nothing
         self.t_axis = noise_time_axis(self.param["TLEN"], self.param["TAU"])
         self.n_t = len(self.t_axis)
         key = "L_NOISE" + tag
