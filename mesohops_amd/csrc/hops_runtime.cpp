@@ -255,18 +255,24 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   std::sort(states.begin(), states.end());
   states.erase(std::unique(states.begin(), states.end()), states.end());
   {
-    // reference order with the depth of every auxiliary computed once
-    std::vector<std::pair<int, int>> key(aux.size());
-    for (size_t i = 0; i < aux.size(); ++i) key[i] = {aux_depth(aux[i]), (int)i};
-    std::sort(key.begin(), key.end(), [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
-      if (x.first != y.first) return x.first < y.first;
-      return aux_less(aux[x.second], aux[y.second]);
-    });
-    std::vector<Aux> sorted;
-    sorted.reserve(aux.size());
-    for (auto &k : key)
-      if (sorted.empty() || sorted.back() != aux[k.second]) sorted.push_back(std::move(aux[k.second]));
-    aux.swap(sorted);
+    // reference order.  A new basis arrives as the kept auxiliaries of the
+    // old one (already in order) followed by a few new ones: sort the tail,
+    // merge, drop duplicates.
+    auto mid = std::is_sorted_until(aux.begin(), aux.end(), aux_less);
+    if (mid != aux.end()) {
+      std::vector<std::pair<int, int>> key(aux.end() - mid);
+      for (size_t i = 0; i < key.size(); ++i) key[i] = {aux_depth(mid[i]), (int)i};
+      std::sort(key.begin(), key.end(), [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+        if (x.first != y.first) return x.first < y.first;
+        return aux_less(mid[x.second], mid[y.second]);
+      });
+      std::vector<Aux> tail;
+      tail.reserve(key.size());
+      for (auto &k : key) tail.push_back(std::move(mid[k.second]));
+      std::move(tail.begin(), tail.end(), mid);
+      std::inplace_merge(aux.begin(), mid, aux.end(), aux_less);
+    }
+    aux.erase(std::unique(aux.begin(), aux.end()), aux.end());
   }
   if (aux.empty() || !aux[0].empty()) return rt_fail("Zero Vector not contained in list_aux!");
   if (states.empty()) return rt_fail("empty state basis");
@@ -1624,7 +1630,7 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
     b.pool->parallel_for((int)ch.size(), [&](int i) {
       Slot &sl = b.slots[ch[i]];
       const int j = idx_of[ch[i]];
-      rcs[i] = build_basis(*sl.sys, sl.prm.eom, tgt_states[j], tgt_aux[j], sl.Bnew);
+      rcs[i] = build_basis(*sl.sys, sl.prm.eom, std::move(tgt_states[j]), std::move(tgt_aux[j]), sl.Bnew);
     });
   }
   for (int rc : rcs)
