@@ -116,6 +116,10 @@ int hops_rt_debug_select_states(void *basis, const double *e_state);
 int hops_rt_debug_result(void *basis, int32_t *n_states, int32_t *states_out, int32_t *n_aux, int32_t *aux_ptr,
                          int32_t *aux_mode, int32_t *aux_val, long long cap);
 int hops_rt_debug_free(void *basis);
+/* error_threshold(sort(v), max_err, offset) of basis/basis_functions.py:8-41
+   as the selection computes it (only the needed prefix of v is sorted), and
+   the sum of the values <= it. */
+double hops_rt_debug_threshold(const double *v, long long n, double max_err, double offset, double *below);
 
 #ifdef __cplusplus
 }

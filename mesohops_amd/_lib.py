@@ -133,6 +133,8 @@ def lib():
     L.hops_rt_debug_select_states.argtypes = [_vp, _vp]
     L.hops_rt_debug_result.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ll]
     L.hops_rt_debug_free.argtypes = [_vp]
+    L.hops_rt_debug_threshold.restype = ctypes.c_double
+    L.hops_rt_debug_threshold.argtypes = [_vp, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _vp]
     if L.hops_version() != ABI_VERSION:
         raise EngineError("libhops_mi355x.so ABI %d != %d; rebuild" % (L.hops_version(), ABI_VERSION))
     _lib = L
@@ -153,7 +155,7 @@ EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_prof
             "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
             "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
             "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",
-            "hops_rt_debug_result", "hops_rt_debug_free"]
+            "hops_rt_debug_result", "hops_rt_debug_free", "hops_rt_debug_threshold"]
 
 
 def set_option(name, value):

@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <malloc.h>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -396,49 +397,48 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
     }
   }
   // connections (hops_hierarchy.py:470-523): j = i + e_m, as flat lists
-  // sorted by (aux, relative mode)
-  struct Conn {
-    int32_t key, lm, other, v;
-  };
-  std::vector<Conn> ups, dns;
+  // sorted by (aux, relative mode).  The down list comes out in that order
+  // (j ascending, the entries of an auxiliary by mode); the up list is
+  // placed by a counting sort on i, and each aux's few entries are then
+  // ordered by mode.
   {
-    ups.reserve((size_t)nh * 4);
-    dns.reserve((size_t)nh * 4);
+    std::vector<int32_t> up_i, up_j, up_m;
+    up_i.reserve((size_t)nh * 4);
+    up_j.reserve((size_t)nh * 4);
+    up_m.reserve((size_t)nh * 4);
+    B.dn_ptr.assign(nh + 1, 0);
     for (int j = 0; j < nh; ++j) {
       const AKey kj{B.aux_hash[j], B.aux_hash2[j]};
       for (char32_t e : aux[j]) {
         const int m = aux_mode(e), v = aux_val(e);
         const int i = B.index.find(key_step(S, kj, m, v, v - 1));
         if (i < 0) continue;
-        ups.push_back({i, B.rel_mode[m], j, 0});
-        dns.push_back({j, B.rel_mode[m], i, v});
+        up_i.push_back(i);
+        up_j.push_back(j);
+        up_m.push_back(B.rel_mode[m]);
+        B.dn_aux.push_back(i);
+        B.dn_mode.push_back(B.rel_mode[m]);
+        B.dn_val.push_back(v);
       }
+      B.dn_ptr[j + 1] = (int)B.dn_aux.size();
     }
-  }
-  auto by_key = [](const Conn &x, const Conn &y) { return x.key != y.key ? x.key < y.key : x.lm < y.lm; };
-  std::sort(ups.begin(), ups.end(), by_key);
-  std::sort(dns.begin(), dns.end(), by_key);
-  B.up_ptr.assign(nh + 1, 0);
-  B.dn_ptr.assign(nh + 1, 0);
-  for (const Conn &c : ups) B.up_ptr[c.key + 1]++;
-  for (const Conn &c : dns) B.dn_ptr[c.key + 1]++;
-  for (int i = 0; i < nh; ++i) {
-    B.up_ptr[i + 1] += B.up_ptr[i];
-    B.dn_ptr[i + 1] += B.dn_ptr[i];
-  }
-  B.up_aux.reserve(ups.size());
-  B.up_mode.reserve(ups.size());
-  for (const Conn &c : ups) {
-    B.up_aux.push_back(c.other);
-    B.up_mode.push_back(c.lm);
-  }
-  B.dn_aux.reserve(dns.size());
-  B.dn_mode.reserve(dns.size());
-  B.dn_val.reserve(dns.size());
-  for (const Conn &c : dns) {
-    B.dn_aux.push_back(c.other);
-    B.dn_mode.push_back(c.lm);
-    B.dn_val.push_back(c.v);
+    B.up_ptr.assign(nh + 1, 0);
+    for (int32_t i : up_i) B.up_ptr[i + 1]++;
+    for (int i = 0; i < nh; ++i) B.up_ptr[i + 1] += B.up_ptr[i];
+    B.up_aux.resize(up_i.size());
+    B.up_mode.resize(up_i.size());
+    std::vector<int32_t> fill(B.up_ptr.begin(), B.up_ptr.end() - 1);
+    for (size_t q = 0; q < up_i.size(); ++q) {
+      const int32_t at = fill[up_i[q]]++;
+      B.up_aux[at] = up_j[q];
+      B.up_mode[at] = up_m[q];
+    }
+    for (int i = 0; i < nh; ++i)
+      for (int a = B.up_ptr[i] + 1; a < B.up_ptr[i + 1]; ++a)
+        for (int c = a; c > B.up_ptr[i] && B.up_mode[c - 1] > B.up_mode[c]; --c) {
+          std::swap(B.up_mode[c - 1], B.up_mode[c]);
+          std::swap(B.up_aux[c - 1], B.up_aux[c]);
+        }
   }
   // per relative L: row and column CSR tables with running bases
   B.lr_ptr.assign((size_t)nl * (n + 1), 0);
@@ -611,10 +611,10 @@ struct Params {
 double error_threshold(const std::vector<double> &sorted, double max_err, double offset = 0.0) {
   // basis/basis_functions.py:8-41
   if (sorted.empty()) return 0.0;
-  double cum = offset;
+  double cum = 0;  // cumsum(sorted) + offset, summed in that order
   for (size_t i = 0; i < sorted.size(); ++i) {
     cum += sorted[i];
-    if (cum >= max_err) return i > 0 ? sorted[i - 1] : 0.0;
+    if (cum + offset >= max_err) return i > 0 ? sorted[i - 1] : 0.0;
   }
   return sorted[0] + offset < max_err ? INFINITY : 0.0;
 }
@@ -646,11 +646,68 @@ void sort_nonneg(std::vector<double> &v) {
 }
 
 // error_threshold(sort(v), max_err, offset) and the sum of the values <= the
-// threshold (v: non-negative error terms)
-double partial_threshold(std::vector<double> &v, double max_err, double offset, double *below) {
-  std::vector<double> w(v);
-  sort_nonneg(w);
-  const double thr = error_threshold(w, max_err, offset);
+// threshold (v: non-negative error terms).  Only a prefix of the sorted order
+// is sorted: the values are binned by binary exponent (a bin holds a range of
+// values, and the bins are ordered), the bins are walked up to the first one
+// where the running sum reaches max_err, and the values of those bins, which
+// are exactly the smallest values of v, are sorted and scanned as
+// error_threshold scans them.  The binned sum is summed in another order than
+// the scan, so the prefix is taken with a margin and widened to the whole of v
+// if the scan does not reach max_err inside it.
+double partial_threshold(const std::vector<double> &v, double max_err, double offset, double *below) {
+  double thr;
+  if (v.empty()) {
+    thr = 0.0;
+  } else {
+    constexpr int kBins = 2048;
+    uint32_t cnt[kBins] = {};
+    double sum[kBins] = {};
+    for (double x : v) {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      const int e = (int)(u >> 52) & (kBins - 1);
+      cnt[e]++;
+      sum[e] += x;
+    }
+    double run = offset;
+    int last = kBins - 1;
+    for (int e = 0; e < kBins; ++e) {
+      run += sum[e];
+      if (cnt[e] && run >= max_err * (1.0 - 1e-6)) {
+        last = e;
+        break;
+      }
+    }
+    std::vector<double> w;
+    if (last == kBins - 1) {
+      w = v;
+    } else {
+      size_t m = 0;
+      for (int e = 0; e <= last; ++e) m += cnt[e];
+      w.reserve(m);
+      for (double x : v) {
+        uint64_t u;
+        std::memcpy(&u, &x, 8);
+        if ((int)(u >> 52) <= last) w.push_back(x);
+      }
+    }
+    sort_nonneg(w);
+    double cum = 0;
+    size_t i = 0;
+    for (; i < w.size(); ++i) {
+      cum += w[i];
+      if (cum + offset >= max_err) break;
+    }
+    if (i < w.size()) {
+      thr = i > 0 ? w[i - 1] : 0.0;
+    } else if (w.size() == v.size()) {
+      thr = INFINITY;  // the sum of every value stays below max_err
+    } else {
+      std::vector<double> all(v);
+      sort_nonneg(all);
+      thr = error_threshold(all, max_err, offset);
+    }
+  }
   double sb = 0;
   for (double x : v)
     if (x <= thr) sb += x;
@@ -684,15 +741,16 @@ std::vector<int> list_sc(const System &S, const Basis &B) {
 }
 
 void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
-  // hops_fluxfilters.py:354-397
-  F.assign((size_t)B.nm * B.nh, 1);
+  // hops_fluxfilters.py:354-397; F[k * nm + lm]
+  F.assign((size_t)B.nh * B.nm, 1);
   if (P.markov.empty()) return;
-  for (int lm = 0; lm < B.nm; ++lm) {
+  const int nm = B.nm;
+  for (int lm = 0; lm < nm; ++lm) {
     if (!P.markov[B.modes[lm]]) continue;
-    for (int k = 1; k < B.nh; ++k) F[(size_t)lm * B.nh + k] = 0;
+    for (int k = 1; k < B.nh; ++k) F[(size_t)k * nm + lm] = 0;
     const int j = B.index.find(key_step(*B.sys, AKey(), B.modes[lm], 0, 1));
     if (j >= 0)
-      for (int m2 = 0; m2 < B.nm; ++m2) F[(size_t)m2 * B.nh + j] = 0;
+      for (int m2 = 0; m2 < nm; ++m2) F[(size_t)j * nm + m2] = 0;
   }
 }
 
@@ -750,55 +808,62 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
       err[k] += acc;
     }
   }
-  // flux up / down (basis_functions_adaptive.py:165-390)
-  std::vector<uint8_t> kval((size_t)nm * nh, 0);  // K2_aux_by_mode
+  // flux up / down (basis_functions_adaptive.py:165-390); per-(aux, mode)
+  // arrays are aux-major, [k * nm + lm]
+  std::vector<uint8_t> kval((size_t)nh * nm, 0);  // K2_aux_by_mode
   for (int k = 0; k < nh; ++k)
-    for (int q = B.aux_ptr[k]; q < B.aux_ptr[k + 1]; ++q) kval[(size_t)B.rel_mode[B.aux_mode_t[q]] * nh + k] = B.aux_val_t[q];
-  std::vector<double> E_up((size_t)nm * nh, 0.0), E_dn((size_t)nm * nh, 0.0);
+    for (int q = B.aux_ptr[k]; q < B.aux_ptr[k + 1]; ++q) kval[(size_t)k * nm + B.rel_mode[B.aux_mode_t[q]]] = B.aux_val_t[q];
+  std::vector<double> E_up((size_t)nh * nm, 0.0), E_dn((size_t)nh * nm, 0.0);
   std::vector<uint8_t> F_up;
   markov_up(P, B, F_up);
-  std::vector<double> colsum(nh, 0.0);
-  for (int k = 0; k < nh; ++k) {
-    const double *c2 = &out.absC2[(size_t)k * n];
-    double a = 0;
-    for (int s2 = 0; s2 < n; ++s2) a += c2[s2];
-    colsum[k] = a;
-  }
+  std::vector<double> aw(nm), gw2(nm), nX(nm);
+  // the diagonal L entries of each mode as CSR: state, |M2|^2, |M2 - X|^2
+  std::vector<int> m2p(nm + 1, 0), m2s;
+  std::vector<double> m2u, m2d;
   for (int lm = 0; lm < nm; ++lm) {
     const int m = B.modes[lm];
-    const double aw = std::abs(S.W[m]);
-    const double gw2 = std::norm(S.G[m] / S.W[m]);
-    const cplx X = out.X2[lm];
-    const double nX = std::norm(X);
-    const auto &M2 = out.M2[lm];
-    double *eu = &E_up[(size_t)lm * nh], *ed = &E_dn[(size_t)lm * nh];
-    const uint8_t *kv = &kval[(size_t)lm * nh];
-    for (int k = 0; k < nh; ++k) {
-      const double *c2 = &out.absC2[(size_t)k * n];
+    aw[lm] = std::abs(S.W[m]);
+    gw2[lm] = std::norm(S.G[m] / S.W[m]);
+    nX[lm] = std::norm(out.X2[lm]);
+    for (auto &e : out.M2[lm]) {
+      m2s.push_back(e.first);
+      m2u.push_back(std::norm(e.second));
+      m2d.push_back(std::norm(e.second - out.X2[lm]));
+    }
+    m2p[lm + 1] = (int)m2s.size();
+  }
+  for (int k = 0; k < nh; ++k) {
+    const double *c2 = &out.absC2[(size_t)k * n];
+    double colsum = 0;
+    for (int s2 = 0; s2 < n; ++s2) colsum += c2[s2];
+    double *eu = &E_up[(size_t)k * nm], *ed = &E_dn[(size_t)k * nm];
+    const uint8_t *kv = &kval[(size_t)k * nm];
+    for (int lm = 0; lm < nm; ++lm) {
       // sum_s |M2[m,s]|^2 |C2[s,k]|^2 and sum_s |M2[m,s] - X|^2 |C2[s,k]|^2;
       // the states without an M2 entry contribute |X|^2 |C2[s,k]|^2
       double p_up = 0, p_dn = 0, on = 0;
-      for (auto &e : M2) {
-        const double c = c2[e.first];
-        p_up += std::norm(e.second) * c;
-        p_dn += std::norm(e.second - X) * c;
+      for (int q = m2p[lm]; q < m2p[lm + 1]; ++q) {
+        const double c = c2[m2s[q]];
+        p_up += m2u[q] * c;
+        p_dn += m2d[q] * c;
         on += c;
       }
-      if (nX != 0) p_dn += nX * (colsum[k] - on);
-      const double kw = aw * (1.0 + kv[k]);
-      eu[k] = (kw * kw * p_up) / h2;
-      ed[k] = (gw2 * p_dn) / h2;
+      if (nX[lm] != 0) p_dn += nX[lm] * (colsum - on);
+      const double kw = aw[lm] * (1.0 + kv[lm]);
+      eu[lm] = (kw * kw * p_up) / h2;
+      ed[lm] = (gw2[lm] * p_dn) / h2;
     }
   }
   for (int k = 0; k < nh; ++k) {
     const bool top = B.depth[k] == P.maxhier;
+    const size_t o = (size_t)k * nm;
     double acc = 0;
     for (int lm = 0; lm < nm; ++lm)
-      if (!top && F_up[(size_t)lm * nh + k]) acc += E_up[(size_t)lm * nh + k];
+      if (!top && F_up[o + lm]) acc += E_up[o + lm];
     err[k] += acc;
     acc = 0;
     for (int lm = 0; lm < nm; ++lm)
-      if (kval[(size_t)lm * nh + k] > 0) acc += E_dn[(size_t)lm * nh + k];
+      if (kval[o + lm] > 0) acc += E_dn[o + lm];
     err[k] += acc;
   }
   // stable auxiliaries (hops_basis.py:1000-1028)
@@ -818,62 +883,93 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   double unstable = 0;
   for (double v : un) unstable += v;
   const double dbound = dsq - unstable;
-  // boundary auxiliaries (hops_basis.py:667-793)
-  std::vector<uint8_t> has_up((size_t)nm * nh, 0), has_dn((size_t)nm * nh, 0);
-  for (int k = 0; k < nh; ++k) {
-    for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) has_up[(size_t)B.up_mode[u] * nh + k] = 1;
-    for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) has_dn[(size_t)B.dn_mode[u] * nh + k] = 1;
+  // boundary auxiliaries (hops_basis.py:667-793): flux out of the basis, up
+  // (depth < maxhier, k+e_m not in the basis) and down (k-e_m not in the
+  // basis), in the reference's insertion order: stable column, flux up by
+  // mode, then flux down by mode.  cand bit 0: up candidate, bit 1: down.
+  std::vector<uint8_t> cand((size_t)nh * nm, 0);
+  for (int k : out.stable_idx) {
+    const size_t o = (size_t)k * nm;
+    if (B.depth[k] < P.maxhier)
+      for (int lm = 0; lm < nm; ++lm) cand[o + lm] = (F_up[o + lm] && E_up[o + lm] != 0) ? 1 : 0;
+    for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) cand[o + B.up_mode[u]] &= (uint8_t)~1u;
+    for (int lm = 0; lm < nm; ++lm)
+      if (kval[o + lm] > 0 && E_dn[o + lm] != 0) cand[o + lm] |= 2;
+    for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) cand[o + B.dn_mode[u]] &= (uint8_t)~2u;
   }
-  // flux out of the basis: up (depth < maxhier, k+e_m not in the basis) and
-  // down (k-e_m not in the basis), in the reference's insertion order:
-  // stable column, flux up by mode, then flux down by mode
-  auto up_ok = [&](int k, int lm) {
-    const size_t q = (size_t)lm * nh + k;
-    return B.depth[k] < P.maxhier && !has_up[q] && F_up[q] && E_up[q] != 0;
-  };
-  auto dn_ok = [&](int k, int lm) {
-    const size_t q = (size_t)lm * nh + k;
-    return kval[q] > 0 && !has_dn[q] && E_dn[q] != 0;
-  };
   std::vector<double> nz;
-  for (int k : out.stable_idx)
+  nz.reserve((size_t)out.stable_idx.size() * 4);
+  for (int k : out.stable_idx) {
+    const size_t o = (size_t)k * nm;
     for (int lm = 0; lm < nm; ++lm) {
-      if (up_ok(k, lm)) nz.push_back(E_up[(size_t)lm * nh + k]);
-      if (dn_ok(k, lm)) nz.push_back(E_dn[(size_t)lm * nh + k]);
+      const uint8_t c = cand[o + lm];
+      if (c & 1) nz.push_back(E_up[o + lm]);
+      if (c & 2) nz.push_back(E_dn[o + lm]);
     }
+  }
   double discarded = 0;
   const double thr1 = partial_threshold(nz, dbound * P.f_discard * P.f_discard, 0.0, &discarded);
-  // accumulate the survivors by target auxiliary (Zobrist key)
-  struct Ent {
+  // accumulate the survivors by target auxiliary (Zobrist key), in the
+  // order above, in an open-addressing table that doubles at half load; the
+  // slots of the next candidates are prefetched
+  struct Cand {
     AKey key;
-    int32_t id;
+    int k, lm, dir;
+    double v;
   };
-  size_t cap = 16;
-  while (cap < nz.size() * 2) cap <<= 1;
-  std::vector<Ent> tab(cap, Ent{AKey(), -1});
+  std::vector<Cand> cs;
+  cs.reserve(nz.size());
+  for (int k : out.stable_idx) {
+    const size_t o = (size_t)k * nm;
+    const AKey kk{B.aux_hash[k], B.aux_hash2[k]};
+    for (int d = 0; d < 2; ++d) {
+      const double *E = d == 0 ? &E_up[o] : &E_dn[o];
+      const int dir = d == 0 ? 1 : -1;
+      for (int lm = 0; lm < nm; ++lm)
+        if ((cand[o + lm] & (1 << d)) && E[lm] > thr1) {
+          const int v0 = kval[o + lm];
+          cs.push_back(Cand{key_step(S, kk, B.modes[lm], v0, v0 + dir), k, lm, dir, E[lm]});
+        }
+    }
+  }
+  struct Ent {
+    uint64_t a;
+    int32_t id, pad;
+  };
+  size_t cap = 1024;
+  while (cap < cs.size() / 2) cap <<= 1;
+  std::vector<Ent> tab(cap, Ent{0, -1, 0});
+  std::vector<uint64_t> keyb;
   struct Rep {
     int k, lm, dir;
   };
   std::vector<Rep> rep;
   std::vector<double> vals;
-  auto add = [&](int k, int lm, int dir, double v) {
-    const int m = B.modes[lm];
-    const int v0 = kval[(size_t)lm * nh + k];
-    const AKey key = key_step(S, AKey{B.aux_hash[k], B.aux_hash2[k]}, m, v0, v0 + dir);
-    size_t h = KeyTable::hh(key) & (cap - 1);
-    while (tab[h].id >= 0 && !(tab[h].key == key)) h = (h + 1) & (cap - 1);
+  auto slot_of = [&](uint64_t a, uint64_t b) {
+    size_t h = KeyTable::hh(AKey{a, b}) & (cap - 1);
+    while (tab[h].id >= 0 && !(tab[h].a == a && keyb[tab[h].id] == b)) h = (h + 1) & (cap - 1);
+    return h;
+  };
+  for (size_t i = 0; i < cs.size(); ++i) {
+    if (i + 8 < cs.size()) __builtin_prefetch(&tab[KeyTable::hh(cs[i + 8].key) & (cap - 1)]);
+    const Cand &c = cs[i];
+    size_t h = slot_of(c.key.a, c.key.b);
     if (tab[h].id < 0) {
-      tab[h] = Ent{key, (int32_t)vals.size()};
-      rep.push_back({k, lm, dir});
+      if (2 * (vals.size() + 1) > cap) {
+        std::vector<Ent> old;
+        old.swap(tab);
+        cap <<= 1;
+        tab.assign(cap, Ent{0, -1, 0});
+        for (const Ent &e : old)
+          if (e.id >= 0) tab[slot_of(e.a, keyb[e.id])] = e;
+        h = slot_of(c.key.a, c.key.b);
+      }
+      tab[h] = Ent{c.key.a, (int32_t)vals.size(), 0};
+      keyb.push_back(c.key.b);
+      rep.push_back({c.k, c.lm, c.dir});
       vals.push_back(0.0);
     }
-    vals[tab[h].id] += v;
-  };
-  for (int k : out.stable_idx) {
-    for (int lm = 0; lm < nm; ++lm)
-      if (up_ok(k, lm) && E_up[(size_t)lm * nh + k] > thr1) add(k, lm, +1, E_up[(size_t)lm * nh + k]);
-    for (int lm = 0; lm < nm; ++lm)
-      if (dn_ok(k, lm) && E_dn[(size_t)lm * nh + k] > thr1) add(k, lm, -1, E_dn[(size_t)lm * nh + k]);
+    vals[tab[h].id] += c.v;
   }
   double dummy = 0;
   const double thr2 = partial_threshold(vals, dbound, discarded, &dummy);
@@ -1536,20 +1632,20 @@ int define_bases(Batch &b, const std::vector<int> &list) {
   RT_HIP(hipMemcpyAsync(b.h_down + tot * 16, de, etot * 8, hipMemcpyDeviceToHost, b.stream));
   RT_HIP(hipMemcpyAsync(b.nrms.data(), dnrm, S * 8, hipMemcpyDeviceToHost, b.stream));
   RT_HIP(hipStreamSynchronize(b.stream));
+  // the selection reads Pn and the error sums in place (pinned memory); the
+  // slot keeps psi = Pn[:n]
   const cplx *hpn = reinterpret_cast<const cplx *>(b.h_down);
   const double *he = reinterpret_cast<const double *>(b.h_down + tot * 16);
   for (int s : list) {
     Slot &sl = b.slots[s];
-    sl.pn.assign(hpn + off[s], hpn + off[s] + (size_t)sl.B.n * sl.B.nh);
+    sl.pn.assign(hpn + off[s], hpn + off[s] + sl.B.n);
     sl.nrm = b.nrms[s];
   }
-  std::vector<std::vector<double>> eaux(S);
-  for (int s : list) eaux[s].assign(he + eoff[s], he + eoff[s] + b.slots[s].B.nh);
   auto t_h = std::chrono::steady_clock::now();
   b.tm[1] += std::chrono::duration<double>(t_h - t_dev).count();
   b.pool->parallel_for((int)list.size(), [&](int i) {
     Slot &sl = b.slots[list[i]];
-    select_hier(sl.prm, sl.B, sl.pn.data(), eaux[list[i]].data(), sl.sel);
+    select_hier(sl.prm, sl.B, hpn + off[list[i]], he + eoff[list[i]], sl.sel);
   });
   auto t_s = std::chrono::steady_clock::now();
   b.tm[2] += std::chrono::duration<double>(t_s - t_h).count();
@@ -1583,14 +1679,13 @@ int define_bases(Batch &b, const std::vector<int> &list) {
                              (const int64_t *)doff, dpn, (const int64_t *)dmoff, (const uint8_t *)dmask,
                              (const int64_t *)dsoff, (double *)des, (const double *)b.arena.ptr(b.dt_off), kHbar,
                              b.stream));
-    RT_HIP(hipMemcpyAsync(b.h_down, des, stot * 8, hipMemcpyDeviceToHost, b.stream));
+    char *hs_at = b.h_down + tot * 16 + etot * 8;  // after Pn and the aux sums
+    RT_HIP(hipMemcpyAsync(hs_at, des, stot * 8, hipMemcpyDeviceToHost, b.stream));
     RT_HIP(hipStreamSynchronize(b.stream));
-    const double *hs = reinterpret_cast<const double *>(b.h_down);
-    std::vector<std::vector<double>> es(S);
-    for (int s : ls) es[s].assign(hs + soff[s], hs + soff[s] + b.slots[s].B.n);
+    const double *hs = reinterpret_cast<const double *>(hs_at);
     b.pool->parallel_for((int)ls.size(), [&](int i) {
       Slot &sl = b.slots[ls[i]];
-      select_states(sl.prm, sl.B, sl.pn.data(), es[ls[i]].data(), sl.sel);
+      select_states(sl.prm, sl.B, hpn + off[ls[i]], hs + soff[ls[i]], sl.sel);
     });
   }
   for (int s : list) {
@@ -1951,6 +2046,19 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
   return 0;
 }
 
+// The selection and table building allocate and free buffers of several MB
+// per slot and step.  glibc hands blocks above its mmap threshold back to the
+// kernel when they are freed, and trims the heap top above its trim
+// threshold, so every step pays the page faults of those buffers again;
+// raising both keeps freed blocks in the process heap for reuse.
+void keep_heap() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    mallopt(M_MMAP_THRESHOLD, 32 << 20);  // the largest value glibc accepts
+    mallopt(M_TRIM_THRESHOLD, 1 << 30);
+  });
+}
+
 int check_batch(void *p) {
   if (!p) return rt_fail("batch is NULL");
   return 0;
@@ -1961,6 +2069,7 @@ int check_batch(void *p) {
 extern "C" {
 
 void *hops_rt_system_create(const hops_rt_system_desc_t *d) {
+  keep_heap();
   if (!d || d->n_state <= 0 || d->n_hmode <= 0 || d->n_lop <= 0 || !d->H || !d->g || !d->w || !d->l_of_mode ||
       !d->l_ptr || !d->l_row || !d->l_col || !d->l_val) {
     rt_fail("hops_rt_system_create: bad description");
@@ -2051,6 +2160,7 @@ void *hops_rt_batch_create(int precision, void *arena_d, size_t arena_bytes, int
     rt_fail("arena not 256-byte aligned");
     return nullptr;
   }
+  keep_heap();
   auto *b = new Batch();
   b->precision = precision;
   b->stream = (hipStream_t)stream;
@@ -2438,6 +2548,14 @@ int hops_rt_debug_result(void *p, int32_t *n_states, int32_t *states_out, int32_
 int hops_rt_debug_free(void *p) {
   delete static_cast<DebugBasis *>(p);
   return 0;
+}
+
+double hops_rt_debug_threshold(const double *v, long long n, double max_err, double offset, double *below) {
+  std::vector<double> w(v, v + (n > 0 ? n : 0));
+  double b = 0;
+  const double thr = partial_threshold(w, max_err, offset, &b);
+  if (below) *below = b;
+  return thr;
 }
 
 }  // extern "C"

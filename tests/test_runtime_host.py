@@ -3,6 +3,9 @@ a GPU: its basis tables equal mesohops_amd/basis.py::BasisTables (which the
 oracle pins), and its basis selection equals the oracle's define_basis on the
 bases and hierarchy vectors of oracle adaptive trajectories (derivative sums
 from the oracle's dsystem_dt)."""
+import ctypes
+import math
+
 import numpy as np
 import pytest
 
@@ -120,3 +123,33 @@ def test_runtime_selection_matches_oracle(name):
         assert set(aux_new) == set(ref[1]), step
         sizes.add((n, nh))
     assert len(sizes) > 1
+
+
+def test_threshold_prefix_sort_matches_full_sort():
+    """The selection's error threshold sorts only the prefix of the errors it
+    needs (values binned by binary exponent).  It equals the oracle's
+    determine_error_thresh on the fully sorted errors: heavy-tailed values over
+    30 decades, zeros, ties, max_err from 0 to above the total, max_err equal
+    to an exact prefix sum, and offsets below and above max_err."""
+    from mesohops_amd._lib import lib
+    L = lib()
+    rng = np.random.default_rng(11)
+    for trial in range(240):
+        n = int(rng.choice([0, 1, 2, 7, 100, 5000, 70000]))
+        v = 10.0 ** rng.uniform(-30, 0, n)
+        if n and trial % 3 == 0:
+            v[rng.random(n) < 0.2] = 0.0
+        if n and trial % 5 == 0:
+            v = np.round(v, 3)
+        v = np.ascontiguousarray(v)
+        total = float(v.sum())
+        choices = [0.0, total * 1e-8, total * 1e-3, total * 0.5, total, total * 2 + 1e-300]
+        if n:
+            choices.append(float(np.cumsum(np.sort(v))[rng.integers(n)]))
+        max_err = float(rng.choice(choices))
+        offset = float(rng.choice([0.0, max_err * 0.3, max_err * 1.5]))
+        below = ctypes.c_double()
+        thr = L.hops_rt_debug_threshold(v.ctypes.data, n, max_err, offset, ctypes.byref(below))
+        ref = O.determine_error_thresh(np.sort(v), max_err, offset)
+        assert thr == ref, (trial, n, max_err, offset)
+        assert below.value == pytest.approx(math.fsum(v[v <= ref]), rel=1e-12, abs=0), trial
