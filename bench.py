@@ -16,6 +16,12 @@ short measurement of the adaptive batch ("adaptive_c3", below).
 bath), depth 10, delta_a = delta_s = 1e-3, psi0 on site 50, dt 1 fs, TAU
 0.5 fs, TLEN 512 fs, noise seeds 0..B-1; every trajectory keeps its own basis
 (ragged batch); complex64 stages, FP64 state carry, norms and error bounds.
+--workload c5: adaptive NORMALIZED NONLINEAR batch on a 30 x 30 lattice,
+V = 40 cm^-1, E_n ~ N(0, 150 cm^-1) (seed 0); per site the DL bath (lambda
+35, gamma 50 cm^-1, 77 K) with 4 Matsubara terms: the high-T mode and the
+first Matsubara mode in the hierarchy, the other three in the low-temperature
+correction (sum g/w) and all five in the noise; depth 8, delta 1e-3, psi0 on
+site (15, 15), dt 1 fs, TAU 0.5 fs, TLEN 1024 fs, 1024 trajectories.
 --workload c4: serving stream — Poisson arrivals (200/s, seed 0) of adaptive
 requests with chain length U{2..200}, depth U{2..10}, delta U[1e-4, 1e-2],
 t_max U[200, 2000] fs, dt 1 fs; continuous batching; p50/p99 latency.
@@ -104,18 +110,29 @@ def algorithmic_bytes_per_traj_step(tables, B):
     return 96 * n_phi + 3 * tables.n_act * 8 * 2 + s_op / B
 
 
-def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_bytes=None, log=None):
+def adaptive_workload(kind):
+    """(system, psi0, noise, hierarchy) of the adaptive batch workloads: c3 the
+    100-site chain at depth 10, c5 the 30 x 30 lattice at depth 8 with the
+    low-temperature correction."""
+    from mesohops_amd.workloads import chain_system, lattice_system
+    if kind == "c5":
+        sysp = lattice_system(30, 30)
+        psi = np.zeros(900, np.complex128)
+        psi[15 * 30 + 15] = 1.0
+        return sysp, psi, {"MODEL": "FFT_FILTER", "TLEN": 1024.0, "TAU": 0.5}, {"MAXHIER": 8}
+    sysp = chain_system(100)
+    psi = np.zeros(100, np.complex128)
+    psi[50] = 1.0
+    return sysp, psi, {"MODEL": "FFT_FILTER", "TLEN": 512.0, "TAU": 0.5}, {"MAXHIER": 10}
+
+
+def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_bytes=None, log=None, kind="c3"):
     """The adaptive batch: B trajectories, each with its own basis."""
     import torch
     from mesohops_amd import HopsTrajectory, _lib
     from mesohops_amd.ragged import AdaptiveBatch
-    from mesohops_amd.workloads import chain_system
-    sysp = chain_system(100)
-    psi = np.zeros(100, np.complex128)
-    psi[50] = 1.0
+    sysp, psi, noise, hier = adaptive_workload(kind)
     eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
-    noise = {"MODEL": "FFT_FILTER", "TLEN": 512.0, "TAU": 0.5}
-    hier = {"MAXHIER": 10}
 
     def make(nb, p):
         ab = AdaptiveBatch(sysp, eom, dict(noise, SEED=0), None, hier, seeds=list(range(nb)), precision_mode=p,
@@ -143,7 +160,7 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
                        "max_n_hier": int(max(i["n_hier"] for i in infos))})
         done += c
         if log:
-            log("c3 t=%d fs: %.0f traj*steps/s, mean N_phi %.0f" % (done, B * c / chunks[-1]["s"],
+            log("%s t=%d fs: %.0f traj*steps/s, mean N_phi %.0f" % (kind, done, B * c / chunks[-1]["s"],
                                                                   chunks[-1]["mean_n_phi"]))
     elapsed = time.perf_counter() - t0
     ms, n = _lib.profile_read()
@@ -206,16 +223,25 @@ def run_c4(n_req, prec="mixed", time_limit=600.0, log=None):
 
 
 def main_c3(args):
-    out3 = run_c3(args.batch if args.batch != 256 else 256, args.steps, args.precision, args.single_steps,
-                  args.ref_traj, log=lambda m: print(m, file=sys.stderr, flush=True))
+    c5 = args.workload == "c5"
+    B = args.batch if args.batch != 256 else (1024 if c5 else 256)
+    out3 = run_c3(B, args.steps, args.precision, args.single_steps, args.ref_traj,
+                  log=lambda m: print(m, file=sys.stderr, flush=True), kind=args.workload)
     sk = out3["stage_kernel"]
+    if c5:
+        wl = ("adaptive NORMALIZED NONLINEAR batch: 30x30 lattice (V 40, E ~ N(0,150)), per site 2 DL modes "
+              "(high-T + 1 Matsubara, 77 K) in the hierarchy and 3 more Matsubara modes in the low-temperature "
+              "correction and the noise, depth 8, delta_a = delta_s = 1e-3, %d trajectories with their own bases, "
+              "dt 1 fs, TLEN 1024 fs, t = 0..%d fs" % (out3["trajectories"], args.steps))
+    else:
+        wl = ("adaptive NORMALIZED NONLINEAR batch: 100-site chain, 2 DL modes/site, depth 10, "
+              "delta_a = delta_s = 1e-3, %d trajectories with their own bases, dt 1 fs, "
+              "t = 0..%d fs" % (out3["trajectories"], args.steps))
     out = {"metric": METRIC, "value": out3["value"], "unit": "traj·RK4-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": 0, "ms_per_step": out3["ms_per_step"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "c64" if args.precision == "mixed" else "c128",
            "data": "synthetic",
-           "config": {"workload": "adaptive NORMALIZED NONLINEAR batch: 100-site chain, 2 DL modes/site, depth 10, "
-                                  "delta_a = delta_s = 1e-3, %d trajectories with their own bases, dt 1 fs, "
-                                  "t = 0..%d fs" % (out3["trajectories"], args.steps)},
+           "config": {"workload": wl},
            "roofline": {"bound": "hbm", "achieved": sk["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": sk["frac_of_8TBs"], "traffic": None, "kernel": "k_stage_slots",
                         "kernel_avg_ms": sk["avg_ms"]},
@@ -250,7 +276,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="trajectories per cache-resident chunk with --lanes")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
     ap.add_argument("--requests", type=int, default=64, help="c4: requests in the stream")
     ap.add_argument("--c3-steps", type=int, default=40, help="steps of the adaptive summary in the c2 run (0: off)")
     args = ap.parse_args()
@@ -258,9 +284,11 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (trajectories would shard "
               "one batch per GPU, see DESIGN.md); got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    if args.workload == "c3":
-        if args.steps > 500:
+    if args.workload in ("c3", "c5"):
+        if args.workload == "c3" and args.steps > 500:
             sys.exit("c3 runs at most 500 steps of 1 fs (TLEN 512 fs)")
+        if args.workload == "c5" and args.steps > 1000:
+            sys.exit("c5 runs at most 1000 steps of 1 fs (TLEN 1024 fs)")
         return main_c3(args)
     if args.workload == "c4":
         return main_c4(args)

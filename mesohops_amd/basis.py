@@ -79,16 +79,20 @@ class SystemInfo:
         self.lt_param = lt
         # noise maps: hierarchy L index -> noise modes of that L-operator
         self.noise_modes_by_l = {}
+        self.noise_n_l2 = {}  # distinct L-operators of each noise
         for tag in ("1", "2"):
             key = "L_NOISE" + tag
             if key not in system_param:
                 continue
             by_l = [[] for _ in range(len(sig))]
+            seen = set()
             for i, L in enumerate(system_param[key]):
                 s = operator_signature(L)
+                seen.add(s)
                 if s in order:
                     by_l[order[s]].append(i)
             self.noise_modes_by_l[tag] = by_l
+            self.noise_n_l2[tag] = len(seen)
         self.n_noise_modes = len(system_param.get("L_NOISE1", gw))
 
 

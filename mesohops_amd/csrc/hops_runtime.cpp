@@ -457,22 +457,36 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
     B.lc_row.insert(B.lc_row.end(), R.ri.begin(), R.ri.end());
     B.lc_val.insert(B.lc_val.end(), R.cv.begin(), R.cv.end());
   }
-  // ELL width: densest row of K2_kp1 + K2_km1
+  // ELL width: densest row of K2_kp1 + K2_km1.  Row (i, s) holds one entry
+  // per nonzero of row s of L_m for each up connection of i in mode m and of
+  // column s for each down connection; the nonzero rows / columns of every
+  // reduced L are listed once, so an aux costs its connections, not n of them.
   {
-    int w = 0;
-    for (int i = 0; i < nh; ++i)
+    std::vector<std::vector<std::pair<int, int>>> nzr(nl), nzc(nl);
+    for (int j = 0; j < nl; ++j)
       for (int s = 0; s < n; ++s) {
-        int c = 0;
-        for (int u = B.up_ptr[i]; u < B.up_ptr[i + 1]; ++u) {
-          const RedL &R = B.L[B.mode_lrel[B.up_mode[u]]];
-          c += R.rp[s + 1] - R.rp[s];
-        }
-        for (int u = B.dn_ptr[i]; u < B.dn_ptr[i + 1]; ++u) {
-          const RedL &R = B.L[B.mode_lrel[B.dn_mode[u]]];
-          c += R.cp[s + 1] - R.cp[s];
-        }
-        w = std::max(w, c);
+        const RedL &R = B.L[j];
+        if (R.rp[s + 1] > R.rp[s]) nzr[j].push_back({s, R.rp[s + 1] - R.rp[s]});
+        if (R.cp[s + 1] > R.cp[s]) nzc[j].push_back({s, R.cp[s + 1] - R.cp[s]});
       }
+    std::vector<int> cnt(n, 0);
+    int w = 0;
+    for (int i = 0; i < nh; ++i) {
+      for (int u = B.up_ptr[i]; u < B.up_ptr[i + 1]; ++u)
+        for (auto &e : nzr[B.mode_lrel[B.up_mode[u]]]) cnt[e.first] += e.second;
+      for (int u = B.dn_ptr[i]; u < B.dn_ptr[i + 1]; ++u)
+        for (auto &e : nzc[B.mode_lrel[B.dn_mode[u]]]) cnt[e.first] += e.second;
+      for (int u = B.up_ptr[i]; u < B.up_ptr[i + 1]; ++u)
+        for (auto &e : nzr[B.mode_lrel[B.up_mode[u]]]) {
+          w = std::max(w, cnt[e.first]);
+          cnt[e.first] = 0;
+        }
+      for (int u = B.dn_ptr[i]; u < B.dn_ptr[i + 1]; ++u)
+        for (auto &e : nzc[B.mode_lrel[B.dn_mode[u]]]) {
+          w = std::max(w, cnt[e.first]);
+          cnt[e.first] = 0;
+        }
+    }
     B.ell_w = w;
   }
   // active L-operators: COO of L and L^2, LTC
@@ -645,69 +659,153 @@ void sort_nonneg(std::vector<double> &v) {
   for (size_t i = 0; i < n; ++i) std::memcpy(&v[i], &a[i], 8);
 }
 
-// error_threshold(sort(v), max_err, offset) and the sum of the values <= the
-// threshold (v: non-negative error terms).  Only a prefix of the sorted order
-// is sorted: the values are binned by binary exponent (a bin holds a range of
-// values, and the bins are ordered), the bins are walked up to the first one
-// where the running sum reaches max_err, and the values of those bins, which
-// are exactly the smallest values of v, are sorted and scanned as
-// error_threshold scans them.  The binned sum is summed in another order than
-// the scan, so the prefix is taken with a margin and widened to the whole of v
-// if the scan does not reach max_err inside it.
-double partial_threshold(const std::vector<double> &v, double max_err, double offset, double *below) {
-  double thr;
-  if (v.empty()) {
-    thr = 0.0;
+// error_threshold(sort(v), max_err, offset) for non-negative v, sorting only
+// a prefix of the sorted order: the values are binned by binary exponent (a
+// bin holds a range of values, and the bins are ordered), the bins are walked
+// up to the first one where the running sum reaches max_err, and the values
+// of those bins, which are exactly the smallest values of v, are sorted and
+// scanned as error_threshold scans them.  The binned sum is summed in another
+// order than the scan, so the prefix is taken with a margin and widened to the
+// whole of v if the scan does not reach max_err inside it.
+double prefix_threshold(const std::vector<double> &v, double max_err, double offset) {
+  if (v.empty()) return 0.0;
+  constexpr int kBins = 2048;
+  uint32_t cnt[kBins] = {};
+  double sum[kBins] = {};
+  for (double x : v) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    const int e = (int)(u >> 52) & (kBins - 1);
+    cnt[e]++;
+    sum[e] += x;
+  }
+  double run = offset;
+  int last = kBins - 1;
+  for (int e = 0; e < kBins; ++e) {
+    run += sum[e];
+    if (cnt[e] && run >= max_err * (1.0 - 1e-6)) {
+      last = e;
+      break;
+    }
+  }
+  std::vector<double> w;
+  if (last == kBins - 1) {
+    w = v;
   } else {
-    constexpr int kBins = 2048;
-    uint32_t cnt[kBins] = {};
-    double sum[kBins] = {};
+    size_t m = 0;
+    for (int e = 0; e <= last; ++e) m += cnt[e];
+    w.reserve(m);
     for (double x : v) {
       uint64_t u;
       std::memcpy(&u, &x, 8);
-      const int e = (int)(u >> 52) & (kBins - 1);
-      cnt[e]++;
-      sum[e] += x;
-    }
-    double run = offset;
-    int last = kBins - 1;
-    for (int e = 0; e < kBins; ++e) {
-      run += sum[e];
-      if (cnt[e] && run >= max_err * (1.0 - 1e-6)) {
-        last = e;
-        break;
-      }
-    }
-    std::vector<double> w;
-    if (last == kBins - 1) {
-      w = v;
-    } else {
-      size_t m = 0;
-      for (int e = 0; e <= last; ++e) m += cnt[e];
-      w.reserve(m);
-      for (double x : v) {
-        uint64_t u;
-        std::memcpy(&u, &x, 8);
-        if ((int)(u >> 52) <= last) w.push_back(x);
-      }
-    }
-    sort_nonneg(w);
-    double cum = 0;
-    size_t i = 0;
-    for (; i < w.size(); ++i) {
-      cum += w[i];
-      if (cum + offset >= max_err) break;
-    }
-    if (i < w.size()) {
-      thr = i > 0 ? w[i - 1] : 0.0;
-    } else if (w.size() == v.size()) {
-      thr = INFINITY;  // the sum of every value stays below max_err
-    } else {
-      std::vector<double> all(v);
-      sort_nonneg(all);
-      thr = error_threshold(all, max_err, offset);
+      if ((int)(u >> 52) <= last) w.push_back(x);
     }
   }
+  sort_nonneg(w);
+  double cum = 0;
+  size_t i = 0;
+  for (; i < w.size(); ++i) {
+    cum += w[i];
+    if (cum + offset >= max_err) break;
+  }
+  if (i < w.size()) return i > 0 ? w[i - 1] : 0.0;
+  if (w.size() == v.size()) return INFINITY;  // the sum of every value stays below max_err
+  std::vector<double> all(v);
+  sort_nonneg(all);
+  return error_threshold(all, max_err, offset);
+}
+
+// The same threshold by radix selection, without sorting the prefix.  The
+// sorted order's running sum S_i (rounded at each addition, as the reference's
+// cumsum) first reaches max_err - offset at some rank i*; the threshold is the
+// value of rank i* - 1.  The bins of the value bits (the binary exponent,
+// then 11 mantissa bits at a time) are walked to the bin holding rank i*,
+// which is refined until it is small and then sorted.  The sums here are
+// added in another order, so each differs from the reference's by at most
+// about n * eps * (total); when the crossing is closer than that to max_err
+// the result is undecided (*ok = false) and the caller sorts instead.
+double select_threshold(const std::vector<double> &v, double max_err, double offset, bool *ok) {
+  *ok = true;
+  const size_t n = v.size();
+  if (n == 0) return 0.0;
+  if (offset >= max_err) return 0.0;  // x0 + offset >= offset >= max_err
+  double total = 0;
+  for (double x : v) total += x;
+  const double tol = (4.0 * (double)n + 16.0) * 2.220446049250313e-16 * (total + std::abs(offset) + std::abs(max_err));
+  if (total + offset < max_err - tol) return INFINITY;
+  if (total + offset < max_err + tol) {
+    *ok = false;
+    return 0.0;
+  }
+  std::vector<double> cur(v), nxt;
+  double base = 0;          // sum of every value ranked below the current bin
+  double below_max = -1.0;  // largest such value (-1: none)
+  constexpr int kB = 2048;
+  static thread_local std::vector<uint32_t> cnt(kB);
+  static thread_local std::vector<double> sum(kB), mx(kB);
+  int shift = 52, width = 11;  // exponent first, then the mantissa from the top
+  while (cur.size() > 48 && shift >= 0) {
+    const uint64_t mask = (1ull << width) - 1;
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    std::fill(sum.begin(), sum.end(), 0.0);
+    std::fill(mx.begin(), mx.end(), -1.0);
+    for (double x : cur) {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      const int e = (int)((u >> shift) & mask);
+      cnt[e]++;
+      sum[e] += x;
+      mx[e] = std::max(mx[e], x);
+    }
+    int hit = -1;
+    for (int e = 0; e <= (int)mask; ++e) {
+      if (!cnt[e]) continue;
+      if (base + sum[e] + offset >= max_err) {
+        hit = e;
+        break;
+      }
+      base += sum[e];
+      below_max = std::max(below_max, mx[e]);
+    }
+    if (hit < 0) {
+      *ok = false;
+      return 0.0;
+    }
+    nxt.clear();
+    for (double x : cur) {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      if ((int)((u >> shift) & mask) == hit) nxt.push_back(x);
+    }
+    cur.swap(nxt);
+    if (shift == 0) break;
+    width = std::min(11, shift == 52 ? 52 : shift);
+    shift = shift == 52 ? 41 : shift - width;
+  }
+  std::sort(cur.begin(), cur.end());
+  double s = base;
+  for (size_t j = 0; j < cur.size(); ++j) {
+    const double prev = s;
+    s += cur[j];
+    if (s + offset >= max_err) {
+      if (prev + offset >= max_err - tol || s + offset < max_err + tol) {
+        *ok = false;
+        return 0.0;
+      }
+      if (j > 0) return cur[j - 1];
+      return below_max >= 0 ? below_max : 0.0;
+    }
+  }
+  *ok = false;
+  return 0.0;
+}
+
+// error_threshold(sort(v), max_err, offset) and the sum of the values <= the
+// threshold (v: non-negative error terms)
+double partial_threshold(const std::vector<double> &v, double max_err, double offset, double *below) {
+  bool ok = false;
+  double thr = select_threshold(v, max_err, offset, &ok);
+  if (!ok) thr = prefix_threshold(v, max_err, offset);
   double sb = 0;
   for (double x : v)
     if (x <= thr) sb += x;

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -62,8 +63,11 @@ class HopsNoise:
         self.t_axis = noise_time_axis(self.param["TLEN"], self.param["TAU"])
         self.n_t = len(self.t_axis)
         key = "L_NOISE" + tag
-        lops = system_param.get(key, system_param["L_HIER"]) if tag == "1" else system_param[key]
-        self.n_l2 = len(set(operator_signature(L) for L in lops))
+        if tag in info.noise_n_l2 and key in system_param and info.param.get(key) is system_param[key]:
+            self.n_l2 = info.noise_n_l2[tag]  # counted once for the system
+        else:
+            lops = system_param.get(key, system_param["L_HIER"]) if tag == "1" else system_param[key]
+            self.n_l2 = len(set(operator_signature(L) for L in lops))
         self.corr = system_param.get("ALPHA_NOISE" + tag)
         self.corr_param = system_param.get("PARAM_NOISE" + tag)
         self.modes_by_l = info.noise_modes_by_l.get(tag, [])
@@ -96,8 +100,9 @@ class HopsNoise:
                                          "Noise._prepare_rand", True)
             return np.complex64(seed)
         draws = np.zeros((self.n_l2, 4 * (nt - 1)))
+        base = np.random.PCG64(seed=seed)  # jumped() leaves base unchanged
         for l in range(self.n_l2):
-            rng = np.random.default_rng(np.random.PCG64(seed=seed).jumped(l))
+            rng = np.random.Generator(base.jumped(l))
             if self.param["RAND_MODEL"] == "BOX_MULLER":
                 draws[l] = rng.random(size=4 * (nt - 1))
             elif self.param["RAND_MODEL"] == "SUM_GAUSSIAN":
@@ -108,7 +113,10 @@ class HopsNoise:
         ia, ib = raw_slot_index(nt)
         if self.param["RAND_MODEL"] == "BOX_MULLER":
             return np.complex64(np.sqrt(-2.0 * np.log(draws[:, ia])) * np.exp(2.0j * np.pi * draws[:, ib]))
-        return np.complex64(draws[:, ia] + 1j * draws[:, ib])
+        out = np.empty((self.n_l2, len(ia)), dtype=np.complex64)
+        out.real = draws[:, ia]  # each part rounded to float32, as complex64(a + 1j b) rounds it
+        out.imag = draws[:, ib]
+        return out
 
     def precalculated(self):
         seed = self.param["SEED"]
@@ -159,16 +167,27 @@ def device_noise(noises, rows, device, stream_ptr, raws=None):
                                          nz.n_l2, nt,
                                          ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
                                          stream_ptr))
-        if raws is None:
-            raws = [n.raw() for n in noises]
-        raw = np.stack([r[rows] for r in raws]).reshape(B * len(rows), N)
-        z_d = torch.from_numpy(np.ascontiguousarray(raw)).to(device)
-        row_lop = torch.from_numpy(np.tile(np.array(rows, dtype=np.int32), B)).to(device)
-        check(lib().hops_noise_filter(ctypes.c_void_p(z_d.data_ptr()), B * len(rows), nt,
-                                      ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
-                                      ctypes.c_void_p(row_lop.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                      stream_ptr))
-        torch.cuda.current_stream(device).synchronize()
+        # raw draws per trajectory on worker threads (NumPy's generators and
+        # array arithmetic run without the GIL), filtered chunk by chunk so
+        # that about 1 GiB of raw draws is held on the host at a time
+        nr = len(rows)
+        chunk = max(1, min(B, (1 << 30) // max(1, nr * N * 8)))
+        row_lop = torch.from_numpy(np.tile(np.array(rows, dtype=np.int32), chunk)).to(device)
+        workers = max(1, min(16, os.cpu_count() or 1, B))
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            for b0 in range(0, B, chunk):
+                b1 = min(B, b0 + chunk)
+                if raws is None:
+                    part = list(pool.map(lambda n: n.raw()[rows], noises[b0:b1]))
+                else:
+                    part = [r[rows] for r in raws[b0:b1]]
+                z_d = torch.from_numpy(np.ascontiguousarray(np.stack(part).reshape((b1 - b0) * nr, N))).to(device)
+                check(lib().hops_noise_filter(ctypes.c_void_p(z_d.data_ptr()), (b1 - b0) * nr, nt,
+                                              ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
+                                              ctypes.c_void_p(row_lop.data_ptr()), ctypes.c_void_p(out[b0:b1].data_ptr()),
+                                              stream_ptr))
+                torch.cuda.synchronize(device)  # the filter has read z_d before it is freed
+                del z_d
     if nz.param["FLAG_REAL"]:
         torch.view_as_real(out)[..., 1] = 0
     return out

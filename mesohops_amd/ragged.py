@@ -431,4 +431,11 @@ class AdaptiveBatch:
         return self.rt.aux_list(self.slots[b])
 
     def z_mem(self, b):
-        return self.rt.zmem(self.slots[b], self.setup.info.n_hmodes)
+        """Noise memory over the absolute modes, len(L_NOISE1) entries as the
+        reference sizes it (hops_trajectory.py:290-292); the modes beyond the
+        hierarchy modes stay zero."""
+        info = self.setup.info
+        zm = self.rt.zmem(self.slots[b], info.n_hmodes)
+        out = np.zeros(max(info.n_noise_modes, len(zm)), dtype=np.complex128)
+        out[: len(zm)] = zm
+        return out

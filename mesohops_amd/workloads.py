@@ -4,6 +4,10 @@ chain_system: nearest-neighbour chain, E_n ~ N(0, sigma) from
 np.random.default_rng(seed_h), Drude-Lorentz modes per site from
 bcf_convert_dl_to_exp (util/bath_corr_functions.py:70-112), L_n = |n><n|,
 optional low-temperature correction on every site.
+lattice_system: nx x ny nearest-neighbour lattice with the low-temperature
+correction: per site the DL correlation function with k_hier + k_ltc Matsubara
+terms, the first modes in the hierarchy, sum g/w of the last k_ltc in the
+correction, every mode in the noise (trajectory/dyadic_spectra.py:808-840).
 c4_requests: the mixed-size serving stream — Poisson arrivals, chain length,
 depth, delta, t_max drawn per request from one seeded generator; the noise
 seed of a request is its id.
@@ -36,6 +40,36 @@ def chain_system(nsite, V=50.0, sigma=100.0, seed_h=0, lam=35.0, gam=50.0, temp=
         p["L_LT_CORR"] = [sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite)) for i in range(nsite)]
         p["PARAM_LT_CORR"] = [ltc] * nsite
     return p
+
+
+def lattice_system(nx, ny, V=40.0, sigma=150.0, seed_h=0, lam=35.0, gam=50.0, temp=77.0, k_hier=1, k_ltc=3):
+    n = nx * ny
+    rng = np.random.default_rng(seed_h)
+    H = np.diag(rng.normal(0.0, sigma, n)).astype(np.complex128)
+    for x in range(nx):
+        for y in range(ny):
+            i = x * ny + y
+            if x + 1 < nx:
+                H[i, i + ny] = H[i + ny, i] = V
+            if y + 1 < ny:
+                H[i, i + 1] = H[i + 1, i] = V
+    modes = bcf_convert_dl_to_exp(lam, gam, temp, k_matsubara=k_hier + k_ltc)
+    pairs = [(modes[j], modes[j + 1]) for j in range(0, len(modes), 2)]
+    gw, ll, gwn, lln, ltc = [], [], [], [], []
+    for i in range(n):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n))
+        c = 0.0
+        for j, (g, w) in enumerate(pairs):
+            gwn.append([g, w])
+            lln.append(L)
+            if len(pairs) - j > k_ltc:
+                gw.append([g, w])
+                ll.append(L)
+            else:
+                c += g / w
+        ltc.append(c)
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=lln, ALPHA_NOISE1=bcf_exp, PARAM_NOISE1=gwn,
+                L_LT_CORR=[sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)) for i in range(n)], PARAM_LT_CORR=ltc)
 
 
 @dataclass

@@ -81,3 +81,40 @@ def dimer_c1():
     ll = [sp.coo_matrix(([1.0], ([i], [i])), shape=(2, 2)) for i in range(2)]
     gw = [[g, w], [g, w]]
     return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+
+
+def lattice(nx, ny, V=40.0, sigma=150.0, seed_h=0, lam=35.0, gam=50.0, temp=77.0, k_hier=1, k_ltc=3):
+    """nx x ny nearest-neighbour lattice, E ~ N(0, sigma), L_n = |n><n|.
+    Per site the Drude-Lorentz correlation function with k_hier + k_ltc
+    Matsubara terms (util/bath_corr_functions.py:70-112): the high-temperature
+    mode and the first k_hier Matsubara modes go into the hierarchy, the last
+    k_ltc into the low-temperature correction as sum g/w, and every mode into
+    the noise (the recipe of trajectory/dyadic_spectra.py:808-840)."""
+    n = nx * ny
+    rng = np.random.default_rng(seed_h)
+    H = np.diag(rng.normal(0.0, sigma, n)).astype(np.complex128)
+    for x in range(nx):
+        for y in range(ny):
+            i = x * ny + y
+            if x + 1 < nx:
+                H[i, i + ny] = H[i + ny, i] = V
+            if y + 1 < ny:
+                H[i, i + 1] = H[i + 1, i] = V
+    modes = O.bcf_convert_dl_to_exp(lam, gam, temp, k_matsubara=k_hier + k_ltc)
+    pairs = [(modes[j], modes[j + 1]) for j in range(0, len(modes), 2)]
+    gw, ll, gwn, lln, ltc = [], [], [], [], []
+    for i in range(n):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n))
+        c = 0.0
+        for j, (g, w) in enumerate(pairs):
+            gwn.append([g, w])
+            lln.append(L)
+            if len(pairs) - j > k_ltc:
+                gw.append([g, w])
+                ll.append(L)
+            else:
+                c += g / w
+        ltc.append(c)
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=lln, ALPHA_NOISE1=O.bcf_exp,
+                PARAM_NOISE1=gwn, L_LT_CORR=[sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)) for i in range(n)],
+                PARAM_LT_CORR=ltc)

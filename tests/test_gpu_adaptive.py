@@ -100,6 +100,9 @@ ADAPTIVE_CASES = {
     "chain_states_only": (systems.chain(8), None, 0.0, 2e-3, 2, "NONLINEAR", 60.0, 2.0),
     "markov_chain": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR", 60.0, 2.0),
     "chain10_update3": (systems.chain(10), None, 3e-3, 3e-3, 3, "NORMALIZED NONLINEAR", 60.0, 1.0),
+    # 2D lattice with a low-temperature correction and noise modes beyond the
+    # hierarchy modes (the Matsubara modes treated by the correction)
+    "lattice4_ltc": (systems.lattice(4, 4), None, 2e-3, 2e-3, 3, "NORMALIZED NONLINEAR", 40.0, 1.0),
 }
 
 

@@ -38,6 +38,7 @@ CASES = {
     "dod": (systems.dimer_of_dimers(), "NORMALIZED NONLINEAR", 3, 1e-3, 1e-3, 60.0, 2.0, 1),
     "chain8_states": (systems.chain(8), "NONLINEAR", 2, 0.0, 2e-3, 40.0, 2.0, 1),
     "chain10_update2": (systems.chain(10), "NORMALIZED NONLINEAR", 4, 2e-3, 2e-3, 40.0, 1.0, 2),
+    "lattice4_ltc": (systems.lattice(4, 4), "NORMALIZED NONLINEAR", 3, 2e-3, 2e-3, 40.0, 1.0, 1),
 }
 
 

@@ -45,12 +45,13 @@ def _bases(sysp, rng):
     return info, out
 
 
-@pytest.mark.parametrize("name", ["dod", "chain_ltc", "three_diag", "three_general"])
+@pytest.mark.parametrize("name", ["dod", "chain_ltc", "three_diag", "three_general", "lattice_ltc"])
 def test_runtime_tables_equal_host_tables(name):
     from mesohops_amd.basis import BasisTables
     from mesohops_amd.ragged import DebugBasis, RtSystem
     sysp = {"dod": systems.dimer_of_dimers(), "chain_ltc": systems.chain(5, ltc=12.0 - 3.0j),
-            "three_diag": systems.three_site(False), "three_general": systems.three_site(True)}[name]
+            "three_diag": systems.three_site(False), "three_general": systems.three_site(True),
+            "lattice_ltc": systems.lattice(3, 2)}[name]
     rng = np.random.default_rng(5)
     info, bases = _bases(sysp, rng)
     rs = RtSystem(info)
@@ -80,6 +81,7 @@ CASES = {
     "chain_states_only": (systems.chain(8), None, 0.0, 2e-3, 2, "NONLINEAR"),
     "markov_chain": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR"),
     "chain12": (systems.chain(12), None, 2e-3, 2e-3, 5, "NORMALIZED NONLINEAR"),
+    "lattice4_ltc": (systems.lattice(4, 4), None, 2e-3, 2e-3, 3, "NORMALIZED NONLINEAR"),
 }
 
 
