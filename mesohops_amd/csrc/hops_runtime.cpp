@@ -895,11 +895,26 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   std::vector<double> err(e_aux, e_aux + nh);
   // flux out of the state basis through H (basis_functions_adaptive.py:76-163)
   if (!out.sc.empty()) {
+    // -i H[sp, s] over the basis states s coupled to each outside state sp
+    // (a zero entry adds an exact zero to the sum, so it is skipped)
+    std::vector<int> hp(out.sc.size() + 1, 0), hs;
+    std::vector<cplx> hv;
+    for (size_t i = 0; i < out.sc.size(); ++i) {
+      for (int s = 0; s < n; ++s) {
+        const cplx h = S.H[(size_t)out.sc[i] * S.n + B.states[s]];
+        if (h != cplx(0, 0)) {
+          hs.push_back(s);
+          hv.push_back(cmi(h));
+        }
+      }
+      hp[i + 1] = (int)hs.size();
+    }
     for (int k = 0; k < nh; ++k) {
+      const cplx *pk = pn + (size_t)k * n;
       double acc = 0;
-      for (int sp : out.sc) {
+      for (size_t i = 0; i < out.sc.size(); ++i) {
         cplx z = 0;
-        for (int s = 0; s < n; ++s) z += cm(cmi(S.H[(size_t)sp * S.n + B.states[s]]), pn[(size_t)k * n + s]);
+        for (int q = hp[i]; q < hp[i + 1]; ++q) z += cm(hv[q], pk[hs[q]]);
         z /= kHbar;
         acc += std::norm(z);
       }
@@ -1104,20 +1119,37 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
     err[s] += v1 * c1 / h2;
   }
   // filters from the boundary auxiliaries (hops_fluxfilters.py:292-352)
+  // F_dn[lm, j]: j = a + e_lm for a boundary aux a; F_up[lm, j]: j = a - e_lm
   std::vector<uint8_t> F_dn((size_t)nm * nh, 0), F_up((size_t)nm * nh, 0);
-  for (const Aux &a : sel.aux_bound) {
+  const bool from_basis = sel.aux_bound.size() * (size_t)nm > B.aux_mode_t.size();
+  KeyTable btab;
+  if (from_basis) btab.reset(sel.aux_bound.size());
+  for (size_t i = 0; i < sel.aux_bound.size(); ++i) {
+    const Aux &a = sel.aux_bound[i];
     const AKey ka = key_of(S, a);
-    for (int lm = 0; lm < nm; ++lm) {
-      const int v = aux_get(a, B.modes[lm]);
-      if (v >= 255) continue;
-      const int j = B.index.find(key_step(S, ka, B.modes[lm], v, v + 1));
-      if (j >= 0) F_dn[(size_t)lm * nh + j] = 1;
+    if (from_basis) {
+      btab.insert(ka, (int)i);
+    } else {
+      for (int lm = 0; lm < nm; ++lm) {
+        const int v = aux_get(a, B.modes[lm]);
+        if (v >= 255) continue;
+        const int j = B.index.find(key_step(S, ka, B.modes[lm], v, v + 1));
+        if (j >= 0) F_dn[(size_t)lm * nh + j] = 1;
+      }
     }
     for (char32_t e : a) {
       const int j = B.index.find(key_step(S, ka, aux_mode(e), aux_val(e), aux_val(e) - 1));
       if (j >= 0) F_up[(size_t)B.rel_mode[aux_mode(e)] * nh + j] = 1;
     }
   }
+  if (from_basis)  // the same F_dn from the basis side: j - e_m is a boundary aux
+    for (int j = 0; j < nh; ++j) {
+      const AKey kj{B.aux_hash[j], B.aux_hash2[j]};
+      for (int q = B.aux_ptr[j]; q < B.aux_ptr[j + 1]; ++q) {
+        const int m = B.aux_mode_t[q], v = B.aux_val_t[q];
+        if (btab.find(key_step(S, kj, m, v, v - 1)) >= 0) F_dn[(size_t)B.rel_mode[m] * nh + j] = 1;
+      }
+    }
   for (int lm = 0; lm < nm; ++lm) {
     const int m = B.modes[lm];
     const double gw2 = std::norm(S.G[m] / S.W[m]);
@@ -1158,12 +1190,22 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
   std::vector<int> bound;
   if ((int)rel_st.size() < S.n && !sel.sc.empty()) {
     std::vector<double> E(sel.sc.size(), 0.0);
+    std::vector<int> qs;
+    std::vector<cplx> qh;
     for (size_t i = 0; i < sel.sc.size(); ++i) {
       const int sp = sel.sc[i];
+      qs.clear();
+      qh.clear();
+      for (size_t q = 0; q < rel_st.size(); ++q) {  // zero entries add exact zeros: skipped
+        const cplx h = S.H[(size_t)sp * S.n + st_stable[q]];
+        if (h != cplx(0, 0)) {
+          qs.push_back(rel_st[q]);
+          qh.push_back(cmi(h));
+        }
+      }
       for (int k : sel.stable_idx) {
         cplx z = 0;
-        for (size_t q = 0; q < rel_st.size(); ++q)
-          z += cm(cmi(S.H[(size_t)sp * S.n + st_stable[q]]), pn[(size_t)k * n + rel_st[q]] / kHbar);
+        for (size_t q = 0; q < qs.size(); ++q) z += cm(qh[q], pn[(size_t)k * n + qs[q]] / kHbar);
         E[i] += std::norm(z);
       }
     }
