@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="trajectories interleaved per row (0: autotune)")
     ap.add_argument("--chunk", type=int, default=0, help="trajectories per cache-resident chunk with --lanes")
+    ap.add_argument("--nt", type=int, default=0, help="nontemporal hint on streamed arrays with --lanes")
+    ap.add_argument("--groups-q", type=int, default=0, help="trajectory-group split with --lanes (0: automatic)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
@@ -315,8 +317,8 @@ def main():
         choice, tune_t = autotune.tune(batch, psi0(), DT, verbose=True)
     else:
         batch.relayout(args.lanes)
-        _lib.set_option("chunk_traj", args.chunk)
-        choice, tune_t = (args.lanes, True, args.chunk), {}
+        choice, tune_t = (args.lanes, True, args.chunk, args.nt, args.groups_q), {}
+        autotune.apply(choice)
     st = batch.state
 
     def run(steps):
@@ -392,6 +394,13 @@ def main():
     # algorithmic bytes over the stage launches that moved them
     bytes_launch = bytes_step * B * args.steps / max(n[1], 1)
     achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
+    rt = "double" if args.precision == "fp64" else "float"
+    pat_w = int(np.max(np.diff(np.asarray(tb.tables["pat_ptr"])))) if len(tb.tables["pat_ptr"]) > 1 else 0
+    if choice[1] and 0 < pat_w <= 8 and tb.ell_w <= 16:
+        kernel_name = "k_stage_shared<%s, %d, %d, %d>" % (rt, 4 if tb.ell_w <= 4 else (8 if tb.ell_w <= 8 else 16),
+                                                          4 if pat_w <= 4 else 8, choice[3])
+    else:
+        kernel_name = "k_stage<%s>" % rt
     out = {
         "metric": METRIC,
         "value": value,
@@ -421,10 +430,12 @@ def main():
         "speedup_vs_gpu_fp64_single": value / single_rate,
         "init_s": t_init,
         "layout": {"lanes": choice[0], "stationary_stage": bool(choice[1]), "chunk_traj": choice[2],
-                   "autotune_ms_per_step": {"%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
+                   "nt_stream": choice[3], "groups_q": choice[4],
+                   "autotune_ms_per_step (lanes/stationary/chunk/nt/q)":
+                       {"%d/%d/%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage<float>", "kernel_avg_ms": stage_ms,
+                     "kernel": kernel_name, "kernel_avg_ms": stage_ms,
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "prologue_avg_ms": ms[0] / max(n[0], 1)},
     }

@@ -14,26 +14,40 @@ import torch
 
 from . import _lib
 
-# (lanes, stationary stage kernel, trajectories per cache-resident chunk; 0 = whole batch)
-CANDIDATES = [(1, True, 0), (4, True, 0), (8, True, 0), (16, True, 0), (4, False, 0),
-              (4, True, 32), (4, True, 64), (8, True, 64), (8, True, 128), (2, True, 32)]
+# (lanes, stationary stage kernel, trajectories per cache-resident chunk (0 =
+# whole batch), nontemporal hint (0 none, 1 accumulator and carry, 2 also the
+# stage outputs), trajectory-group split
+# of the stationary kernel (0 = automatic))
+CANDIDATES = [(1, True, 0, 0, 0), (4, True, 0, 0, 0), (8, True, 0, 0, 0), (16, True, 0, 0, 0),
+              (4, False, 0, 0, 0), (4, True, 64, 0, 0), (2, True, 32, 0, 0),
+              (4, True, 0, 1, 0), (8, True, 0, 1, 0), (4, True, 0, 2, 0), (8, True, 0, 2, 0), (4, True, 64, 1, 0),
+              (4, True, 0, 0, 2), (4, True, 0, 1, 2)]
+
+
+def apply(choice):
+    """Set the library options of a candidate (the layout is set by the batch)."""
+    lanes, stationary, chunk, nt, q = choice
+    _lib.set_option("stationary_stage", 1 if stationary else 0)
+    _lib.set_option("chunk_traj", chunk)
+    _lib.set_option("nt_stream", nt)
+    _lib.set_option("groups_q", q)
 
 
 def tune(batch, psi_0, tau, steps=12, candidates=None, verbose=False):
-    """Returns ((lanes, stationary, chunk), {candidate: seconds per step})."""
+    """Returns ((lanes, stationary, chunk, nt, q), {candidate: seconds per step})."""
     candidates = candidates or CANDIDATES
     nz = batch.noise1[0]
     ns = int(round(tau * 0.5 / nz.param["TAU"]))
     timings = {}
-    for lanes, stationary, chunk in candidates:
+    for cand in candidates:
+        lanes, stationary, chunk, nt, q = cand
         if lanes > 1 and batch.B < 8 * lanes and stationary:
             continue
         if chunk and (chunk >= batch.B or chunk < 8 * lanes):
             continue
         batch.relayout(lanes)
         batch.rewind(psi_0)
-        _lib.set_option("stationary_stage", 1 if stationary else 0)
-        _lib.set_option("chunk_traj", chunk)
+        apply(cand)
         st = batch.state
         st.bt.psi_store_d = None
         st.propagate(0, 1, tau, ns, 0)
@@ -41,13 +55,12 @@ def tune(batch, psi_0, tau, steps=12, candidates=None, verbose=False):
         t0 = time.perf_counter()
         st.propagate(0, steps, tau, ns, 0)
         torch.cuda.synchronize(batch.device)
-        timings[(lanes, stationary, chunk)] = (time.perf_counter() - t0) / steps
+        timings[cand] = (time.perf_counter() - t0) / steps
         if verbose:
-            print("autotune lanes=%d stationary=%d chunk=%d: %.3f ms/step"
-                  % (lanes, stationary, chunk, timings[(lanes, stationary, chunk)] * 1e3), file=sys.stderr)
+            print("autotune lanes=%d stationary=%d chunk=%d nt=%d q=%d: %.3f ms/step"
+                  % (lanes, stationary, chunk, nt, q, timings[cand] * 1e3), file=sys.stderr)
     best = min(timings, key=timings.get)
     batch.relayout(best[0])
-    _lib.set_option("stationary_stage", 1 if best[1] else 0)
-    _lib.set_option("chunk_traj", best[2])
+    apply(best)
     batch.rewind(psi_0)
     return best, timings

@@ -94,6 +94,47 @@ __device__ __forceinline__ cx<R> cscale(cx<R> a, R s) {
 template <typename R> __device__ __forceinline__ cx<R> cconj(cx<R> a) {
   return mk<R>(a.x, -a.y);
 }
+// Loads and stores with the nontemporal hint, for the arrays a stage streams
+// through once (RK4 accumulator, FP64 carry): they then displace less of the
+// L2 that holds the neighbour rows the stage gathers.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  static_assert(sizeof(T) == 8 || sizeof(T) == 16, "8- or 16-byte elements");
+  T v;
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long u = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+    __builtin_memcpy(&v, &u, 8);
+  } else {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v u = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p));
+    __builtin_memcpy(&v, &u, 16);
+  }
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  static_assert(sizeof(T) == 8 || sizeof(T) == 16, "8- or 16-byte elements");
+  if constexpr (sizeof(T) == 8) {
+    unsigned long long u;
+    __builtin_memcpy(&u, &v, 8);
+    __builtin_nontemporal_store(u, reinterpret_cast<unsigned long long *>(p));
+  } else {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    d2v u;
+    __builtin_memcpy(&u, &v, 16);
+    __builtin_nontemporal_store(u, reinterpret_cast<d2v *>(p));
+  }
+}
+template <int NT, typename T>
+__device__ __forceinline__ T ld_s(const T *p) {
+  if constexpr (NT != 0) return ld_stream(p);
+  else return *p;
+}
+template <int NT, typename T>
+__device__ __forceinline__ void st_s(T *p, T v) {
+  if constexpr (NT != 0) st_stream(p, v);
+  else *p = v;
+}
 // Complex division by Smith's (1962) scaled algorithm with a reciprocal scale,
 // the same operation sequence NumPy uses for complex128 division, so that
 // g/w reproduces the reference's super-operator values bit for bit.
@@ -639,8 +680,10 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
 
 // Row update with the row's pattern and ELL entries in registers and every
 // load issued before the arithmetic (more bytes in flight per wave).
-// pidx[p] < 0 and ecol[e] < 0 mark unused slots.
-template <typename R, int WR, int PW>
+// pidx[p] < 0 and ecol[e] < 0 mark unused slots.  NT >= 1: the accumulator
+// and the carry are read and written with the nontemporal hint; NT = 2: the
+// stage output and the complex64 state copy are stored with it too.
+template <typename R, int WR, int PW, int NT>
 __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_batch_t &bt, int stage,
                                               const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
                                               int b, long r, int k, cx<R> k0, const int *ecol,
@@ -674,15 +717,15 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
   double2 ph = make_double2(0.0, 0.0);
   if (stage < 3) {
     if (bt.precision == HOPS_PREC_MIXED) {
-      base = reinterpret_cast<const cx<R> *>(bt.y0_d)[gi];
+      base = ld_s<NT>(reinterpret_cast<const cx<R> *>(bt.y0_d) + gi);
     } else {
-      const double2 p = phi[gi];
+      const double2 p = ld_s<NT>(phi + gi);
       base = mk<R>((R)p.x, (R)p.y);
     }
   } else {
-    ph = phi[gi];
+    ph = ld_s<NT>(phi + gi);
   }
-  if (stage > 0) a_in = acc[gi];
+  if (stage > 0) a_in = ld_s<NT>(acc + gi);
   // arithmetic
   cx<R> d = cmul(k0, yr);
 #pragma unroll
@@ -704,26 +747,26 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
     if (bt.precision == HOPS_PREC_MIXED) {
       base = cscale(base, (R)sigma);
     } else {
-      const double2 p = phi[gi];
+      const double2 p = ld_s<NT>(phi + gi);
       base = mk<R>((R)(sigma * p.x), (R)(sigma * p.y));
     }
     const R c = (R)((stage == 2 ? 1.0 : 0.5) * h);
-    yout[gi] = mk<R>(base.x + c * d.x, base.y + c * d.y);
+    st_s<NT == 2 ? 1 : 0>(yout + gi, mk<R>(base.x + c * d.x, base.y + c * d.y));
     if (stage == 0) {
-      acc[gi] = d;
+      st_s<NT>(acc + gi, d);
     } else {
       a_in.x += (R)2 * d.x;
       a_in.y += (R)2 * d.y;
-      acc[gi] = a_in;
+      st_s<NT>(acc + gi, a_in);
     }
   } else {
     const double c = h / 6.0;
     double2 q;
     q.x = sigma * ph.x + c * ((double)a_in.x + (double)d.x);
     q.y = sigma * ph.y + c * ((double)a_in.y + (double)d.y);
-    phi[gi] = q;
+    st_s<NT>(phi + gi, q);
     if (bt.precision == HOPS_PREC_MIXED)
-      reinterpret_cast<cx<float> *>(bt.y0_d)[gi] = mk<float>((float)q.x, (float)q.y);
+      st_s<NT == 2 ? 1 : 0>(reinterpret_cast<cx<float> *>(bt.y0_d) + gi, mk<float>((float)q.x, (float)q.y));
   }
 }
 
@@ -733,7 +776,7 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
 // operator is read once per tile and stage instead of once per trajectory,
 // and every XCD works on the same few groups at a time (their rows stay in
 // its L2).
-template <typename R, int WR, int PW>
+template <typename R, int WR, int PW, int NT>
 __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const hops_batch_t bt, int stage,
                                                       const cx<R> *__restrict__ yin,
                                                       cx<R> *__restrict__ yout, int n_tiles, int Q) {
@@ -769,7 +812,7 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
   const int G = (bt.B + (1 << bt.lane_shift) - 1) >> bt.lane_shift;
   for (int g = x + 8 * q; g < G; g += 8 * Q) {
     const int b = (g << bt.lane_shift) + lane;
-    if (b < bt.B) stage_row_reg<R, WR, PW>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
+    if (b < bt.B) stage_row_reg<R, WR, PW, NT>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
   }
 }
 
@@ -850,6 +893,8 @@ struct Profile {
 Profile g_prof;
 bool g_stationary = true;
 int g_chunk_traj = 0;
+int g_nt_stream = 0;  // nontemporal hint in k_stage_shared: 1 accumulator and carry, 2 also the outputs
+int g_groups_q = 0;        // trajectory-group split of k_stage_shared (0: automatic)
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
   if (!g_prof.on) return 0;
@@ -924,7 +969,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     const long rows_per_tile = 512 >> bt->lane_shift;
     sh_tiles = (int)((nr + rows_per_tile - 1) / rows_per_tile);
     const int per_xcd = (Gc + 7) / 8;
-    sh_q = (per_xcd + 31) / 32;
+    sh_q = g_groups_q > 0 ? std::min(g_groups_q, per_xcd) : (per_xcd + 31) / 32;
     sh_grid = (unsigned)(8L * sh_q * sh_tiles);
   } else {
     chunk = bt->B;  // chunks need the stationary kernel (no tile list per chunk)
@@ -942,7 +987,12 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       k_prologue<R><<<sb.B, 256, shm, st>>>(*op, sb, i, itb + toff[i], yin, slot);
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 0, &used)) return rc;
-#define HOPS_SH(W, P) k_stage_shared<R, W, P><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q)
+#define HOPS_SH(W, P)                                                                            \
+  do {                                                                                           \
+    if (g_nt_stream == 1) k_stage_shared<R, W, P, 1><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    else if (g_nt_stream == 2) k_stage_shared<R, W, P, 2><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    else k_stage_shared<R, W, P, 0><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+  } while (0)
       if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
       else if (sh_w == 4 && sh_p == 8) HOPS_SH(4, 8);
       else if (sh_w == 8 && sh_p == 4) HOPS_SH(8, 4);
@@ -1343,6 +1393,16 @@ int hops_set_option(const char *name, int value) {
   if (std::string(name) == "chunk_traj") {
     if (value < 0) return fail("chunk_traj must be >= 0");
     g_chunk_traj = value;
+    return 0;
+  }
+  if (std::string(name) == "nt_stream") {
+    if (value < 0 || value > 2) return fail("nt_stream must be 0, 1 or 2");
+    g_nt_stream = value;
+    return 0;
+  }
+  if (std::string(name) == "groups_q") {
+    if (value < 0) return fail("groups_q must be >= 0");
+    g_groups_q = value;
     return 0;
   }
   return fail("unknown option '%s'", name);

@@ -237,6 +237,33 @@ def test_stationary_kernel_equals_streaming(sysname):
             assert d == 0.0
 
 
+def test_stationary_kernel_options_bit_identical():
+    """The nontemporal hint on the streamed arrays and the trajectory-group
+    split of the stationary kernel change caching and work distribution only:
+    trajectories are bit-identical to the default launch."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1)
+    tlen, tau = 50.0, 0.5
+    B = 70
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[0] = 1
+    for prec in ("mixed", "fp64"):
+        ref = engine_run(sysp, "NORMALIZED NONLINEAR", zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, lanes=2)
+        for nt, q in ((1, 0), (2, 0), (0, 2), (1, 5)):
+            _lib.set_option("nt_stream", nt)
+            _lib.set_option("groups_q", q)
+            try:
+                got = engine_run(sysp, "NORMALIZED NONLINEAR", zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, lanes=2)
+            finally:
+                _lib.set_option("nt_stream", 0)
+                _lib.set_option("groups_q", 0)
+            d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+            print(prec, "nt", nt, "q", q, "max diff", d)
+            assert d == 0.0
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_interleaved_lanes_equal_plain_layout(lanes):
     """Interleaving T trajectories per row changes only the memory layout:
