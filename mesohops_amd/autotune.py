@@ -15,13 +15,14 @@ import torch
 from . import _lib
 
 # (lanes, stationary stage kernel, trajectories per cache-resident chunk (0 =
-# whole batch), nontemporal hint (0 none, 1 accumulator and carry, 2 also the
-# stage outputs), trajectory-group split
+# whole batch), nontemporal hint level (0 none, 1 accumulator and carry, 2 also
+# the stage outputs, 3 also the row's own aux block, 4 also the ELL
+# neighbours), trajectory-group split
 # of the stationary kernel (0 = automatic))
 CANDIDATES = [(1, True, 0, 0, 0), (4, True, 0, 0, 0), (8, True, 0, 0, 0), (16, True, 0, 0, 0),
               (4, False, 0, 0, 0), (4, True, 64, 0, 0), (2, True, 32, 0, 0),
-              (4, True, 0, 1, 0), (8, True, 0, 1, 0), (4, True, 0, 2, 0), (8, True, 0, 2, 0), (4, True, 64, 1, 0),
-              (4, True, 0, 0, 2), (4, True, 0, 1, 2)]
+              (4, True, 0, 1, 0), (4, True, 0, 2, 0), (8, True, 0, 2, 0), (4, True, 0, 3, 0), (8, True, 0, 3, 0),
+              (4, True, 0, 4, 0), (16, True, 0, 3, 0), (4, True, 64, 2, 0), (4, True, 0, 2, 2)]
 
 
 def apply(choice):

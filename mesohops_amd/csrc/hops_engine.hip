@@ -681,8 +681,9 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
 // Row update with the row's pattern and ELL entries in registers and every
 // load issued before the arithmetic (more bytes in flight per wave).
 // pidx[p] < 0 and ecol[e] < 0 mark unused slots.  NT >= 1: the accumulator
-// and the carry are read and written with the nontemporal hint; NT = 2: the
-// stage output and the complex64 state copy are stored with it too.
+// and the carry are read and written with the nontemporal hint; NT >= 2: the
+// stage output and the complex64 state copy are stored with it too; NT >= 3:
+// the row's own aux block is read with it; NT = 4: the ELL neighbours too.
 template <typename R, int WR, int PW, int NT>
 __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_batch_t &bt, int stage,
                                               const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
@@ -702,16 +703,18 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
   double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
   const cx<R> zero = mk<R>((R)0, (R)0);
   // loads
-  const cx<R> yr = Y[r * T];
+  constexpr int NTO = NT >= 3 ? 1 : 0;  // the row's own aux block
+  constexpr int NTG = NT >= 4 ? 1 : 0;  // the ELL neighbours
+  const cx<R> yr = ld_s<NTO>(Y + r * T);
   cx<R> yp[PW], ye[WR], ap[PW];
   const cx<R> *A = cT + (k == 0 ? (long)op.n_pat * cs : 0);
 #pragma unroll
   for (int p = 0; p < PW; ++p) {
-    yp[p] = pidx[p] >= 0 ? Y[(rowk + pcol[p]) * T] : zero;
+    yp[p] = pidx[p] >= 0 ? ld_s<NTO>(Y + (rowk + pcol[p]) * T) : zero;
     ap[p] = pidx[p] >= 0 ? A[(long)pidx[p] * cs] : zero;
   }
 #pragma unroll
-  for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? Y[(long)ecol[e] * T] : zero;
+  for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? ld_s<NTG>(Y + (long)ecol[e] * T) : zero;
   const double sigma = bt.sigT_d[b];
   cx<R> base = zero, a_in = zero;
   double2 ph = make_double2(0.0, 0.0);
@@ -751,7 +754,7 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
       base = mk<R>((R)(sigma * p.x), (R)(sigma * p.y));
     }
     const R c = (R)((stage == 2 ? 1.0 : 0.5) * h);
-    st_s<NT == 2 ? 1 : 0>(yout + gi, mk<R>(base.x + c * d.x, base.y + c * d.y));
+    st_s<NT >= 2 ? 1 : 0>(yout + gi, mk<R>(base.x + c * d.x, base.y + c * d.y));
     if (stage == 0) {
       st_s<NT>(acc + gi, d);
     } else {
@@ -766,7 +769,7 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
     q.y = sigma * ph.y + c * ((double)a_in.y + (double)d.y);
     st_s<NT>(phi + gi, q);
     if (bt.precision == HOPS_PREC_MIXED)
-      st_s<NT == 2 ? 1 : 0>(reinterpret_cast<cx<float> *>(bt.y0_d) + gi, mk<float>((float)q.x, (float)q.y));
+      st_s<NT >= 2 ? 1 : 0>(reinterpret_cast<cx<float> *>(bt.y0_d) + gi, mk<float>((float)q.x, (float)q.y));
   }
 }
 
@@ -893,7 +896,7 @@ struct Profile {
 Profile g_prof;
 bool g_stationary = true;
 int g_chunk_traj = 0;
-int g_nt_stream = 0;  // nontemporal hint in k_stage_shared: 1 accumulator and carry, 2 also the outputs
+int g_nt_stream = 0;  // nontemporal hint level of k_stage_shared (see stage_row_reg)
 int g_groups_q = 0;        // trajectory-group split of k_stage_shared (0: automatic)
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
@@ -991,6 +994,8 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   do {                                                                                           \
     if (g_nt_stream == 1) k_stage_shared<R, W, P, 1><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
     else if (g_nt_stream == 2) k_stage_shared<R, W, P, 2><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    else if (g_nt_stream == 3) k_stage_shared<R, W, P, 3><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    else if (g_nt_stream == 4) k_stage_shared<R, W, P, 4><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
     else k_stage_shared<R, W, P, 0><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
   } while (0)
       if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
@@ -1396,7 +1401,7 @@ int hops_set_option(const char *name, int value) {
     return 0;
   }
   if (std::string(name) == "nt_stream") {
-    if (value < 0 || value > 2) return fail("nt_stream must be 0, 1 or 2");
+    if (value < 0 || value > 4) return fail("nt_stream must be 0..4");
     g_nt_stream = value;
     return 0;
   }

@@ -251,7 +251,7 @@ def test_stationary_kernel_options_bit_identical():
     psi0[0] = 1
     for prec in ("mixed", "fp64"):
         ref = engine_run(sysp, "NORMALIZED NONLINEAR", zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, lanes=2)
-        for nt, q in ((1, 0), (2, 0), (0, 2), (1, 5)):
+        for nt, q in ((1, 0), (2, 0), (3, 0), (4, 0), (0, 2), (1, 5)):
             _lib.set_option("nt_stream", nt)
             _lib.set_option("groups_q", q)
             try:
