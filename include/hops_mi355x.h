@@ -169,6 +169,17 @@ int hops_derivative(const hops_op_t *op, const hops_batch_t *bt, const void *y_d
 int hops_permute_c128(const void *src_d, void *dst_d, const int32_t *src_idx_d, const int32_t *dst_idx_d,
                       int n, double scale, void *stream);
 
+/* An operator on the system states applied to every auxiliary of the
+   hierarchy vector (complex128): phi_out[k, s] = sum_s' op[s, s'] phi_in[k, s']
+   for B trajectories in the interleaved layout of hops_batch_t (element
+   (b, r) at ((b >> lane_shift) * n_state * n_hier + r) * 2^lane_shift +
+   (b & (2^lane_shift - 1)), r = k * n_state + s).  op_d: dense row-major
+   [n_state][n_state] complex128 in device memory; phi_in_d and phi_out_d must
+   not overlap.  Replaces the matrix product of HopsTrajectory._operator
+   (trajectory/hops_trajectory.py:540-577). */
+int hops_apply_state_op(const void *phi_in_d, void *phi_out_d, const void *op_d, int n_state, int n_hier, int B,
+                        int lane_shift, void *stream);
+
 /* Circulant-embedding spectrum s_w[l][j] = max(0, Re FFT(embed(tilde c_l)))
    for n_lop correlation functions of n_t points (FFT length 2(n_t-1)).
    ang_d, mag_d: float32 [n_lop][n_t] phase and modulus of the complex64

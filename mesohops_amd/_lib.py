@@ -88,6 +88,7 @@ def lib():
                                  ctypes.c_int, ctypes.c_int, _vp]
     L.hops_derivative.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), _vp, _vp, ctypes.c_int, _vp]
     L.hops_permute_c128.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp]
+    L.hops_apply_state_op.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
     L.hops_noise_embedding.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
     L.hops_noise_window_mean.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
@@ -147,7 +148,7 @@ def check(rc):
 
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
-            "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128",
+            "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128", "hops_apply_state_op",
             "hops_noise_embedding", "hops_noise_filter", "hops_noise_window_mean",
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",

@@ -61,6 +61,17 @@ def _step_aux(aux, mode, step):
     return tuple((m, d[m]) for m in sorted(d))
 
 
+def aux_tuple(a):
+    """An auxiliary as this package's sorted ((mode, value), ...) tuple, from
+    such a tuple or list of pairs, a {mode: value} dict, or an object with the
+    reference AuxiliaryVector's dict_aux_vec (basis/hops_aux.py:60-66)."""
+    if hasattr(a, "dict_aux_vec"):
+        a = a.dict_aux_vec
+    if isinstance(a, dict):
+        a = a.items()
+    return tuple(sorted((int(m), int(v)) for m, v in a if int(v) != 0))
+
+
 def _depth(aux):
     return sum(v for _, v in aux)
 
@@ -265,7 +276,7 @@ class AdaptiveTrajectoryEngine:
     """One adaptive trajectory: device state + basis bookkeeping."""
 
     def __init__(self, setup, noise1, noise2, precision_mode, device, delta_a, delta_s, update_step,
-                 f_discard, early_integrator="INCH_WORM", early_steps=5, inchworm_cap=5):
+                 f_discard, early_integrator="INCH_WORM", early_steps=5, inchworm_cap=5, static_basis=None):
         self.setup = setup
         self.info = setup.info
         self.noise1, self.noise2 = noise1, noise2
@@ -274,9 +285,17 @@ class AdaptiveTrajectoryEngine:
         self.delta_a, self.delta_s = float(delta_a), float(delta_s)
         self.update_step = int(update_step) if update_step else 1
         self.f_discard = float(f_discard)
-        if early_integrator not in ("INCH_WORM", "STATIC"):
+        if early_integrator not in ("INCH_WORM", "STATIC", "STATIC_STATE_INCHWORM_HIERARCHY"):
             raise UnsupportedRequest(early_integrator, "early time integrator clause of the propagate")
         self.early_integrator = early_integrator
+        # STATIC_BASIS: [state list, auxiliary list] kept in the basis from the
+        # start (trajectory/hops_trajectory.py:303-317)
+        self.static_basis = None
+        if static_basis is not None:
+            self.static_basis = ([int(x) for x in static_basis[0]], [aux_tuple(a) for a in static_basis[1]])
+        if early_integrator == "STATIC_STATE_INCHWORM_HIERARCHY" and self.static_basis is None:
+            raise UnsupportedRequest("STATIC_STATE_INCHWORM_HIERARCHY without a STATIC_BASIS",
+                                     "early time integrator clause of the propagate")
         self.early_steps, self.inchworm_cap = int(early_steps), int(inchworm_cap)
         self.maxhier = setup.hier["MAXHIER"]
         self.markov = None
@@ -426,7 +445,11 @@ class AdaptiveTrajectoryEngine:
         phi = np.zeros(tb.n_hier * tb.n_state, complex)
         phi[: tb.n_state] = psi_0[tb.state_list]
         self.st.set_phi(phi)
-        st, ax = self.define_basis(phi, 1.0, t0_index)
+        if self.static_basis is None:
+            st, ax = self.define_basis(phi, 1.0, t0_index)
+        else:
+            st = sorted(set(int(s) for s in tb.state_list) | set(self.static_basis[0]))
+            ax = list(set(tb.aux_list) | set(self.static_basis[1]))
         self.update_basis(phi, st, ax)
 
     def _rk(self, it, tau, ns):
@@ -441,14 +464,18 @@ class AdaptiveTrajectoryEngine:
         """One adaptive RK4 step from noise index ``it``
         (trajectory/hops_trajectory.py:456-533, :636-714)."""
         early = self.early_counter < self.early_steps
-        if early and self.early_integrator == "INCH_WORM":
+        inchworm = self.early_integrator in ("INCH_WORM", "STATIC_STATE_INCHWORM_HIERARCHY")
+        static_states = self.early_integrator == "STATIC_STATE_INCHWORM_HIERARCHY"
+        if early and inchworm:
             phi_old = self.phi_host()
             zmem_old = self.st.zmem.clone()
         self._rk(it, tau, ns)
         if early:
-            if self.early_integrator == "INCH_WORM":
+            if inchworm:
                 phi = self.phi_host()
                 s_new, a_new = self.define_basis(phi, tau, it)
+                if static_states:
+                    s_new = list(self.static_basis[0])
                 k = 0
                 while not self._same_basis(s_new, a_new):
                     # union basis, then the step again from the state before it
@@ -462,6 +489,8 @@ class AdaptiveTrajectoryEngine:
                     self._rk(it, tau, ns)
                     phi = self.phi_host()
                     s_new, a_new = self.define_basis(phi, tau, it)
+                    if static_states:
+                        s_new = list(self.static_basis[0])
                     k += 1
                     if k >= self.inchworm_cap:
                         break
@@ -472,3 +501,28 @@ class AdaptiveTrajectoryEngine:
             s_new, a_new = self.define_basis(phi, tau, it)
             self.update_basis(phi, s_new, a_new)
         self._sync_zmem_from_state()
+
+    # -- operators -----------------------------------------------------------
+    def apply_operator(self, op):
+        """HopsTrajectory._operator (trajectory/hops_trajectory.py:540-577):
+        with state adaptivity the states that op populates join the basis
+        first; op, restricted to the state basis, acts on every auxiliary on
+        the device; then the early-time integrator starts over.  The
+        reference's define_basis / update_basis calls after the product keep
+        the basis as it is."""
+        import scipy.sparse as sps
+        sparse = sps.issparse(op)
+        if sparse:
+            op = sps.csr_matrix(op)
+        if self.delta_s > 0:
+            cur = [int(s) for s in self.tb.state_list]
+            block = op[:, cur]
+            rows = block.nonzero()[0] if sparse else np.nonzero(np.asarray(block))[0]
+            states = sorted(set(cur) | set(int(r) for r in rows))
+            self.update_basis(self.phi_host(), states, list(self.tb.aux_list))
+        sl = [int(s) for s in self.tb.state_list]
+        sub = op[sl][:, sl]
+        sub = sub.toarray() if sparse else np.asarray(sub)
+        self.st.apply_state_op(sub)
+        if self.delta_s > 0:
+            self.early_counter = 0

@@ -830,6 +830,29 @@ __global__ __launch_bounds__(256) void k_permute(const double2 *__restrict__ src
   dst[di[i]] = make_double2(scale * v.x, scale * v.y);
 }
 
+// phi_out[b, k, s] = sum_s' op[s, s'] phi_in[b, k, s'] (HopsTrajectory._operator,
+// trajectory/hops_trajectory.py:560-562); one thread per output element.
+__global__ __launch_bounds__(256) void k_state_op(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                  const double2 *__restrict__ op, int n, int nh, int B,
+                                                  int lane_shift) {
+  const long nr = (long)n * nh;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr * B) return;
+  const int b = (int)(i / nr);
+  const long r = i - (long)b * nr;
+  const int k = (int)(r / n), s = (int)(r - (long)k * n);
+  const int T = 1 << lane_shift;
+  const long base = (long)(b >> lane_shift) * nr * T + (b & (T - 1));
+  const double2 *row = in + base + (long)k * n * T;
+  double re = 0.0, im = 0.0;
+  for (int q = 0; q < n; ++q) {
+    const double2 a = op[(long)s * n + q], v = row[(long)q * T];
+    re += a.x * v.x - a.y * v.y;
+    im += a.x * v.y + a.y * v.x;
+  }
+  out[base + r * T] = make_double2(re, im);
+}
+
 // y0 = complex64(Phi) at the start of a mixed-precision propagate call.
 __global__ __launch_bounds__(256) void k_prime_y0(const double2 *__restrict__ phi,
                                                   float2 *__restrict__ y0, long n) {
@@ -1469,6 +1492,21 @@ int hops_permute_c128(const void *src_d, void *dst_d, const int32_t *src_idx_d, 
   k_permute<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
       reinterpret_cast<const double2 *>(src_d), reinterpret_cast<double2 *>(dst_d), src_idx_d, dst_idx_d, n,
       scale);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_apply_state_op(const void *phi_in_d, void *phi_out_d, const void *op_d, int n_state, int n_hier, int B,
+                        int lane_shift, void *stream) {
+  if (!phi_in_d || !phi_out_d || !op_d) return fail("hops_apply_state_op: NULL argument");
+  if (n_state <= 0 || n_hier <= 0 || B <= 0 || lane_shift < 0 || lane_shift > 6)
+    return fail("hops_apply_state_op: bad sizes (n_state=%d n_hier=%d B=%d lane_shift=%d)", n_state, n_hier, B,
+                lane_shift);
+  if (phi_in_d == phi_out_d) return fail("hops_apply_state_op: input and output must differ");
+  const long tot = (long)n_state * n_hier * B;
+  k_state_op<<<(unsigned)((tot + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      reinterpret_cast<const double2 *>(phi_in_d), reinterpret_cast<double2 *>(phi_out_d),
+      reinterpret_cast<const double2 *>(op_d), n_state, n_hier, B, lane_shift);
   HIP_TRY(hipGetLastError());
   return 0;
 }

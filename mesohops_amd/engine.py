@@ -197,6 +197,19 @@ class BatchState:
         nr, T = self.dop.n_rows, self.lanes
         return flat.reshape(self.G, nr, T).transpose(0, 2, 1).reshape(self.G * T, nr)[: self.B]
 
+    def apply_state_op(self, op):
+        """phi[b, k, :] <- op @ phi[b, k, :] on the device for every trajectory
+        and auxiliary (op: dense [n_state, n_state], the operator restricted to
+        the state basis)."""
+        n = self.dop.tables.n_state
+        opd = torch.from_numpy(np.ascontiguousarray(op, dtype=np.complex128).reshape(n, n)).to(self.device)
+        out = torch.zeros_like(self.phi)
+        check(lib().hops_apply_state_op(ctypes.c_void_p(self.phi.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                        ctypes.c_void_p(opd.data_ptr()), n, self.dop.tables.n_hier, self.B,
+                                        int(self.lanes).bit_length() - 1, _stream_ptr(self.device)))
+        self.phi.copy_(out)  # same buffer: the batch descriptor keeps its pointer
+        torch.cuda.synchronize(self.device)
+
     def set_phi(self, phi_host):
         """phi_host: complex [B, n_rows] (or [n_rows], broadcast)."""
         nr = self.dop.n_rows

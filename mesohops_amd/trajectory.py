@@ -363,8 +363,6 @@ class HopsTrajectory:
         self.noise1 = self._batch.noise1[0]
         self.noise2 = self._batch.noise2[0]
         self.integration_param = self._batch.setup.integ
-        if self.integration_param["STATIC_BASIS"] is not None:
-            raise UnsupportedRequest("STATIC_BASIS", "HopsTrajectory")
         self.storage = HopsStorage(self._batch.setup.eom["ADAPTIVE"], storage_param or {})
         self._adaptive = None
         self.__initialized__ = False
@@ -411,7 +409,7 @@ class HopsTrajectory:
                 b.setup, self.noise1, self.noise2, b.precision_mode, b.device,
                 e["DELTA_A"] if e["ADAPTIVE_H"] else 0.0, e["DELTA_S"] if e["ADAPTIVE_S"] else 0.0,
                 e["UPDATE_STEP"], e["F_DISCARD"], ip["EARLY_ADAPTIVE_INTEGRATOR"],
-                ip["EARLY_INTEGRATOR_STEPS"], ip["INCHWORM_CAP"])
+                ip["EARLY_INTEGRATOR_STEPS"], ip["INCHWORM_CAP"], ip["STATIC_BASIS"])
             self._adaptive.initialize(psi_0, self.noise1.time_index(0.0))
             self._t = 0.0
         else:
@@ -451,6 +449,38 @@ class HopsTrajectory:
                 if self.storage.check_storage_time(t):
                     self._store(t, psi, self.auxiliary_list, list(self.state_list), None)
         self.storage.metadata["LIST_PROPAGATION_TIME"].append(time.time() - timer_checkpoint)
+
+    def _operator(self, op):
+        """Acts with a system operator (dense or SciPy sparse, over all
+        states) on every auxiliary of the hierarchy vector
+        (trajectory/hops_trajectory.py:540-577).  With state adaptivity the
+        states op populates are added to the basis first and the early-time
+        integrator starts over."""
+        if not self.__initialized__:
+            raise TrajectoryError("_operator called before initialize")
+        import scipy.sparse as sps
+        if self.adaptive:
+            self._adaptive.apply_operator(op)
+            return
+        full = op.toarray() if sps.issparse(op) else np.asarray(op)
+        self._batch.state.apply_state_op(full)
+
+    def reset_early_time_integrator(self):
+        """The next propagate starts with EARLY_INTEGRATOR_STEPS early-time
+        steps again (trajectory/hops_trajectory.py:761-772)."""
+        if self.early_integrator != "INCH_WORM":
+            raise UnsupportedRequest("Early type integrators of type other than INCH_WORM",
+                                     "reset_early_time_integrator", override=True)
+        if self._adaptive is not None:
+            self._adaptive.early_counter = 0
+
+    @property
+    def early_integrator(self):
+        return self.integration_param["EARLY_ADAPTIVE_INTEGRATOR"]
+
+    @property
+    def static_basis(self):
+        return self.integration_param["STATIC_BASIS"]
 
     def _plan_adaptive(self, t_advance, tau):
         b = self._batch

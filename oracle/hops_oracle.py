@@ -629,6 +629,27 @@ class Trajectory:
         self.t_axis.append(t)
         self.z_mem_traj.append(self.z_mem.copy())
 
+    def operator(self, op):
+        """HopsTrajectory._operator (trajectory/hops_trajectory.py:540-577):
+        populated states join the basis (state adaptivity), op restricted to
+        the state basis multiplies the (n_state x n_hier) hierarchy matrix, and
+        the early-time integrator is reset; the reference's define_basis /
+        update_basis pair after the product leaves the basis unchanged."""
+        import scipy.sparse as sps
+        if sps.issparse(op):
+            op = op.tocsr()
+        adaptive_s = getattr(self, "ad", None) is not None and self.ad.adaptive_s
+        if adaptive_s:
+            sl = [int(s) for s in self.basis.state_list]
+            new = list(sl) + list(np.nonzero(op[:, sl])[0])
+            self.phi = self.update_basis(self.phi, list(set(int(s) for s in new)), list(self.basis.aux_list))
+        sl = [int(s) for s in self.basis.state_list]
+        opr = op[np.ix_(sl, sl)]
+        phi_mat = np.reshape(self.phi, [self.basis.n_state, self.basis.n_hier], order="F")
+        self.phi = np.reshape(opr @ phi_mat, len(self.phi), order="F")
+        if adaptive_s:
+            self.early_counter = 0
+
     def noise_block(self, t, tau):
         lops = list(self.basis.absL)
         if self.effective:
@@ -949,13 +970,16 @@ class AdaptiveTrajectory(Trajectory):
     :456-533, :636-714) for site-projector L-operators."""
 
     def __init__(self, *args, delta_a=1e-4, delta_s=1e-4, update_step=1, f_discard=0.01,
-                 early_steps=5, inchworm_cap=5, markov=None, **kw):
+                 early_steps=5, inchworm_cap=5, markov=None, early_integrator="INCH_WORM", static_basis=None,
+                 **kw):
         super().__init__(*args, **kw)
         self.ad = AdaptiveState(self.sysd, self.eom_type, self.maxhier, delta_a, delta_s, f_discard,
                                 markov=markov)
         self.update_step = update_step
         self.early_steps = early_steps
         self.inchworm_cap = inchworm_cap
+        self.early_integrator = early_integrator  # INCH_WORM, STATIC_STATE_INCHWORM_HIERARCHY or STATIC
+        self.static_basis = static_basis          # [states, aux] (hops_trajectory.py:303-317)
         self.early_counter = 0
         self.state_traj = []
         self.aux_traj = []
@@ -998,7 +1022,11 @@ class AdaptiveTrajectory(Trajectory):
         phi[:b.n_state] = psi_0[b.state_list]
         self.z_mem = np.zeros(len(self.sysd["L_NOISE1"]), complex)
         self.t = 0.0
-        st, ax = self.ad.define_basis(self.dsystem_dt, b, phi, 1, self.z_step(self.z_mem))
+        if self.static_basis is None:
+            st, ax = self.ad.define_basis(self.dsystem_dt, b, phi, 1, self.z_step(self.z_mem))
+        else:  # hops_trajectory.py:303-317
+            st = list(set(int(s) for s in b.state_list) | set(int(s) for s in self.static_basis[0]))
+            ax = list(set(b.aux_list) | set(tuple(a) for a in self.static_basis[1]))
         self.phi = self.update_basis(phi, st, ax)
         self._store(0.0)
 
@@ -1025,20 +1053,27 @@ class AdaptiveTrajectory(Trajectory):
         for index_t, t in enumerate(t_axis):
             phi, z_mem = self._rk(self.phi, self.z_mem, tau)
             if self.early_counter < self.early_steps:
-                st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
-                k = 0
-                while (set(st) != set(int(s) for s in self.basis.state_list)
-                       or set(ax) != set(self.basis.aux_list)):
-                    # inchworm: union basis, redo the step from the old phi
-                    st_u = sorted(set(st) | set(int(s) for s in self.basis.state_list))
-                    ax_u = list(set(ax) | set(self.basis.aux_list))
-                    self.phi = self.update_basis(self.phi, st_u, ax_u)
-                    phi, z_mem = self._rk(self.phi, self.z_mem, tau)
+                # hops_trajectory.py:469-509
+                if self.early_integrator in ("INCH_WORM", "STATIC_STATE_INCHWORM_HIERARCHY"):
+                    static_states = self.early_integrator == "STATIC_STATE_INCHWORM_HIERARCHY"
                     st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
-                    k += 1
-                    if k >= self.inchworm_cap:
-                        break
-                phi = self.update_basis(phi, st, ax)
+                    if static_states:
+                        st = list(self.static_basis[0])
+                    k = 0
+                    while (set(st) != set(int(s) for s in self.basis.state_list)
+                           or set(ax) != set(self.basis.aux_list)):
+                        # inchworm: union basis, redo the step from the old phi
+                        st_u = sorted(set(st) | set(int(s) for s in self.basis.state_list))
+                        ax_u = list(set(ax) | set(self.basis.aux_list))
+                        self.phi = self.update_basis(self.phi, st_u, ax_u)
+                        phi, z_mem = self._rk(self.phi, self.z_mem, tau)
+                        st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
+                        if static_states:
+                            st = list(self.static_basis[0])
+                        k += 1
+                        if k >= self.inchworm_cap:
+                            break
+                    phi = self.update_basis(phi, st, ax)
                 self.early_counter += 1
             elif (index_t + 1) % self.update_step == 0:
                 st, ax = self.ad.define_basis(self.dsystem_dt, self.basis, phi, tau, self.z_step(z_mem))
@@ -1054,3 +1089,7 @@ class AdaptiveTrajectory(Trajectory):
             self.aux_traj.append(list(b.aux_list))
             self.z_mem = z_mem
             self.t = float(t)
+
+    def reset_early_time_integrator(self):
+        """hops_trajectory.py:761-772."""
+        self.early_counter = 0

@@ -143,3 +143,80 @@ def test_adaptive_trajectory_matches_oracle(name):
     assert err < FP64_TOL
     assert np.max(np.abs(tr.phi - ref.phi)) < FP64_TOL
     assert len(sizes) > 1   # the basis moved during the run
+
+
+@pytest.mark.parametrize("early", ["INCH_WORM", "STATIC_STATE_INCHWORM_HIERARCHY"])
+def test_static_basis_matches_oracle(early):
+    """STATIC_BASIS and the STATIC_STATE_INCHWORM_HIERARCHY early integrator
+    (trajectory/hops_trajectory.py:303-317, :469-509): basis sequence equal to
+    the oracle's at every step and psi(t) to 1e-10."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.chain(6)
+    tlen, tau = 60.0, 0.5
+    z = _noise(sysp, 5, tlen, tau)
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+    static = [[1, 2, 3], [((4, 1),), ((5, 1),), ((4, 1), (5, 1))]]
+    ref = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 3}, delta_a=1e-3, delta_s=1e-3,
+                               early_integrator=early, static_basis=static)
+    ref.initialize(psi0)
+    ref.propagate(30.0, 1.0)
+    tr = HopsTrajectory(sysp, eom, dict(noise), hierarchy_param={"MAXHIER": 3},
+                        integration_param={"STATIC_BASIS": static, "EARLY_ADAPTIVE_INTEGRATOR": early})
+    tr.make_adaptive(1e-3, 1e-3)
+    tr.initialize(psi0)
+    tr.propagate(30.0, 1.0)
+    for i, (s_got, s_ref) in enumerate(zip(tr.storage["state_list"], ref.state_traj)):
+        assert list(s_got) == list(s_ref), (i, s_got, s_ref)
+        assert tr.storage["list_nhier"][i] == len(ref.aux_traj[i]), i
+    assert set(tr.auxiliary_list) == set(ref.basis.aux_list)
+    err = np.max(np.abs(tr.storage["psi_traj"] - np.array(ref.psi_traj)))
+    print("static basis", early, "max |dpsi| = %.3e" % err)
+    assert err < FP64_TOL
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_operator_matches_oracle(adaptive):
+    """HopsTrajectory._operator between two propagate calls
+    (trajectory/hops_trajectory.py:540-577): with state adaptivity the
+    populated states join the basis and the early integrator restarts; psi(t)
+    and the basis sequence follow the oracle to 1e-10."""
+    _need_gpu()
+    import scipy.sparse as sp
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.chain(6)
+    tlen, tau = 60.0, 0.5
+    z = _noise(sysp, 8, tlen, tau)
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+    op = sp.csr_matrix(([1.0, 1.0, 0.3j], ([2, 5, 0], [5, 2, 2])), shape=(6, 6)) + sp.eye(6) * 0.5
+    if adaptive:
+        ref = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2}, delta_a=1e-3, delta_s=1e-3)
+    else:
+        ref = O.Trajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2})
+    ref.initialize(psi0)
+    ref.propagate(10.0, 1.0)
+    ref.operator(op)
+    ref.propagate(10.0, 1.0)
+    tr = HopsTrajectory(sysp, eom, dict(noise), hierarchy_param={"MAXHIER": 2})
+    if adaptive:
+        tr.make_adaptive(1e-3, 1e-3)
+    tr.initialize(psi0)
+    tr.propagate(10.0, 1.0)
+    tr._operator(op)
+    if adaptive:
+        assert 5 in tr.state_list and 0 in tr.state_list
+    tr.propagate(10.0, 1.0)
+    psi = np.array(tr.storage["psi_traj"])
+    err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+    print("operator adaptive=%s max |dpsi| = %.3e" % (adaptive, err))
+    assert err < FP64_TOL
+    if adaptive:
+        for i, (s_got, s_ref) in enumerate(zip(tr.storage["state_list"], ref.state_traj)):
+            assert list(s_got) == list(s_ref), (i, s_got, s_ref)
+    assert np.max(np.abs(tr.phi.reshape(-1) - ref.phi)) < FP64_TOL

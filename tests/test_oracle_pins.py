@@ -317,3 +317,74 @@ def test_effective_noise_matches_reference_hand_values():
     assert np.allclose(got, want)
     # the point samples of the plain integrator are columns 0, 3, 6 of the same window
     assert np.allclose(nz.get_noise([5.0, 5.75, 6.5], [0, 1]), known[:, [0, 3, 6]])
+
+
+# ---------------------------------------------------------------------------
+# _operator and STATIC_BASIS (trajectory/hops_trajectory.py:303-317, 540-577)
+# ---------------------------------------------------------------------------
+def _blockwise(op_sub, phi, n):
+    out = phi.copy()
+    for k in range(len(phi) // n):
+        out[k * n:(k + 1) * n] = op_sub @ phi[k * n:(k + 1) * n]
+    return out
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_operator_acts_on_every_auxiliary(sparse):
+    """The oracle's _operator equals op applied aux block by aux block (a
+    reshape in the wrong order would mix auxiliaries), for a plain and for a
+    state-adaptive trajectory (populated states join the basis, with zero
+    amplitude before the product)."""
+    sysp = systems.chain(6)
+    noise = dict(SEED=3, MODEL="FFT_FILTER", TLEN=100.0, TAU=0.5)
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+    rng = np.random.default_rng(4)
+    op = rng.normal(size=(6, 6)) + 1j * rng.normal(size=(6, 6))
+    op[np.abs(op) < 1.0] = 0
+    op[5, :] = 0
+    op[5, 2] = 0.5  # populates state 5 from state 2
+    if sparse:
+        op = sp.csr_matrix(op)
+    dense = op.toarray() if sparse else op
+    tr = O.Trajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2})
+    tr.initialize(psi0)
+    tr.propagate(10.0, 1.0)
+    before = tr.phi.copy()
+    tr.operator(op)
+    np.testing.assert_allclose(tr.phi, _blockwise(dense, before, 6), rtol=1e-13, atol=1e-15)
+    ad = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2}, delta_a=1e-3, delta_s=1e-3)
+    ad.initialize(psi0)
+    ad.propagate(10.0, 1.0)
+    old_states = [int(s) for s in ad.basis.state_list]
+    assert 5 not in old_states
+    full_before = {(a, int(s)): ad.phi[i * ad.basis.n_state + j]
+                   for i, a in enumerate(ad.basis.aux_list) for j, s in enumerate(ad.basis.state_list)}
+    ad.early_counter = 7
+    ad.operator(op)
+    st = [int(s) for s in ad.basis.state_list]
+    assert 5 in st and set(old_states) <= set(st)
+    n = len(st)
+    vec = np.array([full_before.get((a, s), 0) for a in ad.basis.aux_list for s in st])
+    np.testing.assert_allclose(ad.phi, _blockwise(dense[np.ix_(st, st)], vec, n), rtol=1e-13, atol=1e-15)
+    assert ad.early_counter == 0
+
+
+def test_static_basis_is_kept_from_the_start():
+    """STATIC_BASIS joins the initial basis without a define_basis pass, and
+    STATIC_STATE_INCHWORM_HIERARCHY holds the state basis at the static states
+    through the early steps."""
+    sysp = systems.chain(6)
+    noise = dict(SEED=3, MODEL="FFT_FILTER", TLEN=100.0, TAU=0.5)
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+    static = [[1, 2, 3], [((4, 1),), ((5, 1),)]]
+    tr = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 3}, delta_a=1e-3, delta_s=1e-3,
+                              early_integrator="STATIC_STATE_INCHWORM_HIERARCHY", static_basis=static)
+    tr.initialize(psi0)
+    assert [int(s) for s in tr.basis.state_list] == [1, 2, 3]
+    assert set(tr.basis.aux_list) == {(), ((4, 1),), ((5, 1),)}
+    tr.propagate(5.0, 1.0)
+    assert all(s == [1, 2, 3] for s in tr.state_traj[1:6])
