@@ -104,7 +104,11 @@ typedef struct hops_batch {
   int32_t coef_stride, ws_stride;  /* per-trajectory doubles            */
   int32_t store_stride, lane_shift; /* psi slots per trajectory; log2 T */
   double dt_over_hbar;             /* tau / hbar                        */
-  int32_t noise_step, pad1;        /* noise points per tau/2            */
+  int32_t noise_step;              /* noise points per tau/2            */
+  int32_t sigma_raw;               /* 1: the first step of the call takes
+                                      Phi as it is (sigma = 1) instead of
+                                      normalising it, as the reference steps
+                                      a state an operator has just changed */
   void *phi_d;   /* complex128 [G*T*n_rows] state carry                 */
   void *y0_d;    /* complex64 [G*T*n_rows] (mixed only)                 */
   void *ya_d;    /* stage input ping   (complex R)                      */

@@ -45,7 +45,7 @@ class HopsOp(ctypes.Structure):
 class HopsBatch(ctypes.Structure):
     _fields_ = [("B", _i32), ("precision", _i32), ("eom", _i32), ("n_t", _i32),
                 ("coef_stride", _i32), ("ws_stride", _i32), ("store_stride", _i32), ("lane_shift", _i32),
-                ("dt_over_hbar", ctypes.c_double), ("noise_step", _i32), ("pad1", _i32),
+                ("dt_over_hbar", ctypes.c_double), ("noise_step", _i32), ("sigma_raw", _i32),
                 ("phi_d", _vp), ("y0_d", _vp), ("ya_d", _vp), ("yb_d", _vp), ("acc_d", _vp),
                 ("z1_d", _vp), ("z2_d", _vp), ("zmem_d", _vp), ("ws_d", _vp), ("coef_d", _vp),
                 ("psi_store_d", _vp), ("tile_d", _vp), ("n_tiles", _i32), ("tile_rows", _i32),

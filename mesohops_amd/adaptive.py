@@ -307,6 +307,7 @@ class AdaptiveTrajectoryEngine:
         self.selector = BasisSelector(self.info, self.delta_a, self.delta_s, self.maxhier, self.f_discard,
                                       self.markov)
         self.early_counter = 0
+        self.raw_pending = False  # the next step takes Phi as an operator left it
         self.sp = _stream_ptr(self.device)
         # noise of every hierarchy L-operator; a basis takes its active rows
         from .noise import device_noise
@@ -454,6 +455,7 @@ class AdaptiveTrajectoryEngine:
 
     def _rk(self, it, tau, ns):
         self._use_effective_noise(ns)
+        self.st.raw_first = self.raw_pending
         self.st.propagate(it, 1, tau, ns, 0)
 
     def _same_basis(self, states, aux):
@@ -500,6 +502,7 @@ class AdaptiveTrajectoryEngine:
             phi = self.phi_host()
             s_new, a_new = self.define_basis(phi, tau, it)
             self.update_basis(phi, s_new, a_new)
+        self.raw_pending = False
         self._sync_zmem_from_state()
 
     # -- operators -----------------------------------------------------------
@@ -524,5 +527,6 @@ class AdaptiveTrajectoryEngine:
         sub = op[sl][:, sl]
         sub = sub.toarray() if sparse else np.asarray(sub)
         self.st.apply_state_op(sub)
+        self.raw_pending = True
         if self.delta_s > 0:
             self.early_counter = 0

@@ -337,7 +337,7 @@ __device__ __forceinline__ void prologue_body(const hops_op_t &op, const hops_ba
       acc.x += v.x * v.x + v.y * v.y;
     }
     acc = block_sum(acc, red);
-    sigma = normed ? 1.0 / sqrt(acc.x) : 1.0;
+    sigma = normed && !bt.sigma_raw ? 1.0 / sqrt(acc.x) : 1.0;
     for (int s = tid; s < n; s += nt) {
       const double2 v = phi[(long)s * T];
       psi[2 * s] = sigma * v.x;
@@ -1523,10 +1523,21 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_s
   if (bt->ws_stride < 4 * op->n_mode_abs + 1) return fail("ws_stride too small");
   if (!bt->coefT_d || !bt->sigT_d || bt->coefT_stride < bt->B) return fail("stage coefficient table missing or too small");
   if (n_steps < 0) return fail("n_steps < 0");
+  if (bt->precision != HOPS_PREC_FP64 && bt->precision != HOPS_PREC_MIXED)
+    return fail("unknown precision %d", bt->precision);
   hipStream_t st = (hipStream_t)stream;
-  if (bt->precision == HOPS_PREC_FP64) return propagate_t<double>(op, bt, it0, n_steps, store0, st);
-  if (bt->precision == HOPS_PREC_MIXED) return propagate_t<float>(op, bt, it0, n_steps, store0, st);
-  return fail("unknown precision %d", bt->precision);
+  auto run = [&](const hops_batch_t *b, int i0, int n, int s0) {
+    return b->precision == HOPS_PREC_FP64 ? propagate_t<double>(op, b, i0, n, s0, st)
+                                          : propagate_t<float>(op, b, i0, n, s0, st);
+  };
+  if (bt->sigma_raw && n_steps > 1) {
+    // the raw first step alone, then the rest normalised as usual
+    if (int rc = run(bt, it0, 1, store0)) return rc;
+    hops_batch_t rest = *bt;
+    rest.sigma_raw = 0;
+    return run(&rest, it0 + 2 * bt->noise_step, n_steps - 1, store0 + 1);
+  }
+  return run(bt, it0, n_steps, store0);
 }
 
 

@@ -125,6 +125,7 @@ class BatchState:
         self.device = dop.device
         t = dop.tables
         self.B = B
+        self.raw_first = False
         self.eom_kind = eom_kind
         nr = dop.n_rows
         prec = dop.precision
@@ -209,6 +210,7 @@ class BatchState:
                                         int(self.lanes).bit_length() - 1, _stream_ptr(self.device)))
         self.phi.copy_(out)  # same buffer: the batch descriptor keeps its pointer
         torch.cuda.synchronize(self.device)
+        self.raw_first = True
 
     def set_phi(self, phi_host):
         """phi_host: complex [B, n_rows] (or [n_rows], broadcast)."""
@@ -234,6 +236,14 @@ class BatchState:
         self.bt.noise_step = int(noise_step)
         if self.bt.psi_store_d and store0 + n_steps > self.store_slots:
             raise EngineError("psi store too small (%d + %d > %d)" % (store0, n_steps, self.store_slots))
-        with torch.cuda.device(self.device):
-            check(lib().hops_propagate(ctypes.byref(self.dop.op), ctypes.byref(self.bt), int(it0),
-                                       int(n_steps), int(store0), _stream_ptr(self.device)))
+        # raw_first: the first step takes Phi as an operator left it (no
+        # normalisation before the step), as the reference steps it
+        self.bt.sigma_raw = 1 if self.raw_first else 0
+        try:
+            with torch.cuda.device(self.device):
+                check(lib().hops_propagate(ctypes.byref(self.dop.op), ctypes.byref(self.bt), int(it0),
+                                           int(n_steps), int(store0), _stream_ptr(self.device)))
+        finally:
+            self.bt.sigma_raw = 0
+            if n_steps > 0:
+                self.raw_first = False
