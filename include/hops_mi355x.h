@@ -114,8 +114,9 @@ typedef struct hops_batch {
   void *ya_d;    /* stage input ping   (complex R)                      */
   void *yb_d;    /* stage input pong   (complex R)                      */
   void *acc_d;   /* RK4 accumulator    (complex R)                      */
-  const void *z1_d; /* complex64 [B][n_act][n_t] noise 1 (active rows)  */
-  const void *z2_d; /* complex64 [B][n_act][n_t] noise 2, may be NULL   */
+  const void *z1_d; /* complex64 (or complex128, noise_c128) [B][n_act][n_t]
+                       noise 1 (active rows)                              */
+  const void *z2_d; /* same layout, noise 2, may be NULL                  */
   double *zmem_d;   /* complex128 [B][n_mode_abs] noise memory          */
   double *ws_d;     /* per-trajectory workspace, ws_stride doubles      */
   double *coef_d;   /* per-trajectory stage coefficients                */
@@ -127,7 +128,8 @@ typedef struct hops_batch {
      sigT_d: double [coefT_stride] scale of Phi in the current step */
   void *coefT_d;
   double *sigT_d;
-  int32_t coefT_stride, pad2;
+  int32_t coefT_stride;
+  int32_t noise_c128; /* 1: z1_d / z2_d hold complex128 instead of complex64 */
 } hops_batch_t;
 
 const char *hops_last_error(void);
@@ -200,6 +202,21 @@ int hops_noise_embedding(const float *ang_d, const float *mag_d, int n_lop, int 
 int hops_noise_filter(const void *z_d, int R, int n_t, const double *s_w_d,
                       const double *nu_d, const int32_t *row_lop_d, void *out_d,
                       void *stream);
+
+/* The same filter with the complex128 values before the complex64 cast
+   (out_d: complex128 [R][n_t]): the reference builds its cubic interpolation
+   on these (noise/hops_noise.py:236-243). */
+int hops_noise_filter_c128(const void *z_d, int R, int n_t, const double *s_w_d, const double *nu_d,
+                           const int32_t *row_lop_d, void *out_d, void *stream);
+
+/* Cubic interpolation of R complex128 rows y[n] given on the grid i*h, with
+   not-a-knot ends (scipy.interpolate.interp1d(kind="cubic"), which the
+   reference builds for INTERPOLATE = True, noise/hops_noise.py:242-243,
+   259-261, 379-385), evaluated at t0 + j*dt for j < n_out into out_d
+   (complex128 [R][n_out]).  cp_d: double [>= n - 4] pivots c_0 = 1/4,
+   c_k = 1/(4 - c_{k-1}); work_d: complex128 [R][n]; n >= 4. */
+int hops_noise_spline(const void *y_d, int R, int n, double h, double t0, double dt, int n_out,
+                      const double *cp_d, void *work_d, void *out_d, void *stream);
 
 /* Moving average of the noise for EFFECTIVE_NOISE_INTEGRATION
    (integrator/integrator_rk.py:115-128): out[r][i] = mean(z[r][i .. i+w)),

@@ -49,7 +49,7 @@ class HopsBatch(ctypes.Structure):
                 ("phi_d", _vp), ("y0_d", _vp), ("ya_d", _vp), ("yb_d", _vp), ("acc_d", _vp),
                 ("z1_d", _vp), ("z2_d", _vp), ("zmem_d", _vp), ("ws_d", _vp), ("coef_d", _vp),
                 ("psi_store_d", _vp), ("tile_d", _vp), ("n_tiles", _i32), ("tile_rows", _i32),
-                ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("pad2", _i32)]
+                ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("noise_c128", _i32)]
 
 
 class RtSystemDesc(ctypes.Structure):
@@ -91,6 +91,9 @@ def lib():
     L.hops_apply_state_op.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
     L.hops_noise_embedding.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.hops_noise_filter.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+    L.hops_noise_filter_c128.argtypes = L.hops_noise_filter.argtypes
+    L.hops_noise_spline.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, _vp]
     L.hops_noise_window_mean.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
     L.hops_profile_enable.argtypes = [ctypes.c_int]
     L.hops_set_option.argtypes = [ctypes.c_char_p, ctypes.c_int]
@@ -149,7 +152,7 @@ def check(rc):
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
             "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128", "hops_apply_state_op",
-            "hops_noise_embedding", "hops_noise_filter", "hops_noise_window_mean",
+            "hops_noise_embedding", "hops_noise_filter", "hops_noise_filter_c128", "hops_noise_spline", "hops_noise_window_mean",
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
             "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",

@@ -311,10 +311,16 @@ class AdaptiveTrajectoryEngine:
         self.sp = _stream_ptr(self.device)
         # noise of every hierarchy L-operator; a basis takes its active rows
         from .noise import device_noise
+        if noise1.interpolate or noise2.interpolate:
+            raise UnsupportedRequest("INTERPOLATE noise with an adaptive basis", "HopsTrajectory")
         nl = self.info.n_l2
-        self.z1_all = device_noise([noise1], list(range(nl)), self.device, self.sp)[0]
+        c128 = noise1.needs_c128() or noise2.needs_c128()  # PRE_CALCULATED complex128, used as given
+        if c128 and setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
+            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION with complex128 PRE_CALCULATED noise",
+                                     "HopsTrajectory")
+        self.z1_all = device_noise([noise1], list(range(nl)), self.device, self.sp, c128=c128)[0]
         self.z2_all = None if noise2.model == "ZERO" else device_noise([noise2], list(range(nl)), self.device,
-                                                                       self.sp)[0]
+                                                                       self.sp, c128=c128)[0]
         self.zmem = torch.zeros(self.info.n_hmodes + 1, dtype=torch.complex128, device=self.device)
         # EFFECTIVE_NOISE_INTEGRATION: the RK stages read moving averages of
         # the noise (integrator/integrator_rk.py:115-128); the error bound

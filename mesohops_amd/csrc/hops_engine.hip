@@ -435,12 +435,22 @@ __device__ __forceinline__ void prologue_body(const hops_op_t &op, const hops_ba
   __syncthreads();
 
   // 4. z-hat and the noise coefficient of each active L-operator
-  const float2 *z1 = reinterpret_cast<const float2 *>(bt.z1_d) + (long)b * na * bt.n_t;
-  const float2 *z2 = bt.z2_d ? reinterpret_cast<const float2 *>(bt.z2_d) + (long)b * na * bt.n_t : nullptr;
+  // noise values are complex64, or complex128 (interpolated or
+  // user-supplied complex128 noise, used unrounded as the reference uses it)
+  const bool z128 = bt.noise_c128 != 0;
+  auto znoise = [&](const void *base, long idx) -> zd {
+    if (z128) {
+      const double2 v = reinterpret_cast<const double2 *>(base)[idx];
+      return mk<double>(v.x, v.y);
+    }
+    const float2 v = reinterpret_cast<const float2 *>(base)[idx];
+    return mk<double>((double)v.x, (double)v.y);
+  };
+  const long zoff = (long)b * na * bt.n_t;
   for (int j = tid; j < na; j += nt) {
     const long zr = op.act_zrow_d ? op.act_zrow_d[j] : j;
-    const float2 v1 = z1[zr * bt.n_t + tidx];
-    zd zz = mk<double>((double)v1.x, -(double)v1.y);
+    const zd v1 = znoise(bt.z1_d, zoff + zr * bt.n_t + tidx);
+    zd zz = mk<double>(v1.x, -v1.y);
     if (nonlin) {
       for (int q = op.am_ptr_d[j]; q < op.am_ptr_d[j + 1]; ++q) {
         const int ma = op.mode_abs_d[op.am_mode_d[q]];
@@ -451,16 +461,19 @@ __device__ __forceinline__ void prologue_body(const hops_op_t &op, const hops_ba
     zh[2 * j] = zz.x;
     zh[2 * j + 1] = zz.y;
     zd c = zz;
-    if (z2) {
-      const float2 v2 = z2[zr * bt.n_t + tidx];
+    if (bt.z2_d) {
+      const zd v2 = znoise(bt.z2_d, zoff + zr * bt.n_t + tidx);
       if (nonlin) {
-        c.x += (double)v2.y;
-        c.y -= (double)v2.x;
+        c.x += v2.y;
+        c.y -= v2.x;
+      } else if (z128) {
+        c.x = v1.x + v2.y;
+        c.y = -v1.y - v2.x;
       } else {
         // LINEAR: conj(z1) - i z2 of two complex64 noise values is itself
         // complex64, rounded in single precision (eom/hops_eom.py:519,534-536)
-        c.x = (double)(v1.x + v2.y);
-        c.y = (double)(-v1.y - v2.x);
+        c.x = (double)((float)v1.x + (float)v2.y);
+        c.y = (double)(-(float)v1.y - (float)v2.x);
       }
     }
     cj[2 * j] = c.x;
@@ -1044,8 +1057,9 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     sb.ya_d = (char *)bt->ya_d + pe * es;
     sb.yb_d = (char *)bt->yb_d + pe * es;
     sb.acc_d = (char *)bt->acc_d + pe * es;
-    sb.z1_d = (const char *)bt->z1_d + (size_t)b0 * op->n_act * bt->n_t * 8;
-    if (bt->z2_d) sb.z2_d = (const char *)bt->z2_d + (size_t)b0 * op->n_act * bt->n_t * 8;
+    const size_t zes = bt->noise_c128 ? 16 : 8;
+    sb.z1_d = (const char *)bt->z1_d + (size_t)b0 * op->n_act * bt->n_t * zes;
+    if (bt->z2_d) sb.z2_d = (const char *)bt->z2_d + (size_t)b0 * op->n_act * bt->n_t * zes;
     sb.zmem_d = bt->zmem_d + (size_t)b0 * 2 * op->n_mode_abs;
     sb.ws_d = bt->ws_d + (size_t)b0 * bt->ws_stride;
     sb.coef_d = bt->coef_d + (size_t)b0 * bt->coef_stride;

@@ -162,13 +162,21 @@ __global__ __launch_bounds__(kThreads) void k_embedding(const float *__restrict_
   if (threadIdx.x == 0) nu[row] = nur;
 }
 
-// Correlated noise of each raw row.
+__device__ __forceinline__ void put_noise(float2 *out, long i, double2 v) {
+  out[i] = make_float2((float)v.x, (float)v.y);
+}
+__device__ __forceinline__ void put_noise(double2 *out, long i, double2 v) { out[i] = v; }
+
+// Correlated noise of each raw row; OutT float2 (the complex64 noise of the
+// reference) or double2 (the complex128 values its cubic interpolation is
+// built on).
+template <typename OutT>
 __global__ __launch_bounds__(kThreads) void k_filter(const float2 *__restrict__ z, int n_t, int N, int M,
                                                      int logm, const double2 *Bhat,
                                                      const double *__restrict__ s_w,
                                                      const double *__restrict__ nu,
                                                      const int32_t *__restrict__ row_lop,
-                                                     float2 *__restrict__ out, double2 *gscratch) {
+                                                     OutT *__restrict__ out, double2 *gscratch) {
   extern __shared__ double2 lds_[];
   const int row = blockIdx.x;
   double2 *lds = gscratch ? gscratch + (long)row * M : lds_;
@@ -204,7 +212,68 @@ __global__ __launch_bounds__(kThreads) void k_filter(const float2 *__restrict__ 
     sincospi(2.0 * nur * (double)j, &s, &c);  // conj(E2)
     const double2 y = make_double2(lds[j].x * inv, lds[j].y * inv);
     const double2 v = cmul2(y, make_double2(c, s));
-    out[(long)row * n_t + j] = make_float2((float)v.x, (float)v.y);
+    put_noise(out, (long)row * n_t + j, v);
+  }
+}
+
+// Cubic interpolation of each complex128 row y[0..n) on the uniform grid
+// x_i = i h, with not-a-knot ends (the spline scipy.interpolate.interp1d
+// builds for kind="cubic"), evaluated at t_j = j dt for j < n_out.  With
+// uniform spacing the not-a-knot rows fix M_1 = r_1 / 6 and M_{n-2} =
+// r_{n-2} / 6 (r_i = 6 (y_{i+1} - 2 y_i + y_{i-1}) / h^2); the second
+// derivatives M_2..M_{n-3} solve M_{i-1} + 4 M_i + M_{i+1} = r_i by the Thomas
+// algorithm (cp: its shared pivots 1 / (4 - cp_{k-1})), and M_0 = 2 M_1 - M_2,
+// M_{n-1} = 2 M_{n-2} - M_{n-3}.  Evaluated at t = t0 + j dt.  One thread per
+// row; work: [R][n] complex128.
+__global__ __launch_bounds__(kThreads) void k_spline(const double2 *__restrict__ y, int R, int n, double h,
+                                                     double t0, double dt, int n_out,
+                                                     const double *__restrict__ cp,
+                                                     double2 *__restrict__ work, double2 *__restrict__ out) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= R) return;
+  const double2 *yr = y + (long)row * n;
+  double2 *M = work + (long)row * n;
+  const double s6 = 6.0 / (h * h);
+  auto rhs = [&](int i) {
+    return make_double2(s6 * (yr[i + 1].x - 2.0 * yr[i].x + yr[i - 1].x),
+                        s6 * (yr[i + 1].y - 2.0 * yr[i].y + yr[i - 1].y));
+  };
+  const double2 r1 = rhs(1), rl = rhs(n - 2);
+  M[1] = make_double2(r1.x / 6.0, r1.y / 6.0);
+  M[n - 2] = make_double2(rl.x / 6.0, rl.y / 6.0);
+  const int m = n - 4;  // unknowns M_2 .. M_{n-3}
+  if (m > 0) {
+    // forward sweep: M[2 + k] holds d'_k
+    for (int k = 0; k < m; ++k) {
+      const int i = 2 + k;
+      double2 r = rhs(i);
+      if (k == 0) r = make_double2(r.x - M[1].x, r.y - M[1].y);
+      if (k == m - 1) r = make_double2(r.x - M[n - 2].x, r.y - M[n - 2].y);
+      const double2 prev = k == 0 ? make_double2(0.0, 0.0) : M[i - 1];
+      const double piv = k == 0 ? 0.25 : cp[k];  // 1 / (4 - cp_{k-1})
+      M[i] = make_double2((r.x - prev.x) * piv, (r.y - prev.y) * piv);
+    }
+    // back substitution: c'_k = cp[k] as the pivot of row k (c'_k = 1 * piv)
+    for (int k = m - 2; k >= 0; --k) {
+      const int i = 2 + k;
+      const double c = k == 0 ? 0.25 : cp[k];
+      M[i] = make_double2(M[i].x - c * M[i + 1].x, M[i].y - c * M[i + 1].y);
+    }
+  }
+  M[0] = make_double2(2.0 * M[1].x - M[2].x, 2.0 * M[1].y - M[2].y);
+  M[n - 1] = make_double2(2.0 * M[n - 2].x - M[n - 3].x, 2.0 * M[n - 2].y - M[n - 3].y);
+  const double inv6h = 1.0 / (6.0 * h), h6 = h / 6.0;
+  for (int j = 0; j < n_out; ++j) {
+    const double t = t0 + (double)j * dt;
+    int i = (int)floor(t / h);
+    if (i > n - 2) i = n - 2;
+    if (i < 0) i = 0;
+    const double B = t - (double)i * h, A = h - B;
+    const double a3 = A * A * A * inv6h, b3 = B * B * B * inv6h;
+    const double2 y0 = yr[i], y1 = yr[i + 1], m0 = M[i], m1 = M[i + 1];
+    out[(long)row * n_out + j] =
+        make_double2(m0.x * a3 + m1.x * b3 + (y0.x / h - m0.x * h6) * A + (y1.x / h - m1.x * h6) * B,
+                     m0.y * a3 + m1.y * b3 + (y0.y / h - m0.y * h6) * A + (y1.y / h - m1.y * h6) * B);
   }
 }
 
@@ -362,10 +431,44 @@ extern "C" int hops_noise_filter(const void *z_d, int R, int n_t, const double *
     if (int rc = scratch_rows(sizeof(double2) * p.M * (size_t)R, st, &gs)) return rc;
   }
   const size_t lds = gs ? 0 : sizeof(double2) * p.M;
-  if (int rc = set_lds((const void *)k_filter, lds)) return rc;
-  k_filter<<<R, kThreads, lds, st>>>((const float2 *)z_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d,
-                                      row_lop_d, (float2 *)out_d, gs);
+  if (int rc = set_lds((const void *)k_filter<float2>, lds)) return rc;
+  k_filter<float2><<<R, kThreads, lds, st>>>((const float2 *)z_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d,
+                                              row_lop_d, (float2 *)out_d, gs);
   if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_filter launch failed");
+  return 0;
+}
+
+extern "C" int hops_noise_filter_c128(const void *z_d, int R, int n_t, const double *s_w_d, const double *nu_d,
+                                      const int32_t *row_lop_d, void *out_d, void *stream) {
+  if (R <= 0) return 0;
+  if (!z_d || !s_w_d || !nu_d || !row_lop_d || !out_d)
+    return hops_internal_set_error("hops_noise_filter_c128: NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  Plan p;
+  if (int rc = make_plan(n_t, st, &p)) return rc;
+  double2 *gs = nullptr;
+  if (p.M > kMaxLds) {
+    if (int rc = scratch_rows(sizeof(double2) * p.M * (size_t)R, st, &gs)) return rc;
+  }
+  const size_t lds = gs ? 0 : sizeof(double2) * p.M;
+  if (int rc = set_lds((const void *)k_filter<double2>, lds)) return rc;
+  k_filter<double2><<<R, kThreads, lds, st>>>((const float2 *)z_d, n_t, p.N, p.M, p.logm, p.Bhat, s_w_d, nu_d,
+                                               row_lop_d, (double2 *)out_d, gs);
+  if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_filter launch failed");
+  return 0;
+}
+
+extern "C" int hops_noise_spline(const void *y_d, int R, int n, double h, double t0, double dt, int n_out,
+                                 const double *cp_d, void *work_d, void *out_d, void *stream) {
+  if (R <= 0 || n_out <= 0) return 0;
+  if (!y_d || !cp_d || !work_d || !out_d) return hops_internal_set_error("hops_noise_spline: NULL pointer");
+  if (n < 4) return hops_internal_set_error("cubic interpolation needs at least 4 noise points");
+  if (!(h > 0.0) || !(dt > 0.0)) return hops_internal_set_error("hops_noise_spline: spacing must be positive");
+  if (t0 < 0.0 || t0 + (double)(n_out - 1) * dt > (double)(n - 1) * h * (1.0 + 1e-12))
+    return hops_internal_set_error("hops_noise_spline: evaluation outside the noise time axis");
+  k_spline<<<(R + kThreads - 1) / kThreads, kThreads, 0, (hipStream_t)stream>>>(
+      (const double2 *)y_d, R, n, h, t0, dt, n_out, cp_d, (double2 *)work_d, (double2 *)out_d);
+  if (hipGetLastError() != hipSuccess) return hops_internal_set_error("k_spline launch failed");
   return 0;
 }
 

@@ -146,6 +146,10 @@ class BatchState:
             raise EngineError("noise1 shape %s != %s" % (tuple(noise1.shape), (B, t.n_act, n_t)))
         self.z1 = noise1.contiguous()
         self.z2 = None if noise2 is None else noise2.contiguous()
+        if self.z2 is not None and self.z2.dtype != self.z1.dtype:
+            raise EngineError("noise 1 and noise 2 must share one dtype")
+        if self.z1.dtype not in (torch.complex64, torch.complex128):
+            raise EngineError("noise must be complex64 or complex128")
         self.zmem = torch.zeros(B * t.n_mode_abs + 1, dtype=torch.complex128, device=dev)
         self.ws_stride = 4 * t.n_mode_abs + 2
         self.coef_stride = 4 * t.n_pat + 2 * t.n_mode + 4
@@ -185,7 +189,18 @@ class BatchState:
         bt.coefT_d = self.coefT.data_ptr()
         bt.sigT_d = self.sigT.data_ptr()
         bt.coefT_stride = self.coefT_stride
+        bt.noise_c128 = 1 if self.z1.dtype == torch.complex128 else 0
         self.bt = bt
+
+    def set_noise(self, z1, z2, n_t):
+        """Point the batch at other noise arrays [B, n_act, n_t] (the
+        interpolated noise of one propagate call)."""
+        self.z1 = z1.contiguous()
+        self.z2 = None if z2 is None else z2.contiguous()
+        self.bt.z1_d = self.z1.data_ptr()
+        self.bt.z2_d = None if self.z2 is None else self.z2.data_ptr()
+        self.bt.n_t = int(n_t)
+        self.bt.noise_c128 = 1 if self.z1.dtype == torch.complex128 else 0
 
     def _to_pool(self, a):
         """[B, n_rows] -> interleaved pool order [G, n_rows, T]."""

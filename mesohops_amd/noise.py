@@ -54,9 +54,6 @@ class HopsNoise:
         self.param = initialize_dict(p, NOISE_DICT_DEFAULT, NOISE_TYPE_DEFAULT, "HopsNoise")
         if self.param["MODEL"] not in ("FFT_FILTER", "ZERO", "PRE_CALCULATED"):
             raise UnsupportedRequest("Noise.param[MODEL] {}".format(self.param["MODEL"]), "Noise.prepare_noise")
-        for key in ("INTERPOLATE",):
-            if self.param[key]:
-                raise UnsupportedRequest("noise {} = True".format(key), "HopsNoise")
         # NOISE_WINDOW only bounds the slice of the noise the reference keeps in
         # a working copy (noise/hops_noise.py:385-433); the values are those of
         # the full noise, which stays resident in HBM here, so it changes nothing
@@ -75,6 +72,25 @@ class HopsNoise:
     @property
     def model(self):
         return self.param["MODEL"]
+
+    @property
+    def interpolate(self):
+        return bool(self.param["INTERPOLATE"]) and self.model != "ZERO"
+
+    def needs_c128(self):
+        """The reference uses this noise in complex128: a cubic interpolation
+        of it (noise/hops_noise.py:242-243, 259-261), or a PRE_CALCULATED
+        array given in a wider type than complex64 (:262-263, used as given)
+        whose values complex64 would round."""
+        if self.interpolate:
+            return True
+        if self.model != "PRE_CALCULATED":
+            return False
+        arr = np.asarray(self._precalc_array())
+        if arr.dtype == np.complex64:
+            return False
+        with np.errstate(over="ignore"):
+            return not np.array_equal(np.complex64(arr), arr)  # values complex64 cannot hold
 
     def alpha(self):
         """Correlation function of each L-operator, complex64 [n_l2, n_t]
@@ -118,7 +134,7 @@ class HopsNoise:
         out.imag = draws[:, ib]
         return out
 
-    def precalculated(self):
+    def _precalc_array(self):
         seed = self.param["SEED"]
         if isinstance(seed, str):
             if not os.path.isfile(seed) or not seed.endswith(".npy"):
@@ -129,7 +145,11 @@ class HopsNoise:
         if arr.shape != (self.n_l2, self.n_t):
             raise UnsupportedRequest("Noise.param[SEED] is an array of the wrong length",
                                      "Noise.prepare_noise", True)
-        return np.complex64(arr)
+        return arr
+
+    def precalculated(self, c128=False):
+        arr = self._precalc_array()
+        return np.complex128(arr) if c128 else np.complex64(arr)
 
     def time_index(self, t):
         """Index of t on the noise grid; off-grid times are errors
@@ -140,20 +160,22 @@ class HopsNoise:
         return int(np.where(hit)[0][0])
 
 
-def device_noise(noises, rows, device, stream_ptr, raws=None):
+def device_noise(noises, rows, device, stream_ptr, raws=None, c128=False):
     """Correlated noise of B trajectories for the hierarchy L-operators
-    ``rows`` (absolute indices), complex64 [B, len(rows), n_t] in HBM.
-    All noises must share the system (same correlation functions).  raws:
-    the host raw draws of each noise (HopsNoise.raw()), when already made."""
+    ``rows`` (absolute indices), complex64 [B, len(rows), n_t] in HBM, or
+    complex128 with ``c128`` (the filter's values before the complex64 cast,
+    PRE_CALCULATED arrays as given).  All noises must share the system (same
+    correlation functions).  raws: the host raw draws of each noise
+    (HopsNoise.raw()), when already made."""
     B = len(noises)
     nz = noises[0]
     nt = nz.n_t
     rows = [int(r) for r in rows]
-    out = torch.zeros((B, len(rows), nt), dtype=torch.complex64, device=device)
+    out = torch.zeros((B, len(rows), nt), dtype=torch.complex128 if c128 else torch.complex64, device=device)
     if nz.model == "ZERO":
         return out
     if nz.model == "PRE_CALCULATED":
-        host = np.stack([n.precalculated()[rows] for n in noises])
+        host = np.stack([n.precalculated(c128)[rows] for n in noises])
         out.copy_(torch.from_numpy(host).to(device))
     else:
         alpha = nz.alpha()
@@ -182,13 +204,39 @@ def device_noise(noises, rows, device, stream_ptr, raws=None):
                 else:
                     part = [r[rows] for r in raws[b0:b1]]
                 z_d = torch.from_numpy(np.ascontiguousarray(np.stack(part).reshape((b1 - b0) * nr, N))).to(device)
-                check(lib().hops_noise_filter(ctypes.c_void_p(z_d.data_ptr()), (b1 - b0) * nr, nt,
-                                              ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
-                                              ctypes.c_void_p(row_lop.data_ptr()), ctypes.c_void_p(out[b0:b1].data_ptr()),
-                                              stream_ptr))
+                filt = lib().hops_noise_filter_c128 if c128 else lib().hops_noise_filter
+                check(filt(ctypes.c_void_p(z_d.data_ptr()), (b1 - b0) * nr, nt,
+                           ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
+                           ctypes.c_void_p(row_lop.data_ptr()), ctypes.c_void_p(out[b0:b1].data_ptr()), stream_ptr))
                 torch.cuda.synchronize(device)  # the filter has read z_d before it is freed
                 del z_d
     if nz.param["FLAG_REAL"]:
+        torch.view_as_real(out)[..., 1] = 0
+    return out
+
+
+def spline_noise(z, tau_noise, t0, dt, n_out, stream_ptr, flag_real=False):
+    """Cubic interpolation (not-a-knot ends, as scipy's interp1d(kind="cubic")
+    that the reference builds, noise/hops_noise.py:242-243) of complex128
+    device noise z [..., n_t] on the grid j * tau_noise, evaluated at
+    t0 + j * dt for j < n_out: complex128 [..., n_out]."""
+    n_t = z.shape[-1]
+    R = z.numel() // n_t
+    m = max(n_t - 4, 1)
+    cp = np.empty(max(n_t, 1))
+    c = 0.25
+    for k in range(m):  # Thomas pivots of the constant (1, 4, 1) system
+        c = 0.25 if k == 0 else 1.0 / (4.0 - c)
+        cp[k] = c
+    cp_d = torch.from_numpy(cp).to(z.device)
+    work = torch.empty_like(z)
+    out = torch.empty(tuple(z.shape[:-1]) + (int(n_out),), dtype=torch.complex128, device=z.device)
+    # the spline of each row on a shifted grid: evaluation at t0 + j dt
+    check(lib().hops_noise_spline(ctypes.c_void_p(z.data_ptr()), R, n_t, float(tau_noise), float(t0), float(dt),
+                                  int(n_out), ctypes.c_void_p(cp_d.data_ptr()), ctypes.c_void_p(work.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), stream_ptr))
+    torch.cuda.synchronize(z.device)
+    if flag_real:
         torch.view_as_real(out)[..., 1] = 0
     return out
 

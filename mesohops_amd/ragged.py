@@ -352,6 +352,8 @@ class AdaptiveBatch:
         self.rt = RaggedBatch(self.precision_mode, self.arena_bytes, max_slots=max(self.B, 1),
                               n_threads=self.n_threads, device=self.device)
         sp = self.rt.stream
+        if self.noise1[0].interpolate or self.noise2[0].interpolate:
+            raise UnsupportedRequest("INTERPOLATE noise", "AdaptiveBatch")
         nl = info.n_l2
         self.z1 = device_noise(self.noise1, list(range(nl)), self.device, sp)
         self.z2 = None

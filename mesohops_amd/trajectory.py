@@ -230,10 +230,20 @@ class HopsBatch:
         tb = self.tables
         self.dop = DeviceOperator(tb, _PREC[self.precision_mode], self.device)
         sp = _stream_ptr(self.device)
-        z1 = device_noise(self.noise1, tb.absL_active, self.device, sp)
+        nz1, nz2 = self.noise1[0], self.noise2[0]
+        # noise the reference uses in complex128 (interpolated, or a wider
+        # PRE_CALCULATED array) stays complex128 here; noise 2 follows noise 1
+        c128 = any(n.needs_c128() for n in self.noise1 + self.noise2)
+        self._interp = nz1.interpolate
+        if nz2.model != "ZERO" and nz2.interpolate != self._interp:
+            raise UnsupportedRequest("INTERPOLATE set for one noise and not the other", "HopsTrajectory")
+        if c128 and self.setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
+            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION with complex128 (interpolated or "
+                                     "PRE_CALCULATED complex128) noise", "HopsTrajectory")
+        z1 = device_noise(self.noise1, tb.absL_active, self.device, sp, c128=c128)
         z2 = None
-        if self.noise2[0].model != "ZERO":
-            z2 = device_noise(self.noise2, tb.absL_active, self.device, sp)
+        if nz2.model != "ZERO":
+            z2 = device_noise(self.noise2, tb.absL_active, self.device, sp, c128=c128)
         self._z_raw = (z1, z2)
         self._eff_w = None
         self.state = BatchState(self.dop, self.B, self.setup.eom_kind, z1, z2, self.noise1[0].n_t,
@@ -250,6 +260,17 @@ class HopsBatch:
 
     def _plan(self, t_advance, tau):
         nz1, nz2 = self.noise1[0], self.noise2[0]
+        if getattr(self, "_interp", False):
+            # cubic noise: any stage time passing the reference's step check
+            # (trajectory/hops_trajectory.py:432-446); the noise of the call
+            # is evaluated on t + j tau/2 (noise index j, one point per half step)
+            if not _check_tau(tau, nz1, nz2):
+                raise TrajectoryError("Timesteps(" + str(tau * 0.5) + ") that do not match noise.param['TAU'] ("
+                                      + str(nz1.param["TAU"]) + ")")
+            if (self.t + t_advance + tau) > nz1.param["TLEN"]:
+                raise TrajectoryError("Trajectory times longer than noise.param['TLEN'] ="
+                                      + str(nz1.param["TLEN"]))
+            return int(np.ceil(t_advance / tau)), 0, 1
         if not _check_tau(tau, nz1, nz2):
             raise TrajectoryError("Timesteps(" + str(tau * 0.5) + ") that do not match noise.param['TAU'] ("
                                   + str(nz1.param["TAU"]) + ")")
@@ -282,6 +303,8 @@ class HopsBatch:
                                        device=self.device)
             st.bt.store_stride = n_steps
         st.bt.psi_store_d = st.psi_store.data_ptr() if store else None
+        if self._interp and n_steps > 0:
+            self._interpolated_noise(tau, n_steps)
         st.propagate(it0, n_steps, tau, ns, 0)
         t_axis = self.t + np.arange(1, n_steps + 1) * tau
         if store:
@@ -290,6 +313,18 @@ class HopsBatch:
                 self.psi_traj[b].extend(list(psi[b]))
             self.t_axis.extend([float(t) for t in t_axis])
         self.t = float(t_axis[-1]) if n_steps else self.t
+
+    def _interpolated_noise(self, tau, n_steps):
+        """The cubic noise at t + j tau/2, j = 0..2 n_steps, for one call."""
+        from .noise import spline_noise
+        sp = _stream_ptr(self.device)
+        z1g, z2g = self._z_raw
+        n_out = 2 * n_steps + 1
+        nz1, nz2 = self.noise1[0], self.noise2[0]
+        z1 = spline_noise(z1g, nz1.param["TAU"], self.t, 0.5 * tau, n_out, sp, nz1.param["FLAG_REAL"])
+        z2 = None if z2g is None else spline_noise(z2g, nz2.param["TAU"], self.t, 0.5 * tau, n_out, sp,
+                                                   nz2.param["FLAG_REAL"])
+        self.state.set_noise(z1, z2, n_out)
 
     def _use_effective_noise(self, w):
         """Stage noise = moving averages over w samples (integrator/integrator_rk.py:115-128)."""
@@ -326,7 +361,7 @@ class HopsBatch:
         zmem = old.zmem.clone()
         self.state = None
         del old.phi, old.y0, old.ya, old.yb, old.acc
-        st = BatchState(self.dop, self.B, self.setup.eom_kind, old.z1, old.z2, self.noise1[0].n_t,
+        st = BatchState(self.dop, self.B, self.setup.eom_kind, old.z1, old.z2, int(old.z1.shape[-1]),
                         store_slots=0, lanes=lanes)
         st.set_phi(phi)
         st.zmem.copy_(zmem)

@@ -544,9 +544,16 @@ class Noise:
                 z_unc = uncorrelated_noise(seed, lops, nt, p["RAND_MODEL"])
             self.z_unc = z_unc
             a = np.complex64(self.alpha(lops))
-            self._noise = np.complex64(correlated_noise(a, z_unc))
+            corr = correlated_noise(a, z_unc)
+            self._noise = np.complex64(corr)
+            if p["INTERPOLATE"]:  # hops_noise.py:242-243: cubic spline of the unrounded noise
+                from scipy.interpolate import interp1d
+                self._spline = interp1d(self.t_axis, corr, kind="cubic", axis=1)
         elif p["MODEL"] == "PRE_CALCULATED":
-            self._noise = np.asarray(p["SEED"])
+            self._noise = np.asarray(p["SEED"])  # used as given (hops_noise.py:255-263)
+            if p["INTERPOLATE"]:
+                from scipy.interpolate import interp1d
+                self._spline = interp1d(self.t_axis, p["SEED"], kind="cubic", axis=1)
         else:
             raise ValueError(p["MODEL"])
 
@@ -557,17 +564,21 @@ class Noise:
         return self._noise
 
     def get_noise(self, t_list, lops):
-        """hops_noise.py:339-451 (grid lookup, no interpolation)."""
+        """hops_noise.py:339-451: grid lookup, or the cubic spline with
+        INTERPOLATE (:379-385)."""
         if self.p["MODEL"] == "ZERO":
             return np.zeros([len(lops), len(t_list)], dtype=np.complex64)
         Z = self.noise
+        if self.p["INTERPOLATE"]:
+            vals = self._spline(np.asarray(t_list, dtype=float))[list(lops), :]
+            return np.real(vals) if self.p["FLAG_REAL"] else vals
         it = []
         for t in t_list:
             hit = np.abs(self.t_axis - t) < PRECISION
             if np.sum(hit) != 1:
                 raise ValueError("off-grid noise time %r" % t)
             it.append(int(np.where(hit)[0][0]))
-        out = np.zeros((len(lops), len(t_list)), dtype=np.complex64)
+        out = np.zeros((len(lops), len(t_list)), dtype=Z.dtype)
         for i, l in enumerate(lops):
             for j, t in enumerate(it):
                 out[i, j] = Z[l, t]

@@ -395,3 +395,119 @@ def test_adaptive_effective_noise_integration():
     err = np.max(np.abs(tr.storage["psi_traj"] - np.array(ref.psi_traj)))
     print("adaptive effective noise max |dpsi| = %.3e" % err)
     assert err < FP64_TOL
+
+
+def test_noise_spline_matches_scipy():
+    """hops_noise_spline against scipy.interpolate.interp1d(kind="cubic") (the
+    reference's interpolation, noise/hops_noise.py:242-243) on random complex
+    rows: 4, 5 and 6 points (the smallest not-a-knot systems) and long rows,
+    evaluated on and between grid points from an offset."""
+    _need_gpu()
+    from scipy.interpolate import interp1d
+    from mesohops_amd.engine import _stream_ptr
+    from mesohops_amd.noise import spline_noise
+    rng = np.random.default_rng(2)
+    dev = torch.device("cuda:0")
+    for n, h, t0, dt in ((4, 0.5, 0.0, 0.25), (5, 1.0, 0.3, 0.35), (6, 0.5, 0.0, 0.5), (301, 0.5, 1.25, 0.75),
+                         (2049, 0.5, 0.0, 0.375)):
+        y = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+        tmax = (n - 1) * h
+        n_out = int(np.floor((tmax - t0) / dt + 1e-9)) + 1
+        got = spline_noise(torch.from_numpy(y).to(dev), h, t0, dt, n_out, _stream_ptr(dev)).cpu().numpy()
+        tq = t0 + np.arange(n_out) * dt
+        want = interp1d(np.arange(n) * h, y, kind="cubic", axis=1)(tq)
+        err = np.max(np.abs(got - want)) / np.max(np.abs(y))
+        print("spline n=%d h=%g t0=%g dt=%g: max rel err %.2e" % (n, h, t0, dt, err))
+        assert err < 1e-12
+
+
+@pytest.mark.parametrize("eom", ["NORMALIZED NONLINEAR", "LINEAR"])
+def test_fp64_interpolated_noise(eom):
+    """INTERPOLATE = True with a half step between noise points (TAU 1, step
+    1.5 fs: stage times t + 0.75 fs): the equations take the complex128 cubic
+    interpolation of the noise, as the reference does
+    (noise/hops_noise.py:242-243, 379-385); noise 1 and a FLAG_REAL noise 2
+    share the oracle's complex128 correlated rows (PRE_CALCULATED)."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.three_site(False)
+    sysp.update(L_NOISE2=sysp["L_NOISE1"], ALPHA_NOISE2=O.bcf_exp, PARAM_NOISE2=sysp["PARAM_NOISE1"])
+    sysd = O.derive_system(sysp)
+    tlen, tau = 60.0, 1.0
+    zs = []
+    for seed in (3, 4):
+        nz = O.Noise({"SEED": seed, "MODEL": "FFT_FILTER", "TLEN": tlen, "TAU": tau}, sysd, 1)
+        nz.prepare()
+        a = np.complex64(nz.alpha(list(range(nz.n_l2))))
+        zs.append(O.correlated_noise(a, nz.z_unc))  # complex128, before the reference's cast
+    n1 = {"SEED": zs[0], "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau, "INTERPOLATE": True}
+    n2 = {"SEED": zs[1], "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau, "INTERPOLATE": True,
+          "FLAG_REAL": True}
+    ref, psi0 = oracle_run(sysp, eom, n1, 3, 45.0, 1.5, noise2_param=n2)
+    tr = HopsTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, dict(n1), dict(n2), hierarchy_param={"MAXHIER": 3})
+    tr.initialize(psi0)
+    tr.propagate(45.0, 1.5)
+    got = np.array(tr.storage["psi_traj"])
+    want = np.array(ref.psi_traj)
+    err = np.max(np.abs(got - want)) / max(1.0, np.max(np.abs(want)))
+    print("interpolated noise %s: max |dpsi| = %.3e over %d steps" % (eom, err, len(want) - 1))
+    assert len(got) == len(want) == 31
+    assert err < FP64_TOL
+
+
+def test_device_noise_filter_complex128():
+    """hops_noise_filter_c128: the filter's complex128 values (on which the
+    reference builds its interpolation) against the oracle's, at the bar of
+    the complex64 filter test; rounding them to complex64 gives the
+    complex64 filter's output exactly."""
+    _need_gpu()
+    from mesohops_amd.basis import SystemInfo
+    from mesohops_amd.engine import _stream_ptr
+    from mesohops_amd.noise import HopsNoise, device_noise
+    sysp = systems.dimer_of_dimers()
+    info = SystemInfo(sysp)
+    sysd = O.derive_system(sysp)
+    p = {"SEED": 7, "MODEL": "FFT_FILTER", "TLEN": 1000.0, "TAU": 1.0}
+    nz = HopsNoise(p, info, sysp, 1)
+    dev = torch.device("cuda:0")
+    rows = list(range(info.n_l2))
+    got = device_noise([nz], rows, dev, _stream_ptr(dev), c128=True).cpu().numpy()[0]
+    g64 = device_noise([nz], rows, dev, _stream_ptr(dev)).cpu().numpy()[0]
+    on = O.Noise(p, sysd, 1)
+    on.prepare()
+    want = O.correlated_noise(np.complex64(on.alpha(rows)), on.z_unc)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print("complex128 filter max rel err %.2e" % err)
+    assert got.dtype == np.complex128 and err < 1e-6
+    assert np.array_equal(np.complex64(got), g64)
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_fp64_precalculated_complex128_noise(adaptive):
+    """A PRE_CALCULATED complex128 array that complex64 cannot hold is used as
+    given (noise/hops_noise.py:255-263), not rounded: psi(t) to 1e-10 against
+    the oracle on the same array, plain and adaptive."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.chain(5)
+    tlen, tau = 50.0, 0.5
+    z = np.complex128(_oracle_noise(sysp, 6, tlen, tau))
+    z = z * (1.0 + 1e-9 * np.arange(z.size).reshape(z.shape))  # below complex64 resolution
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(5, complex)
+    psi0[2] = 1
+    if adaptive:
+        ref = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 3}, delta_a=1e-3, delta_s=1e-3)
+    else:
+        ref = O.Trajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 3})
+    ref.initialize(psi0)
+    ref.propagate(30.0, 1.0)
+    tr = HopsTrajectory(sysp, eom, dict(noise), hierarchy_param={"MAXHIER": 3})
+    if adaptive:
+        tr.make_adaptive(1e-3, 1e-3)
+    tr.initialize(psi0)
+    tr.propagate(30.0, 1.0)
+    err = np.max(np.abs(np.array(tr.storage["psi_traj"]) - np.array(ref.psi_traj)))
+    print("complex128 PRE_CALCULATED adaptive=%s max |dpsi| = %.3e" % (adaptive, err))
+    assert err < FP64_TOL

@@ -388,3 +388,22 @@ def test_static_basis_is_kept_from_the_start():
     assert set(tr.basis.aux_list) == {(), ((4, 1),), ((5, 1),)}
     tr.propagate(5.0, 1.0)
     assert all(s == [1, 2, 3] for s in tr.state_traj[1:6])
+
+
+def test_interpolated_noise_reproduces_cubics():
+    """INTERPOLATE (noise/hops_noise.py:242-243, 379-385): the oracle's noise
+    lookup goes through scipy's cubic interp1d along the time axis; a
+    not-a-knot cubic spline reproduces a cubic polynomial exactly, so noise
+    rows that are cubics in t are returned exactly at off-grid times, and the
+    grid values are returned at grid times."""
+    sysp = systems.dimer_of_dimers()
+    sysd = O.derive_system(sysp)
+    t = O.noise_time_axis(20.0, 0.5)
+    n_l2 = len(set(O.l_signature(L) for L in sysd["L_NOISE1"]))
+    rows = np.array([(0.3 + 0.1j * l) * t ** 3 - (1.0 + l) * t + 2.0j for l in range(n_l2)])
+    nz = O.Noise({"SEED": rows, "MODEL": "PRE_CALCULATED", "TLEN": 20.0, "TAU": 0.5, "INTERPOLATE": True}, sysd, 1)
+    tq = [0.1, 3.75, 7.3, 19.9]
+    got = nz.get_noise(tq, list(range(n_l2)))
+    want = np.array([(0.3 + 0.1j * l) * np.array(tq) ** 3 - (1.0 + l) * np.array(tq) + 2.0j for l in range(n_l2)])
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(nz.get_noise([1.5, 2.0], [0]), rows[[0]][:, [3, 4]], rtol=1e-12, atol=1e-12)
