@@ -276,6 +276,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="trajectories per cache-resident chunk with --lanes")
     ap.add_argument("--nt", type=int, default=0, help="nontemporal hint on streamed arrays with --lanes")
     ap.add_argument("--groups-q", type=int, default=0, help="trajectory-group split with --lanes (0: automatic)")
+    ap.add_argument("--tile-order", type=int, default=0, help="locality tile order with --lanes (0 or 1)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
@@ -317,8 +318,8 @@ def main():
         choice, tune_t = autotune.tune(batch, psi0(), DT, verbose=True)
     else:
         batch.relayout(args.lanes)
-        choice, tune_t = (args.lanes, True, args.chunk, args.nt, args.groups_q), {}
-        autotune.apply(choice)
+        choice, tune_t = (args.lanes, True, args.chunk, args.nt, args.groups_q, args.tile_order), {}
+        autotune.apply(choice, batch)
     st = batch.state
 
     def run(steps):
@@ -430,9 +431,9 @@ def main():
         "speedup_vs_gpu_fp64_single": value / single_rate,
         "init_s": t_init,
         "layout": {"lanes": choice[0], "stationary_stage": bool(choice[1]), "chunk_traj": choice[2],
-                   "nt_stream": choice[3], "groups_q": choice[4],
-                   "autotune_ms_per_step (lanes/stationary/chunk/nt/q)":
-                       {"%d/%d/%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
+                   "nt_stream": choice[3], "groups_q": choice[4], "tile_order": bool(choice[5]),
+                   "autotune_ms_per_step (lanes/stationary/chunk/nt/q/order)":
+                       {"%d/%d/%d/%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": kernel_name, "kernel_avg_ms": stage_ms,

@@ -130,6 +130,10 @@ typedef struct hops_batch {
   double *sigT_d;
   int32_t coefT_stride;
   int32_t noise_c128; /* 1: z1_d / z2_d hold complex128 instead of complex64 */
+  /* order in which the operator-stationary stage kernel visits its tiles of
+     512 >> lane_shift rows (a permutation of the tiles), or NULL for tile
+     order; any order gives the same results */
+  const int32_t *tile_order_d;
 } hops_batch_t;
 
 const char *hops_last_error(void);

@@ -49,7 +49,7 @@ class HopsBatch(ctypes.Structure):
                 ("phi_d", _vp), ("y0_d", _vp), ("ya_d", _vp), ("yb_d", _vp), ("acc_d", _vp),
                 ("z1_d", _vp), ("z2_d", _vp), ("zmem_d", _vp), ("ws_d", _vp), ("coef_d", _vp),
                 ("psi_store_d", _vp), ("tile_d", _vp), ("n_tiles", _i32), ("tile_rows", _i32),
-                ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("noise_c128", _i32)]
+                ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("noise_c128", _i32), ("tile_order_d", _vp)]
 
 
 class RtSystemDesc(ctypes.Structure):

@@ -697,51 +697,64 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
 // and the carry are read and written with the nontemporal hint; NT >= 2: the
 // stage output and the complex64 state copy are stored with it too; NT >= 3:
 // the row's own aux block is read with it; NT = 4: the ELL neighbours too.
+// Element i of an array addressed as a uniform base pointer plus a 32-bit
+// byte offset, which the compiler issues as a global load/store with a scalar
+// base and one offset register (the pools of the stationary kernel are below
+// 4 GiB; the launch checks it).
+template <typename T>
+__device__ __forceinline__ T *el(T *base, uint32_t i) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T *el(const T *base, uint32_t i) {
+  return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
+}
+
 template <typename R, int WR, int PW, int NT>
 __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_batch_t &bt, int stage,
                                               const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
-                                              int b, long r, int k, cx<R> k0, const int *ecol,
+                                              int b, long r_, int k, cx<R> k0, const int *ecol,
                                               const cx<R> *ev, const int *pidx, const int *pcol) {
 #pragma clang fp contract(off)
-  const int n = op.n_state;
-  const long nr = (long)n * op.n_hier;
-  const Lane ln = lane_of(bt, nr, b);
-  const int T = ln.T;
-  const cx<R> *Y = yin + ln.base;
-  const cx<R> *cT = reinterpret_cast<const cx<R> *>(bt.coefT_d) + b;
-  const long cs = bt.coefT_stride;
-  const long rowk = (long)k * n;
-  const long gi = ln.base + r * T;
+  const uint32_t n = (uint32_t)op.n_state;
+  const uint32_t nr = n * (uint32_t)op.n_hier;
+  const uint32_t r = (uint32_t)r_;
+  const uint32_t ls = (uint32_t)bt.lane_shift, T = 1u << ls;
+  const uint32_t gb = ((uint32_t)b >> ls) * nr * T + ((uint32_t)b & (T - 1));  // lane base
+  const uint32_t rowk = (uint32_t)k * n;
+  const uint32_t gi = gb + r * T;
+  const cx<R> *cT = reinterpret_cast<const cx<R> *>(bt.coefT_d);
+  const uint32_t cs = (uint32_t)bt.coefT_stride;
   cx<R> *acc = reinterpret_cast<cx<R> *>(bt.acc_d);
   double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
   const cx<R> zero = mk<R>((R)0, (R)0);
   // loads
   constexpr int NTO = NT >= 3 ? 1 : 0;  // the row's own aux block
   constexpr int NTG = NT >= 4 ? 1 : 0;  // the ELL neighbours
-  const cx<R> yr = ld_s<NTO>(Y + r * T);
+  const cx<R> yr = ld_s<NTO>(el(yin, gi));
   cx<R> yp[PW], ye[WR], ap[PW];
-  const cx<R> *A = cT + (k == 0 ? (long)op.n_pat * cs : 0);
+  const uint32_t a0 = (uint32_t)b + (k == 0 ? (uint32_t)op.n_pat * cs : 0u);
 #pragma unroll
   for (int p = 0; p < PW; ++p) {
-    yp[p] = pidx[p] >= 0 ? ld_s<NTO>(Y + (rowk + pcol[p]) * T) : zero;
-    ap[p] = pidx[p] >= 0 ? A[(long)pidx[p] * cs] : zero;
+    yp[p] = pidx[p] >= 0 ? ld_s<NTO>(el(yin, gb + (rowk + (uint32_t)pcol[p]) * T)) : zero;
+    ap[p] = pidx[p] >= 0 ? *el(cT, a0 + (uint32_t)pidx[p] * cs) : zero;
   }
 #pragma unroll
-  for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? ld_s<NTG>(Y + (long)ecol[e] * T) : zero;
+  for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? ld_s<NTG>(el(yin, gb + (uint32_t)ecol[e] * T)) : zero;
   const double sigma = bt.sigT_d[b];
   cx<R> base = zero, a_in = zero;
   double2 ph = make_double2(0.0, 0.0);
   if (stage < 3) {
     if (bt.precision == HOPS_PREC_MIXED) {
-      base = ld_s<NT>(reinterpret_cast<const cx<R> *>(bt.y0_d) + gi);
+      base = ld_s<NT>(el(reinterpret_cast<const cx<R> *>(bt.y0_d), gi));
     } else {
-      const double2 p = ld_s<NT>(phi + gi);
+      const double2 p = ld_s<NT>(el(phi, gi));
       base = mk<R>((R)p.x, (R)p.y);
     }
   } else {
-    ph = ld_s<NT>(phi + gi);
+    ph = ld_s<NT>(el(phi, gi));
   }
-  if (stage > 0) a_in = ld_s<NT>(acc + gi);
+  if (stage > 0) a_in = ld_s<NT>(el(acc, gi));
   // arithmetic
   cx<R> d = cmul(k0, yr);
 #pragma unroll
@@ -749,11 +762,12 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
 #pragma unroll
   for (int e = 0; e < WR; ++e) cfma(d, ev[e], ye[e]);
   if (bt.eom != HOPS_EOM_LINEAR) {
-    const cx<R> *zc = cT + 2L * op.n_pat * cs;
+    const uint32_t zc0 = (uint32_t)b + 2u * (uint32_t)op.n_pat * cs;
     for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
-      cfma(d, zc[(long)op.up_mode_d[u] * cs], Y[((long)op.up_aux_d[u] * n + (r - rowk)) * T]);
+      cfma(d, *el(cT, zc0 + (uint32_t)op.up_mode_d[u] * cs),
+           *el(yin, gb + ((uint32_t)op.up_aux_d[u] * n + (r - rowk)) * T));
     if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
-      const cx<R> nc = cT[(2L * op.n_pat + op.n_mode) * cs];
+      const cx<R> nc = *el(cT, (uint32_t)b + (2u * (uint32_t)op.n_pat + (uint32_t)op.n_mode) * cs);
       d = csub(d, cmul(nc, yr));
     }
   }
@@ -763,26 +777,26 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
     if (bt.precision == HOPS_PREC_MIXED) {
       base = cscale(base, (R)sigma);
     } else {
-      const double2 p = ld_s<NT>(phi + gi);
+      const double2 p = ld_s<NT>(el(phi, gi));
       base = mk<R>((R)(sigma * p.x), (R)(sigma * p.y));
     }
     const R c = (R)((stage == 2 ? 1.0 : 0.5) * h);
-    st_s<NT >= 2 ? 1 : 0>(yout + gi, mk<R>(base.x + c * d.x, base.y + c * d.y));
+    st_s<NT >= 2 ? 1 : 0>(el(yout, gi), mk<R>(base.x + c * d.x, base.y + c * d.y));
     if (stage == 0) {
-      st_s<NT>(acc + gi, d);
+      st_s<NT>(el(acc, gi), d);
     } else {
       a_in.x += (R)2 * d.x;
       a_in.y += (R)2 * d.y;
-      st_s<NT>(acc + gi, a_in);
+      st_s<NT>(el(acc, gi), a_in);
     }
   } else {
     const double c = h / 6.0;
     double2 q;
     q.x = sigma * ph.x + c * ((double)a_in.x + (double)d.x);
     q.y = sigma * ph.y + c * ((double)a_in.y + (double)d.y);
-    st_s<NT>(phi + gi, q);
+    st_s<NT>(el(phi, gi), q);
     if (bt.precision == HOPS_PREC_MIXED)
-      st_s<NT >= 2 ? 1 : 0>(reinterpret_cast<cx<float> *>(bt.y0_d) + gi, mk<float>((float)q.x, (float)q.y));
+      st_s<NT >= 2 ? 1 : 0>(el(reinterpret_cast<cx<float> *>(bt.y0_d), gi), mk<float>((float)q.x, (float)q.y));
   }
 }
 
@@ -797,7 +811,8 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
                                                       const cx<R> *__restrict__ yin,
                                                       cx<R> *__restrict__ yout, int n_tiles, int Q) {
   const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int tile = slot % n_tiles, q = slot / n_tiles;
+  const int q = slot / n_tiles;
+  const int tile = bt.tile_order_d ? bt.tile_order_d[slot - q * n_tiles] : slot - q * n_tiles;
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
   const int lane = threadIdx.x & ((1 << bt.lane_shift) - 1);
@@ -1002,7 +1017,9 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const int Gc = (chunk + T - 1) / T;
   int sh_w = 0, sh_p = 0, sh_tiles = 0, sh_q = 1;
   unsigned sh_grid = 0;
-  if (g_stationary && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
+  // the stationary kernel addresses its pools with 32-bit byte offsets
+  const bool pools32 = (double)nr * (double)((long)G << bt->lane_shift) * 16.0 < 4294967296.0;
+  if (g_stationary && pools32 && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
     sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
     sh_p = max_pat <= 4 ? 4 : 8;
     const long rows_per_tile = 512 >> bt->lane_shift;

@@ -110,6 +110,28 @@ def tile_map(G, n_rows, lanes=1):
     return out
 
 
+def locality_tile_order(tables, lanes):
+    """Visiting order of the operator-stationary tiles (512 / lanes rows):
+    a tile whose rows gather from deeper auxiliaries (K2_kp1) is placed just
+    before the median tile of those neighbours, so the gathered rows are
+    still in L2 when their own tile runs, and their gathers back up the
+    hierarchy find the parent rows there.  Other tiles keep their place."""
+    n, nh = tables.n_state, tables.n_hier
+    nr = n * nh
+    rpt = 512 // lanes
+    ntile = (nr + rpt - 1) // rpt
+    up_ptr = np.asarray(tables.tables["up_ptr"], dtype=np.int64)
+    up_aux = np.asarray(tables.tables["up_aux"], dtype=np.int64)
+    child_tile = (up_aux * n + n // 2) // rpt
+    keys = []
+    for t in range(ntile):
+        k0, k1 = (t * rpt) // n, min(nh - 1, ((t + 1) * rpt - 1) // n)
+        ch = child_tile[up_ptr[k0]:up_ptr[k1 + 1]]
+        ch = ch[ch > t]
+        keys.append((float(np.median(ch)), 0, t) if ch.size else (float(t), 1, t))
+    return np.array([k[2] for k in sorted(keys)], dtype=np.int32)
+
+
 def default_lanes(B):
     """Trajectories interleaved per row: 4 once there are enough trajectories
     to give every XCD at least one group of 4, else 1."""
@@ -191,6 +213,16 @@ class BatchState:
         bt.coefT_stride = self.coefT_stride
         bt.noise_c128 = 1 if self.z1.dtype == torch.complex128 else 0
         self.bt = bt
+
+    def use_tile_order(self, on):
+        """Visit the stationary kernel's tiles in locality order (same results)."""
+        if on:
+            order = locality_tile_order(self.dop.tables, self.lanes)
+            self.tile_order = torch.from_numpy(order).to(self.device)
+            self.bt.tile_order_d = self.tile_order.data_ptr()
+        else:
+            self.tile_order = None
+            self.bt.tile_order_d = None
 
     def set_noise(self, z1, z2, n_t):
         """Point the batch at other noise arrays [B, n_act, n_t] (the

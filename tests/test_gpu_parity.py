@@ -36,7 +36,7 @@ def oracle_run(sysp, eom, noise_param, maxhier, t_max, dt, noise2_param=None):
 
 
 def engine_run(sysp, eom, noise_arrays, tlen, tau_noise, maxhier, t_max, dt, psi0, prec="fp64",
-               noise2_arrays=None, lanes=None):
+               noise2_arrays=None, lanes=None, tile_order=False):
     from mesohops_amd import HopsBatch
     nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau_noise} for z in noise_arrays]
     n2 = None
@@ -46,6 +46,8 @@ def engine_run(sysp, eom, noise_arrays, tlen, tau_noise, maxhier, t_max, dt, psi
     b = HopsBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], n2, {"MAXHIER": maxhier},
                   noise_params=nps, precision_mode=prec, lanes=lanes)
     b.initialize(psi0)
+    if tile_order:
+        b.state.use_tile_order(True)
     b.propagate(t_max, dt)
     return b
 
@@ -238,9 +240,10 @@ def test_stationary_kernel_equals_streaming(sysname):
 
 
 def test_stationary_kernel_options_bit_identical():
-    """The nontemporal hint on the streamed arrays and the trajectory-group
-    split of the stationary kernel change caching and work distribution only:
-    trajectories are bit-identical to the default launch."""
+    """The nontemporal hint on the streamed arrays, the trajectory-group split
+    and the locality tile order of the stationary kernel change caching and
+    work distribution only: trajectories are bit-identical to the default
+    launch."""
     _need_gpu()
     from mesohops_amd import _lib
     sysp = systems.chain(4, k_mats=1)
@@ -262,6 +265,11 @@ def test_stationary_kernel_options_bit_identical():
             d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
             print(prec, "nt", nt, "q", q, "max diff", d)
             assert d == 0.0
+        got = engine_run(sysp, "NORMALIZED NONLINEAR", zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, lanes=2,
+                         tile_order=True)
+        d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+        print(prec, "locality tile order max diff", d)
+        assert d == 0.0
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8])
