@@ -311,16 +311,24 @@ class AdaptiveTrajectoryEngine:
         self.sp = _stream_ptr(self.device)
         # noise of every hierarchy L-operator; a basis takes its active rows
         from .noise import device_noise
-        if noise1.interpolate or noise2.interpolate:
-            raise UnsupportedRequest("INTERPOLATE noise with an adaptive basis", "HopsTrajectory")
+        self.interp = noise1.interpolate
+        if noise2.model != "ZERO" and noise2.interpolate != self.interp:
+            raise UnsupportedRequest("INTERPOLATE set for one noise and not the other", "HopsTrajectory")
         nl = self.info.n_l2
-        c128 = noise1.needs_c128() or noise2.needs_c128()  # PRE_CALCULATED complex128, used as given
+        # complex128 noise: interpolated, or PRE_CALCULATED complex128 used as given
+        c128 = noise1.needs_c128() or noise2.needs_c128()
         if c128 and setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
-            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION with complex128 PRE_CALCULATED noise",
-                                     "HopsTrajectory")
+            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION with complex128 (interpolated or "
+                                     "PRE_CALCULATED complex128) noise", "HopsTrajectory")
         self.z1_all = device_noise([noise1], list(range(nl)), self.device, self.sp, c128=c128)[0]
         self.z2_all = None if noise2.model == "ZERO" else device_noise([noise2], list(range(nl)), self.device,
                                                                        self.sp, c128=c128)[0]
+        self.n_t = noise1.n_t
+        if self.interp:
+            # the grid noise stays; each propagate call reads the cubic noise
+            # on its own half-step times (begin_call)
+            self._z_grid = (self.z1_all, self.z2_all)
+            self.begin_call(0.0, 2.0 * noise1.param["TAU"], 0)
         self.zmem = torch.zeros(self.info.n_hmodes + 1, dtype=torch.complex128, device=self.device)
         # EFFECTIVE_NOISE_INTEGRATION: the RK stages read moving averages of
         # the noise (integrator/integrator_rk.py:115-128); the error bound
@@ -360,16 +368,38 @@ class AdaptiveTrajectoryEngine:
         act = torch.as_tensor(tb.absL_active, dtype=torch.long, device=self.device)
         z1 = self._rows(self.z1_all, act)
         z2 = self._rows(self.z2_all, act)
-        st = BatchState(dop, 1, self.setup.eom_kind, z1, z2, self.noise1.n_t, store_slots=0, lanes=1)
+        st = BatchState(dop, 1, self.setup.eom_kind, z1, z2, self.n_t, store_slots=0, lanes=1)
         st.zmem[: self.info.n_hmodes].copy_(self.zmem[: self.info.n_hmodes])
         self.tb, self.dop, self.st = tb, dop, st
         # derivative evaluation runs in FP64 on its own operator when the
         # propagation is mixed precision
         self.dop64 = dop if self.prec == _lib.HOPS_PREC_FP64 else DeviceOperator(tb, _lib.HOPS_PREC_FP64, self.device)
         self.st64 = st if self.prec == _lib.HOPS_PREC_FP64 else BatchState(
-            self.dop64, 1, self.setup.eom_kind, z1, z2, self.noise1.n_t, store_slots=0, lanes=1)
+            self.dop64, 1, self.setup.eom_kind, z1, z2, self.n_t, store_slots=0, lanes=1)
         self._z_point = (z1, z2)
         self._attach_noise()
+
+    def begin_call(self, t0, tau, n_steps):
+        """INTERPOLATE: the cubic noise of every L-operator at t0 + j tau/2,
+        j = 0..2 n_steps (noise index j for this call), for the current and
+        every later basis of the call."""
+        if not self.interp:
+            return
+        from .noise import spline_noise
+        n_out = 2 * int(n_steps) + 1
+        z1g, z2g = self._z_grid
+        p1, p2 = self.noise1.param, self.noise2.param
+        self.z1_all = spline_noise(z1g, p1["TAU"], t0, 0.5 * tau, n_out, self.sp, p1["FLAG_REAL"])
+        self.z2_all = None if z2g is None else spline_noise(z2g, p2["TAU"], t0, 0.5 * tau, n_out, self.sp,
+                                                            p2["FLAG_REAL"])
+        self.n_t = n_out
+        if getattr(self, "tb", None) is not None:
+            act = torch.as_tensor(self.tb.absL_active, dtype=torch.long, device=self.device)
+            z1, z2 = self._rows(self.z1_all, act), self._rows(self.z2_all, act)
+            self.st.set_noise(z1, z2, n_out)
+            if self.st64 is not self.st:
+                self.st64.set_noise(z1, z2, n_out)
+            self._z_point = (z1, z2)
 
     def phi_host(self):
         return self.st.phi_host()[0]

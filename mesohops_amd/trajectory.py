@@ -464,6 +464,7 @@ class HopsTrajectory:
         print("Integration from ", self.t, " to ", np.max(t_axis) if len(t_axis) else self.t)
         if self.adaptive:
             ad = self._adaptive
+            ad.begin_call(self.t, tau, n_steps)
             for index_t, t in enumerate(t_axis):
                 z_lag = self.z_mem
                 ad.step(index_t, it0 + 2 * ns * index_t, tau, ns)
@@ -519,12 +520,13 @@ class HopsTrajectory:
 
     def _plan_adaptive(self, t_advance, tau):
         b = self._batch
-        saved = b.t
+        saved, saved_interp = b.t, getattr(b, "_interp", False)
         b.t = self._t
+        b._interp = self._adaptive.interp
         try:
             return b._plan(t_advance, tau)
         finally:
-            b.t = saved
+            b.t, b._interp = saved, saved_interp
 
     # -- checkpoints (trajectory/hops_trajectory.py:853-1034) ------------
     _SYSTEM_KEYS = ["HAMILTONIAN", "GW_SYSBATH", "L_HIER", "L_NOISE1", "ALPHA_NOISE1", "PARAM_NOISE1",

@@ -220,3 +220,37 @@ def test_operator_matches_oracle(adaptive):
         for i, (s_got, s_ref) in enumerate(zip(tr.storage["state_list"], ref.state_traj)):
             assert list(s_got) == list(s_ref), (i, s_got, s_ref)
     assert np.max(np.abs(tr.phi.reshape(-1) - ref.phi)) < FP64_TOL
+
+
+def test_adaptive_interpolated_noise_matches_oracle():
+    """INTERPOLATE with an adaptive basis and stage times between noise
+    points (TAU 1, step 1.5 fs): each propagate call reads the cubic noise on
+    its half-step times; basis sequence and psi(t) follow the oracle (scipy's
+    interp1d, as the reference) to 1e-10, across two propagate calls."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp = systems.chain(6)
+    sysd = O.derive_system(sysp)
+    tlen, tau = 60.0, 1.0
+    nz = O.Noise({"SEED": 12, "MODEL": "FFT_FILTER", "TLEN": tlen, "TAU": tau}, sysd, 1)
+    nz.prepare()
+    z = O.correlated_noise(np.complex64(nz.alpha(list(range(nz.n_l2)))), nz.z_unc)
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau, "INTERPOLATE": True}
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+    ref = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 3}, delta_a=1e-3, delta_s=1e-3)
+    ref.initialize(psi0)
+    ref.propagate(21.0, 1.5)
+    ref.propagate(15.0, 1.5)
+    tr = HopsTrajectory(sysp, eom, dict(noise), hierarchy_param={"MAXHIER": 3})
+    tr.make_adaptive(1e-3, 1e-3)
+    tr.initialize(psi0)
+    tr.propagate(21.0, 1.5)
+    tr.propagate(15.0, 1.5)
+    for i, (s_got, s_ref) in enumerate(zip(tr.storage["state_list"], ref.state_traj)):
+        assert list(s_got) == list(s_ref), (i, s_got, s_ref)
+    err = np.max(np.abs(np.array(tr.storage["psi_traj"]) - np.array(ref.psi_traj)))
+    print("adaptive interpolated noise max |dpsi| = %.3e" % err)
+    assert len(tr.storage["psi_traj"]) == len(ref.psi_traj) == 25
+    assert err < FP64_TOL
