@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="nontemporal hint on streamed arrays with --lanes")
     ap.add_argument("--groups-q", type=int, default=0, help="trajectory-group split with --lanes (0: automatic)")
     ap.add_argument("--tile-order", type=int, default=0, help="locality tile order with --lanes (0 or 1)")
+    ap.add_argument("--vec", type=int, default=1, help="trajectory lanes per thread with --lanes (1, 2 or 4)")
+    ap.add_argument("--lds", type=int, default=0, help="LDS-staged stationary kernel with --lanes (0 or 1)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
@@ -318,7 +320,7 @@ def main():
         choice, tune_t = autotune.tune(batch, psi0(), DT, verbose=True)
     else:
         batch.relayout(args.lanes)
-        choice, tune_t = (args.lanes, True, args.chunk, args.nt, args.groups_q, args.tile_order), {}
+        choice, tune_t = (args.lanes, True, args.chunk, args.nt, args.groups_q, args.tile_order, args.vec, args.lds), {}
         autotune.apply(choice, batch)
     st = batch.state
 
@@ -398,8 +400,11 @@ def main():
     rt = "double" if args.precision == "fp64" else "float"
     pat_w = int(np.max(np.diff(np.asarray(tb.tables["pat_ptr"])))) if len(tb.tables["pat_ptr"]) > 1 else 0
     if choice[1] and 0 < pat_w <= 8 and tb.ell_w <= 16:
-        kernel_name = "k_stage_shared<%s, %d, %d, %d>" % (rt, 4 if tb.ell_w <= 4 else (8 if tb.ell_w <= 8 else 16),
-                                                          4 if pat_w <= 4 else 8, choice[3])
+        vec = choice[6]
+        while vec > 1 and (vec > choice[0] or B % vec != 0):
+            vec //= 2
+        kernel_name = "%s<%s, %d, %d, %d, %d>" % ("k_stage_lds" if choice[7] else "k_stage_shared", rt, 4 if tb.ell_w <= 4 else (8 if tb.ell_w <= 8 else 16),
+                                                              4 if pat_w <= 4 else 8, choice[3], vec)
     else:
         kernel_name = "k_stage<%s>" % rt
     out = {
@@ -432,8 +437,9 @@ def main():
         "init_s": t_init,
         "layout": {"lanes": choice[0], "stationary_stage": bool(choice[1]), "chunk_traj": choice[2],
                    "nt_stream": choice[3], "groups_q": choice[4], "tile_order": bool(choice[5]),
-                   "autotune_ms_per_step (lanes/stationary/chunk/nt/q/order)":
-                       {"%d/%d/%d/%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
+                   "vec_lanes": choice[6], "lds_stage": bool(choice[7]),
+                   "autotune_ms_per_step (lanes/stationary/chunk/nt/q/order/vec/lds)":
+                       {"%d/%d/%d/%d/%d/%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": kernel_name, "kernel_avg_ms": stage_ms,

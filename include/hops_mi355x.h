@@ -151,7 +151,14 @@ int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
 
 /* Engine options (no counterpart in the reference).  "stationary_stage":
    1 (default) runs the operator-stationary stage kernel for batches of at
-   least 8 trajectories with ell_w <= 16; 0 forces the streaming kernel. */
+   least 8 trajectories with ell_w <= 16; 0 forces the streaming kernel.
+   "chunk_traj": trajectories per chunk propagated through all steps before
+   the next (0, default: the whole batch).  "nt_stream": nontemporal hint
+   level 0..4 of the stationary kernel.  "groups_q": trajectory-group split
+   of the stationary kernel (0: automatic).  "vec_lanes": 1 (default), 2 or
+   4 adjacent trajectory lanes per thread of the stationary kernel (reduced
+   to the interleave and to a divisor of the batch size).  No option changes
+   any result. */
 int hops_set_option(const char *name, int value);
 
 /* Fill op->ell_col_d/ell_val_d/k0_d from the connection tables.

@@ -691,12 +691,6 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
   stage_row<R, 0>(op, bt, stage, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
 }
 
-// Row update with the row's pattern and ELL entries in registers and every
-// load issued before the arithmetic (more bytes in flight per wave).
-// pidx[p] < 0 and ecol[e] < 0 mark unused slots.  NT >= 1: the accumulator
-// and the carry are read and written with the nontemporal hint; NT >= 2: the
-// stage output and the complex64 state copy are stored with it too; NT >= 3:
-// the row's own aux block is read with it; NT = 4: the ELL neighbours too.
 // Element i of an array addressed as a uniform base pointer plus a 32-bit
 // byte offset, which the compiler issues as a global load/store with a scalar
 // base and one offset register (the pools of the stationary kernel are below
@@ -710,11 +704,67 @@ __device__ __forceinline__ const T *el(const T *base, uint32_t i) {
   return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
 }
 
-template <typename R, int WR, int PW, int NT>
+// V consecutive elements starting at p (p aligned to V * sizeof(T)) as one
+// 8-byte access or a run of 16-byte accesses, optionally nontemporal.
+template <int NT, int V, typename T>
+__device__ __forceinline__ void ldv(const T *p, T (&o)[V]) {
+  constexpr int NB = V * (int)sizeof(T);
+  if constexpr (NB == 8) {
+    o[0] = ld_s<NT>(p);
+  } else {
+    static_assert(NB % 16 == 0, "16-byte runs");
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const u4 *q = reinterpret_cast<const u4 *>(p);
+#pragma unroll
+    for (int i = 0; i < NB / 16; ++i) {
+      u4 u;
+      if constexpr (NT != 0) u = __builtin_nontemporal_load(q + i);
+      else u = q[i];
+      __builtin_memcpy(reinterpret_cast<char *>(o) + 16 * i, &u, 16);
+    }
+  }
+}
+template <int NT, int V, typename T>
+__device__ __forceinline__ void stv(T *p, const T (&v)[V]) {
+  constexpr int NB = V * (int)sizeof(T);
+  if constexpr (NB == 8) {
+    st_s<NT>(p, v[0]);
+  } else {
+    static_assert(NB % 16 == 0, "16-byte runs");
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 *q = reinterpret_cast<u4 *>(p);
+#pragma unroll
+    for (int i = 0; i < NB / 16; ++i) {
+      u4 u;
+      __builtin_memcpy(&u, reinterpret_cast<const char *>(v) + 16 * i, 16);
+      if constexpr (NT != 0) __builtin_nontemporal_store(u, q + i);
+      else q[i] = u;
+    }
+  }
+}
+
+// Row update of V adjacent trajectory lanes b .. b+V-1 of one row (b a
+// multiple of V), with the row's pattern and ELL entries in registers and
+// every load issued before the arithmetic.  The lanes of a row are contiguous
+// in every pool, so each operand of the V lanes is one wide access: V = 2
+// halves the vector-memory instructions per element, V = 4 quarters them.
+// Each lane performs exactly the operation sequence of stage_row.
+// pidx[p] < 0 and ecol[e] < 0 mark unused slots.  NT >= 1: the accumulator
+// and the carry are read and written with the nontemporal hint; NT >= 2: the
+// stage output and the complex64 state copy are stored with it too; NT >= 3:
+// the row's own aux block is read with it; NT = 4: the ELL neighbours too.
+//
+// LDS = true (k_stage_lds): the row's own value comes in yr_pre, and its
+// pattern values and coefficients are read from LDS: ly points at lane b of
+// the row's aux block (element s' of lane b + v at ly[s' T + v]), lc at lane
+// b of the coefficient set of the row (coefficient p at lc[p T + v]).
+template <typename R, int WR, int PW, int NT, int V, bool LDS = false>
 __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_batch_t &bt, int stage,
                                               const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
                                               int b, long r_, int k, cx<R> k0, const int *ecol,
-                                              const cx<R> *ev, const int *pidx, const int *pcol) {
+                                              const cx<R> *ev, const int *pidx, const int *pcol,
+                                              const cx<R> *ly = nullptr, const cx<R> *lc = nullptr,
+                                              const cx<R> *yr_pre = nullptr) {
 #pragma clang fp contract(off)
   const uint32_t n = (uint32_t)op.n_state;
   const uint32_t nr = n * (uint32_t)op.n_hier;
@@ -731,82 +781,128 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
   // loads
   constexpr int NTO = NT >= 3 ? 1 : 0;  // the row's own aux block
   constexpr int NTG = NT >= 4 ? 1 : 0;  // the ELL neighbours
-  const cx<R> yr = ld_s<NTO>(el(yin, gi));
-  cx<R> yp[PW], ye[WR], ap[PW];
+  cx<R> yr[V];
+  if constexpr (LDS) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) yr[v] = yr_pre[v];
+  } else {
+    ldv<NTO, V>(el(yin, gi), yr);
+  }
+  cx<R> yp[PW][V], ye[WR][V], ap[PW][V];
   const uint32_t a0 = (uint32_t)b + (k == 0 ? (uint32_t)op.n_pat * cs : 0u);
 #pragma unroll
   for (int p = 0; p < PW; ++p) {
-    yp[p] = pidx[p] >= 0 ? ld_s<NTO>(el(yin, gb + (rowk + (uint32_t)pcol[p]) * T)) : zero;
-    ap[p] = pidx[p] >= 0 ? *el(cT, a0 + (uint32_t)pidx[p] * cs) : zero;
+    if (pidx[p] >= 0) {
+      if constexpr (LDS) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          yp[p][v] = ly[pcol[p] * T + v];
+          ap[p][v] = lc[pidx[p] * T + v];
+        }
+      } else {
+        ldv<NTO, V>(el(yin, gb + (rowk + (uint32_t)pcol[p]) * T), yp[p]);
+        ldv<0, V>(el(cT, a0 + (uint32_t)pidx[p] * cs), ap[p]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) yp[p][v] = ap[p][v] = zero;
+    }
   }
 #pragma unroll
-  for (int e = 0; e < WR; ++e) ye[e] = ecol[e] >= 0 ? ld_s<NTG>(el(yin, gb + (uint32_t)ecol[e] * T)) : zero;
-  const double sigma = bt.sigT_d[b];
-  cx<R> base = zero, a_in = zero;
-  double2 ph = make_double2(0.0, 0.0);
+  for (int e = 0; e < WR; ++e) {
+    if (ecol[e] >= 0) {
+      ldv<NTG, V>(el(yin, gb + (uint32_t)ecol[e] * T), ye[e]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) ye[e][v] = zero;
+    }
+  }
+  double sigma[V];
+  ldv<0, V>(bt.sigT_d + b, sigma);
+  cx<R> base[V], a_in[V];
+  double2 ph[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    base[v] = a_in[v] = zero;
+    ph[v] = make_double2(0.0, 0.0);
+  }
   if (stage < 3) {
     if (bt.precision == HOPS_PREC_MIXED) {
-      base = ld_s<NT>(el(reinterpret_cast<const cx<R> *>(bt.y0_d), gi));
+      ldv<NT, V>(el(reinterpret_cast<const cx<R> *>(bt.y0_d), gi), base);
     } else {
-      const double2 p = ld_s<NT>(el(phi, gi));
-      base = mk<R>((R)p.x, (R)p.y);
+      ldv<NT, V>(el(phi, gi), ph);
     }
   } else {
-    ph = ld_s<NT>(el(phi, gi));
+    ldv<NT, V>(el(phi, gi), ph);
   }
-  if (stage > 0) a_in = ld_s<NT>(el(acc, gi));
-  // arithmetic
-  cx<R> d = cmul(k0, yr);
+  if (stage > 0) ldv<NT, V>(el(acc, gi), a_in);
+  // arithmetic, lane by lane
+  cx<R> d[V];
 #pragma unroll
-  for (int p = 0; p < PW; ++p) cfma(d, ap[p], yp[p]);
+  for (int v = 0; v < V; ++v) {
+    d[v] = cmul(k0, yr[v]);
 #pragma unroll
-  for (int e = 0; e < WR; ++e) cfma(d, ev[e], ye[e]);
+    for (int p = 0; p < PW; ++p) cfma(d[v], ap[p][v], yp[p][v]);
+#pragma unroll
+    for (int e = 0; e < WR; ++e) cfma(d[v], ev[e], ye[e][v]);
+  }
   if (bt.eom != HOPS_EOM_LINEAR) {
     const uint32_t zc0 = (uint32_t)b + 2u * (uint32_t)op.n_pat * cs;
-    for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u)
-      cfma(d, *el(cT, zc0 + (uint32_t)op.up_mode_d[u] * cs),
-           *el(yin, gb + ((uint32_t)op.up_aux_d[u] * n + (r - rowk)) * T));
+    for (int u = op.up_ptr_d[k]; u < op.up_ptr_d[k + 1]; ++u) {
+      cx<R> zc[V], yu[V];
+      ldv<0, V>(el(cT, zc0 + (uint32_t)op.up_mode_d[u] * cs), zc);
+      ldv<0, V>(el(yin, gb + ((uint32_t)op.up_aux_d[u] * n + (r - rowk)) * T), yu);
+#pragma unroll
+      for (int v = 0; v < V; ++v) cfma(d[v], zc[v], yu[v]);
+    }
     if (bt.eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
-      const cx<R> nc = *el(cT, (uint32_t)b + (2u * (uint32_t)op.n_pat + (uint32_t)op.n_mode) * cs);
-      d = csub(d, cmul(nc, yr));
+      cx<R> nc[V];
+      ldv<0, V>(el(cT, (uint32_t)b + (2u * (uint32_t)op.n_pat + (uint32_t)op.n_mode) * cs), nc);
+#pragma unroll
+      for (int v = 0; v < V; ++v) d[v] = csub(d[v], cmul(nc[v], yr[v]));
     }
   }
-  if (stage == 0) d = cscale(d, (R)sigma);
   const double h = bt.dt_over_hbar;
   if (stage < 3) {
-    if (bt.precision == HOPS_PREC_MIXED) {
-      base = cscale(base, (R)sigma);
-    } else {
-      const double2 p = ld_s<NT>(el(phi, gi));
-      base = mk<R>((R)(sigma * p.x), (R)(sigma * p.y));
-    }
     const R c = (R)((stage == 2 ? 1.0 : 0.5) * h);
-    st_s<NT >= 2 ? 1 : 0>(el(yout, gi), mk<R>(base.x + c * d.x, base.y + c * d.y));
-    if (stage == 0) {
-      st_s<NT>(el(acc, gi), d);
-    } else {
-      a_in.x += (R)2 * d.x;
-      a_in.y += (R)2 * d.y;
-      st_s<NT>(el(acc, gi), a_in);
+    cx<R> o[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      if (stage == 0) d[v] = cscale(d[v], (R)sigma[v]);
+      if (bt.precision == HOPS_PREC_MIXED) base[v] = cscale(base[v], (R)sigma[v]);
+      else base[v] = mk<R>((R)(sigma[v] * ph[v].x), (R)(sigma[v] * ph[v].y));
+      o[v] = mk<R>(base[v].x + c * d[v].x, base[v].y + c * d[v].y);
+      if (stage == 0) {
+        a_in[v] = d[v];
+      } else {
+        a_in[v].x += (R)2 * d[v].x;
+        a_in[v].y += (R)2 * d[v].y;
+      }
     }
+    stv<NT >= 2 ? 1 : 0, V>(el(yout, gi), o);
+    stv<NT, V>(el(acc, gi), a_in);
   } else {
     const double c = h / 6.0;
-    double2 q;
-    q.x = sigma * ph.x + c * ((double)a_in.x + (double)d.x);
-    q.y = sigma * ph.y + c * ((double)a_in.y + (double)d.y);
-    st_s<NT>(el(phi, gi), q);
-    if (bt.precision == HOPS_PREC_MIXED)
-      st_s<NT >= 2 ? 1 : 0>(el(reinterpret_cast<cx<float> *>(bt.y0_d), gi), mk<float>((float)q.x, (float)q.y));
+    double2 q[V];
+    cx<float> q32[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      q[v].x = sigma[v] * ph[v].x + c * ((double)a_in[v].x + (double)d[v].x);
+      q[v].y = sigma[v] * ph[v].y + c * ((double)a_in[v].y + (double)d[v].y);
+      q32[v] = mk<float>((float)q[v].x, (float)q[v].y);
+    }
+    stv<NT, V>(el(phi, gi), q);
+    if (bt.precision == HOPS_PREC_MIXED) stv<NT >= 2 ? 1 : 0, V>(el(reinterpret_cast<cx<float> *>(bt.y0_d), gi), q32);
   }
 }
 
 // Operator-stationary form for a batch that shares one operator: workgroup i
-// owns a tile of 512 / T rows, keeps its K0 and ELL rows in registers and
-// walks the trajectory groups g = x + 8 (q + Q j) of XCD x = i % 8, so the
-// operator is read once per tile and stage instead of once per trajectory,
-// and every XCD works on the same few groups at a time (their rows stay in
-// its L2).
-template <typename R, int WR, int PW, int NT>
+// owns a tile of 512 V / T rows (T / V threads per row, V lanes each), keeps
+// its K0 and ELL rows in registers and walks the trajectory groups
+// g = x + 8 (q + Q j) of XCD x = i % 8, so the operator is read once per tile
+// and stage instead of once per trajectory, and every XCD works on the same
+// few groups at a time (their rows stay in its L2).
+template <typename R, int WR, int PW, int NT, int V>
 __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const hops_batch_t bt, int stage,
                                                       const cx<R> *__restrict__ yin,
                                                       cx<R> *__restrict__ yout, int n_tiles, int Q) {
@@ -815,8 +911,9 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
   const int tile = bt.tile_order_d ? bt.tile_order_d[slot - q * n_tiles] : slot - q * n_tiles;
   const int n = op.n_state;
   const long nr = (long)n * op.n_hier;
-  const int lane = threadIdx.x & ((1 << bt.lane_shift) - 1);
-  const long r = (long)tile * (512 >> bt.lane_shift) + (threadIdx.x >> bt.lane_shift);
+  const int tps = bt.lane_shift - (V == 4 ? 2 : (V == 2 ? 1 : 0));  // log2 threads per row
+  const int lane = (threadIdx.x & ((1 << tps) - 1)) * V;
+  const long r = (long)tile * (512 >> tps) + (threadIdx.x >> tps);
   if (r >= nr) return;
   const int k = (int)(r / n), s = (int)(r - (long)k * n);
   int ecol[WR];
@@ -843,7 +940,128 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
   const int G = (bt.B + (1 << bt.lane_shift) - 1) >> bt.lane_shift;
   for (int g = x + 8 * q; g < G; g += 8 * Q) {
     const int b = (g << bt.lane_shift) + lane;
-    if (b < bt.B) stage_row_reg<R, WR, PW, NT>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
+    if (b < bt.B) stage_row_reg<R, WR, PW, NT, V>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
+  }
+}
+
+// LDS-staged operator-stationary form.  As k_stage_shared, but a tile is
+// aux_per_tile whole aux blocks, and for each trajectory group the tile's own
+// values and the group's stage coefficients are staged in LDS; the
+// system-block pattern values and coefficients of every row are then LDS
+// reads instead of L1/L2 gathers.  The LDS is double-buffered and the next
+// group's own values and coefficients are loaded into registers while the
+// current group computes, so each group costs one barrier and no exposed
+// load.  Threads beyond the tile's rows only help stage the coefficients.
+// Every thread reaches the barrier of every group iteration.
+template <typename R, int V>
+struct LdsStage {
+  cx<R> yr[V];  // own values of the group (active rows)
+  cx<R> c[V];   // one staged coefficient vector of the group (if cidx >= 0)
+  int cidx;     // its LDS offset, or -1
+};
+template <typename R, int WR, int PW, int NT, int V>
+__global__ __launch_bounds__(512) void k_stage_lds(const hops_op_t op, const hops_batch_t bt, int stage,
+                                                   const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
+                                                   int n_tiles, int Q, int aux_per_tile) {
+  extern __shared__ double lds_raw[];
+  const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int q = slot / n_tiles;
+  const int tile = slot - q * n_tiles;
+  const int n = op.n_state, np = op.n_pat;
+  const int ls = bt.lane_shift, T = 1 << ls;
+  const int tps = ls - (V == 4 ? 2 : (V == 2 ? 1 : 0));  // log2 threads per row
+  const int TR = 1 << tps;
+  const int lr = threadIdx.x >> tps;                       // row within the tile
+  const int lane = (threadIdx.x & (TR - 1)) * V;
+  const int k_first = tile * aux_per_tile;
+  const int n_aux = min(aux_per_tile, op.n_hier - k_first);
+  const bool active = lr < n_aux * n;
+  const int kl = active ? lr / n : 0;
+  const int k = k_first + kl, s = active ? lr - kl * n : 0;
+  const long nr = (long)n * op.n_hier;
+  const long r = (long)k * n + s;
+  const int nsets = k_first == 0 ? 2 : 1;  // the k = 0 coefficient set follows the k > 0 set
+  // LDS buffer h (0, 1): rows [aux_per_tile n][T], then coefficients [2 np][T]
+  const size_t buf = (size_t)T * ((size_t)aux_per_tile * n + 2 * (size_t)np);
+  cx<R> *lds = reinterpret_cast<cx<R> *>(lds_raw);
+  int ecol[WR];
+  cx<R> ev[WR];
+  const cx<R> *evm = reinterpret_cast<const cx<R> *>(op.ell_val_d);
+#pragma unroll
+  for (int e = 0; e < WR; ++e) {
+    if (active && e < op.ell_w) {
+      ecol[e] = op.ell_col_d[(long)e * nr + r];
+      ev[e] = evm[(long)e * nr + r];
+    } else {
+      ecol[e] = -1;
+      ev[e] = mk<R>((R)0, (R)0);
+    }
+  }
+  int pidx[PW], pcol[PW];
+  const int p0 = op.pat_ptr_d[s], npr = active ? op.pat_ptr_d[s + 1] - p0 : 0;
+#pragma unroll
+  for (int p = 0; p < PW; ++p) {
+    pidx[p] = p < npr ? p0 + p : -1;
+    pcol[p] = p < npr ? op.pat_col_d[p0 + p] : 0;
+  }
+  const cx<R> k0 = ccast<R>(ld2(op.k0_d, k));
+  const cx<R> *cT = reinterpret_cast<const cx<R> *>(bt.coefT_d);
+  const uint32_t cs = (uint32_t)bt.coefT_stride;
+  const uint32_t nrT = (uint32_t)nr * (uint32_t)T;
+  constexpr int NTO = NT >= 3 ? 1 : 0;
+  const int G = (bt.B + T - 1) >> ls;
+  const int ncv = nsets * np * TR;  // coefficient vectors of a group
+  // global loads of group g into registers (coefficient vectors
+  // threadIdx.x + 512 j: at most one per thread when ncv <= 512, else a loop)
+  auto fetch = [&](int g, LdsStage<R, V> &f) {
+    f.cidx = -1;
+    if (threadIdx.x < ncv) {
+      const int pr = threadIdx.x >> tps, sl = threadIdx.x & (TR - 1);
+      const int bb = (g << ls) + sl * V;
+      if (bb < bt.B) {
+        ldv<0, V>(el(cT, (uint32_t)bb + (uint32_t)pr * cs), f.c);
+        f.cidx = pr * T + sl * V;
+      }
+    }
+    const int b = (g << ls) + lane;
+    if (active && b < bt.B) ldv<NTO, V>(el(yin, (uint32_t)g * nrT + (uint32_t)r * (uint32_t)T + (uint32_t)lane), f.yr);
+  };
+  int h = 0;
+  LdsStage<R, V> f;
+  int g = x + 8 * q;
+  if (g < G) fetch(g, f);
+  for (; g < G; g += 8 * Q, h ^= 1) {
+    cx<R> *ys = lds + h * buf;
+    cx<R> *cl = ys + (size_t)aux_per_tile * n * T;
+    const int b = (g << ls) + lane;
+    const bool run = active && b < bt.B;
+    if (run) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) ys[lr * T + lane + v] = f.yr[v];
+    }
+    if (f.cidx >= 0) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) cl[f.cidx + v] = f.c[v];
+    }
+    for (int i = threadIdx.x + 512; i < ncv; i += 512) {  // groups with more than 512 coefficient vectors
+      const int pr = i >> tps, sl = i & (TR - 1);
+      const int bb = (g << ls) + sl * V;
+      if (bb < bt.B) {
+        cx<R> c[V];
+        ldv<0, V>(el(cT, (uint32_t)bb + (uint32_t)pr * cs), c);
+#pragma unroll
+        for (int v = 0; v < V; ++v) cl[pr * T + sl * V + v] = c[v];
+      }
+    }
+    cx<R> yr[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) yr[v] = f.yr[v];
+    __syncthreads();
+    if (g + 8 * Q < G) fetch(g + 8 * Q, f);  // next group, in flight during this one
+    if (run)
+      stage_row_reg<R, WR, PW, NT, V, true>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol,
+                                            ys + (size_t)(kl * n) * T + lane,
+                                            cl + (size_t)(k == 0 ? np : 0) * T + lane, yr);
   }
 }
 
@@ -949,6 +1167,8 @@ bool g_stationary = true;
 int g_chunk_traj = 0;
 int g_nt_stream = 0;  // nontemporal hint level of k_stage_shared (see stage_row_reg)
 int g_groups_q = 0;        // trajectory-group split of k_stage_shared (0: automatic)
+int g_vec_lanes = 1;       // trajectory lanes per thread of k_stage_shared (1, 2 or 4)
+int g_lds_stage = 0;       // 1: LDS-staged form k_stage_lds where it fits
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
   if (!g_prof.on) return 0;
@@ -1017,16 +1237,35 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const int Gc = (chunk + T - 1) / T;
   int sh_w = 0, sh_p = 0, sh_tiles = 0, sh_q = 1;
   unsigned sh_grid = 0;
+  int lds_apt = 0, lds_tiles = 0;
+  size_t lds_bytes = 0;
+  unsigned lds_grid = 0;
+  // lanes per thread of the stationary kernel: at most the interleave, and a
+  // divisor of every (sub-)batch size so no vector straddles the last lane
+  int vl = g_vec_lanes;
+  while (vl > 1 && (vl > T || bt->B % vl != 0 || chunk % vl != 0)) vl >>= 1;
   // the stationary kernel addresses its pools with 32-bit byte offsets
   const bool pools32 = (double)nr * (double)((long)G << bt->lane_shift) * 16.0 < 4294967296.0;
   if (g_stationary && pools32 && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
     sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
     sh_p = max_pat <= 4 ? 4 : 8;
-    const long rows_per_tile = 512 >> bt->lane_shift;
+    const long rows_per_tile = (512L * vl) >> bt->lane_shift;
     sh_tiles = (int)((nr + rows_per_tile - 1) / rows_per_tile);
     const int per_xcd = (Gc + 7) / 8;
     sh_q = g_groups_q > 0 ? std::min(g_groups_q, per_xcd) : (per_xcd + 31) / 32;
     sh_grid = (unsigned)(8L * sh_q * sh_tiles);
+    // LDS-staged form: whole aux blocks per tile, staged rows and
+    // coefficients within 64 KiB
+    const int apt = (int)(rows_per_tile / n);
+    if (g_lds_stage && apt >= 1) {
+      const size_t lds = 2 * sizeof(cx<R>) * (size_t)T * ((size_t)apt * n + 2 * (size_t)op->n_pat);
+      if (lds <= 64 * 1024) {
+        lds_apt = apt;
+        lds_bytes = lds;
+        lds_tiles = (op->n_hier + apt - 1) / apt;
+        lds_grid = (unsigned)(8L * sh_q * lds_tiles);
+      }
+    }
   } else {
     chunk = bt->B;  // chunks need the stationary kernel (no tile list per chunk)
   }
@@ -1043,13 +1282,23 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       k_prologue<R><<<sb.B, 256, shm, st>>>(*op, sb, i, itb + toff[i], yin, slot);
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 0, &used)) return rc;
+#define HOPS_SHV(W, P, NT)                                                                       \
+  do {                                                                                           \
+    if (lds_apt > 0) {                                                                           \
+      if (vl == 4) k_stage_lds<R, W, P, NT, 4><<<lds_grid, 512, lds_bytes, st>>>(*op, sb, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else if (vl == 2) k_stage_lds<R, W, P, NT, 2><<<lds_grid, 512, lds_bytes, st>>>(*op, sb, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else k_stage_lds<R, W, P, NT, 1><<<lds_grid, 512, lds_bytes, st>>>(*op, sb, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+    } else if (vl == 4) k_stage_shared<R, W, P, NT, 4><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    else if (vl == 2) k_stage_shared<R, W, P, NT, 2><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    else k_stage_shared<R, W, P, NT, 1><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+  } while (0)
 #define HOPS_SH(W, P)                                                                            \
   do {                                                                                           \
-    if (g_nt_stream == 1) k_stage_shared<R, W, P, 1><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
-    else if (g_nt_stream == 2) k_stage_shared<R, W, P, 2><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
-    else if (g_nt_stream == 3) k_stage_shared<R, W, P, 3><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
-    else if (g_nt_stream == 4) k_stage_shared<R, W, P, 4><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
-    else k_stage_shared<R, W, P, 0><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+    if (g_nt_stream == 1) HOPS_SHV(W, P, 1);                                                     \
+    else if (g_nt_stream == 2) HOPS_SHV(W, P, 2);                                                \
+    else if (g_nt_stream == 3) HOPS_SHV(W, P, 3);                                                \
+    else if (g_nt_stream == 4) HOPS_SHV(W, P, 4);                                                \
+    else HOPS_SHV(W, P, 0);                                                                      \
   } while (0)
       if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
       else if (sh_w == 4 && sh_p == 8) HOPS_SH(4, 8);
@@ -1059,6 +1308,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       else if (sh_w == 16 && sh_p == 8) HOPS_SH(16, 8);
       else k_stage<R><<<sb.n_tiles, 256, 0, st>>>(*op, sb, i, yin, yout);
 #undef HOPS_SH
+#undef HOPS_SHV
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(st, 1, &used)) return rc;
     }
@@ -1457,6 +1707,15 @@ int hops_set_option(const char *name, int value) {
   if (std::string(name) == "nt_stream") {
     if (value < 0 || value > 4) return fail("nt_stream must be 0..4");
     g_nt_stream = value;
+    return 0;
+  }
+  if (std::string(name) == "lds_stage") {
+    g_lds_stage = value != 0;
+    return 0;
+  }
+  if (std::string(name) == "vec_lanes") {
+    if (value != 1 && value != 2 && value != 4) return fail("vec_lanes must be 1, 2 or 4");
+    g_vec_lanes = value;
     return 0;
   }
   if (std::string(name) == "groups_q") {

@@ -272,6 +272,40 @@ def test_stationary_kernel_options_bit_identical():
         assert d == 0.0
 
 
+@pytest.mark.parametrize("eom", ["LINEAR", "NORMALIZED NONLINEAR"])
+def test_vector_lanes_bit_identical(eom):
+    """Two or four trajectory lanes per thread of the stationary kernel change
+    only the width of its memory accesses, and its LDS-staged form only where
+    the pattern values and coefficients are read from: every lane performs the
+    same operations, so trajectories are bit-identical to one lane per thread,
+    at every interleave and hint level, in both precisions."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1)
+    tlen, tau = 50.0, 0.5
+    B = 72
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[0] = 1
+    for prec in ("mixed", "fp64"):
+        ref = engine_run(sysp, eom, zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, lanes=4)
+        for lanes, vec, nt, lds in ((4, 2, 0, 0), (4, 4, 0, 0), (8, 2, 2, 0), (8, 4, 2, 0), (8, 4, 3, 0),
+                                    (2, 4, 0, 0), (4, 1, 0, 1), (4, 2, 2, 1), (8, 2, 2, 1), (8, 4, 3, 1),
+                                    (1, 1, 0, 1)):
+            _lib.set_option("vec_lanes", vec)
+            _lib.set_option("nt_stream", nt)
+            _lib.set_option("lds_stage", lds)
+            try:
+                got = engine_run(sysp, eom, zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, lanes=lanes)
+            finally:
+                _lib.set_option("vec_lanes", 1)
+                _lib.set_option("nt_stream", 0)
+                _lib.set_option("lds_stage", 0)
+            d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+            print(prec, "lanes", lanes, "vec", vec, "nt", nt, "lds", lds, "max diff", d)
+            assert d == 0.0
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_interleaved_lanes_equal_plain_layout(lanes):
     """Interleaving T trajectories per row changes only the memory layout:
