@@ -39,6 +39,13 @@ sys.path.insert(0, HERE)
 
 METRIC = "traj·RK4-steps/s at ≤1e-4 pop. error vs FP64; HBM GB/s; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per stage launch of this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes (scaled x2.00 and x1.00 by a 512 MiB device copy counted
+# in the same runs), by stage kernel and interleave
+PROFILED_TRAFFIC = {
+    ("k_stage_lds<float, 4, 4, 2, 2>", 8): (1.305e9, "profiles/k_stage_lds_float_v2_nt2_chain10_d4_b256_lanes8.md"),
+    ("k_stage_shared<float, 4, 4, 2, 1>", 4): (1.436e9, "profiles/k_stage_shared_float_nt2_chain10_d4_b256_lanes4.md"),
+}
 
 
 def c2_system():
@@ -407,6 +414,9 @@ def main():
                                                               4 if pat_w <= 4 else 8, choice[3], vec)
     else:
         kernel_name = "k_stage<%s>" % rt
+    traffic, traffic_src = PROFILED_TRAFFIC.get((kernel_name, choice[0]), (None, None))
+    if B != 256 or args.precision != "mixed" or choice[2]:
+        traffic, traffic_src = None, None  # profiled for the default workload only
     out = {
         "metric": METRIC,
         "value": value,
@@ -441,7 +451,7 @@ def main():
                    "autotune_ms_per_step (lanes/stationary/chunk/nt/q/order/vec/lds)":
                        {"%d/%d/%d/%d/%d/%d/%d/%d" % k: v * 1e3 for k, v in tune_t.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": kernel_name, "kernel_avg_ms": stage_ms,
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "prologue_avg_ms": ms[0] / max(n[0], 1)},

@@ -18,6 +18,7 @@ from .errors import UnsupportedRequest
 h = 33357.179  # cm^-1 fs
 hbar = h / (2 * np.pi)  # cm^-1 fs / rad
 kB = 0.69503  # cm^-1 / K
+c = 299.79  # nm / fs
 precision = 1e-8
 
 EOM_DICT_DEFAULT = {
@@ -126,3 +127,31 @@ def bcf_convert_dl_to_exp(lambda_dl, gamma_dl, temp, k_matsubara=0):
         out += [2j * jw / beta, nu]
         g0 += 2 * lambda_dl * (1 / beta) * (2 * gamma_dl ** 2) / (gamma_dl ** 2 - nu ** 2)
     return [g0, gamma_dl] + out
+
+
+def ishizaki_decomposition_bcf_dl(lambda_dl, gamma_dl, temp, k_matsubara, epsilon=None):
+    """Drude-Lorentz-like correlation function as exponential modes
+    [g0, w0, g0', w0', g1, w1, ...] after Ishizaki, J. Phys. Soc. Jpn. 89,
+    015001 (2020): a complex-conjugate pair of underdamped-like modes at
+    w = 2 gamma +- i epsilon (epsilon defaults to gamma / 10, and must be much
+    smaller than gamma), then k_matsubara Matsubara modes g_k = 2i J(i nu_k) /
+    beta, w_k = nu_k, each also moving the real part of the first pair's
+    prefactor (util/bath_corr_functions.py:115-182)."""
+    eps = gamma_dl / 10 if epsilon is None else epsilon
+    big = 2 * gamma_dl
+    w_up, w_dn = big + 1j * eps, big - 1j * eps
+    beta = 1.0 / (kB * temp)
+    nu1 = 2 * np.pi / beta
+
+    def spectral(w):
+        return 4 * lambda_dl * big ** 3 * w / ((w ** 2 + big ** 2) ** 2)
+
+    g_re = 1j * lambda_dl * w_dn / (eps * beta)
+    g_im = 1j * lambda_dl * w_up * w_dn / (2 * eps)
+    mats = []
+    for k in range(1, int(k_matsubara) + 1):
+        nu = k * nu1
+        mats += [(2j / beta) * spectral(1j * nu), nu]
+        e_tilde = (2 * w_up ** 2) / ((w_up ** 2 - nu ** 2) * beta)
+        g_re += 2 * np.real((1j * lambda_dl * w_dn / eps) * e_tilde)
+    return [g_re - 1j * g_im, w_up, np.conj(g_re) - 1j * np.conj(g_im), np.conj(w_up)] + mats

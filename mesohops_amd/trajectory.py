@@ -635,6 +635,23 @@ class HopsTrajectory:
             return self._adaptive.phi_host()
         return self._batch.phi[0]
 
+    @phi.setter
+    def phi(self, phi):
+        """Replaces the whole hierarchy vector (in the current basis) on the
+        device (trajectory/hops_trajectory.py:1099-1101).  The next step takes
+        it as given, like a state an operator has just produced."""
+        phi = np.asarray(phi, dtype=np.complex128).reshape(-1)
+        if len(phi) != self.n_state * self.n_hier:
+            raise TrajectoryError("phi of length %d does not match the basis (%d states x %d auxiliaries)"
+                                  % (len(phi), self.n_state, self.n_hier))
+        if self._adaptive is not None:
+            self._adaptive.st.set_phi(phi)
+            self._adaptive.raw_pending = True
+        else:
+            self._batch.state.set_phi(phi)
+            self._batch.state.raw_first = True
+        torch.cuda.synchronize(self._batch.device)
+
     @property
     def psi(self):
         return self.phi[:self.n_state]

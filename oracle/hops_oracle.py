@@ -14,6 +14,9 @@ trajectory), brute force and closed forms (derivative, RK4, z-memory,
 determine_error_thresh, cubic interpolation of cubics), invariants (adaptive
 with vanishing thresholds equals the full hierarchy; derivative error below
 delta; _operator equal to a block-by-block product; STATIC_BASIS kept).
+The dyadic part (tests/test_dyadic_host.py): the reference's dyadic-matrix
+and side-operator answers, and absorption / fluorescence without a bath
+against exact RK4 matrix powers.
 
 Parity unpinned: the raw draws of L-operator rows beyond those the fixture
 holds (index >= 2 for SUM_GAUSSIAN, >= 1 for BOX_MULLER) are NumPy's own
@@ -1117,3 +1120,131 @@ class AdaptiveTrajectory(Trajectory):
     def reset_early_time_integrator(self):
         """hops_trajectory.py:761-772."""
         self.early_counter = 0
+
+
+# ---------------------------------------------------------------------------
+# Dyadic HOPS (trajectory/hops_dyadic.py:15-246), the linear-spectroscopy
+# pathways (trajectory/dyadic_spectra.py:198-356) and the response function
+# (util/spectroscopy_analysis.py:67-113), on the trajectories above.
+# ---------------------------------------------------------------------------
+def dyad(m):
+    """[[M, 0], [0, M]] (hops_dyadic.py:113-143); dense input -> complex128."""
+    n = m.shape[0]
+    if sp.issparse(m):
+        return sp.block_diag((m, m), format="coo")
+    out = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    out[:n, :n] = m
+    out[n:, n:] = m
+    return out
+
+
+def dyadic_system(system_param):
+    """A copy of system_param with the Hamiltonian and every L-operator list
+    in dyadic form (hops_dyadic.py:63-78)."""
+    sysp = dict(system_param)
+    sysp["HAMILTONIAN"] = dyad(system_param["HAMILTONIAN"])
+    for key in ("L_HIER", "L_NOISE1", "L_NOISE2", "L_LT_CORR"):
+        if key in sysp:
+            sysp[key] = [dyad(m) for m in system_param[key]]
+    return sysp
+
+
+class Dyadic:
+    """A dyadic trajectory driving an oracle Trajectory (or AdaptiveTrajectory)
+    built on dyadic_system(...)."""
+
+    def __init__(self, traj):
+        self.tr = traj
+        self.norm_sq = []
+
+    def initialize(self, psi_ket, psi_bra):
+        """hops_dyadic.py:83-111."""
+        psi = np.concatenate((np.asarray(psi_ket, complex), np.asarray(psi_bra, complex)))
+        n2 = float(np.sum(np.abs(psi) ** 2))
+        self.norm_sq = [n2]
+        self.tr.initialize(psi / np.sqrt(n2))
+
+    def side_operator(self, op_h, side):
+        """hops_dyadic.py:145-214: op on the chosen side's block, the identity
+        on the other side's basis states; the response normalisation factor;
+        renormalisation for the normalised EOM."""
+        op_h = op_h.toarray() if sp.issparse(op_h) else np.asarray(op_h)
+        n = op_h.shape[0]
+        states = [int(s) for s in self.tr.basis.state_list]
+        before = float(np.sum(np.abs(self.tr.phi[:len(states)]) ** 2))
+        full = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+        for s in states:
+            if (s < n) == (side == "bra"):
+                full[s, s] = 1.0
+        blk = slice(n, 2 * n) if side == "bra" else slice(0, n)
+        full[blk, blk] = op_h
+        self.tr.operator(sp.csr_matrix(full))
+        ns = self.tr.basis.n_state
+        after = float(np.sum(np.abs(self.tr.phi[:ns]) ** 2))
+        if self.tr.eom_type == "NORMALIZED NONLINEAR":
+            self.norm_sq.append(after)
+            self.tr.phi = self.tr.phi / np.sqrt(after)
+        else:
+            self.norm_sq.append(after / before)
+
+    def full_traj(self):
+        """Stored wave functions over the whole dyadic state space."""
+        tr = self.tr
+        if hasattr(tr, "state_traj"):
+            return np.array(tr.psi_traj)
+        out = np.zeros((len(tr.psi_traj), tr.sysd["NSTATES"]), complex)
+        out[:, tr.basis.state_list] = np.array(tr.psi_traj)
+        return out
+
+    def response(self, F, index_t):
+        """spectroscopy_analysis.py:67-113 on the stored trajectory."""
+        F = F.toarray() if sp.issparse(F) else np.asarray(F)
+        scale = float(np.prod(self.norm_sq))
+        return np.array([scale / np.sum(np.abs(p) ** 2) * (np.conj(p) @ (F @ p))
+                         for p in self.full_traj()[int(index_t) + 1:]])
+
+
+def hilbert_field_operator(mu, field, sites, n_hilb, lower=False):
+    """Raising sum_s (mu_s . E) |s><0| or lowering |0><s| over sites
+    counted from 1 (dyadic_spectra.py:198-238)."""
+    op = np.zeros((n_hilb, n_hilb))
+    for s in np.asarray(sites):
+        w = float(np.dot(mu[s - 1], field))
+        if lower:
+            op[0, s] += w
+        else:
+            op[s, 0] += w
+    return op
+
+
+def signal_operator(mu, e_sig, n_chrom):
+    """sum_n (mu_n . E_sig) |0_bra><n_ket| in the doubled space
+    (dyadic_spectra.py:240-266)."""
+    n = n_chrom + 1
+    F = np.zeros((2 * n, 2 * n))
+    for k in range(n_chrom):
+        F[n, k + 1] = float(np.dot(mu[k], e_sig))
+    return F
+
+
+def dyadic_spectrum(dy, kind, mu, fields, ket_sites, bra_sites, t1, t2, t3, dt):
+    """The ABSORPTION / FLUORESCENCE pathways of calculate_spectrum
+    (dyadic_spectra.py:268-356) on an initialised-from-ground Dyadic."""
+    n_chrom = len(mu)
+    n = n_chrom + 1
+    g = np.zeros(n)
+    g[0] = 1.0
+    dy.initialize(g, g)
+    F = signal_operator(mu, fields["E_sig"], n_chrom)
+    if kind == "ABSORPTION":
+        dy.side_operator(hilbert_field_operator(mu, fields["E_1"], ket_sites, n), "ket")
+        dy.tr.propagate(t1, dt)
+        scale = 2
+    else:
+        dy.side_operator(hilbert_field_operator(mu, fields["E_1"], ket_sites, n), "ket")
+        dy.side_operator(hilbert_field_operator(mu, fields["E_2"], bra_sites, n), "bra")
+        dy.tr.propagate(t2, dt)
+        dy.side_operator(hilbert_field_operator(mu, fields["E_3"], np.arange(1, n), n, lower=True), "bra")
+        dy.tr.propagate(t3, dt)
+        scale = 4
+    return scale * dy.response(F, int(t2 / dt))
