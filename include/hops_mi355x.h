@@ -157,8 +157,11 @@ int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
    level 0..4 of the stationary kernel.  "groups_q": trajectory-group split
    of the stationary kernel (0: automatic).  "vec_lanes": 1 (default), 2 or
    4 adjacent trajectory lanes per thread of the stationary kernel (reduced
-   to the interleave and to a divisor of the batch size).  No option changes
-   any result. */
+   to the interleave and to a divisor of the batch size).  "lds_stage": 1
+   runs the LDS-staged form of the stationary kernel where its tiles fit.
+   "fuse_prologue": 1 (default) computes the four stage prologues of a LINEAR
+   step in one launch (stream-ordered scratch for the tables of stages 1-3).
+   No option changes any result. */
 int hops_set_option(const char *name, int value);
 
 /* Fill op->ell_col_d/ell_val_d/k0_d from the connection tables.

@@ -583,6 +583,26 @@ __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops
   prologue_body<R>(op, bt, blockIdx.x, stage, tidx, yin, store_slot);
 }
 
+// The prologues of all four stages of one LINEAR step in one launch.  LINEAR
+// stage coefficients depend on the noise only, not on the stage input
+// (eom/hops_eom.py:443-542), so workgroup (b, stage) can run before the step:
+// stage 0 writes the batch's own tables (and the psi store, sigma), stages
+// 1..3 write the tables coefT_s / coef_s + (stage - 1) * blk that their
+// stage kernels then read.
+template <typename R>
+__global__ __launch_bounds__(256) void k_prologue_linear4(const hops_op_t op, hops_batch_t bt, int tidx0, int ns,
+                                                         const cx<R> *__restrict__ yin, int store_slot,
+                                                         char *coefT_s, size_t coefT_blk, double *coef_s,
+                                                         size_t coef_blk) {
+  const int stage = blockIdx.y;
+  const int off = stage == 0 ? 0 : (stage == 3 ? 2 * ns : ns);  // noise index map [0, 1, 1, 2]
+  if (stage > 0) {
+    bt.coefT_d = coefT_s + (size_t)(stage - 1) * coefT_blk;
+    bt.coef_d = coef_s + (size_t)(stage - 1) * coef_blk;
+  }
+  prologue_body<R>(op, bt, blockIdx.x, stage, tidx0 + off, yin, stage == 0 ? store_slot : -1);
+}
+
 // Derivative of row r = (k, s) of trajectory b fused with its RK4 stage
 // update (eom/hops_eom.py:386-426, integrator/integrator_rk.py:53-73).  The
 // ELL row of the super-operator comes in ecol/ev: registers when WR > 0,
@@ -1169,6 +1189,7 @@ int g_nt_stream = 0;  // nontemporal hint level of k_stage_shared (see stage_row
 int g_groups_q = 0;        // trajectory-group split of k_stage_shared (0: automatic)
 int g_vec_lanes = 1;       // trajectory lanes per thread of k_stage_shared (1, 2 or 4)
 int g_lds_stage = 0;       // 1: LDS-staged form k_stage_lds where it fits
+int g_fuse_linear = 1;     // 1: one prologue launch per LINEAR step (k_prologue_linear4)
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
   if (!g_prof.on) return 0;
@@ -1269,28 +1290,65 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   } else {
     chunk = bt->B;  // chunks need the stationary kernel (no tile list per chunk)
   }
+  // fused LINEAR prologue: stream-ordered scratch for the coefficient
+  // tables of stages 1..3, freed on the same stream when the call ends
+  const bool fuse = g_fuse_linear && bt->eom == HOPS_EOM_LINEAR;
+  const size_t coefT_blk = sizeof(cx<R>) * (size_t)bt->coefT_stride *
+                           (2 * (size_t)op->n_pat + (size_t)op->n_mode + 1);
+  const size_t coef_blk = (size_t)bt->B * (size_t)bt->coef_stride;
+  struct StreamScratch {
+    void *p = nullptr;
+    hipStream_t s;
+    ~StreamScratch() {
+      if (p) (void)hipFreeAsync(p, s);
+    }
+  } scr;
+  scr.s = st;
+  if (fuse && n_steps > 0) {
+    const size_t bytes = 3 * coefT_blk + 3 * coef_blk * sizeof(double);
+    HIP_TRY(hipMallocAsync(&scr.p, bytes, st));
+    HIP_TRY(hipMemsetAsync(scr.p, 0, bytes, st));
+  }
+  char *coefT_s = reinterpret_cast<char *>(scr.p);
+  double *coef_s = reinterpret_cast<double *>(coefT_s + 3 * coefT_blk);
   size_t used = 0;
   if (int rc = prof_mark(st, 2, &used)) return rc;
   // one RK4 step of a (sub-)batch; slot: psi store slot written at stage 0
   auto one_step = [&](const hops_batch_t &sb, int step) -> int {
     const int itb = it0 + step * 2 * ns;
+    // sub-batch offsets into the (scratch) tables
+    const size_t offT = (size_t)((const char *)sb.coefT_d - (const char *)bt->coefT_d);
+    const size_t offC = (size_t)(sb.coef_d - bt->coef_d);
+    if (fuse) {
+      const cx<R> *yin0 = reinterpret_cast<const cx<R> *>(sb.precision == HOPS_PREC_MIXED ? sb.y0_d : sb.phi_d);
+      const int slot = (step > 0 && sb.psi_store_d) ? store0 + step - 1 : -1;
+      k_prologue_linear4<R><<<dim3(sb.B, 4), 256, shm, st>>>(*op, sb, itb, ns, yin0, slot, coefT_s + offT, coefT_blk,
+                                                              coef_s + offC, coef_blk);
+      HIP_TRY(hipGetLastError());
+      if (int rc = prof_mark(st, 0, &used)) return rc;
+    }
     for (int i = 0; i < 4; ++i) {
       const cx<R> *yin = i == 0 ? reinterpret_cast<const cx<R> *>(sb.precision == HOPS_PREC_MIXED ? sb.y0_d : sb.phi_d)
                                 : reinterpret_cast<const cx<R> *>(i == 2 ? sb.yb_d : sb.ya_d);
       cx<R> *yout = reinterpret_cast<cx<R> *>(i == 1 ? sb.yb_d : sb.ya_d);
       const int slot = (i == 0 && step > 0 && sb.psi_store_d) ? store0 + step - 1 : -1;
-      k_prologue<R><<<sb.B, 256, shm, st>>>(*op, sb, i, itb + toff[i], yin, slot);
-      HIP_TRY(hipGetLastError());
-      if (int rc = prof_mark(st, 0, &used)) return rc;
+      hops_batch_t sbi = sb;
+      if (fuse) {
+        if (i > 0) sbi.coefT_d = coefT_s + (size_t)(i - 1) * coefT_blk + offT;
+      } else {
+        k_prologue<R><<<sb.B, 256, shm, st>>>(*op, sb, i, itb + toff[i], yin, slot);
+        HIP_TRY(hipGetLastError());
+        if (int rc = prof_mark(st, 0, &used)) return rc;
+      }
 #define HOPS_SHV(W, P, NT)                                                                       \
   do {                                                                                           \
     if (lds_apt > 0) {                                                                           \
-      if (vl == 4) k_stage_lds<R, W, P, NT, 4><<<lds_grid, 512, lds_bytes, st>>>(*op, sb, i, yin, yout, lds_tiles, sh_q, lds_apt); \
-      else if (vl == 2) k_stage_lds<R, W, P, NT, 2><<<lds_grid, 512, lds_bytes, st>>>(*op, sb, i, yin, yout, lds_tiles, sh_q, lds_apt); \
-      else k_stage_lds<R, W, P, NT, 1><<<lds_grid, 512, lds_bytes, st>>>(*op, sb, i, yin, yout, lds_tiles, sh_q, lds_apt); \
-    } else if (vl == 4) k_stage_shared<R, W, P, NT, 4><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
-    else if (vl == 2) k_stage_shared<R, W, P, NT, 2><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
-    else k_stage_shared<R, W, P, NT, 1><<<sh_grid, 512, 0, st>>>(*op, sb, i, yin, yout, sh_tiles, sh_q); \
+      if (vl == 4) k_stage_lds<R, W, P, NT, 4><<<lds_grid, 512, lds_bytes, st>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else if (vl == 2) k_stage_lds<R, W, P, NT, 2><<<lds_grid, 512, lds_bytes, st>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else k_stage_lds<R, W, P, NT, 1><<<lds_grid, 512, lds_bytes, st>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+    } else if (vl == 4) k_stage_shared<R, W, P, NT, 4><<<sh_grid, 512, 0, st>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+    else if (vl == 2) k_stage_shared<R, W, P, NT, 2><<<sh_grid, 512, 0, st>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+    else k_stage_shared<R, W, P, NT, 1><<<sh_grid, 512, 0, st>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
   } while (0)
 #define HOPS_SH(W, P)                                                                            \
   do {                                                                                           \
@@ -1306,7 +1364,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       else if (sh_w == 8 && sh_p == 8) HOPS_SH(8, 8);
       else if (sh_w == 16 && sh_p == 4) HOPS_SH(16, 4);
       else if (sh_w == 16 && sh_p == 8) HOPS_SH(16, 8);
-      else k_stage<R><<<sb.n_tiles, 256, 0, st>>>(*op, sb, i, yin, yout);
+      else k_stage<R><<<sb.n_tiles, 256, 0, st>>>(*op, sbi, i, yin, yout);
 #undef HOPS_SH
 #undef HOPS_SHV
       HIP_TRY(hipGetLastError());
@@ -1707,6 +1765,10 @@ int hops_set_option(const char *name, int value) {
   if (std::string(name) == "nt_stream") {
     if (value < 0 || value > 4) return fail("nt_stream must be 0..4");
     g_nt_stream = value;
+    return 0;
+  }
+  if (std::string(name) == "fuse_prologue") {
+    g_fuse_linear = value != 0;
     return 0;
   }
   if (std::string(name) == "lds_stage") {

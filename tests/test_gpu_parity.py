@@ -306,6 +306,38 @@ def test_vector_lanes_bit_identical(eom):
             assert d == 0.0
 
 
+def test_fused_linear_prologue_bit_identical():
+    """The four LINEAR stage prologues of a step in one launch (the default)
+    give the same trajectories as one prologue launch per stage, bit for bit:
+    with the low-temperature correction and a second noise, in both
+    precisions, with the streaming (lanes 1, B < 8 groups) and stationary
+    kernels."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1, ltc=37.5 - 12.0j)
+    sysp.update(L_NOISE2=sysp["L_NOISE1"], ALPHA_NOISE2=O.bcf_exp, PARAM_NOISE2=sysp["PARAM_NOISE1"])
+    tlen, tau = 50.0, 0.5
+    B = 40
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    z2 = [_oracle_noise(sysp, 100 + s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[1] = 1
+    for prec in ("mixed", "fp64"):
+        for lanes in (1, 4):
+            _lib.set_option("fuse_prologue", 0)
+            try:
+                ref = engine_run(sysp, "LINEAR", zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, noise2_arrays=z2,
+                                 lanes=lanes)
+            finally:
+                _lib.set_option("fuse_prologue", 1)
+            got = engine_run(sysp, "LINEAR", zs, tlen, tau, 3, 30.0, 1.0, psi0, prec=prec, noise2_arrays=z2,
+                             lanes=lanes)
+            d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+            print(prec, "lanes", lanes, "fused vs per-stage prologue max diff", d)
+            assert d == 0.0
+            assert np.array_equal(got.phi, ref.phi)
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_interleaved_lanes_equal_plain_layout(lanes):
     """Interleaving T trajectories per row changes only the memory layout:
