@@ -149,9 +149,13 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
         return ab
 
     t0 = time.perf_counter()
+    if log:
+        log("%s: preparing %d trajectories (noise, first bases)" % (kind, B))
     ab = make(B, prec)
     torch.cuda.synchronize()
     t_init = time.perf_counter() - t0
+    if log:
+        log("%s: prepared in %.1f s" % (kind, t_init))
     _lib.profile_enable(True)
     chunks = []
     t0 = time.perf_counter()
@@ -177,6 +181,8 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
     stage_bytes = timers["rows_stepped"] * (8 * es + 32)
     achieved = stage_bytes / (ms[1] * 1e-3) / 1e9 if ms[1] > 0 else 0.0
     # plain per-trajectory FP64 GPU path over the first single_steps steps
+    if log:
+        log("%s: plain per-trajectory FP64 path, %d steps" % (kind, single_steps))
     tr = HopsTrajectory(sysp, eom, dict(noise, SEED=0), hierarchy_param=hier)
     tr.make_adaptive(1e-3, 1e-3)
     tr.initialize(psi)
@@ -205,7 +211,10 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
                             "frac_of_8TBs": achieved / HBM_PEAK_GBS}}
     if ref_traj > 0:
         ref = make(ref_traj, "fp64")
-        ref.propagate(float(steps), 1.0)
+        for t_done in range(0, steps, chunk):
+            ref.propagate(float(min(chunk, steps - t_done)), 1.0)
+            if log:
+                log("%s: FP64 reference, t=%d fs" % (kind, min(steps, t_done + chunk)))
         err = 0.0
         for b in range(ref_traj):
             pa = np.abs(ab.results(b)[0]) ** 2

@@ -662,7 +662,14 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
   double2 *phi = reinterpret_cast<double2 *>(bt.phi_d);
   if (stage < 3) {
     cx<R> base;
-    if (bt.precision == HOPS_PREC_MIXED) {
+    cx<R> a_in = mk<R>((R)0, (R)0);
+    if (stage > 0) a_in = acc[gi];
+    if (stage == 1) {
+      // the stage base sigma * y0 from this row's stage-1 input and k0:
+      // y1 - c0 k0, one rounding of y1 away from it, without reading y0
+      const R c0 = (R)(0.5 * h);
+      base = mk<R>(yr.x - c0 * a_in.x, yr.y - c0 * a_in.y);
+    } else if (bt.precision == HOPS_PREC_MIXED) {
       base = cscale(reinterpret_cast<const cx<R> *>(bt.y0_d)[gi], (R)sigma);
     } else {
       const double2 p = phi[gi];
@@ -673,7 +680,7 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
     if (stage == 0) {
       acc[gi] = d;
     } else {
-      cx<R> a = acc[gi];
+      cx<R> a = a_in;
       a.x += (R)2 * d.x;
       a.y += (R)2 * d.y;
       acc[gi] = a;
@@ -846,7 +853,9 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
     base[v] = a_in[v] = zero;
     ph[v] = make_double2(0.0, 0.0);
   }
-  if (stage < 3) {
+  if (stage == 1) {
+    // the stage base comes from this row's input and k0 (below)
+  } else if (stage < 3) {
     if (bt.precision == HOPS_PREC_MIXED) {
       ldv<NT, V>(el(reinterpret_cast<const cx<R> *>(bt.y0_d), gi), base);
     } else {
@@ -889,8 +898,15 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       if (stage == 0) d[v] = cscale(d[v], (R)sigma[v]);
-      if (bt.precision == HOPS_PREC_MIXED) base[v] = cscale(base[v], (R)sigma[v]);
-      else base[v] = mk<R>((R)(sigma[v] * ph[v].x), (R)(sigma[v] * ph[v].y));
+      if (stage == 1) {
+        // sigma * y0 = y1 - c0 k0, one rounding of y1 away, as in stage_row
+        const R c0 = (R)(0.5 * h);
+        base[v] = mk<R>(yr[v].x - c0 * a_in[v].x, yr[v].y - c0 * a_in[v].y);
+      } else if (bt.precision == HOPS_PREC_MIXED) {
+        base[v] = cscale(base[v], (R)sigma[v]);
+      } else {
+        base[v] = mk<R>((R)(sigma[v] * ph[v].x), (R)(sigma[v] * ph[v].y));
+      }
       o[v] = mk<R>(base[v].x + c * d[v].x, base[v].y + c * d[v].y);
       if (stage == 0) {
         a_in[v] = d[v];
