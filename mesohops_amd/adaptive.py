@@ -83,7 +83,7 @@ class BasisSelector:
     auxiliary lists (basis/hops_basis.py:117-182, :294-998).  It touches no
     device memory; the trajectory engine supplies the derivative."""
 
-    def __init__(self, info, delta_a, delta_s, maxhier, f_discard, markov=None):
+    def __init__(self, info, delta_a, delta_s, maxhier, f_discard, markov=None, filters=None):
         for L in info.l_ops:
             if np.any(L.row != L.col):
                 raise UnsupportedRequest("adaptive basis with off-diagonal L-operators", "make_adaptive")
@@ -92,6 +92,9 @@ class BasisSelector:
         self.maxhier = int(maxhier)
         self.f_discard = float(f_discard)
         self.markov = None if markov is None else np.array(markov, bool)
+        # Triangular / LongEdge static filters: (name, mask over the absolute
+        # modes, kmax2), in the order given
+        self.filters = [(n, np.array(p[0], bool), int(p[1])) for n, p in (filters or [])]
 
     def _bind(self, tb):
         self.tb = tb
@@ -133,6 +136,44 @@ class BasisSelector:
                 F[:, j] = False
         return F
 
+    def _filter_up(self):
+        """Allowed flux up along (mode, aux), hops_fluxfilters.py:313-501: the
+        Markovian mask; a Triangular filter closes the filtered modes of an
+        auxiliary whose depth in them has reached kmax2; a LongEdge filter,
+        from total depth kmax2 on, closes every mode of an auxiliary with depth
+        in a filtered mode except the own mode of an edge (single-mode)
+        auxiliary, and the filtered modes of any other auxiliary.  Filters of
+        one kind act on one mask in order, and the kinds multiply."""
+        F = self._markov_up()
+        if not self.filters:
+            return F
+        tb = self.tb
+        nm, nh = tb.n_mode, tb.n_hier
+        K = np.zeros((nh, self.info.n_hmodes), dtype=np.int64)  # aux depth by absolute mode
+        for j, a in enumerate(tb.aux_list):
+            for (m, v) in a:
+                K[j, m] = v
+        depth = K.sum(axis=1)
+        n_nonzero = (K > 0).sum(axis=1)
+        for kind in ("Triangular", "LongEdge"):
+            G = np.ones((nm, nh), bool)
+            for name, mask, kmax2 in self.filters:
+                if name != kind:
+                    continue
+                rel = mask[tb.modes]                       # filtered modes of the basis
+                fdep = K[:, mask].sum(axis=1)              # depth in the filtered modes
+                if kind == "Triangular":
+                    G[np.ix_(rel, fdep >= kmax2)] = False
+                    continue
+                deep = depth >= kmax2
+                with_f = deep & (fdep > 0)
+                G[:, with_f] = False
+                for j in np.nonzero(with_f & (n_nonzero == 1))[0]:
+                    G[self.rel_mode[int(tb.aux_list[j][0][0])], j] = True
+                G[np.ix_(rel, deep & (fdep == 0))] = False
+            F &= G
+        return F
+
     def _list_sc(self):
         info, st = self.info, set(int(s) for s in self.tb.state_list)
         out = set()
@@ -165,7 +206,7 @@ class BasisSelector:
                 err = err + np.sum(np.abs(Hsc @ C2 / hbar) ** 2, axis=0)
             KW = np.abs(w).reshape(-1, 1) * (1 + K2)
             E_up = np.abs(KW) ** 2 * ((np.abs(M2) ** 2) @ absC2) / hbar ** 2
-            F_up = self._markov_up()
+            F_up = self._filter_up()
             depth = np.array([_depth(a) for a in tb.aux_list])
             F_up[:, depth == self.maxhier] = False
             err = err + np.sum(F_up * E_up, axis=0)
@@ -206,7 +247,7 @@ class BasisSelector:
                 m = int(m)
                 if m not in d or _step_aux(a, m, -1) in self.aux_index:
                     F_dn[lm, j] = False
-        F_up &= self._markov_up()
+        F_up &= self._filter_up()
         Eu = (E_up * F_up)[:, idx_st]
         Ed = (E_dn * F_dn)[:, idx_st]
         srt = np.sort(np.append(Eu[Eu != 0], Ed[Ed != 0]))
@@ -299,13 +340,17 @@ class AdaptiveTrajectoryEngine:
         self.early_steps, self.inchworm_cap = int(early_steps), int(inchworm_cap)
         self.maxhier = setup.hier["MAXHIER"]
         self.markov = None
+        filters = []
         for name, prm in setup.hier["STATIC_FILTERS"]:
-            if name != "Markovian":
+            if name == "Markovian":
+                mk = np.array(prm, bool)
+                self.markov = mk if self.markov is None else (self.markov | mk)
+            elif name in ("Triangular", "LongEdge"):
+                filters.append((name, [list(prm[0]), int(prm[1])]))
+            else:
                 raise UnsupportedRequest("static filter %s with an adaptive hierarchy" % name, "make_adaptive")
-            mk = np.array(prm, bool)
-            self.markov = mk if self.markov is None else (self.markov | mk)
         self.selector = BasisSelector(self.info, self.delta_a, self.delta_s, self.maxhier, self.f_discard,
-                                      self.markov)
+                                      self.markov, filters)
         self.early_counter = 0
         self.raw_pending = False  # the next step takes Phi as an operator left it
         self.sp = _stream_ptr(self.device)

@@ -743,8 +743,11 @@ class AdaptiveState:
     trajectory, built on Basis (which is rebuilt for every new basis, the
     invariant of eom/eom_hops_ksuper.py:387-572)."""
 
-    def __init__(self, sysd, eom_type, maxhier, delta_a, delta_s, f_discard, markov=None):
+    def __init__(self, sysd, eom_type, maxhier, delta_a, delta_s, f_discard, markov=None, filters=None):
         self.markov = None if markov is None else np.array(markov, bool)
+        # Triangular / LongEdge static filters: (name, bool mask over the
+        # absolute modes, kmax2)
+        self.filters = [(n, np.array(p[0], bool), int(p[1])) for n, p in (filters or [])]
         self.sysd = sysd
         self.eom_type = eom_type
         self.maxhier = maxhier
@@ -797,6 +800,38 @@ class AdaptiveState:
                     F[:, j] = False
         return F
 
+    def up_filter(self, b):
+        """Allowed flux up along (mode, aux): the Markovian mask, times the
+        Triangular mask (no step up along a filtered mode from an auxiliary
+        whose depth in the filtered modes is already >= kmax2), times the
+        LongEdge mask (from depth >= kmax2: an auxiliary with depth in a
+        filtered mode steps up only along its own mode when it is an edge
+        term, one without steps up along no filtered mode); the filters of one
+        kind share one mask, applied in order (hops_fluxfilters.py:313-501)."""
+        F = self.markov_up(b)
+        nm, nh = len(b.modes), b.n_hier
+        modes = np.array(b.modes)
+        for kind in ("Triangular", "LongEdge"):
+            G = np.ones((nm, nh), bool)
+            for name, mask_abs, kmax2 in self.filters:
+                if name != kind:
+                    continue
+                mask = mask_abs[modes]
+                filt = set(int(m) for m in modes[mask])
+                for j, a in enumerate(b.aux_list):
+                    if kind == "Triangular":
+                        if sum(v for m, v in a if m in filt) >= kmax2:
+                            G[mask, j] = False
+                    elif sum(v for _, v in a) >= kmax2:
+                        if any(m in filt for m, _ in a):
+                            G[:, j] = False
+                            if len(a) == 1:
+                                G[b.rel_mode[a[0][0]], j] = True
+                        else:
+                            G[mask, j] = False
+            F = F & G
+        return F
+
     def list_sc(self, b):
         """States outside the basis reachable by L or H (hops_system.py:191-204)."""
         st = set(int(s) for s in b.state_list)
@@ -839,7 +874,7 @@ class AdaptiveState:
         w = np.abs(b.w)
         KW = w.reshape(nm, 1) * (1 + K2)
         E_up = np.abs(KW) ** 2 * ((np.abs(M2) ** 2) @ (np.abs(C2) ** 2)) / HBAR ** 2
-        F_up = self.markov_up(b)
+        F_up = self.up_filter(b)
         for j, a in enumerate(b.aux_list):
             if _aux_depth(a) == self.maxhier:
                 F_up[:, j] = False
@@ -870,7 +905,7 @@ class AdaptiveState:
                 m = int(m)
                 if m not in d or aux_step(a, m, -1) in b.aux_index:
                     F_down[lm, j] = False
-        F_up &= self.markov_up(b)
+        F_up &= self.up_filter(b)
         E_up = (E_up * F_up)[:, stable_idx]
         E_down = (E_down * F_down)[:, stable_idx]
         srt = np.sort(np.append(E_up[E_up != 0], E_down[E_down != 0]))
@@ -998,10 +1033,12 @@ class AdaptiveTrajectory(Trajectory):
 
     def __init__(self, *args, delta_a=1e-4, delta_s=1e-4, update_step=1, f_discard=0.01,
                  early_steps=5, inchworm_cap=5, markov=None, early_integrator="INCH_WORM", static_basis=None,
-                 **kw):
+                 filters=None, **kw):
         super().__init__(*args, **kw)
         self.ad = AdaptiveState(self.sysd, self.eom_type, self.maxhier, delta_a, delta_s, f_discard,
-                                markov=markov)
+                                markov=markov, filters=filters)
+        if filters and not self.ad.adaptive_h:
+            raise NotImplementedError("Triangular / LongEdge filters need an adaptive hierarchy here")
         self.update_step = update_step
         self.early_steps = early_steps
         self.inchworm_cap = inchworm_cap

@@ -27,6 +27,13 @@ CASES = {
     "chain_hier_only": (systems.chain(6), None, 1e-3, 0.0, 4, "NORMALIZED NONLINEAR"),
     "chain_states_only": (systems.chain(8), None, 0.0, 2e-3, 2, "NONLINEAR"),
     "markov_chain": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR"),
+    # Triangular / LongEdge static filters on the flux up (hops_fluxfilters.py:372-501)
+    "dod_triangular": (systems.dimer_of_dimers(), None, 1e-3, 1e-3, 3, "NORMALIZED NONLINEAR",
+                       [("Triangular", [[True, False] * 4, 1])]),
+    "chain_longedge": (systems.chain(6), None, 1e-3, 0.0, 4, "NORMALIZED NONLINEAR",
+                       [("LongEdge", [[False, True] * 6, 1])]),
+    "markov_triangular": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR",
+                          [("Triangular", [[True, False] * 7, 2]), ("LongEdge", [[True, False] * 7, 3])]),
 }
 
 
@@ -34,16 +41,17 @@ CASES = {
 def test_selector_matches_oracle_define_basis(name):
     from mesohops_amd.adaptive import BasisSelector
     from mesohops_amd.basis import BasisTables, SystemInfo
-    sysp, markov, da, ds, K, eom = CASES[name]
+    sysp, markov, da, ds, K, eom, *rest = CASES[name]
+    filters = rest[0] if rest else None
     nst = sysp["HAMILTONIAN"].shape[0]
     noise = {"SEED": 7, "MODEL": "FFT_FILTER", "TLEN": 200.0, "TAU": 1.0}
     tr = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
-                              delta_a=da, delta_s=ds, markov=markov)
+                              delta_a=da, delta_s=ds, markov=markov, filters=filters)
     psi0 = np.zeros(nst, complex)
     psi0[nst // 2] = 1.0
     tr.initialize(psi0)
     info = SystemInfo(sysp)
-    sel = BasisSelector(info, da, ds, K, 0.01, markov)
+    sel = BasisSelector(info, da, ds, K, 0.01, markov, filters)
     n_checked = 0
     sizes = set()
     for step in range(12):

@@ -103,6 +103,13 @@ ADAPTIVE_CASES = {
     # 2D lattice with a low-temperature correction and noise modes beyond the
     # hierarchy modes (the Matsubara modes treated by the correction)
     "lattice4_ltc": (systems.lattice(4, 4), None, 2e-3, 2e-3, 3, "NORMALIZED NONLINEAR", 40.0, 1.0),
+    # Triangular / LongEdge static filters on the flux up (hops_fluxfilters.py:372-501)
+    "dod_triangular": (systems.dimer_of_dimers(), None, 1e-3, 1e-3, 3, "NORMALIZED NONLINEAR", 60.0, 2.0,
+                       [["Triangular", [[True, False] * 4, 1]]]),
+    "chain_longedge": (systems.chain(6), None, 1e-3, 0.0, 4, "NORMALIZED NONLINEAR", 60.0, 2.0,
+                       [["LongEdge", [[False, True] * 6, 1]]]),
+    "markov_triangular_longedge": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR", 60.0, 2.0,
+                                   [["Triangular", [[True, False] * 7, 2]], ["LongEdge", [[True, False] * 7, 3]]]),
 }
 
 
@@ -110,7 +117,8 @@ ADAPTIVE_CASES = {
 def test_adaptive_trajectory_matches_oracle(name):
     _need_gpu()
     from mesohops_amd import HopsTrajectory
-    sysp, markov, da, ds, K, eom, t_max, dt = ADAPTIVE_CASES[name]
+    sysp, markov, da, ds, K, eom, t_max, dt, *rest = ADAPTIVE_CASES[name]
+    filters = rest[0] if rest else []
     update = 3 if name.endswith("update3") else 1
     nst = sysp["HAMILTONIAN"].shape[0]
     tlen, tau = t_max + 20.0, 0.5
@@ -119,12 +127,15 @@ def test_adaptive_trajectory_matches_oracle(name):
     psi0 = np.zeros(nst, complex)
     psi0[nst // 2] = 1.0
     ref = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
-                               delta_a=da, delta_s=ds, update_step=update, markov=markov)
+                               delta_a=da, delta_s=ds, update_step=update, markov=markov,
+                               filters=[(f[0], f[1]) for f in filters])
     ref.initialize(psi0)
     ref.propagate(t_max, dt)
     hp = {"MAXHIER": K}
     if markov is not None:
         hp["STATIC_FILTERS"] = [["Markovian", markov]]
+    if filters:
+        hp["STATIC_FILTERS"] = hp.get("STATIC_FILTERS", []) + [[f[0], [list(f[1][0]), f[1][1]]] for f in filters]
     tr = HopsTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, dict(noise), hierarchy_param=hp)
     tr.make_adaptive(da, ds, update_step=update)
     tr.initialize(psi0)

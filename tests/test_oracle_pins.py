@@ -245,6 +245,52 @@ def test_adaptive_delta_to_zero_equals_full(da, ds):
     assert np.max(np.abs(np.array(tr.psi_traj) - np.array(ref.psi_traj))) < 1e-12
 
 
+@pytest.mark.parametrize("kind", ["Triangular", "LongEdge", "both"])
+def test_adaptive_static_filters_reach_the_filtered_hierarchy(kind):
+    """With vanishing thresholds the adaptive hierarchy grows into exactly the
+    statically filtered hierarchy: every legal auxiliary and no other, where
+    legality is the reference's membership rule, applied here by brute force
+    to the triangular hierarchy (basis/hierarchy_functions.py:7-95: Triangular
+    keeps depth <= kmax2 in the filtered modes; LongEdge keeps total depth <=
+    k, no depth in the filtered modes, or a single mode)."""
+    import itertools
+    from collections import Counter
+    sysp = systems.dimer_of_dimers()
+    noise = dict(SEED=0, MODEL="FFT_FILTER", TLEN=200.0, TAU=1.0)
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    mask = [True, False] * 4
+    mask2 = [False, True] * 4
+    filters = []
+    if kind in ("Triangular", "both"):
+        filters.append(("Triangular", [mask, 1]))
+    if kind in ("LongEdge", "both"):
+        filters.append(("LongEdge", [mask2 if kind == "both" else mask, 1]))
+    maxhier = 3
+    tr = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": maxhier}, delta_a=1e-14, delta_s=0,
+                              filters=filters)
+    psi0 = np.zeros(4, complex)
+    psi0[1] = 1
+    tr.initialize(psi0)
+    tr.propagate(40.0, 2.0)
+    legal = set()
+    for depth in range(maxhier + 1):
+        for raw in itertools.combinations_with_replacement(range(8), depth):
+            a = tuple(sorted(Counter(raw).items()))
+            ok = True
+            for name, (mk, k2) in filters:
+                fd = sum(v for m, v in a if mk[m])
+                if name == "Triangular":
+                    ok &= fd <= k2
+                else:
+                    ok &= sum(v for _, v in a) <= k2 or fd == 0 or len(a) <= 1
+            if ok:
+                legal.add(a)
+    got = set(tr.basis.aux_list)
+    print(kind, "legal", len(legal), "adaptive", len(got))
+    assert got == legal
+    assert len(legal) < 165  # the filter removed auxiliaries
+
+
 def _embed(b_small, b_full, phi):
     out = np.zeros(b_full.n_hier * b_full.n_state, complex)
     pos = {int(s): i for i, s in enumerate(b_full.state_list)}
