@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes (scaled x2.00 and x1.00 by a 512 MiB device copy counted
 # in the same runs), by stage kernel and interleave
 PROFILED_TRAFFIC = {
-    ("k_stage_lds<float, 4, 4, 2, 2>", 8): (1.305e9, "profiles/k_stage_lds_float_v2_nt2_chain10_d4_b256_lanes8.md"),
+    ("k_stage_lds<float, 4, 4, 2, 2>", 8): (1.250e9, "profiles/k_stage_lds_float_v2_nt2_chain10_d4_b256_lanes8.md"),
     ("k_stage_shared<float, 4, 4, 2, 1>", 4): (1.436e9, "profiles/k_stage_shared_float_nt2_chain10_d4_b256_lanes4.md"),
 }
 

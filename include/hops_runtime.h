@@ -57,6 +57,12 @@ typedef struct hops_rt_traj_param {
   double delta_a, delta_s, f_discard, dt;
   const uint8_t *markov;   /* [n_hmode] Markovian filter flags, or NULL */
   const int32_t *aux0_ptr, *aux0_mode, *aux0_val;
+  /* Triangular (kind 0) and LongEdge (kind 1) static filters on the flux up
+     of the adaptive hierarchy (basis/hops_fluxfilters.py:372-501), in order:
+     kind, kmax2 and a [n_hmode] mask of filtered modes each */
+  int32_t n_filters, pad2;
+  const int32_t *filter_kind, *filter_kmax;
+  const uint8_t *filter_mask; /* [n_filters][n_hmode] */
 } hops_rt_traj_param_t;
 
 void *hops_rt_system_create(const hops_rt_system_desc_t *desc);

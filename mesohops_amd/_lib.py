@@ -16,7 +16,7 @@ HOPS_PREC_MIXED = 1
 HOPS_EOM_LINEAR = 0
 HOPS_EOM_NONLINEAR = 1
 HOPS_EOM_NORMALIZED_NONLINEAR = 2
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _i32 = ctypes.c_int32
 _vp = ctypes.c_void_p
@@ -66,7 +66,8 @@ class RtTrajParam(ctypes.Structure):
                 ("it0", _i32), ("record", _i32), ("n_aux0", _i32), ("pad", _i32),
                 ("delta_a", ctypes.c_double), ("delta_s", ctypes.c_double), ("f_discard", ctypes.c_double),
                 ("dt", ctypes.c_double), ("markov", _vp), ("aux0_ptr", _vp), ("aux0_mode", _vp),
-                ("aux0_val", _vp)]
+                ("aux0_val", _vp), ("n_filters", _i32), ("pad2", _i32), ("filter_kind", _vp), ("filter_kmax", _vp),
+                ("filter_mask", _vp)]
 
 
 _lib = None

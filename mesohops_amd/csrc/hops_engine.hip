@@ -1756,7 +1756,7 @@ int hops_internal_set_error(const char *msg) {
   return -1;
 }
 
-int hops_version(void) { return 2; }
+int hops_version(void) { return 3; }
 
 int hops_profile_enable(int on) {
   g_prof.on = on != 0;

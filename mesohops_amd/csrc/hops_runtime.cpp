@@ -619,8 +619,29 @@ struct Params {
   int noise_step = 1, n_t = 0, it0 = 0, record = 1;
   double delta_a = 0, delta_s = 0, f_discard = 0, dt = 1;
   std::vector<uint8_t> markov;
+  struct UpFilter {
+    int kind, kmax;  // kind 0: Triangular, 1: LongEdge
+    std::vector<uint8_t> mask;  // [n_hmode] filtered modes
+  };
+  std::vector<UpFilter> filters;
   std::vector<Aux> aux0;
 };
+
+// copies the Markovian and Triangular / LongEdge filter parameters
+int copy_filters(const hops_rt_traj_param_t *prm, int n_hm, Params &P) {
+  if (prm->markov) P.markov.assign(prm->markov, prm->markov + n_hm);
+  P.filters.clear();
+  if (prm->n_filters < 0) return rt_fail("n_filters < 0");
+  if (prm->n_filters > 0 && (!prm->filter_kind || !prm->filter_kmax || !prm->filter_mask))
+    return rt_fail("filter arrays missing");
+  for (int i = 0; i < prm->n_filters; ++i) {
+    if (prm->filter_kind[i] != 0 && prm->filter_kind[i] != 1) return rt_fail("unknown filter kind %d", prm->filter_kind[i]);
+    Params::UpFilter f{prm->filter_kind[i], prm->filter_kmax[i],
+                       std::vector<uint8_t>(prm->filter_mask + (size_t)i * n_hm, prm->filter_mask + (size_t)(i + 1) * n_hm)};
+    P.filters.push_back(std::move(f));
+  }
+  return 0;
+}
 
 double error_threshold(const std::vector<double> &sorted, double max_err, double offset = 0.0) {
   // basis/basis_functions.py:8-41
@@ -841,14 +862,51 @@ std::vector<int> list_sc(const System &S, const Basis &B) {
 void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
   // hops_fluxfilters.py:354-397; F[k * nm + lm]
   F.assign((size_t)B.nh * B.nm, 1);
-  if (P.markov.empty()) return;
   const int nm = B.nm;
-  for (int lm = 0; lm < nm; ++lm) {
-    if (!P.markov[B.modes[lm]]) continue;
-    for (int k = 1; k < B.nh; ++k) F[(size_t)k * nm + lm] = 0;
-    const int j = B.index.find(key_step(*B.sys, AKey(), B.modes[lm], 0, 1));
-    if (j >= 0)
-      for (int m2 = 0; m2 < nm; ++m2) F[(size_t)j * nm + m2] = 0;
+  if (!P.markov.empty())
+    for (int lm = 0; lm < nm; ++lm) {
+      if (!P.markov[B.modes[lm]]) continue;
+      for (int k = 1; k < B.nh; ++k) F[(size_t)k * nm + lm] = 0;
+      const int j = B.index.find(key_step(*B.sys, AKey(), B.modes[lm], 0, 1));
+      if (j >= 0)
+        for (int m2 = 0; m2 < nm; ++m2) F[(size_t)j * nm + m2] = 0;
+    }
+  if (P.filters.empty()) return;
+  // Triangular: no flux up along a filtered mode from an auxiliary whose
+  // depth in the filtered modes has reached kmax2.  LongEdge: from total
+  // depth kmax2 on, an auxiliary with depth in a filtered mode steps up only
+  // along its own mode if it is an edge (single-mode) term, and one without
+  // steps up along no filtered mode.  The filters of one kind share one mask
+  // in order; the kinds multiply (hops_fluxfilters.py:372-501).
+  std::vector<uint8_t> G;
+  for (int kind = 0; kind < 2; ++kind) {
+    bool any = false;
+    for (const auto &f : P.filters) any |= f.kind == kind;
+    if (!any) continue;
+    G.assign((size_t)B.nh * nm, 1);
+    for (const auto &f : P.filters) {
+      if (f.kind != kind) continue;
+      for (int k = 0; k < B.nh; ++k) {
+        int fdep = 0;
+        for (int q = B.aux_ptr[k]; q < B.aux_ptr[k + 1]; ++q)
+          if (f.mask[B.aux_mode_t[q]]) fdep += B.aux_val_t[q];
+        uint8_t *g = &G[(size_t)k * nm];
+        if (kind == 0) {
+          if (fdep >= f.kmax)
+            for (int lm = 0; lm < nm; ++lm)
+              if (f.mask[B.modes[lm]]) g[lm] = 0;
+        } else if (B.depth[k] >= f.kmax) {
+          if (fdep > 0) {
+            for (int lm = 0; lm < nm; ++lm) g[lm] = 0;
+            if (B.aux_ptr[k + 1] - B.aux_ptr[k] == 1) g[B.rel_mode[B.aux_mode_t[B.aux_ptr[k]]]] = 1;
+          } else {
+            for (int lm = 0; lm < nm; ++lm)
+              if (f.mask[B.modes[lm]]) g[lm] = 0;
+          }
+        }
+      }
+    }
+    for (size_t i = 0; i < F.size(); ++i) F[i] &= G[i];
   }
 }
 
@@ -2374,7 +2432,7 @@ int hops_rt_batch_add(void *p, void *sys, const hops_rt_traj_param_t *prm, const
   if (P.noise_step <= 0 || P.n_t <= 0 || P.it0 < 0 || P.it0 >= P.n_t) return rt_fail("bad noise grid");
   if ((P.delta_a > 0 || P.delta_s > 0) && !S.diag_L)
     return rt_fail("adaptive basis with off-diagonal L-operators is not supported");
-  if (prm->markov) P.markov.assign(prm->markov, prm->markov + S.n_hm);
+  RT_TRY(copy_filters(prm, S.n_hm, P));
   if (P.delta_a <= 0) {
     if (prm->n_aux0 <= 0 || !prm->aux0_ptr) return rt_fail("non-adaptive hierarchy needs aux0");
     for (int i = 0; i < prm->n_aux0; ++i) {
@@ -2643,7 +2701,7 @@ int hops_rt_debug_select(void *p, const hops_rt_traj_param_t *prm, const double 
   d->P.delta_a = prm->delta_a;
   d->P.delta_s = prm->delta_s;
   d->P.f_discard = prm->f_discard;
-  if (prm->markov) d->P.markov.assign(prm->markov, prm->markov + d->B.sys->n_hm);
+  if (int rc = copy_filters(prm, d->B.sys->n_hm, d->P)) return rc;
   (void)dt;
   const size_t N = (size_t)d->B.n * d->B.nh;
   d->pn.resize(N);

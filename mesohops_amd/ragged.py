@@ -88,7 +88,7 @@ def _aux_from_csr(ptr, mode, val):
 
 def traj_param(eom_kind, maxhier, delta_a, delta_s, update_step, f_discard, dt, noise_step, n_t, it0,
                early_integrator="INCH_WORM", early_steps=5, inchworm_cap=5, markov=None, aux0=None,
-               record=True):
+               record=True, filters=None):
     """hops_rt_traj_param_t and the arrays it points to (kept alive on the
     returned object)."""
     p = RtTrajParam()
@@ -105,6 +105,14 @@ def traj_param(eom_kind, maxhier, delta_a, delta_s, update_step, f_discard, dt, 
         mk = np.ascontiguousarray(np.asarray(markov, dtype=np.uint8))
         keep.append(mk)
         p.markov = _ptr(mk)
+    if filters:
+        kinds = {"Triangular": 0, "LongEdge": 1}
+        fk = np.ascontiguousarray([kinds[name] for name, _ in filters], dtype=np.int32)
+        fm = np.ascontiguousarray([int(prm[1]) for _, prm in filters], dtype=np.int32)
+        fmask = np.ascontiguousarray(np.array([np.asarray(prm[0], bool) for _, prm in filters], dtype=np.uint8))
+        keep += [fk, fm, fmask]
+        p.n_filters = len(filters)
+        p.filter_kind, p.filter_kmax, p.filter_mask = _ptr(fk), _ptr(fm), _ptr(fmask)
     if aux0 is not None:
         a_ptr, a_mode, a_val = _aux_csr(aux0)
         keep += [a_ptr, a_mode, a_val]
@@ -361,12 +369,16 @@ class AdaptiveBatch:
             self.z2 = device_noise(self.noise2, list(range(nl)), self.device, sp)
         torch.cuda.synchronize(self.device)
         markov = None
+        filters = []
         for name, prm in self.setup.hier["STATIC_FILTERS"]:
-            if self.delta_a > 0 and name != "Markovian":
-                raise UnsupportedRequest("static filter %s with an adaptive hierarchy" % name, "make_adaptive")
             if name == "Markovian":
                 mk = np.array(prm, bool)
                 markov = mk if markov is None else markov | mk
+            elif name in ("Triangular", "LongEdge"):
+                filters.append((name, [list(prm[0]), int(prm[1])]))
+            elif self.delta_a > 0:
+                raise UnsupportedRequest("static filter %s with an adaptive hierarchy" % name, "make_adaptive")
+        self._filters = filters
         aux0 = None
         if self.delta_a <= 0:
             aux0 = triangular_aux_list(info.n_hmodes, self.setup.hier["MAXHIER"])
@@ -390,7 +402,7 @@ class AdaptiveBatch:
             prm = traj_param(self.setup.eom_kind, self.setup.hier["MAXHIER"], self.delta_a, self.delta_s,
                              self.update_step, self.f_discard, tau, ns, nz1.n_t, nz1.time_index(0.0),
                              integ["EARLY_ADAPTIVE_INTEGRATOR"], integ["EARLY_INTEGRATOR_STEPS"],
-                             integ["INCHWORM_CAP"], self._markov, self._aux0, self.record)
+                             integ["INCHWORM_CAP"], self._markov, self._aux0, self.record, filters=self._filters)
             z1 = self.z1[b]
             z2 = None if self.z2 is None else self.z2[b]
             self.slots.append(self.rt.add(self.system, prm, self._psi0, z1, z2))

@@ -25,10 +25,11 @@ def _noise(sysp, seed, tlen, tau):
     return O.Noise({"SEED": seed, "MODEL": "FFT_FILTER", "TLEN": tlen, "TAU": tau}, O.derive_system(sysp), 1).noise
 
 
-def _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0, update=1, markov=None):
+def _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0, update=1, markov=None, filters=None):
     noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
     tr = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
-                              delta_a=da, delta_s=ds, update_step=update, markov=markov)
+                              delta_a=da, delta_s=ds, update_step=update, markov=markov,
+                              filters=None if filters is None else [(f[0], f[1]) for f in filters])
     tr.initialize(psi0)
     tr.propagate(t_max, dt)
     return tr
@@ -39,6 +40,9 @@ CASES = {
     "chain8_states": (systems.chain(8), "NONLINEAR", 2, 0.0, 2e-3, 40.0, 2.0, 1),
     "chain10_update2": (systems.chain(10), "NORMALIZED NONLINEAR", 4, 2e-3, 2e-3, 40.0, 1.0, 2),
     "lattice4_ltc": (systems.lattice(4, 4), "NORMALIZED NONLINEAR", 3, 2e-3, 2e-3, 40.0, 1.0, 1),
+    # Triangular and LongEdge static filters on the flux up (hops_fluxfilters.py:372-501)
+    "dod_triangular_longedge": (systems.dimer_of_dimers(), "NORMALIZED NONLINEAR", 3, 1e-3, 1e-3, 60.0, 2.0, 1,
+                                [["Triangular", [[True, False] * 4, 1]], ["LongEdge", [[False, True] * 4, 2]]]),
 }
 
 
@@ -46,7 +50,8 @@ CASES = {
 def test_adaptive_batch_matches_oracle(name):
     _need_gpu()
     from mesohops_amd.ragged import AdaptiveBatch
-    sysp, eom, K, da, ds, t_max, dt, update = CASES[name]
+    sysp, eom, K, da, ds, t_max, dt, update, *rest = CASES[name]
+    filters = rest[0] if rest else None
     nst = sysp["HAMILTONIAN"].shape[0]
     tlen, tau = t_max + 20.0, 0.5
     seeds = [3, 17, 29, 41]
@@ -54,14 +59,17 @@ def test_adaptive_batch_matches_oracle(name):
     psi0 = np.zeros(nst, complex)
     psi0[nst // 2] = 1.0
     nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
-    ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], None, {"MAXHIER": K}, noise_params=nps,
+    hp = {"MAXHIER": K}
+    if filters:
+        hp["STATIC_FILTERS"] = [[f[0], [list(f[1][0]), f[1][1]]] for f in filters]
+    ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], None, hp, noise_params=nps,
                        precision_mode="fp64", arena_bytes=ARENA)
     ab.make_adaptive(da, ds, update_step=update)
     ab.initialize(psi0)
     ab.propagate(t_max, dt)
     for b, z in enumerate(zs):
         print("case", name, "trajectory", b, flush=True)
-        ref = _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0, update)
+        ref = _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0, update, filters=filters)
         psi, states, nhier = ab.results(b)
         assert len(states) == len(ref.state_traj)
         for i, (s_got, s_ref) in enumerate(zip(states, ref.state_traj)):

@@ -82,6 +82,13 @@ CASES = {
     "markov_chain": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR"),
     "chain12": (systems.chain(12), None, 2e-3, 2e-3, 5, "NORMALIZED NONLINEAR"),
     "lattice4_ltc": (systems.lattice(4, 4), None, 2e-3, 2e-3, 3, "NORMALIZED NONLINEAR"),
+    # Triangular / LongEdge static filters on the flux up (hops_fluxfilters.py:372-501)
+    "dod_triangular": (systems.dimer_of_dimers(), None, 1e-3, 1e-3, 3, "NORMALIZED NONLINEAR",
+                       [("Triangular", [[True, False] * 4, 1])]),
+    "chain_longedge": (systems.chain(6), None, 1e-3, 0.0, 4, "NORMALIZED NONLINEAR",
+                       [("LongEdge", [[False, True] * 6, 1])]),
+    "markov_triangular_longedge": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR",
+                                   [("Triangular", [[True, False] * 7, 2]), ("LongEdge", [[True, False] * 7, 3])]),
 }
 
 
@@ -89,17 +96,18 @@ CASES = {
 def test_runtime_selection_matches_oracle(name):
     from mesohops_amd.basis import SystemInfo
     from mesohops_amd.ragged import DebugBasis, RtSystem, traj_param
-    sysp, markov, da, ds, K, eom = CASES[name]
+    sysp, markov, da, ds, K, eom, *rest = CASES[name]
+    filters = rest[0] if rest else None
     nst = sysp["HAMILTONIAN"].shape[0]
     noise = {"SEED": 9, "MODEL": "FFT_FILTER", "TLEN": 200.0, "TAU": 1.0}
     tr = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
-                              delta_a=da, delta_s=ds, markov=markov)
+                              delta_a=da, delta_s=ds, markov=markov, filters=filters)
     psi0 = np.zeros(nst, complex)
     psi0[nst // 2] = 1.0
     tr.initialize(psi0)
     info = SystemInfo(sysp)
     rs = RtSystem(info)
-    prm = traj_param(2, K, da, ds, 1, 0.01, 2.0, 1, 10, 0, markov=markov)
+    prm = traj_param(2, K, da, ds, 1, 0.01, 2.0, 1, 10, 0, markov=markov, filters=filters)
     dt = 2.0
     sizes = set()
     for step in range(14):
