@@ -385,7 +385,9 @@ def main():
     max_pop = float(np.max(pr))
     del ref
 
-    # plain per-trajectory FP64 GPU path (B = 1, no batching)
+    # plain per-trajectory FP64 GPU path (B = 1, no batching, every kernel
+    # launched eagerly)
+    _lib.set_option("graph_steps", 0)
     single = HopsBatch(sysp, EOM, dict(NOISE, SEED=0), None, HIER, seeds=[0], precision_mode="fp64",
                        device="cuda:0", store_psi=True)
     single.initialize(psi0())
@@ -398,6 +400,7 @@ def main():
     torch.cuda.synchronize()
     single_rate = ss / (time.perf_counter() - t1)
     del single
+    _lib.set_option("graph_steps", 16)
 
     if args.calib_copy:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)

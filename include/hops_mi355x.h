@@ -161,6 +161,12 @@ int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
    runs the LDS-staged form of the stationary kernel where its tiles fit.
    "fuse_prologue": 1 (default) computes the four stage prologues of a LINEAR
    step in one launch (stream-ordered scratch for the tables of stages 1-3).
+   "graph_steps": K (default 16) RK4 steps of hops_propagate captured in one
+   HIP graph and replayed for steps 1, 2, ... (0, or profiling enabled, or a
+   call with fewer than K + 1 steps: every kernel launched eagerly).
+   "resident_rows": bases of at most this many rows per trajectory (default
+   1024) run a whole hops_propagate call in one launch, one workgroup per
+   trajectory looping over the steps (0: never).
    No option changes any result. */
 int hops_set_option(const char *name, int value);
 

@@ -20,6 +20,12 @@
 // Normalisation (NORMALIZED NONLINEAR) is lazy: the scale sigma = 1/|psi| of
 // the state produced by the previous step is computed in FP64 by the stage-0
 // prologue and folded into every read of Phi during the next step.
+//
+// Floating-point contraction is off for the whole file: a multiply and an add
+// stay two roundings unless the code asks for fma() itself.  The launched
+// stage forms, the resident form and the graph-replayed steps inline the same
+// functions into different kernels; with contraction left to the compiler
+// their FP64 prologue scalars could differ in the last bit between kernels.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,6 +37,8 @@
 #include <algorithm>
 
 #include "../../include/hops_mi355x.h"
+
+#pragma clang fp contract(off)
 
 namespace {
 
@@ -579,7 +587,11 @@ __device__ __forceinline__ void prologue_body(const hops_op_t &op, const hops_ba
 template <typename R>
 __global__ __launch_bounds__(256) void k_prologue(const hops_op_t op, const hops_batch_t bt,
                                                   int stage, int tidx, const cx<R> *__restrict__ yin,
-                                                  int store_slot) {
+                                                  int store_slot, const int *__restrict__ step_ctr) {
+  if (step_ctr) {  // replayed graph: noise index and store slot advance by the counter
+    tidx += step_ctr[0];
+    if (store_slot >= 0) store_slot += step_ctr[1];
+  }
   prologue_body<R>(op, bt, blockIdx.x, stage, tidx, yin, store_slot);
 }
 
@@ -593,8 +605,12 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_prologue_linear4(const hops_op_t op, hops_batch_t bt, int tidx0, int ns,
                                                          const cx<R> *__restrict__ yin, int store_slot,
                                                          char *coefT_s, size_t coefT_blk, double *coef_s,
-                                                         size_t coef_blk) {
+                                                         size_t coef_blk, const int *__restrict__ step_ctr) {
   const int stage = blockIdx.y;
+  if (step_ctr) {
+    tidx0 += step_ctr[0];
+    if (store_slot >= 0) store_slot += step_ctr[1];
+  }
   const int off = stage == 0 ? 0 : (stage == 3 ? 2 * ns : ns);  // noise index map [0, 1, 1, 2]
   if (stage > 0) {
     bt.coefT_d = coefT_s + (size_t)(stage - 1) * coefT_blk;
@@ -716,6 +732,38 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
   if (r >= nr) return;
   const int k = (int)(r / n), s = (int)(r - (long)k * n);
   stage_row<R, 0>(op, bt, stage, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
+}
+
+// Resident form for small bases: one workgroup per trajectory runs every
+// RK4 step of the call, the four prologues and stages of a step separated by
+// workgroup barriers (a trajectory's rows only read rows of the same
+// trajectory).  Each stage is the same stage_row per (row, lane) as k_stage,
+// so results are bit-identical to the launched forms; a call costs one
+// launch instead of five or eight per step.
+template <typename R>
+__global__ __launch_bounds__(256) void k_steps_resident(const hops_op_t op, const hops_batch_t bt, int it0,
+                                                        int n_steps, int store0, const cx<R> *__restrict__ y_first,
+                                                        cx<R> *__restrict__ ya, cx<R> *__restrict__ yb) {
+  const int b = blockIdx.x;
+  if (b >= bt.B) return;
+  const int n = op.n_state, ns = bt.noise_step;
+  const long nr = (long)n * op.n_hier;
+  const int toff[4] = {0, ns, ns, 2 * ns};
+  for (int step = 0; step < n_steps; ++step) {
+    const int itb = it0 + step * 2 * ns;
+    for (int i = 0; i < 4; ++i) {
+      const cx<R> *yin = i == 0 ? y_first : (i == 2 ? yb : ya);
+      cx<R> *yout = i == 1 ? yb : ya;
+      const int slot = (i == 0 && step > 0 && bt.psi_store_d) ? store0 + step - 1 : -1;
+      prologue_body<R>(op, bt, b, i, itb + toff[i], yin, slot);
+      __syncthreads();
+      for (long r = threadIdx.x; r < nr; r += blockDim.x) {
+        const int k = (int)(r / n), s = (int)(r - (long)k * n);
+        stage_row<R, 0>(op, bt, i, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // Element i of an array addressed as a uniform base pointer plus a 32-bit
@@ -995,8 +1043,15 @@ struct LdsStage {
   cx<R> c[V];   // one staged coefficient vector of the group (if cidx >= 0)
   int cidx;     // its LDS offset, or -1
 };
+// The complex64 forms with ELL and pattern widths 4 and up to 2 lanes per
+// thread fit 128 VGPRs without spilling: they ask for 4 waves per SIMD (two
+// workgroups per CU); the others take what their registers allow.
+template <typename R, int WR, int PW, int V>
+constexpr int lds_min_waves() {
+  return (sizeof(R) == 4 && WR == 4 && PW == 4 && V <= 2) ? 4 : 1;
+}
 template <typename R, int WR, int PW, int NT, int V>
-__global__ __launch_bounds__(512) void k_stage_lds(const hops_op_t op, const hops_batch_t bt, int stage,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(lds_min_waves<R, WR, PW, V>()))) void k_stage_lds(const hops_op_t op, const hops_batch_t bt, int stage,
                                                    const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
                                                    int n_tiles, int Q, int aux_per_tile) {
   extern __shared__ double lds_raw[];
@@ -1206,6 +1261,66 @@ int g_groups_q = 0;        // trajectory-group split of k_stage_shared (0: autom
 int g_vec_lanes = 1;       // trajectory lanes per thread of k_stage_shared (1, 2 or 4)
 int g_lds_stage = 0;       // 1: LDS-staged form k_stage_lds where it fits
 int g_fuse_linear = 1;     // 1: one prologue launch per LINEAR step (k_prologue_linear4)
+int g_graph_steps = 16;    // RK4 steps per captured HIP graph (0: every kernel launched eagerly)
+int g_resident_rows = 1024;  // rows per trajectory up to which k_steps_resident runs a whole call
+
+// counter {noise index offset, store slot offset} read by the prologues of a
+// replayed graph; the graph's last node advances it by the captured steps
+__global__ void k_step_advance(int *ctr, int d_tidx, int d_slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctr[0] += d_tidx;
+    ctr[1] += d_slot;
+  }
+}
+
+// Capture happens on a private non-blocking stream per device (the caller's
+// stream may be the legacy default stream, which cannot capture); the graph
+// is then launched on the caller's stream.  An executable graph is destroyed
+// only after the event recorded behind its last launch has completed.
+constexpr int kMaxDevices = 64;
+hipStream_t g_capture_stream[kMaxDevices] = {};
+struct RetiredGraph {
+  hipGraphExec_t ex;
+  hipEvent_t done;
+};
+std::vector<RetiredGraph> g_retired;
+
+int capture_stream(hipStream_t *out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device %d outside [0, %d)", dev, kMaxDevices);
+  if (!g_capture_stream[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_capture_stream[dev], hipStreamNonBlocking));
+  *out = g_capture_stream[dev];
+  return 0;
+}
+
+void reap_graphs() {
+  size_t w = 0;
+  for (size_t i = 0; i < g_retired.size(); ++i) {
+    if (hipEventQuery(g_retired[i].done) == hipSuccess) {
+      (void)hipGraphExecDestroy(g_retired[i].ex);
+      (void)hipEventDestroy(g_retired[i].done);
+    } else {
+      g_retired[w++] = g_retired[i];
+    }
+  }
+  g_retired.resize(w);
+}
+
+int retire_graph(hipGraphExec_t ex, hipStream_t st) {
+  hipEvent_t e = nullptr;
+  hipError_t rc = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  if (rc == hipSuccess) rc = hipEventRecord(e, st);
+  if (rc != hipSuccess) {
+    // no completion event: wait for the stream before the graph goes
+    (void)hipStreamSynchronize(st);
+    (void)hipGraphExecDestroy(ex);
+    if (e) (void)hipEventDestroy(e);
+    return fail("graph retirement event: %s", hipGetErrorString(rc));
+  }
+  g_retired.push_back({ex, e});
+  return 0;
+}
 
 int prof_mark(hipStream_t st, int kind, size_t *used) {
   if (!g_prof.on) return 0;
@@ -1248,8 +1363,13 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const long nr = (long)n * op->n_hier;
   const size_t shm = sizeof(double) * (2 * (size_t)n + 8 * (size_t)op->n_act + 2 * (size_t)op->n_mode_abs + 16);
   if (shm > 160 * 1024) return fail("prologue LDS %zu B exceeds 160 KiB", shm);
-  if (shm > 64 * 1024)
+  if (shm > 64 * 1024) {
     HIP_TRY(hipFuncSetAttribute((const void *)k_prologue<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_prologue_linear4<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_steps_resident<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+  }
   const int ns = bt->noise_step;
   if (it0 < 0 || it0 + (long)n_steps * 2 * ns >= bt->n_t)
     return fail("noise index range [%d, %ld] outside [0, %d)", it0, it0 + (long)n_steps * 2 * ns, bt->n_t);
@@ -1264,6 +1384,22 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     HIP_TRY(hipGetLastError());
   }
   const int toff[4] = {0, ns, ns, 2 * ns};
+  if (n_steps > 0 && nr <= g_resident_rows) {
+    size_t used = 0;
+    if (int rc = prof_mark(st, 2, &used)) return rc;
+    k_steps_resident<R><<<bt->B, 256, shm, st>>>(*op, *bt, it0, n_steps, store0, Y0, ya, yb);
+    HIP_TRY(hipGetLastError());
+    if (int rc = prof_mark(st, 1, &used)) return rc;
+    if (int rc = prof_collect(used)) return rc;
+    const int slot = bt->psi_store_d ? store0 + n_steps - 1 : -1;
+    k_final_sigma<<<bt->B, 256, 0, st>>>(*op, *bt, slot);
+    HIP_TRY(hipGetLastError());
+    if (bt->eom == HOPS_EOM_NORMALIZED_NONLINEAR) {
+      k_final_scale<<<bt->n_tiles, 256, 0, st>>>(*op, *bt);
+      HIP_TRY(hipGetLastError());
+    }
+    return 0;
+  }
   // operator-stationary stage kernel when every XCD has a trajectory group and
   // the row's ELL and pattern entries fit in registers
   const int max_pat = op->pat_w;
@@ -1312,6 +1448,19 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const size_t coefT_blk = sizeof(cx<R>) * (size_t)bt->coefT_stride *
                            (2 * (size_t)op->n_pat + (size_t)op->n_mode + 1);
   const size_t coef_blk = (size_t)bt->B * (size_t)bt->coef_stride;
+  // graph replay: step 0 eagerly, then steps 1 .. in launches of K captured
+  // steps whose prologues add the device step counter (zeroed below)
+  const int K = g_graph_steps;
+  bool graph = K > 0 && !g_prof.on && chunk >= bt->B && n_steps - 1 >= K;
+  if (graph) {
+    // a caller that captures this call into a graph of its own gets eager
+    // launches (no capture inside a capture)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    graph = hipStreamIsCapturing(st, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+  }
+  const size_t fuse_bytes = fuse && n_steps > 0 ? 3 * coefT_blk + 3 * coef_blk * sizeof(double) : 0;
+  const size_t ctr_off = (fuse_bytes + 255) & ~(size_t)255;
+  const size_t scr_bytes = graph ? ctr_off + 256 : fuse_bytes;
   struct StreamScratch {
     void *p = nullptr;
     hipStream_t s;
@@ -1320,17 +1469,18 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     }
   } scr;
   scr.s = st;
-  if (fuse && n_steps > 0) {
-    const size_t bytes = 3 * coefT_blk + 3 * coef_blk * sizeof(double);
-    HIP_TRY(hipMallocAsync(&scr.p, bytes, st));
-    HIP_TRY(hipMemsetAsync(scr.p, 0, bytes, st));
+  if (scr_bytes > 0) {
+    HIP_TRY(hipMallocAsync(&scr.p, scr_bytes, st));
+    HIP_TRY(hipMemsetAsync(scr.p, 0, scr_bytes, st));
   }
   char *coefT_s = reinterpret_cast<char *>(scr.p);
   double *coef_s = reinterpret_cast<double *>(coefT_s + 3 * coefT_blk);
+  int *step_ctr = graph ? reinterpret_cast<int *>(coefT_s + ctr_off) : nullptr;
   size_t used = 0;
   if (int rc = prof_mark(st, 2, &used)) return rc;
-  // one RK4 step of a (sub-)batch; slot: psi store slot written at stage 0
-  auto one_step = [&](const hops_batch_t &sb, int step) -> int {
+  // one RK4 step of a (sub-)batch on stream ls; slot: psi store slot written
+  // at stage 0; sc: step counter of a captured graph (nullptr: eager)
+  auto one_step = [&](const hops_batch_t &sb, int step, hipStream_t ls, const int *sc) -> int {
     const int itb = it0 + step * 2 * ns;
     // sub-batch offsets into the (scratch) tables
     const size_t offT = (size_t)((const char *)sb.coefT_d - (const char *)bt->coefT_d);
@@ -1338,10 +1488,10 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     if (fuse) {
       const cx<R> *yin0 = reinterpret_cast<const cx<R> *>(sb.precision == HOPS_PREC_MIXED ? sb.y0_d : sb.phi_d);
       const int slot = (step > 0 && sb.psi_store_d) ? store0 + step - 1 : -1;
-      k_prologue_linear4<R><<<dim3(sb.B, 4), 256, shm, st>>>(*op, sb, itb, ns, yin0, slot, coefT_s + offT, coefT_blk,
-                                                              coef_s + offC, coef_blk);
+      k_prologue_linear4<R><<<dim3(sb.B, 4), 256, shm, ls>>>(*op, sb, itb, ns, yin0, slot, coefT_s + offT, coefT_blk,
+                                                              coef_s + offC, coef_blk, sc);
       HIP_TRY(hipGetLastError());
-      if (int rc = prof_mark(st, 0, &used)) return rc;
+      if (int rc = prof_mark(ls, 0, &used)) return rc;
     }
     for (int i = 0; i < 4; ++i) {
       const cx<R> *yin = i == 0 ? reinterpret_cast<const cx<R> *>(sb.precision == HOPS_PREC_MIXED ? sb.y0_d : sb.phi_d)
@@ -1352,19 +1502,19 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       if (fuse) {
         if (i > 0) sbi.coefT_d = coefT_s + (size_t)(i - 1) * coefT_blk + offT;
       } else {
-        k_prologue<R><<<sb.B, 256, shm, st>>>(*op, sb, i, itb + toff[i], yin, slot);
+        k_prologue<R><<<sb.B, 256, shm, ls>>>(*op, sb, i, itb + toff[i], yin, slot, sc);
         HIP_TRY(hipGetLastError());
-        if (int rc = prof_mark(st, 0, &used)) return rc;
+        if (int rc = prof_mark(ls, 0, &used)) return rc;
       }
 #define HOPS_SHV(W, P, NT)                                                                       \
   do {                                                                                           \
     if (lds_apt > 0) {                                                                           \
-      if (vl == 4) k_stage_lds<R, W, P, NT, 4><<<lds_grid, 512, lds_bytes, st>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
-      else if (vl == 2) k_stage_lds<R, W, P, NT, 2><<<lds_grid, 512, lds_bytes, st>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
-      else k_stage_lds<R, W, P, NT, 1><<<lds_grid, 512, lds_bytes, st>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
-    } else if (vl == 4) k_stage_shared<R, W, P, NT, 4><<<sh_grid, 512, 0, st>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
-    else if (vl == 2) k_stage_shared<R, W, P, NT, 2><<<sh_grid, 512, 0, st>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
-    else k_stage_shared<R, W, P, NT, 1><<<sh_grid, 512, 0, st>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+      if (vl == 4) k_stage_lds<R, W, P, NT, 4><<<lds_grid, 512, lds_bytes, ls>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else if (vl == 2) k_stage_lds<R, W, P, NT, 2><<<lds_grid, 512, lds_bytes, ls>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else k_stage_lds<R, W, P, NT, 1><<<lds_grid, 512, lds_bytes, ls>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+    } else if (vl == 4) k_stage_shared<R, W, P, NT, 4><<<sh_grid, 512, 0, ls>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+    else if (vl == 2) k_stage_shared<R, W, P, NT, 2><<<sh_grid, 512, 0, ls>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+    else k_stage_shared<R, W, P, NT, 1><<<sh_grid, 512, 0, ls>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
   } while (0)
 #define HOPS_SH(W, P)                                                                            \
   do {                                                                                           \
@@ -1380,11 +1530,11 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       else if (sh_w == 8 && sh_p == 8) HOPS_SH(8, 8);
       else if (sh_w == 16 && sh_p == 4) HOPS_SH(16, 4);
       else if (sh_w == 16 && sh_p == 8) HOPS_SH(16, 8);
-      else k_stage<R><<<sb.n_tiles, 256, 0, st>>>(*op, sbi, i, yin, yout);
+      else k_stage<R><<<sb.n_tiles, 256, 0, ls>>>(*op, sbi, i, yin, yout);
 #undef HOPS_SH
 #undef HOPS_SHV
       HIP_TRY(hipGetLastError());
-      if (int rc = prof_mark(st, 1, &used)) return rc;
+      if (int rc = prof_mark(ls, 1, &used)) return rc;
     }
     return 0;
   };
@@ -1409,16 +1559,46 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     if (bt->psi_store_d) sb.psi_store_d = bt->psi_store_d + (size_t)b0 * bt->store_stride * 2 * op->n_state;
     return sb;
   };
-  if (chunk >= bt->B) {
+  if (graph) {
+    if (int rc = one_step(*bt, 0, st, nullptr)) return rc;
+    reap_graphs();
+    hipStream_t cs = nullptr;
+    if (int rc = capture_stream(&cs)) return rc;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int j = 0; j < K && !rc; ++j) rc = one_step(*bt, 1 + j, cs, step_ctr);
+    if (!rc) {
+      k_step_advance<<<1, 64, 0, cs>>>(step_ctr, K * 2 * ns, K);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail("k_step_advance launch: %s", hipGetErrorString(e));
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(cs, &g);
+    if (rc || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail("hipStreamEndCapture failed: %s", hipGetErrorString(ce));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiateWithFlags(&ex, g, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail("hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+    const int n_rep = (n_steps - 1) / K;
+    hipError_t le = hipSuccess;
+    for (int c = 0; c < n_rep && le == hipSuccess; ++c) le = hipGraphLaunch(ex, st);
+    if (int rr = retire_graph(ex, st)) return rr;
+    if (le != hipSuccess) return fail("hipGraphLaunch failed: %s", hipGetErrorString(le));
+    for (int step = 1 + n_rep * K; step < n_steps; ++step)
+      if (int r2 = one_step(*bt, step, st, nullptr)) return r2;
+  } else if (chunk >= bt->B) {
     for (int step = 0; step < n_steps; ++step)
-      if (int rc = one_step(*bt, step)) return rc;
+      if (int rc = one_step(*bt, step, st, nullptr)) return rc;
   } else {
     // all steps of one chunk before the next: the chunk's pools stay in the
     // last-level cache across the whole propagation
     for (int b0 = 0; b0 < bt->B; b0 += chunk) {
       const hops_batch_t sb = sub_batch(b0, std::min(chunk, bt->B - b0));
       for (int step = 0; step < n_steps; ++step)
-        if (int rc = one_step(sb, step)) return rc;
+        if (int rc = one_step(sb, step, st, nullptr)) return rc;
     }
   }
   if (int rc = prof_collect(used)) return rc;
@@ -1796,6 +1976,16 @@ int hops_set_option(const char *name, int value) {
     g_vec_lanes = value;
     return 0;
   }
+  if (std::string(name) == "resident_rows") {
+    if (value < 0) return fail("resident_rows must be >= 0");
+    g_resident_rows = value;
+    return 0;
+  }
+  if (std::string(name) == "graph_steps") {
+    if (value < 0 || value > 4096) return fail("graph_steps must be 0..4096");
+    g_graph_steps = value;
+    return 0;
+  }
   if (std::string(name) == "groups_q") {
     if (value < 0) return fail("groups_q must be >= 0");
     g_groups_q = value;
@@ -1846,7 +2036,7 @@ int hops_derivative(const hops_op_t *op, const hops_batch_t *bt, const void *y_d
   if (shm > 64 * 1024)
     HIP_TRY(hipFuncSetAttribute((const void *)k_prologue<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   const cx<double> *y = reinterpret_cast<const cx<double> *>(y_d);
-  k_prologue<double><<<bt->B, 256, shm, st>>>(*op, *bt, -1, tidx, y, -1);
+  k_prologue<double><<<bt->B, 256, shm, st>>>(*op, *bt, -1, tidx, y, -1, nullptr);
   HIP_TRY(hipGetLastError());
   k_stage<double><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, -1, y, reinterpret_cast<cx<double> *>(d_d));
   HIP_TRY(hipGetLastError());

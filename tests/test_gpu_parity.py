@@ -18,11 +18,27 @@ pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
 POP_TOL = 1e-4
+RESIDENT_DEFAULT = 1024  # the library's default "resident_rows"
 
 
 def _need_gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
+
+
+@pytest.fixture(autouse=True)
+def _launched_forms():
+    """Small bases run in the resident form by default; this module tests the
+    launched stage kernels (the resident tests switch it on themselves)."""
+    if not torch.cuda.is_available():
+        yield
+        return
+    from mesohops_amd import _lib
+    _lib.set_option("resident_rows", 0)
+    try:
+        yield
+    finally:
+        _lib.set_option("resident_rows", RESIDENT_DEFAULT)
 
 
 def oracle_run(sysp, eom, noise_param, maxhier, t_max, dt, noise2_param=None):
@@ -336,6 +352,108 @@ def test_fused_linear_prologue_bit_identical():
             print(prec, "lanes", lanes, "fused vs per-stage prologue max diff", d)
             assert d == 0.0
             assert np.array_equal(got.phi, ref.phi)
+
+
+def _batch_two_calls(sysp, eom, zs, tlen, tau, psi0, prec, lanes, z2=None):
+    """Two propagate calls (17 and 13 steps of 1 fs) of one batch."""
+    from mesohops_amd import HopsBatch
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+    n2 = None
+    if z2 is not None:
+        n2 = {"SEED": z2[0], "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau, "FLAG_REAL": False}
+    b = HopsBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], n2, {"MAXHIER": 3}, noise_params=nps,
+                  precision_mode=prec, lanes=lanes)
+    b.initialize(psi0)
+    b.propagate(17.0, 1.0)
+    b.propagate(13.0, 1.0)
+    return b
+
+
+@pytest.mark.parametrize("eom", ["LINEAR", "NORMALIZED NONLINEAR"])
+def test_graph_replay_bit_identical(eom):
+    """RK4 steps replayed from a captured HIP graph (the default, 16 steps per
+    graph) give the same trajectories and hierarchy as eager launches, bit
+    for bit: graphs of 1, 4, 16 and 29 steps (remainder steps run eagerly),
+    two propagate calls in a row, both precisions, streaming and stationary
+    stage kernels, the LINEAR fused prologue and the per-stage prologues."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1, ltc=37.5 - 12.0j)
+    tlen, tau = 80.0, 0.5
+    B = 40
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[1] = 1
+    for prec in ("mixed", "fp64"):
+        for lanes in (1, 4):
+            _lib.set_option("graph_steps", 0)
+            try:
+                ref = _batch_two_calls(sysp, eom, zs, tlen, tau, psi0, prec, lanes)
+                for K in (1, 4, 16, 29):
+                    _lib.set_option("graph_steps", K)
+                    got = _batch_two_calls(sysp, eom, zs, tlen, tau, psi0, prec, lanes)
+                    d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i])))
+                            for i in range(B))
+                    print(eom, prec, "lanes", lanes, "graph_steps", K, "max diff vs eager", d, flush=True)
+                    assert len(got.psi_traj[0]) == len(ref.psi_traj[0]) == 31
+                    assert d == 0.0
+                    assert np.array_equal(got.phi, ref.phi)
+            finally:
+                _lib.set_option("graph_steps", 16)
+
+
+@pytest.mark.parametrize("eom", ["LINEAR", "NONLINEAR", "NORMALIZED NONLINEAR"])
+def test_resident_steps_bit_identical(eom):
+    """The resident form (one workgroup per trajectory running every step of
+    a call) gives the same trajectories and hierarchy as the launched stage
+    kernels, bit for bit: two propagate calls, both precisions, one and four
+    interleaved lanes, the low-temperature correction and a second noise."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.chain(4, k_mats=1, ltc=37.5 - 12.0j)
+    sysp.update(L_NOISE2=sysp["L_NOISE1"], ALPHA_NOISE2=O.bcf_exp, PARAM_NOISE2=sysp["PARAM_NOISE1"])
+    tlen, tau = 80.0, 0.5
+    B = 40
+    zs = [_oracle_noise(sysp, s, tlen, tau) for s in range(B)]
+    z2 = [_oracle_noise(sysp, 100 + s, tlen, tau) for s in range(B)]
+    psi0 = np.zeros(4, complex)
+    psi0[1] = 1
+    for prec in ("mixed", "fp64"):
+        for lanes in (1, 4):
+            _lib.set_option("resident_rows", 0)
+            ref = _batch_two_calls(sysp, eom, zs, tlen, tau, psi0, prec, lanes, z2=z2)
+            _lib.set_option("resident_rows", 1 << 20)
+            try:
+                got = _batch_two_calls(sysp, eom, zs, tlen, tau, psi0, prec, lanes, z2=z2)
+            finally:
+                _lib.set_option("resident_rows", 0)
+            d = max(np.max(np.abs(np.array(got.psi_traj[i]) - np.array(ref.psi_traj[i]))) for i in range(B))
+            print(eom, prec, "lanes", lanes, "resident vs launched max diff", d, flush=True)
+            assert d == 0.0
+            assert np.array_equal(got.phi, ref.phi)
+
+
+def test_resident_steps_match_oracle_c1():
+    """The C1 dimer (2 sites, depth 4, dt 4 fs to 1000 fs) in the resident
+    form against the FP64 oracle."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    sysp = systems.dimer_c1()
+    tlen, tau = 1024.0, 2.0
+    z = _oracle_noise(sysp, 0, tlen, tau)
+    nparam = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    tr = O.Trajectory(sysp, {"EQUATION_OF_MOTION": "LINEAR"}, nparam, hierarchy_param={"MAXHIER": 4})
+    psi0 = np.array([1.0, 0.0], complex)
+    tr.initialize(psi0)
+    tr.propagate(1000.0, 4.0)
+    _lib.set_option("resident_rows", 1 << 20)
+    try:
+        b = engine_run(sysp, "LINEAR", [z], tlen, tau, 4, 1000.0, 4.0, psi0)
+    finally:
+        _lib.set_option("resident_rows", 0)
+    err = np.max(np.abs(np.array(b.psi_traj[0]) - np.array(tr.psi_traj)))
+    print("resident C1 dimer max |dpsi| =", err)
+    assert err < FP64_TOL * max(1.0, np.max(np.abs(tr.psi_traj)))
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8])
