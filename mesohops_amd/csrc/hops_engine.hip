@@ -16,6 +16,10 @@
 //               block, ELL super-operator K2_kp1+K2_km1, Z2 term, -norm_corr)
 //               fused with the RK4 combination.  Stages run in complex R
 //               (double or float); the state carry Phi stays complex128.
+// A LINEAR step computes its four prologues in one launch
+// (k_prologue_linear4).  Bases of at most resident_rows rows per trajectory
+// run a whole call in one launch instead (k_steps_resident); larger ones
+// replay K captured steps from a HIP graph.
 //
 // Normalisation (NORMALIZED NONLINEAR) is lazy: the scale sigma = 1/|psi| of
 // the state produced by the previous step is computed in FP64 by the stage-0
