@@ -1268,6 +1268,10 @@ int g_fuse_linear = 1;     // 1: one prologue launch per LINEAR step (k_prologue
 int g_graph_steps = 16;    // RK4 steps per captured HIP graph (0: every kernel launched eagerly)
 int g_resident_rows = 1024;  // rows per trajectory up to which k_steps_resident runs a whole call
 
+__global__ __launch_bounds__(256) void k_zero_words(unsigned long long *p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
 // counter {noise index offset, store slot offset} read by the prologues of a
 // replayed graph; the graph's last node advances it by the captured steps
 __global__ void k_step_advance(int *ctr, int d_tidx, int d_slot) {
@@ -1475,7 +1479,11 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   scr.s = st;
   if (scr_bytes > 0) {
     HIP_TRY(hipMallocAsync(&scr.p, scr_bytes, st));
-    HIP_TRY(hipMemsetAsync(scr.p, 0, scr_bytes, st));
+    // cleared by a kernel, not a memset node, should the caller capture
+    const long words = (long)(scr_bytes / 8);
+    k_zero_words<<<(unsigned)std::min<long>((words + 255) / 256, 4096), 256, 0, st>>>(
+        reinterpret_cast<unsigned long long *>(scr.p), words);
+    HIP_TRY(hipGetLastError());
   }
   char *coefT_s = reinterpret_cast<char *>(scr.p);
   double *coef_s = reinterpret_cast<double *>(coefT_s + 3 * coefT_blk);
