@@ -1003,7 +1003,27 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     }
     m2p[lm + 1] = (int)m2s.size();
   }
-  for (int k = 0; k < nh; ++k) {
+  // diagonal L-operators give every mode one M2 entry: then the sums below
+  // are one product each, and the loop over modes is a plain gather
+  bool single = true;
+  for (int lm = 0; lm < nm; ++lm) single &= m2p[lm + 1] - m2p[lm] == 1;
+  for (int k = 0; k < nh && single; ++k) {
+    const double *c2 = &out.absC2[(size_t)k * n];
+    double colsum = 0;
+    for (int s2 = 0; s2 < n; ++s2) colsum += c2[s2];
+    double *eu = &E_up[(size_t)k * nm], *ed = &E_dn[(size_t)k * nm];
+    const uint8_t *kv = &kval[(size_t)k * nm];
+    for (int lm = 0; lm < nm; ++lm) {
+      const double c = c2[m2s[lm]];
+      const double p_up = m2u[lm] * c;
+      double p_dn = m2d[lm] * c;
+      if (nX[lm] != 0) p_dn += nX[lm] * (colsum - c);
+      const double kw = aw[lm] * (1.0 + kv[lm]);
+      eu[lm] = (kw * kw * p_up) / h2;
+      ed[lm] = (gw2[lm] * p_dn) / h2;
+    }
+  }
+  for (int k = 0; k < nh && !single; ++k) {
     const double *c2 = &out.absC2[(size_t)k * n];
     double colsum = 0;
     for (int s2 = 0; s2 < n; ++s2) colsum += c2[s2];
