@@ -36,6 +36,28 @@ def test_library_exports_every_declared_symbol():
     assert _lib.lib().hops_version() == _lib.ABI_VERSION
 
 
+def test_engine_options_validate_without_a_device():
+    """hops_set_option takes every documented option within its range and
+    rejects unknown names and out-of-range values with a message (it sets
+    library state only, so it runs without a GPU)."""
+    from mesohops_amd import _lib
+    from mesohops_amd.errors import EngineError
+    L = _lib.lib()
+    good = {"stationary_stage": 1, "chunk_traj": 0, "nt_stream": 2, "groups_q": 0, "vec_lanes": 2,
+            "lds_stage": 0, "fuse_prologue": 1, "graph_steps": 16, "resident_rows": 1024}
+    for name, v in good.items():
+        assert L.hops_set_option(name.encode(), v) == 0, name
+    for name, v in (("nt_stream", 5), ("vec_lanes", 3), ("graph_steps", -1), ("graph_steps", 5000),
+                    ("resident_rows", -2), ("chunk_traj", -1), ("no_such_option", 1)):
+        assert L.hops_set_option(name.encode(), v) != 0, name
+        assert L.hops_last_error().decode(), name
+    with pytest.raises(EngineError):
+        _lib.set_option("graph_steps", -1)
+    # the library defaults again
+    for name, v in (("vec_lanes", 1), ("nt_stream", 0), ("graph_steps", 16), ("resident_rows", 1024)):
+        assert L.hops_set_option(name.encode(), v) == 0
+
+
 def test_struct_layout_matches_header():
     """ctypes mirrors of hops_op_t / hops_batch_t have the header's field order."""
     from mesohops_amd._lib import HopsBatch, HopsOp
