@@ -1004,7 +1004,10 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     m2p[lm + 1] = (int)m2s.size();
   }
   // diagonal L-operators give every mode one M2 entry: then the sums below
-  // are one product each, and the loop over modes is a plain gather
+  // are one product each, and the loop over modes is a plain gather.  Only
+  // the terms read later are formed (flux up below the top depth where the
+  // filters allow it, flux down where the auxiliary has depth in the mode),
+  // and err[k] takes their sums in the same order as the general loop.
   bool single = true;
   for (int lm = 0; lm < nm; ++lm) single &= m2p[lm + 1] - m2p[lm] == 1;
   for (int k = 0; k < nh && single; ++k) {
@@ -1013,15 +1016,26 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     for (int s2 = 0; s2 < n; ++s2) colsum += c2[s2];
     double *eu = &E_up[(size_t)k * nm], *ed = &E_dn[(size_t)k * nm];
     const uint8_t *kv = &kval[(size_t)k * nm];
+    const uint8_t *fu = &F_up[(size_t)k * nm];
+    const bool top = B.depth[k] == P.maxhier;
+    double acc_up = 0, acc_dn = 0;
     for (int lm = 0; lm < nm; ++lm) {
       const double c = c2[m2s[lm]];
-      const double p_up = m2u[lm] * c;
-      double p_dn = m2d[lm] * c;
-      if (nX[lm] != 0) p_dn += nX[lm] * (colsum - c);
-      const double kw = aw[lm] * (1.0 + kv[lm]);
-      eu[lm] = (kw * kw * p_up) / h2;
-      ed[lm] = (gw2[lm] * p_dn) / h2;
+      if (!top && fu[lm]) {
+        const double p_up = m2u[lm] * c;
+        const double kw = aw[lm] * (1.0 + kv[lm]);
+        eu[lm] = (kw * kw * p_up) / h2;
+        acc_up += eu[lm];
+      }
+      if (kv[lm] > 0) {
+        double p_dn = m2d[lm] * c;
+        if (nX[lm] != 0) p_dn += nX[lm] * (colsum - c);
+        ed[lm] = (gw2[lm] * p_dn) / h2;
+        acc_dn += ed[lm];
+      }
     }
+    err[k] += acc_up;
+    err[k] += acc_dn;
   }
   for (int k = 0; k < nh && !single; ++k) {
     const double *c2 = &out.absC2[(size_t)k * n];
@@ -1045,7 +1059,7 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
       ed[lm] = (gw2[lm] * p_dn) / h2;
     }
   }
-  for (int k = 0; k < nh; ++k) {
+  for (int k = 0; k < nh && !single; ++k) {
     const bool top = B.depth[k] == P.maxhier;
     const size_t o = (size_t)k * nm;
     double acc = 0;
