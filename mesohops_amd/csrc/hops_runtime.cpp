@@ -984,7 +984,9 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   std::vector<uint8_t> kval((size_t)nh * nm, 0);  // K2_aux_by_mode
   for (int k = 0; k < nh; ++k)
     for (int q = B.aux_ptr[k]; q < B.aux_ptr[k + 1]; ++q) kval[(size_t)k * nm + B.rel_mode[B.aux_mode_t[q]]] = B.aux_val_t[q];
-  std::vector<double> E_up((size_t)nh * nm, 0.0), E_dn((size_t)nh * nm, 0.0);
+  // every entry read below is written first (all of them in the general
+  // loop), so the arrays start uninitialised
+  std::unique_ptr<double[]> E_up(new double[(size_t)nh * nm]), E_dn(new double[(size_t)nh * nm]);
   std::vector<uint8_t> F_up;
   markov_up(P, B, F_up);
   std::vector<double> aw(nm), gw2(nm), nX(nm);
