@@ -249,10 +249,12 @@ struct Basis {
   }
 };
 
-int build_basis(const System &S, int eom, std::vector<int> states, std::vector<Aux> aux, Basis &B) {
+int build_basis(const System &S, int eom, std::vector<int> states, std::vector<Aux> aux_in, Basis &B) {
   B = Basis();
   B.sys = &S;
   B.eom = eom;
+  B.aux = std::move(aux_in);  // ordered in place: no copy of the auxiliaries
+  std::vector<Aux> &aux = B.aux;
   std::sort(states.begin(), states.end());
   states.erase(std::unique(states.begin(), states.end()), states.end());
   {
@@ -278,7 +280,6 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   if (aux.empty() || !aux[0].empty()) return rt_fail("Zero Vector not contained in list_aux!");
   if (states.empty()) return rt_fail("empty state basis");
   B.states = states;
-  B.aux = aux;
   const int n = (int)states.size(), nh = (int)aux.size();
   B.n = n;
   B.nh = nh;
@@ -2125,7 +2126,7 @@ int init_pending(Batch &b) {
   for (int s : adapt) {
     Slot &sl = b.slots[s];
     ts.push_back(sl.sel.states_new);
-    ta.push_back(sl.sel.aux_new);
+    ta.push_back(std::move(sl.sel.aux_new));  // not read again before the next selection
     std::vector<cplx> psi(sl.pn.begin(), sl.pn.begin() + sl.B.n);
     for (auto &v : psi) v *= sl.nrm;
     std::vector<int> sn(sl.sel.states_new);
@@ -2235,7 +2236,7 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
   for (int s : upd) {
     Slot &sl = b.slots[s];
     ts.push_back(sl.sel.states_new);
-    ta.push_back(sl.sel.aux_new);
+    ta.push_back(std::move(sl.sel.aux_new));  // not read again before the next selection
     std::vector<cplx> psi(sl.pn.begin(), sl.pn.begin() + sl.B.n);
     for (auto &v : psi) v *= sl.nrm;
     std::vector<int> sn(sl.sel.states_new);
