@@ -36,6 +36,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 #include <algorithm>
@@ -1292,8 +1293,10 @@ struct RetiredGraph {
   hipEvent_t done;
 };
 std::vector<RetiredGraph> g_retired;
+std::mutex g_graph_mu;  // the capture streams and the retired list, across host threads
 
 int capture_stream(hipStream_t *out) {
+  std::lock_guard<std::mutex> lk(g_graph_mu);
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) return fail("device %d outside [0, %d)", dev, kMaxDevices);
@@ -1303,6 +1306,7 @@ int capture_stream(hipStream_t *out) {
 }
 
 void reap_graphs() {
+  std::lock_guard<std::mutex> lk(g_graph_mu);
   size_t w = 0;
   for (size_t i = 0; i < g_retired.size(); ++i) {
     if (hipEventQuery(g_retired[i].done) == hipSuccess) {
@@ -1326,6 +1330,7 @@ int retire_graph(hipGraphExec_t ex, hipStream_t st) {
     if (e) (void)hipEventDestroy(e);
     return fail("graph retirement event: %s", hipGetErrorString(rc));
   }
+  std::lock_guard<std::mutex> lk(g_graph_mu);
   g_retired.push_back({ex, e});
   return 0;
 }
