@@ -167,7 +167,8 @@ int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
    "resident_rows": bases of at most this many rows per trajectory (default
    1024) run a whole hops_propagate call in one launch, one workgroup per
    trajectory looping over the steps (0: never).
-   No option changes any result. */
+   Options are process-wide library state: set them from one thread while
+   no other call is in progress.  No option changes any result. */
 int hops_set_option(const char *name, int value);
 
 /* Fill op->ell_col_d/ell_val_d/k0_d from the connection tables.
