@@ -8,21 +8,34 @@ operations are used in the same order and dtype.
 
 Citations are to MesoHOPS v1.6.0 ``src/mesohops/...``.
 
-Pins (tests/test_oracle_pins.py): the reference's golden fixtures (super-
-operators, raw noise, correlation function, filtered noise, a whole
-trajectory), brute force and closed forms (derivative, RK4, z-memory,
-determine_error_thresh, cubic interpolation of cubics), invariants (adaptive
-with vanishing thresholds equals the full hierarchy; derivative error below
-delta; _operator equal to a block-by-block product; STATIC_BASIS kept).
+Pins (tests/test_oracle_pins*.py):
+- golden fixtures of the reference: super-operators, raw noise, correlation
+  function, filtered noise, a whole trajectory, the raw noise rows of the
+  basis tests;
+- the reference's hand-computed answers: low-temperature-correction terms
+  (matrices, norm correction, linear term) and their effect on the three
+  equations of motion; every adaptive error term (deletion flux, state flux
+  out of auxiliaries and states, boundary-state flux, flux up / down for
+  site-projector, Peierls and mixed L-operators, by destination state, the
+  mode-from-state and expectation tables, the noise and low-temperature
+  operators of off-diagonal couplings), the boundary-auxiliary binning with
+  f_discard, and the stable / boundary auxiliaries and states chosen on the
+  reference's noise;
+- brute force and closed forms (derivative, RK4, z-memory,
+  determine_error_thresh, flux terms by explicit loops, cubic interpolation
+  of cubics);
+- invariants (adaptive with vanishing thresholds equals the full hierarchy;
+  derivative error below delta, for site projectors and for off-diagonal
+  L-operators with and without the low-temperature correction; _operator
+  equal to a block-by-block product; STATIC_BASIS kept).
 The dyadic part (tests/test_dyadic_host.py): the reference's dyadic-matrix
 and side-operator answers, and absorption / fluorescence without a bath
 against exact RK4 matrix powers.
 
-Parity unpinned: the raw draws of L-operator rows beyond those the fixture
-holds (index >= 2 for SUM_GAUSSIAN, >= 1 for BOX_MULLER) are NumPy's own
+Parity unpinned: the raw draws of L-operator rows beyond those the fixtures
+hold (index >= 2 for SUM_GAUSSIAN, >= 1 for BOX_MULLER) are NumPy's own
 PCG64 output in the reference's call sequence, with no fixture of their
-values; and the exact adaptive basis sequence over long runs is pinned only
-through the invariants above.
+values.
 """
 from __future__ import annotations
 
@@ -324,6 +337,47 @@ def expectation(op, vec):
     return (np.conj(vec) @ (op @ vec)) / (np.conj(vec) @ vec)
 
 
+def _dense(op):
+    return op.toarray() if sp.issparse(op) else np.asarray(op)
+
+
+def lt_corr(lt, L_list, avg, Lsq_list):
+    """Low-temperature correction operators of the nonlinear equations
+    (eom/eom_functions.py:215-256): the physical-wave-function term
+    sum_n (c_n <L_n> L_n - c_n L_n^2) and the all-auxiliary term
+    sum_n conj(c_n) <L_n> L_n, summed in list order.  Operators may be dense
+    or sparse; with no operators both terms are the scalar 0."""
+    c_phys, c_hier = 0, 0
+    for c, L, a, L2 in zip(lt, L_list, avg, Lsq_list):
+        c_phys = c_phys + ((c * a) * L - c * L2)
+        c_hier = c_hier + (np.conj(c) * a) * L
+    return c_phys, c_hier
+
+
+def lt_corr_to_norm_corr(lt, avg, avg_sq):
+    """Low-temperature part of the normalisation correction,
+    sum_n Re(c_n) (2 <L_n>^2 - <L_n^2>)  (eom/eom_functions.py:258-284)."""
+    if len(lt) == 0:
+        return 0.0
+    terms = np.real(np.asarray(lt)) * (2 * np.asarray(avg) ** 2 - np.asarray(avg_sq))
+    return np.sum(terms)
+
+
+def lt_corr_linear(lt, Lsq_list):
+    """Low-temperature correction of the linear equation, -sum_n c_n L_n^2,
+    for the physical wave function only (eom/eom_functions.py:286-309).  The
+    coefficients and squared operators must pair up one to one: the reference
+    raises on a length mismatch and so does this."""
+    if len(lt) != len(Lsq_list):
+        raise ValueError("%d correction coefficients for %d operators" % (len(lt), len(Lsq_list)))
+    if len(lt) == 0:
+        return 0
+    acc = lt[0] * Lsq_list[0]
+    for c, L2 in zip(lt[1:], Lsq_list[1:]):
+        acc = acc + c * L2
+    return -acc
+
+
 def make_derivative(basis, eom_type):
     """Returns dsystem_dt(phi, z_mem, z1, z2) -> (hbar dphi/dt, hbar dz_mem/dt).
 
@@ -348,8 +402,7 @@ def make_derivative(basis, eom_type):
             d += (b.K0[:, None] * phi.reshape(nh, n)).reshape(-1)
             d += rows_apply(-1j * H, phi)
             if np.any(b.lt_active):
-                corr = -1 * np.sum(np.array(b.lt_active) * np.array(b.L_sq_csr, dtype=object), axis=0)
-                d[:n] += corr @ phi[:n]
+                d[:n] += lt_corr_linear(list(b.lt_active), list(b.L_sq_csr)) @ phi[:n]
             zhat = np.conj(z1)
             for j in range(len(b.L_csr)):
                 d += (zhat[j] - 1j * z2[j]) * rows_apply(b.L_csr[j], phi)
@@ -383,9 +436,8 @@ def make_derivative(basis, eom_type):
             avg_sq = [expectation(b.L_sq_csr[j], psi) for j in act]
             L_act = [b.L_csr[j] for j in act]
             Lsq_act = [b.L_sq_csr[j] for j in act]
-            c_phys = sum((lt[k] * avg[k]) * L_act[k] - lt[k] * Lsq_act[k] for k in range(len(act)))
-            c_hier = sum((np.conj(lt[k]) * avg[k]) * L_act[k] for k in range(len(act)))
-            lt_norm = np.sum(np.real(lt) * (2 * np.array(avg) ** 2 - np.array(avg_sq)))
+            c_phys, c_hier = lt_corr(lt, L_act, avg, Lsq_act)
+            lt_norm = lt_corr_to_norm_corr(lt, avg, avg_sq)
         d = b.K2_stable @ phi
         d += (b.K0[:, None] * phi.reshape(nh, n)).reshape(-1)
         d += rows_apply(-1j * H, phi)
@@ -711,9 +763,12 @@ class Trajectory:
 # ---------------------------------------------------------------------------
 # Adaptive basis (basis/hops_basis.py:117-292, :294-998, :1000-1151;
 # basis/basis_functions_adaptive.py; basis/basis_functions.py;
-# basis/hops_fluxfilters.py).  Restated for L-operators without off-diagonal
-# entries (site projectors), where the noise/LTC state-flux terms vanish
-# (hops_basis.py:914-998 return empty / None) and no static filters.
+# basis/hops_fluxfilters.py).  General L-operators: the diagonal parts enter
+# through M2_diag [mode, state], the off-diagonal parts through one
+# [mode, state] table per destination state, plus the noise and
+# low-temperature terms that off-diagonal operators add to the state flux
+# (hops_basis.py:914-998).  Arrays are dense; Phi is aux-major
+# (row k*n_state + s), so C2 = Phi.reshape(n_hier, n_state).T is [s, k].
 # ---------------------------------------------------------------------------
 def determine_error_thresh(sorted_error, max_error, offset=0.0):
     """basis/basis_functions.py:8-41."""
@@ -738,6 +793,214 @@ def _aux_depth(aux):
     return sum(v for (m, v) in aux)
 
 
+def _C2(phi, n_state, n_hier):
+    return np.asarray(phi).reshape(n_hier, n_state).T
+
+
+def _abs2(x):
+    x = x.toarray() if sp.issparse(x) else np.asarray(x)
+    return np.abs(x) ** 2
+
+
+def _off_super(M2_off, shape):
+    """sum over destinations d of |M_d|^2."""
+    out = np.zeros(shape)
+    for M in M2_off:
+        out = out + _abs2(M)
+    return out
+
+
+def mode_from_state_tables(L_by_mode, states, dests):
+    """(M2_diag, [M_d for d in dests]) for the L-operators of the modes given
+    on the full state space: M2_diag[m, i] = L_m[s_i, s_i] over the basis
+    states s_i, and M_d[m, i] = L_m[d, s_i] for s_i != d
+    (hops_basis.py:1030-1100)."""
+    L_by_mode = [_dense(L) for L in L_by_mode]
+    nm, n = len(L_by_mode), len(states)
+    diag = np.zeros((nm, n), complex)
+    for m, L in enumerate(L_by_mode):
+        for i, s in enumerate(states):
+            diag[m, i] = L[s, s]
+    off = []
+    for d in dests:
+        M = np.zeros((nm, n), complex)
+        for m, L in enumerate(L_by_mode):
+            for i, s in enumerate(states):
+                if s != d:
+                    M[m, i] = L[d, s]
+        off.append(M)
+    return diag, off
+
+
+def expectation_table(avg_by_mode, n_state):
+    """X2[m, s] = <L_m> for every basis state s (hops_basis.py:1118-1129)."""
+    return np.repeat(np.asarray(avg_by_mode, dtype=complex).reshape(-1, 1), n_state, axis=1)
+
+
+def error_deriv(dsystem_dt, phi, z_step, n_state, n_hier, dt, stable_idx=None):
+    """|dPhi/dt / hbar + Phi / dt|^2 as [s, k]: the deletion flux, optionally
+    out of the stable auxiliaries only (basis_functions_adaptive.py:10-74)."""
+    if stable_idx is not None:
+        P = np.zeros(n_state * n_hier, complex)
+        Pv = P.reshape(n_hier, n_state)
+        Pv[stable_idx] = np.asarray(phi).reshape(n_hier, n_state)[stable_idx]
+    else:
+        P = np.asarray(phi)
+    d = dsystem_dt(P, z_step[2], z_step[0], z_step[1])[0] / HBAR
+    d = d + P / dt
+    D = _C2(d, n_state, n_hier)
+    if stable_idx is not None:
+        D = D[:, stable_idx]
+    return np.abs(D) ** 2
+
+
+def error_sflux_hier(phi, s0, sc, n_state, n_hier, H2, T_phys=None, T_hier=None):
+    """Per auxiliary k: sum_d |((H + T_k)[d, s] @ psi_k)[d]|^2 / hbar^2 over the
+    states d outside the basis; H2 is the full-space (-iH + noise) operator and
+    T_k the low-temperature operator (T_phys for k = 0, T_hier otherwise)
+    (basis_functions_adaptive.py:76-163)."""
+    C2 = _C2(phi, n_state, n_hier)
+    s0, sc = list(s0), list(sc)
+    H2 = _dense(H2)
+    if T_phys is None:
+        Hsc = H2[np.ix_(sc, s0)]
+        return np.sum(np.abs(Hsc @ C2 / HBAR) ** 2, axis=0)
+    Hp = (H2 + _dense(T_phys))[np.ix_(sc, s0)]
+    Hh = (H2 + _dense(T_hier))[np.ix_(sc, s0)]
+    Caux = C2.copy()
+    Caux[:, 0] = 0
+    out = np.sum(np.abs(Hh @ Caux / HBAR) ** 2, axis=0)
+    out[0] = np.sum(np.abs(Hp @ C2[:, :1] / HBAR) ** 2)
+    return out
+
+
+def error_flux_up_hier_stable(phi, n_state, n_hier, n_modes, w, K2, M2_diag, M2_off):
+    """E[m, k] = |w_m (1 + k_m)|^2 (sum_d |M_d|^2 @ |Phi|^2)[m, k] / hbar^2,
+    the diagonal part counted as one more destination
+    (basis_functions_adaptive.py:165-266)."""
+    KW = np.abs(np.asarray(w)).reshape(n_modes, 1) * (1 + np.asarray(K2))
+    Msup = _abs2(M2_diag)
+    if len(M2_off):
+        Msup = Msup + _off_super(M2_off, Msup.shape)
+    return np.abs(KW) ** 2 * (Msup @ (np.abs(_C2(phi, n_state, n_hier)) ** 2)) / HBAR ** 2
+
+
+def error_flux_up_state_stable(phi, n_state, n_hier, n_modes, w, K2, M2_diag, M2_off,
+                               F_diag=None, F_off=None):
+    """E[s, k] = (|M|^2.T @ |F * w (1 + k)|^2)[s, k] |Phi[s, k]|^2 / hbar^2, for
+    the diagonal table with filter F_diag plus the off-diagonal tables with
+    F_off (basis_functions_adaptive.py:268-384)."""
+    KW = np.abs(np.asarray(w)).reshape(n_modes, 1) * (1 + np.asarray(K2))
+    C2sq = np.abs(_C2(phi, n_state, n_hier)) ** 2
+    Fd = 1 if F_diag is None else F_diag
+    E = (_abs2(M2_diag).T @ (np.abs(Fd * KW) ** 2)) * C2sq / HBAR ** 2
+    Fo = 1 if F_off is None else F_off
+    Moff = _off_super(M2_off, np.shape(M2_diag))
+    if np.sum(Moff) > 0:
+        E = E + (Moff.T @ (np.abs(Fo * KW) ** 2)) * C2sq / HBAR ** 2
+    return E
+
+
+def error_flux_up_by_dest_state(phi, n_state, n_hier, n_modes, w, K2, M2_off_dest,
+                                stable_idx=None, F_off=None, stable_states=None):
+    """Total off-diagonal flux up into each destination state, out of the
+    stable auxiliaries and stable states (basis_functions_adaptive.py:386-515)."""
+    KW = np.abs(np.asarray(w)).reshape(n_modes, 1) * (1 + np.asarray(K2))
+    C2 = _C2(phi, n_state, n_hier)
+    if stable_states is not None:
+        keep = np.zeros(n_state, bool)
+        keep[list(stable_states)] = True
+        C2 = np.where(keep[:, None], C2, 0)
+    idx = list(range(n_hier)) if stable_idx is None else list(stable_idx)
+    Fo = 1 if F_off is None else np.asarray(F_off)[:, idx]
+    KWo = np.abs(Fo * KW[:, idx]) ** 2
+    C2sq = np.abs(C2[:, idx]) ** 2
+    return np.array([np.sum((_abs2(M).T @ KWo) * C2sq / HBAR ** 2) for M in M2_off_dest])
+
+
+def error_flux_down_hier_stable(phi, n_state, n_hier, n_modes, g, w, M2_diag, M2_off, X2):
+    """E[m, k] = |g_m / w_m|^2 ((|M_diag - X2|^2 + sum_d |M_d|^2) @ |Phi|^2)[m, k]
+    / hbar^2 (basis_functions_adaptive.py:517-624)."""
+    gw2 = (np.abs(np.asarray(g) / np.asarray(w)) ** 2).reshape(n_modes, 1)
+    D = np.abs(_dense(M2_diag) - _dense(X2)) ** 2
+    if len(M2_off):
+        D = D + _off_super(M2_off, D.shape)
+    return gw2 * (D @ (np.abs(_C2(phi, n_state, n_hier)) ** 2)) / HBAR ** 2
+
+
+def error_flux_down_state_stable(phi, n_state, n_hier, n_modes, g, w, M2_diag, M2_off, X2,
+                                 F_diag=None, F_off=None):
+    """E[s, k] of the flux down, off-diagonal tables (filter F_off) then the
+    diagonal one minus <L> (filter F_diag) (basis_functions_adaptive.py:626-758)."""
+    gw2 = (np.abs(np.asarray(g) / np.asarray(w)) ** 2).reshape(n_modes, 1)
+    C2sq = np.abs(_C2(phi, n_state, n_hier)) ** 2
+    E = 0
+    Fo = 1 if F_off is None else F_off
+    Moff = _off_super(M2_off, np.shape(M2_diag))
+    if np.sum(Moff) != 0:
+        E = (Moff.T @ (Fo * gw2)) * C2sq / HBAR ** 2
+    Fd = 1 if F_diag is None else F_diag
+    D = np.abs(_dense(M2_diag) - _dense(X2)) ** 2
+    return E + (D.T @ (Fd * gw2)) * C2sq / HBAR ** 2
+
+
+def error_flux_down_by_dest_state(phi, n_state, n_hier, n_modes, g, w, M2_off_dest,
+                                  stable_idx=None, F_off=None, stable_states=None):
+    """Total off-diagonal flux down into each destination state
+    (basis_functions_adaptive.py:760-891)."""
+    gw2 = (np.abs(np.asarray(g) / np.asarray(w)) ** 2).reshape(n_modes, 1)
+    C2sq = np.abs(_C2(phi, n_state, n_hier)) ** 2
+    if stable_states is not None:
+        keep = np.zeros(n_state, bool)
+        keep[list(stable_states)] = True
+        C2sq = np.where(keep[:, None], C2sq, 0.0)
+    idx = list(range(n_hier)) if stable_idx is None else list(stable_idx)
+    Fo = 1 if F_off is None else np.asarray(F_off)[:, idx]
+    G = Fo * gw2
+    return np.array([np.sum((_abs2(M).T @ G) * C2sq[:, idx] / HBAR ** 2) for M in M2_off_dest])
+
+
+def error_sflux_stable_state(phi, n_state, n_hier, H2, stable_idx, states, T_phys=None, T_hier=None):
+    """Per basis state s: sum_d |H2[d, s]|^2 (d != s, over the full space) times
+    sum_k |Phi[s, k]|^2 over the stable auxiliaries, with the low-temperature
+    couplings split between k = 0 and k > 0 (basis_functions_adaptive.py:893-995)."""
+    C2 = _C2(phi, n_state, n_hier)[:, list(stable_idx)]
+    H2 = _dense(H2)
+    Hc = H2 - np.diag(np.diag(H2))
+    if T_phys is None:
+        V1 = np.sum(np.abs(Hc[:, list(states)]) ** 2, axis=0)
+        return V1 * np.sum(np.abs(C2) ** 2, axis=1) / HBAR ** 2
+    Tp, Th = _dense(T_phys), _dense(T_hier)
+    Vp = np.sum(np.abs((Hc + Tp - np.diag(np.diag(Tp)))[:, list(states)]) ** 2, axis=0)
+    Vh = np.sum(np.abs((Hc + Th - np.diag(np.diag(Th)))[:, list(states)]) ** 2, axis=0)
+    Caux = C2.copy()
+    Caux[:, 0] = 0
+    return (Vp * np.abs(C2[:, 0]) ** 2 + Vh * np.sum(np.abs(Caux) ** 2, axis=1)) / HBAR ** 2
+
+
+def error_sflux_boundary_state(phi, s0, sc, n_state, n_hier, H2, state_stable_idx, stable_idx,
+                               sc_dest, flux_updown, T_phys=None, T_hier=None):
+    """Flux from the stable states and auxiliaries into each state outside the
+    basis, plus the given flux up / down into some of them; returns the states
+    with nonzero flux and their errors (basis_functions_adaptive.py:997-1132)."""
+    H2 = _dense(H2)
+    if not len(state_stable_idx) < H2.shape[0]:
+        return [], []
+    C2 = _C2(phi, n_state, n_hier)[np.ix_(list(state_stable_idx), list(stable_idx))]
+    s0, sc = list(s0), list(sc)
+    if T_phys is None:
+        E = np.sum(np.abs(H2[np.ix_(sc, s0)] @ (C2 / HBAR)) ** 2, axis=1)
+    else:
+        E = (np.abs((H2 + _dense(T_phys))[np.ix_(sc, s0)] @ (C2[:, :1] / HBAR)) ** 2)[:, 0]
+        if C2.shape[1] > 1:
+            E = E + np.sum(np.abs((H2 + _dense(T_hier))[np.ix_(sc, s0)] @ (C2[:, 1:] / HBAR)) ** 2, axis=1)
+    E = np.array(E, dtype=float)
+    for i, f in zip(sc_dest, flux_updown):
+        E[i] += f
+    nz = np.nonzero(E)[0]
+    return np.array(sc, dtype=int)[nz], E[nz]
+
+
 class AdaptiveState:
     """The pieces of HopsBasis used by define_basis/update_basis for one
     trajectory, built on Basis (which is rebuilt for every new basis, the
@@ -756,24 +1019,76 @@ class AdaptiveState:
         self.f_discard = f_discard
         self.adaptive_h = delta_a > 0
         self.adaptive_s = delta_s > 0
-        for L in sysd["LIST_L2_COO"]:
-            if np.any(L.row != L.col):
-                raise NotImplementedError("adaptive basis with off-diagonal L-operators")
+        self.L_full = [L.toarray() for L in sysd["LIST_L2_COO"]]
+        self.off_diag = np.array([bool(np.any(L.row != L.col)) for L in sysd["LIST_L2_COO"]], dtype=bool)
 
     # -- per-basis arrays (hops_basis.py:1030-1151) --
+    def off_couplings(self, b):
+        """Any off-diagonal L-operator among the basis modes
+        (hops_basis.py:1223-1227)."""
+        return bool(np.any(self.off_diag[b.absL]))
+
+    def dest_states(self, b):
+        """States reachable by the L-operators of the basis states
+        (hops_system.py:218-226)."""
+        dest = set()
+        for s in b.state_list:
+            dest |= set(self.sysd["LIST_DESTINATION_STATES_BY_STATE_INDEX"][int(s)])
+        return sorted(dest)
+
     def tables(self, b):
-        nm, n, nh = len(b.modes), b.n_state, b.n_hier
-        M2 = np.zeros((nm, n), complex)            # L_m[s, s]
+        """M2_diag over the basis states (diagonal of each mode's reduced L) and,
+        with off-diagonal couplings, the per-destination tables."""
+        nm, n = len(b.modes), b.n_state
+        M2 = np.zeros((nm, n), complex)
         for i in range(nm):
             L = b.L_coo[b.relL_by_mode[i]]
             for r, c, d in zip(L.row, L.col, L.data):
                 if r == c:
                     M2[i, c] += d
-        psi = None
-        return M2
+        if not self.off_couplings(b):
+            return M2, [], []
+        dests = self.dest_states(b)
+        L_by_mode = [self.L_full[b.absL[b.relL_by_mode[i]]] for i in range(nm)]
+        _, off = mode_from_state_tables(L_by_mode, [int(s) for s in b.state_list], dests)
+        return M2, off, dests
 
     def expectations(self, b, psi):
         return np.array([expectation(L, psi) for L in b.L_csr])
+
+    def noise_operator(self, b, z_step):
+        """Full-space noise and memory operator of the off-diagonal L-operators,
+        added to -iH in the state-flux terms (hops_basis.py:914-953); None
+        without off-diagonal couplings."""
+        if not self.off_couplings(b):
+            return None
+        z1, z2, zm = z_step
+        comp = np.zeros(len(b.absL), dtype=np.complex128)
+        for i, m in enumerate(b.modes):
+            comp[b.relL_by_mode[i]] += zm[m]
+        coef = np.conj(np.asarray(z1)) - 1j * np.asarray(z2) + comp
+        N = self.sysd["NSTATES"]
+        Z = np.zeros((N, N), complex)
+        for j, l in enumerate(b.absL):
+            if self.off_diag[l]:
+                Z = Z + coef[j] * self.L_full[l]
+        return Z
+
+    def ltc_operators(self, b, avg):
+        """(T_phys, T_hier) of the off-diagonal L-operators with a
+        low-temperature coefficient, or (None, None) (hops_basis.py:955-998):
+        T_hier = sum conj(c) <L> L, T_phys = T_hier + T_hier^dagger - sum c L^2."""
+        lt = np.array(self.sysd["LIST_LT_PARAM"], dtype=np.complex128)[b.absL]
+        if not (np.any(lt) and self.off_couplings(b)):
+            return None, None
+        N = self.sysd["NSTATES"]
+        Th = np.zeros((N, N), complex)
+        Lsq = np.zeros((N, N), complex)
+        for j, l in enumerate(b.absL):
+            if self.off_diag[l]:
+                Th = Th + np.conj(lt[j]) * avg[j] * self.L_full[l]
+                Lsq = Lsq + lt[j] * (self.L_full[l] @ self.L_full[l])
+        return Th + Th.conj().T - Lsq, Th
 
     def K2_aux_by_mode(self, b):
         K = np.zeros((len(b.modes), b.n_hier), dtype=np.int64)
@@ -832,6 +1147,24 @@ class AdaptiveState:
             F = F & G
         return F
 
+    def state_filters(self, b, aux_set):
+        """(F_down, F_up) over [mode, aux] of the current basis: F_down marks
+        auxiliaries k with k - e_m in aux_set, F_up those with k + e_m in
+        aux_set (hops_fluxfilters.py:292-352)."""
+        nm, nh = len(b.modes), b.n_hier
+        F_dn = np.zeros((nm, nh), bool)
+        F_up = np.zeros((nm, nh), bool)
+        for a in aux_set:
+            for lm, m in enumerate(b.modes):
+                j = b.aux_index.get(aux_step(a, int(m), 1))
+                if j is not None:
+                    F_dn[lm, j] = True
+            for (m, v) in a:
+                j = b.aux_index.get(aux_step(a, m, -1))
+                if j is not None and m in b.rel_mode:
+                    F_up[b.rel_mode[m], j] = True
+        return F_dn, F_up
+
     def list_sc(self, b):
         """States outside the basis reachable by L or H (hops_system.py:191-204)."""
         st = set(int(s) for s in b.state_list)
@@ -843,46 +1176,30 @@ class AdaptiveState:
             coupled |= set(self.sysd["COUPLED_STATES"][s])
         return sorted((dest - st) | (coupled - st))
 
-    def error_deriv(self, dsys, b, Phi, z_step, dt, stable_idx=None):
-        """basis_functions_adaptive.py:10-74 (returns |.|^2 as [n, n_hier])."""
-        n, nh = b.n_state, b.n_hier
-        if stable_idx is not None:
-            P = np.zeros(n * nh, complex)
-            Pv = P.reshape(nh, n)
-            Pv[stable_idx] = Phi.reshape(nh, n)[stable_idx]
-        else:
-            P = Phi
-        d = dsys(P, z_step[2], z_step[0], z_step[1])[0] / HBAR
-        d = d + P / dt
-        D = d.reshape(nh, n).T
-        if stable_idx is not None:
-            D = D[:, stable_idx]
-        return np.abs(D) ** 2
+    def _H2(self, b, z_step):
+        H2 = -1j * self.sysd["H_DENSE"]
+        Z = self.noise_operator(b, z_step)
+        return H2 if Z is None else H2 + Z
 
     def hier_stable_error(self, dsys, b, Phi, dt, z_step):
         """hops_basis.py:560-665."""
         n, nh, nm = b.n_state, b.n_hier, len(b.modes)
-        C2 = Phi.reshape(nh, n).T          # [n, n_hier]
-        E1 = np.sum(self.error_deriv(dsys, b, Phi, z_step, dt), axis=0)
-        # state flux out of the basis through H (basis_functions_adaptive.py:76-163)
+        E1 = np.sum(error_deriv(dsys, Phi, z_step, n, nh, dt), axis=0)
         sc = self.list_sc(b)
+        avg = self.expectations(b, Phi[:n])
+        T_phys, T_hier = self.ltc_operators(b, avg)
         if len(sc):
-            Hsc = -1j * self.sysd["H_DENSE"][np.ix_(sc, b.state_list)]
-            E1 = E1 + np.sum(np.abs(Hsc @ C2 / HBAR) ** 2, axis=0)
-        M2 = self.tables(b)
+            E1 = E1 + error_sflux_hier(Phi, b.state_list, sc, n, nh, self._H2(b, z_step), T_phys, T_hier)
+        M2, M2_off, _ = self.tables(b)
         K2 = self.K2_aux_by_mode(b)
-        w = np.abs(b.w)
-        KW = w.reshape(nm, 1) * (1 + K2)
-        E_up = np.abs(KW) ** 2 * ((np.abs(M2) ** 2) @ (np.abs(C2) ** 2)) / HBAR ** 2
+        E_up = error_flux_up_hier_stable(Phi, n, nh, nm, b.w, K2, M2, M2_off)
         F_up = self.up_filter(b)
         for j, a in enumerate(b.aux_list):
             if _aux_depth(a) == self.maxhier:
                 F_up[:, j] = False
         E1 = E1 + np.sum(F_up * E_up, axis=0)
-        avg = self.expectations(b, Phi[:n])
-        X2 = np.repeat(avg[b.relL_by_mode].reshape(nm, 1), n, axis=1)
-        gw2 = np.square(np.abs(b.g / b.w)).reshape(nm, 1)
-        E_down = gw2 * ((np.abs(M2 - X2) ** 2) @ (np.abs(C2) ** 2)) / HBAR ** 2
+        X2 = expectation_table(avg[b.relL_by_mode], n)
+        E_down = error_flux_down_hier_stable(Phi, n, nh, nm, b.g, b.w, M2, M2_off, X2)
         F_down = K2 > 0
         E1 = E1 + np.sum(F_down * E_down, axis=0)
         return E1, [E_up, E_down]
@@ -928,72 +1245,78 @@ class AdaptiveState:
         thr = determine_error_thresh(np.sort(np.array(list(acc.values()))), dbound_sq, offset=discarded)
         return [k for k, v in acc.items() if v > thr]
 
-    def state_stable_error(self, dsys, b, Phi, dt, z_step, stable_idx, aux_bound):
+    def state_stable_error(self, dsys, b, Phi, dt, z_step, stable_idx, aux_bound, aux_new):
         """hops_basis.py:795-912."""
         n, nh, nm = b.n_state, b.n_hier, len(b.modes)
-        E1 = np.sum(self.error_deriv(dsys, b, Phi, z_step, dt, stable_idx), axis=1)
-        C2 = Phi.reshape(nh, n).T
-        # state flux through H couplings (basis_functions_adaptive.py:893-995)
-        H = -1j * self.sysd["H_DENSE"]
-        Hc = H - np.diag(np.diag(H))
-        V1 = np.sum(np.abs(Hc[:, b.state_list]) ** 2, axis=0)
-        C1 = np.sum(np.abs(C2[:, stable_idx]) ** 2, axis=1)
-        E1 = E1 + V1 * C1 / HBAR ** 2
-        M2 = self.tables(b)
+        E1 = np.sum(error_deriv(dsys, Phi, z_step, n, nh, dt, stable_idx), axis=1)
         avg = self.expectations(b, Phi[:n])
-        X2 = np.repeat(avg[b.relL_by_mode].reshape(nm, 1), n, axis=1)
-        # filters from the boundary auxiliaries (hops_fluxfilters.py:292-352)
-        F_dn = np.zeros((nm, nh), bool)
-        F_up = np.zeros((nm, nh), bool)
-        for a in aux_bound:
-            for lm, m in enumerate(b.modes):
-                j = b.aux_index.get(aux_step(a, int(m), 1))
-                if j is not None:
-                    F_dn[lm, j] = True
-            for (m, v) in a:
-                j = b.aux_index.get(aux_step(a, m, -1))
-                if j is not None:
-                    F_up[b.rel_mode[m], j] = True
-        gw2 = np.square(np.abs(b.g / b.w)).reshape(nm, 1)
-        E_down = ((np.abs(M2 - X2) ** 2).T @ (F_dn * gw2)) * np.abs(C2) ** 2 / HBAR ** 2
+        T_phys, T_hier = self.ltc_operators(b, avg)
+        E1 = E1 + error_sflux_stable_state(Phi, n, nh, self._H2(b, z_step), stable_idx,
+                                           [int(s) for s in b.state_list], T_phys, T_hier)
+        M2, M2_off, _ = self.tables(b)
+        X2 = expectation_table(avg[b.relL_by_mode], n)
+        F_dn, F_up = self.state_filters(b, aux_bound)
+        if len(M2_off):
+            F_dn_off, F_up_off = self.state_filters(b, aux_new)
+        else:
+            F_dn_off = F_up_off = None
+        E_down = error_flux_down_state_stable(Phi, n, nh, nm, b.g, b.w, M2, M2_off, X2, F_dn, F_dn_off)
         E1 = E1 + np.sum(E_down[:, stable_idx], axis=1)
         K2 = self.K2_aux_by_mode(b)
-        KW = F_up * (np.abs(b.w).reshape(nm, 1) * (1 + K2))
-        E_up = ((np.abs(M2) ** 2).T @ (np.abs(KW) ** 2)) * np.abs(C2) ** 2 / HBAR ** 2
+        E_up = error_flux_up_state_stable(Phi, n, nh, nm, b.w, K2, M2, M2_off, F_up, F_up_off)
         E1 = E1 + np.sum(E_up[:, stable_idx], axis=1)
         return E1
 
-    def state_basis(self, dsys, b, Phi, dt, z_step, stable_idx, aux_bound):
-        """hops_basis.py:294-482 (no off-diagonal couplings)."""
-        n, nh = b.n_state, b.n_hier
+    def state_basis(self, dsys, b, Phi, dt, z_step, stable_idx, aux_bound, aux_new):
+        """hops_basis.py:294-482."""
+        n, nh, nm = b.n_state, b.n_hier, len(b.modes)
         dsq = self.delta_s ** 2
-        err = self.state_stable_error(dsys, b, Phi, dt, z_step, stable_idx, aux_bound)
+        err = self.state_stable_error(dsys, b, Phi, dt, z_step, stable_idx, aux_bound, aux_new)
         rel_stable, st_stable = basis_from_list(err, dsq / 2.0, list(b.state_list))
         unstable = np.setdiff1d(np.arange(n), rel_stable)
         dbound = dsq - np.sum(err[unstable])
         sc = self.list_sc(b)
+        avg = self.expectations(b, Phi[:n])
+        T_phys, T_hier = self.ltc_operators(b, avg)
+        H2 = self._H2(b, z_step)
+        M2, M2_off, dests = self.tables(b)
+        sc_dest, flux = [], []
+        if len(M2_off):
+            M2_sc = []
+            for i, d in enumerate(dests):
+                if d in sc:
+                    sc_dest.append(sc.index(d))
+                    M2_sc.append(M2_off[i])
+            F_dn_off, F_up_off = self.state_filters(b, aux_new)
+            K2 = self.K2_aux_by_mode(b)
+            E_dn = error_flux_down_by_dest_state(Phi, n, nh, nm, b.g, b.w, M2_sc, stable_idx, F_dn_off, rel_stable)
+            E_up = error_flux_up_by_dest_state(Phi, n, nh, nm, b.w, K2, M2_sc, stable_idx, F_up_off, rel_stable)
+            flux = list(E_up + E_dn)
+        idx, E = error_sflux_boundary_state(Phi, [int(s) for s in st_stable], sc, n, nh, H2, rel_stable,
+                                            stable_idx, sc_dest, flux, T_phys, T_hier)
         bound = []
-        if len(rel_stable) < self.sysd["NSTATES"] and len(sc):
-            C2 = Phi.reshape(nh, n).T[np.ix_(rel_stable, stable_idx)]
-            Hsc = -1j * self.sysd["H_DENSE"][np.ix_(sc, st_stable)]
-            E = np.sum(np.abs(Hsc @ (C2 / HBAR)) ** 2, axis=1)
-            nz = np.nonzero(E)[0]
-            if len(nz):
-                _, bound = basis_from_list(E[nz], dbound, list(np.array(sc)[nz]))
+        if len(E):
+            _, bound = basis_from_list(np.asarray(E), dbound, [int(s) for s in idx])
         bound = sorted(set(bound) - set(int(s) for s in b.state_list))
         return [int(s) for s in st_stable], bound
+
+    def hierarchy_basis(self, dsys, b, Pn, dt, z_step):
+        """Stable and boundary auxiliaries of a normalised Pn
+        (hops_basis.py:484-558)."""
+        dsq = self.delta_a ** 2
+        err, E_updown = self.hier_stable_error(dsys, b, Pn, dt, z_step)
+        idx_stable, aux_stable = basis_from_list(err, dsq / 2.0, b.aux_list)
+        unstable = np.setdiff1d(np.arange(b.n_hier), idx_stable)
+        dbound = dsq - np.sum(err[unstable])
+        aux_bound = self.boundary_hier(b, E_updown, idx_stable, dbound)
+        return aux_stable, aux_bound
 
     def define_basis(self, dsys, b, Phi, dt, z_step):
         """hops_basis.py:117-182."""
         n = b.n_state
         Pn = Phi / np.linalg.norm(Phi[:n])
         if self.adaptive_h:
-            dsq = self.delta_a ** 2
-            err, E_updown = self.hier_stable_error(dsys, b, Pn, dt, z_step)
-            idx_stable, aux_stable = basis_from_list(err, dsq / 2.0, b.aux_list)
-            unstable = np.setdiff1d(np.arange(b.n_hier), idx_stable)
-            dbound = dsq - np.sum(err[unstable])
-            aux_bound = self.boundary_hier(b, E_updown, idx_stable, dbound)
+            aux_stable, aux_bound = self.hierarchy_basis(dsys, b, Pn, dt, z_step)
             aux_new = list(set(aux_stable) | set(aux_bound))
             stable_idx = sorted(b.aux_index[a] for a in aux_stable)
         else:
@@ -1001,7 +1324,7 @@ class AdaptiveState:
             aux_bound = []
             stable_idx = list(range(b.n_hier))
         if self.adaptive_s:
-            st, sb = self.state_basis(dsys, b, Pn, dt, z_step, stable_idx, aux_bound)
+            st, sb = self.state_basis(dsys, b, Pn, dt, z_step, stable_idx, aux_bound, aux_new)
             states_new = sorted(set(st) | set(sb))
         else:
             states_new = [int(s) for s in b.state_list]

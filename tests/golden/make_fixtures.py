@@ -40,5 +40,8 @@ out["pre_calculated_uncorrelated_noise_test"] = load("pre_calculated_uncorrelate
 out["pre_calculated_correlated_noise_test"] = load("pre_calculated_correlated_noise_test.npy")
 out["known_param_g"] = load("known_param_g.npy")
 out["known_param_w"] = load("known_param_w.npy")
+# raw (uncorrelated) noise rows that the reference's basis tests give as SEED
+out["hops_basis_noise_2site"] = load("hops_basis_noise_2site.npy")
+out["hops_basis_noise_10site"] = load("hops_basis_noise_10site.npy")
 np.savez_compressed(OUT, **out)
 print(OUT, os.path.getsize(OUT))

@@ -453,3 +453,80 @@ def test_interpolated_noise_reproduces_cubics():
     want = np.array([(0.3 + 0.1j * l) * np.array(tq) ** 3 - (1.0 + l) * np.array(tq) + 2.0j for l in range(n_l2)])
     np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11)
     np.testing.assert_allclose(nz.get_noise([1.5, 2.0], [0]), rows[[0]][:, [3, 4]], rtol=1e-12, atol=1e-12)
+
+
+def _peierls_chain(ltc):
+    """5 sites where H couples only 0-1 and 3-4: state 2 is reached only
+    through the off-diagonal (Peierls) L-operators of bonds 1-2 and 2-3, next
+    to a site projector (Holstein) on every site."""
+    n = 5
+    H = np.diag([0.0, 100.0, -50.0, 80.0, 20.0]).astype(complex)
+    H[0, 1] = H[1, 0] = 40.0
+    H[3, 4] = H[4, 3] = 40.0
+    g0, w0 = O.bcf_convert_dl_to_exp(40.0, 50.0, 295.0)
+    gw, ll = [], []
+    for i in range(n):
+        ll.append(sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)))
+        gw.append([g0, w0])
+    for i in (1, 2):
+        L = np.zeros((n, n), complex)
+        L[i, i + 1] = L[i + 1, i] = 1.0
+        ll.append(L)
+        gw.append([g0 / 4, w0])
+    p = dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+    if ltc:
+        p["L_LT_CORR"] = ll
+        p["PARAM_LT_CORR"] = [3.0 + 0.5j] * n + [1.0, 1.0]
+    return p
+
+
+@pytest.mark.parametrize("ltc", [False, True], ids=["plain", "ltc"])
+def test_adaptive_offdiagonal_delta_to_zero_equals_full(ltc):
+    """Off-diagonal L-operators: with vanishing thresholds the adaptive
+    trajectory (its state basis grown only through the Peierls fluxes and
+    noise) equals the plain one."""
+    sysp = _peierls_chain(ltc)
+    noise = dict(SEED=4, MODEL="FFT_FILTER", TLEN=200.0, TAU=1.0)
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(5, complex)
+    psi0[0] = 1
+    ref = O.Trajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2})
+    ref.initialize(psi0)
+    ref.propagate(60.0, 2.0)
+    tr = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 2}, delta_a=1e-15, delta_s=1e-100)
+    tr.initialize(psi0)
+    tr.propagate(60.0, 2.0)
+    assert 2 in tr.state_traj[-1] and 2 not in tr.state_traj[0]
+    assert np.max(np.abs(np.array(tr.psi_traj) - np.array(ref.psi_traj))) < 1e-12
+
+
+@pytest.mark.parametrize("mode", ["hier", "state"])
+@pytest.mark.parametrize("ltc", [False, True], ids=["plain", "ltc"])
+def test_adaptive_offdiagonal_derivative_error_below_delta(mode, ltc):
+    """|D_full(phi_adaptive) - D_adaptive(phi_adaptive)| <= delta at every step
+    for off-diagonal L-operators (the bound tests/test_adap_basis.py:110-361
+    checks for site projectors): a missing or underestimated flux term of the
+    Peierls couplings lets the truncation error exceed delta."""
+    sysp = _peierls_chain(ltc)
+    da, ds = (3e-3, 0) if mode == "hier" else (0, 3e-3)
+    maxhier = 3
+    noise = dict(SEED=7, MODEL="FFT_FILTER", TLEN=200.0, TAU=1.0)
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(5, complex)
+    psi0[0] = 1.0
+    full = O.Basis(O.derive_system(sysp), np.arange(5), O.triangular_hierarchy(7, maxhier))
+    f_full = O.make_derivative(full, "NORMALIZED NONLINEAR")
+    tr = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": maxhier}, delta_a=da, delta_s=ds)
+    tr.initialize(psi0)
+    errs = []
+    for t in np.arange(0, 80.0, 2.0):
+        tr.propagate(2.0, 2.0)
+        b = tr.basis
+        # the basis was chosen with the noise at the step's start
+        z1 = tr.noise1.get_noise([t], list(b.absL))[:, 0]
+        d_ad = tr.dsystem_dt(tr.phi, tr.z_mem, z1, np.zeros(len(b.absL)))[0] / O.HBAR
+        z1f = tr.noise1.get_noise([t], list(full.absL))[:, 0]
+        d_full = f_full(_embed(b, full, tr.phi), tr.z_mem, z1f, np.zeros(len(full.absL)))[0] / O.HBAR
+        errs.append(np.linalg.norm(d_full - _embed(b, full, d_ad)))
+    assert b.n_hier < full.n_hier or b.n_state < full.n_state
+    assert max(errs) <= max(da, ds), max(errs)
