@@ -182,6 +182,27 @@ int hops_build_super(const hops_op_t *op, int precision, void *stream);
 int hops_propagate(const hops_op_t *op, const hops_batch_t *batch, int it0,
                    int n_steps, int store0, void *stream);
 
+/* Largest scratch hops_propagate_noalloc uses: the coefficient tables of
+   the stages of a fused LINEAR prologue, 3 * 16 * coefT_stride *
+   (2 n_pat + n_mode + 1) + 3 * 8 * B * coef_stride bytes in FP64
+   (complex64 tables in mixed precision take half of the first term). */
+#define HOPS_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* hops_propagate without any device allocation, for callers that cannot
+   allocate (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  It has no counterpart in the reference.  Same
+   arguments, kernels and results as hops_propagate, except that:
+   - the stage-coefficient scratch is a library-owned device array of
+     HOPS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a
+     batch that needs more fails with an error;
+   - steps are launched directly: no HIP graph is built and no profiler
+     event is recorded.
+   Every call on one GPU shares that GPU's array: calls that overlap on one
+   GPU (on different streams or host threads) race.  Calls on one stream
+   are ordered and safe. */
+int hops_propagate_noalloc(const hops_op_t *op, const hops_batch_t *batch, int it0,
+                           int n_steps, int store0, void *stream);
+
 /* hbar dY/dt for every trajectory of the batch at the input pool y_d
    (complex128, same layout as phi_d), with the batch's current z-memory and
    the noise at index tidx; output d_d (complex128, same layout).  FP64

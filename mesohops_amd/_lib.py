@@ -87,6 +87,7 @@ def lib():
     L.hops_build_super.argtypes = [ctypes.POINTER(HopsOp), ctypes.c_int, _vp]
     L.hops_propagate.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, _vp]
+    L.hops_propagate_noalloc.argtypes = L.hops_propagate.argtypes
     L.hops_derivative.argtypes = [ctypes.POINTER(HopsOp), ctypes.POINTER(HopsBatch), _vp, _vp, ctypes.c_int, _vp]
     L.hops_permute_c128.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp]
     L.hops_apply_state_op.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
@@ -152,7 +153,7 @@ def check(rc):
 
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",
-            "hops_build_super", "hops_propagate", "hops_derivative", "hops_permute_c128", "hops_apply_state_op",
+            "hops_build_super", "hops_propagate", "hops_propagate_noalloc", "hops_derivative", "hops_permute_c128", "hops_apply_state_op",
             "hops_noise_embedding", "hops_noise_filter", "hops_noise_filter_c128", "hops_noise_spline", "hops_noise_window_mean",
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",

@@ -1335,8 +1335,31 @@ int retire_graph(hipGraphExec_t ex, hipStream_t st) {
   return 0;
 }
 
-int prof_mark(hipStream_t st, int kind, size_t *used) {
-  if (!g_prof.on) return 0;
+// Scratch of the allocation-free entry point hops_propagate_noalloc: the
+// coefficient tables of the fused LINEAR prologue.  One copy per device, held
+// until the process ends; host code uses only the per-device address taken
+// with hipGetSymbolAddress, never the array's name.
+constexpr size_t kNoAllocScratchBytes = HOPS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(void **out) {
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device %d outside [0, %d)", dev, kMaxDevices);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int prof_mark(bool on, hipStream_t st, int kind, size_t *used) {
+  if (!on) return 0;
   if (*used >= g_prof.ev.size()) {
     hipEvent_t e;
     HIP_TRY(hipEventCreate(&e));
@@ -1369,9 +1392,14 @@ int check_op(const hops_op_t *op) {
   return 0;
 }
 
+// ext_scr: caller-provided scratch of kNoAllocScratchBytes (the
+// allocation-free form: no stream-ordered allocation, no graph capture, no
+// profiler events); nullptr: scratch from hipMallocAsync
 template <typename R>
 int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
-                hipStream_t st) {
+                hipStream_t st, void *ext_scr) {
+  const bool noalloc = ext_scr != nullptr;
+  const bool prof = g_prof.on && !noalloc;
   const int n = op->n_state;
   const long nr = (long)n * op->n_hier;
   const size_t shm = sizeof(double) * (2 * (size_t)n + 8 * (size_t)op->n_act + 2 * (size_t)op->n_mode_abs + 16);
@@ -1386,6 +1414,15 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const int ns = bt->noise_step;
   if (it0 < 0 || it0 + (long)n_steps * 2 * ns >= bt->n_t)
     return fail("noise index range [%d, %ld] outside [0, %d)", it0, it0 + (long)n_steps * 2 * ns, bt->n_t);
+  // scratch of the fused LINEAR prologue: the coefficient tables of stages 1..3
+  const bool fuse = g_fuse_linear && bt->eom == HOPS_EOM_LINEAR;
+  const size_t coefT_blk = sizeof(cx<R>) * (size_t)bt->coefT_stride *
+                           (2 * (size_t)op->n_pat + (size_t)op->n_mode + 1);
+  const size_t coef_blk = (size_t)bt->B * (size_t)bt->coef_stride;
+  const size_t fuse_bytes = fuse && n_steps > 0 ? 3 * coefT_blk + 3 * coef_blk * sizeof(double) : 0;
+  if (noalloc && fuse_bytes > kNoAllocScratchBytes)
+    return fail("hops_propagate_noalloc needs %zu B of scratch for this batch, more than its %zu B", fuse_bytes,
+                kNoAllocScratchBytes);
   const int G = (bt->B + (1 << bt->lane_shift) - 1) >> bt->lane_shift;
   cx<R> *Y0 = reinterpret_cast<cx<R> *>(bt->precision == HOPS_PREC_MIXED ? bt->y0_d : bt->phi_d);
   cx<R> *ya = reinterpret_cast<cx<R> *>(bt->ya_d);
@@ -1399,10 +1436,10 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const int toff[4] = {0, ns, ns, 2 * ns};
   if (n_steps > 0 && nr <= g_resident_rows) {
     size_t used = 0;
-    if (int rc = prof_mark(st, 2, &used)) return rc;
+    if (int rc = prof_mark(prof, st, 2, &used)) return rc;
     k_steps_resident<R><<<bt->B, 256, shm, st>>>(*op, *bt, it0, n_steps, store0, Y0, ya, yb);
     HIP_TRY(hipGetLastError());
-    if (int rc = prof_mark(st, 1, &used)) return rc;
+    if (int rc = prof_mark(prof, st, 1, &used)) return rc;
     if (int rc = prof_collect(used)) return rc;
     const int slot = bt->psi_store_d ? store0 + n_steps - 1 : -1;
     k_final_sigma<<<bt->B, 256, 0, st>>>(*op, *bt, slot);
@@ -1456,22 +1493,18 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     chunk = bt->B;  // chunks need the stationary kernel (no tile list per chunk)
   }
   // fused LINEAR prologue: stream-ordered scratch for the coefficient
-  // tables of stages 1..3, freed on the same stream when the call ends
-  const bool fuse = g_fuse_linear && bt->eom == HOPS_EOM_LINEAR;
-  const size_t coefT_blk = sizeof(cx<R>) * (size_t)bt->coefT_stride *
-                           (2 * (size_t)op->n_pat + (size_t)op->n_mode + 1);
-  const size_t coef_blk = (size_t)bt->B * (size_t)bt->coef_stride;
+  // tables of stages 1..3, freed on the same stream when the call ends (or
+  // the caller's scratch of the allocation-free form)
   // graph replay: step 0 eagerly, then steps 1 .. in launches of K captured
   // steps whose prologues add the device step counter (zeroed below)
   const int K = g_graph_steps;
-  bool graph = K > 0 && !g_prof.on && chunk >= bt->B && n_steps - 1 >= K;
+  bool graph = !noalloc && K > 0 && !prof && chunk >= bt->B && n_steps - 1 >= K;
   if (graph) {
     // a caller that captures this call into a graph of its own gets eager
     // launches (no capture inside a capture)
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     graph = hipStreamIsCapturing(st, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
   }
-  const size_t fuse_bytes = fuse && n_steps > 0 ? 3 * coefT_blk + 3 * coef_blk * sizeof(double) : 0;
   const size_t ctr_off = (fuse_bytes + 255) & ~(size_t)255;
   const size_t scr_bytes = graph ? ctr_off + 256 : fuse_bytes;
   struct StreamScratch {
@@ -1482,19 +1515,25 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     }
   } scr;
   scr.s = st;
+  void *scr_base = nullptr;  // no graph in the allocation-free form: scr_bytes == fuse_bytes there
   if (scr_bytes > 0) {
-    HIP_TRY(hipMallocAsync(&scr.p, scr_bytes, st));
+    if (noalloc) {
+      scr_base = ext_scr;
+    } else {
+      HIP_TRY(hipMallocAsync(&scr.p, scr_bytes, st));
+      scr_base = scr.p;
+    }
     // cleared by a kernel, not a memset node, should the caller capture
     const long words = (long)(scr_bytes / 8);
     k_zero_words<<<(unsigned)std::min<long>((words + 255) / 256, 4096), 256, 0, st>>>(
-        reinterpret_cast<unsigned long long *>(scr.p), words);
+        reinterpret_cast<unsigned long long *>(scr_base), words);
     HIP_TRY(hipGetLastError());
   }
-  char *coefT_s = reinterpret_cast<char *>(scr.p);
+  char *coefT_s = reinterpret_cast<char *>(scr_base);
   double *coef_s = reinterpret_cast<double *>(coefT_s + 3 * coefT_blk);
   int *step_ctr = graph ? reinterpret_cast<int *>(coefT_s + ctr_off) : nullptr;
   size_t used = 0;
-  if (int rc = prof_mark(st, 2, &used)) return rc;
+  if (int rc = prof_mark(prof, st, 2, &used)) return rc;
   // one RK4 step of a (sub-)batch on stream ls; slot: psi store slot written
   // at stage 0; sc: step counter of a captured graph (nullptr: eager)
   auto one_step = [&](const hops_batch_t &sb, int step, hipStream_t ls, const int *sc) -> int {
@@ -1508,7 +1547,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       k_prologue_linear4<R><<<dim3(sb.B, 4), 256, shm, ls>>>(*op, sb, itb, ns, yin0, slot, coefT_s + offT, coefT_blk,
                                                               coef_s + offC, coef_blk, sc);
       HIP_TRY(hipGetLastError());
-      if (int rc = prof_mark(ls, 0, &used)) return rc;
+      if (int rc = prof_mark(prof, ls, 0, &used)) return rc;
     }
     for (int i = 0; i < 4; ++i) {
       const cx<R> *yin = i == 0 ? reinterpret_cast<const cx<R> *>(sb.precision == HOPS_PREC_MIXED ? sb.y0_d : sb.phi_d)
@@ -1521,7 +1560,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       } else {
         k_prologue<R><<<sb.B, 256, shm, ls>>>(*op, sb, i, itb + toff[i], yin, slot, sc);
         HIP_TRY(hipGetLastError());
-        if (int rc = prof_mark(ls, 0, &used)) return rc;
+        if (int rc = prof_mark(prof, ls, 0, &used)) return rc;
       }
 #define HOPS_SHV(W, P, NT)                                                                       \
   do {                                                                                           \
@@ -1551,7 +1590,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
 #undef HOPS_SH
 #undef HOPS_SHV
       HIP_TRY(hipGetLastError());
-      if (int rc = prof_mark(ls, 1, &used)) return rc;
+      if (int rc = prof_mark(prof, ls, 1, &used)) return rc;
     }
     return 0;
   };
@@ -1899,24 +1938,25 @@ int slots_step_t(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *l
   if (shm > 64 * 1024)
     HIP_TRY(hipFuncSetAttribute((const void *)k_prologue_slots<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
+  const bool prof = g_prof.on;
   size_t used = 0;
-  if (int rc = prof_mark(st, 2, &used)) return rc;
+  if (int rc = prof_mark(prof, st, 2, &used)) return rc;
   if (mixed) {
     k_prime_slots<<<n_tiles, 256, 0, st>>>(ops, bts, tiles);
     HIP_TRY(hipGetLastError());
-    if (int rc = prof_mark(st, 2, &used)) return rc;
+    if (int rc = prof_mark(prof, st, 2, &used)) return rc;
   }
   for (int i = 0; i < 4; ++i) {
     k_prologue_slots<R><<<n_list, 256, shm, st>>>(ops, bts, list, it, i);
     HIP_TRY(hipGetLastError());
-    if (int rc = prof_mark(st, 0, &used)) return rc;
+    if (int rc = prof_mark(prof, st, 0, &used)) return rc;
     k_stage_slots<R><<<n_tiles, 256, 0, st>>>(ops, bts, tiles, i);
     HIP_TRY(hipGetLastError());
-    if (int rc = prof_mark(st, 1, &used)) return rc;
+    if (int rc = prof_mark(prof, st, 1, &used)) return rc;
   }
   k_final_slots<<<n_list, 256, 0, st>>>(ops, bts, list);
   HIP_TRY(hipGetLastError());
-  if (int rc = prof_mark(st, 2, &used)) return rc;
+  if (int rc = prof_mark(prof, st, 2, &used)) return rc;
   return prof_collect(used);
 }
 
@@ -2086,8 +2126,8 @@ int hops_apply_state_op(const void *phi_in_d, void *phi_out_d, const void *op_d,
   return 0;
 }
 
-int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
-                   void *stream) {
+static int propagate_checked(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
+                             void *stream, void *ext_scr) {
   if (int rc = check_op(op)) return rc;
   if (!bt) return fail("batch is NULL");
   if (bt->B <= 0 || bt->tile_rows != 256 || bt->n_tiles <= 0) return fail("bad batch geometry");
@@ -2102,8 +2142,8 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_s
     return fail("unknown precision %d", bt->precision);
   hipStream_t st = (hipStream_t)stream;
   auto run = [&](const hops_batch_t *b, int i0, int n, int s0) {
-    return b->precision == HOPS_PREC_FP64 ? propagate_t<double>(op, b, i0, n, s0, st)
-                                          : propagate_t<float>(op, b, i0, n, s0, st);
+    return b->precision == HOPS_PREC_FP64 ? propagate_t<double>(op, b, i0, n, s0, st, ext_scr)
+                                          : propagate_t<float>(op, b, i0, n, s0, st, ext_scr);
   };
   if (bt->sigma_raw && n_steps > 1) {
     // the raw first step alone, then the rest normalised as usual
@@ -2113,6 +2153,18 @@ int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_s
     return run(&rest, it0 + 2 * bt->noise_step, n_steps - 1, store0 + 1);
   }
   return run(bt, it0, n_steps, store0);
+}
+
+int hops_propagate(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
+                   void *stream) {
+  return propagate_checked(op, bt, it0, n_steps, store0, stream, nullptr);
+}
+
+int hops_propagate_noalloc(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_steps, int store0,
+                           void *stream) {
+  void *scr = nullptr;
+  if (int rc = noalloc_scratch(&scr)) return rc;
+  return propagate_checked(op, bt, it0, n_steps, store0, stream, scr);
 }
 
 

@@ -277,8 +277,10 @@ class BatchState:
         a = self.psi_store.cpu().numpy().reshape(self.B, self.store_slots, n)
         return a[:, :n_slots]
 
-    def propagate(self, it0: int, n_steps: int, tau: float, noise_step: int, store0: int = 0):
-        """n_steps RK4 steps of size tau from noise index it0."""
+    def propagate(self, it0: int, n_steps: int, tau: float, noise_step: int, store0: int = 0,
+                  noalloc: bool = False):
+        """n_steps RK4 steps of size tau from noise index it0.  noalloc: through
+        hops_propagate_noalloc, which allocates no device memory."""
         self.bt.dt_over_hbar = float(tau / hbar)
         self.bt.noise_step = int(noise_step)
         if self.bt.psi_store_d and store0 + n_steps > self.store_slots:
@@ -288,8 +290,9 @@ class BatchState:
         self.bt.sigma_raw = 1 if self.raw_first else 0
         try:
             with torch.cuda.device(self.device):
-                check(lib().hops_propagate(ctypes.byref(self.dop.op), ctypes.byref(self.bt), int(it0),
-                                           int(n_steps), int(store0), _stream_ptr(self.device)))
+                fn = lib().hops_propagate_noalloc if noalloc else lib().hops_propagate
+                check(fn(ctypes.byref(self.dop.op), ctypes.byref(self.bt), int(it0), int(n_steps), int(store0),
+                         _stream_ptr(self.device)))
         finally:
             self.bt.sigma_raw = 0
             if n_steps > 0:
