@@ -78,10 +78,9 @@ def psi0():
     return p
 
 
-def _cpu_worker(args):
-    """One trajectory of the workload on the FP64 oracle (1 thread)."""
-    seed, steps = args
-    os.environ["OMP_NUM_THREADS"] = "1"
+def _cpu_worker(seed, steps):
+    """One trajectory of the workload on the FP64 oracle (1 thread); returns
+    the seconds its propagation took."""
     from oracle import hops_oracle as O
     sysp = c2_system()
     sysp["ALPHA_NOISE1"] = O.bcf_exp
@@ -94,17 +93,25 @@ def _cpu_worker(args):
 
 
 def cpu_baseline(steps_per_proc, procs):
-    import multiprocessing as mp
-    ctx = mp.get_context("spawn")
+    """procs child processes (this script with --cpu-worker), one trajectory
+    each with single-threaded BLAS, all waited for before returning."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
     t0 = time.perf_counter()
-    with ctx.Pool(procs) as pool:
-        per = pool.map(_cpu_worker, [(s, steps_per_proc) for s in range(procs)])
+    kids = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(s), str(steps_per_proc)],
+                             stdout=subprocess.PIPE, env=env, text=True) for s in range(procs)]
+    per = []
+    for k in kids:
+        out, _ = k.communicate()
+        if k.returncode != 0:
+            raise RuntimeError("CPU baseline worker failed (%d)" % k.returncode)
+        per.append(float(out.strip().split()[-1]))
     wall = time.perf_counter() - t0
     # throughput over the propagation parts, all processes in parallel
     value = procs * steps_per_proc / max(per)
     return {"value": value, "unit": "traj·RK4-steps/s", "cores": procs, "kind": "port",
             "sample": "%d trajectories x %d RK4 steps of the workload on the FP64 NumPy/SciPy oracle, one "
-                      "process per core (OMP_NUM_THREADS=1); slowest process %.1f s, pool wall %.1f s"
+                      "process per core (OMP_NUM_THREADS=1); slowest process %.1f s, wall %.1f s"
                       % (procs, steps_per_proc, max(per), wall)}
 
 
@@ -201,10 +208,23 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
         win.append((c["t_fs"] - prev, c["s"]))
         prev = c["t_fs"]
     batch_win = B * sum(w[0] for w in win) / sum(w[1] for w in win)
+    # one trajectory alone in the same native runtime, FP64, every step
+    if log:
+        log("%s: one trajectory in the same runtime, FP64, %d steps" % (kind, steps))
+    one = make(1, "fp64")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for t_done in range(0, steps, chunk):
+        one.propagate(float(min(chunk, steps - t_done)), 1.0)
+    torch.cuda.synchronize()
+    one_rate = steps / (time.perf_counter() - t1)
+    one.rt.close()
     out = {"value": B * steps / elapsed, "trajectories": B, "steps": steps, "elapsed_s": elapsed, "init_s": t_init,
            "ms_per_step": elapsed / steps * 1e3, "chunks": chunks,
            "gpu_fp64_single_traj_steps_per_s": single_rate, "single_window_steps": single_steps,
            "batch_rate_same_window": batch_win, "speedup_vs_gpu_fp64_single_same_window": batch_win / single_rate,
+           "same_runtime_fp64_b1_steps_per_s": one_rate,
+           "speedup_vs_same_runtime_fp64_b1": (B * steps / elapsed) / one_rate,
            "host_phase_s": {k: v for k, v in timers.items() if k != "rows_stepped"},
            "stage_kernel": {"avg_ms": ms[1] / max(n[1], 1), "launches": n[1],
                             "algorithmic_bytes": stage_bytes, "achieved_GBs": achieved,
@@ -278,13 +298,17 @@ def main_c4(args):
 
 
 def main():
+    if len(sys.argv) == 4 and sys.argv[1] == "--cpu-worker":
+        print(_cpu_worker(int(sys.argv[2]), int(sys.argv[3])), flush=True)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--ref-traj", type=int, default=8, help="trajectories re-run in FP64 for the population error")
+    ap.add_argument("--ref-traj", type=int, default=32,
+                    help="adaptive workloads: trajectories re-run in FP64 for the population error")
     ap.add_argument("--single-steps", type=int, default=200, help="steps of the plain B=1 FP64 GPU path")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
@@ -299,7 +323,7 @@ def main():
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
     ap.add_argument("--requests", type=int, default=64, help="c4: requests in the stream")
-    ap.add_argument("--c3-steps", type=int, default=40, help="steps of the adaptive summary in the c2 run (0: off)")
+    ap.add_argument("--c3-steps", type=int, default=500, help="steps of the adaptive leg in the c2 run (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (trajectories would shard "
@@ -315,6 +339,14 @@ def main():
         return main_c4(args)
     if (args.steps + 1) * DT > NOISE["TLEN"] or (args.warmup + 1) * DT > NOISE["TLEN"]:
         sys.exit("steps x dt exceeds the noise length %.0f fs" % NOISE["TLEN"])
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    cpu = None
+    if not args.no_cpu:
+        # child processes started before this process touches the GPU
+        procs = min(16, os.cpu_count() or 1)
+        log("cpu baseline: %d processes x %d steps of the oracle" % (procs, args.cpu_steps))
+        cpu = cpu_baseline(args.cpu_steps, procs)
+        log("cpu baseline: %.1f traj*steps/s" % cpu["value"])
 
     import torch
     from mesohops_amd import HopsBatch, _lib
@@ -340,13 +372,17 @@ def main():
         autotune.apply(choice, batch)
     st = batch.state
 
+    def run_on(state, steps):
+        """steps RK4 steps from t = 0, psi stored on the device (no download)."""
+        if state.psi_store is None or state.store_slots < steps:
+            state.store_slots = steps
+            state.psi_store = torch.zeros(B * steps * tb.n_state, dtype=torch.complex128, device=dev)
+            state.bt.store_stride = steps
+        state.bt.psi_store_d = state.psi_store.data_ptr()
+        state.propagate(0, steps, DT, ns, 0)
+
     def run(steps):
-        if st.psi_store is None or st.store_slots < steps:
-            st.store_slots = steps
-            st.psi_store = torch.zeros(B * steps * tb.n_state, dtype=torch.complex128, device=dev)
-            st.bt.store_stride = steps
-        st.bt.psi_store_d = st.psi_store.data_ptr()
-        st.propagate(0, steps, DT, ns, 0)
+        run_on(st, steps)
 
     # warmup, then rewind to t = 0 with the psi store sized for the timed region
     run(args.warmup)
@@ -358,32 +394,54 @@ def main():
     st.bt.store_stride = args.steps
     torch.cuda.synchronize()
 
-    _lib.profile_enable(True)
+    # timed region: the default dispatch (graph-replayed steps), no profiler
+    log("c2: timed region, %d steps" % args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     run(args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms, n = _lib.profile_read()
-    _lib.profile_enable(False)
     value = B * args.steps / elapsed
     psi_mixed = st.psi_store_host(args.steps)
+    # the same steps again with a HIP event after every kernel (launched
+    # eagerly): per-kernel times for the roofline
+    batch.rewind(psi0())
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    elapsed_prof = time.perf_counter() - t0
+    ms, n = _lib.profile_read()
+    _lib.profile_enable(False)
+    prof_equal = bool(np.array_equal(st.psi_store_host(args.steps), psi_mixed))
 
-    # FP64 reference run on identical noise for the population error
-    nref = min(args.ref_traj, B)
-    ref = HopsBatch(sysp, EOM, dict(NOISE, SEED=0), None, HIER, seeds=list(range(nref)),
+    # the same batch with complex128 stages: its rate, and the FP64 reference
+    # of the population error on every trajectory (identical noise)
+    log("c2: complex128 leg, %d trajectories x %d steps" % (B, args.steps))
+    ref = HopsBatch(sysp, EOM, dict(NOISE, SEED=0), None, HIER, seeds=list(range(B)),
                     precision_mode="fp64", device="cuda:0", store_psi=True)
     ref.initialize(psi0())
-    ref.propagate(args.steps * DT, DT)
-    pref = np.array([np.array(ref.psi_traj[b][1:]) for b in range(nref)])
-    pm = np.abs(psi_mixed[:nref]) ** 2
+    if ref.state.lanes != st.lanes:
+        ref.relayout(st.lanes)
+    run_on(ref.state, 2)   # warm-up, then rewind
+    ref.rewind(psi0())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_on(ref.state, args.steps)
+    torch.cuda.synchronize()
+    c128_elapsed = time.perf_counter() - t0
+    pref = ref.state.psi_store_host(args.steps)
+    pm = np.abs(psi_mixed) ** 2
     pr = np.abs(pref) ** 2
-    # LINEAR trajectories are not normalised (their norm wanders by orders of
-    # magnitude); the population error is taken on |psi_n|^2 / |psi|^2
-    pop_err = float(np.max(np.abs(pm / pm.sum(-1, keepdims=True) - pr / pr.sum(-1, keepdims=True))))
     pop_err_abs = float(np.max(np.abs(pm - pr)))
+    # LINEAR trajectories are not normalised: also the error of the
+    # normalised populations |psi_n|^2 / |psi|^2
+    pop_err_norm = float(np.max(np.abs(pm / pm.sum(-1, keepdims=True) - pr / pr.sum(-1, keepdims=True))))
     max_pop = float(np.max(pr))
-    del ref
+    nref = B
+    del ref, pref
+    torch.cuda.empty_cache()
 
     # plain per-trajectory FP64 GPU path (B = 1, no batching, every kernel
     # launched eagerly)
@@ -448,12 +506,16 @@ def main():
                                "same workload in complex128",
                    "trajectories": B, "n_state": tb.n_state, "n_hier": tb.n_hier, "dt_fs": DT,
                    "precision": args.precision},
-        "pop_err_vs_fp64": pop_err,
-        "pop_err_definition": "max over t, n, trajectory of | |psi_n|^2/|psi|^2 (this run) - same (FP64 GPU run, "
-                              "identical noise) |",
-        "pop_err_abs_vs_fp64": pop_err_abs,
+        "pop_err_vs_fp64": pop_err_abs,
+        "pop_err_definition": "max over t, n, trajectory of | |psi_n|^2 (this run) - |psi_n|^2 (complex128 run of "
+                              "the same batch, identical noise) |",
+        "pop_err_normalised_vs_fp64": pop_err_norm,
         "max_unnormalised_population": max_pop,
         "pop_err_trajectories": nref,
+        "profiled_ms_per_step": elapsed_prof / args.steps * 1e3,
+        "profiled_pass_bitwise_equal": prof_equal,
+        "c128_leg": {"value": B * args.steps / c128_elapsed, "ms_per_step": c128_elapsed / args.steps * 1e3,
+                     "dtype": "c128", "note": "same batch and steps with complex128 stages"},
         "gpu_fp64_single_traj_steps_per_s": single_rate,
         "speedup_vs_gpu_fp64_single": value / single_rate,
         "init_s": t_init,
@@ -471,14 +533,14 @@ def main():
     if args.c3_steps > 0:
         del batch, st
         torch.cuda.empty_cache()
-        c3 = run_c3(256, args.c3_steps, "mixed", min(20, args.c3_steps), 4)
-        c3.pop("chunks", None)
+        c3 = run_c3(256, args.c3_steps, "mixed", min(20, args.c3_steps), args.ref_traj, chunk=25, log=log)
+        c3["chunks"] = [{"t_fs": c["t_fs"], "rate": round(256 * 25 / c["s"], 1), "mean_n_phi": round(c["mean_n_phi"])}
+                        for c in c3["chunks"]]
         out["adaptive_c3"] = dict(c3, config="adaptive NORMALIZED NONLINEAR batch: 100-site chain, depth 10, "
                                              "delta 1e-3, 256 trajectories with their own bases, t = 0..%d fs"
                                              % args.c3_steps)
-    if not args.no_cpu:
-        procs = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, procs)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -163,7 +163,7 @@ int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
    step in one launch (stream-ordered scratch for the tables of stages 1-3).
    "graph_steps": K (default 16) RK4 steps of hops_propagate captured in one
    HIP graph and replayed for steps 1, 2, ... (0, or profiling enabled, or a
-   call with fewer than K + 1 steps: every kernel launched eagerly).
+   call with fewer than 4 K + 1 steps: every kernel launched eagerly).
    "resident_rows": bases of at most this many rows per trajectory (default
    1024) run a whole hops_propagate call in one launch, one workgroup per
    trajectory looping over the steps (0: never).

@@ -1498,7 +1498,9 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   // graph replay: step 0 eagerly, then steps 1 .. in launches of K captured
   // steps whose prologues add the device step counter (zeroed below)
   const int K = g_graph_steps;
-  bool graph = !noalloc && K > 0 && !prof && chunk >= bt->B && n_steps - 1 >= K;
+  // (a call replays its graph at least 4 times, or the capture and
+  // instantiation cost more than the launches they save)
+  bool graph = !noalloc && K > 0 && !prof && chunk >= bt->B && n_steps - 1 >= 4 * K;
   if (graph) {
     // a caller that captures this call into a graph of its own gets eager
     // launches (no capture inside a capture)
