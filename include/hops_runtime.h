@@ -95,11 +95,16 @@ long long hops_rt_arena_free(void *batch);
 int hops_rt_batch_timers(void *batch, double *out /*[8]*/);
 
 /* Advance every active slot (slots_or_null == NULL) or the listed slots by
-   n_steps steps of their own dt. */
+   n_steps steps of their own dt.  Returns HOPS_RT_ARENA_FULL when the new
+   basis of some slots did not fit in the arena: those slots are marked out
+   of memory (info[8]) and left mid-step, every other slot completed the step,
+   and the call stops there.  Remove the marked slots before the next call
+   (a serving loop requeues their requests). */
+#define HOPS_RT_ARENA_FULL (-2)
 int hops_rt_batch_step(void *batch, int n_steps, const int32_t *slots_or_null, int n_slots);
 
-/* Read-backs.  info[0..7] = n_state, n_hier, steps taken, noise index,
-   recorded steps, device bytes, early steps left, active. */
+/* Read-backs.  info[0..8] = n_state, n_hier, steps taken, noise index,
+   recorded steps, device bytes, early steps left, active, out of memory. */
 int hops_rt_slot_info(void *batch, int slot, long long *info);
 int hops_rt_slot_states(void *batch, int slot, int32_t *states_out);
 int hops_rt_slot_aux(void *batch, int slot, int32_t *ptr_out, int32_t *mode_out, int32_t *val_out);

@@ -6,7 +6,7 @@ loudly when it is missing: there is no CPU fallback for the propagation path.
 import ctypes
 import os
 
-from .errors import EngineError
+from .errors import ArenaFull, EngineError
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhops_mi355x.so")
@@ -147,9 +147,15 @@ def lib():
     return L
 
 
+HOPS_RT_ARENA_FULL = -2
+
+
 def check(rc):
     if rc != 0:
-        raise EngineError(lib().hops_last_error().decode())
+        msg = lib().hops_last_error().decode()
+        if rc == HOPS_RT_ARENA_FULL:
+            raise ArenaFull(msg)
+        raise EngineError(msg)
 
 
 EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_profile_read", "hops_set_option",

@@ -1258,7 +1258,9 @@ struct Profile {
   double ms[3] = {0, 0, 0};
   long long n[3] = {0, 0, 0};
 };
-Profile g_prof;
+// per host thread: concurrent callers on other threads keep their own events
+// and counters
+thread_local Profile g_prof;
 bool g_stationary = true;
 int g_chunk_traj = 0;
 int g_nt_stream = 0;  // nontemporal hint level of k_stage_shared (see stage_row_reg)
@@ -1282,28 +1284,26 @@ __global__ void k_step_advance(int *ctr, int d_tidx, int d_slot) {
   }
 }
 
-// Capture happens on a private non-blocking stream per device (the caller's
-// stream may be the legacy default stream, which cannot capture); the graph
-// is then launched on the caller's stream.  An executable graph is destroyed
-// only after the event recorded behind its last launch has completed.
-constexpr int kMaxDevices = 64;
-hipStream_t g_capture_stream[kMaxDevices] = {};
+// Capture happens on a non-blocking stream created for the call (the
+// caller's stream may be the legacy default stream, which cannot capture,
+// and concurrent callers on other host threads capture on streams of their
+// own); the graph is then launched on the caller's stream.  An executable
+// graph is destroyed only after the event recorded behind its last launch
+// has completed.
 struct RetiredGraph {
   hipGraphExec_t ex;
   hipEvent_t done;
 };
 std::vector<RetiredGraph> g_retired;
-std::mutex g_graph_mu;  // the capture streams and the retired list, across host threads
+std::mutex g_graph_mu;  // the retired list and per-device scratch addresses, across host threads
+constexpr int kMaxDevices = 64;
 
-int capture_stream(hipStream_t *out) {
-  std::lock_guard<std::mutex> lk(g_graph_mu);
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  if (dev < 0 || dev >= kMaxDevices) return fail("device %d outside [0, %d)", dev, kMaxDevices);
-  if (!g_capture_stream[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_capture_stream[dev], hipStreamNonBlocking));
-  *out = g_capture_stream[dev];
-  return 0;
-}
+struct CaptureStream {
+  hipStream_t s = nullptr;
+  ~CaptureStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
 
 void reap_graphs() {
   std::lock_guard<std::mutex> lk(g_graph_mu);
@@ -1620,8 +1620,9 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   if (graph) {
     if (int rc = one_step(*bt, 0, st, nullptr)) return rc;
     reap_graphs();
-    hipStream_t cs = nullptr;
-    if (int rc = capture_stream(&cs)) return rc;
+    CaptureStream cap;
+    HIP_TRY(hipStreamCreateWithFlags(&cap.s, hipStreamNonBlocking));
+    hipStream_t cs = cap.s;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     int rc = 0;
     for (int j = 0; j < K && !rc; ++j) rc = one_step(*bt, 1 + j, cs, step_ctr);

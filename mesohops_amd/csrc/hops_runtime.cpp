@@ -1421,6 +1421,21 @@ class Arena {
       }
     return -1;
   }
+  // the highest-addressed fit: the staging area lives at the top of the
+  // arena so that slot blocks pack from the bottom without holes around it
+  long long alloc_top(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    for (auto it = free_.rbegin(); it != free_.rend(); ++it)
+      if (it->second >= bytes) {
+        const size_t off = it->first + it->second - bytes;
+        if (it->second == bytes) free_.erase(std::next(it).base());
+        else it->second -= bytes;
+        used_[off] = bytes;
+        return (long long)off;
+      }
+    return -1;
+  }
   void release(long long off) {
     if (off < 0) return;
     auto u = used_.find((size_t)off);
@@ -1487,6 +1502,9 @@ struct Slot {
   Basis Bnew;
   bool changed = false;
   double scale = 1;
+  // the new basis of the last step did not fit in the arena: the slot was
+  // left mid-step (Phi advanced, basis and time not) and must be removed
+  bool oom = false;
 };
 
 struct Batch {
@@ -1668,7 +1686,7 @@ int ensure_staging(Batch &b, size_t bytes) {
   RT_HIP(hipStreamSynchronize(b.stream));
   if (b.scratch_off >= 0) b.arena.release(b.scratch_off);
   const size_t nb = std::max(bytes, b.scratch_bytes * 3 / 2);
-  b.scratch_off = b.arena.alloc(nb);
+  b.scratch_off = b.arena.alloc_top(nb);
   if (b.scratch_off < 0) {
     b.scratch_bytes = 0;
     return rt_fail("arena full: %zu bytes of staging", nb);
@@ -1965,24 +1983,38 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
   }
   for (int rc : rcs)
     if (rc < 0) return rc;
-  // blocks, descriptors, staging layout (serial: the arena is not shared)
+  // blocks, descriptors, staging layout (serial: the arena is not shared).
+  // A new basis that does not fit marks its slot out of memory; the others
+  // are placed as usual.
   Timer tmr_u(&b.tm[5]);
-  std::vector<long long> nblk(ch.size());
+  std::vector<long long> nblk;
+  std::vector<Layout> lays;
+  {
+    std::vector<int> fit;
+    for (int s : ch) {
+      Slot &sl = b.slots[s];
+      Layout L = layout_of(sl.Bnew, b.precision);
+      const long long blk = b.arena.alloc(L.total);
+      if (blk < 0) {
+        sl.oom = true;
+        sl.changed = false;
+        Basis().swap_out(sl.Bnew);
+      } else {
+        fit.push_back(s);
+        nblk.push_back(blk);
+        lays.push_back(std::move(L));
+      }
+    }
+    ch.swap(fit);
+    if (ch.empty()) return 0;
+  }
   std::vector<hops_op_t> nop(ch.size());
   std::vector<hops_batch_t> nbt(ch.size());
   std::vector<size_t> tab_bytes(ch.size()), tab_off(ch.size()), inv_off(ch.size()), prev_off(ch.size());
-  std::vector<Layout> lays(ch.size());
   size_t stage_bytes_needed = 0;
   for (size_t i = 0; i < ch.size(); ++i) {
     Slot &sl = b.slots[ch[i]];
-    lays[i] = layout_of(sl.Bnew, b.precision);
     const Layout &L = lays[i];
-    nblk[i] = b.arena.alloc(L.total);
-    if (nblk[i] < 0) {
-      for (size_t j = 0; j < i; ++j) b.arena.release(nblk[j]);
-      return rt_fail("arena full: a basis of %d states x %d auxiliaries needs %zu bytes", sl.Bnew.n, sl.Bnew.nh,
-                     L.total);
-    }
     char *base = b.arena.ptr(nblk[i]);
     set_op(sl.Bnew, base, L, b.precision, nop[i]);
     fill_bt(b, sl, sl.Bnew, L, base, nbt[i]);
@@ -2138,6 +2170,7 @@ int init_pending(Batch &b) {
   RT_TRY(update_bases(b, adapt, ts, ta, sc));
   for (size_t i = 0; i < adapt.size(); ++i) {
     Slot &sl = b.slots[adapt[i]];
+    if (sl.oom) continue;
     if (!sl.changed) sl.scale = 1.0;
     else sl.scale = sc[i];
     record_step(sl, old_states[i], old_psi[i]);
@@ -2155,6 +2188,17 @@ int init_pending(Batch &b) {
     sl.pending_init = false;
   }
   return 0;
+}
+
+// status of a step in which some new bases did not fit in the arena: the
+// other slots completed it; the listed out-of-memory slots are left mid-step
+int arena_full(const Batch &b, const std::vector<int> &list) {
+  std::string ids;
+  for (int s : list)
+    if (b.slots[s].oom) ids += (ids.empty() ? "" : ",") + std::to_string(s);
+  rt_fail("arena full: the new bases of slots %s do not fit; remove them (the other slots completed the step)",
+          ids.c_str());
+  return HOPS_RT_ARENA_FULL;
 }
 
 // index_t: step index within this call (the update schedule restarts with every
@@ -2212,6 +2256,10 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
     }
     RT_TRY(cp.run(b));
     RT_TRY(update_bases(b, redo, ts, ta, sc));
+    std::vector<int> live;
+    for (int s : redo)
+      if (!b.slots[s].oom) live.push_back(s);
+    redo.swap(live);
     for (int s : redo) {
       Slot &sl = b.slots[s];
       cp.add(sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16);
@@ -2222,7 +2270,7 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
     std::vector<int> nxt;
     for (int s : redo) {
       Slot &sl = b.slots[s];
-      if (++tries[s] >= sl.prm.inchworm_cap) continue;
+      if (sl.oom || ++tries[s] >= sl.prm.inchworm_cap) continue;
       if (!same_basis(sl.B, sl.sel.states_new, sl.sel.aux_new)) nxt.push_back(s);
     }
     redo.swap(nxt);
@@ -2233,6 +2281,12 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
   std::vector<double> sc;
   std::vector<std::vector<int>> old_states;
   std::vector<std::vector<cplx>> old_psi;
+  {
+    std::vector<int> live;
+    for (int s : upd)
+      if (!b.slots[s].oom) live.push_back(s);
+    upd.swap(live);
+  }
   for (int s : upd) {
     Slot &sl = b.slots[s];
     ts.push_back(sl.sel.states_new);
@@ -2248,6 +2302,7 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
   RT_TRY(update_bases(b, upd, ts, ta, sc));
   for (size_t i = 0; i < upd.size(); ++i) {
     Slot &sl = b.slots[upd[i]];
+    if (sl.oom) continue;
     sl.scale = sl.changed ? sc[i] : 1.0;
     record_step(sl, old_states[i], old_psi[i]);
   }
@@ -2271,13 +2326,19 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
     r.psi = sl.pn;
     sl.rec.push_back(std::move(r));
   }
+  int n_oom = 0;
   for (int s : list) {
     Slot &sl = b.slots[s];
+    if (sl.oom) {
+      ++n_oom;
+      continue;
+    }
     const bool adaptive = sl.prm.delta_a > 0 || sl.prm.delta_s > 0;
     if (adaptive && sl.early_counter < sl.prm.early_steps) sl.early_counter++;
     sl.steps++;
     sl.it += 2 * sl.prm.noise_step;
   }
+  if (n_oom) return arena_full(b, list);
   return 0;
 }
 
@@ -2562,7 +2623,16 @@ long long hops_rt_arena_free(void *p) {
 int hops_rt_batch_step(void *p, int n_steps, const int32_t *slots_or_null, int n_slots) {
   RT_TRY(check_batch(p));
   auto &b = *static_cast<Batch *>(p);
+  for (size_t s = 0; s < b.slots.size(); ++s)
+    if (b.slots[s].used && b.slots[s].oom) return rt_fail("slot %d ran out of arena memory: remove it first", (int)s);
   RT_TRY(init_pending(b));
+  {
+    std::vector<int> all;
+    for (size_t s = 0; s < b.slots.size(); ++s)
+      if (b.slots[s].used) all.push_back((int)s);
+    for (int s : all)
+      if (b.slots[s].oom) return arena_full(b, all);
+  }
   std::vector<int> list;
   if (slots_or_null) {
     for (int i = 0; i < n_slots; ++i) {
@@ -2597,6 +2667,7 @@ int hops_rt_slot_info(void *p, int slot, long long *info) {
   info[5] = (long long)(sl.blk_bytes + (sl.sys ? pers_bytes(*sl.sys) : 0));
   info[6] = std::max(0, sl.prm.early_steps - sl.early_counter);
   info[7] = sl.used;
+  info[8] = sl.oom;
   return 0;
 }
 

@@ -58,3 +58,9 @@ class TrajectoryError(Exception):
 
 class EngineError(RuntimeError):
     """A call into the HIP engine failed; the message is the engine's own."""
+
+
+class ArenaFull(EngineError):
+    """A ragged-batch step could not place the new bases of some slots in the
+    device arena; those slots are marked out of memory and left mid-step,
+    every other slot completed the step."""

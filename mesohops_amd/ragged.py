@@ -214,10 +214,11 @@ class RaggedBatch:
             check(lib().hops_rt_batch_step(self.h, int(n_steps), _ptr(s), len(s)))
 
     def info(self, slot):
-        a = np.zeros(8, dtype=np.int64)
+        a = np.zeros(9, dtype=np.int64)
         check(lib().hops_rt_slot_info(self.h, int(slot), _ptr(a)))
         return {"n_state": int(a[0]), "n_hier": int(a[1]), "steps": int(a[2]), "it": int(a[3]),
-                "records": int(a[4]), "bytes": int(a[5]), "early_left": int(a[6]), "used": bool(a[7])}
+                "records": int(a[4]), "bytes": int(a[5]), "early_left": int(a[6]), "used": bool(a[7]),
+                "oom": bool(a[8])}
 
     def states(self, slot):
         n = self.info(slot)["n_state"]

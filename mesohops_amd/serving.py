@@ -12,6 +12,13 @@ device at admission.
 
 Results do not depend on which requests share the batch: every slot's
 kernels and basis bookkeeping touch only that slot's data.
+
+Back-pressure: admission needs room for twice a request's estimated
+footprint.  When bases grow until a step cannot place some slots' new bases,
+those slots are preempted (removed, their requests queued again at the front
+to restart from t = 0) and the rest of the batch carries on; a preempted
+request returns only once the arena has room for the footprint it had
+reached.  A request that cannot fit even alone raises UnsupportedRequest.
 """
 from __future__ import annotations
 
@@ -26,6 +33,7 @@ import torch
 from . import _lib
 from .basis import SystemInfo
 from .engine import _stream_ptr
+from .errors import ArenaFull, UnsupportedRequest
 from .noise import HopsNoise, device_noise
 from .ragged import RaggedBatch, RtSystem, traj_param
 from .workloads import chain_system
@@ -81,11 +89,14 @@ class ServingLoop:
         a dict of per-request results and the serving statistics."""
         reqs = deque(sorted(requests, key=lambda r: r.arrival))
         futures = {}
-        active = {}          # slot -> [req, steps_done, keep, t_admit]
+        active = {}          # slot -> [req, steps_done, keep, t_admit, peak device bytes]
         done = {}
         start = time.perf_counter()
         steps_total = 0
         n_iter = 0
+        n_preempt = 0
+        need = {}            # rid -> device bytes a preempted request had reached
+        lone_retry = set()   # requests restarted once after outgrowing the arena alone
         last_log = start
         while reqs or active:
             now = time.perf_counter() - start
@@ -105,12 +116,16 @@ class ServingLoop:
                 if not fut.done():
                     break
                 prep = fut.result()
-                if self.rt.free_bytes() < 2 * self._footprint(prep[2], r):
+                if self.rt.free_bytes() < max(2 * self._footprint(prep[2], r), need.get(r.rid, 0)):
+                    if not active:
+                        raise UnsupportedRequest("request %d (%d sites, depth %d) needs more device memory than "
+                                                 "the whole arena holds" % (r.rid, r.nsite, r.maxhier),
+                                                 "ServingLoop")
                     break
                 reqs.popleft()
                 del futures[r.rid]
                 slot, keep = self._admit(r, prep)
-                active[slot] = [r, 0, keep, time.perf_counter() - start]
+                active[slot] = [r, 0, keep, time.perf_counter() - start, 0]
             if not active:
                 if reqs:
                     wait = reqs[0].arrival - (time.perf_counter() - start)
@@ -122,13 +137,37 @@ class ServingLoop:
                             time.sleep(0.001)
                 continue
             slots = list(active)
-            self.rt.step(1, slots=slots)
+            try:
+                self.rt.step(1, slots=slots)
+            except ArenaFull:
+                # back-pressure: the slots whose grown bases did not fit are
+                # preempted and their requests queued again (they restart from
+                # t = 0 once memory frees up); the others completed the step
+                oom = [s for s in slots if self.rt.info(s)["oom"]]
+                if len(active) == 1 and oom:
+                    # alone and out of room: the arena may only have been
+                    # fragmented by earlier requests, so the request restarts
+                    # once in an otherwise empty arena before it is refused
+                    r = active[oom[0]][0]
+                    if r.rid in lone_retry:
+                        raise UnsupportedRequest("request %d (%d sites, depth %d) outgrowing the arena while "
+                                                 "running alone" % (r.rid, r.nsite, r.maxhier), "ServingLoop")
+                    lone_retry.add(r.rid)
+                for s in oom:
+                    r = active.pop(s)[0]
+                    need[r.rid] = 2 * self.rt.info(s)["bytes"]
+                    self.rt.remove(s)
+                    reqs.appendleft(r)
+                    n_preempt += 1
+                slots = [s for s in slots if s not in oom]
             steps_total += len(slots)
             n_iter += 1
             t_now = time.perf_counter() - start
             for s in slots:
                 a = active[s]
                 a[1] += 1
+                if self.record:
+                    a[4] = max(a[4], self.rt.info(s)["bytes"])
                 if a[1] >= a[0].n_steps:
                     r = a[0]
                     res = {"rid": r.rid, "arrival": r.arrival, "admitted": a[3], "finished": t_now,
@@ -138,6 +177,7 @@ class ServingLoop:
                     res["n_state_final"], res["n_hier_final"] = inf["n_state"], inf["n_hier"]
                     if self.record:
                         res["psi"], res["states"], res["nhier"] = self.rt.records(s, r.nsite)
+                        res["peak_bytes"] = a[4]
                     done[r.rid] = res
                     self.rt.remove(s)
                     del active[s]
@@ -149,7 +189,7 @@ class ServingLoop:
         lat = np.array([d["latency"] for d in done.values()]) if done else np.zeros(0)
         stats = {"completed": len(done), "admitted_unfinished": len(active), "not_admitted": len(reqs),
                  "wall_s": wall, "traj_steps": steps_total, "traj_steps_per_s": steps_total / wall if wall else 0.0,
-                 "iterations": n_iter,
+                 "iterations": n_iter, "preempted": n_preempt,
                  "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
                  "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None}
         return done, stats

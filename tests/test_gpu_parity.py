@@ -704,3 +704,52 @@ def test_fp64_precalculated_complex128_noise(adaptive):
     err = np.max(np.abs(np.array(tr.storage["psi_traj"]) - np.array(ref.psi_traj)))
     print("complex128 PRE_CALCULATED adaptive=%s max |dpsi| = %.3e" % (adaptive, err))
     assert err < FP64_TOL
+
+
+def test_two_host_threads_two_streams_bitwise_equal_serial():
+    """Two host threads propagate two batches at once, each on its own stream
+    (graph-replayed steps, LINEAR fused prologue and NORMALIZED NONLINEAR):
+    each result equals the same batch run alone, bit for bit."""
+    _need_gpu()
+    import threading
+    from mesohops_amd import HopsBatch
+    sysp = systems.chain(5, k_mats=1)
+    tlen, tau = 160.0, 0.5
+    psi0 = np.zeros(5, complex)
+    psi0[2] = 1
+
+    def make(eom, seeds):
+        zs = [_oracle_noise(sysp, s, tlen, tau) for s in seeds]
+        nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+        b = HopsBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], None, {"MAXHIER": 3}, noise_params=nps,
+                      precision_mode="mixed", lanes=4)
+        b.initialize(psi0)
+        return b
+
+    jobs = [("LINEAR", list(range(16))), ("NORMALIZED NONLINEAR", list(range(16, 40)))]
+    serial = []
+    for eom, seeds in jobs:
+        b = make(eom, seeds)
+        b.propagate(140.0, 2.0)   # 70 steps: 4 replays of a 16-step graph
+        serial.append(np.array(b.psi_traj))
+    batches = [make(eom, seeds) for eom, seeds in jobs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    errs = []
+
+    def work(i):
+        try:
+            with torch.cuda.stream(streams[i]):
+                batches[i].propagate(140.0, 2.0)
+            streams[i].synchronize()
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for i in range(2):
+        assert np.array_equal(np.array(batches[i].psi_traj), serial[i]), jobs[i][0]
+    print("two threads: both batches equal their serial runs bit for bit")
