@@ -380,6 +380,40 @@ class HopsBatch:
         return self.state.phi_host()[:, :self.tables.n_state]
 
 
+def write_checkpoint(filepath, phi, z_mem, t, state_list, aux_list, early_counter, storage_data, storage_meta,
+                     params, compress=True):
+    """The reference's checkpoint file (trajectory/hops_trajectory.py:884-936):
+    keys phi, z_mem (the sparse (len(L_NOISE1), 1) noise memory), t,
+    state_list, aux_list (an object array of (k, 2) [mode, depth] int arrays),
+    early_counter, storage_data, storage_meta and params."""
+    aux = np.empty(len(aux_list), dtype=object)
+    for i, a in enumerate(aux_list):
+        aux[i] = np.array([[m, v] for (m, v) in a], dtype=int).reshape(-1, 2)
+    ck = {"phi": np.asarray(phi, dtype=np.complex128), "z_mem": z_mem, "t": float(t),
+          "state_list": np.array(state_list, dtype=int), "aux_list": aux, "early_counter": int(early_counter),
+          "storage_data": storage_data, "storage_meta": storage_meta, "params": params}
+    (np.savez_compressed if compress else np.savez)(filepath, **ck)
+
+
+def read_checkpoint(filename):
+    """Parse a checkpoint in the reference's layout (as written by its
+    save_checkpoint or by write_checkpoint; trajectory/hops_trajectory.py:
+    939-1034).  The auxiliary list may come back as a 1-D object array of
+    (k, 2) arrays or, when every auxiliary array has the same shape, stacked
+    by NumPy into one [n_aux, k, 2] array; both give the same (mode, depth)
+    tuples.  The noise memory comes back as a dense vector.  Reads pickled
+    parameter objects (functions, sparse matrices), as the reference does:
+    load only files you trust."""
+    data = np.load(filename, allow_pickle=True)
+    aux = [tuple((int(m), int(v)) for m, v in np.asarray(a).reshape(-1, 2)) for a in data["aux_list"]]
+    zm = data["z_mem"].item()
+    zm = np.asarray(zm.toarray() if hasattr(zm, "toarray") else zm, dtype=np.complex128).reshape(-1)
+    return {"params": data["params"].item(), "phi": np.asarray(data["phi"], dtype=np.complex128),
+            "z_mem": zm, "t": float(data["t"]), "state_list": [int(s) for s in data["state_list"]],
+            "aux_list": aux, "early_counter": int(data["early_counter"]),
+            "storage_data": data["storage_data"].item(), "storage_meta": data["storage_meta"].item()}
+
+
 class HopsTrajectory:
     """One HOPS trajectory with the reference's constructor and methods
     (trajectory/hops_trajectory.py:84-538, :1071-1125).  make_adaptive()
@@ -564,14 +598,9 @@ class HopsTrajectory:
         if not drop_seed and self.noise1.param["SEED"] is None:
             warnings.warn("Noise1 seed is None but drop_seed is False; this may cause unphysical behavior.",
                           UserWarning)
-        aux = np.empty(len(self.auxiliary_list), dtype=object)
-        for i, a in enumerate(self.auxiliary_list):
-            aux[i] = np.array([[m, v] for (m, v) in a], dtype=int).reshape(-1, 2)
         early = self._adaptive.early_counter if self._adaptive is not None else 0
-        ck = {"phi": self.phi, "z_mem": self.z_mem, "t": self.t,
-              "state_list": np.array(self.state_list, dtype=int), "aux_list": aux, "early_counter": early,
-              "storage_data": self.storage.data, "storage_meta": self.storage.metadata, "params": params}
-        (np.savez_compressed if compress else np.savez)(filepath, **ck)
+        write_checkpoint(filepath, self.phi, self.z_mem, self.t, self.state_list, self.auxiliary_list, early,
+                         self.storage.data, self.storage.metadata, params, compress)
 
     @classmethod
     def load_checkpoint(cls, filename, add_seed1=None, add_seed2=None, add_system_param=None,
@@ -582,8 +611,8 @@ class HopsTrajectory:
         that propagation continues exactly where it stopped.  Reads pickled
         parameter objects (functions, sparse matrices), as the reference's
         checkpoints do: load only files you trust."""
-        data = np.load(filename, allow_pickle=True)
-        params = data["params"].item()
+        ck = read_checkpoint(filename)
+        params = ck["params"]
         if add_seed1 is not None:
             params["noise1_param"]["SEED"] = add_seed1
         if add_seed2 is not None:
@@ -594,14 +623,10 @@ class HopsTrajectory:
                    noise2_param=params["noise2_param"], hierarchy_param=params["hierarchy_param"],
                    storage_param=params["storage_param"], integration_param=params["integration_param"],
                    precision_mode=precision_mode, device=device)
-        states = [int(s) for s in data["state_list"]]
-        phi = np.asarray(data["phi"], dtype=np.complex128)
+        states, phi, aux, zm = ck["state_list"], ck["phi"], ck["aux_list"], ck["z_mem"]
         psi_0 = np.zeros(traj._batch.setup.info.nstates, dtype=np.complex128)
         psi_0[states] = phi[:len(states)]
         traj.initialize(psi_0)
-        aux = [tuple((int(m), int(v)) for m, v in np.asarray(a).reshape(-1, 2)) for a in data["aux_list"]]
-        zm = data["z_mem"].item()
-        zm = np.asarray(zm.toarray() if hasattr(zm, "toarray") else zm, dtype=np.complex128).reshape(-1)
         info = traj._batch.setup.info
         zdev = torch.zeros(info.n_hmodes + 1, dtype=torch.complex128)
         zdev[: info.n_hmodes] = torch.from_numpy(zm[: info.n_hmodes])
@@ -613,19 +638,19 @@ class HopsTrajectory:
             ad.st.set_phi(phi)
             ad.st.zmem.copy_(zdev.to(ad.device))
             ad._sync_zmem_from_state()
-            ad.early_counter = int(data["early_counter"])
-            traj._t = float(data["t"])
+            ad.early_counter = ck["early_counter"]
+            traj._t = ck["t"]
         else:
             b = traj._batch
             if [tuple(a) for a in b.tables.aux_list] != sorted(aux, key=aux_order_key):
                 raise TrajectoryError("checkpoint hierarchy does not match the parameters")
             b.state.set_phi(phi)
             b.state.zmem[: info.n_hmodes + 1].copy_(zdev.to(b.device)[: info.n_hmodes + 1])
-            b.t = float(data["t"])
+            b.t = ck["t"]
             b.t_axis = [b.t]
             b.psi_traj = [[phi[: b.tables.n_state].copy()]]
-        traj.storage.data = data["storage_data"].item()
-        traj.storage.metadata = data["storage_meta"].item()
+        traj.storage.data = ck["storage_data"]
+        traj.storage.metadata = ck["storage_meta"]
         return traj
 
     # -- readbacks (trajectory/hops_trajectory.py:1071-1125) -------------
