@@ -94,6 +94,15 @@ long long hops_rt_arena_free(void *batch);
    [7] (a count, not seconds) hierarchy rows advanced by RK4 steps. */
 int hops_rt_batch_timers(void *batch, double *out /*[8]*/);
 
+/* Batch options (no counterpart in the reference).  "device_select": 1 runs
+   the adaptive basis selection of the eligible slots on the device
+   (hierarchy adaptivity on, no Triangular / LongEdge filter, at most one
+   diagonal entry per mode's L-operator in the state basis); 0 (default) runs
+   every selection on the host threads.  The device form is experimental: it
+   equals the host selection bit for bit in a CPU emulation of its kernels,
+   but has not passed on the GPU. */
+int hops_rt_batch_option(void *batch, const char *name, long long value);
+
 /* Advance every active slot (slots_or_null == NULL) or the listed slots by
    n_steps steps of their own dt.  Returns HOPS_RT_ARENA_FULL when the new
    basis of some slots did not fit in the arena: those slots are marked out

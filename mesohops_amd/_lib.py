@@ -123,6 +123,7 @@ def lib():
     L.hops_rt_arena_free.restype = _ll
     L.hops_rt_arena_free.argtypes = [_vp]
     L.hops_rt_batch_timers.argtypes = [_vp, _vp]
+    L.hops_rt_batch_option.argtypes = [_vp, ctypes.c_char_p, _ll]
     L.hops_rt_batch_step.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int]
     L.hops_rt_slot_info.argtypes = [_vp, ctypes.c_int, _vp]
     L.hops_rt_slot_states.argtypes = [_vp, ctypes.c_int, _vp]
@@ -164,7 +165,7 @@ EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_prof
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
             "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",
-            "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
+            "hops_rt_batch_option", "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
             "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
             "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",
             "hops_rt_debug_result", "hops_rt_debug_free", "hops_rt_debug_threshold"]

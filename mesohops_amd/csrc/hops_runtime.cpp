@@ -46,6 +46,7 @@
 
 #include "../../include/hops_mi355x.h"
 #include "../../include/hops_runtime.h"
+#include "hops_select.h"
 
 extern "C" int hops_internal_set_error(const char *msg);
 
@@ -822,16 +823,31 @@ double select_threshold(const std::vector<double> &v, double max_err, double off
   return 0.0;
 }
 
-// error_threshold(sort(v), max_err, offset) and the sum of the values <= the
-// threshold (v: non-negative error terms)
+// Sum of f(0) .. f(n-1) in a fixed blocked order: a sequential sum over each
+// block of 256 consecutive terms, then a sequential sum of the block sums.
+// The device selection (hops_select.hip) adds its sums in the same order, a
+// tile of 256 entries per block, so both give the same bits.  A term left out
+// of a sum is passed as +0.0, which leaves every partial sum unchanged.
+constexpr size_t kSumBlock = 256;
+template <class F>
+double bsum(size_t n, F f) {
+  double tot = 0;
+  for (size_t b = 0; b < n; b += kSumBlock) {
+    double part = 0;
+    const size_t e = std::min(n, b + kSumBlock);
+    for (size_t i = b; i < e; ++i) part += f(i);
+    tot += part;
+  }
+  return tot;
+}
+
+// error_threshold(sort(v), max_err, offset) and the blocked sum of the values
+// <= the threshold (v: non-negative error terms)
 double partial_threshold(const std::vector<double> &v, double max_err, double offset, double *below) {
   bool ok = false;
   double thr = select_threshold(v, max_err, offset, &ok);
   if (!ok) thr = prefix_threshold(v, max_err, offset);
-  double sb = 0;
-  for (double x : v)
-    if (x <= thr) sb += x;
-  *below = sb;
+  *below = bsum(v.size(), [&](size_t i) { return v[i] <= thr ? v[i] : 0.0; });
   return thr;
 }
 
@@ -911,13 +927,13 @@ void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
   }
 }
 
-// hierarchy half: stable auxiliaries, boundary auxiliaries, new aux list
-void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *e_aux, Selection &out) {
+// The part of the selection that needs only psi = Pn[:n]: the diagonal
+// entries M2 of each mode's reduced L, <L> over psi (X2) and the outside
+// states coupled to the basis (sc)
+void sel_common(const Basis &B, const cplx *pn, Selection &out) {
   const System &S = *B.sys;
-  const int n = B.n, nh = B.nh, nm = B.nm;
+  const int n = B.n, nm = B.nm;
   out = Selection();
-  out.absC2.resize((size_t)nh * n);
-  for (size_t i = 0; i < (size_t)nh * n; ++i) out.absC2[i] = std::norm(pn[i]);
   // M2: diagonal entries of each mode's reduced L (basis/hops_basis.py:1030-1151)
   out.M2.resize(nm);
   for (int lm = 0; lm < nm; ++lm) {
@@ -943,6 +959,36 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   out.X2.resize(nm);
   for (int lm = 0; lm < nm; ++lm) out.X2[lm] = avg[B.mode_lrel[lm]];
   out.sc = list_sc(S, B);
+}
+
+// the couplings -iH[sp, s] of each outside state sp (in sc order) to the
+// basis states s, as CSR; a zero entry adds an exact zero to the flux sum,
+// so it is left out
+void sc_couplings(const Basis &B, const std::vector<int> &sc, std::vector<int> &hp, std::vector<int> &hs,
+                  std::vector<cplx> &hv) {
+  const System &S = *B.sys;
+  hp.assign(sc.size() + 1, 0);
+  hs.clear();
+  hv.clear();
+  for (size_t i = 0; i < sc.size(); ++i) {
+    for (int s = 0; s < B.n; ++s) {
+      const cplx h = S.H[(size_t)sc[i] * S.n + B.states[s]];
+      if (h != cplx(0, 0)) {
+        hs.push_back(s);
+        hv.push_back(cmi(h));
+      }
+    }
+    hp[i + 1] = (int)hs.size();
+  }
+}
+
+// hierarchy half: stable auxiliaries, boundary auxiliaries, new aux list
+void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *e_aux, Selection &out) {
+  const System &S = *B.sys;
+  const int n = B.n, nh = B.nh, nm = B.nm;
+  sel_common(B, pn, out);
+  out.absC2.resize((size_t)nh * n);
+  for (size_t i = 0; i < (size_t)nh * n; ++i) out.absC2[i] = std::norm(pn[i]);
   if (P.delta_a <= 0) {
     out.aux_new = B.aux;
     out.stable_idx.resize(nh);
@@ -954,20 +1000,9 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   std::vector<double> err(e_aux, e_aux + nh);
   // flux out of the state basis through H (basis_functions_adaptive.py:76-163)
   if (!out.sc.empty()) {
-    // -i H[sp, s] over the basis states s coupled to each outside state sp
-    // (a zero entry adds an exact zero to the sum, so it is skipped)
-    std::vector<int> hp(out.sc.size() + 1, 0), hs;
+    std::vector<int> hp, hs;
     std::vector<cplx> hv;
-    for (size_t i = 0; i < out.sc.size(); ++i) {
-      for (int s = 0; s < n; ++s) {
-        const cplx h = S.H[(size_t)out.sc[i] * S.n + B.states[s]];
-        if (h != cplx(0, 0)) {
-          hs.push_back(s);
-          hv.push_back(cmi(h));
-        }
-      }
-      hp[i + 1] = (int)hs.size();
-    }
+    sc_couplings(B, out.sc, hp, hs, hv);
     for (int k = 0; k < nh; ++k) {
       const cplx *pk = pn + (size_t)k * n;
       double acc = 0;
@@ -1079,17 +1114,12 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   double below = 0;
   const double thr = partial_threshold(err, dsq / 2.0, 0.0, &below);
   out.stable_mask.assign(nh, 0);
-  std::vector<double> un;
-  for (int k = 0; k < nh; ++k) {
+  for (int k = 0; k < nh; ++k)
     if (err[k] > thr) {
       out.stable_idx.push_back(k);
       out.stable_mask[k] = 1;
-    } else {
-      un.push_back(err[k]);
     }
-  }
-  double unstable = 0;
-  for (double v : un) unstable += v;
+  const double unstable = bsum(nh, [&](size_t k) { return err[k] > thr ? 0.0 : err[k]; });
   const double dbound = dsq - unstable;
   // boundary auxiliaries (hops_basis.py:667-793): flux out of the basis, up
   // (depth < maxhier, k+e_m not in the basis) and down (k-e_m not in the
@@ -1209,8 +1239,7 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
     double v1 = 0;
     for (int sp = 0; sp < S.n; ++sp)
       if (sp != B.states[s]) v1 += std::norm(S.H[(size_t)sp * S.n + B.states[s]]);
-    double c1 = 0;
-    for (int k : sel.stable_idx) c1 += sel.absC2[(size_t)k * n + s];
+    const double c1 = bsum(nh, [&](size_t k) { return sel.stable_mask[k] ? sel.absC2[k * n + s] : 0.0; });
     err[s] += v1 * c1 / h2;
   }
   // filters from the boundary auxiliaries (hops_fluxfilters.py:292-352)
@@ -1298,11 +1327,12 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
           qh.push_back(cmi(h));
         }
       }
-      for (int k : sel.stable_idx) {
+      E[i] = bsum(nh, [&](size_t k) {
+        if (!sel.stable_mask[k]) return 0.0;
         cplx z = 0;
-        for (size_t q = 0; q < qs.size(); ++q) z += cm(qh[q], pn[(size_t)k * n + qs[q]] / kHbar);
-        E[i] += std::norm(z);
-      }
+        for (size_t q = 0; q < qs.size(); ++q) z += cm(qh[q], pn[k * n + qs[q]] / kHbar);
+        return std::norm(z);
+      });
     }
     std::vector<double> ev;
     std::vector<int> es;
@@ -1505,6 +1535,8 @@ struct Slot {
   // the new basis of the last step did not fit in the arena: the slot was
   // left mid-step (Phi advanced, basis and time not) and must be removed
   bool oom = false;
+  // the system's Zobrist tables on the device (a then b, [n_hmode * 256] each)
+  const uint64_t *zob_d = nullptr;
 };
 
 struct Batch {
@@ -1525,10 +1557,18 @@ struct Batch {
   std::vector<hops_slot_perm_t> perms;
   std::vector<int32_t> its;
   std::vector<double> dts, nrms;
-  // accumulated wall time (s) per phase: step, errors on the device, host
+  // accumulated wall time (s) per phase: step, errors on the device,
   // hierarchy selection, state errors + selection, table build, device
   // update, psi fetch; [7]: hierarchy rows advanced by RK4 steps
   double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // selection on the device (1) or on the host threads (0, default: the
+  // device selection has not yet passed on the GPU)
+  int sel_device = 0;
+  // device work space of the selection: one region per phase
+  long long sel_off[4] = {-1, -1, -1, -1};
+  size_t sel_bytes[4] = {0, 0, 0, 0};
+  // Zobrist tables on the device per system: arena offset, slots using them
+  std::map<const System *, std::pair<long long, int>> zob;
 };
 
 struct Timer {
@@ -1840,8 +1880,8 @@ int fetch_psi(Batch &b, const std::vector<int> &list) {
   return 0;
 }
 
-// define_basis for the listed slots; dt per slot in b.dts
-int define_bases(Batch &b, const std::vector<int> &list) {
+// define_basis for the listed slots on the host threads; dt per slot in b.dts
+int define_bases_host(Batch &b, const std::vector<int> &list) {
   if (list.empty()) return 0;
   const size_t S = b.slots.size();
   std::vector<int64_t> off(S, 0), eoff(S, 0);
@@ -1946,6 +1986,780 @@ int define_bases(Batch &b, const std::vector<int> &list) {
     if (sl.prm.delta_s <= 0) sl.sel.states_new = sl.B.states;
   }
   return 0;
+}
+
+// ---- selection on the device (hops_select.hip) ------------------------------
+//
+// The same selection as select_hier / select_states, with the per-auxiliary
+// work on the device: the error bound of every auxiliary, the stable set,
+// the boundary candidates, the pre-trim, the binning by target auxiliary and
+// the per-state sums.  Only psi, a few counts per slot, the stable flags and
+// the boundary auxiliaries cross PCIe; the host keeps the per-slot work that
+// is O(n_state) or O(boundary) (X2, the coupled outside states, the filters
+// of the state half and the state thresholds).
+
+// the slot's system's Zobrist tables on the device, shared by the slots of
+// that system in the batch (uploaded on the first device selection); a slot
+// whose tables do not fit in the arena keeps the host selection
+int ensure_zob(Batch &b, Slot &sl) {
+  if (sl.zob_d || !sl.sys) return 0;
+  auto it = b.zob.find(sl.sys);
+  if (it == b.zob.end()) {
+    const System &S = *sl.sys;
+    const size_t zb = S.zob.size() * 8;
+    const long long zo = b.arena.alloc(2 * zb);
+    if (zo < 0) return 0;
+    RT_HIP(hipMemcpyAsync(b.arena.ptr(zo), S.zob.data(), zb, hipMemcpyHostToDevice, b.stream));
+    RT_HIP(hipMemcpyAsync(b.arena.ptr(zo) + zb, S.zob2.data(), zb, hipMemcpyHostToDevice, b.stream));
+    RT_HIP(hipStreamSynchronize(b.stream));  // pageable sources
+    it = b.zob.emplace(sl.sys, std::make_pair(zo, 0)).first;
+  }
+  it->second.second++;
+  sl.zob_d = reinterpret_cast<const uint64_t *>(b.arena.ptr(it->second.first));
+  return 0;
+}
+
+// a slot whose selection runs on the device: hierarchy adaptivity on, no
+// Triangular / LongEdge filter, at most one diagonal entry per mode's L
+bool dev_eligible(const Slot &sl) {
+  if (!(sl.prm.delta_a > 0) || !sl.prm.filters.empty() || !sl.zob_d) return false;
+  const Basis &B = sl.B;
+  for (int lm = 0; lm < B.nm; ++lm) {
+    const RedL &R = B.L[B.mode_lrel[lm]];
+    int c = 0;
+    for (int s = 0; s < B.n; ++s)
+      for (int q = R.rp[s]; q < R.rp[s + 1]; ++q) c += R.ci[q] == s;
+    if (c > 1) return false;
+  }
+  return true;
+}
+
+// per relative mode: the terms select_hier forms from S.W, S.G, M2 and X2
+void sel_modes(const Params &P, const Basis &B, const Selection &sel, hops_sel_mode_t *md) {
+  const System &S = *B.sys;
+  for (int lm = 0; lm < B.nm; ++lm) {
+    const int m = B.modes[lm];
+    hops_sel_mode_t &c = md[lm];
+    c.aw = std::abs(S.W[m]);
+    c.gw2 = std::norm(S.G[m] / S.W[m]);
+    c.nX = std::norm(sel.X2[lm]);
+    if (sel.M2[lm].empty()) {
+      c.m2s = -1;
+      c.m2u = c.m2d = 0.0;
+    } else {
+      const auto &e = sel.M2[lm][0];
+      c.m2s = e.first;
+      c.m2u = std::norm(e.second);
+      c.m2d = std::norm(e.second - sel.X2[lm]);
+    }
+    c.markov = !P.markov.empty() && P.markov[m] ? 1 : 0;
+  }
+}
+
+// the (mode, auxiliary) pairs of the state half's flux filters
+// (hops_fluxfilters.py:292-352): F_dn[lm, j] for j = a + e_lm and F_up[lm, j]
+// for j = a - e_lm, a a boundary auxiliary, j a stable auxiliary; sorted by
+// (lm, j), each as (lm << 2 | F_dn << 1 | F_up, j)
+void state_flag_list(const Basis &B, const Selection &sel, std::vector<uint32_t> &out) {
+  const System &S = *B.sys;
+  std::vector<std::pair<uint64_t, uint32_t>> f;  // ((lm << 32 | j), flag bit)
+  for (const Aux &a : sel.aux_bound) {
+    const AKey ka = key_of(S, a);
+    for (int lm = 0; lm < B.nm; ++lm) {
+      const int v = aux_get(a, B.modes[lm]);
+      if (v >= 255) continue;
+      const int j = B.index.find(key_step(S, ka, B.modes[lm], v, v + 1));
+      if (j >= 0 && sel.stable_mask[j]) f.push_back({((uint64_t)lm << 32) | (uint32_t)j, 2u});
+    }
+    for (char32_t e : a) {
+      const int j = B.index.find(key_step(S, ka, aux_mode(e), aux_val(e), aux_val(e) - 1));
+      if (j >= 0 && sel.stable_mask[j]) f.push_back({((uint64_t)B.rel_mode[aux_mode(e)] << 32) | (uint32_t)j, 1u});
+    }
+  }
+  std::sort(f.begin(), f.end());
+  out.clear();
+  for (size_t i = 0; i < f.size();) {
+    uint32_t bits = 0;
+    size_t e = i;
+    for (; e < f.size() && f[e].first == f[i].first; ++e) bits |= f[e].second;
+    out.push_back((uint32_t)(f[i].first >> 32) << 2 | bits);
+    out.push_back((uint32_t)(f[i].first & 0xffffffffu));
+    i = e;
+  }
+}
+
+// bump layout of a device region: offsets first (base null), pointers after
+struct Lay {
+  char *base = nullptr;
+  size_t cur = 0;
+  template <typename T>
+  T *take(size_t n) {
+    T *p = reinterpret_cast<T *>(base + cur);
+    cur += rsz(std::max<size_t>(n * sizeof(T), 16));
+    return p;
+  }
+};
+
+// device region r of at least bytes (its contents are not kept on growth)
+int ensure_sel(Batch &b, int r, size_t bytes) {
+  if (bytes <= b.sel_bytes[r]) return 0;
+  RT_HIP(hipStreamSynchronize(b.stream));
+  if (b.sel_off[r] >= 0) b.arena.release(b.sel_off[r]);
+  const size_t nb = std::max(rsz(bytes) + (1 << 20), b.sel_bytes[r] * 3 / 2);
+  b.sel_off[r] = b.arena.alloc_top(nb);
+  if (b.sel_off[r] < 0) {
+    b.sel_bytes[r] = 0;
+    return HOPS_RT_ARENA_FULL;
+  }
+  b.sel_bytes[r] = nb;
+  return 0;
+}
+
+// host<->device copies through the pinned staging buffers; each phase starts
+// at offset 0 and ends with a stream synchronisation
+struct Xfer {
+  Batch &b;
+  size_t up = 0, down = 0;
+  int put(void *dev, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    if (up + rsz(bytes) > b.h_up_bytes) return rt_fail("selection upload exceeds the staging buffer");
+    std::memcpy(b.h_up + up, src, bytes);
+    RT_HIP(hipMemcpyAsync(dev, b.h_up + up, bytes, hipMemcpyHostToDevice, b.stream));
+    up += rsz(bytes);
+    return 0;
+  }
+  int get(const void *dev, size_t bytes, char **host) {
+    if (down + rsz(bytes) > b.h_down_bytes) return rt_fail("selection download exceeds the staging buffer");
+    *host = b.h_down + down;
+    if (bytes) RT_HIP(hipMemcpyAsync(*host, dev, bytes, hipMemcpyDeviceToHost, b.stream));
+    down += rsz(std::max<size_t>(bytes, 16));
+    return 0;
+  }
+  int sync() {
+    RT_HIP(hipStreamSynchronize(b.stream));
+    up = down = 0;
+    return 0;
+  }
+};
+
+template <typename T>
+size_t vbytes(const std::vector<T> &v) {
+  return v.size() * sizeof(T);
+}
+
+int define_bases_dev(Batch &b, const std::vector<int> &list) {
+  const int L = (int)list.size();
+  const size_t S = b.slots.size();
+  auto t_dev = std::chrono::steady_clock::now();
+  // ---- layout of the per-auxiliary work --------------------------------------
+  std::vector<hops_sel_seg_t> seg(L);
+  std::vector<int64_t> off(S, 0), eoff(S, 0), poff(L), ebeg(L), eend(L);
+  std::vector<int2> atl;
+  size_t tot = 0, etot = 0, htot = 0, ttot = 0, tctot = 0, mtot = 0, stot = 0;
+  bool any_s = false;
+  for (int i = 0; i < L; ++i) {
+    const Slot &sl = b.slots[list[i]];
+    const Basis &B = sl.B;
+    hops_sel_seg_t &g = seg[i];
+    g = hops_sel_seg_t{};
+    g.slot = list[i];
+    g.n = B.n;
+    g.nh = B.nh;
+    g.nm = B.nm;
+    g.maxhier = sl.prm.maxhier;
+    for (uint8_t f : sl.prm.markov) g.markov |= f ? 1 : 0;
+    g.n_tiles = (B.nh + 255) / 256;
+    g.pn_off = off[list[i]] = (int64_t)tot;
+    tot += (size_t)B.n * B.nh;
+    g.e_off = eoff[list[i]] = (int64_t)etot;
+    etot += std::max(B.nh, B.n);
+    g.h_off = ebeg[i] = (int64_t)htot;
+    htot += B.nh;
+    eend[i] = (int64_t)htot;
+    g.t_off = (int64_t)ttot;
+    ttot += g.n_tiles;
+    g.tc_off = (int64_t)tctot;
+    tctot += (size_t)g.n_tiles * B.n;
+    g.m_off = (int64_t)mtot;
+    mtot += B.nm;
+    g.s_off = poff[i] = (int64_t)stot;
+    stot += B.n;
+    g.dsq = sl.prm.delta_a * sl.prm.delta_a;
+    g.f_discard = sl.prm.f_discard;
+    g.zob_a = sl.zob_d;
+    g.zob_b = sl.zob_d + (size_t)sl.sys->n_hm * 256;
+    for (int t = 0; t < g.n_tiles; ++t) atl.push_back(make_int2(i, t));
+    any_s |= sl.prm.delta_s > 0;
+  }
+  // couplings of the outside states (basis only)
+  std::vector<std::vector<int>> sc(L), hp(L), hs(L);
+  std::vector<std::vector<cplx>> hv(L);
+  b.pool->parallel_for(L, [&](int i) {
+    const Slot &sl = b.slots[list[i]];
+    sc[i] = list_sc(*sl.sys, sl.B);
+    sc_couplings(sl.B, sc[i], hp[i], hs[i], hv[i]);
+  });
+  std::vector<int32_t> scp;
+  std::vector<hops_sel_sc_t> sce;
+  for (int i = 0; i < L; ++i) {
+    seg[i].n_sc = (int)sc[i].size();
+    seg[i].sc_off = (int64_t)scp.size();
+    seg[i].sce_off = (int64_t)sce.size();
+    for (int v : hp[i]) scp.push_back(v);
+    for (size_t q = 0; q < hs[i].size(); ++q) sce.push_back(hops_sel_sc_t{hv[i][q].real(), hv[i][q].imag(), hs[i][q], 0});
+  }
+  const std::vector<uint32_t> tiles = tiles_of(b, list);
+  size_t tmp_err = 0;
+  RT_TRY(hops_sel_sort_keys(nullptr, &tmp_err, nullptr, nullptr, htot, L, nullptr, nullptr, b.stream));
+  // ---- region 0 ------------------------------------------------------------------
+  struct R0 {
+    int32_t *list;
+    uint32_t *tiles;
+    int64_t *off, *eoff, *poff, *ebeg, *eend, *gbeg, *gend;
+    double *pn, *e, *psi;
+    hops_sel_seg_t *segs;
+    int2 *atl;
+    hops_sel_mode_t *modes;
+    int32_t *scp;
+    hops_sel_sc_t *sce;
+    double *err, *err_sorted, *part_u, *part_c, *c1, *scal;
+    uint8_t *flag;
+    int32_t *cnt, *scnt;
+    int64_t *tot;
+    void *tmp;
+  } a{};
+  auto lay0 = [&](Lay &l) {
+    a.list = l.take<int32_t>(L);
+    a.tiles = l.take<uint32_t>(tiles.size());
+    a.off = l.take<int64_t>(S);
+    a.eoff = l.take<int64_t>(S);
+    a.poff = l.take<int64_t>(L);
+    a.ebeg = l.take<int64_t>(L);
+    a.eend = l.take<int64_t>(L);
+    a.gbeg = l.take<int64_t>(L);
+    a.gend = l.take<int64_t>(L);
+    a.pn = l.take<double>(2 * tot);
+    a.e = l.take<double>(etot);
+    a.psi = l.take<double>(2 * stot);
+    a.segs = l.take<hops_sel_seg_t>(L);
+    a.atl = l.take<int2>(atl.size());
+    a.modes = l.take<hops_sel_mode_t>(mtot);
+    a.scp = l.take<int32_t>(scp.size());
+    a.sce = l.take<hops_sel_sc_t>(sce.size());
+    a.err = l.take<double>(htot);
+    a.err_sorted = l.take<double>(htot);
+    a.part_u = l.take<double>(ttot);
+    a.part_c = l.take<double>(tctot);
+    a.c1 = l.take<double>(stot);
+    a.scal = l.take<double>(8 * (size_t)L);
+    a.flag = l.take<uint8_t>(htot);
+    a.cnt = l.take<int32_t>(htot);
+    a.scnt = l.take<int32_t>(htot);
+    a.tot = l.take<int64_t>(8 * (size_t)L);
+    a.tmp = l.take<char>(tmp_err);
+  };
+  {
+    Lay l;
+    lay0(l);
+    int rc = ensure_sel(b, 0, l.cur);
+    if (rc == HOPS_RT_ARENA_FULL) return rc;
+    RT_TRY(rc);
+    Lay l2;
+    l2.base = b.arena.ptr(b.sel_off[0]);
+    lay0(l2);
+  }
+  RT_TRY(ensure_staging(b, std::max({vbytes(seg) + vbytes(atl) + vbytes(scp) + vbytes(sce) + tiles.size() * 4 +
+                                         S * 16 + (size_t)L * 48 + mtot * sizeof(hops_sel_mode_t),
+                                     stot * 16 + S * 8, htot + (size_t)L * 160}) +
+                            (size_t)L * 4096 + 65536));
+  hops_sel_work_t w{};
+  w.ops = d_ops(b);
+  w.segs = a.segs;
+  w.pn = a.pn;
+  w.e_aux = a.e;
+  w.modes = a.modes;
+  w.sc_ptr = a.scp;
+  w.sc_ent = a.sce;
+  w.err = a.err;
+  w.err_sorted = a.err_sorted;
+  w.flag = a.flag;
+  w.cnt = a.cnt;
+  w.scnt = a.scnt;
+  w.part_u = a.part_u;
+  w.part_c = a.part_c;
+  w.c1 = a.c1;
+  w.scal = a.scal;
+  w.tot = a.tot;
+  w.seg_beg = a.gbeg;
+  w.seg_end = a.gend;
+  w.hbar = kHbar;
+  w.n_list = L;
+  Xfer x{b};
+  // ---- phase A: Pn, deletion-flux sums, psi --------------------------------------
+  for (int s : list) b.its[s] = b.slots[s].it;
+  RT_TRY(upload_desc(b));
+  RT_TRY(x.put(a.list, list.data(), list.size() * 4));
+  RT_TRY(x.put(a.tiles, tiles.data(), tiles.size() * 4));
+  RT_TRY(x.put(a.off, off.data(), S * 8));
+  RT_TRY(x.put(a.eoff, eoff.data(), S * 8));
+  RT_TRY(x.put(a.poff, poff.data(), (size_t)L * 8));
+  RT_TRY(x.put(a.ebeg, ebeg.data(), (size_t)L * 8));
+  RT_TRY(x.put(a.eend, eend.data(), (size_t)L * 8));
+  RT_TRY(x.put(a.segs, seg.data(), vbytes(seg)));
+  RT_TRY(x.put(a.atl, atl.data(), vbytes(atl)));
+  RT_TRY(x.put(a.scp, scp.data(), vbytes(scp)));
+  RT_TRY(x.put(a.sce, sce.data(), vbytes(sce)));
+  double *dnrm = reinterpret_cast<double *>(b.arena.ptr(b.nrm_off));
+  const int shm = shm_of(b, list);
+  RT_TRY(hops_slots_pn(d_ops(b), d_bts(b), a.list, L, a.off, a.pn, dnrm, b.precision, 0, b.stream));
+  RT_TRY(hops_slots_errors(d_ops(b), d_bts(b), a.list, L, a.tiles, (int)tiles.size(),
+                           (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm, a.off, a.pn, nullptr, nullptr,
+                           a.eoff, a.e, (const double *)b.arena.ptr(b.dt_off), kHbar, b.stream));
+  RT_TRY(hops_sel_gather_psi(w, a.psi, a.poff, b.stream));
+  char *hpsi, *hnrm;
+  RT_TRY(x.get(a.psi, stot * 16, &hpsi));
+  RT_TRY(x.get(dnrm, S * 8, &hnrm));
+  RT_TRY(x.sync());
+  std::vector<hops_sel_mode_t> modes(mtot);
+  const cplx *psi = reinterpret_cast<const cplx *>(hpsi);
+  for (int i = 0; i < L; ++i) {
+    Slot &sl = b.slots[list[i]];
+    sl.pn.assign(psi + poff[i], psi + poff[i] + sl.B.n);
+    sl.nrm = reinterpret_cast<const double *>(hnrm)[list[i]];
+  }
+  auto t_h = std::chrono::steady_clock::now();
+  b.tm[1] += std::chrono::duration<double>(t_h - t_dev).count();
+  b.pool->parallel_for(L, [&](int i) {
+    Slot &sl = b.slots[list[i]];
+    sel_common(sl.B, sl.pn.data(), sl.sel);
+    sel_modes(sl.prm, sl.B, sl.sel, modes.data() + seg[i].m_off);
+  });
+  // ---- phase B: errors, stable set, candidate counts -------------------------------
+  RT_TRY(x.put(a.modes, modes.data(), vbytes(modes)));
+  const int2 *datl = a.atl;
+  RT_TRY(hops_sel_err(w, datl, (int)atl.size(), b.stream));
+  RT_TRY(hops_sel_sort_keys(a.tmp, &tmp_err, a.err, a.err_sorted, htot, L, a.ebeg, a.eend, b.stream));
+  RT_TRY(hops_sel_threshold(w, a.err_sorted, 0, b.stream));
+  RT_TRY(hops_sel_stable(w, datl, (int)atl.size(), any_s ? 1 : 0, b.stream));
+  RT_TRY(hops_sel_fin1(w, any_s ? 1 : 0, b.stream));
+  char *htt;
+  RT_TRY(x.get(a.tot, (size_t)L * 64, &htt));
+  RT_TRY(x.sync());
+  std::vector<int64_t> tt(reinterpret_cast<const int64_t *>(htt), reinterpret_cast<const int64_t *>(htt) + 8 * L);
+  // ---- phase C: candidates, pre-trim threshold, survivor counts ----------------------
+  size_t nztot = 0, nzt = 0;
+  std::vector<int2> ntl;
+  std::vector<int64_t> nzb(L), nze(L);
+  for (int i = 0; i < L; ++i) {
+    const int64_t nz = tt[8 * i + TT_NZ];
+    seg[i].nz_off = nzb[i] = (int64_t)nztot;
+    nztot += nz;
+    nze[i] = (int64_t)nztot;
+    seg[i].nzt_off = (int64_t)nzt;
+    const int64_t nt = (nz + 255) / 256;
+    nzt += nt;
+    for (int64_t t = 0; t < nt; ++t) ntl.push_back(make_int2(i, (int)t));
+  }
+  size_t tmp_nz = 0;
+  RT_TRY(hops_sel_sort_keys(nullptr, &tmp_nz, nullptr, nullptr, nztot, L, nullptr, nullptr, b.stream));
+  struct R1 {
+    double *nz, *nzs, *part_d;
+    int64_t *nzb, *nze;
+    int2 *ntl;
+    void *tmp;
+  } c{};
+  auto lay1 = [&](Lay &l) {
+    c.nz = l.take<double>(nztot);
+    c.nzs = l.take<double>(nztot);
+    c.part_d = l.take<double>(nzt);
+    c.nzb = l.take<int64_t>(L);
+    c.nze = l.take<int64_t>(L);
+    c.ntl = l.take<int2>(ntl.size());
+    c.tmp = l.take<char>(tmp_nz);
+  };
+  {
+    Lay l;
+    lay1(l);
+    int rc = ensure_sel(b, 1, l.cur);
+    if (rc == HOPS_RT_ARENA_FULL) return rc;
+    RT_TRY(rc);
+    Lay l2;
+    l2.base = b.arena.ptr(b.sel_off[1]);
+    lay1(l2);
+  }
+  w.nz = c.nz;
+  w.nz_sorted = c.nzs;
+  w.part_d = c.part_d;
+  RT_TRY(ensure_staging(b, vbytes(seg) + vbytes(ntl) + (size_t)L * 96 + 65536));
+  RT_TRY(x.put(a.segs, seg.data(), vbytes(seg)));
+  RT_TRY(x.put(c.nzb, nzb.data(), (size_t)L * 8));
+  RT_TRY(x.put(c.nze, nze.data(), (size_t)L * 8));
+  RT_TRY(x.put(c.ntl, ntl.data(), vbytes(ntl)));
+  RT_TRY(hops_sel_cand(w, datl, (int)atl.size(), b.stream));
+  if (nztot) RT_TRY(hops_sel_sort_keys(c.tmp, &tmp_nz, c.nz, c.nzs, nztot, L, c.nzb, c.nze, b.stream));
+  RT_TRY(hops_sel_threshold(w, c.nzs, 1, b.stream));
+  RT_TRY(hops_sel_disc(w, c.ntl, (int)ntl.size(), datl, (int)atl.size(), b.stream));
+  RT_TRY(hops_sel_fin2(w, b.stream));
+  RT_TRY(x.get(a.tot, (size_t)L * 64, &htt));
+  RT_TRY(x.sync());
+  tt.assign(reinterpret_cast<const int64_t *>(htt), reinterpret_cast<const int64_t *>(htt) + 8 * L);
+  // ---- phase D: survivors, binning, boundary threshold -------------------------------
+  size_t svtot = 0;
+  std::vector<int2> stl;
+  std::vector<int64_t> svb(L), sve(L);
+  for (int i = 0; i < L; ++i) {
+    const int64_t nsv = tt[8 * i + TT_SURV];
+    seg[i].sv_off = svb[i] = (int64_t)svtot;
+    svtot += nsv;
+    sve[i] = (int64_t)svtot;
+    for (int64_t t = 0; t < (nsv + 255) / 256; ++t) stl.push_back(make_int2(i, (int)t));
+  }
+  size_t tmp_sv = 0, tmp_g = 0;
+  RT_TRY(hops_sel_sort_pairs(nullptr, &tmp_sv, nullptr, nullptr, nullptr, nullptr, svtot, L, nullptr, nullptr,
+                             b.stream));
+  RT_TRY(hops_sel_sort_keys(nullptr, &tmp_g, nullptr, nullptr, svtot, L, nullptr, nullptr, b.stream));
+  struct R2 {
+    uint64_t *ka, *kas, *kb, *desc, *gdesc, *bound;
+    uint32_t *pos, *poss;
+    double *sval, *gv, *gvals, *gs;
+    uint8_t *lead;
+    int64_t *svb, *sve, *boff;
+    int2 *stl;
+    void *tmp;
+  } d{};
+  auto lay2 = [&](Lay &l) {
+    d.ka = l.take<uint64_t>(svtot);
+    d.kas = l.take<uint64_t>(svtot);
+    d.kb = l.take<uint64_t>(svtot);
+    d.desc = l.take<uint64_t>(svtot);
+    d.gdesc = l.take<uint64_t>(svtot);
+    d.bound = l.take<uint64_t>(svtot);
+    d.pos = l.take<uint32_t>(svtot);
+    d.poss = l.take<uint32_t>(svtot);
+    d.sval = l.take<double>(svtot);
+    d.gv = l.take<double>(svtot);
+    d.gvals = l.take<double>(svtot);
+    d.gs = l.take<double>(svtot);
+    d.lead = l.take<uint8_t>(svtot);
+    d.svb = l.take<int64_t>(L);
+    d.sve = l.take<int64_t>(L);
+    d.boff = l.take<int64_t>((size_t)L + 1);
+    d.stl = l.take<int2>(stl.size());
+    d.tmp = l.take<char>(std::max(tmp_sv, tmp_g));
+  };
+  {
+    Lay l;
+    lay2(l);
+    int rc = ensure_sel(b, 2, l.cur);
+    if (rc == HOPS_RT_ARENA_FULL) return rc;
+    RT_TRY(rc);
+    Lay l2;
+    l2.base = b.arena.ptr(b.sel_off[2]);
+    lay2(l2);
+  }
+  w.ka = d.ka;
+  w.ka_sorted = d.kas;
+  w.kb = d.kb;
+  w.desc = d.desc;
+  w.pos = d.pos;
+  w.pos_sorted = d.poss;
+  w.sval = d.sval;
+  w.lead = d.lead;
+  w.gv = d.gv;
+  w.gvals = d.gvals;
+  w.gsorted = d.gs;
+  w.gdesc = d.gdesc;
+  w.bound = d.bound;
+  w.bound_off = d.boff;
+  RT_TRY(ensure_staging(b, std::max(vbytes(seg) + vbytes(stl) + (size_t)L * 96, htot + (size_t)L * 200) + 65536));
+  RT_TRY(x.put(a.segs, seg.data(), vbytes(seg)));
+  RT_TRY(x.put(d.svb, svb.data(), (size_t)L * 8));
+  RT_TRY(x.put(d.sve, sve.data(), (size_t)L * 8));
+  RT_TRY(x.put(d.stl, stl.data(), vbytes(stl)));
+  RT_TRY(hops_sel_surv(w, datl, (int)atl.size(), b.stream));
+  if (svtot) {
+    size_t tb = std::max(tmp_sv, tmp_g);
+    RT_TRY(hops_sel_sort_pairs(d.tmp, &tb, d.ka, d.kas, d.pos, d.poss, svtot, L, d.svb, d.sve, b.stream));
+  }
+  RT_TRY(hops_sel_group(w, d.stl, (int)stl.size(), b.stream));
+  RT_TRY(hops_sel_fin3(w, b.stream));
+  if (svtot) {
+    size_t tb = std::max(tmp_sv, tmp_g);
+    RT_TRY(hops_sel_sort_keys(d.tmp, &tb, d.gvals, d.gs, svtot, L, a.gbeg, a.gend, b.stream));
+  }
+  RT_TRY(hops_sel_threshold(w, d.gs, 2, b.stream));
+  RT_TRY(hops_sel_bound(w, b.stream));
+  char *hboff, *hflag;
+  RT_TRY(x.get(d.boff, ((size_t)L + 1) * 8, &hboff));
+  RT_TRY(x.get(a.flag, htot, &hflag));
+  RT_TRY(x.sync());
+  const std::vector<int64_t> boff(reinterpret_cast<const int64_t *>(hboff),
+                                  reinterpret_cast<const int64_t *>(hboff) + L + 1);
+  std::vector<uint8_t> flags(reinterpret_cast<const uint8_t *>(hflag), reinterpret_cast<const uint8_t *>(hflag) + htot);
+  RT_TRY(ensure_staging(b, (size_t)boff[L] * 8 + 65536));
+  char *hb;
+  RT_TRY(x.get(d.bound, (size_t)boff[L] * 8, &hb));
+  RT_TRY(x.sync());
+  const uint64_t *bd = reinterpret_cast<const uint64_t *>(hb);
+  b.pool->parallel_for(L, [&](int i) {
+    Slot &sl = b.slots[list[i]];
+    const Basis &B = sl.B;
+    Selection &out = sl.sel;
+    out.stable_mask.assign(flags.begin() + seg[i].h_off, flags.begin() + seg[i].h_off + B.nh);
+    out.stable_idx.clear();
+    for (int k = 0; k < B.nh; ++k)
+      if (out.stable_mask[k]) out.stable_idx.push_back(k);
+    out.aux_new.clear();
+    out.aux_new.reserve(out.stable_idx.size() + (size_t)(boff[i + 1] - boff[i]));
+    for (int k : out.stable_idx) out.aux_new.push_back(B.aux[k]);
+    out.aux_bound.clear();
+    Aux tgt;
+    for (int64_t q = boff[i]; q < boff[i + 1]; ++q) {
+      const uint64_t dsc = bd[q];
+      const int k = (int)(dsc & 0xffffffffu), lm = (int)((dsc >> 32) & 0xffff);
+      aux_step(B.aux[k], B.modes[lm], (dsc >> 48) & 1 ? 1 : -1, tgt);
+      out.aux_bound.push_back(tgt);
+      out.aux_new.push_back(tgt);
+    }
+  });
+  auto t_s = std::chrono::steady_clock::now();
+  b.tm[2] += std::chrono::duration<double>(t_s - t_h).count();
+  // ---- state half ------------------------------------------------------------------
+  Timer tmr_s(&b.tm[3]);
+  if (any_s) {
+    std::vector<int> ls, li;  // slots, list indices
+    for (int i = 0; i < L; ++i)
+      if (b.slots[list[i]].prm.delta_s > 0) {
+        ls.push_back(list[i]);
+        li.push_back(i);
+      }
+    std::vector<int64_t> moff(S, 0), soff(S, 0);
+    for (int i : li) {
+      moff[list[i]] = seg[i].h_off;
+      soff[list[i]] = seg[i].s_off;
+    }
+    std::vector<std::vector<uint32_t>> fls(L);
+    std::vector<double> v1(stot, 0.0);
+    b.pool->parallel_for((int)li.size(), [&](int q) {
+      const int i = li[q];
+      const Slot &sl = b.slots[list[i]];
+      const Basis &B = sl.B;
+      const System &Sy = *B.sys;
+      state_flag_list(B, sl.sel, fls[i]);
+      for (int s = 0; s < B.n; ++s) {
+        double v = 0;
+        for (int sp = 0; sp < Sy.n; ++sp)
+          if (sp != B.states[s]) v += std::norm(Sy.H[(size_t)sp * Sy.n + B.states[s]]);
+        v1[seg[i].s_off + s] = v;
+      }
+    });
+    std::vector<uint32_t> flat;
+    for (int i = 0; i < L; ++i) {
+      seg[i].fl_off = (int64_t)flat.size() / 2;
+      seg[i].n_fl = (int)fls[i].size() / 2;
+      flat.insert(flat.end(), fls[i].begin(), fls[i].end());
+    }
+    const std::vector<uint32_t> stiles = tiles_of(b, ls);
+    struct R3 {
+      int32_t *ls;
+      uint32_t *stiles;
+      int64_t *moff, *soff;
+      double *es, *v1, *err_s;
+      uint32_t *fl;
+    } e{};
+    auto lay3 = [&](Lay &l) {
+      e.ls = l.take<int32_t>(ls.size());
+      e.stiles = l.take<uint32_t>(stiles.size());
+      e.moff = l.take<int64_t>(S);
+      e.soff = l.take<int64_t>(S);
+      e.es = l.take<double>(stot);
+      e.v1 = l.take<double>(stot);
+      e.err_s = l.take<double>(stot);
+      e.fl = l.take<uint32_t>(flat.size());
+    };
+    {
+      Lay l;
+      lay3(l);
+      int rc = ensure_sel(b, 3, l.cur);
+      if (rc == HOPS_RT_ARENA_FULL) return rc;
+      RT_TRY(rc);
+      Lay l2;
+      l2.base = b.arena.ptr(b.sel_off[3]);
+      lay3(l2);
+    }
+    RT_TRY(ensure_staging(b, vbytes(seg) + vbytes(flat) + stot * 24 + S * 16 + stiles.size() * 4 +
+                                 ls.size() * 4 + 65536));
+    RT_TRY(x.put(a.segs, seg.data(), vbytes(seg)));
+    RT_TRY(x.put(e.ls, ls.data(), ls.size() * 4));
+    RT_TRY(x.put(e.stiles, stiles.data(), stiles.size() * 4));
+    RT_TRY(x.put(e.moff, moff.data(), S * 8));
+    RT_TRY(x.put(e.soff, soff.data(), S * 8));
+    RT_TRY(x.put(e.v1, v1.data(), vbytes(v1)));
+    RT_TRY(x.put(e.fl, flat.data(), vbytes(flat)));
+    RT_TRY(hops_slots_errors(d_ops(b), d_bts(b), e.ls, (int)ls.size(), e.stiles, (int)stiles.size(),
+                             (const int32_t *)b.arena.ptr(b.it_off), b.precision, shm_of(b, ls), a.off, a.pn, e.moff,
+                             a.flag, e.soff, e.es, (const double *)b.arena.ptr(b.dt_off), kHbar, b.stream));
+    w.fl = e.fl;
+    w.err_s = e.err_s;
+    w.e_state = e.es;
+    w.v1 = e.v1;
+    RT_TRY(hops_sel_state_err(w, b.stream));
+    char *hes;
+    RT_TRY(x.get(e.err_s, stot * 8, &hes));
+    RT_TRY(x.sync());
+    const std::vector<double> err_s(reinterpret_cast<const double *>(hes), reinterpret_cast<const double *>(hes) + stot);
+    // state thresholds (host: O(n_state) per slot), then the flux into the
+    // outside coupled states from the stable states
+    std::vector<std::vector<int>> rel_st(L), st_stable(L);
+    std::vector<double> dbound(L, 0.0);
+    std::vector<std::vector<hops_sel_sc_t>> qe(L);
+    std::vector<int> nq(L, 0), nsc(L, 0);
+    b.pool->parallel_for((int)li.size(), [&](int q) {
+      const int i = li[q];
+      const Slot &sl = b.slots[list[i]];
+      const Basis &B = sl.B;
+      const System &Sy = *B.sys;
+      const int n = B.n;
+      std::vector<double> srt(err_s.begin() + seg[i].s_off, err_s.begin() + seg[i].s_off + n);
+      const std::vector<double> er(srt);
+      std::sort(srt.begin(), srt.end());
+      const double dsq = sl.prm.delta_s * sl.prm.delta_s;
+      const double thr = error_threshold(srt, dsq / 2.0);
+      double unstable = 0;
+      for (int s = 0; s < n; ++s) {
+        if (er[s] > thr) {
+          rel_st[i].push_back(s);
+          st_stable[i].push_back(B.states[s]);
+        } else {
+          unstable += er[s];
+        }
+      }
+      dbound[i] = dsq - unstable;
+      if ((int)rel_st[i].size() < Sy.n && !sl.sel.sc.empty()) {
+        std::vector<std::vector<hops_sel_sc_t>> per(sl.sel.sc.size());
+        size_t mx = 1;
+        for (size_t ii = 0; ii < sl.sel.sc.size(); ++ii) {
+          const int sp = sl.sel.sc[ii];
+          for (size_t qq = 0; qq < rel_st[i].size(); ++qq) {
+            const cplx h = Sy.H[(size_t)sp * Sy.n + st_stable[i][qq]];
+            if (h != cplx(0, 0)) {
+              const cplx mh = cmi(h);
+              per[ii].push_back(hops_sel_sc_t{mh.real(), mh.imag(), rel_st[i][qq], 0});
+            }
+          }
+          mx = std::max(mx, per[ii].size());
+        }
+        nq[i] = (int)mx;
+        nsc[i] = (int)sl.sel.sc.size();
+        qe[i].assign(sl.sel.sc.size() * mx, hops_sel_sc_t{0, 0, -1, 0});
+        for (size_t ii = 0; ii < per.size(); ++ii)
+          std::copy(per[ii].begin(), per[ii].end(), qe[i].begin() + ii * mx);
+      }
+    });
+    std::vector<hops_sel_sc_t> qflat;
+    size_t tetot = 0, esctot = 0;
+    for (int i = 0; i < L; ++i) {
+      seg[i].n_sc = nsc[i];
+      seg[i].n_q = nq[i];
+      seg[i].q_off = (int64_t)qflat.size();
+      qflat.insert(qflat.end(), qe[i].begin(), qe[i].end());
+      seg[i].te_off = (int64_t)tetot;
+      tetot += (size_t)seg[i].n_tiles * nsc[i];
+      seg[i].esc_off = (int64_t)esctot;
+      esctot += nsc[i];
+    }
+    if (esctot) {
+      struct R4 {
+        hops_sel_sc_t *q;
+        double *pe, *esc;
+      } f{};
+      auto lay4 = [&](Lay &l) {
+        f.q = l.take<hops_sel_sc_t>(qflat.size());
+        f.pe = l.take<double>(tetot);
+        f.esc = l.take<double>(esctot);
+      };
+      // region 3 holds the state arrays above; region 1 (candidates) is
+      // free again and takes these
+      {
+        Lay l;
+        lay4(l);
+        int rc = ensure_sel(b, 1, l.cur);
+        if (rc == HOPS_RT_ARENA_FULL) return rc;
+        RT_TRY(rc);
+        Lay l2;
+        l2.base = b.arena.ptr(b.sel_off[1]);
+        lay4(l2);
+      }
+      RT_TRY(ensure_staging(b, vbytes(seg) + vbytes(qflat) + esctot * 8 + 65536));
+      RT_TRY(x.put(a.segs, seg.data(), vbytes(seg)));
+      RT_TRY(x.put(f.q, qflat.data(), vbytes(qflat)));
+      w.q_ent = f.q;
+      w.part_e = f.pe;
+      w.e_sc = f.esc;
+      RT_TRY(hops_sel_state_flux(w, datl, (int)atl.size(), b.stream));
+      char *hesc;
+      RT_TRY(x.get(f.esc, esctot * 8, &hesc));
+      RT_TRY(x.sync());
+      const double *Esc = reinterpret_cast<const double *>(hesc);
+      for (int i : li) {
+        Slot &sl = b.slots[list[i]];
+        std::vector<int> bound;
+        if (nsc[i]) {
+          std::vector<double> ev;
+          std::vector<int> es;
+          for (int ii = 0; ii < nsc[i]; ++ii)
+            if (Esc[seg[i].esc_off + ii] != 0) {
+              ev.push_back(Esc[seg[i].esc_off + ii]);
+              es.push_back(sl.sel.sc[ii]);
+            }
+          if (!ev.empty()) {
+            std::vector<double> s2(ev);
+            std::sort(s2.begin(), s2.end());
+            const double t2 = error_threshold(s2, dbound[i]);
+            for (size_t q = 0; q < ev.size(); ++q)
+              if (ev[q] > t2) bound.push_back(es[q]);
+          }
+        }
+        std::vector<int> res(st_stable[i]);
+        for (int bs : bound)
+          if (std::find(sl.B.states.begin(), sl.B.states.end(), bs) == sl.B.states.end()) res.push_back(bs);
+        std::sort(res.begin(), res.end());
+        res.erase(std::unique(res.begin(), res.end()), res.end());
+        sl.sel.states_new = res;
+      }
+    } else {
+      for (int i : li) {
+        std::vector<int> res(st_stable[i]);
+        std::sort(res.begin(), res.end());
+        res.erase(std::unique(res.begin(), res.end()), res.end());
+        b.slots[list[i]].sel.states_new = res;
+      }
+    }
+  }
+  for (int s : list) {
+    Slot &sl = b.slots[s];
+    if (sl.prm.delta_s <= 0) sl.sel.states_new = sl.B.states;
+  }
+  return 0;
+}
+
+// define_basis for the listed slots: on the device where the slot's
+// selection runs there, on the host threads otherwise (and for every slot
+// when the arena cannot hold the device work space)
+int define_bases(Batch &b, const std::vector<int> &list) {
+  if (list.empty()) return 0;
+  std::vector<int> dev, host;
+  if (b.sel_device)
+    for (int s : list) RT_TRY(ensure_zob(b, b.slots[s]));
+  for (int s : list) (b.sel_device && dev_eligible(b.slots[s]) ? dev : host).push_back(s);
+  if (!dev.empty()) {
+    const int rc = define_bases_dev(b, dev);
+    if (rc == HOPS_RT_ARENA_FULL)
+      host.insert(host.end(), dev.begin(), dev.end());
+    else
+      RT_TRY(rc);
+  }
+  return define_bases_host(b, host);
 }
 
 bool same_basis(const Basis &B, const std::vector<int> &states, const std::vector<Aux> &aux) {
@@ -2347,6 +3161,17 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
 // kernel when they are freed, and trims the heap top above its trim
 // threshold, so every step pays the page faults of those buffers again;
 // raising both keeps freed blocks in the process heap for reuse.
+// drop a slot's use of its system's device Zobrist tables
+void release_zob(Batch &b, Slot &sl) {
+  if (!sl.zob_d) return;
+  auto it = b.zob.find(sl.sys);
+  if (it != b.zob.end() && --it->second.second == 0) {
+    b.arena.release(it->second.first);
+    b.zob.erase(it);
+  }
+  sl.zob_d = nullptr;
+}
+
 void keep_heap() {
   static std::once_flag once;
   std::call_once(once, [] {
@@ -2553,6 +3378,7 @@ int hops_rt_batch_add(void *p, void *sys, const hops_rt_traj_param_t *prm, const
   RT_TRY(build_basis(S, P.eom, states, aux, sl.B));
   sl.pers = b.arena.alloc(pers_bytes(S));
   if (sl.pers < 0) return rt_fail("arena full");
+
   std::vector<char> tabs;
   hops_op_t op;
   hops_batch_t bt;
@@ -2592,6 +3418,7 @@ int hops_rt_batch_remove(void *p, int slot) {
   Slot &sl = b.slots[slot];
   b.arena.release(sl.blk);
   b.arena.release(sl.pers);
+  release_zob(b, sl);
   sl = Slot();
   b.ops[slot] = hops_op_t{};
   b.bts[slot] = hops_batch_t{};
@@ -2606,6 +3433,17 @@ long long hops_rt_slot_estimate(void *p, void *sys, int n_state, int n_hier) {
   const size_t ell_w = 2 * (size_t)std::max(1, std::min(S.n_hm, 2 * n_state));  // up + down per row
   const size_t tables = (size_t)n_hier * (16 + 8 * ell_w) + (size_t)n_state * n_state * 48 + 4096;
   return (long long)(pers_bytes(S) + tables + ell_w * nr * (4 + es) + nr * (16 * 2 + 3 * es + 8) + n_hier * 16);
+}
+
+int hops_rt_batch_option(void *p, const char *name, long long value) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (!name) return rt_fail("hops_rt_batch_option: NULL name");
+  if (std::strcmp(name, "device_select") == 0) {
+    b.sel_device = value != 0;
+    return 0;
+  }
+  return rt_fail("unknown batch option '%s'", name);
 }
 
 int hops_rt_batch_timers(void *p, double *out) {

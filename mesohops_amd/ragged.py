@@ -192,6 +192,12 @@ class RaggedBatch:
             raise EngineError(lib().hops_last_error().decode())
         self._keep = {}
 
+    def set_option(self, name, value):
+        """Runtime option: "device_select" 1 selects the adaptive bases on the
+        device (experimental: verified in a CPU emulation of its kernels, not
+        yet on the GPU), 0 (default) on the host threads."""
+        check(lib().hops_rt_batch_option(self.h, name.encode(), int(value)))
+
     def add(self, system: RtSystem, prm: RtTrajParam, psi0, z1, z2=None):
         psi = np.ascontiguousarray(np.asarray(psi0, dtype=np.complex128))
         rc = lib().hops_rt_batch_add(self.h, system.handle, ctypes.byref(prm), _ptr(psi),
@@ -307,7 +313,8 @@ class AdaptiveBatch:
 
     def __init__(self, system_param, eom_param=None, noise_param=None, noise2_param=None,
                  hierarchy_param=None, integration_param=None, seeds=None, noise_params=None,
-                 precision_mode="fp64", device="cuda:0", arena_bytes=None, n_threads=16, record=True):
+                 precision_mode="fp64", device="cuda:0", arena_bytes=None, n_threads=16, record=True,
+                 device_select=False):
         from .trajectory import _Setup
         self.setup = _Setup(system_param, eom_param, noise_param or {"TLEN": 1000.0, "TAU": 1.0}, noise2_param,
                             hierarchy_param, integration_param)
@@ -334,6 +341,7 @@ class AdaptiveBatch:
         self.arena_bytes = arena_bytes
         self.n_threads = n_threads
         self.record = record
+        self.device_select = bool(device_select)
         self.delta_a = self.delta_s = 0.0
         self.update_step = 1
         self.f_discard = 0.0
@@ -360,6 +368,7 @@ class AdaptiveBatch:
         self.system = RtSystem(info)
         self.rt = RaggedBatch(self.precision_mode, self.arena_bytes, max_slots=max(self.B, 1),
                               n_threads=self.n_threads, device=self.device)
+        self.rt.set_option("device_select", self.device_select)
         sp = self.rt.stream
         if self.noise1[0].interpolate or self.noise2[0].interpolate:
             raise UnsupportedRequest("INTERPOLATE noise", "AdaptiveBatch")

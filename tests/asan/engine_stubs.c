@@ -20,3 +20,19 @@ int hops_slots_errors(void) { return -1; }
 int hops_slots_build(void) { return -1; }
 int hops_slots_permute(void) { return -1; }
 int hops_copy_segments(void) { return -1; }
+int hops_sel_err(void) { return -1; }
+int hops_sel_stable(void) { return -1; }
+int hops_sel_fin1(void) { return -1; }
+int hops_sel_cand(void) { return -1; }
+int hops_sel_disc(void) { return -1; }
+int hops_sel_fin2(void) { return -1; }
+int hops_sel_surv(void) { return -1; }
+int hops_sel_group(void) { return -1; }
+int hops_sel_fin3(void) { return -1; }
+int hops_sel_bound(void) { return -1; }
+int hops_sel_threshold(void) { return -1; }
+int hops_sel_sort_keys(void) { return -1; }
+int hops_sel_sort_pairs(void) { return -1; }
+int hops_sel_state_err(void) { return -1; }
+int hops_sel_state_flux(void) { return -1; }
+int hops_sel_gather_psi(void) { return -1; }
