@@ -89,6 +89,15 @@ typedef struct hops_op {
   int32_t *ell_col_d; /* [ell_w * n_rows], column-major            */
   void *ell_val_d;    /* [ell_w * n_rows] complex, fp64 or fp32     */
   double *k0_d;       /* [n_hier] complex: -sum_m k_m w_m           */
+  /* dense system block (no counterpart in the reference's tables: the
+     reference multiplies (-1j*H) @ Phi, eom/hops_eom.py:389-391, 526).  NULL:
+     -iH is in the pattern above.  Otherwise -iH over the state basis is
+     applied on the matrix cores (hops_dense_h) and the pattern holds the L
+     and L^2 entries only: two planes (real, imaginary) of hd_n x hd_n values
+     in stage precision, row-major, zero beyond n_state; hd_n a multiple of
+     16. */
+  const void *hd_d;
+  int32_t hd_n, hd_pad;
 } hops_op_t;
 
 /*
@@ -134,6 +143,9 @@ typedef struct hops_batch {
      512 >> lane_shift rows (a permutation of the tiles), or NULL for tile
      order; any order gives the same results */
   const int32_t *tile_order_d;
+  /* (-iH) Y of the current stage input (complex R, the pool layout), when
+     the operator has a dense system block (hd_d), else NULL */
+  void *hy_d;
 } hops_batch_t;
 
 const char *hops_last_error(void);

@@ -16,7 +16,7 @@ HOPS_PREC_MIXED = 1
 HOPS_EOM_LINEAR = 0
 HOPS_EOM_NONLINEAR = 1
 HOPS_EOM_NORMALIZED_NONLINEAR = 2
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _i32 = ctypes.c_int32
 _vp = ctypes.c_void_p
@@ -39,7 +39,8 @@ _OP_PTRS = ["aux_ptr", "aux_mode", "aux_val",
 
 
 class HopsOp(ctypes.Structure):
-    _fields_ = [(n, _i32) for n in _OP_INTS] + [(n + "_d", _vp) for n in _OP_PTRS]
+    _fields_ = ([(n, _i32) for n in _OP_INTS] + [(n + "_d", _vp) for n in _OP_PTRS]
+                + [("hd_d", _vp), ("hd_n", _i32), ("hd_pad", _i32)])
 
 
 class HopsBatch(ctypes.Structure):
@@ -49,7 +50,8 @@ class HopsBatch(ctypes.Structure):
                 ("phi_d", _vp), ("y0_d", _vp), ("ya_d", _vp), ("yb_d", _vp), ("acc_d", _vp),
                 ("z1_d", _vp), ("z2_d", _vp), ("zmem_d", _vp), ("ws_d", _vp), ("coef_d", _vp),
                 ("psi_store_d", _vp), ("tile_d", _vp), ("n_tiles", _i32), ("tile_rows", _i32),
-                ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("noise_c128", _i32), ("tile_order_d", _vp)]
+                ("coefT_d", _vp), ("sigT_d", _vp), ("coefT_stride", _i32), ("noise_c128", _i32), ("tile_order_d", _vp),
+                ("hy_d", _vp)]
 
 
 class RtSystemDesc(ctypes.Structure):
@@ -180,8 +182,9 @@ def profile_enable(on=True):
 
 
 def profile_read():
-    """(ms, launches) for [prologue, stage, other] since profile_enable."""
-    ms = (ctypes.c_double * 3)()
-    n = (ctypes.c_longlong * 3)()
+    """(ms, launches) for [prologue, stage, other, dense system block] since
+    profile_enable."""
+    ms = (ctypes.c_double * 4)()
+    n = (ctypes.c_longlong * 4)()
     check(lib().hops_profile_read(ms, n))
     return list(ms), list(n)

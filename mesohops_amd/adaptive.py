@@ -408,7 +408,7 @@ class AdaptiveTrajectoryEngine:
 
     # -- basis objects -------------------------------------------------------
     def _make(self, states, aux):
-        tb = BasisTables(self.info, states, aux, self.setup.eom_kind)
+        tb = BasisTables(self.info, states, aux, self.setup.eom_kind, dense_h=False)
         dop = DeviceOperator(tb, self.prec, self.device)
         act = torch.as_tensor(tb.absL_active, dtype=torch.long, device=self.device)
         z1 = self._rows(self.z1_all, act)

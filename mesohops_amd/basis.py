@@ -119,12 +119,22 @@ def triangular_aux_list(n_hmodes, maxhier):
     return out
 
 
+# a state basis above this size whose H has more than this many nonzeros per
+# row on average gets the dense system block on the matrix cores
+DENSE_H_MIN_STATES = 64
+DENSE_H_MIN_ROW_NNZ = 8
+
+
 class BasisTables:
     """Relative indexing of one basis (state list x aux list) and the tables of
     hops_op_t (include/hops_mi355x.h).  Follows the relative/absolute index
     conventions of basis/hops_system.py:206-286 and basis/hops_modes.py:111-171."""
 
-    def __init__(self, info: SystemInfo, state_list, aux_list, eom_kind):
+    def __init__(self, info: SystemInfo, state_list, aux_list, eom_kind, dense_h=None):
+        """dense_h: apply -iH over the state basis as a dense block on the
+        matrix cores (True), in the sparse pattern (False), or the first when
+        the basis has more than DENSE_H_MIN_STATES states and H more than
+        DENSE_H_MIN_ROW_NNZ nonzeros per state row on average (None)."""
         self.info = info
         st = np.array(sorted(int(s) for s in state_list), dtype=np.int64)
         aux = sorted(set(aux_list), key=aux_order_key)
@@ -278,9 +288,15 @@ class BasisTables:
             j = index.get(((int(m), 1),))
             if j is not None and mode_lact[i] >= 0:
                 first.append((j, int(mode_lact[i]), i))
-        # system-block pattern: union of H, active L and (LTC) L^2
+        # system-block pattern: union of H, active L and (LTC) L^2; with a
+        # dense system block, of the L and L^2 entries only (-iH is applied
+        # as a dense matrix, hd)
         Hn = info.H[np.ix_(st, st)]
-        pat = sp.csr_matrix(np.abs(Hn) > 0, dtype=np.int8)
+        if dense_h is None:
+            dense_h = n > DENSE_H_MIN_STATES and np.count_nonzero(Hn) > DENSE_H_MIN_ROW_NNZ * n
+        self.dense_h = bool(dense_h)
+        self.hd = -1j * Hn if self.dense_h else None
+        pat = sp.csr_matrix((n, n), dtype=np.int8) if self.dense_h else sp.csr_matrix(np.abs(Hn) > 0, dtype=np.int8)
         for lr in act_lrel:
             pat = pat + sp.csr_matrix(abs(self.L[lr]) > 0, dtype=np.int8)
         if self.has_lt:
@@ -295,7 +311,7 @@ class BasisTables:
         for s in range(n):
             for p in range(pat_ptr[s], pat_ptr[s + 1]):
                 pidx[(s, int(pat_col[p]))] = p
-        pat_h = np.array([-1j * Hn[s, int(pat_col[p])] for s in range(n)
+        pat_h = np.array([0.0 if self.dense_h else -1j * Hn[s, int(pat_col[p])] for s in range(n)
                           for p in range(pat_ptr[s], pat_ptr[s + 1])], dtype=np.complex128)
         contrib = [[] for _ in range(len(pat_col))]
         for j, lr in enumerate(act_lrel):

@@ -644,7 +644,9 @@ __device__ __forceinline__ void stage_row(const hops_op_t &op, const hops_batch_
 
   const cx<R> yr = Y[r * T];
   cx<R> d = cmul(k0, yr);
-  // system block: -iH + noise / LTC terms
+  // dense system block: (-iH) Y from the matrix cores (k_dense_h)
+  if (bt.hy_d) d = cadd(d, reinterpret_cast<const cx<R> *>(bt.hy_d)[ln.base + r * T]);
+  // system block: -iH (when not dense) + noise / LTC terms
   const cx<R> *A = cT + (k == 0 ? (long)op.n_pat * cs : 0);
   const long rowk = (long)k * n;
   for (int p = op.pat_ptr_d[s]; p < op.pat_ptr_d[s + 1]; ++p)
@@ -737,6 +739,92 @@ __global__ __launch_bounds__(256) void k_stage(const hops_op_t op, const hops_ba
   if (r >= nr) return;
   const int k = (int)(r / n), s = (int)(r - (long)k * n);
   stage_row<R, 0>(op, bt, stage, yin, yout, b, r, k, s, ccast<R>(ld2(op.k0_d, k)), nullptr, nullptr);
+}
+
+// Dense system block on the matrix cores: HY = (-iH) Y for every auxiliary
+// and trajectory lane of a stage input (the reference's (-1j*H) @ Phi,
+// eom/hops_eom.py:389-391, 526).  For each trajectory group the product is
+// the GEMM C[s, j] = sum_s' A[s, s'] Y[(k, s'), t] over the columns
+// j = k * T + t.  A wave computes one 16 x 16 tile of C with the 16x16x4 MFMA
+// (f32 for complex64 stages, f64 for complex128), the complex product as four
+// real ones: Cr += Ar Yr + (-Ai) Yi, Ci += Ar Yi + Ai Yr.  Lane l supplies
+// A[i0 + l % 16][kk + l / 16] and Y[kk + l / 16][j0 + l % 16]; element v of its
+// accumulator is C[i0 + 4 (l / 16) + v][j0 + l % 16] for f32 and
+// C[i0 + l / 16 + 4 v][j0 + l % 16] for f64 (both measured on gfx950).  A
+// workgroup holds four waves, four row tiles of one column tile (the same Y
+// fragments, from L1).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename R> struct Acc4;
+template <> struct Acc4<float> {
+  typedef f32x4 T;
+  // accumulator row of element v of a lane in row quarter q = lane / 16
+  static __device__ __forceinline__ int row(int q, int v) { return 4 * q + v; }
+};
+template <> struct Acc4<double> {
+  typedef f64x4 T;
+  static __device__ __forceinline__ int row(int q, int v) { return q + 4 * v; }
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_h(const hops_op_t op, const hops_batch_t bt,
+                                                 const cx<R> *__restrict__ yin, cx<R> *__restrict__ hy) {
+  typedef typename Acc4<R>::T V4;
+  const int n = op.n_state, hn = op.hd_n, ls = bt.lane_shift;
+  const int T = 1 << ls;
+  const long nr = (long)n * op.n_hier;
+  const long ncol = (long)op.n_hier << ls;
+  const long n_ct = (ncol + 15) >> 4;
+  const int n_rt = hn >> 4, n_rq = (n_rt + 3) >> 2;
+  long blk = blockIdx.x;
+  const int rq = (int)(blk % n_rq);
+  blk /= n_rq;
+  const long ct = blk % n_ct;
+  const long g = blk / n_ct;
+  const int rt = rq * 4 + (int)(threadIdx.x >> 6);
+  if (rt >= n_rt) return;  // whole waves
+  const int l = threadIdx.x & 63;
+  const int i0 = rt << 4;
+  const long j = (ct << 4) + (l & 15);
+  const bool jv = j < ncol;
+  const long kaux = jv ? (j >> ls) : 0;
+  const int t = (int)(j & (T - 1));
+  const long cbase = (g * nr + kaux * n) * T + t;  // Y[(kaux, s'), t] at cbase + s' * T
+  const R *Ar = reinterpret_cast<const R *>(op.hd_d);
+  const R *Ai = Ar + (long)hn * hn;
+  const long arow = (long)(i0 + (l & 15)) * hn;
+  const int kq = l >> 4;
+  V4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int kk = 0; kk < hn; kk += 4) {
+    const int kc = kk + kq;
+    const R ar = Ar[arow + kc], ai = Ai[arow + kc];
+    cx<R> y = mk<R>((R)0, (R)0);
+    if (jv && kc < n) y = yin[cbase + (long)kc * T];
+    cr = mfma16(ar, y.x, cr);
+    cr = mfma16(-ai, y.y, cr);
+    ci = mfma16(ar, y.y, ci);
+    ci = mfma16(ai, y.x, ci);
+  }
+  if (!jv) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int s = i0 + Acc4<R>::row(kq, v);
+    if (s < n) hy[cbase + (long)s * T] = mk<R>(cr[v], ci[v]);
+  }
+}
+
+// grid of k_dense_h for a batch of G trajectory groups
+unsigned dense_grid(const hops_op_t &op, int G, int lane_shift) {
+  const long ncol = (long)op.n_hier << lane_shift;
+  const long n_ct = (ncol + 15) >> 4;
+  const int n_rq = ((op.hd_n >> 4) + 3) >> 2;
+  return (unsigned)((long)G * n_ct * n_rq);
 }
 
 // Resident form for small bases: one workgroup per trajectory runs every
@@ -1254,9 +1342,9 @@ __global__ __launch_bounds__(256) void k_final_scale(const hops_op_t op, const h
 struct Profile {
   bool on = false;
   std::vector<hipEvent_t> ev;
-  std::vector<int> kind;  // 0 prologue, 1 stage, 2 other
-  double ms[3] = {0, 0, 0};
-  long long n[3] = {0, 0, 0};
+  std::vector<int> kind;  // 0 prologue, 1 stage, 2 other, 3 dense system block
+  double ms[4] = {0, 0, 0, 0};
+  long long n[4] = {0, 0, 0, 0};
 };
 // per host thread: concurrent callers on other threads keep their own events
 // and counters
@@ -1389,6 +1477,8 @@ int check_op(const hops_op_t *op) {
   if (op->n_state <= 0 || op->n_hier <= 0) return fail("empty basis (n_state=%d n_hier=%d)", op->n_state, op->n_hier);
   if ((long)op->n_state * op->n_hier >= (1L << 31)) return fail("basis too large for int32 rows");
   if (op->ell_w < 0 || op->n_pat < 0 || op->n_act < 0) return fail("negative sizes");
+  if (op->hd_d && (op->hd_n < op->n_state || op->hd_n % 16 != 0))
+    return fail("dense system block of side %d for %d states (a multiple of 16 >= n_state)", op->hd_n, op->n_state);
   return 0;
 }
 
@@ -1414,6 +1504,8 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   const int ns = bt->noise_step;
   if (it0 < 0 || it0 + (long)n_steps * 2 * ns >= bt->n_t)
     return fail("noise index range [%d, %ld] outside [0, %d)", it0, it0 + (long)n_steps * 2 * ns, bt->n_t);
+  const bool dense = op->hd_d != nullptr;
+  if (dense != (bt->hy_d != nullptr)) return fail("dense system block needs its HY pool (and only it)");
   // scratch of the fused LINEAR prologue: the coefficient tables of stages 1..3
   const bool fuse = g_fuse_linear && bt->eom == HOPS_EOM_LINEAR;
   const size_t coefT_blk = sizeof(cx<R>) * (size_t)bt->coefT_stride *
@@ -1434,7 +1526,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     HIP_TRY(hipGetLastError());
   }
   const int toff[4] = {0, ns, ns, 2 * ns};
-  if (n_steps > 0 && nr <= g_resident_rows) {
+  if (n_steps > 0 && nr <= g_resident_rows && !dense) {
     size_t used = 0;
     if (int rc = prof_mark(prof, st, 2, &used)) return rc;
     k_steps_resident<R><<<bt->B, 256, shm, st>>>(*op, *bt, it0, n_steps, store0, Y0, ya, yb);
@@ -1469,7 +1561,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   while (vl > 1 && (vl > T || bt->B % vl != 0 || chunk % vl != 0)) vl >>= 1;
   // the stationary kernel addresses its pools with 32-bit byte offsets
   const bool pools32 = (double)nr * (double)((long)G << bt->lane_shift) * 16.0 < 4294967296.0;
-  if (g_stationary && pools32 && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
+  if (g_stationary && !dense && pools32 && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
     sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
     sh_p = max_pat <= 4 ? 4 : 8;
     const long rows_per_tile = (512L * vl) >> bt->lane_shift;
@@ -1582,6 +1674,12 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     else if (g_nt_stream == 4) HOPS_SHV(W, P, 4);                                                \
     else HOPS_SHV(W, P, 0);                                                                      \
   } while (0)
+      if (dense) {
+        k_dense_h<R><<<dense_grid(*op, G, sb.lane_shift), 256, 0, ls>>>(*op, sbi, yin,
+                                                                          reinterpret_cast<cx<R> *>(sbi.hy_d));
+        HIP_TRY(hipGetLastError());
+        if (int rc = prof_mark(prof, ls, 3, &used)) return rc;
+      }
       if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
       else if (sh_w == 4 && sh_p == 8) HOPS_SH(4, 8);
       else if (sh_w == 8 && sh_p == 4) HOPS_SH(8, 4);
@@ -1996,11 +2094,11 @@ int hops_internal_set_error(const char *msg) {
   return -1;
 }
 
-int hops_version(void) { return 3; }
+int hops_version(void) { return 4; }
 
 int hops_profile_enable(int on) {
   g_prof.on = on != 0;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     g_prof.ms[k] = 0.0;
     g_prof.n[k] = 0;
   }
@@ -2056,7 +2154,7 @@ int hops_set_option(const char *name, int value) {
 
 int hops_profile_read(double *ms_out, long long *n_out) {
   if (!ms_out || !n_out) return fail("hops_profile_read: NULL pointer");
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     ms_out[k] = g_prof.ms[k];
     n_out[k] = g_prof.n[k];
   }
@@ -2096,8 +2194,16 @@ int hops_derivative(const hops_op_t *op, const hops_batch_t *bt, const void *y_d
   if (shm > 64 * 1024)
     HIP_TRY(hipFuncSetAttribute((const void *)k_prologue<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   const cx<double> *y = reinterpret_cast<const cx<double> *>(y_d);
+  if ((op->hd_d != nullptr) != (bt->hy_d != nullptr))
+    return fail("dense system block needs its HY pool (and only it)");
   k_prologue<double><<<bt->B, 256, shm, st>>>(*op, *bt, -1, tidx, y, -1, nullptr);
   HIP_TRY(hipGetLastError());
+  if (op->hd_d) {
+    const int G = (bt->B + (1 << bt->lane_shift) - 1) >> bt->lane_shift;
+    k_dense_h<double><<<dense_grid(*op, G, bt->lane_shift), 256, 0, st>>>(*op, *bt, y,
+                                                                          reinterpret_cast<cx<double> *>(bt->hy_d));
+    HIP_TRY(hipGetLastError());
+  }
   k_stage<double><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, -1, y, reinterpret_cast<cx<double> *>(d_d));
   HIP_TRY(hipGetLastError());
   return 0;

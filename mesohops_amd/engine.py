@@ -72,6 +72,17 @@ class DeviceOperator:
         op.ell_col_d = self.ell_col.data_ptr()
         op.ell_val_d = self.ell_val.data_ptr()
         op.k0_d = self.k0.data_ptr()
+        self.hd = None
+        if getattr(tables, "hd", None) is not None:
+            # dense -iH: real and imaginary planes, zero-padded to a multiple of 16
+            n = tables.n_state
+            hn = ((n + 15) // 16) * 16
+            planes = np.zeros((2, hn, hn), dtype=np.float64 if precision == _lib.HOPS_PREC_FP64 else np.float32)
+            planes[0, :n, :n] = tables.hd.real
+            planes[1, :n, :n] = tables.hd.imag
+            self.hd = torch.from_numpy(planes).to(self.device)
+            op.hd_d = self.hd.data_ptr()
+            op.hd_n = hn
         self.op = op
         with torch.cuda.device(self.device):
             check(lib().hops_build_super(ctypes.byref(op), precision, _stream_ptr(self.device)))
@@ -164,6 +175,8 @@ class BatchState:
         self.ya = torch.empty(P, dtype=cR, device=dev)
         self.yb = torch.empty(P, dtype=cR, device=dev)
         self.acc = torch.empty(P, dtype=cR, device=dev)
+        # (-iH) Y of each stage input when the operator has a dense system block
+        self.hy = torch.empty(P, dtype=cR, device=dev) if dop.op.hd_d else None
         if noise1.shape != (B, t.n_act, n_t):
             raise EngineError("noise1 shape %s != %s" % (tuple(noise1.shape), (B, t.n_act, n_t)))
         self.z1 = noise1.contiguous()
@@ -199,6 +212,7 @@ class BatchState:
         bt.ya_d = self.ya.data_ptr()
         bt.yb_d = self.yb.data_ptr()
         bt.acc_d = self.acc.data_ptr()
+        bt.hy_d = self.hy.data_ptr() if self.hy is not None else None
         bt.z1_d = self.z1.data_ptr()
         bt.z2_d = None if self.z2 is None else self.z2.data_ptr()
         bt.zmem_d = self.zmem.data_ptr()

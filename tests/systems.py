@@ -118,3 +118,22 @@ def lattice(nx, ny, V=40.0, sigma=150.0, seed_h=0, lam=35.0, gam=50.0, temp=77.0
     return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=lln, ALPHA_NOISE1=O.bcf_exp,
                 PARAM_NOISE1=gwn, L_LT_CORR=[sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)) for i in range(n)],
                 PARAM_LT_CORR=ltc)
+
+
+def aggregate(nsite, V=100.0, sigma=100.0, seed_h=0, lam=35.0, gam=50.0, temp=295.0, k_mats=0):
+    """A fully coupled aggregate: sites on a line with point-dipole couplings
+    J_ij = V / |i - j|^3 between every pair (a dense H), E ~ N(0, sigma), one
+    high-temperature DL mode per site (plus k_mats Matsubara modes)."""
+    rng = np.random.default_rng(seed_h)
+    H = np.diag(rng.normal(0.0, sigma, nsite)).astype(np.complex128)
+    for i in range(nsite):
+        for j in range(i + 1, nsite):
+            H[i, j] = H[j, i] = V / (j - i) ** 3
+    modes = O.bcf_convert_dl_to_exp(lam, gam, temp, k_matsubara=k_mats)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        for j in range(0, len(modes), 2):
+            gw.append([modes[j], modes[j + 1]])
+            ll.append(L)
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
