@@ -12,8 +12,10 @@ Follows basis/hops_basis.py:117-292 (define_basis / update_basis), :294-998
 (error bounds), basis/basis_functions_adaptive.py (error terms),
 basis/basis_functions.py:8-65 (thresholds), basis/hops_fluxfilters.py (flux
 filters) and trajectory/hops_trajectory.py:456-533, :636-714 (adaptive time
-loop and INCH_WORM early integrator).  L-operators with off-diagonal entries
-are not supported in adaptive mode (UnsupportedRequest).
+loop and INCH_WORM early integrator).  L-operators with entries off the
+diagonal add their destination-state flux terms, their noise operator and
+their low-temperature operators to the bounds (hops_basis.py:914-998,
+:1030-1100).
 """
 from __future__ import annotations
 
@@ -84,10 +86,24 @@ class BasisSelector:
     device memory; the trajectory engine supplies the derivative."""
 
     def __init__(self, info, delta_a, delta_s, maxhier, f_discard, markov=None, filters=None):
-        for L in info.l_ops:
-            if np.any(L.row != L.col):
-                raise UnsupportedRequest("adaptive basis with off-diagonal L-operators", "make_adaptive")
+        import scipy.sparse as sps
         self.info = info
+        # L-operators with entries off the diagonal couple a basis state to
+        # other states (Peierls-type couplings): their flux terms sum over the
+        # destination states and their noise enters the state flux
+        # (hops_basis.py:1030-1100, :914-998)
+        self.off_diag = np.array([bool(np.any(L.row != L.col)) for L in info.l_ops], dtype=bool)
+        self.L_csc = []
+        for L in info.l_ops:
+            c = sps.csc_array(L)
+            c.sum_duplicates()
+            c.sort_indices()
+            self.L_csc.append(c)
+        dest = [set() for _ in range(info.nstates)]
+        for L in info.l_ops:
+            for r, c in zip(L.row, L.col):
+                dest[int(r)].add(int(c))
+        self.dest = [sorted(d) for d in dest]  # hops_system.py:218-226
         self.delta_a, self.delta_s = float(delta_a), float(delta_s)
         self.maxhier = int(maxhier)
         self.f_discard = float(f_discard)
@@ -120,8 +136,83 @@ class BasisSelector:
         psi = self._pn[: n]
         nrm = np.vdot(psi, psi)
         avg = np.array([np.vdot(psi, tb.L[j] @ psi) / nrm for j in range(tb.n_lop)])
+        self._avg = avg
         X2 = np.repeat(avg[tb.mode_lrel].reshape(nm, 1), n, axis=1)
         return M2, K2, g, w, X2
+
+    # -- off-diagonal L-operators -------------------------------------------
+    def _off_couplings(self):
+        """Any L-operator of the basis with entries off the diagonal
+        (hops_basis.py:1223-1227)."""
+        return bool(np.any(self.off_diag[self.tb.absL]))
+
+    def _off_tables(self, sc):
+        """Destination tables of the basis modes: for every destination state
+        d of the basis states (hops_system.py:218-226) M_d[m, i] = L_m[d, s_i]
+        for s_i != d (hops_basis.py:1030-1100).  Returns off2[m, i] = sum_d
+        |M_d[m, i]|^2 (added over d in ascending order) and the tables of the
+        destinations outside the basis, {d: M_d}, for d in sc."""
+        tb = self.tb
+        st = [int(s) for s in tb.state_list]
+        dests = set()
+        for s in st:
+            dests.update(self.dest[s])
+        want = set(sc) & dests
+        nm, n = tb.n_mode, tb.n_state
+        off2 = np.zeros((nm, n))
+        Md = {d: np.zeros((nm, n), complex) for d in sorted(want)}
+        for lm in range(nm):
+            Lc = self.L_csc[int(tb.absL[tb.mode_lrel[lm]])]
+            for i, s in enumerate(st):
+                lo, hi = Lc.indptr[s], Lc.indptr[s + 1]
+                acc = 0.0
+                for d, v in zip(Lc.indices[lo:hi], Lc.data[lo:hi]):
+                    d = int(d)
+                    if d == s or d not in dests:
+                        continue
+                    acc += abs(v) ** 2
+                    if d in want:
+                        Md[d][lm, i] = v
+                off2[lm, i] = acc
+        return off2, Md
+
+    def _h2(self, z_step):
+        """-iH over the full state space plus, with off-diagonal L-operators in
+        the basis, their noise and noise-memory operator sum_j (conj(z1_j) -
+        i z2_j + sum_{m of L_j} z_mem,m) L_j (hops_basis.py:914-953)."""
+        tb = self.tb
+        H2 = -1j * self.info.H
+        if not self._off_couplings():
+            return H2
+        z1, z2, zm = z_step
+        comp = np.zeros(len(tb.absL), dtype=np.complex128)
+        for lm, m in enumerate(tb.modes):
+            comp[tb.mode_lrel[lm]] += zm[int(m)]
+        coef = np.conj(np.asarray(z1)) - 1j * np.asarray(z2) + comp
+        Z = np.zeros_like(H2)
+        for j, l in enumerate(tb.absL):
+            if self.off_diag[l]:
+                Z = Z + coef[j] * self.info.l_ops[int(l)].toarray()
+        return H2 + Z
+
+    def _ltc_ops(self):
+        """(T_phys, T_hier) of the off-diagonal L-operators with a
+        low-temperature coefficient (hops_basis.py:955-998): T_hier = sum
+        conj(c_j) <L_j> L_j and T_phys = T_hier + T_hier^dagger - sum c_j
+        L_j^2; (None, None) without such operators in the basis."""
+        tb = self.tb
+        lt = np.asarray(self.info.lt_param, dtype=np.complex128)[tb.absL]
+        if not (np.any(lt) and self._off_couplings()):
+            return None, None
+        N = self.info.nstates
+        Th = np.zeros((N, N), complex)
+        Lsq = np.zeros((N, N), complex)
+        for j, l in enumerate(tb.absL):
+            if self.off_diag[l]:
+                Ld = self.info.l_ops[int(l)].toarray()
+                Th = Th + np.conj(lt[j]) * self._avg[j] * Ld
+                Lsq = Lsq + lt[j] * (Ld @ Ld)
+        return Th + Th.conj().T - Lsq, Th
 
     def _markov_up(self):
         tb = self.tb
@@ -185,10 +276,18 @@ class BasisSelector:
                 out |= set(int(x) for x in c.col[c.row == s])
         return sorted(out - st)
 
-    def define_basis(self, tb, phi, dt, derivative):
-        """hops_basis.py:117-182: new state list and auxiliary list."""
+    def define_basis(self, tb, phi, dt, derivative, z_step=None):
+        """hops_basis.py:117-182: new state list and auxiliary list.  z_step
+        (a callable returning (z1, z2, z_mem): the noise of the basis
+        L-operators at the step's start and the noise memory by absolute
+        mode) is called only when an L-operator of the basis has entries off
+        the diagonal."""
         self._bind(tb)
         self._deriv = derivative
+        # the error terms of this call, kept for inspection: per auxiliary,
+        # per basis state, and per outside state sc
+        self.err_aux = self.err_state = None
+        self.err_outside = {}
         n, nh = tb.n_state, tb.n_hier
         self._pn = phi / np.linalg.norm(phi[:n])
         Pn = self._pn
@@ -196,23 +295,44 @@ class BasisSelector:
         absC2 = np.abs(C2) ** 2
         M2, K2, g, w, X2 = self._mode_tables()
         sc = self._list_sc()
-        H = self.info.H
+        off = self._off_couplings()
+        off2, Md = self._off_tables(sc) if off else (0.0, {})
+        H2 = self._h2(z_step() if off else None)
+        T = self._ltc_ops()
+        st = [int(s) for s in tb.state_list]
         if self.delta_a > 0:
             # hier_stable_error (hops_basis.py:560-665)
             D = self._deriv(Pn) / hbar + Pn / dt
             err = np.sum(np.abs(D.reshape(nh, n).T) ** 2, axis=0)
             if sc:
-                Hsc = -1j * H[np.ix_(sc, tb.state_list)]
-                err = err + np.sum(np.abs(Hsc @ C2 / hbar) ** 2, axis=0)
+                # flux out of the state basis (basis_functions_adaptive.py:76-163)
+                if T[0] is None:
+                    Hsc = H2[np.ix_(sc, st)]
+                    err = err + np.sum(np.abs(Hsc @ C2 / hbar) ** 2, axis=0)
+                else:
+                    Hp = (H2 + T[0])[np.ix_(sc, st)]
+                    Hh = (H2 + T[1])[np.ix_(sc, st)]
+                    Caux = C2.copy()
+                    Caux[:, 0] = 0
+                    e = np.sum(np.abs(Hh @ Caux / hbar) ** 2, axis=0)
+                    e[0] = np.sum(np.abs(Hp @ C2[:, :1] / hbar) ** 2)
+                    err = err + e
             KW = np.abs(w).reshape(-1, 1) * (1 + K2)
-            E_up = np.abs(KW) ** 2 * ((np.abs(M2) ** 2) @ absC2) / hbar ** 2
+            Msup = np.abs(M2) ** 2
+            if off:
+                Msup = Msup + off2
+            E_up = np.abs(KW) ** 2 * (Msup @ absC2) / hbar ** 2
             F_up = self._filter_up()
             depth = np.array([_depth(a) for a in tb.aux_list])
             F_up[:, depth == self.maxhier] = False
             err = err + np.sum(F_up * E_up, axis=0)
             gw2 = np.square(np.abs(g / w)).reshape(-1, 1)
-            E_dn = gw2 * ((np.abs(M2 - X2) ** 2) @ absC2) / hbar ** 2
+            Dsup = np.abs(M2 - X2) ** 2
+            if off:
+                Dsup = Dsup + off2
+            E_dn = gw2 * (Dsup @ absC2) / hbar ** 2
             err = err + np.sum((K2 > 0) * E_dn, axis=0)
+            self.err_aux = err
             dsq = self.delta_a ** 2
             idx_st, aux_st = keep_above(err, dsq / 2.0, tb.aux_list)
             unstable = np.setdiff1d(np.arange(nh), idx_st)
@@ -223,10 +343,11 @@ class BasisSelector:
         else:
             aux_new, aux_b, stable_idx = list(tb.aux_list), [], list(range(nh))
         if self.delta_s > 0:
-            st_stable, st_bound = self._state_basis(Pn, C2, absC2, M2, K2, g, w, X2, sc, dt, stable_idx, aux_b)
+            st_stable, st_bound = self._state_basis(Pn, C2, absC2, M2, K2, g, w, X2, sc, dt, stable_idx, aux_b,
+                                                    aux_new, off2 if off else None, Md, H2, T)
             states_new = sorted(set(st_stable) | set(st_bound))
         else:
-            states_new = [int(s) for s in tb.state_list]
+            states_new = st
         return states_new, aux_new
 
     def _boundary_aux(self, E_up, E_dn, idx_st, dbound, depth):
@@ -270,47 +391,97 @@ class BasisSelector:
         thr = error_threshold(np.sort(np.array(list(acc.values()))), dbound, offset=discarded)
         return [k for k, v in acc.items() if v > thr]
 
-    def _state_basis(self, Pn, C2, absC2, M2, K2, g, w, X2, sc, dt, stable_idx, aux_b):
-        """hops_basis.py:294-482 and :795-912."""
+    def _aux_filters(self, aux_set):
+        """(F_down, F_up) over [mode, aux]: F_down marks the auxiliaries k with
+        k - e_m in aux_set, F_up those with k + e_m in aux_set
+        (hops_fluxfilters.py:292-352)."""
         tb = self.tb
-        n, nh, nm = tb.n_state, tb.n_hier, tb.n_mode
-        P_st = np.zeros(nh * n, complex)
-        P_st.reshape(nh, n)[stable_idx] = Pn.reshape(nh, n)[stable_idx]
-        D = self._deriv(P_st) / hbar + P_st / dt
-        err = np.sum(np.abs(D.reshape(nh, n).T[:, stable_idx]) ** 2, axis=1)
-        Hm = -1j * self.info.H
-        Hc = Hm - np.diag(np.diag(Hm))
-        V1 = np.sum(np.abs(Hc[:, tb.state_list]) ** 2, axis=0)
-        err = err + V1 * np.sum(absC2[:, stable_idx], axis=1) / hbar ** 2
-        F_dn = np.zeros((nm, nh), bool)
-        F_up = np.zeros((nm, nh), bool)
-        for a in aux_b:
+        F_dn = np.zeros((tb.n_mode, tb.n_hier), bool)
+        F_up = np.zeros((tb.n_mode, tb.n_hier), bool)
+        for a in aux_set:
             for lm, m in enumerate(tb.modes):
                 j = self.aux_index.get(_step_aux(a, int(m), 1))
                 if j is not None:
                     F_dn[lm, j] = True
             for (m, v) in a:
                 j = self.aux_index.get(_step_aux(a, m, -1))
-                if j is not None:
+                if j is not None and m in self.rel_mode:
                     F_up[self.rel_mode[m], j] = True
+        return F_dn, F_up
+
+    def _state_basis(self, Pn, C2, absC2, M2, K2, g, w, X2, sc, dt, stable_idx, aux_b, aux_new, off2, Md, H2, T):
+        """hops_basis.py:294-482 and :795-912.  off2 / Md: the destination
+        tables of off-diagonal L-operators (None / {} without them); H2: -iH
+        plus their noise operator; T: their low-temperature operators."""
+        tb = self.tb
+        n, nh, nm = tb.n_state, tb.n_hier, tb.n_mode
+        st = [int(s) for s in tb.state_list]
+        P_st = np.zeros(nh * n, complex)
+        P_st.reshape(nh, n)[stable_idx] = Pn.reshape(nh, n)[stable_idx]
+        D = self._deriv(P_st) / hbar + P_st / dt
+        err = np.sum(np.abs(D.reshape(nh, n).T[:, stable_idx]) ** 2, axis=1)
+        # flux out of each basis state into every other state through H2
+        # (basis_functions_adaptive.py:893-995)
+        Hc = H2 - np.diag(np.diag(H2))
+        C2s = C2[:, stable_idx]
+        if T[0] is None:
+            V1 = np.sum(np.abs(Hc[:, st]) ** 2, axis=0)
+            err = err + V1 * np.sum(np.abs(C2s) ** 2, axis=1) / hbar ** 2
+        else:
+            Tp, Th = T
+            Vp = np.sum(np.abs((Hc + Tp - np.diag(np.diag(Tp)))[:, st]) ** 2, axis=0)
+            Vh = np.sum(np.abs((Hc + Th - np.diag(np.diag(Th)))[:, st]) ** 2, axis=0)
+            Caux = C2s.copy()
+            Caux[:, 0] = 0
+            err = err + (Vp * np.abs(C2s[:, 0]) ** 2 + Vh * np.sum(np.abs(Caux) ** 2, axis=1)) / hbar ** 2
+        F_dn, F_up = self._aux_filters(aux_b)
         gw2 = np.square(np.abs(g / w)).reshape(-1, 1)
+        KW = np.abs(w).reshape(-1, 1) * (1 + K2)
         E_dn = ((np.abs(M2 - X2) ** 2).T @ (F_dn * gw2)) * absC2 / hbar ** 2
+        E_up = ((np.abs(M2) ** 2).T @ (np.abs(F_up * KW) ** 2)) * absC2 / hbar ** 2
+        if off2 is not None:
+            # flux through the destination tables, filtered by the new basis
+            # (basis_functions_adaptive.py:268-384, :626-758)
+            F_dn_o, F_up_o = self._aux_filters(aux_new)
+            if np.sum(off2) != 0:
+                E_dn = (off2.T @ (F_dn_o * gw2)) * absC2 / hbar ** 2 + E_dn
+                E_up = E_up + (off2.T @ (np.abs(F_up_o * KW) ** 2)) * absC2 / hbar ** 2
         err = err + np.sum(E_dn[:, stable_idx], axis=1)
-        KW = F_up * (np.abs(w).reshape(-1, 1) * (1 + K2))
-        E_up = ((np.abs(M2) ** 2).T @ (np.abs(KW) ** 2)) * absC2 / hbar ** 2
         err = err + np.sum(E_up[:, stable_idx], axis=1)
+        self.err_state = err
         dsq = self.delta_s ** 2
-        rel_st, st_stable = keep_above(err, dsq / 2.0, [int(s) for s in tb.state_list])
+        rel_st, st_stable = keep_above(err, dsq / 2.0, st)
         dbound = dsq - np.sum(err[np.setdiff1d(np.arange(n), rel_st)])
         bound = []
         if len(rel_st) < self.info.nstates and sc:
-            C2s = C2[np.ix_(rel_st, stable_idx)]
-            Hsc = -1j * self.info.H[np.ix_(sc, st_stable)]
-            E = np.sum(np.abs(Hsc @ (C2s / hbar)) ** 2, axis=1)
+            # flux into the outside states (basis_functions_adaptive.py:997-1132)
+            Cs = C2[np.ix_(rel_st, stable_idx)]
+            if T[0] is None:
+                E = np.sum(np.abs(H2[np.ix_(sc, st_stable)] @ (Cs / hbar)) ** 2, axis=1)
+            else:
+                E = (np.abs((H2 + T[0])[np.ix_(sc, st_stable)] @ (Cs[:, :1] / hbar)) ** 2)[:, 0]
+                if Cs.shape[1] > 1:
+                    E = E + np.sum(np.abs((H2 + T[1])[np.ix_(sc, st_stable)] @ (Cs[:, 1:] / hbar)) ** 2, axis=1)
+            E = np.array(E, dtype=float)
+            if Md:
+                # flux up and down into each outside destination state, out of
+                # the stable states and auxiliaries (basis_functions_adaptive.py:386-515, :760-891)
+                keep = np.zeros(n, bool)
+                keep[rel_st] = True
+                C2sq = np.abs(np.where(keep[:, None], C2, 0)[:, stable_idx]) ** 2
+                F_dn_o, F_up_o = self._aux_filters(aux_new)
+                KWo = np.abs(F_up_o[:, stable_idx] * KW[:, stable_idx]) ** 2
+                G = F_dn_o[:, stable_idx] * gw2
+                for d, M in Md.items():
+                    A = np.abs(M) ** 2
+                    f_up = np.sum((A.T @ KWo) * C2sq / hbar ** 2)
+                    f_dn = np.sum((A.T @ G) * C2sq / hbar ** 2)
+                    E[sc.index(d)] += f_up + f_dn
+            self.err_outside = dict(zip(sc, E))
             nz = np.nonzero(E)[0]
             if len(nz):
                 _, bound = keep_above(E[nz], dbound, list(np.array(sc)[nz]))
-        bound = sorted(set(int(b) for b in bound) - set(int(s) for s in tb.state_list))
+        bound = sorted(set(int(b) for b in bound) - set(st))
         return [int(s) for s in st_stable], bound
 
 class AdaptiveTrajectoryEngine:
@@ -471,8 +642,22 @@ class AdaptiveTrajectoryEngine:
             st.bt.z1_d, st.bt.z2_d = saved
         return d.cpu().numpy()
 
+    def _z_step(self, tidx):
+        """(z1, z2, z_mem) for the selection's noise operator: the point
+        noise of the basis L-operators at noise index tidx (the step's start,
+        hops_trajectory.py:693-714) and the noise memory by absolute mode."""
+        absL = torch.as_tensor(self.tb.absL, dtype=torch.long, device=self.device)
+        z1 = self.z1_all.index_select(0, absL)[:, int(tidx)].cpu().numpy()
+        if self.z2_all is None:
+            z2 = np.zeros_like(z1)
+        else:
+            z2 = self.z2_all.index_select(0, absL)[:, int(tidx)].cpu().numpy()
+        zm = self.st.zmem[: self.info.n_hmodes].cpu().numpy()
+        return z1, z2, zm
+
     def define_basis(self, phi, dt, tidx):
-        return self.selector.define_basis(self.tb, phi, dt, lambda y: self._derivative(y, tidx))
+        return self.selector.define_basis(self.tb, phi, dt, lambda y: self._derivative(y, tidx),
+                                          lambda: self._z_step(tidx))
 
     # -- basis change ----------------------------------------------------------
     def update_basis(self, phi_host_vec, states, aux):

@@ -137,3 +137,56 @@ def aggregate(nsite, V=100.0, sigma=100.0, seed_h=0, lam=35.0, gam=50.0, temp=29
             gw.append([modes[j], modes[j + 1]])
             ll.append(L)
     return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+
+
+def peierls_chain(ltc=False):
+    """5 sites where H couples only 0-1 and 3-4: state 2 is reached only
+    through the off-diagonal (Peierls) L-operators of bonds 1-2 and 2-3, next
+    to a site projector (Holstein) on every site; ltc adds a low-temperature
+    correction on every L-operator."""
+    n = 5
+    H = np.diag([0.0, 100.0, -50.0, 80.0, 20.0]).astype(complex)
+    H[0, 1] = H[1, 0] = 40.0
+    H[3, 4] = H[4, 3] = 40.0
+    g0, w0 = O.bcf_convert_dl_to_exp(40.0, 50.0, 295.0)
+    gw, ll = [], []
+    for i in range(n):
+        ll.append(sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)))
+        gw.append([g0, w0])
+    for i in (1, 2):
+        L = np.zeros((n, n), complex)
+        L[i, i + 1] = L[i + 1, i] = 1.0
+        ll.append(L)
+        gw.append([g0 / 4, w0])
+    p = dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+    if ltc:
+        p["L_LT_CORR"] = ll
+        p["PARAM_LT_CORR"] = [3.0 + 0.5j] * n + [1.0, 1.0]
+    return p
+
+
+def peierls_ring(nsite, V=50.0, sigma=80.0, seed_h=0, lam=40.0, gam=50.0, temp=295.0, ltc=None):
+    """A chain of nsite sites with nearest-neighbour H couplings on every
+    other bond only, a site projector on every site and an off-diagonal
+    (Peierls) L-operator |n><n+1| + |n+1><n| on every bond: the bonds
+    without an H coupling are crossed only through the Peierls fluxes and
+    noise."""
+    rng = np.random.default_rng(seed_h)
+    n = nsite
+    H = np.diag(rng.normal(0.0, sigma, n)).astype(np.complex128)
+    for i in range(0, n - 1, 2):
+        H[i, i + 1] = H[i + 1, i] = V
+    g0, w0 = O.bcf_convert_dl_to_exp(lam, gam, temp)
+    gw, ll = [], []
+    for i in range(n):
+        ll.append(sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)))
+        gw.append([g0, w0])
+    for i in range(n - 1):
+        L = sp.coo_matrix(([1.0, 1.0], ([i, i + 1], [i + 1, i])), shape=(n, n))
+        ll.append(L)
+        gw.append([g0 / 4, w0])
+    p = dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
+    if ltc is not None:
+        p["L_LT_CORR"] = ll
+        p["PARAM_LT_CORR"] = [ltc] * len(ll)
+    return p

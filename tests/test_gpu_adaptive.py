@@ -156,6 +156,58 @@ def test_adaptive_trajectory_matches_oracle(name):
     assert len(sizes) > 1   # the basis moved during the run
 
 
+def _strong_ltc_peierls(c):
+    p = systems.peierls_chain(True)
+    p["PARAM_LT_CORR"] = [c] * len(p["L_LT_CORR"])
+    return p
+
+
+# off-diagonal (Peierls) L-operators; the run starts on a state from which the
+# middle of the chain is reached only through the Peierls couplings
+OFFDIAG_CASES = {
+    "peierls_both": (systems.peierls_chain(), 2e-3, 3e-4, 3, "NORMALIZED NONLINEAR", 0, 50.0, 1.0),
+    "peierls_strong_ltc": (_strong_ltc_peierls(40.0 + 15j), 2e-3, 3e-4, 3, "NORMALIZED NONLINEAR", 0, 50.0, 1.0),
+    "peierls_hier_only": (systems.peierls_chain(), 1e-3, 0.0, 4, "NONLINEAR", 0, 40.0, 2.0),
+    "peierls_states_nonlinear": (_strong_ltc_peierls(40.0 + 15j), 0.0, 2e-4, 2, "NONLINEAR", 0, 40.0, 1.0),
+    "peierls_ring_linear": (systems.peierls_ring(8, ltc=30.0 + 5j), 2e-3, 3e-4, 3, "LINEAR", 3, 40.0, 1.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(OFFDIAG_CASES))
+def test_adaptive_offdiagonal_matches_oracle(name):
+    """Adaptive HopsTrajectory with off-diagonal L-operators (the reference's
+    Peierls couplings, hops_basis.py:914-998, :1030-1100): basis sequence
+    equal to the oracle's at every step and psi(t) to 1e-10."""
+    _need_gpu()
+    from mesohops_amd import HopsTrajectory
+    sysp, da, ds, K, eom, s0, t_max, dt = OFFDIAG_CASES[name]
+    nst = sysp["HAMILTONIAN"].shape[0]
+    tlen, tau = t_max + 20.0, 0.5
+    z = _noise(sysp, 23, tlen, tau)
+    noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+    psi0 = np.zeros(nst, complex)
+    psi0[s0] = 1.0
+    print("case", name, flush=True)
+    ref = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
+                               delta_a=da, delta_s=ds)
+    ref.initialize(psi0)
+    ref.propagate(t_max, dt)
+    tr = HopsTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, dict(noise), hierarchy_param={"MAXHIER": K})
+    tr.make_adaptive(da, ds)
+    tr.initialize(psi0)
+    tr.propagate(t_max, dt)
+    assert len(tr.storage["state_list"]) == len(ref.state_traj)
+    for i, (s_got, s_ref) in enumerate(zip(tr.storage["state_list"], ref.state_traj)):
+        assert list(s_got) == list(s_ref), (i, s_got, s_ref)
+        assert tr.storage["list_nhier"][i] == len(ref.aux_traj[i]), i
+    assert set(tr.auxiliary_list) == set(ref.basis.aux_list)
+    err = np.max(np.abs(tr.storage["psi_traj"] - np.array(ref.psi_traj)))
+    sizes = sorted(set(zip(tr.storage["list_nstate"], tr.storage["list_nhier"])))
+    print("off-diagonal %s: max |dpsi| = %.3e, (n_state, n_hier) from %s to %s" % (name, err, sizes[0], sizes[-1]))
+    assert err < FP64_TOL
+    assert len(sizes) > 1
+
+
 @pytest.mark.parametrize("early", ["INCH_WORM", "STATIC_STATE_INCHWORM_HIERARCHY"])
 def test_static_basis_matches_oracle(early):
     """STATIC_BASIS and the STATIC_STATE_INCHWORM_HIERARCHY early integrator

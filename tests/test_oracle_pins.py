@@ -456,28 +456,9 @@ def test_interpolated_noise_reproduces_cubics():
 
 
 def _peierls_chain(ltc):
-    """5 sites where H couples only 0-1 and 3-4: state 2 is reached only
-    through the off-diagonal (Peierls) L-operators of bonds 1-2 and 2-3, next
-    to a site projector (Holstein) on every site."""
-    n = 5
-    H = np.diag([0.0, 100.0, -50.0, 80.0, 20.0]).astype(complex)
-    H[0, 1] = H[1, 0] = 40.0
-    H[3, 4] = H[4, 3] = 40.0
-    g0, w0 = O.bcf_convert_dl_to_exp(40.0, 50.0, 295.0)
-    gw, ll = [], []
-    for i in range(n):
-        ll.append(sp.coo_matrix(([1.0], ([i], [i])), shape=(n, n)))
-        gw.append([g0, w0])
-    for i in (1, 2):
-        L = np.zeros((n, n), complex)
-        L[i, i + 1] = L[i + 1, i] = 1.0
-        ll.append(L)
-        gw.append([g0 / 4, w0])
-    p = dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=O.bcf_exp, PARAM_NOISE1=gw)
-    if ltc:
-        p["L_LT_CORR"] = ll
-        p["PARAM_LT_CORR"] = [3.0 + 0.5j] * n + [1.0, 1.0]
-    return p
+    """The 5-site Peierls chain of systems.peierls_chain: state 2 is reached
+    only through the off-diagonal L-operators of bonds 1-2 and 2-3."""
+    return systems.peierls_chain(ltc)
 
 
 @pytest.mark.parametrize("ltc", [False, True], ids=["plain", "ltc"])
