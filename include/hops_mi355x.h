@@ -346,6 +346,13 @@ typedef struct hops_segment {
 } hops_segment_t;
 int hops_copy_segments(const hops_segment_t *segs_d, int n, void *stream);
 
+/* Gathers n 8-byte words: out_d[i] = *src_d[i] (no counterpart in the
+   reference: it collects, for the adaptive selection of a whole batch in
+   one launch, the noise values and noise memory that the noise operator of
+   off-diagonal L-operators needs, hops_basis.py:914-953).  src_d and out_d
+   are device arrays; every src_d[i] is an 8-byte aligned device address. */
+int hops_gather_u64(const uint64_t *const *src_d, uint64_t *out_d, int n, void *stream);
+
 /* Phi of the new bases; tiles run over the new rows of each slot and index
    perm_d by slot. */
 int hops_slots_permute(const hops_slot_perm_t *perm_d, const uint32_t *tiles_d, int n_tiles, void *stream);

@@ -133,6 +133,14 @@ long long hops_rt_debug_table(void *basis, const char *name, void *out, long lon
 int hops_rt_debug_select(void *basis, const hops_rt_traj_param_t *prm, const double *pn, const double *e_aux,
                          double dt, uint8_t *stable_out);
 int hops_rt_debug_select_states(void *basis, const double *e_state);
+/* The noise of the step for a basis with off-diagonal L-operators, before
+   hops_rt_debug_select: z1 / z2 (complex64 [n_lop of the basis], z2 may be
+   NULL) of the basis L-operators and the noise memory (complex128 [n_hmode]).
+   The error terms of the last selection: which 0 per auxiliary, 1 per basis
+   state, 2 (state, error) pairs of the outside states; returns the count of
+   doubles, copied to out when it fits in cap. */
+int hops_rt_debug_noise(void *basis, const float *z1, const float *z2, const double *zmem);
+long long hops_rt_debug_errors(void *basis, int which, double *out, long long cap);
 int hops_rt_debug_result(void *basis, int32_t *n_states, int32_t *states_out, int32_t *n_aux, int32_t *aux_ptr,
                          int32_t *aux_mode, int32_t *aux_val, long long cap);
 int hops_rt_debug_free(void *basis);

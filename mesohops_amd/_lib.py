@@ -112,6 +112,7 @@ def lib():
     L.hops_slots_build.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]
     L.hops_slots_permute.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.hops_copy_segments.argtypes = [_vp, ctypes.c_int, _vp]
+    L.hops_gather_u64.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.hops_rt_system_create.restype = _vp
     L.hops_rt_system_create.argtypes = [ctypes.POINTER(RtSystemDesc)]
     L.hops_rt_system_destroy.argtypes = [_vp]
@@ -140,6 +141,9 @@ def lib():
     L.hops_rt_debug_table.argtypes = [_vp, ctypes.c_char_p, _vp, _ll]
     L.hops_rt_debug_select.argtypes = [_vp, ctypes.POINTER(RtTrajParam), _vp, _vp, ctypes.c_double, _vp]
     L.hops_rt_debug_select_states.argtypes = [_vp, _vp]
+    L.hops_rt_debug_noise.argtypes = [_vp, _vp, _vp, _vp]
+    L.hops_rt_debug_errors.restype = _ll
+    L.hops_rt_debug_errors.argtypes = [_vp, ctypes.c_int, _vp, _ll]
     L.hops_rt_debug_result.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ll]
     L.hops_rt_debug_free.argtypes = [_vp]
     L.hops_rt_debug_threshold.restype = ctypes.c_double
@@ -165,12 +169,14 @@ EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_prof
             "hops_build_super", "hops_propagate", "hops_propagate_noalloc", "hops_derivative", "hops_permute_c128", "hops_apply_state_op",
             "hops_noise_embedding", "hops_noise_filter", "hops_noise_filter_c128", "hops_noise_spline", "hops_noise_window_mean",
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
+            "hops_gather_u64",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
             "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",
             "hops_rt_batch_option", "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
             "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
             "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",
-            "hops_rt_debug_result", "hops_rt_debug_free", "hops_rt_debug_threshold"]
+            "hops_rt_debug_result", "hops_rt_debug_free", "hops_rt_debug_threshold", "hops_rt_debug_noise",
+            "hops_rt_debug_errors"]
 
 
 def set_option(name, value):

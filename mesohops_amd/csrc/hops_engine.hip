@@ -2032,6 +2032,12 @@ __global__ __launch_bounds__(256) void k_copy_segments(const hops_segment_t *__r
   for (long i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(256) void k_gather_u64(const uint64_t *const *__restrict__ src,
+                                                   uint64_t *__restrict__ out, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = *src[i];
+}
+
 template <typename R>
 int slots_step_t(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *list, int n_list,
                  const uint32_t *tiles, int n_tiles, const int32_t *it, size_t shm, bool mixed,
@@ -2367,6 +2373,14 @@ int hops_slots_permute(const hops_slot_perm_t *perm_d, const uint32_t *tiles_d, 
   if (n_tiles <= 0) return 0;
   if (!perm_d || !tiles_d) return fail("hops_slots_permute: NULL argument");
   k_permute_slots<<<n_tiles, 256, 0, (hipStream_t)stream>>>(perm_d, tiles_d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int hops_gather_u64(const uint64_t *const *src_d, uint64_t *out_d, int n, void *stream) {
+  if (n <= 0) return 0;
+  if (!src_d || !out_d) return fail("hops_gather_u64: NULL argument");
+  k_gather_u64<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(src_d, out_d, n);
   HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -149,6 +149,11 @@ bool aux_less(const Aux &a, const Aux &b) {
 struct LOp {
   std::vector<int> row, col;
   std::vector<cplx> val;
+  // the same entries by column (cp/ri/cv) and by row (rp/ci/rv), each in
+  // ascending order of the other index, over the full state space
+  std::vector<int> cp, ri, rp, ci;
+  std::vector<cplx> cv, rv;
+  bool off = false;  // an entry off the diagonal (a Peierls-type coupling)
 };
 
 struct System {
@@ -160,7 +165,7 @@ struct System {
   std::vector<cplx> lt;
   std::vector<std::vector<int>> modes_by_state, l_by_state, coupled, dest;
   std::vector<int> site_of_mode;  // the single state of a site-projector mode, or -1
-  bool diag_L = true;
+  bool diag_L = true;  // no L-operator has an entry off the diagonal
   std::vector<uint64_t> zob, zob2;  // two independent Zobrist tables [n_hm * 256]
 };
 
@@ -233,6 +238,7 @@ struct Basis {
   KeyTable index;  // auxiliary -> position
   int n = 0, nh = 0, nm = 0, nl = 0, na = 0, n_pat = 0, pat_w = 0, ell_w = 0;
   bool has_lt = false;
+  bool has_off = false;  // an L-operator of the basis has entries off the diagonal
   std::vector<int> modes, rel_mode, absL, mode_lrel, mode_lact, act_abs, act_lrel, pos;
   std::vector<RedL> L;
   std::vector<uint64_t> aux_hash, aux_hash2;
@@ -327,6 +333,7 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   }
   const int nl = (int)B.absL.size();
   B.nl = nl;
+  for (int l : B.absL) B.has_off |= S.L[l].off;
   for (int m : B.modes) B.mode_lrel.push_back(relL[S.l_of_mode[m]]);
   {
     std::vector<char> af(S.n_l, 0);
@@ -627,6 +634,7 @@ struct Params {
   };
   std::vector<UpFilter> filters;
   std::vector<Aux> aux0;
+  bool keep_err = false;  // keep the error terms of a selection (inspection)
 };
 
 // copies the Markovian and Triangular / LongEdge filter parameters
@@ -861,6 +869,24 @@ struct Selection {
   std::vector<cplx> X2;               // [nm]
   std::vector<std::vector<std::pair<int, cplx>>> M2;  // per rel mode: (state, L[s,s])
   std::vector<int> sc;
+  // L-operators with entries off the diagonal in the basis
+  // (hops_basis.py:914-998, :1030-1100); empty without them
+  std::vector<double> off2;           // [nm * n] sum over destinations d != s of |L_m[d, s]|^2
+  std::vector<int> dsc;               // destination states outside the basis (in sc), ascending
+  std::vector<std::vector<std::pair<int, double>>> md2;  // per dsc: (lm * n + i, |L_m[d, s_i]|^2)
+  std::vector<cplx> coef;             // per relative L: conj(z1) - i z2 + its noise memory
+  // couplings of the outside states sc to the basis states through H2 = -iH
+  // + noise operator, as CSR over sc (state, value); with low-temperature
+  // operators also through H2 + T_phys (physical wave function) and H2 +
+  // T_hier (auxiliaries)
+  std::vector<int> h2p, h2s, hpp, hps, hhp, hhs;
+  std::vector<cplx> h2v, hpv, hhv;
+  std::vector<double> v1, vp, vh;     // [n] sum over d != s of |H2[d, s]|^2 (and with T_phys, T_hier)
+  bool lt = false;                    // low-temperature operators of off-diagonal L
+  // with Params::keep_err: the error per auxiliary, per basis state and per
+  // outside state (state, error) of the last selection
+  std::vector<double> err_aux, err_state;
+  std::vector<std::pair<int, double>> err_out;
 };
 
 std::vector<int> list_sc(const System &S, const Basis &B) {
@@ -930,7 +956,12 @@ void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
 // The part of the selection that needs only psi = Pn[:n]: the diagonal
 // entries M2 of each mode's reduced L, <L> over psi (X2) and the outside
 // states coupled to the basis (sc)
-void sel_common(const Basis &B, const cplx *pn, Selection &out) {
+void sel_offdiag(const Basis &B, const cplx *coef, const std::vector<cplx> &avg, Selection &out);
+
+// coef: per relative L of the basis, the noise coefficient of the noise
+// operator (noise_coef); used only when the basis has an L-operator with
+// entries off the diagonal
+void sel_common(const Basis &B, const cplx *pn, Selection &out, const cplx *coef = nullptr) {
   const System &S = *B.sys;
   const int n = B.n, nm = B.nm;
   out = Selection();
@@ -959,6 +990,176 @@ void sel_common(const Basis &B, const cplx *pn, Selection &out) {
   out.X2.resize(nm);
   for (int lm = 0; lm < nm; ++lm) out.X2[lm] = avg[B.mode_lrel[lm]];
   out.sc = list_sc(S, B);
+  if (B.has_off && coef) sel_offdiag(B, coef, avg, out);
+}
+
+// Noise coefficient of each relative L of the basis in the noise operator of
+// the state flux: conj(z1) - i z2 in complex64 arithmetic (the noise is
+// complex64, as the reference's get_noise returns it) plus the noise memory
+// of the L's modes in the basis (hops_basis.py:936-953).  z2 may be NULL
+// (noise 2 ZERO); zmem is indexed by absolute mode.
+void noise_coef(const Basis &B, const std::complex<float> *z1, const std::complex<float> *z2, const cplx *zmem,
+                std::vector<cplx> &coef) {
+  std::vector<cplx> comp(B.nl, cplx(0, 0));
+  for (int lm = 0; lm < B.nm; ++lm) comp[B.mode_lrel[lm]] += zmem[B.modes[lm]];
+  coef.resize(B.nl);
+  for (int j = 0; j < B.nl; ++j) {
+    const float re = z1[j].real() + (z2 ? z2[j].imag() : 0.0f);
+    const float im = -z1[j].imag() - (z2 ? z2[j].real() : 0.0f);
+    coef[j] = cplx(re, im) + comp[j];
+  }
+}
+
+// The off-diagonal part of a selection (hops_basis.py:914-998, :1030-1100):
+// - off2[lm, i] = sum over the destination states d != s_i of the basis
+//   states of |L_m[d, s_i]|^2, and those terms per outside destination d;
+// - H2 = -iH + sum_j coef_j L_j over the off-diagonal L of the basis, and
+//   with a low-temperature correction on the basis L's T_hier = sum_j
+//   conj(c_j) <L_j> L_j and T_phys = T_hier + T_hier^dagger - sum_j c_j L_j^2
+//   (off-diagonal L only): their couplings from the basis states to the
+//   outside states sc, and the column sums over d != s_i of |H2[d, s_i]|^2
+//   (plus T_phys, T_hier).
+// Built column by column over the basis states, from the full-space L.
+void sel_offdiag(const Basis &B, const cplx *coef, const std::vector<cplx> &avg, Selection &out) {
+  const System &S = *B.sys;
+  const int n = B.n, nm = B.nm, N = S.n;
+  std::vector<char> isdest(N, 0);
+  for (int s : B.states)
+    for (int c : S.dest[s]) isdest[c] = 1;
+  std::vector<int> dpos(N, -1), scpos(N, -1);
+  for (size_t q = 0; q < out.sc.size(); ++q) {
+    scpos[out.sc[q]] = (int)q;
+    if (isdest[out.sc[q]]) {
+      dpos[out.sc[q]] = (int)out.dsc.size();
+      out.dsc.push_back(out.sc[q]);
+    }
+  }
+  out.off2.assign((size_t)nm * n, 0.0);
+  out.md2.assign(out.dsc.size(), {});
+  for (int lm = 0; lm < nm; ++lm) {
+    const LOp &L = S.L[B.absL[B.mode_lrel[lm]]];
+    if (!L.off) continue;
+    for (int i = 0; i < n; ++i) {
+      const int s = B.states[i];
+      double acc = 0;
+      for (int q = L.cp[s]; q < L.cp[s + 1]; ++q) {
+        const int d = L.ri[q];
+        if (d == s || !isdest[d]) continue;
+        const double a = std::norm(L.cv[q]);
+        acc += a;
+        if (dpos[d] >= 0) out.md2[dpos[d]].push_back({lm * n + i, a});
+      }
+      out.off2[(size_t)lm * n + i] = acc;
+    }
+  }
+  bool any_lt = false;
+  if (!S.lt.empty())
+    for (int l : B.absL) any_lt |= S.lt[l] != cplx(0, 0);
+  out.lt = any_lt;
+  const size_t nsc = out.sc.size();
+  std::vector<std::vector<std::pair<int, cplx>>> r2(nsc), rp(any_lt ? nsc : 0), rh(any_lt ? nsc : 0);
+  out.v1.assign(n, 0.0);
+  if (any_lt) {
+    out.vp.assign(n, 0.0);
+    out.vh.assign(n, 0.0);
+  }
+  std::vector<cplx> h2(N), th(N), tp(N), sq(N);
+  std::vector<char> hit(N, 0);
+  std::vector<int> touched;
+  auto touch = [&](int d) {
+    if (!hit[d]) {
+      hit[d] = 1;
+      touched.push_back(d);
+    }
+  };
+  for (int i = 0; i < n; ++i) {
+    const int s = B.states[i];
+    for (int d = 0; d < N; ++d)
+      if (S.H[(size_t)d * N + s] != cplx(0, 0)) {
+        h2[d] = cmi(S.H[(size_t)d * N + s]);
+        touch(d);
+      }
+    for (int j = 0; j < B.nl; ++j) {
+      const LOp &L = S.L[B.absL[j]];
+      if (!L.off) continue;
+      for (int q = L.cp[s]; q < L.cp[s + 1]; ++q) {
+        h2[L.ri[q]] += coef[j] * L.cv[q];
+        touch(L.ri[q]);
+      }
+    }
+    if (any_lt) {
+      for (int j = 0; j < B.nl; ++j) {
+        const LOp &L = S.L[B.absL[j]];
+        if (!L.off) continue;
+        const cplx c = S.lt[B.absL[j]];
+        const cplx a = std::conj(c) * avg[j];
+        for (int q = L.cp[s]; q < L.cp[s + 1]; ++q) {  // T_hier[:, s]
+          th[L.ri[q]] += a * L.cv[q];
+          touch(L.ri[q]);
+        }
+        const cplx ad = c * std::conj(avg[j]);
+        for (int q = L.rp[s]; q < L.rp[s + 1]; ++q) {  // T_hier^dagger[:, s] = conj(T_hier[s, :])
+          tp[L.ci[q]] += ad * std::conj(L.rv[q]);
+          touch(L.ci[q]);
+        }
+        for (int q = L.cp[s]; q < L.cp[s + 1]; ++q) {  // (L L)[:, s]
+          const int k = L.ri[q];
+          for (int q2 = L.cp[k]; q2 < L.cp[k + 1]; ++q2) {
+            sq[L.ri[q2]] += L.cv[q2] * L.cv[q];
+            touch(L.ri[q2]);
+          }
+        }
+        for (int d : touched) {
+          tp[d] -= c * sq[d];
+          sq[d] = 0;
+        }
+      }
+    }
+    std::sort(touched.begin(), touched.end());
+    double v = 0, vp = 0, vh = 0;
+    for (int d : touched) {
+      const cplx tphys = th[d] + tp[d];  // T_hier + T_hier^dagger - sum c L^2
+      if (d != s) {
+        v += std::norm(h2[d]);
+        if (any_lt) {
+          vp += std::norm(h2[d] + tphys);
+          vh += std::norm(h2[d] + th[d]);
+        }
+      }
+      const int q = scpos[d];
+      if (q >= 0) {
+        if (h2[d] != cplx(0, 0)) r2[q].push_back({i, h2[d]});
+        if (any_lt) {
+          if (h2[d] + tphys != cplx(0, 0)) rp[q].push_back({i, h2[d] + tphys});
+          if (h2[d] + th[d] != cplx(0, 0)) rh[q].push_back({i, h2[d] + th[d]});
+        }
+      }
+      h2[d] = th[d] = tp[d] = 0;
+      hit[d] = 0;
+    }
+    touched.clear();
+    out.v1[i] = v;
+    if (any_lt) {
+      out.vp[i] = vp;
+      out.vh[i] = vh;
+    }
+  }
+  auto flat = [&](const std::vector<std::vector<std::pair<int, cplx>>> &r, std::vector<int> &ptr,
+                  std::vector<int> &st, std::vector<cplx> &val) {
+    ptr.assign(r.size() + 1, 0);
+    for (size_t q = 0; q < r.size(); ++q) {
+      for (auto &e : r[q]) {
+        st.push_back(e.first);
+        val.push_back(e.second);
+      }
+      ptr[q + 1] = (int)st.size();
+    }
+  };
+  flat(r2, out.h2p, out.h2s, out.h2v);
+  if (any_lt) {
+    flat(rp, out.hpp, out.hps, out.hpv);
+    flat(rh, out.hhp, out.hhs, out.hhv);
+  }
 }
 
 // the couplings -iH[sp, s] of each outside state sp (in sc order) to the
@@ -983,10 +1184,23 @@ void sc_couplings(const Basis &B, const std::vector<int> &sc, std::vector<int> &
 }
 
 // hierarchy half: stable auxiliaries, boundary auxiliaries, new aux list
-void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *e_aux, Selection &out) {
+void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *e_aux, Selection &out,
+                 const cplx *coef = nullptr) {
   const System &S = *B.sys;
   const int n = B.n, nh = B.nh, nm = B.nm;
-  sel_common(B, pn, out);
+  if (B.has_off && !coef) {
+    // the noise operator needs the noise of the step: a caller without it
+    // cannot select for off-diagonal L-operators
+    out = Selection();
+    out.aux_new = B.aux;
+    out.states_new = B.states;
+    out.stable_mask.assign(nh, 1);
+    out.stable_idx.resize(nh);
+    for (int k = 0; k < nh; ++k) out.stable_idx[k] = k;
+    return;
+  }
+  sel_common(B, pn, out, coef);
+  const bool off = !out.off2.empty();
   out.absC2.resize((size_t)nh * n);
   for (size_t i = 0; i < (size_t)nh * n; ++i) out.absC2[i] = std::norm(pn[i]);
   if (P.delta_a <= 0) {
@@ -998,17 +1212,23 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   }
   const double h2 = kHbar * kHbar;
   std::vector<double> err(e_aux, e_aux + nh);
-  // flux out of the state basis through H (basis_functions_adaptive.py:76-163)
+  // flux out of the state basis through H, or with off-diagonal L through H2
+  // (-iH plus the noise operator) and the low-temperature operators: T_phys
+  // for the physical wave function, T_hier for the auxiliaries
+  // (basis_functions_adaptive.py:76-163)
   if (!out.sc.empty()) {
     std::vector<int> hp, hs;
     std::vector<cplx> hv;
-    sc_couplings(B, out.sc, hp, hs, hv);
+    if (!off) sc_couplings(B, out.sc, hp, hs, hv);
     for (int k = 0; k < nh; ++k) {
+      const std::vector<int> &cp = !off ? hp : !out.lt ? out.h2p : k == 0 ? out.hpp : out.hhp;
+      const std::vector<int> &cs = !off ? hs : !out.lt ? out.h2s : k == 0 ? out.hps : out.hhs;
+      const std::vector<cplx> &cv = !off ? hv : !out.lt ? out.h2v : k == 0 ? out.hpv : out.hhv;
       const cplx *pk = pn + (size_t)k * n;
       double acc = 0;
       for (size_t i = 0; i < out.sc.size(); ++i) {
         cplx z = 0;
-        for (int q = hp[i]; q < hp[i + 1]; ++q) z += cm(hv[q], pk[hs[q]]);
+        for (int q = cp[i]; q < cp[i + 1]; ++q) z += cm(cv[q], pk[cs[q]]);
         z /= kHbar;
         acc += std::norm(z);
       }
@@ -1026,7 +1246,10 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   std::vector<uint8_t> F_up;
   markov_up(P, B, F_up);
   std::vector<double> aw(nm), gw2(nm), nX(nm);
-  // the diagonal L entries of each mode as CSR: state, |M2|^2, |M2 - X|^2
+  // the diagonal L entries of each mode as CSR: state, |M2|^2, |M2 - X|^2;
+  // with off-diagonal L each state's sum over its destinations d != s of
+  // |L_m[d, s]|^2 is added to both (hops_basis.py:1030-1100), and the states
+  // with only such entries join the list
   std::vector<int> m2p(nm + 1, 0), m2s;
   std::vector<double> m2u, m2d;
   for (int lm = 0; lm < nm; ++lm) {
@@ -1034,10 +1257,27 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     aw[lm] = std::abs(S.W[m]);
     gw2[lm] = std::norm(S.G[m] / S.W[m]);
     nX[lm] = std::norm(out.X2[lm]);
-    for (auto &e : out.M2[lm]) {
-      m2s.push_back(e.first);
-      m2u.push_back(std::norm(e.second));
-      m2d.push_back(std::norm(e.second - out.X2[lm]));
+    if (!off) {
+      for (auto &e : out.M2[lm]) {
+        m2s.push_back(e.first);
+        m2u.push_back(std::norm(e.second));
+        m2d.push_back(std::norm(e.second - out.X2[lm]));
+      }
+    } else {
+      const double *o2 = &out.off2[(size_t)lm * n];
+      size_t q = 0;
+      for (int s2 = 0; s2 < n; ++s2) {
+        cplx dv = 0;
+        bool has = false;
+        if (q < out.M2[lm].size() && out.M2[lm][q].first == s2) {
+          dv = out.M2[lm][q++].second;
+          has = true;
+        }
+        if (!has && o2[s2] == 0) continue;
+        m2s.push_back(s2);
+        m2u.push_back(std::norm(dv) + o2[s2]);
+        m2d.push_back(std::norm(dv - out.X2[lm]) + o2[s2]);
+      }
     }
     m2p[lm + 1] = (int)m2s.size();
   }
@@ -1109,6 +1349,7 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
       if (kval[o + lm] > 0) acc += E_dn[o + lm];
     err[k] += acc;
   }
+  if (P.keep_err) out.err_aux = err;
   // stable auxiliaries (hops_basis.py:1000-1028)
   const double dsq = P.delta_a * P.delta_a;
   double below = 0;
@@ -1233,12 +1474,29 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
     return;
   }
   const double h2 = kHbar * kHbar;
+  const bool off = !sel.off2.empty();
   std::vector<double> err(e_state, e_state + n);
-  // flux through the off-diagonal Hamiltonian
+  // flux through the off-diagonal Hamiltonian (with off-diagonal L: through
+  // H2, and with their low-temperature operators through H2 + T_phys for
+  // the first stable auxiliary and H2 + T_hier for the others,
+  // basis_functions_adaptive.py:893-995)
+  const int k_first = sel.stable_idx.empty() ? -1 : sel.stable_idx[0];
   for (int s = 0; s < n; ++s) {
+    if (off && sel.lt) {
+      if (k_first < 0) continue;
+      const double c0 = sel.absC2[(size_t)k_first * n + s];
+      const double cr =
+          bsum(nh, [&](size_t k) { return sel.stable_mask[k] && (int)k != k_first ? sel.absC2[k * n + s] : 0.0; });
+      err[s] += (sel.vp[s] * c0 + sel.vh[s] * cr) / h2;
+      continue;
+    }
     double v1 = 0;
-    for (int sp = 0; sp < S.n; ++sp)
-      if (sp != B.states[s]) v1 += std::norm(S.H[(size_t)sp * S.n + B.states[s]]);
+    if (off) {
+      v1 = sel.v1[s];
+    } else {
+      for (int sp = 0; sp < S.n; ++sp)
+        if (sp != B.states[s]) v1 += std::norm(S.H[(size_t)sp * S.n + B.states[s]]);
+    }
     const double c1 = bsum(nh, [&](size_t k) { return sel.stable_mask[k] ? sel.absC2[k * n + s] : 0.0; });
     err[s] += v1 * c1 / h2;
   }
@@ -1296,6 +1554,53 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
       }
     }
   }
+  // off-diagonal L: the destination sums off2 enter through the filters of
+  // the new auxiliary list (hops_fluxfilters.py:292-352 on aux_new;
+  // basis_functions_adaptive.py:268-384, :626-758).  fo[p * n_off + o]: bit 0
+  // k + e_m in the new list (flux up), bit 1 k - e_m in it (flux down), for
+  // stable position p and off-diagonal mode o.
+  std::vector<int> offm;
+  std::vector<uint8_t> fo;
+  if (off) {
+    for (int lm = 0; lm < nm; ++lm)
+      for (int s = 0; s < n; ++s)
+        if (sel.off2[(size_t)lm * n + s] != 0) {
+          offm.push_back(lm);
+          break;
+        }
+    KeyTable ntab;
+    ntab.reset(sel.aux_bound.size());
+    for (size_t i = 0; i < sel.aux_bound.size(); ++i) ntab.insert(key_of(S, sel.aux_bound[i]), (int)i);
+    auto in_new = [&](const AKey &key) {
+      const int j = B.index.find(key);
+      return j >= 0 ? sel.stable_mask[j] != 0 : ntab.find(key) >= 0;
+    };
+    const size_t no = offm.size();
+    fo.assign(sel.stable_idx.size() * no, 0);
+    for (size_t p = 0; p < sel.stable_idx.size(); ++p) {
+      const int k = sel.stable_idx[p];
+      const AKey kk{B.aux_hash[k], B.aux_hash2[k]};
+      for (size_t o = 0; o < no; ++o) {
+        const int lm = offm[o], m = B.modes[lm];
+        const int v = aux_get(B.aux[k], m);
+        uint8_t f = 0;
+        if (v < 255 && in_new(key_step(S, kk, m, v, v + 1))) f |= 1;
+        if (v > 0 && in_new(key_step(S, kk, m, v, v - 1))) f |= 2;
+        fo[p * no + o] = f;
+        if (!f) continue;
+        const double gw2 = std::norm(S.G[m] / S.W[m]);
+        const double kw = std::abs(S.W[m]) * (1.0 + v);
+        for (int s = 0; s < n; ++s) {
+          const double o2 = sel.off2[(size_t)lm * n + s];
+          if (o2 == 0) continue;
+          const double c2 = sel.absC2[(size_t)k * n + s];
+          if (f & 2) err[s] += (o2 * gw2) * c2 / h2;
+          if (f & 1) err[s] += (o2 * kw * kw) * c2 / h2;
+        }
+      }
+    }
+  }
+  if (P.keep_err) sel.err_state = err;
   const double dsq = P.delta_s * P.delta_s;
   std::vector<double> srt(err);
   std::sort(srt.begin(), srt.end());
@@ -1316,24 +1621,84 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
     std::vector<double> E(sel.sc.size(), 0.0);
     std::vector<int> qs;
     std::vector<cplx> qh;
+    std::vector<char> is_st(n, 0);
+    for (int q : rel_st) is_st[q] = 1;
+    std::vector<int> qs2;
+    std::vector<cplx> qh2;
     for (size_t i = 0; i < sel.sc.size(); ++i) {
       const int sp = sel.sc[i];
       qs.clear();
       qh.clear();
-      for (size_t q = 0; q < rel_st.size(); ++q) {  // zero entries add exact zeros: skipped
-        const cplx h = S.H[(size_t)sp * S.n + st_stable[q]];
-        if (h != cplx(0, 0)) {
-          qs.push_back(rel_st[q]);
-          qh.push_back(cmi(h));
+      if (!off) {
+        for (size_t q = 0; q < rel_st.size(); ++q) {  // zero entries add exact zeros: skipped
+          const cplx h = S.H[(size_t)sp * S.n + st_stable[q]];
+          if (h != cplx(0, 0)) {
+            qs.push_back(rel_st[q]);
+            qh.push_back(cmi(h));
+          }
         }
+      } else {
+        // H2 (or H2 + T_hier) from the stable states; qs2 / qh2: H2 + T_phys
+        // for the first stable auxiliary
+        const std::vector<int> &cp = sel.lt ? sel.hhp : sel.h2p;
+        const std::vector<int> &cs = sel.lt ? sel.hhs : sel.h2s;
+        const std::vector<cplx> &cv = sel.lt ? sel.hhv : sel.h2v;
+        for (int q = cp[i]; q < cp[i + 1]; ++q)
+          if (is_st[cs[q]]) {
+            qs.push_back(cs[q]);
+            qh.push_back(cv[q]);
+          }
+        qs2.clear();
+        qh2.clear();
+        if (sel.lt)
+          for (int q = sel.hpp[i]; q < sel.hpp[i + 1]; ++q)
+            if (is_st[sel.hps[q]]) {
+              qs2.push_back(sel.hps[q]);
+              qh2.push_back(sel.hpv[q]);
+            }
       }
       E[i] = bsum(nh, [&](size_t k) {
         if (!sel.stable_mask[k]) return 0.0;
+        const bool phys = off && sel.lt && (int)k == k_first;
+        const std::vector<int> &ss = phys ? qs2 : qs;
+        const std::vector<cplx> &hh = phys ? qh2 : qh;
         cplx z = 0;
-        for (size_t q = 0; q < qs.size(); ++q) z += cm(qh[q], pn[k * n + qs[q]] / kHbar);
+        for (size_t q = 0; q < ss.size(); ++q) z += cm(hh[q], pn[k * n + ss[q]] / kHbar);
         return std::norm(z);
       });
     }
+    if (off && !sel.dsc.empty()) {
+      // flux up and down into each outside destination state from the
+      // stable states and auxiliaries (basis_functions_adaptive.py:386-515,
+      // :760-891)
+      std::vector<int> opos(nm, -1);
+      for (size_t o = 0; o < offm.size(); ++o) opos[offm[o]] = (int)o;
+      const size_t no = offm.size();
+      std::vector<int> scpos(S.n, -1);
+      for (size_t q = 0; q < sel.sc.size(); ++q) scpos[sel.sc[q]] = (int)q;
+      for (size_t di = 0; di < sel.dsc.size(); ++di) {
+        double fu = 0, fd = 0;
+        for (size_t p = 0; p < sel.stable_idx.size(); ++p) {
+          const int k = sel.stable_idx[p];
+          for (const auto &e : sel.md2[di]) {
+            const int lm = e.first / n, i = e.first % n;
+            if (!is_st[i] || opos[lm] < 0) continue;
+            const uint8_t f = fo[p * no + opos[lm]];
+            if (!f) continue;
+            const int m = B.modes[lm];
+            const double c2 = sel.absC2[(size_t)k * n + i];
+            if (f & 1) {
+              const double kw = std::abs(S.W[m]) * (1.0 + aux_get(B.aux[k], m));
+              fu += e.second * (kw * kw) * c2 / h2;
+            }
+            if (f & 2) fd += e.second * std::norm(S.G[m] / S.W[m]) * c2 / h2;
+          }
+        }
+        E[scpos[sel.dsc[di]]] += fu + fd;
+      }
+    }
+    if (P.keep_err)
+      for (size_t i = 0; i < E.size(); ++i) sel.err_out.push_back({sel.sc[i], E[i]});
     std::vector<double> ev;
     std::vector<int> es;
     for (size_t i = 0; i < E.size(); ++i)
@@ -1528,6 +1893,7 @@ struct Slot {
   // per-step scratch
   Selection sel;
   std::vector<cplx> pn;
+  std::vector<cplx> zcoef;  // noise coefficients of the noise operator (off-diagonal L only)
   double nrm = 1;
   Basis Bnew;
   bool changed = false;
@@ -1895,18 +2261,42 @@ int define_bases_host(Batch &b, const std::vector<int> &list) {
   auto t_dev = std::chrono::steady_clock::now();
   const std::vector<uint32_t> tiles = tiles_of(b, list);
   for (int s : list) b.its[s] = b.slots[s].it;
+  // the noise operator of slots with off-diagonal L-operators needs z1 / z2
+  // of the basis L at the step's noise index and the noise memory
+  // (hops_basis.py:914-953): one gather of 8-byte words for all of them
+  std::vector<const void *> zsrc;
+  std::vector<size_t> zat(S, 0);
+  for (int s : list) {
+    const Slot &sl = b.slots[s];
+    if (!sl.B.has_off) continue;
+    if (sl.it < 0 || sl.it >= sl.prm.n_t) return rt_fail("noise index %d outside the noise grid", sl.it);
+    zat[s] = zsrc.size();
+    for (int z = 0; z < (sl.z2 ? 2 : 1); ++z) {
+      const char *zb = static_cast<const char *>(z ? sl.z2 : sl.z1);
+      for (int l : sl.B.absL) zsrc.push_back(zb + ((size_t)l * sl.prm.n_t + sl.it) * 8);
+    }
+    const char *zm = reinterpret_cast<const char *>(b.bts[s].zmem_d);
+    for (int w = 0; w < 2 * sl.sys->n_hm; ++w) zsrc.push_back(zm + (size_t)w * 8);
+  }
   RT_TRY(upload_desc(b));
-  RT_TRY(ensure_staging(b, tot * 17 + etot * 16 + S * 8 * 6 + tiles.size() * 8 + list.size() * 8 + 65536));
+  RT_TRY(ensure_staging(b, tot * 17 + etot * 16 + S * 8 * 6 + tiles.size() * 8 + list.size() * 8 + zsrc.size() * 16 +
+                               65536));
   Stage st{&b};
-  void *dl, *dtl, *doff, *deoff, *dpn, *de;
-  size_t opn, oe;
+  void *dl, *dtl, *doff, *deoff, *dpn, *de, *dzs = nullptr, *dz = nullptr;
+  size_t opn, oe, oz;
   RT_TRY(st.push(list.data(), list.size() * 4, &dl));
   RT_TRY(st.push(tiles.data(), tiles.size() * 4, &dtl));
   RT_TRY(st.push(off.data(), S * 8, &doff));
   RT_TRY(st.push(eoff.data(), S * 8, &deoff));
   RT_TRY(st.reserve(tot * 16, &dpn, &opn));
   RT_TRY(st.reserve(etot * 8, &de, &oe));
+  if (!zsrc.empty()) {
+    RT_TRY(st.push(zsrc.data(), zsrc.size() * 8, &dzs));
+    RT_TRY(st.reserve(zsrc.size() * 8, &dz, &oz));
+  }
   RT_TRY(st.flush());
+  if (!zsrc.empty())
+    RT_TRY(hops_gather_u64((const uint64_t *const *)dzs, (uint64_t *)dz, (int)zsrc.size(), b.stream));
   double *dnrm = reinterpret_cast<double *>(b.arena.ptr(b.nrm_off));
   const int shm = shm_of(b, list);
   RT_TRY(hops_slots_pn(d_ops(b), d_bts(b), (const int32_t *)dl, (int)list.size(), (const int64_t *)doff, dpn, dnrm,
@@ -1923,8 +2313,21 @@ int define_bases_host(Batch &b, const std::vector<int> &list) {
                              (const double *)b.arena.ptr(b.dt_off), kHbar, b.stream));
   RT_HIP(hipMemcpyAsync(b.h_down, dpn, tot * 16, hipMemcpyDeviceToHost, b.stream));
   RT_HIP(hipMemcpyAsync(b.h_down + tot * 16, de, etot * 8, hipMemcpyDeviceToHost, b.stream));
+  if (!zsrc.empty())
+    RT_HIP(hipMemcpyAsync(b.h_down + tot * 16 + etot * 8, dz, zsrc.size() * 8, hipMemcpyDeviceToHost, b.stream));
   RT_HIP(hipMemcpyAsync(b.nrms.data(), dnrm, S * 8, hipMemcpyDeviceToHost, b.stream));
   RT_HIP(hipStreamSynchronize(b.stream));
+  if (!zsrc.empty()) {
+    const char *hz = b.h_down + tot * 16 + etot * 8;
+    for (int s : list) {
+      Slot &sl = b.slots[s];
+      if (!sl.B.has_off) continue;
+      const auto *z1 = reinterpret_cast<const std::complex<float> *>(hz + zat[s] * 8);
+      const auto *z2 = sl.z2 ? z1 + sl.B.nl : nullptr;
+      const auto *zm = reinterpret_cast<const cplx *>(hz + (zat[s] + (size_t)sl.B.nl * (sl.z2 ? 2 : 1)) * 8);
+      noise_coef(sl.B, z1, z2, zm, sl.zcoef);
+    }
+  }
   // the selection reads Pn and the error sums in place (pinned memory); the
   // slot keeps psi = Pn[:n]
   const cplx *hpn = reinterpret_cast<const cplx *>(b.h_down);
@@ -1938,7 +2341,8 @@ int define_bases_host(Batch &b, const std::vector<int> &list) {
   b.tm[1] += std::chrono::duration<double>(t_h - t_dev).count();
   b.pool->parallel_for((int)list.size(), [&](int i) {
     Slot &sl = b.slots[list[i]];
-    select_hier(sl.prm, sl.B, hpn + off[list[i]], he + eoff[list[i]], sl.sel);
+    select_hier(sl.prm, sl.B, hpn + off[list[i]], he + eoff[list[i]], sl.sel,
+                sl.B.has_off ? sl.zcoef.data() : nullptr);
   });
   auto t_s = std::chrono::steady_clock::now();
   b.tm[2] += std::chrono::duration<double>(t_s - t_h).count();
@@ -2022,7 +2426,7 @@ int ensure_zob(Batch &b, Slot &sl) {
 // a slot whose selection runs on the device: hierarchy adaptivity on, no
 // Triangular / LongEdge filter, at most one diagonal entry per mode's L
 bool dev_eligible(const Slot &sl) {
-  if (!(sl.prm.delta_a > 0) || !sl.prm.filters.empty() || !sl.zob_d) return false;
+  if (!(sl.prm.delta_a > 0) || !sl.prm.filters.empty() || !sl.zob_d || sl.B.has_off) return false;
   const Basis &B = sl.B;
   for (int lm = 0; lm < B.nm; ++lm) {
     const RedL &R = B.L[B.mode_lrel[lm]];
@@ -3224,8 +3628,43 @@ void *hops_rt_system_create(const hops_rt_system_desc_t *d) {
       S->L[l].row.push_back(r);
       S->L[l].col.push_back(c);
       S->L[l].val.push_back(cplx(d->l_val[2 * q], d->l_val[2 * q + 1]));
-      if (r != c) S->diag_L = false;
+      if (r != c) {
+        S->diag_L = false;
+        S->L[l].off = true;
+      }
     }
+  for (LOp &L : S->L) {
+    // by column and by row, duplicates summed
+    std::vector<std::tuple<int, int, cplx>> e;
+    for (size_t q = 0; q < L.row.size(); ++q) e.emplace_back(L.col[q], L.row[q], L.val[q]);
+    std::sort(e.begin(), e.end(), [](const auto &x, const auto &y) {
+      return std::get<0>(x) != std::get<0>(y) ? std::get<0>(x) < std::get<0>(y) : std::get<1>(x) < std::get<1>(y);
+    });
+    L.cp.assign(S->n + 1, 0);
+    for (size_t q = 0; q < e.size(); ++q) {
+      const auto [c, r, v] = e[q];
+      if (!L.ri.empty() && q > 0 && std::get<0>(e[q - 1]) == c && std::get<1>(e[q - 1]) == r) {
+        L.cv.back() += v;
+        continue;
+      }
+      L.ri.push_back(r);
+      L.cv.push_back(v);
+      L.cp[c + 1]++;
+    }
+    for (int c = 0; c < S->n; ++c) L.cp[c + 1] += L.cp[c];
+    L.rp.assign(S->n + 1, 0);
+    for (int c = 0; c < S->n; ++c)
+      for (int q = L.cp[c]; q < L.cp[c + 1]; ++q) L.rp[L.ri[q] + 1]++;
+    for (int r = 0; r < S->n; ++r) L.rp[r + 1] += L.rp[r];
+    L.ci.resize(L.ri.size());
+    L.rv.resize(L.ri.size());
+    std::vector<int> fill(L.rp.begin(), L.rp.end() - 1);
+    for (int c = 0; c < S->n; ++c)  // columns ascending: each row's entries come out by ascending column
+      for (int q = L.cp[c]; q < L.cp[c + 1]; ++q) {
+        L.ci[fill[L.ri[q]]] = c;
+        L.rv[fill[L.ri[q]]++] = L.cv[q];
+      }
+  }
   if (d->lt)
     for (int l = 0; l < S->n_l; ++l) S->lt.push_back(cplx(d->lt[2 * l], d->lt[2 * l + 1]));
   S->modes_by_state.resize(S->n);
@@ -3353,8 +3792,6 @@ int hops_rt_batch_add(void *p, void *sys, const hops_rt_traj_param_t *prm, const
   P.dt = prm->dt;
   if (P.eom < 0 || P.eom > 2) return rt_fail("unknown eom %d", P.eom);
   if (P.noise_step <= 0 || P.n_t <= 0 || P.it0 < 0 || P.it0 >= P.n_t) return rt_fail("bad noise grid");
-  if ((P.delta_a > 0 || P.delta_s > 0) && !S.diag_L)
-    return rt_fail("adaptive basis with off-diagonal L-operators is not supported");
   RT_TRY(copy_filters(prm, S.n_hm, P));
   if (P.delta_a <= 0) {
     if (prm->n_aux0 <= 0 || !prm->aux0_ptr) return rt_fail("non-adaptive hierarchy needs aux0");
@@ -3581,6 +4018,7 @@ struct DebugBasis {
   Selection sel;
   Params P;
   std::vector<cplx> pn;
+  std::vector<cplx> coef;  // noise coefficients (hops_rt_debug_noise)
 };
 
 void *hops_rt_debug_basis(void *sys, int eom, int n_states, const int32_t *states, int n_aux, const int32_t *aux_ptr,
@@ -3652,9 +4090,41 @@ int hops_rt_debug_select(void *p, const hops_rt_traj_param_t *prm, const double 
   const size_t N = (size_t)d->B.n * d->B.nh;
   d->pn.resize(N);
   for (size_t i = 0; i < N; ++i) d->pn[i] = cplx(pn[2 * i], pn[2 * i + 1]);
-  select_hier(d->P, d->B, d->pn.data(), e_aux, d->sel);
+  d->P.keep_err = true;
+  if (d->B.has_off && d->coef.empty()) return rt_fail("off-diagonal L-operators: hops_rt_debug_noise first");
+  select_hier(d->P, d->B, d->pn.data(), e_aux, d->sel, d->coef.empty() ? nullptr : d->coef.data());
   if (stable_out) std::copy(d->sel.stable_mask.begin(), d->sel.stable_mask.end(), stable_out);
   return 0;
+}
+
+int hops_rt_debug_noise(void *p, const float *z1, const float *z2, const double *zmem) {
+  if (!p || !z1 || !zmem) return rt_fail("NULL argument");
+  auto *d = static_cast<DebugBasis *>(p);
+  std::vector<cplx> zm(d->B.sys->n_hm);
+  for (size_t m = 0; m < zm.size(); ++m) zm[m] = cplx(zmem[2 * m], zmem[2 * m + 1]);
+  noise_coef(d->B, reinterpret_cast<const std::complex<float> *>(z1), reinterpret_cast<const std::complex<float> *>(z2),
+             zm.data(), d->coef);
+  return 0;
+}
+
+long long hops_rt_debug_errors(void *p, int which, double *out, long long cap) {
+  if (!p) return rt_fail("NULL argument");
+  const Selection &s = static_cast<DebugBasis *>(p)->sel;
+  std::vector<double> v;
+  if (which == 0) {
+    v = s.err_aux;
+  } else if (which == 1) {
+    v = s.err_state;
+  } else if (which == 2) {
+    for (auto &e : s.err_out) {
+      v.push_back(e.first);
+      v.push_back(e.second);
+    }
+  } else {
+    return rt_fail("unknown error table %d", which);
+  }
+  if (out && (long long)v.size() <= cap) std::copy(v.begin(), v.end(), out);
+  return (long long)v.size();
 }
 
 int hops_rt_debug_select_states(void *p, const double *e_state) {

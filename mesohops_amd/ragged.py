@@ -153,6 +153,29 @@ class DebugBasis:
         e = None if e_state is None else np.ascontiguousarray(e_state, dtype=np.float64)
         check(lib().hops_rt_debug_select_states(self.h, _ptr(e)))
 
+    def set_noise(self, z1, z2, zmem):
+        """Noise of the step (complex64 z1 / z2 of the basis L-operators, z2
+        may be None; complex128 noise memory by absolute mode) for bases with
+        off-diagonal L-operators."""
+        z1 = np.ascontiguousarray(z1, dtype=np.complex64)
+        z2 = None if z2 is None else np.ascontiguousarray(z2, dtype=np.complex64)
+        zm = np.ascontiguousarray(zmem, dtype=np.complex128)
+        check(lib().hops_rt_debug_noise(self.h, _ptr(z1), _ptr(z2), _ptr(zm)))
+
+    def errors(self, which):
+        """Error terms of the last selection: "aux", "state" or "outside"
+        ({state: error})."""
+        w = {"aux": 0, "state": 1, "outside": 2}[which]
+        n = lib().hops_rt_debug_errors(self.h, w, None, 0)
+        if n < 0:
+            raise EngineError(lib().hops_last_error().decode())
+        out = np.zeros(max(n, 1))
+        lib().hops_rt_debug_errors(self.h, w, _ptr(out), n)
+        out = out[:n]
+        if w == 2:
+            return {int(out[i]): float(out[i + 1]) for i in range(0, n, 2)}
+        return out
+
     def result(self, cap=1 << 20):
         ns, na = ctypes.c_int32(), ctypes.c_int32()
         st = np.zeros(cap, np.int32)

@@ -20,6 +20,7 @@ int hops_slots_errors(void) { return -1; }
 int hops_slots_build(void) { return -1; }
 int hops_slots_permute(void) { return -1; }
 int hops_copy_segments(void) { return -1; }
+int hops_gather_u64(void) { return -1; }
 int hops_sel_err(void) { return -1; }
 int hops_sel_stable(void) { return -1; }
 int hops_sel_fin1(void) { return -1; }

@@ -38,6 +38,10 @@ int hops_copy_segments(const hops_segment_t *s, int n, void *) {
   for (int i = 0; i < n; ++i) memcpy(s[i].dst, s[i].src, (size_t)s[i].bytes);
   return 0;
 }
+int hops_gather_u64(const uint64_t *const *src, uint64_t *out, int n, void *) {
+  for (int i = 0; i < n; ++i) out[i] = *src[i];
+  return 0;
+}
 // toy step: Phi[k, s] += 0.05 * (k + 1)^-1 * Phi[0, (s + k) % n] * i
 int hops_slots_step(const hops_op_t *ops, const hops_batch_t *bts, const int32_t *list, int n_list, const uint32_t *,
                     int, const int32_t *, int, int, void *) {

@@ -83,6 +83,61 @@ def test_adaptive_batch_matches_oracle(name):
         np.testing.assert_allclose(ab.z_mem(b), ref.z_mem, rtol=0, atol=1e-9 * max(1, np.max(np.abs(ref.z_mem))))
 
 
+def _strong_ltc_peierls(c):
+    p = systems.peierls_chain(True)
+    p["PARAM_LT_CORR"] = [c] * len(p["L_LT_CORR"])
+    return p
+
+
+# off-diagonal (Peierls) L-operators in the native runtime; each run starts on
+# a state from which the middle of the chain is reached only through the
+# Peierls couplings
+OFFDIAG_CASES = {
+    "peierls_both": (systems.peierls_chain(), "NORMALIZED NONLINEAR", 3, 2e-3, 3e-4, 50.0, 1.0, 0),
+    "peierls_strong_ltc": (_strong_ltc_peierls(40.0 + 15j), "NORMALIZED NONLINEAR", 3, 2e-3, 3e-4, 50.0, 1.0, 0),
+    "peierls_hier_only": (systems.peierls_chain(), "NONLINEAR", 4, 1e-3, 0.0, 40.0, 2.0, 0),
+    "peierls_ring_linear": (systems.peierls_ring(8, ltc=30.0 + 5j), "LINEAR", 3, 2e-3, 3e-4, 40.0, 1.0, 3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(OFFDIAG_CASES))
+def test_adaptive_batch_offdiagonal_matches_oracle(name):
+    """AdaptiveBatch with off-diagonal L-operators (hops_basis.py:914-998,
+    :1030-1100): every trajectory's basis sequence equals the oracle's and
+    its psi(t) agrees to 1e-10 in FP64."""
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp, eom, K, da, ds, t_max, dt, s0 = OFFDIAG_CASES[name]
+    nst = sysp["HAMILTONIAN"].shape[0]
+    tlen, tau = t_max + 20.0, 0.5
+    seeds = [5, 19, 31]
+    zs = [_noise(sysp, s, tlen, tau) for s in seeds]
+    psi0 = np.zeros(nst, complex)
+    psi0[s0] = 1.0
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+    print("case", name, flush=True)
+    ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": eom}, nps[0], None, {"MAXHIER": K}, noise_params=nps,
+                       precision_mode="fp64", arena_bytes=ARENA)
+    ab.make_adaptive(da, ds)
+    ab.initialize(psi0)
+    ab.propagate(t_max, dt)
+    moved = False
+    for b, z in enumerate(zs):
+        ref = _oracle(sysp, eom, z, tlen, tau, K, da, ds, t_max, dt, psi0)
+        psi, states, nhier = ab.results(b)
+        assert len(states) == len(ref.state_traj)
+        for i, (s_got, s_ref) in enumerate(zip(states, ref.state_traj)):
+            assert list(s_got) == list(s_ref), (b, i, s_got, s_ref)
+            assert nhier[i] == len(ref.aux_traj[i]), (b, i)
+        assert set(ab.auxiliary_list(b)) == set(ref.basis.aux_list)
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        print("ragged off-diagonal %s traj %d: max |dpsi| = %.3e, n_state %d..%d, n_hier %d..%d"
+              % (name, b, err, min(len(x) for x in states), max(len(x) for x in states), min(nhier), max(nhier)))
+        assert err < FP64_TOL
+        moved |= len(set(nhier)) > 1 or len(set(len(x) for x in states)) > 1
+    assert moved
+
+
 def test_alone_equals_in_mix():
     """A trajectory's result does not depend on which others share its batch."""
     _need_gpu()

@@ -89,6 +89,10 @@ CASES = {
                        [("LongEdge", [[False, True] * 6, 1])]),
     "markov_triangular_longedge": (*_markov_chain(7), 2e-3, 2e-3, 4, "NORMALIZED NONLINEAR",
                                    [("Triangular", [[True, False] * 7, 2]), ("LongEdge", [[True, False] * 7, 3])]),
+    # off-diagonal (Peierls) L-operators (hops_basis.py:914-998, :1030-1100)
+    "peierls_both": (systems.peierls_chain(), None, 2e-3, 2e-3, 3, "NORMALIZED NONLINEAR"),
+    "peierls_ltc_hier_only": (systems.peierls_chain(True), None, 1e-3, 0.0, 3, "NONLINEAR"),
+    "peierls_ring_linear": (systems.peierls_ring(8, ltc=2.0 + 0.3j), None, 2e-3, 2e-3, 3, "LINEAR"),
 }
 
 
@@ -120,6 +124,7 @@ def test_runtime_selection_matches_oracle(name):
         D = tr.dsystem_dt(pn, zm, z1, z2)[0] / O.HBAR + pn / dt
         e_aux = np.sum(np.abs(D.reshape(nh, n)) ** 2, axis=1)
         db = DebugBasis(rs, b.state_list, b.aux_list, 2)
+        db.set_noise(z1, z2, zm)
         stable = db.select(prm, pn, e_aux, dt)
         e_state = None
         if ds > 0:
@@ -163,3 +168,61 @@ def test_threshold_prefix_sort_matches_full_sort():
         ref = O.determine_error_thresh(np.sort(v), max_err, offset)
         assert thr == ref, (trial, n, max_err, offset)
         assert below.value == pytest.approx(math.fsum(v[v <= ref]), rel=1e-12, abs=0), trial
+
+
+@pytest.mark.parametrize("name", ["peierls_both", "peierls_strong_ltc", "peierls_states_nonlinear",
+                                  "peierls_ring_linear"])
+def test_runtime_error_terms_match_oracle(name):
+    """Off-diagonal L-operators in the native runtime's selection: the error
+    per auxiliary, per basis state and per outside state equal the oracle's
+    at every step to 1e-10 relative (destination tables, noise operator,
+    low-temperature operators, flux into outside destinations), and the
+    selected bases are the oracle's."""
+    from mesohops_amd.basis import SystemInfo
+    from mesohops_amd.ragged import DebugBasis, RtSystem, traj_param
+    from test_adaptive_host import TERM_CASES, _oracle_terms
+    sysp, da, ds, K, eom, s0 = TERM_CASES[name]
+    nst = sysp["HAMILTONIAN"].shape[0]
+    noise = {"SEED": 7, "MODEL": "FFT_FILTER", "TLEN": 200.0, "TAU": 1.0}
+    tr = O.AdaptiveTrajectory(sysp, {"EQUATION_OF_MOTION": eom}, noise, hierarchy_param={"MAXHIER": K},
+                              delta_a=da, delta_s=ds)
+    psi0 = np.zeros(nst, complex)
+    psi0[s0] = 1.0
+    tr.initialize(psi0)
+    rs = RtSystem(SystemInfo(sysp))
+    prm = traj_param(2, K, da, ds, 1, 0.01, 2.0, 1, 10, 0)
+    dt = 2.0
+    n_out = 0
+    for step in range(15):
+        tr.propagate(dt, dt)
+        b = tr.basis
+        z1, z2, zm = tr.z_step(tr.z_mem)
+        ref = tr.ad.define_basis(tr.dsystem_dt, b, tr.phi, dt, [z1, z2, zm])
+        n, nh = b.n_state, b.n_hier
+        pn = tr.phi / np.linalg.norm(tr.phi[:n])
+        D = tr.dsystem_dt(pn, zm, z1, z2)[0] / O.HBAR + pn / dt
+        e_aux = np.sum(np.abs(D.reshape(nh, n)) ** 2, axis=1)
+        db = DebugBasis(rs, b.state_list, b.aux_list, 2)
+        db.set_noise(z1, z2, zm)
+        stable = db.select(prm, pn, e_aux, dt)
+        e_state = None
+        if ds > 0:
+            pst = np.zeros_like(pn)
+            pst.reshape(nh, n)[stable] = pn.reshape(nh, n)[stable]
+            D2 = tr.dsystem_dt(pst, zm, z1, z2)[0] / O.HBAR + pst / dt
+            e_state = np.sum(np.abs(D2.reshape(nh, n)[stable]) ** 2, axis=0)
+        db.select_states(e_state)
+        st_new, aux_new = db.result()
+        assert st_new == [int(s) for s in ref[0]], (step, st_new, ref[0])
+        assert set(aux_new) == set(ref[1]), step
+        eh, es, eo = _oracle_terms(tr.ad, tr.dsystem_dt, b, tr.phi, dt, [z1, z2, zm])
+        if eh is not None:
+            np.testing.assert_allclose(db.errors("aux"), eh, rtol=1e-10, atol=1e-14 * np.max(eh))
+        if es is not None:
+            np.testing.assert_allclose(db.errors("state"), es, rtol=1e-10, atol=1e-14 * np.max(es))
+        got_out = db.errors("outside") if es is not None else {}
+        assert sorted(got_out) == sorted(eo), (step, got_out, eo)
+        for d, v in eo.items():
+            assert abs(got_out[d] - v) <= 1e-10 * abs(v) + 1e-300, (step, d, got_out[d], v)
+        n_out += len(eo)
+    assert n_out > 0
