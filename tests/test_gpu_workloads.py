@@ -44,6 +44,49 @@ def _pop_err(a, b):
     return float(np.max(np.abs(np.abs(np.asarray(a)) ** 2 - np.abs(np.asarray(b)) ** 2)))
 
 
+# --- 10-site chain, depth 4, one trajectory over all 1000 steps --------------
+def test_chain10_depth4_linear_full_length_against_oracle():
+    """One trajectory of the batched LINEAR workload over its full 1000 steps
+    (2000 fs): FP64 on the device against the FP64 oracle, and complex64
+    stages against both.  LINEAR HOPS does not normalise, so the populations
+    grow by orders of magnitude; the test prints the absolute population
+    differences next to the largest population, which shows how far an
+    absolute population bound is met by two FP64 codes at that scale."""
+    _need_gpu()
+    from mesohops_amd import HopsBatch
+    from mesohops_amd.workloads import chain_system
+    sysp = chain_system(10)
+    eom = {"EQUATION_OF_MOTION": "LINEAR"}
+    psi0 = np.zeros(10, complex)
+    psi0[0] = 1.0
+    tlen, tau, steps, dt = 2048.0, 1.0, 1000, 2.0
+    noise = _oracle_noise(sysp, 0, tlen, tau)
+    print("chain10 depth 4: one trajectory, %d steps, oracle" % steps, flush=True)
+    ref = O.Trajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 4})
+    ref.initialize(psi0)
+    ref.propagate(steps * dt, dt)
+    ref_psi = np.array(ref.psi_traj)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print("chain10 depth 4: one trajectory, %d steps, %s" % (steps, prec), flush=True)
+        b = HopsBatch(sysp, eom, noise, None, {"MAXHIER": 4}, noise_params=[noise], precision_mode=prec)
+        b.initialize(psi0)
+        b.propagate(steps * dt, dt)
+        out[prec] = np.array(b.psi_traj[0])
+        del b
+    pmax = float(np.max(np.abs(ref_psi) ** 2))
+    rel64 = float(np.max(np.abs(out["fp64"] - ref_psi)) / np.max(np.abs(ref_psi)))
+    abs64 = _pop_err(out["fp64"], ref_psi)
+    absmx = _pop_err(out["mixed"], ref_psi)
+    absmx64 = _pop_err(out["mixed"], out["fp64"])
+    print("largest population %.4e; FP64 device vs oracle: psi rel %.3e, population abs %.3e (rel %.3e); "
+          "complex64 stages vs oracle: population abs %.3e (rel %.3e), vs FP64 device abs %.3e"
+          % (pmax, rel64, abs64, abs64 / pmax, absmx, absmx / pmax, absmx64))
+    assert len(ref_psi) == steps + 1
+    assert rel64 < FP64_TOL
+    assert absmx / pmax <= POP_TOL
+
+
 # --- 10-site chain, depth 4, 256 trajectories, LINEAR ----------------------
 def test_chain10_depth4_linear_batch256():
     _need_gpu()
