@@ -140,6 +140,10 @@ int hops_rt_debug_select_states(void *basis, const double *e_state);
    state, 2 (state, error) pairs of the outside states; returns the count of
    doubles, copied to out when it fits in cap. */
 int hops_rt_debug_noise(void *basis, const float *z1, const float *z2, const double *zmem);
+/* Overwrites the two Zobrist words of (mode, depth) of a system, so that a
+   test can make two auxiliaries share a key: every key hit is confirmed
+   against the auxiliaries, and a collision fails the basis or selection. */
+int hops_rt_debug_set_zobrist(void *sys, int mode, int depth, unsigned long long a, unsigned long long b);
 long long hops_rt_debug_errors(void *basis, int which, double *out, long long cap);
 int hops_rt_debug_result(void *basis, int32_t *n_states, int32_t *states_out, int32_t *n_aux, int32_t *aux_ptr,
                          int32_t *aux_mode, int32_t *aux_val, long long cap);

@@ -142,6 +142,7 @@ def lib():
     L.hops_rt_debug_select.argtypes = [_vp, ctypes.POINTER(RtTrajParam), _vp, _vp, ctypes.c_double, _vp]
     L.hops_rt_debug_select_states.argtypes = [_vp, _vp]
     L.hops_rt_debug_noise.argtypes = [_vp, _vp, _vp, _vp]
+    L.hops_rt_debug_set_zobrist.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_ulonglong]
     L.hops_rt_debug_errors.restype = _ll
     L.hops_rt_debug_errors.argtypes = [_vp, ctypes.c_int, _vp, _ll]
     L.hops_rt_debug_result.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ll]
@@ -176,7 +177,7 @@ EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_prof
             "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
             "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",
             "hops_rt_debug_result", "hops_rt_debug_free", "hops_rt_debug_threshold", "hops_rt_debug_noise",
-            "hops_rt_debug_errors"]
+            "hops_rt_debug_errors", "hops_rt_debug_set_zobrist"]
 
 
 def set_option(name, value):

@@ -127,6 +127,30 @@ bool aux_step(const Aux &a, int m, int step, Aux &out) {
   }
   return true;
 }
+// b == a + dir * e_m exactly (dir = +1 or -1): the same (mode, depth)
+// entries except mode m, whose depth differs by dir.  Confirms a Zobrist-key
+// hit against the auxiliaries themselves.
+bool is_step(const Aux &a, const Aux &b, int m, int dir) {
+  size_t ia = 0, ib = 0;
+  int va = 0, vb = 0;
+  while (ia < a.size() || ib < b.size()) {
+    const int ma = ia < a.size() ? aux_mode(a[ia]) : INT32_MAX;
+    const int mb = ib < b.size() ? aux_mode(b[ib]) : INT32_MAX;
+    if (ma == m) {
+      va = aux_val(a[ia++]);
+      continue;
+    }
+    if (mb == m) {
+      vb = aux_val(b[ib++]);
+      continue;
+    }
+    if (ma != mb || a[ia] != b[ib]) return false;
+    ++ia;
+    ++ib;
+  }
+  return vb == va + dir;
+}
+
 // Reference order (basis/hops_aux.py:171-201, :460-480): depth, then the
 // sorted mode sequence (mode m repeated k_m times) lexicographically.
 bool aux_less(const Aux &a, const Aux &b) {
@@ -204,11 +228,15 @@ struct KeyTable {
     vals.assign(cap, -1);
     mask = cap - 1;
   }
-  void insert(const AKey &k, int v) {
+  // inserts k -> v; returns -1, or the value already held under key k (which
+  // is then kept)
+  int insert(const AKey &k, int v) {
     size_t h = hh(k) & mask;
     while (vals[h] >= 0 && !(keys[h] == k)) h = (h + 1) & mask;
+    if (vals[h] >= 0) return vals[h];
     keys[h] = k;
     vals[h] = v;
+    return -1;
   }
   int find(const AKey &k) const {
     if (vals.empty()) return -1;
@@ -299,7 +327,9 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
     B.aux_hash[i] = k.a;
     B.aux_hash2[i] = k.b;
     B.depth[i] = aux_depth(aux[i]);
-    B.index.insert(k, i);
+    // the list holds no duplicates, so an equal key is a collision of two
+    // different auxiliaries: refused rather than merged
+    if (B.index.insert(k, i) >= 0) return rt_fail("two auxiliaries of one basis share a 128-bit Zobrist key");
   }
   // modes: in use by the aux list, or of the states (hops_modes.py:111-171)
   std::vector<char> mflag(S.n_hm, 0);
@@ -422,6 +452,7 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
         const int m = aux_mode(e), v = aux_val(e);
         const int i = B.index.find(key_step(S, kj, m, v, v - 1));
         if (i < 0) continue;
+        if (!is_step(aux[i], aux[j], m, 1)) return rt_fail("Zobrist key collision between auxiliaries");
         up_i.push_back(i);
         up_j.push_back(j);
         up_m.push_back(B.rel_mode[m]);
@@ -550,6 +581,7 @@ int build_basis(const System &S, int eom, std::vector<int> states, std::vector<A
   for (int i = 0; i < nm; ++i) B.abs_lact[B.modes[i]] = B.mode_lact[i];
   for (int i = 0; i < nm; ++i) {
     const int fi = B.index.find(key_step(S, AKey(), B.modes[i], 0, 1));
+    if (fi >= 0 && !is_step(Aux(), aux[fi], B.modes[i], 1)) return rt_fail("Zobrist key collision between auxiliaries");
     if (fi >= 0 && B.mode_lact[i] >= 0) {
       B.first_aux.push_back(fi);
       B.first_lact.push_back(B.mode_lact[i]);
@@ -887,6 +919,9 @@ struct Selection {
   // outside state (state, error) of the last selection
   std::vector<double> err_aux, err_state;
   std::vector<std::pair<int, double>> err_out;
+  // a Zobrist-key hit named a different auxiliary than the one looked up
+  // (the selection is then invalid and the caller fails)
+  bool key_collision = false;
 };
 
 std::vector<int> list_sc(const System &S, const Basis &B) {
@@ -902,7 +937,8 @@ std::vector<int> list_sc(const System &S, const Basis &B) {
   return r;
 }
 
-void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
+// returns false if a Zobrist-key hit named another auxiliary
+bool markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
   // hops_fluxfilters.py:354-397; F[k * nm + lm]
   F.assign((size_t)B.nh * B.nm, 1);
   const int nm = B.nm;
@@ -911,10 +947,11 @@ void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
       if (!P.markov[B.modes[lm]]) continue;
       for (int k = 1; k < B.nh; ++k) F[(size_t)k * nm + lm] = 0;
       const int j = B.index.find(key_step(*B.sys, AKey(), B.modes[lm], 0, 1));
+      if (j >= 0 && !is_step(Aux(), B.aux[j], B.modes[lm], 1)) return false;
       if (j >= 0)
         for (int m2 = 0; m2 < nm; ++m2) F[(size_t)j * nm + m2] = 0;
     }
-  if (P.filters.empty()) return;
+  if (P.filters.empty()) return true;
   // Triangular: no flux up along a filtered mode from an auxiliary whose
   // depth in the filtered modes has reached kmax2.  LongEdge: from total
   // depth kmax2 on, an auxiliary with depth in a filtered mode steps up only
@@ -951,6 +988,7 @@ void markov_up(const Params &P, const Basis &B, std::vector<uint8_t> &F) {
     }
     for (size_t i = 0; i < F.size(); ++i) F[i] &= G[i];
   }
+  return true;
 }
 
 // The part of the selection that needs only psi = Pn[:n]: the diagonal
@@ -1244,7 +1282,10 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   // loop), so the arrays start uninitialised
   std::unique_ptr<double[]> E_up(new double[(size_t)nh * nm]), E_dn(new double[(size_t)nh * nm]);
   std::vector<uint8_t> F_up;
-  markov_up(P, B, F_up);
+  if (!markov_up(P, B, F_up)) {
+    out.key_collision = true;
+    return;
+  }
   std::vector<double> aw(nm), gw2(nm), nX(nm);
   // the diagonal L entries of each mode as CSR: state, |M2|^2, |M2 - X|^2;
   // with off-diagonal L each state's sum over its destinations d != s of
@@ -1429,10 +1470,21 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     while (tab[h].id >= 0 && !(tab[h].a == a && keyb[tab[h].id] == b)) h = (h + 1) & (cap - 1);
     return h;
   };
+  Aux t1, t2;
   for (size_t i = 0; i < cs.size(); ++i) {
     if (i + 8 < cs.size()) __builtin_prefetch(&tab[KeyTable::hh(cs[i + 8].key) & (cap - 1)]);
     const Cand &c = cs[i];
     size_t h = slot_of(c.key.a, c.key.b);
+    if (tab[h].id >= 0) {
+      // a merge: the two candidates must reach the same auxiliary
+      const Rep &r = rep[tab[h].id];
+      aux_step(B.aux[r.k], B.modes[r.lm], r.dir, t1);
+      aux_step(B.aux[c.k], B.modes[c.lm], c.dir, t2);
+      if (t1 != t2) {
+        out.key_collision = true;
+        return;
+      }
+    }
     if (tab[h].id < 0) {
       if (2 * (vals.size() + 1) > cap) {
         std::vector<Ent> old;
@@ -1510,18 +1562,28 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
     const Aux &a = sel.aux_bound[i];
     const AKey ka = key_of(S, a);
     if (from_basis) {
-      btab.insert(ka, (int)i);
+      if (btab.insert(ka, (int)i) >= 0) sel.key_collision = true;
     } else {
       for (int lm = 0; lm < nm; ++lm) {
         const int v = aux_get(a, B.modes[lm]);
         if (v >= 255) continue;
         const int j = B.index.find(key_step(S, ka, B.modes[lm], v, v + 1));
-        if (j >= 0) F_dn[(size_t)lm * nh + j] = 1;
+        if (j < 0) continue;
+        if (!is_step(a, B.aux[j], B.modes[lm], 1)) {
+          sel.key_collision = true;
+          return;
+        }
+        F_dn[(size_t)lm * nh + j] = 1;
       }
     }
     for (char32_t e : a) {
       const int j = B.index.find(key_step(S, ka, aux_mode(e), aux_val(e), aux_val(e) - 1));
-      if (j >= 0) F_up[(size_t)B.rel_mode[aux_mode(e)] * nh + j] = 1;
+      if (j < 0) continue;
+      if (!is_step(a, B.aux[j], aux_mode(e), -1)) {
+        sel.key_collision = true;
+        return;
+      }
+      F_up[(size_t)B.rel_mode[aux_mode(e)] * nh + j] = 1;
     }
   }
   if (from_basis)  // the same F_dn from the basis side: j - e_m is a boundary aux
@@ -1529,7 +1591,13 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
       const AKey kj{B.aux_hash[j], B.aux_hash2[j]};
       for (int q = B.aux_ptr[j]; q < B.aux_ptr[j + 1]; ++q) {
         const int m = B.aux_mode_t[q], v = B.aux_val_t[q];
-        if (btab.find(key_step(S, kj, m, v, v - 1)) >= 0) F_dn[(size_t)B.rel_mode[m] * nh + j] = 1;
+        const int i = btab.find(key_step(S, kj, m, v, v - 1));
+        if (i < 0) continue;
+        if (!is_step(sel.aux_bound[i], B.aux[j], m, 1)) {
+          sel.key_collision = true;
+          return;
+        }
+        F_dn[(size_t)B.rel_mode[m] * nh + j] = 1;
       }
     }
   for (int lm = 0; lm < nm; ++lm) {
@@ -1570,10 +1638,19 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
         }
     KeyTable ntab;
     ntab.reset(sel.aux_bound.size());
-    for (size_t i = 0; i < sel.aux_bound.size(); ++i) ntab.insert(key_of(S, sel.aux_bound[i]), (int)i);
-    auto in_new = [&](const AKey &key) {
+    for (size_t i = 0; i < sel.aux_bound.size(); ++i)
+      if (ntab.insert(key_of(S, sel.aux_bound[i]), (int)i) >= 0) sel.key_collision = true;
+    // membership of base + dir * e_m in the new list, each key hit confirmed
+    // against the auxiliaries
+    auto in_new = [&](const AKey &key, const Aux &base, int m, int dir) {
       const int j = B.index.find(key);
-      return j >= 0 ? sel.stable_mask[j] != 0 : ntab.find(key) >= 0;
+      if (j >= 0) {
+        if (!is_step(base, B.aux[j], m, dir)) sel.key_collision = true;
+        return sel.stable_mask[j] != 0;
+      }
+      const int i = ntab.find(key);
+      if (i >= 0 && !is_step(base, sel.aux_bound[i], m, dir)) sel.key_collision = true;
+      return i >= 0;
     };
     const size_t no = offm.size();
     fo.assign(sel.stable_idx.size() * no, 0);
@@ -1584,8 +1661,8 @@ void select_states(const Params &P, const Basis &B, const cplx *pn, const double
         const int lm = offm[o], m = B.modes[lm];
         const int v = aux_get(B.aux[k], m);
         uint8_t f = 0;
-        if (v < 255 && in_new(key_step(S, kk, m, v, v + 1))) f |= 1;
-        if (v > 0 && in_new(key_step(S, kk, m, v, v - 1))) f |= 2;
+        if (v < 255 && in_new(key_step(S, kk, m, v, v + 1), B.aux[k], m, 1)) f |= 1;
+        if (v > 0 && in_new(key_step(S, kk, m, v, v - 1), B.aux[k], m, -1)) f |= 2;
         fo[p * no + o] = f;
         if (!f) continue;
         const double gw2 = std::norm(S.G[m] / S.W[m]);
@@ -3163,14 +3240,20 @@ int define_bases(Batch &b, const std::vector<int> &list) {
     else
       RT_TRY(rc);
   }
-  return define_bases_host(b, host);
+  RT_TRY(define_bases_host(b, host));
+  for (int s : list)
+    if (b.slots[s].sel.key_collision)
+      return rt_fail("slot %d: two auxiliaries share a 128-bit Zobrist key; the selection is refused", s);
+  return 0;
 }
 
 bool same_basis(const Basis &B, const std::vector<int> &states, const std::vector<Aux> &aux) {
   if (states.size() != B.states.size() || aux.size() != B.aux.size()) return false;
   if (!std::equal(states.begin(), states.end(), B.states.begin())) return false;
-  for (const Aux &a : aux)
-    if (B.index.find(key_of(*B.sys, a)) < 0) return false;
+  for (const Aux &a : aux) {
+    const int j = B.index.find(key_of(*B.sys, a));
+    if (j < 0 || B.aux[j] != a) return false;
+  }
   return true;
 }
 
@@ -3251,7 +3334,9 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
     RT_TRY(st.place(tab_bytes[i], &dtab[i], &htab[i]));
     RT_TRY(st.place((size_t)(Bn.nh + Bn.n) * 4, &dinv[i], &hinv[i]));
   }
-  // fill tables and inverse maps in parallel, straight into pinned memory
+  // fill tables and inverse maps in parallel, straight into pinned memory;
+  // each key hit of the inverse map is confirmed against the auxiliaries
+  std::vector<uint8_t> collide(ch.size(), 0);
   b.pool->parallel_for((int)ch.size(), [&](int i) {
     Slot &sl = b.slots[ch[i]];
     const Layout &L = lays[i];
@@ -3259,9 +3344,15 @@ int update_bases(Batch &b, const std::vector<int> &list, std::vector<std::vector
       if (L.tabs[t].second) std::memcpy(htab[i] + L.tab_off[t], L.tabs[t].first, L.tabs[t].second);
     const Basis &Bo = sl.B, &Bn = sl.Bnew;
     int32_t *inv = reinterpret_cast<int32_t *>(hinv[i]);
-    for (int j = 0; j < Bn.nh; ++j) inv[j] = Bo.index.find(AKey{Bn.aux_hash[j], Bn.aux_hash2[j]});
+    for (int j = 0; j < Bn.nh; ++j) {
+      const int io = Bo.index.find(AKey{Bn.aux_hash[j], Bn.aux_hash2[j]});
+      inv[j] = io >= 0 && Bo.aux[io] == Bn.aux[j] ? io : -1;
+      if (io >= 0 && inv[j] < 0) collide[i] = 1;
+    }
     for (int q = 0; q < Bn.n; ++q) inv[Bn.nh + q] = Bo.pos[Bn.states[q]];
   });
+  for (uint8_t c : collide)
+    if (c) return rt_fail("two auxiliaries of an old and a new basis share a 128-bit Zobrist key");
   std::vector<hops_segment_t> segs;
   for (size_t i = 0; i < ch.size(); ++i) {
     Slot &sl = b.slots[ch[i]];
@@ -4093,7 +4184,17 @@ int hops_rt_debug_select(void *p, const hops_rt_traj_param_t *prm, const double 
   d->P.keep_err = true;
   if (d->B.has_off && d->coef.empty()) return rt_fail("off-diagonal L-operators: hops_rt_debug_noise first");
   select_hier(d->P, d->B, d->pn.data(), e_aux, d->sel, d->coef.empty() ? nullptr : d->coef.data());
+  if (d->sel.key_collision) return rt_fail("two auxiliaries share a 128-bit Zobrist key");
   if (stable_out) std::copy(d->sel.stable_mask.begin(), d->sel.stable_mask.end(), stable_out);
+  return 0;
+}
+
+int hops_rt_debug_set_zobrist(void *sys, int mode, int depth, unsigned long long a, unsigned long long b) {
+  if (!sys) return rt_fail("NULL argument");
+  auto *S = static_cast<System *>(sys);
+  if (mode < 0 || mode >= S->n_hm || depth < 1 || depth > 255) return rt_fail("bad (mode, depth)");
+  S->zob[(size_t)mode * 256 + depth] = a;
+  S->zob2[(size_t)mode * 256 + depth] = b;
   return 0;
 }
 
@@ -4132,6 +4233,7 @@ int hops_rt_debug_select_states(void *p, const double *e_state) {
   auto *d = static_cast<DebugBasis *>(p);
   if (d->P.delta_s > 0 && !e_state) return rt_fail("state errors missing");
   select_states(d->P, d->B, d->pn.data(), e_state, d->sel);
+  if (d->sel.key_collision) return rt_fail("two auxiliaries share a 128-bit Zobrist key");
   return 0;
 }
 

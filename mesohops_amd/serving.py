@@ -22,6 +22,7 @@ reached.  A request that cannot fit even alone raises UnsupportedRequest.
 """
 from __future__ import annotations
 
+import itertools
 import math
 import time
 from collections import deque
@@ -88,6 +89,9 @@ class ServingLoop:
         """Serve ``requests`` (arrival times relative to the start).  Returns
         a dict of per-request results and the serving statistics."""
         reqs = deque(sorted(requests, key=lambda r: r.arrival))
+        arrivals = np.array([r.arrival for r in reqs])
+        timeline = []        # (t, arrived, waiting, active, done) about once a second
+        last_sample = -1.0
         futures = {}
         active = {}          # slot -> [req, steps_done, keep, t_admit, peak device bytes]
         done = {}
@@ -100,10 +104,15 @@ class ServingLoop:
         last_log = start
         while reqs or active:
             now = time.perf_counter() - start
+            if now - last_sample >= 1.0:
+                last_sample = now
+                arrived = int(np.searchsorted(arrivals, now, side="right"))
+                waiting = sum(1 for r in reqs if r.arrival <= now)
+                timeline.append((round(now, 2), arrived, waiting, len(active), len(done)))
             if time_limit is not None and now > time_limit:
                 break
             # host preparation of the next requests, in arrival order
-            for r in list(reqs)[:prep_ahead]:
+            for r in itertools.islice(reqs, prep_ahead):
                 if r.rid not in futures:
                     futures[r.rid] = self._prep.submit(self._prepare, r)
             # admission: arrived, prepared, a free slot and room in the arena
@@ -187,7 +196,11 @@ class ServingLoop:
                                                                       steps_total))
         wall = time.perf_counter() - start
         lat = np.array([d["latency"] for d in done.values()]) if done else np.zeros(0)
+        arrived = int(np.searchsorted(arrivals, wall, side="right"))
+        timeline.append((round(wall, 2), arrived, sum(1 for r in reqs if r.arrival <= wall), len(active), len(done)))
         stats = {"completed": len(done), "admitted_unfinished": len(active), "not_admitted": len(reqs),
+                 "arrived": arrived, "waiting_at_end": timeline[-1][2],
+                 "timeline": timeline,
                  "wall_s": wall, "traj_steps": steps_total, "traj_steps_per_s": steps_total / wall if wall else 0.0,
                  "iterations": n_iter, "preempted": n_preempt,
                  "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,

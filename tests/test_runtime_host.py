@@ -226,3 +226,51 @@ def test_runtime_error_terms_match_oracle(name):
             assert abs(got_out[d] - v) <= 1e-10 * abs(v) + 1e-300, (step, d, got_out[d], v)
         n_out += len(eo)
     assert n_out > 0
+
+
+def test_zobrist_collisions_fail_loudly():
+    """Auxiliary identity in the native runtime is a 128-bit Zobrist key, and
+    every key hit is confirmed against the auxiliaries themselves: with two
+    (mode, depth) entries given the same key words, a basis holding both is
+    refused, and a selection whose lookups meet the collision raises instead
+    of returning a basis."""
+    from mesohops_amd import _lib
+    from mesohops_amd.basis import SystemInfo
+    from mesohops_amd.errors import EngineError
+    from mesohops_amd.ragged import DebugBasis, RtSystem, traj_param
+    sysp = systems.chain(4)
+    info = SystemInfo(sysp)
+    rs = RtSystem(info)
+    L = _lib.lib()
+    # distinct auxiliaries are accepted before the tables are changed
+    DebugBasis(rs, [0, 1, 2, 3], [(), ((0, 1),), ((2, 1),)], 2)
+    for m in (0, 2):
+        assert L.hops_rt_debug_set_zobrist(rs.handle, m, 1, 0x1234567890ABCDEF, 0x0FEDCBA987654321) == 0
+    with pytest.raises(EngineError, match="Zobrist"):
+        DebugBasis(rs, [0, 1, 2, 3], [(), ((0, 1),), ((2, 1),)], 2)
+    # the first-level lookup of e_2 hits e_0: even a basis holding only e_0
+    # is refused
+    with pytest.raises(EngineError, match="Zobrist"):
+        DebugBasis(rs, [0, 1, 2, 3], [(), ((0, 1),)], 2)
+    # a collision one level deeper that only a selection meets: 2 e_4 gets the
+    # key of e_5 + e_6; the basis holds e_4 and e_5 + e_6, the selection grows
+    # the hierarchy (tiny delta) and either refuses the selection or returns a
+    # list that no basis accepts
+    rs2 = RtSystem(info)
+    A, B2 = 0x1111222233334444, 0x5555666677778888
+    C, D = 0x9999AAAABBBBCCCC, 0xDDDDEEEEFFFF0001
+    assert L.hops_rt_debug_set_zobrist(rs2.handle, 5, 1, A, C) == 0
+    assert L.hops_rt_debug_set_zobrist(rs2.handle, 6, 1, B2, D) == 0
+    assert L.hops_rt_debug_set_zobrist(rs2.handle, 4, 2, A ^ B2, C ^ D) == 0
+    aux = [(), ((4, 1),), ((5, 1),), ((6, 1),), ((5, 1), (6, 1))]
+    db = DebugBasis(rs2, [0, 1, 2, 3], aux, 2)
+    rng = np.random.default_rng(2)
+    pn = rng.normal(size=4 * len(aux)) + 1j * rng.normal(size=4 * len(aux))
+    pn /= np.linalg.norm(pn[:4])
+    prm = traj_param(2, 3, 1e-9, 0.0, 1, 0.0, 1.0, 1, 10, 0)
+    with pytest.raises(EngineError, match="Zobrist"):
+        db.select(prm, pn, np.ones(len(aux)), 1.0)
+        db.select_states(None)
+        st_new, aux_new = db.result()
+        assert ((4, 2),) in aux_new
+        DebugBasis(rs2, st_new, aux_new, 2)
