@@ -53,7 +53,9 @@ typedef struct hops_rt_system_desc {
 typedef struct hops_rt_traj_param {
   int32_t eom, maxhier, update_step, early_integrator; /* 0 INCH_WORM, 1 STATIC */
   int32_t early_steps, inchworm_cap, noise_step, n_t;
-  int32_t it0, record, n_aux0, pad;
+  int32_t it0, record, n_aux0;
+  int32_t noise_c128;      /* 1: z1_d / z2_d hold complex128 (interpolated or
+                              complex128 PRE_CALCULATED noise), else complex64 */
   double delta_a, delta_s, f_discard, dt;
   const uint8_t *markov;   /* [n_hmode] Markovian filter flags, or NULL */
   const int32_t *aux0_ptr, *aux0_mode, *aux0_val;
@@ -83,6 +85,14 @@ int hops_rt_batch_add(void *batch, void *sys, const hops_rt_traj_param_t *prm, c
                       const void *z1_d, const void *z2_d);
 /* Release a slot and its arena memory. */
 int hops_rt_batch_remove(void *batch, int slot);
+/* Points a slot at new noise arrays for its next steps: z1_d / z2_d
+   ([n_lop][n_t], complex64 or with c128 complex128, caller-owned), the
+   noise index it of the slot's current time and the index step noise_step
+   per half RK4 step.  Used per propagate call for interpolated noise, whose
+   values are sampled on the call's own half-step grid
+   (noise/hops_noise.py:379-385); no counterpart in the reference. */
+int hops_rt_slot_noise(void *batch, int slot, const void *z1_d, const void *z2_d, int n_t, int it,
+                       int noise_step, int c128);
 /* Device bytes a trajectory would take at the given basis size, for
    admission by HBM footprint. */
 long long hops_rt_slot_estimate(void *batch, void *sys, int n_state, int n_hier);

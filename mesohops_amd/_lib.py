@@ -65,7 +65,7 @@ class RtTrajParam(ctypes.Structure):
     """hops_rt_traj_param_t (include/hops_runtime.h)."""
     _fields_ = [("eom", _i32), ("maxhier", _i32), ("update_step", _i32), ("early_integrator", _i32),
                 ("early_steps", _i32), ("inchworm_cap", _i32), ("noise_step", _i32), ("n_t", _i32),
-                ("it0", _i32), ("record", _i32), ("n_aux0", _i32), ("pad", _i32),
+                ("it0", _i32), ("record", _i32), ("n_aux0", _i32), ("noise_c128", _i32),
                 ("delta_a", ctypes.c_double), ("delta_s", ctypes.c_double), ("f_discard", ctypes.c_double),
                 ("dt", ctypes.c_double), ("markov", _vp), ("aux0_ptr", _vp), ("aux0_mode", _vp),
                 ("aux0_val", _vp), ("n_filters", _i32), ("pad2", _i32), ("filter_kind", _vp), ("filter_kmax", _vp),
@@ -112,6 +112,8 @@ def lib():
     L.hops_slots_build.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]
     L.hops_slots_permute.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.hops_copy_segments.argtypes = [_vp, ctypes.c_int, _vp]
+    L.hops_rt_slot_noise.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int]
     L.hops_gather_u64.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.hops_rt_system_create.restype = _vp
     L.hops_rt_system_create.argtypes = [ctypes.POINTER(RtSystemDesc)]
@@ -172,7 +174,7 @@ EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_prof
             "hops_slots_step", "hops_slots_pn", "hops_slots_errors", "hops_slots_build", "hops_slots_permute", "hops_copy_segments",
             "hops_gather_u64",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
-            "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",
+            "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_noise", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",
             "hops_rt_batch_option", "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
             "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
             "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",

@@ -1036,15 +1036,24 @@ void sel_common(const Basis &B, const cplx *pn, Selection &out, const cplx *coef
 // complex64, as the reference's get_noise returns it) plus the noise memory
 // of the L's modes in the basis (hops_basis.py:936-953).  z2 may be NULL
 // (noise 2 ZERO); zmem is indexed by absolute mode.
-void noise_coef(const Basis &B, const std::complex<float> *z1, const std::complex<float> *z2, const cplx *zmem,
+// complex128 noise (c128) is combined in double arithmetic, as the reference
+// combines its complex128 noise.
+void noise_coef(const Basis &B, const void *z1, const void *z2, bool c128, const cplx *zmem,
                 std::vector<cplx> &coef) {
   std::vector<cplx> comp(B.nl, cplx(0, 0));
   for (int lm = 0; lm < B.nm; ++lm) comp[B.mode_lrel[lm]] += zmem[B.modes[lm]];
   coef.resize(B.nl);
   for (int j = 0; j < B.nl; ++j) {
-    const float re = z1[j].real() + (z2 ? z2[j].imag() : 0.0f);
-    const float im = -z1[j].imag() - (z2 ? z2[j].real() : 0.0f);
-    coef[j] = cplx(re, im) + comp[j];
+    if (c128) {
+      const cplx a = static_cast<const cplx *>(z1)[j], b2 = z2 ? static_cast<const cplx *>(z2)[j] : cplx(0, 0);
+      coef[j] = cplx(a.real() + b2.imag(), -a.imag() - b2.real()) + comp[j];
+    } else {
+      const auto a = static_cast<const std::complex<float> *>(z1)[j];
+      const auto b2 = z2 ? static_cast<const std::complex<float> *>(z2)[j] : std::complex<float>(0, 0);
+      const float re = a.real() + b2.imag();
+      const float im = -a.imag() - b2.real();
+      coef[j] = cplx(re, im) + comp[j];
+    }
   }
 }
 
@@ -1960,6 +1969,7 @@ struct Slot {
   hops_op_t op{};
   hops_batch_t bt{};
   const void *z1 = nullptr, *z2 = nullptr;
+  bool c128 = false;  // the noise arrays hold complex128
   int it = 0, early_counter = 0;
   long long steps = 0;
   std::vector<cplx> psi0;
@@ -2238,6 +2248,7 @@ void fill_bt(const Batch &b, const Slot &sl, const Basis &B, const Layout &L, ch
   bt.acc_d = base + L.acc;
   bt.z1_d = sl.z1;
   bt.z2_d = sl.z2;
+  bt.noise_c128 = sl.c128 ? 1 : 0;
   char *pb = b.arena.ptr(sl.pers);
   const size_t Ma = B.sys->n_hm;
   bt.zmem_d = reinterpret_cast<double *>(pb);
@@ -2348,9 +2359,11 @@ int define_bases_host(Batch &b, const std::vector<int> &list) {
     if (!sl.B.has_off) continue;
     if (sl.it < 0 || sl.it >= sl.prm.n_t) return rt_fail("noise index %d outside the noise grid", sl.it);
     zat[s] = zsrc.size();
+    const size_t zw = sl.c128 ? 16 : 8;  // bytes per noise value
     for (int z = 0; z < (sl.z2 ? 2 : 1); ++z) {
       const char *zb = static_cast<const char *>(z ? sl.z2 : sl.z1);
-      for (int l : sl.B.absL) zsrc.push_back(zb + ((size_t)l * sl.prm.n_t + sl.it) * 8);
+      for (int l : sl.B.absL)
+        for (size_t w = 0; w < zw; w += 8) zsrc.push_back(zb + ((size_t)l * sl.prm.n_t + sl.it) * zw + w);
     }
     const char *zm = reinterpret_cast<const char *>(b.bts[s].zmem_d);
     for (int w = 0; w < 2 * sl.sys->n_hm; ++w) zsrc.push_back(zm + (size_t)w * 8);
@@ -2399,10 +2412,11 @@ int define_bases_host(Batch &b, const std::vector<int> &list) {
     for (int s : list) {
       Slot &sl = b.slots[s];
       if (!sl.B.has_off) continue;
-      const auto *z1 = reinterpret_cast<const std::complex<float> *>(hz + zat[s] * 8);
-      const auto *z2 = sl.z2 ? z1 + sl.B.nl : nullptr;
-      const auto *zm = reinterpret_cast<const cplx *>(hz + (zat[s] + (size_t)sl.B.nl * (sl.z2 ? 2 : 1)) * 8);
-      noise_coef(sl.B, z1, z2, zm, sl.zcoef);
+      const size_t zw = sl.c128 ? 16 : 8;
+      const char *z1 = hz + zat[s] * 8;
+      const char *z2 = sl.z2 ? z1 + (size_t)sl.B.nl * zw : nullptr;
+      const auto *zm = reinterpret_cast<const cplx *>(z1 + (size_t)sl.B.nl * zw * (sl.z2 ? 2 : 1));
+      noise_coef(sl.B, z1, z2, sl.c128, zm, sl.zcoef);
     }
   }
   // the selection reads Pn and the error sums in place (pinned memory); the
@@ -3902,6 +3916,7 @@ int hops_rt_batch_add(void *p, void *sys, const hops_rt_traj_param_t *prm, const
   std::vector<Aux> aux = P.delta_a > 0 ? std::vector<Aux>{Aux()} : P.aux0;
   sl.z1 = z1_d;
   sl.z2 = z2_d;
+  sl.c128 = prm->noise_c128 != 0;
   sl.it = P.it0;
   RT_TRY(build_basis(S, P.eom, states, aux, sl.B));
   sl.pers = b.arena.alloc(pers_bytes(S));
@@ -3950,6 +3965,29 @@ int hops_rt_batch_remove(void *p, int slot) {
   sl = Slot();
   b.ops[slot] = hops_op_t{};
   b.bts[slot] = hops_batch_t{};
+  return 0;
+}
+
+int hops_rt_slot_noise(void *p, int slot, const void *z1_d, const void *z2_d, int n_t, int it, int noise_step,
+                       int c128) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  if (!z1_d || n_t <= 0 || it < 0 || it >= n_t || noise_step <= 0) return rt_fail("hops_rt_slot_noise: bad noise");
+  Slot &sl = b.slots[slot];
+  sl.z1 = z1_d;
+  sl.z2 = z2_d;
+  sl.c128 = c128 != 0;
+  sl.it = it;
+  sl.prm.n_t = n_t;
+  sl.prm.noise_step = noise_step;
+  for (hops_batch_t *bt : {&sl.bt, &b.bts[slot]}) {
+    bt->z1_d = z1_d;
+    bt->z2_d = z2_d;
+    bt->n_t = n_t;
+    bt->noise_step = noise_step;
+    bt->noise_c128 = sl.c128 ? 1 : 0;
+  }
   return 0;
 }
 
@@ -4203,8 +4241,7 @@ int hops_rt_debug_noise(void *p, const float *z1, const float *z2, const double 
   auto *d = static_cast<DebugBasis *>(p);
   std::vector<cplx> zm(d->B.sys->n_hm);
   for (size_t m = 0; m < zm.size(); ++m) zm[m] = cplx(zmem[2 * m], zmem[2 * m + 1]);
-  noise_coef(d->B, reinterpret_cast<const std::complex<float> *>(z1), reinterpret_cast<const std::complex<float> *>(z2),
-             zm.data(), d->coef);
+  noise_coef(d->B, z1, z2, false, zm.data(), d->coef);
   return 0;
 }
 

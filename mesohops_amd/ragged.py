@@ -88,7 +88,7 @@ def _aux_from_csr(ptr, mode, val):
 
 def traj_param(eom_kind, maxhier, delta_a, delta_s, update_step, f_discard, dt, noise_step, n_t, it0,
                early_integrator="INCH_WORM", early_steps=5, inchworm_cap=5, markov=None, aux0=None,
-               record=True, filters=None):
+               record=True, filters=None, noise_c128=False):
     """hops_rt_traj_param_t and the arrays it points to (kept alive on the
     returned object)."""
     p = RtTrajParam()
@@ -99,6 +99,7 @@ def traj_param(eom_kind, maxhier, delta_a, delta_s, update_step, f_discard, dt, 
     p.early_integrator = 0 if early_integrator == "INCH_WORM" else 1
     p.early_steps, p.inchworm_cap = int(early_steps), int(inchworm_cap)
     p.noise_step, p.n_t, p.it0, p.record = int(noise_step), int(n_t), int(it0), int(bool(record))
+    p.noise_c128 = int(bool(noise_c128))
     p.delta_a, p.delta_s, p.f_discard, p.dt = float(delta_a), float(delta_s), float(f_discard), float(dt)
     keep = []
     if markov is not None:
@@ -275,6 +276,12 @@ class RaggedBatch:
         check(lib().hops_rt_slot_zmem(self.h, int(slot), _ptr(out)))
         return out
 
+    def slot_noise(self, slot, z1, z2, n_t, it, noise_step, c128):
+        """Point a slot at new device noise arrays (hops_rt_slot_noise)."""
+        check(lib().hops_rt_slot_noise(self.h, int(slot), ctypes.c_void_p(z1.data_ptr()),
+                                       None if z2 is None else ctypes.c_void_p(z2.data_ptr()), int(n_t), int(it),
+                                       int(noise_step), int(bool(c128))))
+
     def records(self, slot, nstates, clear=False):
         """Dense psi(t) [n_rec, nstates], state lists and n_hier per record."""
         inf = self.info(slot)
@@ -393,13 +400,21 @@ class AdaptiveBatch:
                               n_threads=self.n_threads, device=self.device)
         self.rt.set_option("device_select", self.device_select)
         sp = self.rt.stream
-        if self.noise1[0].interpolate or self.noise2[0].interpolate:
-            raise UnsupportedRequest("INTERPOLATE noise", "AdaptiveBatch")
+        n1, n2 = self.noise1[0], self.noise2[0]
+        if n2.model != "ZERO" and n2.interpolate != n1.interpolate:
+            raise UnsupportedRequest("INTERPOLATE set for one noise and not the other", "AdaptiveBatch")
+        # INTERPOLATE: the grid noise stays in HBM in complex128 and each
+        # propagate call samples its cubic interpolation on the call's own
+        # half-step grid (noise/hops_noise.py:242-243, 379-385); complex128
+        # PRE_CALCULATED arrays are used as given
+        self.interp = n1.interpolate
+        self.c128 = any(nz.needs_c128() for nz in self.noise1 + self.noise2)
         nl = info.n_l2
-        self.z1 = device_noise(self.noise1, list(range(nl)), self.device, sp)
+        self.z1 = device_noise(self.noise1, list(range(nl)), self.device, sp, c128=self.c128)
         self.z2 = None
-        if self.noise2[0].model != "ZERO":
-            self.z2 = device_noise(self.noise2, list(range(nl)), self.device, sp)
+        if n2.model != "ZERO":
+            self.z2 = device_noise(self.noise2, list(range(nl)), self.device, sp, c128=self.c128)
+        self._zcall = []
         torch.cuda.synchronize(self.device)
         markov = None
         filters = []
@@ -429,13 +444,17 @@ class AdaptiveBatch:
     def _admit(self, tau):
         info = self.setup.info
         nz1 = self.noise1[0]
-        ns = int(round(tau * 0.5 / nz1.param["TAU"]))
+        # interpolated noise: the slots read each call's own grid
+        # (propagate), so the grid noise given here is never sampled
+        ns = max(1, int(round(tau * 0.5 / nz1.param["TAU"]))) if self.interp else \
+            int(round(tau * 0.5 / nz1.param["TAU"]))
         integ = self.setup.integ
         for b in range(self.B):
             prm = traj_param(self.setup.eom_kind, self.setup.hier["MAXHIER"], self.delta_a, self.delta_s,
                              self.update_step, self.f_discard, tau, ns, nz1.n_t, nz1.time_index(0.0),
                              integ["EARLY_ADAPTIVE_INTEGRATOR"], integ["EARLY_INTEGRATOR_STEPS"],
-                             integ["INCHWORM_CAP"], self._markov, self._aux0, self.record, filters=self._filters)
+                             integ["INCHWORM_CAP"], self._markov, self._aux0, self.record, filters=self._filters,
+                             noise_c128=self.c128)
             z1 = self.z1[b]
             z2 = None if self.z2 is None else self.z2[b]
             self.slots.append(self.rt.add(self.system, prm, self._psi0, z1, z2))
@@ -453,13 +472,26 @@ class AdaptiveBatch:
         if (self.t + t_advance + tau) > nz1.param["TLEN"]:
             raise TrajectoryError("Trajectory times longer than noise.param['TLEN'] =" + str(nz1.param["TLEN"]))
         ns = int(round(tau * 0.5 / nz1.param["TAU"]))
-        if abs(tau * 0.5 - ns * nz1.param["TAU"]) >= _precision:
+        if not self.interp and abs(tau * 0.5 - ns * nz1.param["TAU"]) >= _precision:
             raise UnsupportedRequest("Off axis t-samples when INTERPOLATE = False", "NoiseModel.get_noise()")
         if not self.slots:
             self._admit(tau)
         elif abs(tau - self._tau) > 1e-12:
             raise UnsupportedRequest("changing tau between propagate calls", "AdaptiveBatch.propagate")
         n_steps = int(np.ceil(t_advance / tau))
+        if self.interp:
+            from .noise import spline_noise
+            sp = self.rt.stream
+            n_out = 2 * n_steps + 1
+            zcall = []
+            for b, slot in enumerate(self.slots):
+                z1 = spline_noise(self.z1[b], nz1.param["TAU"], self.t, 0.5 * tau, n_out, sp,
+                                  nz1.param["FLAG_REAL"])
+                z2 = None if self.z2 is None else spline_noise(self.z2[b], nz2.param["TAU"], self.t, 0.5 * tau,
+                                                               n_out, sp, nz2.param["FLAG_REAL"])
+                self.rt.slot_noise(slot, z1, z2, n_out, 0, 1, True)
+                zcall.append((z1, z2))
+            self._zcall = zcall  # the previous call's arrays are no longer read
         self.rt.step(n_steps)
         self.t += n_steps * tau
 

@@ -138,6 +138,95 @@ def test_adaptive_batch_offdiagonal_matches_oracle(name):
     assert moved
 
 
+def _c128_noise(sysp, seed, tlen, tau):
+    """The filter's complex128 values (not rounded to complex64)."""
+    nz = O.Noise({"SEED": seed, "MODEL": "FFT_FILTER", "TLEN": tlen, "TAU": tau}, O.derive_system(sysp), 1)
+    nz.prepare()
+    return O.correlated_noise(np.complex64(nz.alpha(list(range(nz.n_l2)))), nz.z_unc)
+
+
+@pytest.mark.parametrize("name", ["chain6", "peierls"])
+def test_adaptive_batch_interpolated_noise_matches_oracle(name):
+    """INTERPOLATE in AdaptiveBatch (noise/hops_noise.py:242-243, 379-385):
+    stage times between noise points (TAU 1, step 1.5 fs), two propagate
+    calls, each reading the cubic noise of its own half-step grid in
+    complex128; with off-diagonal L-operators the selection's noise operator
+    reads the same complex128 values.  Basis sequence and psi(t) of every
+    trajectory follow the oracle (scipy's interp1d, as the reference) to
+    1e-10."""
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp = systems.chain(6) if name == "chain6" else systems.peierls_chain()
+    s0 = 2 if name == "chain6" else 0
+    nst = sysp["HAMILTONIAN"].shape[0]
+    tlen, tau = 60.0, 1.0
+    zs = [_c128_noise(sysp, seed, tlen, tau) for seed in (12, 40, 77)]
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau, "INTERPOLATE": True} for z in zs]
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(nst, complex)
+    psi0[s0] = 1
+    da, ds = (1e-3, 1e-3) if name == "chain6" else (2e-3, 3e-4)
+    print("case", name, flush=True)
+    ab = AdaptiveBatch(sysp, eom, nps[0], None, {"MAXHIER": 3}, noise_params=nps, precision_mode="fp64",
+                       arena_bytes=ARENA)
+    ab.make_adaptive(da, ds)
+    ab.initialize(psi0)
+    ab.propagate(21.0, 1.5)
+    ab.propagate(15.0, 1.5)
+    for b, n in enumerate(nps):
+        ref = O.AdaptiveTrajectory(sysp, eom, n, hierarchy_param={"MAXHIER": 3}, delta_a=da, delta_s=ds)
+        ref.initialize(psi0)
+        ref.propagate(21.0, 1.5)
+        ref.propagate(15.0, 1.5)
+        psi, states, nhier = ab.results(b)
+        assert len(states) == len(ref.state_traj) == 25
+        for i, (s_got, s_ref) in enumerate(zip(states, ref.state_traj)):
+            assert list(s_got) == list(s_ref), (b, i, s_got, s_ref)
+            assert nhier[i] == len(ref.aux_traj[i]), (b, i)
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        print("ragged interpolated %s traj %d: max |dpsi| = %.3e" % (name, b, err))
+        assert err < FP64_TOL
+
+
+def test_adaptive_batch_complex128_precalculated_matches_oracle():
+    """A PRE_CALCULATED complex128 noise array whose values complex64 would
+    round is used as given in AdaptiveBatch, as the reference uses it
+    (noise/hops_noise.py:262-263): psi(t) and bases follow the oracle to
+    1e-10, and differ from the run on the same noise rounded to complex64."""
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp = systems.chain(6)
+    tlen, tau = 50.0, 0.5
+    zs = [_c128_noise(sysp, seed, tlen, tau) for seed in (5, 6)]
+    assert not np.array_equal(zs[0].astype(np.complex64).astype(np.complex128), zs[0])
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+
+    def run(arrs):
+        nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in arrs]
+        ab = AdaptiveBatch(sysp, eom, nps[0], None, {"MAXHIER": 3}, noise_params=nps, precision_mode="fp64",
+                           arena_bytes=ARENA)
+        ab.make_adaptive(1e-3, 1e-3)
+        ab.initialize(psi0)
+        ab.propagate(30.0, 1.0)
+        return ab
+
+    ab = run(zs)
+    ab32 = run([z.astype(np.complex64) for z in zs])
+    for b, z in enumerate(zs):
+        ref = _oracle(systems.chain(6), "NORMALIZED NONLINEAR", z, tlen, tau, 3, 1e-3, 1e-3, 30.0, 1.0, psi0)
+        psi, states, _ = ab.results(b)
+        for i, (s_got, s_ref) in enumerate(zip(states, ref.state_traj)):
+            assert list(s_got) == list(s_ref), (b, i)
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        d32 = np.max(np.abs(psi - ab32.results(b)[0]))
+        print("ragged complex128 noise traj %d: max |dpsi| = %.3e; against the complex64-rounded noise %.3e"
+              % (b, err, d32))
+        assert err < FP64_TOL
+        assert d32 > 0
+
+
 def test_alone_equals_in_mix():
     """A trajectory's result does not depend on which others share its batch."""
     _need_gpu()
