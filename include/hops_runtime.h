@@ -51,7 +51,8 @@ typedef struct hops_rt_system_desc {
    with delta_a = 0 the hierarchy is aux0 (CSR over auxiliaries of absolute
    mode / depth entries, first entry the zero auxiliary). */
 typedef struct hops_rt_traj_param {
-  int32_t eom, maxhier, update_step, early_integrator; /* 0 INCH_WORM, 1 STATIC */
+  int32_t eom, maxhier, update_step, early_integrator; /* 0 INCH_WORM, 1 STATIC,
+                                                          2 STATIC_STATE_INCHWORM_HIERARCHY */
   int32_t early_steps, inchworm_cap, noise_step, n_t;
   int32_t it0, record, n_aux0;
   int32_t noise_c128;      /* 1: z1_d / z2_d hold complex128 (interpolated or
@@ -93,6 +94,12 @@ int hops_rt_batch_remove(void *batch, int slot);
    (noise/hops_noise.py:379-385); no counterpart in the reference. */
 int hops_rt_slot_noise(void *batch, int slot, const void *z1_d, const void *z2_d, int n_t, int it,
                        int noise_step, int c128);
+/* STATIC_BASIS (trajectory/hops_trajectory.py:299-317, :478-491) of a slot,
+   before its first step: states and auxiliaries (CSR of absolute mode /
+   depth entries) joined to its first basis in place of the first selection;
+   the early integrator 2 keeps these states through the early steps. */
+int hops_rt_slot_static_basis(void *batch, int slot, int n_states, const int32_t *states, int n_aux,
+                              const int32_t *aux_ptr, const int32_t *aux_mode, const int32_t *aux_val);
 /* Device bytes a trajectory would take at the given basis size, for
    admission by HBM footprint. */
 long long hops_rt_slot_estimate(void *batch, void *sys, int n_state, int n_hier);

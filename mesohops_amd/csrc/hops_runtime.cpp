@@ -667,6 +667,12 @@ struct Params {
   std::vector<UpFilter> filters;
   std::vector<Aux> aux0;
   bool keep_err = false;  // keep the error terms of a selection (inspection)
+  // STATIC_BASIS (trajectory/hops_trajectory.py:299-317): joined to the first
+  // basis in place of its selection; with early integrator 2
+  // (STATIC_STATE_INCHWORM_HIERARCHY) the early steps keep these states
+  bool has_static = false;
+  std::vector<int> static_states;
+  std::vector<Aux> static_aux;
 };
 
 // copies the Markovian and Triangular / LongEdge filter parameters
@@ -2313,6 +2319,10 @@ int fetch_psi(Batch &b, const std::vector<int> &list) {
     off[s] = (int64_t)tot;
     tot += b.slots[s].B.n;
   }
+  // the kernel reads the slots' descriptors on the device: upload them, as
+  // this may run before any step or selection of the call has (a first step
+  // whose slots all start from a static basis)
+  RT_TRY(upload_desc(b));
   RT_TRY(ensure_staging(b, tot * 16 + off.size() * 8 + list.size() * 4 + 4096));
   Stage st{&b};
   void *dl, *doff, *dout;
@@ -3466,13 +3476,31 @@ int init_pending(Batch &b) {
   for (size_t s = 0; s < b.slots.size(); ++s)
     if (b.slots[s].used && b.slots[s].pending_init) list.push_back((int)s);
   if (list.empty()) return 0;
-  std::vector<int> adapt;
+  std::vector<int> adapt, sel_list, stat;
   for (int s : list) {
     b.dts[s] = 1.0;
     Slot &sl = b.slots[s];
-    if (sl.prm.delta_a > 0 || sl.prm.delta_s > 0) adapt.push_back(s);
+    if (sl.prm.delta_a > 0 || sl.prm.delta_s > 0) {
+      adapt.push_back(s);
+      (sl.prm.has_static ? stat : sel_list).push_back(s);
+    }
   }
-  RT_TRY(define_bases(b, adapt));
+  RT_TRY(define_bases(b, sel_list));
+  // STATIC_BASIS: the first basis joins the static lists instead of
+  // selecting (trajectory/hops_trajectory.py:303-317); psi as it is
+  RT_TRY(fetch_psi(b, stat));
+  for (int s : stat) {
+    Slot &sl = b.slots[s];
+    std::vector<int> st(sl.B.states);
+    st.insert(st.end(), sl.prm.static_states.begin(), sl.prm.static_states.end());
+    std::sort(st.begin(), st.end());
+    st.erase(std::unique(st.begin(), st.end()), st.end());
+    sl.sel.states_new = st;
+    std::vector<Aux> ax(sl.B.aux);
+    ax.insert(ax.end(), sl.prm.static_aux.begin(), sl.prm.static_aux.end());
+    sl.sel.aux_new = std::move(ax);
+    sl.nrm = 1.0;  // psi = Pn[:n] as fetched
+  }
   std::vector<std::vector<int>> ts;
   std::vector<std::vector<Aux>> ta;
   std::vector<double> sc;
@@ -3534,7 +3562,7 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
     Slot &sl = b.slots[s];
     const bool adaptive = sl.prm.delta_a > 0 || sl.prm.delta_s > 0;
     const bool is_early = adaptive && sl.early_counter < sl.prm.early_steps;
-    if (is_early && sl.prm.early_integrator == 0) {
+    if (is_early && (sl.prm.early_integrator == 0 || sl.prm.early_integrator == 2)) {
       early.push_back(s);
       cp.add(sl.phi_prev, sl.bt.phi_d, (size_t)sl.B.n * sl.B.nh * 16);
       cp.add(sl.zmem_prev, sl.bt.zmem_d, (size_t)sl.sys->n_hm * 16);
@@ -3554,6 +3582,15 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
   std::vector<int> redo;
   std::vector<char> is_early(b.slots.size(), 0);
   for (int s : early) is_early[s] = 1;
+  // STATIC_STATE_INCHWORM_HIERARCHY: the early steps keep the static states
+  // (hops_trajectory.py:478-491)
+  auto pin_states = [&](const std::vector<int> &ls) {
+    for (int s : ls) {
+      Slot &sl = b.slots[s];
+      if (is_early[s] && sl.prm.early_integrator == 2) sl.sel.states_new = sl.prm.static_states;
+    }
+  };
+  pin_states(upd);
   for (int s : upd) {
     Slot &sl = b.slots[s];
     if (is_early[s] && !same_basis(sl.B, sl.sel.states_new, sl.sel.aux_new)) redo.push_back(s);
@@ -3590,6 +3627,7 @@ int batch_step_once(Batch &b, const std::vector<int> &list, int index_t) {
     RT_TRY(cp.run(b));
     RT_TRY(run_step(b, redo));
     RT_TRY(define_bases(b, redo));
+    pin_states(redo);
     std::vector<int> nxt;
     for (int s : redo) {
       Slot &sl = b.slots[s];
@@ -3885,6 +3923,7 @@ int hops_rt_batch_add(void *p, void *sys, const hops_rt_traj_param_t *prm, const
   P.maxhier = std::min(prm->maxhier, 255);
   P.update_step = prm->update_step;
   P.early_integrator = prm->early_integrator;
+  if (P.early_integrator < 0 || P.early_integrator > 2) return rt_fail("unknown early integrator %d", P.early_integrator);
   P.early_steps = prm->early_steps;
   P.inchworm_cap = prm->inchworm_cap;
   P.noise_step = prm->noise_step;
@@ -3988,6 +4027,39 @@ int hops_rt_slot_noise(void *p, int slot, const void *z1_d, const void *z2_d, in
     bt->noise_step = noise_step;
     bt->noise_c128 = sl.c128 ? 1 : 0;
   }
+  return 0;
+}
+
+int hops_rt_slot_static_basis(void *p, int slot, int n_states, const int32_t *states, int n_aux,
+                              const int32_t *aux_ptr, const int32_t *aux_mode, const int32_t *aux_val) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  Slot &sl = b.slots[slot];
+  if (!sl.pending_init) return rt_fail("a static basis is set before the slot's first step");
+  if (n_states < 0 || n_aux < 0 || (n_states && !states) || (n_aux && (!aux_ptr || !aux_mode || !aux_val)))
+    return rt_fail("hops_rt_slot_static_basis: bad arguments");
+  const System &S = *sl.sys;
+  Params &P = sl.prm;
+  P.static_states.clear();
+  for (int i = 0; i < n_states; ++i) {
+    if (states[i] < 0 || states[i] >= S.n) return rt_fail("static state %d out of range", states[i]);
+    P.static_states.push_back(states[i]);
+  }
+  std::sort(P.static_states.begin(), P.static_states.end());
+  P.static_states.erase(std::unique(P.static_states.begin(), P.static_states.end()), P.static_states.end());
+  P.static_aux.clear();
+  for (int i = 0; i < n_aux; ++i) {
+    Aux a;
+    for (int q = aux_ptr[i]; q < aux_ptr[i + 1]; ++q) {
+      if (aux_mode[q] < 0 || aux_mode[q] >= S.n_hm || aux_val[q] < 1 || aux_val[q] > 255)
+        return rt_fail("static auxiliary %d: bad (mode, depth) entry", i);
+      a.push_back(aux_pack(aux_mode[q], aux_val[q]));
+    }
+    std::sort(a.begin(), a.end());
+    P.static_aux.push_back(a);
+  }
+  P.has_static = true;
   return 0;
 }
 

@@ -94,9 +94,10 @@ def traj_param(eom_kind, maxhier, delta_a, delta_s, update_step, f_discard, dt, 
     p = RtTrajParam()
     p.eom, p.maxhier = int(eom_kind), int(min(maxhier, 255))
     p.update_step = int(update_step) if update_step else 1
-    if early_integrator not in ("INCH_WORM", "STATIC"):
+    codes = {"INCH_WORM": 0, "STATIC": 1, "STATIC_STATE_INCHWORM_HIERARCHY": 2}
+    if early_integrator not in codes:
         raise UnsupportedRequest(early_integrator, "early time integrator clause of the propagate")
-    p.early_integrator = 0 if early_integrator == "INCH_WORM" else 1
+    p.early_integrator = codes[early_integrator]
     p.early_steps, p.inchworm_cap = int(early_steps), int(inchworm_cap)
     p.noise_step, p.n_t, p.it0, p.record = int(noise_step), int(n_t), int(it0), int(bool(record))
     p.noise_c128 = int(bool(noise_c128))
@@ -282,6 +283,13 @@ class RaggedBatch:
                                        None if z2 is None else ctypes.c_void_p(z2.data_ptr()), int(n_t), int(it),
                                        int(noise_step), int(bool(c128))))
 
+    def static_basis(self, slot, states, aux_list):
+        """STATIC_BASIS of a slot before its first step (hops_rt_slot_static_basis)."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.int32))
+        a_ptr, a_mode, a_val = _aux_csr(aux_list)
+        check(lib().hops_rt_slot_static_basis(self.h, int(slot), len(st), _ptr(st), len(aux_list), _ptr(a_ptr),
+                                              _ptr(a_mode), _ptr(a_val)))
+
     def records(self, slot, nstates, clear=False):
         """Dense psi(t) [n_rec, nstates], state lists and n_hier per record."""
         inf = self.info(slot)
@@ -348,8 +356,14 @@ class AdaptiveBatch:
         from .trajectory import _Setup
         self.setup = _Setup(system_param, eom_param, noise_param or {"TLEN": 1000.0, "TAU": 1.0}, noise2_param,
                             hierarchy_param, integration_param)
-        if self.setup.integ["STATIC_BASIS"] is not None:
-            raise UnsupportedRequest("STATIC_BASIS", "AdaptiveBatch")
+        self.static_basis = None
+        sb = self.setup.integ["STATIC_BASIS"]
+        if sb is not None:
+            from .adaptive import aux_tuple
+            self.static_basis = ([int(x) for x in sb[0]], [aux_tuple(a) for a in sb[1]])
+        if self.setup.integ["EARLY_ADAPTIVE_INTEGRATOR"] == "STATIC_STATE_INCHWORM_HIERARCHY" and sb is None:
+            raise UnsupportedRequest("STATIC_STATE_INCHWORM_HIERARCHY without a STATIC_BASIS",
+                                     "early time integrator clause of the propagate")
         if self.setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
             raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION in a ragged batch (HopsTrajectory supports it)",
                                      "AdaptiveBatch")
@@ -457,7 +471,10 @@ class AdaptiveBatch:
                              noise_c128=self.c128)
             z1 = self.z1[b]
             z2 = None if self.z2 is None else self.z2[b]
-            self.slots.append(self.rt.add(self.system, prm, self._psi0, z1, z2))
+            slot = self.rt.add(self.system, prm, self._psi0, z1, z2)
+            if self.static_basis is not None and (self.delta_a > 0 or self.delta_s > 0):
+                self.rt.static_basis(slot, *self.static_basis)
+            self.slots.append(slot)
             self._params.append(prm)
         self._tau = tau
 

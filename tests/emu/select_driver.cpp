@@ -76,6 +76,9 @@ int main(int argc, char **argv) {
   const int n = atoi(argv[1]), steps = atoi(argv[2]);
   const double da = atof(argv[3]), ds = atof(argv[4]);
   const int markov = atoi(argv[5]), B = atoi(argv[6]);
+  // optional: every slot starts from a STATIC_BASIS (states 0..2 and the
+  // first-level auxiliaries of modes 0 and 1) instead of its first selection
+  const bool with_static = argc > 7 && atoi(argv[7]) != 0;
   Sys S;
   make_chain(n, S);
   void *sys = hops_rt_system_create(&S.d);
@@ -111,7 +114,12 @@ int main(int argc, char **argv) {
       p.f_discard = s % 2 ? 0.01 : 0.0;
       p.dt = 1.0;
       p.markov = markov ? mk.data() : nullptr;
-      if (hops_rt_batch_add(bat[v], sys, &p, psi0.data(), z.data(), nullptr) < 0) return fail("add");
+      const int slot = hops_rt_batch_add(bat[v], sys, &p, psi0.data(), z.data(), nullptr);
+      if (slot < 0) return fail("add");
+      if (with_static) {
+        const int32_t st[3] = {0, 1, 2}, ap[3] = {0, 1, 2}, am[2] = {0, 1}, av[2] = {1, 1};
+        if (hops_rt_slot_static_basis(bat[v], slot, 3, st, 2, ap, am, av) < 0) return fail("static basis");
+      }
     }
   }
   long long nh_max = 0, rec_total = 0;

@@ -227,6 +227,45 @@ def test_adaptive_batch_complex128_precalculated_matches_oracle():
         assert d32 > 0
 
 
+@pytest.mark.parametrize("early", ["INCH_WORM", "STATIC_STATE_INCHWORM_HIERARCHY"])
+def test_adaptive_batch_static_basis_matches_oracle(early):
+    """STATIC_BASIS in AdaptiveBatch (trajectory/hops_trajectory.py:299-317,
+    :478-491): the first basis joins the static states and auxiliaries, and
+    with STATIC_STATE_INCHWORM_HIERARCHY the early steps keep the static
+    states; every trajectory's basis sequence and psi(t) follow the oracle
+    (1e-10)."""
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp = systems.chain(6)
+    tlen, tau = 60.0, 0.5
+    zs = [_noise(sysp, s, tlen, tau) for s in (5, 9, 13)]
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[2] = 1
+    static = [[1, 2, 3], [((4, 1),), ((5, 1),), ((4, 1), (5, 1))]]
+    ab = AdaptiveBatch(sysp, eom, nps[0], None, {"MAXHIER": 3}, noise_params=nps, precision_mode="fp64",
+                       arena_bytes=ARENA,
+                       integration_param={"STATIC_BASIS": static, "EARLY_ADAPTIVE_INTEGRATOR": early})
+    ab.make_adaptive(1e-3, 1e-3)
+    ab.initialize(psi0)
+    ab.propagate(30.0, 1.0)
+    for b, z in enumerate(zs):
+        noise = {"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau}
+        ref = O.AdaptiveTrajectory(sysp, eom, noise, hierarchy_param={"MAXHIER": 3}, delta_a=1e-3, delta_s=1e-3,
+                                   early_integrator=early, static_basis=static)
+        ref.initialize(psi0)
+        ref.propagate(30.0, 1.0)
+        psi, states, nhier = ab.results(b)
+        assert len(states) == len(ref.state_traj)
+        for i, (s_got, s_ref) in enumerate(zip(states, ref.state_traj)):
+            assert list(s_got) == list(s_ref), (b, i, s_got, s_ref)
+            assert nhier[i] == len(ref.aux_traj[i]), (b, i)
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        print("ragged static basis %s traj %d: max |dpsi| = %.3e" % (early, b, err))
+        assert err < FP64_TOL
+
+
 def test_alone_equals_in_mix():
     """A trajectory's result does not depend on which others share its batch."""
     _need_gpu()

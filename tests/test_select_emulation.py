@@ -47,20 +47,23 @@ def driver(tmp_path_factory):
     return exe
 
 
-# n sites, steps per call, delta_a, delta_s, Markovian filter, slots
+# n sites, steps per call, delta_a, delta_s, Markovian filter, slots, static basis
 CASES = {
-    "hierarchy_only": (10, 3, 5e-4, 0.0, 0, 3),
-    "states_and_hierarchy": (12, 3, 1e-3, 1e-3, 0, 3),
-    "markov": (8, 3, 1e-3, 5e-4, 1, 2),
+    "hierarchy_only": (10, 3, 5e-4, 0.0, 0, 3, 0),
+    "states_and_hierarchy": (12, 3, 1e-3, 1e-3, 0, 3, 0),
+    "markov": (8, 3, 1e-3, 5e-4, 1, 2, 0),
+    # every slot starts from a STATIC_BASIS: the first step reads psi through
+    # the device descriptors before any selection has uploaded them
+    "static_basis": (8, 3, 1e-3, 1e-3, 0, 3, 1),
 }
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_device_selection_emulated_equals_host(driver, name):
-    n, steps, da, ds, mk, B = CASES[name]
+    n, steps, da, ds, mk, B, st = CASES[name]
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
                UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
-    r = subprocess.run([driver, str(n), str(steps), str(da), str(ds), str(mk), str(B)], capture_output=True,
+    r = subprocess.run([driver, str(n), str(steps), str(da), str(ds), str(mk), str(B), str(st)], capture_output=True,
                        text=True, env=env, timeout=900)
     print(r.stdout[-2000:], r.stderr[-4000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
