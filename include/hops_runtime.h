@@ -94,6 +94,13 @@ int hops_rt_batch_remove(void *batch, int slot);
    (noise/hops_noise.py:379-385); no counterpart in the reference. */
 int hops_rt_slot_noise(void *batch, int slot, const void *z1_d, const void *z2_d, int n_t, int it,
                        int noise_step, int c128);
+/* EFFECTIVE_NOISE_INTEGRATION (integrator/integrator_rk.py:115-128): the
+   noise the slot's RK stages read (window means, same layout as z1_d / z2_d
+   of hops_rt_batch_add); the selection keeps reading the point samples, as
+   the reference's error bound does (trajectory/hops_trajectory.py:693-714).
+   z1_d NULL: the stages read the point samples again.  No counterpart in the
+   reference's interfaces. */
+int hops_rt_slot_stage_noise(void *batch, int slot, const void *z1_d, const void *z2_d);
 /* STATIC_BASIS (trajectory/hops_trajectory.py:299-317, :478-491) of a slot,
    before its first step: states and auxiliaries (CSR of absolute mode /
    depth entries) joined to its first basis in place of the first selection;

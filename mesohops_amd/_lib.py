@@ -115,6 +115,7 @@ def lib():
     L.hops_rt_slot_noise.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int]
     L.hops_rt_slot_static_basis.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]
+    L.hops_rt_slot_stage_noise.argtypes = [_vp, ctypes.c_int, _vp, _vp]
     L.hops_gather_u64.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.hops_rt_system_create.restype = _vp
     L.hops_rt_system_create.argtypes = [ctypes.POINTER(RtSystemDesc)]
@@ -176,7 +177,7 @@ EXPORTED = ["hops_last_error", "hops_version", "hops_profile_enable", "hops_prof
             "hops_gather_u64",
             "hops_rt_system_create", "hops_rt_system_destroy", "hops_rt_batch_create", "hops_rt_batch_destroy",
             "hops_rt_batch_add", "hops_rt_batch_remove", "hops_rt_slot_noise",
-            "hops_rt_slot_static_basis", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",
+            "hops_rt_slot_static_basis", "hops_rt_slot_stage_noise", "hops_rt_slot_estimate", "hops_rt_arena_free", "hops_rt_batch_timers",
             "hops_rt_batch_option", "hops_rt_batch_step", "hops_rt_slot_info", "hops_rt_slot_states", "hops_rt_slot_aux",
             "hops_rt_slot_phi", "hops_rt_slot_zmem", "hops_rt_slot_record", "hops_rt_slot_clear_records",
             "hops_rt_debug_basis", "hops_rt_debug_table", "hops_rt_debug_select", "hops_rt_debug_select_states",

@@ -1975,6 +1975,10 @@ struct Slot {
   hops_op_t op{};
   hops_batch_t bt{};
   const void *z1 = nullptr, *z2 = nullptr;
+  // EFFECTIVE_NOISE_INTEGRATION: the RK stages read these window means while
+  // the selection keeps the point samples z1 / z2 (integrator_rk.py:115-128,
+  // hops_trajectory.py:693-714); NULL: the stages read z1 / z2
+  const void *z1s = nullptr, *z2s = nullptr;
   bool c128 = false;  // the noise arrays hold complex128
   int it = 0, early_counter = 0;
   long long steps = 0;
@@ -2252,8 +2256,8 @@ void fill_bt(const Batch &b, const Slot &sl, const Basis &B, const Layout &L, ch
   bt.ya_d = base + L.ya;
   bt.yb_d = base + L.yb;
   bt.acc_d = base + L.acc;
-  bt.z1_d = sl.z1;
-  bt.z2_d = sl.z2;
+  bt.z1_d = sl.z1s ? sl.z1s : sl.z1;
+  bt.z2_d = sl.z1s ? sl.z2s : sl.z2;
   bt.noise_c128 = sl.c128 ? 1 : 0;
   char *pb = b.arena.ptr(sl.pers);
   const size_t Ma = B.sys->n_hm;
@@ -2289,10 +2293,22 @@ int place_basis(Batch &b, Slot &sl, Basis &B, long long *blk_out, hops_op_t &op,
 
 // ---- the step machinery ----------------------------------------------------
 
+// the noise the listed slots' kernels read: the stage noise for an RK step
+// (stage = true), the point samples for the selection's derivatives
+void point_noise(Batch &b, const std::vector<int> &list, bool stage) {
+  for (int s : list) {
+    const Slot &sl = b.slots[s];
+    const bool eff = stage && sl.z1s;
+    b.bts[s].z1_d = eff ? sl.z1s : sl.z1;
+    b.bts[s].z2_d = eff ? sl.z2s : sl.z2;
+  }
+}
+
 int run_step(Batch &b, const std::vector<int> &list) {
   if (list.empty()) return 0;
   Timer tmr(&b.tm[0]);
   for (int s : list) b.its[s] = b.slots[s].it;
+  point_noise(b, list, true);
   RT_TRY(upload_desc(b));
   std::vector<uint32_t> tiles = tiles_of(b, list);
   Stage st{&b};
@@ -3253,6 +3269,7 @@ int define_bases_dev(Batch &b, const std::vector<int> &list) {
 // when the arena cannot hold the device work space)
 int define_bases(Batch &b, const std::vector<int> &list) {
   if (list.empty()) return 0;
+  point_noise(b, list, false);
   std::vector<int> dev, host;
   if (b.sel_device)
     for (int s : list) RT_TRY(ensure_zob(b, b.slots[s]));
@@ -4027,6 +4044,19 @@ int hops_rt_slot_noise(void *p, int slot, const void *z1_d, const void *z2_d, in
     bt->noise_step = noise_step;
     bt->noise_c128 = sl.c128 ? 1 : 0;
   }
+  return 0;
+}
+
+int hops_rt_slot_stage_noise(void *p, int slot, const void *z1_d, const void *z2_d) {
+  RT_TRY(check_batch(p));
+  auto &b = *static_cast<Batch *>(p);
+  if (slot < 0 || slot >= (int)b.slots.size() || !b.slots[slot].used) return rt_fail("no slot %d", slot);
+  Slot &sl = b.slots[slot];
+  if (z1_d && (sl.z2 != nullptr) != (z2_d != nullptr)) return rt_fail("stage noise 2 must match the slot's noise 2");
+  sl.z1s = z1_d;
+  sl.z2s = z1_d ? z2_d : nullptr;
+  sl.bt.z1_d = b.bts[slot].z1_d = sl.z1s ? sl.z1s : sl.z1;
+  sl.bt.z2_d = b.bts[slot].z2_d = sl.z1s ? sl.z2s : sl.z2;
   return 0;
 }
 

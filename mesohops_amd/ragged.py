@@ -283,6 +283,11 @@ class RaggedBatch:
                                        None if z2 is None else ctypes.c_void_p(z2.data_ptr()), int(n_t), int(it),
                                        int(noise_step), int(bool(c128))))
 
+    def stage_noise(self, slot, z1, z2):
+        """Noise the slot's RK stages read (hops_rt_slot_stage_noise)."""
+        check(lib().hops_rt_slot_stage_noise(self.h, int(slot), ctypes.c_void_p(z1.data_ptr()),
+                                             None if z2 is None else ctypes.c_void_p(z2.data_ptr())))
+
     def static_basis(self, slot, states, aux_list):
         """STATIC_BASIS of a slot before its first step (hops_rt_slot_static_basis)."""
         st = np.ascontiguousarray(np.asarray(states, dtype=np.int32))
@@ -364,9 +369,7 @@ class AdaptiveBatch:
         if self.setup.integ["EARLY_ADAPTIVE_INTEGRATOR"] == "STATIC_STATE_INCHWORM_HIERARCHY" and sb is None:
             raise UnsupportedRequest("STATIC_STATE_INCHWORM_HIERARCHY without a STATIC_BASIS",
                                      "early time integrator clause of the propagate")
-        if self.setup.integ["EFFECTIVE_NOISE_INTEGRATION"]:
-            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION in a ragged batch (HopsTrajectory supports it)",
-                                     "AdaptiveBatch")
+        self.effective = bool(self.setup.integ["EFFECTIVE_NOISE_INTEGRATION"])
         if noise_params is None:
             if seeds is None:
                 raise TrajectoryError("AdaptiveBatch needs seeds or noise_params")
@@ -423,6 +426,9 @@ class AdaptiveBatch:
         # PRE_CALCULATED arrays are used as given
         self.interp = n1.interpolate
         self.c128 = any(nz.needs_c128() for nz in self.noise1 + self.noise2)
+        if self.effective and self.c128:
+            raise UnsupportedRequest("EFFECTIVE_NOISE_INTEGRATION with complex128 (interpolated or "
+                                     "PRE_CALCULATED complex128) noise", "AdaptiveBatch")
         nl = info.n_l2
         self.z1 = device_noise(self.noise1, list(range(nl)), self.device, sp, c128=self.c128)
         self.z2 = None
@@ -450,6 +456,7 @@ class AdaptiveBatch:
         self._tau = tau
         self.slots = []
         self._params = []
+        self._zeff = []
         self.initialized = True
         self.t = 0.0
         if tau is not None:
@@ -474,6 +481,16 @@ class AdaptiveBatch:
             slot = self.rt.add(self.system, prm, self._psi0, z1, z2)
             if self.static_basis is not None and (self.delta_a > 0 or self.delta_s > 0):
                 self.rt.static_basis(slot, *self.static_basis)
+            if self.effective:
+                # the RK stages read window means over the step's noise
+                # samples, the selection the point samples
+                # (integrator/integrator_rk.py:115-128)
+                from .noise import window_mean
+                sp = self.rt.stream
+                z1e = window_mean(z1, ns, sp)
+                z2e = None if z2 is None else window_mean(z2, ns, sp)
+                self.rt.stage_noise(slot, z1e, z2e)
+                self._zeff.append((z1e, z2e))
             self.slots.append(slot)
             self._params.append(prm)
         self._tau = tau

@@ -266,6 +266,37 @@ def test_adaptive_batch_static_basis_matches_oracle(early):
         assert err < FP64_TOL
 
 
+def test_adaptive_batch_effective_noise_matches_oracle():
+    """EFFECTIVE_NOISE_INTEGRATION in AdaptiveBatch (integrator/
+    integrator_rk.py:115-128): the RK stages read window means of the noise
+    while the selection keeps the point samples (hops_trajectory.py:693-714);
+    every trajectory's basis sequence and psi(t) follow the oracle (1e-10)."""
+    _need_gpu()
+    from mesohops_amd.ragged import AdaptiveBatch
+    sysp = systems.chain(6)
+    tau_n = 0.25
+    zs = [_noise(sysp, s, 60.0, tau_n) for s in (4, 8, 15)]
+    nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": 60.0, "TAU": tau_n} for z in zs]
+    eom = {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}
+    psi0 = np.zeros(6, complex)
+    psi0[3] = 1.0
+    ab = AdaptiveBatch(sysp, eom, nps[0], None, {"MAXHIER": 3}, noise_params=nps, precision_mode="fp64",
+                       arena_bytes=ARENA, integration_param={"EFFECTIVE_NOISE_INTEGRATION": True})
+    ab.make_adaptive(2e-3, 2e-3)
+    ab.initialize(psi0)
+    ab.propagate(30.0, 1.5)
+    for b, n in enumerate(nps):
+        ref = O.AdaptiveTrajectory(sysp, eom, n, hierarchy_param={"MAXHIER": 3}, delta_a=2e-3, delta_s=2e-3,
+                                   effective_noise_integration=True)
+        ref.initialize(psi0)
+        ref.propagate(30.0, 1.5)
+        psi, states, nhier = ab.results(b)
+        assert [list(x) for x in states] == [list(x) for x in ref.state_traj], b
+        err = np.max(np.abs(psi - np.array(ref.psi_traj)))
+        print("ragged effective noise traj %d: max |dpsi| = %.3e" % (b, err))
+        assert err < FP64_TOL
+
+
 def test_alone_equals_in_mix():
     """A trajectory's result does not depend on which others share its batch."""
     _need_gpu()
