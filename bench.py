@@ -171,7 +171,7 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
     t_init = time.perf_counter() - t0
     if log:
         log("%s: prepared in %.1f s" % (kind, t_init))
-    _lib.profile_enable(True)
+    ab.rt.profile_enable(True)
     chunks = []
     t0 = time.perf_counter()
     done = 0
@@ -189,13 +189,14 @@ def run_c3(B, steps, prec="mixed", single_steps=20, ref_traj=4, chunk=10, arena_
             log("%s t=%d fs: %.0f traj*steps/s, mean N_phi %.0f" % (kind, done, B * c / chunks[-1]["s"],
                                                                   chunks[-1]["mean_n_phi"]))
     elapsed = time.perf_counter() - t0
-    ms, n = _lib.profile_read()
-    _lib.profile_enable(False)
+    ms, n = ab.rt.profile_read()
+    ab.rt.profile_enable(False)
     timers = ab.rt.timers()
     es = 16 if prec == "fp64" else 8
     stage_bytes = timers["rows_stepped"] * (8 * es + 32)
     achieved = stage_bytes / (ms[1] * 1e-3) / 1e9 if ms[1] > 0 else 0.0
-    out = {"value": B * steps / elapsed, "trajectories": B, "steps": steps, "elapsed_s": elapsed, "init_s": t_init,
+    out = {"value": B * steps / elapsed, "trajectories": B, "sub_batches": ab.K, "threads_per_sub_batch": ab.n_threads,
+           "steps": steps, "elapsed_s": elapsed, "init_s": t_init,
            "ms_per_step": elapsed / steps * 1e3, "chunks": chunks,
            "host_phase_s": {k: v for k, v in timers.items() if k != "rows_stepped"},
            "stage_kernel": {"avg_ms": ms[1] / max(n[1], 1), "launches": n[1],

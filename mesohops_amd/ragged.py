@@ -12,6 +12,7 @@ launch per RK4 stage for all of them.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 import torch
@@ -201,7 +202,10 @@ class DebugBasis:
 class RaggedBatch:
     """Thin owner of a runtime batch: arena, stream, slots."""
 
-    def __init__(self, precision_mode="fp64", arena_bytes=None, max_slots=1024, n_threads=16, device="cuda:0"):
+    def __init__(self, precision_mode="fp64", arena_bytes=None, max_slots=1024, n_threads=16, device="cuda:0",
+                 stream=None):
+        """``stream``: a torch.cuda.Stream the batch's work is queued on (the
+        current stream when None)."""
         _require_gpu()
         self.device = torch.device(device)
         self.precision = {"fp64": _lib.HOPS_PREC_FP64, "mixed": _lib.HOPS_PREC_MIXED}[precision_mode]
@@ -210,7 +214,8 @@ class RaggedBatch:
             arena_bytes = int(free * 0.6)
         arena_bytes = (int(arena_bytes) // 256) * 256
         self.arena = torch.empty(arena_bytes, dtype=torch.uint8, device=self.device)
-        self.stream = _stream_ptr(self.device)
+        self._torch_stream = stream
+        self.stream = _stream_ptr(self.device) if stream is None else ctypes.c_void_p(stream.cuda_stream)
         self.h = lib().hops_rt_batch_create(self.precision, ctypes.c_void_p(self.arena.data_ptr()), arena_bytes,
                                             int(max_slots), int(n_threads), self.stream)
         if not self.h:
@@ -345,6 +350,148 @@ class RaggedBatch:
         self.close()
 
 
+class RaggedGroup:
+    """K runtime batches, each with its share of the arena, its own stream
+    and its own worker threads, stepped concurrently from K host threads.
+    A batch's host work (basis selection, table building) runs while the
+    others wait on their device work and transfers, so the CPU cores and the
+    GPU stay busy together.  Trajectories never interact, so a trajectory's
+    result is the same in whichever batch it sits.  Slots are addressed as
+    (batch, slot) pairs."""
+
+    def __init__(self, k, precision_mode="fp64", arena_bytes=None, max_slots=1024, n_threads=16,
+                 device="cuda:0"):
+        if k < 1:
+            raise UnsupportedRequest("sub_batches < 1", "RaggedGroup")
+        self.device = torch.device(device)
+        if arena_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            arena_bytes = int(free * 0.6)
+        # one batch keeps the caller's current stream, as a plain RaggedBatch
+        self.streams = [None] if k == 1 else [torch.cuda.Stream(device=self.device) for _ in range(k)]
+        self.parts = [RaggedBatch(precision_mode, int(arena_bytes) // k, max_slots, n_threads, device, stream=s)
+                      for s in self.streams]
+        self._lock = threading.Lock()
+        self._profiling = False
+        self._prof = ([0.0] * 4, [0] * 4)
+
+    @property
+    def stream(self):
+        return self.parts[0].stream
+
+    def stream_of(self, ref):
+        return self.parts[ref[0]].stream
+
+    def set_option(self, name, value):
+        for p in self.parts:
+            p.set_option(name, value)
+
+    def add(self, k, system, prm, psi0, z1, z2=None):
+        return (k, self.parts[k].add(system, prm, psi0, z1, z2))
+
+    def remove(self, ref):
+        self.parts[ref[0]].remove(ref[1])
+
+    def info(self, ref):
+        return self.parts[ref[0]].info(ref[1])
+
+    def states(self, ref):
+        return self.parts[ref[0]].states(ref[1])
+
+    def aux_list(self, ref):
+        return self.parts[ref[0]].aux_list(ref[1])
+
+    def phi(self, ref):
+        return self.parts[ref[0]].phi(ref[1])
+
+    def zmem(self, ref, n_hmode):
+        return self.parts[ref[0]].zmem(ref[1], n_hmode)
+
+    def slot_noise(self, ref, z1, z2, n_t, it, noise_step, c128):
+        self.parts[ref[0]].slot_noise(ref[1], z1, z2, n_t, it, noise_step, c128)
+
+    def stage_noise(self, ref, z1, z2):
+        self.parts[ref[0]].stage_noise(ref[1], z1, z2)
+
+    def static_basis(self, ref, states, aux_list):
+        self.parts[ref[0]].static_basis(ref[1], states, aux_list)
+
+    def records(self, ref, nstates, clear=False):
+        return self.parts[ref[0]].records(ref[1], nstates, clear)
+
+    def free_bytes(self):
+        return sum(p.free_bytes() for p in self.parts)
+
+    def synchronize(self):
+        torch.cuda.synchronize(self.device)
+
+    def profile_enable(self, on=True):
+        """Time the engine's launches of the following steps with HIP events
+        (hops_profile_*), summed over the batches' host threads."""
+        with self._lock:
+            self._profiling = bool(on)
+            self._prof = ([0.0] * 4, [0] * 4)
+
+    def profile_read(self):
+        """(ms, launches) for [prologue, stage, other, dense system block]."""
+        with self._lock:
+            return list(self._prof[0]), list(self._prof[1])
+
+    def _step_one(self, k, n_steps):
+        prof = self._profiling
+        if prof:
+            _lib.profile_enable(True)  # per host thread
+        try:
+            self.parts[k].step(n_steps)
+        finally:
+            if prof:
+                ms, n = _lib.profile_read()
+                _lib.profile_enable(False)
+                with self._lock:
+                    for i in range(4):
+                        self._prof[0][i] += ms[i]
+                        self._prof[1][i] += n[i]
+
+    def step(self, n_steps=1):
+        """Advance every slot of every batch by n_steps; batch k's steps run
+        on host thread k (the runtime call releases the interpreter lock)."""
+        if len(self.parts) == 1:
+            self._step_one(0, n_steps)
+            return
+        errs = [None] * len(self.parts)
+
+        def run(k):
+            try:
+                self._step_one(k, n_steps)
+            except BaseException as e:  # re-raised on the calling thread
+                errs[k] = e
+
+        th = [threading.Thread(target=run, args=(k,), name="hops-batch-%d" % k) for k in range(len(self.parts))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for e in errs:
+            if e is not None:
+                raise e
+        # every batch's work complete: arrays the caller drops after this
+        # call (per-call noise) are read by no stream any more
+        torch.cuda.synchronize(self.device)
+
+    def timers(self):
+        """Seconds per runtime phase summed over the batches (their phases
+        overlap in time), and rows stepped."""
+        out = {}
+        for p in self.parts:
+            for key, v in p.timers().items():
+                out[key] = out.get(key, 0.0) + v
+        return out
+
+    def close(self):
+        for p in self.parts:
+            p.close()
+
+
 class AdaptiveBatch:
     """B adaptive trajectories of one system that differ in their noise,
     each with its own basis, on one GPU.
@@ -352,12 +499,18 @@ class AdaptiveBatch:
     Same dictionaries as HopsTrajectory; ``make_adaptive`` as the
     reference's; ``initialize(psi_0)``; ``propagate(t_advance, tau)``;
     ``psi_traj[b]`` (dense [n_t, n_state]), ``state_traj[b]``,
-    ``nhier_traj[b]`` after propagate."""
+    ``nhier_traj[b]`` after propagate.
+
+    ``sub_batches``: the trajectories are split over this many runtime
+    batches stepped concurrently (``RaggedGroup``); default one per 32
+    trajectories, at most 8.  ``n_threads``: worker threads per runtime
+    batch (default 16 for one batch, else 32 / sub_batches, at least 2).
+    Neither changes any result."""
 
     def __init__(self, system_param, eom_param=None, noise_param=None, noise2_param=None,
                  hierarchy_param=None, integration_param=None, seeds=None, noise_params=None,
-                 precision_mode="fp64", device="cuda:0", arena_bytes=None, n_threads=16, record=True,
-                 device_select=False):
+                 precision_mode="fp64", device="cuda:0", arena_bytes=None, n_threads=None, record=True,
+                 device_select=False, sub_batches=None):
         from .trajectory import _Setup
         self.setup = _Setup(system_param, eom_param, noise_param or {"TLEN": 1000.0, "TAU": 1.0}, noise2_param,
                             hierarchy_param, integration_param)
@@ -386,7 +539,11 @@ class AdaptiveBatch:
         self.precision_mode = precision_mode
         self.device = torch.device(device)
         self.arena_bytes = arena_bytes
-        self.n_threads = n_threads
+        K = max(1, min(8, self.B // 32)) if sub_batches is None else int(sub_batches)
+        if K < 1:
+            raise UnsupportedRequest("sub_batches < 1", "AdaptiveBatch")
+        self.K = min(K, max(self.B, 1))
+        self.n_threads = (16 if self.K == 1 else max(2, 32 // self.K)) if n_threads is None else int(n_threads)
         self.record = record
         self.device_select = bool(device_select)
         self.delta_a = self.delta_s = 0.0
@@ -413,7 +570,8 @@ class AdaptiveBatch:
             raise LockedException("AdaptiveBatch.initialize()")
         info = self.setup.info
         self.system = RtSystem(info)
-        self.rt = RaggedBatch(self.precision_mode, self.arena_bytes, max_slots=max(self.B, 1),
+        self.per = -(-self.B // self.K)  # trajectories per runtime batch
+        self.rt = RaggedGroup(self.K, self.precision_mode, self.arena_bytes, max_slots=max(self.per, 1),
                               n_threads=self.n_threads, device=self.device)
         self.rt.set_option("device_select", self.device_select)
         sp = self.rt.stream
@@ -478,7 +636,7 @@ class AdaptiveBatch:
                              noise_c128=self.c128)
             z1 = self.z1[b]
             z2 = None if self.z2 is None else self.z2[b]
-            slot = self.rt.add(self.system, prm, self._psi0, z1, z2)
+            slot = self.rt.add(b // self.per, self.system, prm, self._psi0, z1, z2)
             if self.static_basis is not None and (self.delta_a > 0 or self.delta_s > 0):
                 self.rt.static_basis(slot, *self.static_basis)
             if self.effective:
@@ -486,7 +644,7 @@ class AdaptiveBatch:
                 # samples, the selection the point samples
                 # (integrator/integrator_rk.py:115-128)
                 from .noise import window_mean
-                sp = self.rt.stream
+                sp = self.rt.stream_of(slot)
                 z1e = window_mean(z1, ns, sp)
                 z2e = None if z2 is None else window_mean(z2, ns, sp)
                 self.rt.stage_noise(slot, z1e, z2e)
@@ -515,10 +673,10 @@ class AdaptiveBatch:
         n_steps = int(np.ceil(t_advance / tau))
         if self.interp:
             from .noise import spline_noise
-            sp = self.rt.stream
             n_out = 2 * n_steps + 1
             zcall = []
             for b, slot in enumerate(self.slots):
+                sp = self.rt.stream_of(slot)
                 z1 = spline_noise(self.z1[b], nz1.param["TAU"], self.t, 0.5 * tau, n_out, sp,
                                   nz1.param["FLAG_REAL"])
                 z2 = None if self.z2 is None else spline_noise(self.z2[b], nz2.param["TAU"], self.t, 0.5 * tau,

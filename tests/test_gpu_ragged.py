@@ -145,8 +145,8 @@ def _c128_noise(sysp, seed, tlen, tau):
     return O.correlated_noise(np.complex64(nz.alpha(list(range(nz.n_l2)))), nz.z_unc)
 
 
-@pytest.mark.parametrize("name", ["chain6", "peierls"])
-def test_adaptive_batch_interpolated_noise_matches_oracle(name):
+@pytest.mark.parametrize("name,sub", [("chain6", 1), ("peierls", 1), ("chain6", 3)])
+def test_adaptive_batch_interpolated_noise_matches_oracle(name, sub):
     """INTERPOLATE in AdaptiveBatch (noise/hops_noise.py:242-243, 379-385):
     stage times between noise points (TAU 1, step 1.5 fs), two propagate
     calls, each reading the cubic noise of its own half-step grid in
@@ -166,9 +166,9 @@ def test_adaptive_batch_interpolated_noise_matches_oracle(name):
     psi0 = np.zeros(nst, complex)
     psi0[s0] = 1
     da, ds = (1e-3, 1e-3) if name == "chain6" else (2e-3, 3e-4)
-    print("case", name, flush=True)
+    print("case", name, "sub-batches", sub, flush=True)
     ab = AdaptiveBatch(sysp, eom, nps[0], None, {"MAXHIER": 3}, noise_params=nps, precision_mode="fp64",
-                       arena_bytes=ARENA)
+                       arena_bytes=ARENA, sub_batches=sub)
     ab.make_adaptive(da, ds)
     ab.initialize(psi0)
     ab.propagate(21.0, 1.5)
@@ -266,11 +266,13 @@ def test_adaptive_batch_static_basis_matches_oracle(early):
         assert err < FP64_TOL
 
 
-def test_adaptive_batch_effective_noise_matches_oracle():
+@pytest.mark.parametrize("sub", [1, 2])
+def test_adaptive_batch_effective_noise_matches_oracle(sub):
     """EFFECTIVE_NOISE_INTEGRATION in AdaptiveBatch (integrator/
     integrator_rk.py:115-128): the RK stages read window means of the noise
     while the selection keeps the point samples (hops_trajectory.py:693-714);
-    every trajectory's basis sequence and psi(t) follow the oracle (1e-10)."""
+    every trajectory's basis sequence and psi(t) follow the oracle (1e-10),
+    in one runtime batch and split over two."""
     _need_gpu()
     from mesohops_amd.ragged import AdaptiveBatch
     sysp = systems.chain(6)
@@ -281,7 +283,7 @@ def test_adaptive_batch_effective_noise_matches_oracle():
     psi0 = np.zeros(6, complex)
     psi0[3] = 1.0
     ab = AdaptiveBatch(sysp, eom, nps[0], None, {"MAXHIER": 3}, noise_params=nps, precision_mode="fp64",
-                       arena_bytes=ARENA, integration_param={"EFFECTIVE_NOISE_INTEGRATION": True})
+                       arena_bytes=ARENA, integration_param={"EFFECTIVE_NOISE_INTEGRATION": True}, sub_batches=sub)
     ab.make_adaptive(2e-3, 2e-3)
     ab.initialize(psi0)
     ab.propagate(30.0, 1.5)
@@ -308,12 +310,13 @@ def test_alone_equals_in_mix():
     zs = [_noise(sysp, s, tlen, tau) for s in (5, 6, 7)]
     nps = [{"SEED": z, "MODEL": "PRE_CALCULATED", "TLEN": tlen, "TAU": tau} for z in zs]
 
-    def run(params):
+    def run(params, sub=None):
         ab = AdaptiveBatch(sysp, {"EQUATION_OF_MOTION": "NORMALIZED NONLINEAR"}, params[0], None, {"MAXHIER": 4},
-                           noise_params=params, arena_bytes=ARENA)
+                           noise_params=params, arena_bytes=ARENA, sub_batches=sub)
         ab.make_adaptive(2e-3, 2e-3)
         ab.initialize(psi0)
-        ab.propagate(50.0, 1.0)
+        ab.propagate(30.0, 1.0)
+        ab.propagate(20.0, 1.0)
         return ab
 
     mix = run(nps)
@@ -322,6 +325,18 @@ def test_alone_equals_in_mix():
     p_alone, s_alone, h_alone = alone.results(0)
     assert h_mix == h_alone
     assert np.array_equal(p_mix, p_alone)
+    # the same batch split over concurrently stepped runtime batches
+    # (RaggedGroup): two (2 + 1 trajectories) and three (one each)
+    for sub in (2, 3):
+        split = run(nps, sub)
+        assert split.K == sub
+        for b in range(3):
+            p_s, s_s, h_s = split.results(b)
+            p_m, s_m, h_m = mix.results(b)
+            assert h_s == h_m, (sub, b)
+            assert all(np.array_equal(x, y) for x, y in zip(s_s, s_m)), (sub, b)
+            assert np.array_equal(p_s, p_m), (sub, b)
+        split.rt.close()
 
 
 def test_mixed_precision_population_error():
