@@ -437,12 +437,15 @@ class RaggedGroup:
         with self._lock:
             return list(self._prof[0]), list(self._prof[1])
 
-    def _step_one(self, k, n_steps):
+    def estimate(self, system, n_state, n_hier):
+        return self.parts[0].estimate(system, n_state, n_hier)
+
+    def _step_one(self, k, n_steps, slots=None):
         prof = self._profiling
         if prof:
             _lib.profile_enable(True)  # per host thread
         try:
-            self.parts[k].step(n_steps)
+            self.parts[k].step(n_steps, slots)
         finally:
             if prof:
                 ms, n = _lib.profile_read()
@@ -452,21 +455,30 @@ class RaggedGroup:
                         self._prof[0][i] += ms[i]
                         self._prof[1][i] += n[i]
 
-    def step(self, n_steps=1):
-        """Advance every slot of every batch by n_steps; batch k's steps run
-        on host thread k (the runtime call releases the interpreter lock)."""
+    def step(self, n_steps=1, slots=None):
+        """Advance every slot of every batch (or the listed (batch, slot)
+        pairs) by n_steps; batch k's steps run on host thread k (the runtime
+        call releases the interpreter lock).  A batch's error is raised after
+        every batch has finished its steps."""
+        per = None
+        if slots is not None:
+            per = [[] for _ in self.parts]
+            for k, sl in slots:
+                per[k].append(sl)
+        work = [k for k in range(len(self.parts)) if per is None or per[k]]
         if len(self.parts) == 1:
-            self._step_one(0, n_steps)
+            if work:
+                self._step_one(0, n_steps, None if per is None else per[0])
             return
         errs = [None] * len(self.parts)
 
         def run(k):
             try:
-                self._step_one(k, n_steps)
+                self._step_one(k, n_steps, None if per is None else per[k])
             except BaseException as e:  # re-raised on the calling thread
                 errs[k] = e
 
-        th = [threading.Thread(target=run, args=(k,), name="hops-batch-%d" % k) for k in range(len(self.parts))]
+        th = [threading.Thread(target=run, args=(k,), name="hops-batch-%d" % k) for k in work]
         for t in th:
             t.start()
         for t in th:

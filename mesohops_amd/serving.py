@@ -11,7 +11,11 @@ in worker threads ahead of admission; the correlated-noise filter runs on the
 device at admission.
 
 Results do not depend on which requests share the batch: every slot's
-kernels and basis bookkeeping touch only that slot's data.
+kernels and basis bookkeeping touch only that slot's data.  The slots are
+spread over several runtime batches (ragged.RaggedGroup), each with its own
+share of the arena, stream and worker threads, stepped concurrently, so one
+batch's host-side selection overlaps the others' device work; a request goes
+to the batch with a free slot and the most free memory.
 
 Back-pressure: admission needs room for twice a request's estimated
 footprint.  When bases grow until a step cannot place some slots' new bases,
@@ -36,7 +40,7 @@ from .basis import SystemInfo
 from .engine import _stream_ptr
 from .errors import ArenaFull, UnsupportedRequest
 from .noise import HopsNoise, device_noise
-from .ragged import RaggedBatch, RtSystem, traj_param
+from .ragged import RaggedGroup, RtSystem, traj_param
 from .workloads import chain_system
 
 _EOM = {"LINEAR": _lib.HOPS_EOM_LINEAR, "NONLINEAR": _lib.HOPS_EOM_NONLINEAR,
@@ -44,10 +48,21 @@ _EOM = {"LINEAR": _lib.HOPS_EOM_LINEAR, "NONLINEAR": _lib.HOPS_EOM_NONLINEAR,
 
 
 class ServingLoop:
-    def __init__(self, precision_mode="mixed", arena_bytes=None, max_slots=256, n_threads=16, prep_workers=4,
+    def __init__(self, precision_mode="mixed", arena_bytes=None, max_slots=256, n_threads=None, prep_workers=4,
                  eom="NORMALIZED NONLINEAR", record=False, device="cuda:0", system_fn=chain_system,
-                 reserve_states=12, reserve_aux=4000):
-        self.rt = RaggedBatch(precision_mode, arena_bytes, max_slots=max_slots, n_threads=n_threads, device=device)
+                 reserve_states=12, reserve_aux=4000, sub_batches=None):
+        """``sub_batches``: runtime batches stepped concurrently
+        (ragged.RaggedGroup; default one per 64 slots, at most 8), each
+        with its share of the arena and of the slots; ``n_threads``: worker
+        threads per batch (default 16 for one batch, else 32 / sub_batches)."""
+        K = max(1, min(8, max_slots // 64)) if sub_batches is None else int(sub_batches)
+        if K < 1:
+            raise UnsupportedRequest("sub_batches < 1", "ServingLoop")
+        nthr = (16 if K == 1 else max(2, 32 // K)) if n_threads is None else int(n_threads)
+        self.K = K
+        self.per_slots = -(-max_slots // K)
+        self.rt = RaggedGroup(K, precision_mode, arena_bytes, max_slots=self.per_slots, n_threads=nthr,
+                              device=device)
         self.device = self.rt.device
         self.max_slots = max_slots
         self.eom = eom
@@ -74,15 +89,15 @@ class ServingLoop:
     def _footprint(self, rts, req):
         return self.rt.estimate(rts, min(req.nsite, self.reserve[0]), self.reserve[1])
 
-    def _admit(self, req, prep):
+    def _admit(self, req, prep, k):
         sysp, info, rts, nz, raw = prep
-        z1 = device_noise([nz], list(range(info.n_l2)), self.device, self.rt.stream, raws=[raw])[0]
+        z1 = device_noise([nz], list(range(info.n_l2)), self.device, self.rt.parts[k].stream, raws=[raw])[0]
         psi0 = np.zeros(info.nstates, np.complex128)
         psi0[info.nstates // 2] = 1.0
         ns = int(round(req.dt * 0.5 / req.tau_noise))
         prm = traj_param(_EOM[self.eom], req.maxhier, req.delta, req.delta, 1, 0.01, req.dt, ns, nz.n_t,
                          nz.time_index(0.0), record=self.record)
-        slot = self.rt.add(rts, prm, psi0, z1)
+        slot = self.rt.add(k, rts, prm, psi0, z1)
         return slot, (z1, prm)
 
     def run(self, requests, time_limit=None, prep_ahead=16, log=None):
@@ -93,7 +108,8 @@ class ServingLoop:
         timeline = []        # (t, arrived, waiting, active, done) about once a second
         last_sample = -1.0
         futures = {}
-        active = {}          # slot -> [req, steps_done, keep, t_admit, peak device bytes]
+        active = {}          # (batch, slot) -> [req, steps_done, keep, t_admit, peak device bytes]
+        n_in = [0] * self.K  # active slots per runtime batch
         done = {}
         start = time.perf_counter()
         steps_total = 0
@@ -125,15 +141,22 @@ class ServingLoop:
                 if not fut.done():
                     break
                 prep = fut.result()
-                if self.rt.free_bytes() < max(2 * self._footprint(prep[2], r), need.get(r.rid, 0)):
+                # the runtime batch with a free slot and the most free memory
+                room = [q for q in range(self.K) if n_in[q] < self.per_slots]
+                if not room:
+                    break
+                free = {q: self.rt.parts[q].free_bytes() for q in room}
+                k = max(room, key=lambda q: free[q])
+                if free[k] < max(2 * self._footprint(prep[2], r), need.get(r.rid, 0)):
                     if not active:
                         raise UnsupportedRequest("request %d (%d sites, depth %d) needs more device memory than "
-                                                 "the whole arena holds" % (r.rid, r.nsite, r.maxhier),
+                                                 "a runtime batch's arena holds" % (r.rid, r.nsite, r.maxhier),
                                                  "ServingLoop")
                     break
                 reqs.popleft()
                 del futures[r.rid]
-                slot, keep = self._admit(r, prep)
+                slot, keep = self._admit(r, prep, k)
+                n_in[k] += 1
                 active[slot] = [r, 0, keep, time.perf_counter() - start, 0]
             if not active:
                 if reqs:
@@ -153,19 +176,21 @@ class ServingLoop:
                 # preempted and their requests queued again (they restart from
                 # t = 0 once memory frees up); the others completed the step
                 oom = [s for s in slots if self.rt.info(s)["oom"]]
-                if len(active) == 1 and oom:
-                    # alone and out of room: the arena may only have been
-                    # fragmented by earlier requests, so the request restarts
-                    # once in an otherwise empty arena before it is refused
-                    r = active[oom[0]][0]
-                    if r.rid in lone_retry:
-                        raise UnsupportedRequest("request %d (%d sites, depth %d) outgrowing the arena while "
-                                                 "running alone" % (r.rid, r.nsite, r.maxhier), "ServingLoop")
-                    lone_retry.add(r.rid)
+                for s in oom:
+                    if n_in[s[0]] == 1:
+                        # alone in its batch and out of room: the arena may
+                        # only have been fragmented by earlier requests, so
+                        # the request restarts once before it is refused
+                        r = active[s][0]
+                        if r.rid in lone_retry:
+                            raise UnsupportedRequest("request %d (%d sites, depth %d) outgrowing the arena while "
+                                                     "running alone" % (r.rid, r.nsite, r.maxhier), "ServingLoop")
+                        lone_retry.add(r.rid)
                 for s in oom:
                     r = active.pop(s)[0]
                     need[r.rid] = 2 * self.rt.info(s)["bytes"]
                     self.rt.remove(s)
+                    n_in[s[0]] -= 1
                     reqs.appendleft(r)
                     n_preempt += 1
                 slots = [s for s in slots if s not in oom]
@@ -189,6 +214,7 @@ class ServingLoop:
                         res["peak_bytes"] = a[4]
                     done[r.rid] = res
                     self.rt.remove(s)
+                    n_in[s[0]] -= 1
                     del active[s]
             if log is not None and time.perf_counter() - last_log > 5.0:
                 last_log = time.perf_counter()

@@ -22,15 +22,18 @@ def _requests():
             Request(rid=14, arrival=0.05, nsite=12, maxhier=3, delta=8e-3, t_max=15.0)]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_alone_equals_in_mix(prec):
+@pytest.mark.parametrize("prec,sub", [("fp64", 1), ("mixed", 1), ("mixed", 3)])
+def test_alone_equals_in_mix(prec, sub):
+    """sub: runtime batches stepped concurrently (3: the four requests spread
+    over three batches, each with its own stream and worker threads)."""
     _need_gpu()
     from mesohops_amd.serving import ServingLoop
     reqs = _requests()
-    loop = ServingLoop(precision_mode=prec, arena_bytes=1 << 30, max_slots=8, record=True)
+    loop = ServingLoop(precision_mode=prec, arena_bytes=1 << 30, max_slots=9, record=True, sub_batches=sub)
+    assert loop.K == sub
     mix, stats = loop.run(reqs)
     loop.close()
-    print(prec, "mix stats:", stats)
+    print(prec, "sub-batches", sub, "mix stats:", stats)
     assert stats["completed"] == len(reqs)
     assert stats["p50_latency_s"] is not None and stats["p99_latency_s"] >= stats["p50_latency_s"]
     # the requests overlapped on the device
