@@ -790,22 +790,30 @@ double prefix_threshold(const std::vector<double> &v, double max_err, double off
 // value of rank i* - 1.  The bins of the value bits (the binary exponent,
 // then 11 mantissa bits at a time) are walked to the bin holding rank i*,
 // which is refined until it is small and then sorted.  The sums here are
-// added in another order, so each differs from the reference's by at most
-// about n * eps * (total); when the crossing is closer than that to max_err
-// the result is undecided (*ok = false) and the caller sorts instead.
+// added in another order.  Two summation orders of m non-negative values
+// differ by at most 2 (m - 1) eps times their sum, so:
+//  - the total, whose sign against max_err - offset decides whether any
+//    crossing exists, by n eps (total);
+//  - the partial sums below the crossing, which stay below max_err - offset,
+//    by n eps (|max_err| + |offset|); the sum at the crossing adds one value
+//    and one rounding, eps of its own size.
+// When a comparison lies closer than these margins the result is undecided
+// (*ok = false) and the caller sorts instead.
 double select_threshold(const std::vector<double> &v, double max_err, double offset, bool *ok) {
   *ok = true;
   const size_t n = v.size();
   if (n == 0) return 0.0;
   if (offset >= max_err) return 0.0;  // x0 + offset >= offset >= max_err
+  constexpr double kEps = 2.220446049250313e-16;
   double total = 0;
   for (double x : v) total += x;
-  const double tol = (4.0 * (double)n + 16.0) * 2.220446049250313e-16 * (total + std::abs(offset) + std::abs(max_err));
-  if (total + offset < max_err - tol) return INFINITY;
-  if (total + offset < max_err + tol) {
+  const double tol_total = (4.0 * (double)n + 16.0) * kEps * (total + std::abs(offset) + std::abs(max_err));
+  if (total + offset < max_err - tol_total) return INFINITY;
+  if (total + offset < max_err + tol_total) {
     *ok = false;
     return 0.0;
   }
+  const double tol = (4.0 * (double)n + 16.0) * kEps * (std::abs(offset) + std::abs(max_err));
   std::vector<double> cur(v), nxt;
   double base = 0;          // sum of every value ranked below the current bin
   double below_max = -1.0;  // largest such value (-1: none)
@@ -857,7 +865,7 @@ double select_threshold(const std::vector<double> &v, double max_err, double off
     const double prev = s;
     s += cur[j];
     if (s + offset >= max_err) {
-      if (prev + offset >= max_err - tol || s + offset < max_err + tol) {
+      if (prev + offset >= max_err - tol || s + offset < max_err + tol + 4.0 * kEps * std::abs(s + offset)) {
         *ok = false;
         return 0.0;
       }

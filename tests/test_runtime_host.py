@@ -170,6 +170,35 @@ def test_threshold_prefix_sort_matches_full_sort():
         assert below.value == pytest.approx(math.fsum(v[v <= ref]), rel=1e-12, abs=0), trial
 
 
+def test_threshold_crossing_far_below_total():
+    """The shape of the selection's error lists: a few large errors carry
+    almost all of the total, and the threshold crossing lies among the small
+    ones, max_err being many decades below the total.  The threshold equals
+    the oracle's on the fully sorted values, also when max_err is a running
+    sum of the sorted values exactly or one ulp either side of one (where
+    another summation order could decide differently)."""
+    from mesohops_amd._lib import lib
+    L = lib()
+    rng = np.random.default_rng(5)
+    for trial in range(120):
+        n_small = int(rng.choice([300, 5000, 40000]))
+        small = 10.0 ** rng.uniform(-16, -9, n_small)
+        big = 10.0 ** rng.uniform(-3, 1, int(rng.integers(1, 2000)))
+        v = np.concatenate([small, big])
+        rng.shuffle(v)
+        v = np.ascontiguousarray(v)
+        cs = np.cumsum(np.sort(v))
+        k = int(rng.integers(1, n_small))
+        base = float(cs[k])
+        max_err = float(rng.choice([base, np.nextafter(base, np.inf), np.nextafter(base, -np.inf),
+                                    base * (1 + 1e-9), float(small.sum()) * 0.5]))
+        offset = float(rng.choice([0.0, max_err * 0.25]))
+        below = ctypes.c_double()
+        thr = L.hops_rt_debug_threshold(v.ctypes.data, len(v), max_err, offset, ctypes.byref(below))
+        ref = O.determine_error_thresh(np.sort(v), max_err, offset)
+        assert thr == ref, (trial, len(v), max_err, offset)
+
+
 @pytest.mark.parametrize("name", ["peierls_both", "peierls_strong_ltc", "peierls_states_nonlinear",
                                   "peierls_ring_linear"])
 def test_runtime_error_terms_match_oracle(name):
