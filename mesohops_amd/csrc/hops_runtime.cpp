@@ -805,34 +805,53 @@ double select_threshold(const std::vector<double> &v, double max_err, double off
   if (n == 0) return 0.0;
   if (offset >= max_err) return 0.0;  // x0 + offset >= offset >= max_err
   constexpr double kEps = 2.220446049250313e-16;
-  double total = 0;
-  for (double x : v) total += x;
-  const double tol_total = (4.0 * (double)n + 16.0) * kEps * (total + std::abs(offset) + std::abs(max_err));
-  if (total + offset < max_err - tol_total) return INFINITY;
-  if (total + offset < max_err + tol_total) {
-    *ok = false;
-    return 0.0;
-  }
   const double tol = (4.0 * (double)n + 16.0) * kEps * (std::abs(offset) + std::abs(max_err));
-  std::vector<double> cur(v), nxt;
+  auto early = [&](double total, double *thr) {  // true: decided without a crossing
+    const double tol_total = (4.0 * (double)n + 16.0) * kEps * (total + std::abs(offset) + std::abs(max_err));
+    if (total + offset < max_err - tol_total) {
+      *thr = INFINITY;
+      return true;
+    }
+    if (total + offset < max_err + tol_total) {
+      *ok = false;
+      *thr = 0.0;
+      return true;
+    }
+    return false;
+  };
+  std::vector<double> cur, nxt;
   double base = 0;          // sum of every value ranked below the current bin
   double below_max = -1.0;  // largest such value (-1: none)
   constexpr int kB = 2048;
   static thread_local std::vector<uint32_t> cnt(kB);
   static thread_local std::vector<double> sum(kB), mx(kB);
   int shift = 52, width = 11;  // exponent first, then the mantissa from the top
-  while (cur.size() > 48 && shift >= 0) {
+  // the first level bins v itself (no copy), and its bin sums give the total
+  bool first = n > 48;
+  if (!first) {
+    double total = 0, thr = 0;
+    for (double x : v) total += x;
+    if (early(total, &thr)) return thr;
+    cur = v;
+  }
+  while ((first || cur.size() > 48) && shift >= 0) {
+    const std::vector<double> &in = first ? v : cur;
     const uint64_t mask = (1ull << width) - 1;
     std::fill(cnt.begin(), cnt.end(), 0u);
     std::fill(sum.begin(), sum.end(), 0.0);
     std::fill(mx.begin(), mx.end(), -1.0);
-    for (double x : cur) {
+    for (double x : in) {
       uint64_t u;
       std::memcpy(&u, &x, 8);
       const int e = (int)((u >> shift) & mask);
       cnt[e]++;
       sum[e] += x;
       mx[e] = std::max(mx[e], x);
+    }
+    if (first) {
+      double total = 0, thr = 0;
+      for (int e = 0; e <= (int)mask; ++e) total += sum[e];
+      if (early(total, &thr)) return thr;
     }
     int hit = -1;
     for (int e = 0; e <= (int)mask; ++e) {
@@ -849,12 +868,14 @@ double select_threshold(const std::vector<double> &v, double max_err, double off
       return 0.0;
     }
     nxt.clear();
-    for (double x : cur) {
+    nxt.reserve(cnt[hit]);
+    for (double x : in) {
       uint64_t u;
       std::memcpy(&u, &x, 8);
       if ((int)((u >> shift) & mask) == hit) nxt.push_back(x);
     }
     cur.swap(nxt);
+    first = false;
     if (shift == 0) break;
     width = std::min(11, shift == 52 ? 52 : shift);
     shift = shift == 52 ? 41 : shift - width;
@@ -1430,26 +1451,38 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
   // (depth < maxhier, k+e_m not in the basis) and down (k-e_m not in the
   // basis), in the reference's insertion order: stable column, flux up by
   // mode, then flux down by mode.  cand bit 0: up candidate, bit 1: down.
-  std::vector<uint8_t> cand((size_t)nh * nm, 0);
-  for (int k : out.stable_idx) {
+  // The candidates' values in this order (nz), with (mode << 1 | down) of
+  // each (nzc) and where each stable auxiliary's candidates start (nzo).
+  // (per host thread, reused: they keep their capacity between selections)
+  static thread_local std::vector<double> nz;
+  static thread_local std::vector<uint32_t> nzc;
+  nz.clear();
+  nzc.clear();
+  std::vector<size_t> nzo(out.stable_idx.size() + 1, 0);
+  std::vector<uint8_t> cand(nm);
+  for (size_t i = 0; i < out.stable_idx.size(); ++i) {
+    const int k = out.stable_idx[i];
     const size_t o = (size_t)k * nm;
-    if (B.depth[k] < P.maxhier)
-      for (int lm = 0; lm < nm; ++lm) cand[o + lm] = (F_up[o + lm] && E_up[o + lm] != 0) ? 1 : 0;
-    for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) cand[o + B.up_mode[u]] &= (uint8_t)~1u;
+    nzo[i] = nz.size();
+    const bool below_top = B.depth[k] < P.maxhier;
     for (int lm = 0; lm < nm; ++lm)
-      if (kval[o + lm] > 0 && E_dn[o + lm] != 0) cand[o + lm] |= 2;
-    for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) cand[o + B.dn_mode[u]] &= (uint8_t)~2u;
-  }
-  std::vector<double> nz;
-  nz.reserve((size_t)out.stable_idx.size() * 4);
-  for (int k : out.stable_idx) {
-    const size_t o = (size_t)k * nm;
+      cand[lm] = (uint8_t)((below_top && F_up[o + lm] && E_up[o + lm] != 0 ? 1 : 0) |
+                           (kval[o + lm] > 0 && E_dn[o + lm] != 0 ? 2 : 0));
+    for (int u = B.up_ptr[k]; u < B.up_ptr[k + 1]; ++u) cand[B.up_mode[u]] &= (uint8_t)~1u;
+    for (int u = B.dn_ptr[k]; u < B.dn_ptr[k + 1]; ++u) cand[B.dn_mode[u]] &= (uint8_t)~2u;
     for (int lm = 0; lm < nm; ++lm) {
-      const uint8_t c = cand[o + lm];
-      if (c & 1) nz.push_back(E_up[o + lm]);
-      if (c & 2) nz.push_back(E_dn[o + lm]);
+      const uint8_t c = cand[lm];
+      if (c & 1) {
+        nz.push_back(E_up[o + lm]);
+        nzc.push_back((uint32_t)lm << 1);
+      }
+      if (c & 2) {
+        nz.push_back(E_dn[o + lm]);
+        nzc.push_back((uint32_t)lm << 1 | 1u);
+      }
     }
   }
+  nzo[out.stable_idx.size()] = nz.size();
   double discarded = 0;
   const double thr1 = partial_threshold(nz, dbound * P.f_discard * P.f_discard, 0.0, &discarded);
   // accumulate the survivors by target auxiliary (Zobrist key), in the
@@ -1460,18 +1493,21 @@ void select_hier(const Params &P, const Basis &B, const cplx *pn, const double *
     int k, lm, dir;
     double v;
   };
-  std::vector<Cand> cs;
-  cs.reserve(nz.size());
-  for (int k : out.stable_idx) {
+  static thread_local std::vector<Cand> cs;
+  cs.clear();
+  for (size_t i = 0; i < out.stable_idx.size(); ++i) {
+    const size_t q0 = nzo[i], q1 = nzo[i + 1];
+    if (q0 == q1) continue;
+    const int k = out.stable_idx[i];
     const size_t o = (size_t)k * nm;
     const AKey kk{B.aux_hash[k], B.aux_hash2[k]};
-    for (int d = 0; d < 2; ++d) {
-      const double *E = d == 0 ? &E_up[o] : &E_dn[o];
+    for (uint32_t d = 0; d < 2; ++d) {  // flux up by mode, then flux down by mode
       const int dir = d == 0 ? 1 : -1;
-      for (int lm = 0; lm < nm; ++lm)
-        if ((cand[o + lm] & (1 << d)) && E[lm] > thr1) {
+      for (size_t q = q0; q < q1; ++q)
+        if ((nzc[q] & 1u) == d && nz[q] > thr1) {
+          const int lm = (int)(nzc[q] >> 1);
           const int v0 = kval[o + lm];
-          cs.push_back(Cand{key_step(S, kk, B.modes[lm], v0, v0 + dir), k, lm, dir, E[lm]});
+          cs.push_back(Cand{key_step(S, kk, B.modes[lm], v0, v0 + dir), k, lm, dir, nz[q]});
         }
     }
   }
