@@ -220,3 +220,47 @@ def test_host_raw_noise_matches_golden():
     nz = HopsNoise({"SEED": 0, "TLEN": 10.0, "TAU": 1.0}, SystemInfo(sysp), sysp, 1)
     np.testing.assert_allclose(nz.raw(), fx["pre_calculated_uncorrelated_noise_test"])
     np.testing.assert_array_equal(nz.raw(), O.uncorrelated_noise(0, [0, 1], 11, "SUM_GAUSSIAN"))
+
+
+def test_ragged_group_steps_every_batch_and_raises_after_all_finish():
+    """RaggedGroup's host logic, without a device: listed (batch, slot) pairs
+    go to their batch, batches without listed slots are not stepped, every
+    batch runs on its own host thread, and a batch's error is raised only
+    after the other batches have finished their steps."""
+    import threading
+    import time
+    from mesohops_amd import ragged
+    from mesohops_amd.errors import ArenaFull
+
+    class Part:
+        def __init__(self, k, fail=False):
+            self.k, self.fail, self.calls, self.done, self.thread = k, fail, [], False, None
+
+        def step(self, n, slots=None):
+            self.thread = threading.current_thread().name
+            self.calls.append((n, slots))
+            if self.fail:
+                raise ArenaFull("batch %d full" % self.k)
+            time.sleep(0.05)
+            self.done = True
+
+    g = ragged.RaggedGroup.__new__(ragged.RaggedGroup)
+    g.parts = [Part(0), Part(1, fail=True), Part(2)]
+    g.device = "cpu"
+    g._lock = threading.Lock()
+    g._profiling = False
+    g._prof = ([0.0] * 4, [0] * 4)
+    orig = ragged.torch.cuda.synchronize
+    ragged.torch.cuda.synchronize = lambda *a, **k: None
+    try:
+        with pytest.raises(ArenaFull):
+            g.step(1, slots=[(0, 3), (1, 0), (0, 5)])
+        assert g.parts[0].calls == [(1, [3, 5])] and g.parts[0].done
+        assert g.parts[1].calls == [(1, [0])]
+        assert g.parts[2].calls == []  # no listed slot: not stepped
+        g.parts[1].fail = False
+        g.step(2)
+        assert [p.calls[-1] for p in g.parts] == [(2, None)] * 3
+        assert [p.thread for p in g.parts] == ["hops-batch-%d" % k for k in range(3)]
+    finally:
+        ragged.torch.cuda.synchronize = orig
