@@ -29,6 +29,12 @@ t_max U[200, 2000] fs, dt 1 fs; continuous batching; p50/p99 latency.  With
 --c4-rates rates, each reporting offered against completed load, the queue
 over time and whether it stayed bounded.  The default run ends with a 20 s
 200/s stream ("serving_c4").
+--workload dense: the dense system block on the matrix cores — a fully
+coupled 100-site aggregate (point-dipole couplings V / |i - j|^3 between every
+pair, V = 100 cm^-1, E_n ~ N(0, 100 cm^-1), one high-T DL mode per site),
+LINEAR EOM, depth 2 (5151 auxiliaries x 100 states), B trajectories (default
+64), dt 2 fs; -iH Y is a complex GEMM on the 16x16x4 MFMA (f32 with complex64
+stages), reported against the 157.3 TFLOP/s dense fp32 peak.
 """
 import argparse
 import json
@@ -334,6 +340,99 @@ def run_c4_stream(rate, window, prec="mixed", log=None):
     return stats
 
 
+MFMA_PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}  # dense fp32 / fp64 matrix peaks of MI355X
+
+
+def main_dense(args):
+    """Batched LINEAR propagation of a fully coupled aggregate: the dense
+    system block (k_dense_h, MFMA) next to the stage kernel.  Algorithmic
+    flops of one k_dense_h launch: 8 n_state^2 n_hier B (complex GEMM of
+    -iH over every auxiliary block of every trajectory)."""
+    import torch
+    from mesohops_amd import HopsBatch, _lib
+    from mesohops_amd.workloads import aggregate_system
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    nsite, depth = args.dense_sites, args.dense_depth
+    B = args.batch if args.batch != 256 else 64
+    sysp = aggregate_system(nsite)
+    noise = {"MODEL": "FFT_FILTER", "TLEN": max(64.0, (args.steps + 2) * DT + 8.0), "TAU": 1.0}
+    psi = np.zeros(nsite, np.complex128)
+    psi[nsite // 2] = 1.0
+    dev = torch.device("cuda:0")
+    ns = int(round(DT * 0.5 / noise["TAU"]))
+
+    def make(prec):
+        b = HopsBatch(sysp, EOM, dict(noise, SEED=0), None, {"MAXHIER": depth}, seeds=list(range(B)),
+                      precision_mode=prec, device="cuda:0", store_psi=True)
+        b.initialize(psi)
+        return b
+
+    def run_on(state, steps):
+        if state.psi_store is None or state.store_slots < steps:
+            state.store_slots = steps
+            state.psi_store = torch.zeros(B * steps * state_n, dtype=torch.complex128, device=dev)
+            state.bt.store_stride = steps
+        state.bt.psi_store_d = state.psi_store.data_ptr()
+        state.propagate(0, steps, DT, ns, 0)
+
+    out = {}
+    for prec in (args.precision, "fp64") if args.precision != "fp64" else ("fp64",):
+        log("dense: %d-site aggregate, depth %d, %d trajectories, %s" % (nsite, depth, B, prec))
+        batch = make(prec)
+        tb = batch.tables
+        if not tb.dense_h:
+            sys.exit("dense: the system block did not take the dense path")
+        state_n = tb.n_state
+        st = batch.state
+        run_on(st, args.warmup)
+        torch.cuda.synchronize()
+        batch.rewind(psi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_on(st, args.steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        psi_t = st.psi_store_host(args.steps)
+        batch.rewind(psi)
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        run_on(st, args.steps)
+        torch.cuda.synchronize()
+        ms, n = _lib.profile_read()
+        _lib.profile_enable(False)
+        flops = 8.0 * tb.n_state ** 2 * tb.n_hier * B
+        dense_ms = ms[3] / max(n[3], 1)
+        achieved = flops / (dense_ms * 1e-3) / 1e12 if n[3] else 0.0
+        out[prec] = {"value": B * args.steps / elapsed, "elapsed_s": elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+                     "dense_kernel": {"avg_ms": dense_ms, "launches": n[3], "flops_per_launch": flops,
+                                      "achieved_TFLOPs": achieved, "peak_TFLOPs": MFMA_PEAK_TFLOPS[prec],
+                                      "frac": achieved / MFMA_PEAK_TFLOPS[prec]},
+                     "stage_kernel_avg_ms": ms[1] / max(n[1], 1), "prologue_avg_ms": ms[0] / max(n[0], 1),
+                     "psi": psi_t}
+        del batch, st
+        torch.cuda.empty_cache()
+    main = out[args.precision]
+    err = None
+    if args.precision != "fp64":
+        pm, pr = np.abs(out[args.precision]["psi"]) ** 2, np.abs(out["fp64"]["psi"]) ** 2
+        err = float(np.max(np.abs(pm - pr) / pr.sum(-1, keepdims=True)))
+    for v in out.values():
+        v.pop("psi")
+    dk = main["dense_kernel"]
+    line = {"metric": METRIC, "value": main["value"], "unit": "traj·RK4-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": main["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "c64" if args.precision == "mixed" else "c128", "data": "synthetic",
+            "config": {"workload": "batched LINEAR HOPS of a fully coupled %d-site aggregate (dense H), 1 DL mode/site, "
+                                   "depth %d (%d aux x %d states), %d trajectories, dt 2 fs"
+                                   % (nsite, depth, int(round(dk["flops_per_launch"] / (8.0 * nsite ** 2 * B))),
+                                      nsite, B)},
+            "roofline": {"bound": "mfma", "achieved": dk["achieved_TFLOPs"], "peak": dk["peak_TFLOPs"],
+                         "unit": "TFLOP/s", "frac": dk["frac"], "traffic": None, "kernel": "k_dense_h",
+                         "kernel_avg_ms": dk["avg_ms"]},
+            "pop_err_rel_vs_fp64": err, "legs": out}
+    print(json.dumps(line), flush=True)
+
+
 def main_c4(args):
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.c4_window > 0:
@@ -403,7 +502,9 @@ def main():
     ap.add_argument("--lds", type=int, default=0, help="LDS-staged stationary kernel with --lanes (0 or 1)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "dense"])
+    ap.add_argument("--dense-sites", type=int, default=100, help="dense: sites of the fully coupled aggregate")
+    ap.add_argument("--dense-depth", type=int, default=2, help="dense: hierarchy depth")
     ap.add_argument("--requests", type=int, default=64, help="c4: requests in the stream")
     ap.add_argument("--c4-window", type=float, default=0.0,
                     help="c4: serve an open 200/s stream for this many seconds, then the --c4-rates streams "
@@ -425,6 +526,8 @@ def main():
         return main_c3(args)
     if args.workload == "c4":
         return main_c4(args)
+    if args.workload == "dense":
+        return main_dense(args)
     if (args.steps + 1) * DT > NOISE["TLEN"] or (args.warmup + 1) * DT > NOISE["TLEN"]:
         sys.exit("steps x dt exceeds the noise length %.0f fs" % NOISE["TLEN"])
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731

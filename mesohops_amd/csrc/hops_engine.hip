@@ -827,6 +827,113 @@ unsigned dense_grid(const hops_op_t &op, int G, int lane_shift) {
   return (unsigned)((long)G * n_ct * n_rq);
 }
 
+// LDS-staged form of k_dense_h: one workgroup per 16-column tile computes
+// every row tile of it.  The tile's Y block (hd_n rows x 16 columns, zero
+// rows past n_state) is read from HBM once, coalesced, into LDS (column c at
+// ys[c * hs + s'], hs = hd_n + 2), and the four waves take the row tiles
+// rt = w, w + 4, ... two at a time (independent accumulators).  Within each
+// 16-deep slice kb of the sum, MFMA step v pairs lane (q, r)'s A[i0 + r][kb +
+// 4q + v] with Y[kb + 4q + v][j0 + r]: every product enters exactly once, and
+// lane (q, r) reads its four A values of the slice as one 16-byte (f32) or
+// 32-byte (f64) load and its four Y values from consecutive LDS words.
+template <typename R> struct Vec4;
+template <> struct Vec4<float> { typedef float T __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<double> { typedef double T __attribute__((ext_vector_type(4))); };
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_lds(const hops_op_t op, const hops_batch_t bt,
+                                                   const cx<R> *__restrict__ yin, cx<R> *__restrict__ hy,
+                                                   int hs) {
+  typedef typename Acc4<R>::T V4;
+  typedef typename Vec4<R>::T A4;
+  extern __shared__ __align__(16) unsigned char dense_lds[];
+  cx<R> *ys = reinterpret_cast<cx<R> *>(dense_lds);
+  const int n = op.n_state, hn = op.hd_n, ls = bt.lane_shift;
+  const int T = 1 << ls, Tc = T < 16 ? T : 16;
+  const long nr = (long)n * op.n_hier;
+  const long ncol = (long)op.n_hier << ls;
+  const long n_ct = (ncol + 15) >> 4;
+  const long ct = (long)blockIdx.x % n_ct, g = (long)blockIdx.x / n_ct;
+  const long j0 = ct << 4;
+  // the Y block: e runs over (column group, row, lane) with the lane fastest,
+  // the order of the pool (aux-major rows, T lanes interleaved)
+  for (int e = threadIdx.x; e < 16 * hn; e += 256) {
+    const int grp = e / (hn * Tc), rem = e - grp * hn * Tc;
+    const int sp = rem / Tc, tt = rem - sp * Tc;
+    const int c = grp * Tc + tt;
+    const long j = j0 + c;
+    cx<R> v = mk<R>((R)0, (R)0);
+    if (sp < n && j < ncol) v = yin[(g * nr + (j >> ls) * n + sp) * T + (j & (T - 1))];
+    ys[c * hs + sp] = v;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+  const int n_rt = hn >> 4;
+  const R *Ar = reinterpret_cast<const R *>(op.hd_d);
+  const R *Ai = Ar + (long)hn * hn;
+  const long j = j0 + r;
+  const bool jv = j < ncol;
+  const long cbase = jv ? (g * nr + (j >> ls) * n) * T + (j & (T - 1)) : 0;
+  const cx<R> *yc = ys + r * hs + 4 * q;
+  for (int ra = w; ra < n_rt; ra += 8) {
+    const int rb = ra + 4;
+    const bool two = rb < n_rt;
+    const A4 *pa = reinterpret_cast<const A4 *>(Ar + (long)(ra * 16 + r) * hn + 4 * q);
+    const A4 *pai = reinterpret_cast<const A4 *>(Ai + (long)(ra * 16 + r) * hn + 4 * q);
+    const A4 *pb = reinterpret_cast<const A4 *>(Ar + (long)((two ? rb : ra) * 16 + r) * hn + 4 * q);
+    const A4 *pbi = reinterpret_cast<const A4 *>(Ai + (long)((two ? rb : ra) * 16 + r) * hn + 4 * q);
+    V4 cra = {0, 0, 0, 0}, cia = {0, 0, 0, 0}, crb = {0, 0, 0, 0}, cib = {0, 0, 0, 0};
+    for (int kb = 0; kb < hn; kb += 16) {
+      const A4 a = pa[kb >> 2], ai = pai[kb >> 2];
+      A4 b = a, bi = ai;
+      if (two) {
+        b = pb[kb >> 2];
+        bi = pbi[kb >> 2];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const cx<R> y = yc[kb + v];
+        cra = mfma16(a[v], y.x, cra);
+        cra = mfma16(-ai[v], y.y, cra);
+        cia = mfma16(a[v], y.y, cia);
+        cia = mfma16(ai[v], y.x, cia);
+        if (two) {
+          crb = mfma16(b[v], y.x, crb);
+          crb = mfma16(-bi[v], y.y, crb);
+          cib = mfma16(b[v], y.y, cib);
+          cib = mfma16(bi[v], y.x, cib);
+        }
+      }
+    }
+    if (jv) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int sa = ra * 16 + Acc4<R>::row(q, v);
+        if (sa < n) hy[cbase + (long)sa * T] = mk<R>(cra[v], cia[v]);
+        const int sb = rb * 16 + Acc4<R>::row(q, v);
+        if (two && sb < n) hy[cbase + (long)sb * T] = mk<R>(crb[v], cib[v]);
+      }
+    }
+  }
+}
+
+// (-iH) Y of G trajectory groups: the LDS-staged form when its Y block fits
+// in 64 KiB of LDS, else the plain form
+template <typename R>
+int launch_dense_h(const hops_op_t &op, const hops_batch_t &bt, int G, const cx<R> *yin, cx<R> *hy,
+                   hipStream_t st) {
+  const int hs = op.hd_n + 2;
+  const size_t lds = (size_t)16 * hs * sizeof(cx<R>);
+  if (lds <= 64 * 1024) {
+    const long n_ct = (((long)op.n_hier << bt.lane_shift) + 15) >> 4;
+    k_dense_lds<R><<<(unsigned)(G * n_ct), 256, lds, st>>>(op, bt, yin, hy, hs);
+  } else {
+    k_dense_h<R><<<dense_grid(op, G, bt.lane_shift), 256, 0, st>>>(op, bt, yin, hy);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // Resident form for small bases: one workgroup per trajectory runs every
 // RK4 step of the call, the four prologues and stages of a step separated by
 // workgroup barriers (a trajectory's rows only read rows of the same
@@ -1675,9 +1782,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     else HOPS_SHV(W, P, 0);                                                                      \
   } while (0)
       if (dense) {
-        k_dense_h<R><<<dense_grid(*op, G, sb.lane_shift), 256, 0, ls>>>(*op, sbi, yin,
-                                                                          reinterpret_cast<cx<R> *>(sbi.hy_d));
-        HIP_TRY(hipGetLastError());
+        if (int rc = launch_dense_h<R>(*op, sbi, G, yin, reinterpret_cast<cx<R> *>(sbi.hy_d), ls)) return rc;
         if (int rc = prof_mark(prof, ls, 3, &used)) return rc;
       }
       if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
@@ -2206,9 +2311,7 @@ int hops_derivative(const hops_op_t *op, const hops_batch_t *bt, const void *y_d
   HIP_TRY(hipGetLastError());
   if (op->hd_d) {
     const int G = (bt->B + (1 << bt->lane_shift) - 1) >> bt->lane_shift;
-    k_dense_h<double><<<dense_grid(*op, G, bt->lane_shift), 256, 0, st>>>(*op, *bt, y,
-                                                                          reinterpret_cast<cx<double> *>(bt->hy_d));
-    HIP_TRY(hipGetLastError());
+    if (int rc = launch_dense_h<double>(*op, *bt, G, y, reinterpret_cast<cx<double> *>(bt->hy_d), st)) return rc;
   }
   k_stage<double><<<bt->n_tiles, 256, 0, st>>>(*op, *bt, -1, y, reinterpret_cast<cx<double> *>(d_d));
   HIP_TRY(hipGetLastError());

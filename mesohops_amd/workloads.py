@@ -4,6 +4,7 @@ chain_system: nearest-neighbour chain, E_n ~ N(0, sigma) from
 np.random.default_rng(seed_h), Drude-Lorentz modes per site from
 bcf_convert_dl_to_exp (util/bath_corr_functions.py:70-112), L_n = |n><n|,
 optional low-temperature correction on every site.
+aggregate_system: every pair of sites coupled (point dipoles), a dense H.
 lattice_system: nx x ny nearest-neighbour lattice with the low-temperature
 correction: per site the DL correlation function with k_hier + k_ltc Matsubara
 terms, the first modes in the hierarchy, sum g/w of the last k_ltc in the
@@ -40,6 +41,26 @@ def chain_system(nsite, V=50.0, sigma=100.0, seed_h=0, lam=35.0, gam=50.0, temp=
         p["L_LT_CORR"] = [sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite)) for i in range(nsite)]
         p["PARAM_LT_CORR"] = [ltc] * nsite
     return p
+
+
+def aggregate_system(nsite, V=100.0, sigma=100.0, seed_h=0, lam=35.0, gam=50.0, temp=295.0, k_mats=0):
+    """A fully coupled aggregate: sites on a line with point-dipole couplings
+    J_ij = V / |i - j|^3 between every pair, so H has no zero entry (the
+    dense system block), E_n ~ N(0, sigma); Drude-Lorentz modes per site as
+    chain_system."""
+    rng = np.random.default_rng(seed_h)
+    H = np.diag(rng.normal(0.0, sigma, nsite)).astype(np.complex128)
+    for i in range(nsite):
+        for j in range(i + 1, nsite):
+            H[i, j] = H[j, i] = V / (j - i) ** 3
+    modes = bcf_convert_dl_to_exp(lam, gam, temp, k_matsubara=k_mats)
+    gw, ll = [], []
+    for i in range(nsite):
+        L = sp.coo_matrix(([1.0], ([i], [i])), shape=(nsite, nsite))
+        for j in range(0, len(modes), 2):
+            gw.append([modes[j], modes[j + 1]])
+            ll.append(L)
+    return dict(HAMILTONIAN=H, GW_SYSBATH=gw, L_HIER=ll, L_NOISE1=ll, ALPHA_NOISE1=bcf_exp, PARAM_NOISE1=gw)
 
 
 def lattice_system(nx, ny, V=40.0, sigma=150.0, seed_h=0, lam=35.0, gam=50.0, temp=77.0, k_hier=1, k_ltc=3):
