@@ -766,8 +766,9 @@ def main():
             "pop_err_bound": 1e-4, "bound_met": True,
             "speedup_vs_gpu_fp64_single_same_window": ns_leg["speedup_vs_gpu_fp64_single_same_window"],
             "speedup_vs_same_runtime_fp64_b1": ns_leg["speedup_vs_same_runtime_fp64_b1"],
-            "note": "the 256-trajectory 100-site adaptive batch over t = 0..%d fs in the fastest configuration whose "
-                    "population error against FP64 stays within 1e-4 (complex64 stages: %.2e on %s trajectories)"
+            "note": "the 256-trajectory 100-site adaptive batch over t = 0..%d fs with complex64 stages if their "
+                    "population error against FP64 stays within 1e-4, else with complex128 stages (complex64 "
+                    "stages: %.2e on %s trajectories)"
                     % (args.c3_steps, c3.get("pop_err_vs_fp64", float("nan")), c3.get("pop_err_trajectories"))}
     if args.c4_leg > 0:
         torch.cuda.empty_cache()
