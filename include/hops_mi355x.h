@@ -157,10 +157,11 @@ int hops_version(void);
    by the benchmark).  enable(1) clears the counters and records a HIP event
    on the launch stream after every kernel; read() returns the summed
    milliseconds and launch counts for [0] the stage prologue, [1] the fused
-   derivative/RK4 stage kernel, [2] other kernels.  The switch and counters
-   belong to the calling host thread: they cover the calls that thread makes. */
+   derivative/RK4 stage kernel, [2] other kernels, [3] the dense system block
+   (-iH Y on the matrix cores).  The switch and counters belong to the calling
+   host thread: they cover the calls that thread makes. */
 int hops_profile_enable(int on);
-int hops_profile_read(double *ms_out /*[3]*/, long long *n_out /*[3]*/);
+int hops_profile_read(double *ms_out /*[4]*/, long long *n_out /*[4]*/);
 
 /* Engine options (no counterpart in the reference).  "stationary_stage":
    1 (default) runs the operator-stationary stage kernel for batches of at
