@@ -10,7 +10,9 @@ on the 16x16x4 MFMA (f64 for complex128 stages, f32 for complex64).
   within the population bar;
 - a batch against its trajectories alone; the derivative entry point;
 - every tile edge: state counts that are and are not multiples of 16, column
-  counts that are not multiples of 16, one and eight trajectory lanes.
+  counts that are not multiples of 16, one and eight trajectory lanes;
+- the LDS-staged kernel (k_dense_lds) for these sizes, and the plain kernel
+  (k_dense_h) for a basis too large for its LDS block.
 """
 import ctypes
 
@@ -153,3 +155,24 @@ def test_dense_derivative_equals_sparse():
     err = np.max(np.abs(outs[0] - outs[1])) / np.max(np.abs(outs[1]))
     print("derivative dense vs sparse: relative %.3e" % err)
     assert err < 1e-13
+
+
+def test_dense_plain_form_for_large_blocks_equals_sparse():
+    """A state basis whose staged Y block would not fit in 64 KiB of LDS (250
+    states in FP64: 16 columns x 258 rows x 16 bytes) takes the plain dense
+    kernel (k_dense_h): the same psi(t) as the sparse pattern path to 1e-12,
+    over two trajectories in one group of two lanes."""
+    _need_gpu()
+    n = 250
+    sysp = systems.aggregate(n, seed_h=11)
+    tlen, tau, t_max, dt = 30.0, 1.0, 8.0, 2.0
+    zs = [_noise(sysp, 200 + i, tlen, tau) for i in range(2)]
+    eom = "NORMALIZED NONLINEAR"
+    print("case dense plain form, %d states" % n, flush=True)
+    d = _run(sysp, eom, zs, tlen, tau, 1, t_max, dt, _psi0(n), lanes=2)
+    assert d.dop.tables.dense_h
+    s = _run(sysp, eom, zs, tlen, tau, 1, t_max, dt, _psi0(n), lanes=2, dense=False)
+    for i in range(2):
+        err = np.max(np.abs(np.array(d.psi_traj[i]) - np.array(s.psi_traj[i])))
+        print("plain dense form, traj %d: dense vs sparse %.3e" % (i, err))
+        assert err < 1e-12
