@@ -363,7 +363,7 @@ def main_dense(args):
 
     def make(prec):
         b = HopsBatch(sysp, EOM, dict(noise, SEED=0), None, {"MAXHIER": depth}, seeds=list(range(B)),
-                      precision_mode=prec, device="cuda:0", store_psi=True)
+                      precision_mode=prec, device="cuda:0", store_psi=True, lanes=args.lanes or None)
         b.initialize(psi)
         return b
 
