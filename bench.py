@@ -376,6 +376,8 @@ def main_dense(args):
         state.propagate(0, steps, DT, ns, 0)
 
     out = {}
+    choice = None
+    from mesohops_amd import autotune
     for prec in (args.precision, "fp64") if args.precision != "fp64" else ("fp64",):
         log("dense: %d-site aggregate, depth %d, %d trajectories, %s" % (nsite, depth, B, prec))
         batch = make(prec)
@@ -383,6 +385,17 @@ def main_dense(args):
         if not tb.dense_h:
             sys.exit("dense: the system block did not take the dense path")
         state_n = tb.n_state
+        if args.lanes == 0:
+            # the layout and stage-kernel form tuned on the first batch, kept
+            # for the second
+            if choice is None:
+                choice, tune_t = autotune.tune(batch, psi, DT, verbose=True)
+                out["layout"] = {"lanes": choice[0], "stationary_stage": bool(choice[1]), "nt_stream": choice[3],
+                                 "vec_lanes": choice[6], "lds_stage": bool(choice[7])}
+            else:
+                batch.relayout(choice[0])
+                autotune.apply(choice, batch)
+                batch.rewind(psi)
         st = batch.state
         run_on(st, args.warmup)
         torch.cuda.synchronize()
@@ -411,6 +424,7 @@ def main_dense(args):
                      "psi": psi_t}
         del batch, st
         torch.cuda.empty_cache()
+    layout = out.pop("layout", None)
     main = out[args.precision]
     err = None
     if args.precision != "fp64":
@@ -429,7 +443,7 @@ def main_dense(args):
             "roofline": {"bound": "mfma", "achieved": dk["achieved_TFLOPs"], "peak": dk["peak_TFLOPs"],
                          "unit": "TFLOP/s", "frac": dk["frac"], "traffic": None, "kernel": "k_dense_h",
                          "kernel_avg_ms": dk["avg_ms"]},
-            "pop_err_rel_vs_fp64": err, "legs": out}
+            "pop_err_rel_vs_fp64": err, "layout": layout, "legs": out}
     print(json.dumps(line), flush=True)
 
 

@@ -1033,7 +1033,7 @@ __device__ __forceinline__ void stv(T *p, const T (&v)[V]) {
 // pattern values and coefficients are read from LDS: ly points at lane b of
 // the row's aux block (element s' of lane b + v at ly[s' T + v]), lc at lane
 // b of the coefficient set of the row (coefficient p at lc[p T + v]).
-template <typename R, int WR, int PW, int NT, int V, bool LDS = false>
+template <typename R, int WR, int PW, int NT, int V, bool LDS = false, bool HY = false>
 __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_batch_t &bt, int stage,
                                               const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
                                               int b, long r_, int k, cx<R> k0, const int *ecol,
@@ -1092,6 +1092,9 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
       for (int v = 0; v < V; ++v) ye[e][v] = zero;
     }
   }
+  // the dense system block's (-iH) Y of this row (k_dense_lds / k_dense_h)
+  cx<R> hyv[V];
+  if constexpr (HY) ldv<NT >= 1 ? 1 : 0, V>(el(reinterpret_cast<const cx<R> *>(bt.hy_d), gi), hyv);
   double sigma[V];
   ldv<0, V>(bt.sigT_d + b, sigma);
   cx<R> base[V], a_in[V];
@@ -1118,6 +1121,7 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     d[v] = cmul(k0, yr[v]);
+    if constexpr (HY) d[v] = cadd(d[v], hyv[v]);
 #pragma unroll
     for (int p = 0; p < PW; ++p) cfma(d[v], ap[p][v], yp[p][v]);
 #pragma unroll
@@ -1186,7 +1190,7 @@ __device__ __forceinline__ void stage_row_reg(const hops_op_t &op, const hops_ba
 // g = x + 8 (q + Q j) of XCD x = i % 8, so the operator is read once per tile
 // and stage instead of once per trajectory, and every XCD works on the same
 // few groups at a time (their rows stay in its L2).
-template <typename R, int WR, int PW, int NT, int V>
+template <typename R, int WR, int PW, int NT, int V, bool HY = false>
 __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const hops_batch_t bt, int stage,
                                                       const cx<R> *__restrict__ yin,
                                                       cx<R> *__restrict__ yout, int n_tiles, int Q) {
@@ -1224,7 +1228,8 @@ __global__ __launch_bounds__(512) void k_stage_shared(const hops_op_t op, const 
   const int G = (bt.B + (1 << bt.lane_shift) - 1) >> bt.lane_shift;
   for (int g = x + 8 * q; g < G; g += 8 * Q) {
     const int b = (g << bt.lane_shift) + lane;
-    if (b < bt.B) stage_row_reg<R, WR, PW, NT, V>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
+    if (b < bt.B)
+      stage_row_reg<R, WR, PW, NT, V, false, HY>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol);
   }
 }
 
@@ -1246,12 +1251,14 @@ struct LdsStage {
 // The complex64 forms with ELL and pattern widths 4 and up to 2 lanes per
 // thread fit 128 VGPRs without spilling: they ask for 4 waves per SIMD (two
 // workgroups per CU); the others take what their registers allow.
-template <typename R, int WR, int PW, int V>
+// (the forms that also add a dense system block need four more registers per
+// lane pair: they take what their registers allow rather than spill)
+template <typename R, int WR, int PW, int V, bool HY>
 constexpr int lds_min_waves() {
-  return (sizeof(R) == 4 && WR == 4 && PW == 4 && V <= 2) ? 4 : 1;
+  return (!HY && sizeof(R) == 4 && WR == 4 && PW == 4 && V <= 2) ? 4 : 1;
 }
-template <typename R, int WR, int PW, int NT, int V>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(lds_min_waves<R, WR, PW, V>()))) void k_stage_lds(const hops_op_t op, const hops_batch_t bt, int stage,
+template <typename R, int WR, int PW, int NT, int V, bool HY = false>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(lds_min_waves<R, WR, PW, V, HY>()))) void k_stage_lds(const hops_op_t op, const hops_batch_t bt, int stage,
                                                    const cx<R> *__restrict__ yin, cx<R> *__restrict__ yout,
                                                    int n_tiles, int Q, int aux_per_tile) {
   extern __shared__ double lds_raw[];
@@ -1350,7 +1357,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(lds_min_wav
     __syncthreads();
     if (g + 8 * Q < G) fetch(g + 8 * Q, f);  // next group, in flight during this one
     if (run)
-      stage_row_reg<R, WR, PW, NT, V, true>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol,
+      stage_row_reg<R, WR, PW, NT, V, true, HY>(op, bt, stage, yin, yout, b, r, k, k0, ecol, ev, pidx, pcol,
                                             ys + (size_t)(kl * n) * T + lane,
                                             cl + (size_t)(k == 0 ? np : 0) * T + lane, yr);
   }
@@ -1666,9 +1673,12 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   // divisor of every (sub-)batch size so no vector straddles the last lane
   int vl = g_vec_lanes;
   while (vl > 1 && (vl > T || bt->B % vl != 0 || chunk % vl != 0)) vl >>= 1;
+  if (dense && vl > 2) vl = 2;  // the forms with a dense block take at most 2 (the grid below follows vl)
   // the stationary kernel addresses its pools with 32-bit byte offsets
   const bool pools32 = (double)nr * (double)((long)G << bt->lane_shift) * 16.0 < 4294967296.0;
-  if (g_stationary && !dense && pools32 && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= 8) {
+  // (a dense system block: the stationary forms add its (-iH) Y, built for
+  // pattern width 4, which the L terms alone fill at most)
+  if (g_stationary && pools32 && Gc >= 8 && op->ell_w <= 16 && max_pat > 0 && max_pat <= (dense ? 4 : 8)) {
     sh_w = op->ell_w <= 4 ? 4 : (op->ell_w <= 8 ? 8 : 16);
     sh_p = max_pat <= 4 ? 4 : 8;
     const long rows_per_tile = (512L * vl) >> bt->lane_shift;
@@ -1691,6 +1701,7 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
   } else {
     chunk = bt->B;  // chunks need the stationary kernel (no tile list per chunk)
   }
+  if (dense) chunk = bt->B;  // the dense block's launch covers the whole batch
   // fused LINEAR prologue: stream-ordered scratch for the coefficient
   // tables of stages 1..3, freed on the same stream when the call ends (or
   // the caller's scratch of the allocation-free form)
@@ -1781,11 +1792,30 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
     else if (g_nt_stream == 4) HOPS_SHV(W, P, 4);                                                \
     else HOPS_SHV(W, P, 0);                                                                      \
   } while (0)
+// with the dense block: hint levels 0 and 2 (a level only changes what L2
+// keeps), 1 or 2 lanes per thread (vl was capped at 2 before the grid)
+#define HOPS_SHD(W, NT)                                                                          \
+  do {                                                                                           \
+    if (lds_apt > 0) {                                                                           \
+      if (vl >= 2) k_stage_lds<R, W, 4, NT, 2, true><<<lds_grid, 512, lds_bytes, ls>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+      else k_stage_lds<R, W, 4, NT, 1, true><<<lds_grid, 512, lds_bytes, ls>>>(*op, sbi, i, yin, yout, lds_tiles, sh_q, lds_apt); \
+    } else if (vl >= 2) k_stage_shared<R, W, 4, NT, 2, true><<<sh_grid, 512, 0, ls>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+    else k_stage_shared<R, W, 4, NT, 1, true><<<sh_grid, 512, 0, ls>>>(*op, sbi, i, yin, yout, sh_tiles, sh_q); \
+  } while (0)
+#define HOPS_SHD_NT(W)                                                                           \
+  do {                                                                                           \
+    if (g_nt_stream >= 2) HOPS_SHD(W, 2);                                                        \
+    else HOPS_SHD(W, 0);                                                                         \
+  } while (0)
       if (dense) {
         if (int rc = launch_dense_h<R>(*op, sbi, G, yin, reinterpret_cast<cx<R> *>(sbi.hy_d), ls)) return rc;
         if (int rc = prof_mark(prof, ls, 3, &used)) return rc;
       }
-      if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
+      if (dense && sh_w == 4 && sh_p == 4) HOPS_SHD_NT(4);
+      else if (dense && sh_w == 8 && sh_p == 4) HOPS_SHD_NT(8);
+      else if (dense && sh_w == 16 && sh_p == 4) HOPS_SHD_NT(16);
+      else if (dense) k_stage<R><<<sb.n_tiles, 256, 0, ls>>>(*op, sbi, i, yin, yout);
+      else if (sh_w == 4 && sh_p == 4) HOPS_SH(4, 4);
       else if (sh_w == 4 && sh_p == 8) HOPS_SH(4, 8);
       else if (sh_w == 8 && sh_p == 4) HOPS_SH(8, 4);
       else if (sh_w == 8 && sh_p == 8) HOPS_SH(8, 8);
@@ -1794,6 +1824,8 @@ int propagate_t(const hops_op_t *op, const hops_batch_t *bt, int it0, int n_step
       else k_stage<R><<<sb.n_tiles, 256, 0, ls>>>(*op, sbi, i, yin, yout);
 #undef HOPS_SH
 #undef HOPS_SHV
+#undef HOPS_SHD
+#undef HOPS_SHD_NT
       HIP_TRY(hipGetLastError());
       if (int rc = prof_mark(prof, ls, 1, &used)) return rc;
     }

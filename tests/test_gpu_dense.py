@@ -176,3 +176,36 @@ def test_dense_plain_form_for_large_blocks_equals_sparse():
         err = np.max(np.abs(np.array(d.psi_traj[i]) - np.array(s.psi_traj[i])))
         print("plain dense form, traj %d: dense vs sparse %.3e" % (i, err))
         assert err < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_dense_stationary_forms_equal_streaming(prec):
+    """With eight trajectory groups of four lanes the stage runs in the
+    operator-stationary forms, which add the dense block's (-iH) Y in
+    registers: plain and LDS-staged, one, two and four lanes per thread (four
+    is served with two, the grid following), hint levels 0, 2 and 3 — every
+    form bit for bit the streaming kernel's result."""
+    _need_gpu()
+    from mesohops_amd import _lib
+    n = 70
+    sysp = systems.aggregate(n, seed_h=8)
+    tlen, tau, t_max, dt = 40.0, 1.0, 10.0, 2.0
+    B = 32
+    zs = [_noise(sysp, 300 + i, tlen, tau) for i in range(B)]
+    eom = "NORMALIZED NONLINEAR"
+    opts = {"stationary_stage": 1, "lds_stage": 0, "vec_lanes": 1, "nt_stream": 0}
+    try:
+        _lib.set_option("stationary_stage", 0)
+        ref = _run(sysp, eom, zs, tlen, tau, 1, t_max, dt, _psi0(n), prec=prec, lanes=4)
+        _lib.set_option("stationary_stage", 1)
+        for lds, vec, nt in ((0, 1, 0), (0, 2, 2), (1, 1, 0), (1, 2, 2), (0, 4, 2), (1, 4, 3)):
+            _lib.set_option("lds_stage", lds)
+            _lib.set_option("vec_lanes", vec)
+            _lib.set_option("nt_stream", nt)
+            print("case dense stationary %s lds=%d vec=%d nt=%d" % (prec, lds, vec, nt), flush=True)
+            got = _run(sysp, eom, zs, tlen, tau, 1, t_max, dt, _psi0(n), prec=prec, lanes=4)
+            for i in range(B):
+                assert np.array_equal(np.array(got.psi_traj[i]), np.array(ref.psi_traj[i])), (lds, vec, nt, i)
+    finally:
+        for k, v in opts.items():
+            _lib.set_option(k, v)
