@@ -321,10 +321,6 @@ def run_c4_stream(rate, window, prec="mixed", log=None):
     in_sys = [(t, w + a) for (t, _, w, a, _) in tl]
     mid = min(in_sys, key=lambda x: abs(x[0] - window / 2))
     end = in_sys[-1]
-    grew = end[1] - mid[1]
-    # bounded: no request waits for admission in the second half, and the
-    # number in the system grew by at most a tenth of the second half's
-    # arrivals (or 5 requests)
     stats["in_system_mid"], stats["in_system_end"] = mid[1], end[1]
     wait2 = max((w for (t, _, w, _, _) in tl if t >= mid[0]), default=0)
     stats["max_waiting_second_half"] = wait2

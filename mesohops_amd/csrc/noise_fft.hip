@@ -333,27 +333,35 @@ int set_lds(const void *fn, size_t bytes) {
   return 0;
 }
 
-// global scratch rows for transforms longer than the LDS holds (library-owned,
-// grown on demand, kept until the process ends)
+// scratch rows for transforms longer than the LDS holds, one buffer per
+// stream: transforms queued on different streams (host threads serving
+// different batches) never share rows, and those on one stream are ordered
+// by it.  Library-owned, grown on demand, kept until the process ends.
+struct Scratch {
+  double2 *p = nullptr;
+  size_t bytes = 0;
+};
 std::mutex g_scratch_mu;
-double2 *g_scratch = nullptr;
-size_t g_scratch_bytes = 0;
+std::map<hipStream_t, Scratch> g_scratch;
 
 int scratch_rows(size_t bytes, hipStream_t st, double2 **out) {
   std::lock_guard<std::mutex> lk(g_scratch_mu);
-  if (bytes > g_scratch_bytes) {
-    if (g_scratch) {
-      if (hipStreamSynchronize(st) != hipSuccess || hipFree(g_scratch) != hipSuccess)
+  Scratch &s = g_scratch[st];
+  if (bytes > s.bytes) {
+    if (s.p) {
+      // only this stream's work reads the old rows
+      if (hipStreamSynchronize(st) != hipSuccess || hipFree(s.p) != hipSuccess)
         return hops_internal_set_error("releasing the noise FFT scratch failed");
-      g_scratch = nullptr;
+      s.p = nullptr;
+      s.bytes = 0;
     }
-    if (hipMalloc(&g_scratch, bytes) != hipSuccess) {
-      g_scratch_bytes = 0;
+    if (hipMalloc(&s.p, bytes) != hipSuccess) {
+      s.p = nullptr;
       return hops_internal_set_error("hipMalloc of the noise FFT scratch failed");
     }
-    g_scratch_bytes = bytes;
+    s.bytes = bytes;
   }
-  *out = g_scratch;
+  *out = s.p;
   return 0;
 }
 
@@ -391,6 +399,8 @@ int make_plan(int n_t, hipStream_t st, Plan *out) {
     if (!big && set_lds((const void *)k_bluestein_filter, sizeof(double2) * p.M)) return -1;
     k_bluestein_filter<<<1, kThreads, big ? 0 : sizeof(double2) * p.M, st>>>(p.Bhat, p.N, p.M, big ? 1 : 0);
     if (hipGetLastError() != hipSuccess) return hops_internal_set_error("Bluestein filter launch failed");
+    // a plan is shared by every stream: publish it once its filter is written
+    if (hipStreamSynchronize(st) != hipSuccess) return hops_internal_set_error("Bluestein filter failed");
     g_plans[p.N] = p;
   }
   *out = p;

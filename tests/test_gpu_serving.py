@@ -107,3 +107,51 @@ def test_request_larger_than_arena_raises():
                      time_limit=120.0)
     finally:
         loop.close()
+
+
+def test_long_noise_filtered_on_concurrent_streams_equals_sequential():
+    """Noise of long requests (t_max 1.1-1.7 ps: 5,120- to 7,168-point
+    transforms done as 16,384-point Bluestein transforms, longer than the
+    LDS holds, so the filter works in device scratch rows) filtered
+    from four host threads on four streams at once, as the serving loop's
+    batch threads do at admission, equals the same noise filtered one
+    request at a time — bitwise."""
+    _need_gpu()
+    import ctypes
+    import threading
+    from mesohops_amd.basis import SystemInfo
+    from mesohops_amd.engine import _stream_ptr
+    from mesohops_amd.noise import HopsNoise, device_noise
+    from mesohops_amd.workloads import Request, chain_system
+    dev = torch.device("cuda:0")
+    reqs = [Request(rid=40 + i, arrival=0.0, nsite=n, maxhier=2, delta=1e-3, t_max=t)
+            for i, (n, t) in enumerate([(24, 1500.0), (7, 1100.0), (40, 1700.0), (15, 1500.0)])]
+    preps = []
+    for r in reqs:
+        sysp = chain_system(r.nsite)
+        info = SystemInfo(sysp)
+        nz = HopsNoise({"SEED": r.rid, "MODEL": "FFT_FILTER", "TLEN": r.tlen, "TAU": r.tau_noise}, info, sysp, 1)
+        n = 2 * (nz.n_t - 1)
+        assert n & (n - 1) and 2 * n - 1 > 8192  # Bluestein, over the LDS length
+        preps.append((nz, list(range(info.n_l2)), nz.raw()))
+    seq = [device_noise([nz], rows, dev, _stream_ptr(dev), raws=[raw])[0].cpu() for nz, rows, raw in preps]
+    for rnd in range(2):
+        out = [None] * len(preps)
+        go = threading.Barrier(len(preps))
+
+        def run(i):
+            s = torch.cuda.Stream(device=dev)
+            nz, rows, raw = preps[i]
+            go.wait()
+            with torch.cuda.stream(s):
+                z = device_noise([nz], rows, dev, ctypes.c_void_p(s.cuda_stream), raws=[raw])[0]
+                s.synchronize()
+            out[i] = z.cpu()
+
+        th = [threading.Thread(target=run, args=(i,)) for i in range(len(preps))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for i, (a, b) in enumerate(zip(seq, out)):
+            assert b is not None and torch.equal(a, b), (rnd, i)

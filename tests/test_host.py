@@ -264,3 +264,107 @@ def test_ragged_group_steps_every_batch_and_raises_after_all_finish():
         assert [p.thread for p in g.parts] == ["hops-batch-%d" % k for k in range(3)]
     finally:
         ragged.torch.cuda.synchronize = orig
+
+
+def _fake_serving(K, per_slots, capacity):
+    """A ServingLoop whose runtime batches are host fakes: a slot holds
+    ``est`` bytes at admission and grows by ``grow`` bytes a step; a step
+    that cannot place a slot's growth marks it out of room and raises
+    ArenaFull after stepping the others, as the runtime does."""
+    import threading
+    import types
+    from concurrent.futures import ThreadPoolExecutor
+    from mesohops_amd import serving
+    from mesohops_amd.errors import ArenaFull
+
+    class Part:
+        def __init__(self):
+            self.slots, self.next, self.peak, self.lock = {}, 0, 0, threading.Lock()
+            self.threads = set()
+
+        def free_bytes(self):
+            return capacity - sum(s["bytes"] for s in self.slots.values())
+
+        def estimate(self, est, n_state, n_hier):
+            return est
+
+        def add(self, r):
+            self.next += 1
+            self.slots[self.next] = {"bytes": r.est, "grow": r.grow, "oom": False}
+            self.peak = max(self.peak, len(self.slots))
+            return self.next
+
+        def step(self, n, slots=None):
+            self.threads.add(threading.current_thread().name)
+            oom = False
+            for s in slots:
+                sl = self.slots[s]
+                if self.free_bytes() < sl["grow"]:
+                    sl["oom"] = oom = True
+                else:
+                    sl["bytes"] += sl["grow"]
+            if oom:
+                raise ArenaFull("full")
+
+        def info(self, s):
+            sl = self.slots[s]
+            return {"oom": sl["oom"], "bytes": sl["bytes"], "n_state": 1, "n_hier": 1}
+
+        def remove(self, s):
+            del self.slots[s]
+
+    sl = serving.ServingLoop.__new__(serving.ServingLoop)
+    sl.K, sl.per_slots, sl.max_slots, sl.record, sl.reserve = K, per_slots, K * per_slots, False, (12, 4000)
+    sl.device = "cpu"
+    sl.rt = types.SimpleNamespace(parts=[Part() for _ in range(K)])
+    sl._prep = ThreadPoolExecutor(max_workers=2)
+    sl._prepare = lambda r: (None, None, r.est, None, None)
+    sl._admit = lambda r, prep, k: ((k, sl.rt.parts[k].add(r)), None)
+    return sl
+
+
+def _requests(n, est=10, grow=0, steps=5, gap=0.0):
+    import types
+    return [types.SimpleNamespace(rid=i, arrival=i * gap, n_steps=steps, nsite=4, maxhier=2, delta=1e-3, t_max=1.0,
+                                  est=est, grow=grow) for i in range(n)]
+
+
+def test_serving_batches_run_independently_and_finish_every_request(monkeypatch):
+    """ServingLoop.run's host logic, without a device: every request is
+    admitted, stepped exactly n_steps times and retired; no batch holds more
+    than its share of slots; each batch is stepped only by its own thread;
+    a request that outgrows the arena with others is preempted and served
+    again from the start."""
+    import contextlib
+    from mesohops_amd import serving
+    monkeypatch.setattr(serving.torch.cuda, "device", lambda d: contextlib.nullcontext())
+    sl = _fake_serving(K=3, per_slots=2, capacity=100)
+    reqs = _requests(20, est=10, steps=4, gap=0.001)
+    done, st = sl.run(reqs)
+    assert sorted(done) == list(range(20)) and all(d["steps"] == 4 for d in done.values())
+    assert st["completed"] == 20 and st["traj_steps"] == 80 and st["preempted"] == 0
+    assert all(p.peak <= 2 and not p.slots for p in sl.rt.parts)
+    assert all(p.threads <= {"hops-serve-%d" % k} for k, p in enumerate(sl.rt.parts))
+    sl._prep.shutdown()
+    # bases that grow: two slots of a batch cannot both reach 68 bytes
+    sl = _fake_serving(K=2, per_slots=2, capacity=100)
+    done, st = sl.run(_requests(6, est=20, grow=8, steps=6))
+    assert sorted(done) == list(range(6)) and all(d["steps"] == 6 for d in done.values())
+    assert st["preempted"] > 0 and st["traj_steps"] >= 36
+    sl._prep.shutdown()
+
+
+def test_serving_refuses_requests_no_batch_can_hold(monkeypatch):
+    import contextlib
+    from mesohops_amd import serving
+    from mesohops_amd.errors import UnsupportedRequest
+    monkeypatch.setattr(serving.torch.cuda, "device", lambda d: contextlib.nullcontext())
+    sl = _fake_serving(K=2, per_slots=2, capacity=100)
+    with pytest.raises(UnsupportedRequest, match="needs more device memory"):
+        sl.run(_requests(3, est=60))  # twice the footprint exceeds an empty batch
+    sl._prep.shutdown()
+    # alone in its batch and still outgrowing it: one restart, then refused
+    sl = _fake_serving(K=1, per_slots=1, capacity=100)
+    with pytest.raises(UnsupportedRequest, match="while running alone"):
+        sl.run(_requests(1, est=20, grow=30, steps=6))
+    sl._prep.shutdown()
