@@ -53,17 +53,17 @@ class ServingLoop:
     def __init__(self, precision_mode="mixed", arena_bytes=None, max_slots=256, n_threads=None, prep_workers=4,
                  eom="NORMALIZED NONLINEAR", record=False, device="cuda:0", system_fn=chain_system,
                  reserve_states=12, reserve_aux=4000, sub_batches=None):
-        """``sub_batches``: runtime batches stepped concurrently
-        (ragged.RaggedGroup; default one per 64 slots, at most 8), each
-        with its share of the arena and of the slots; ``n_threads``: worker
-        threads per batch (default 16 for one batch, else 32 / sub_batches)."""
-        K = max(1, min(8, max_slots // 64)) if sub_batches is None else int(sub_batches)
+        """``sub_batches``: runtime batches, each served by its own host
+        thread (ragged.RaggedGroup; default one per 32 slots, at most 16),
+        each with its share of the arena and of the slots; ``n_threads``:
+        worker threads per batch (default 16 / sub_batches, at least 1)."""
+        K = max(1, min(16, max_slots // 32)) if sub_batches is None else int(sub_batches)
         if K < 1:
             raise UnsupportedRequest("sub_batches < 1", "ServingLoop")
-        nthr = (16 if K == 1 else max(2, 32 // K)) if n_threads is None else int(n_threads)
+        nthr = max(1, 16 // K) if n_threads is None else int(n_threads)
         self.K = K
-        self.per_slots = -(-max_slots // K)
-        self.rt = RaggedGroup(K, precision_mode, arena_bytes, max_slots=self.per_slots, n_threads=nthr,
+        self.slot_cap = self.split_slots(max_slots, K)
+        self.rt = RaggedGroup(K, precision_mode, arena_bytes, max_slots=max(1, max(self.slot_cap)), n_threads=nthr,
                               device=device)
         self.device = self.rt.device
         self.max_slots = max_slots
@@ -73,6 +73,12 @@ class ServingLoop:
         self.reserve = (reserve_states, reserve_aux)
         self._systems = {}
         self._prep = ThreadPoolExecutor(max_workers=prep_workers)
+
+    @staticmethod
+    def split_slots(max_slots, K):
+        """Slots of each of the K runtime batches: max_slots split as evenly
+        as it goes, the first batches taking the remainder."""
+        return [max_slots // K + (1 if k < max_slots % K else 0) for k in range(K)]
 
     def _system(self, nsite):
         if nsite not in self._systems:
@@ -138,7 +144,7 @@ class ServingLoop:
             for r in itertools.islice(reqs, prep_ahead):  # host preparation, in arrival order
                 if r.rid not in futures:
                     futures[r.rid] = self._prep.submit(self._prepare, r)
-            if not reqs or reqs[0].arrival > now or n_mine >= self.per_slots:
+            if not reqs or reqs[0].arrival > now or n_mine >= self.slot_cap[k]:
                 return None
             r = reqs[0]
             fut = futures[r.rid]

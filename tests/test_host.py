@@ -266,7 +266,7 @@ def test_ragged_group_steps_every_batch_and_raises_after_all_finish():
         ragged.torch.cuda.synchronize = orig
 
 
-def _fake_serving(K, per_slots, capacity):
+def _fake_serving(K, max_slots, capacity):
     """A ServingLoop whose runtime batches are host fakes: a slot holds
     ``est`` bytes at admission and grows by ``grow`` bytes a step; a step
     that cannot place a slot's growth marks it out of room and raises
@@ -314,7 +314,8 @@ def _fake_serving(K, per_slots, capacity):
             del self.slots[s]
 
     sl = serving.ServingLoop.__new__(serving.ServingLoop)
-    sl.K, sl.per_slots, sl.max_slots, sl.record, sl.reserve = K, per_slots, K * per_slots, False, (12, 4000)
+    sl.K, sl.max_slots, sl.record, sl.reserve = K, max_slots, False, (12, 4000)
+    sl.slot_cap = serving.ServingLoop.split_slots(max_slots, K)
     sl.device = "cpu"
     sl.rt = types.SimpleNamespace(parts=[Part() for _ in range(K)])
     sl._prep = ThreadPoolExecutor(max_workers=2)
@@ -338,16 +339,19 @@ def test_serving_batches_run_independently_and_finish_every_request(monkeypatch)
     import contextlib
     from mesohops_amd import serving
     monkeypatch.setattr(serving.torch.cuda, "device", lambda d: contextlib.nullcontext())
-    sl = _fake_serving(K=3, per_slots=2, capacity=100)
+    assert serving.ServingLoop.split_slots(7, 3) == [3, 2, 2]
+    assert serving.ServingLoop.split_slots(512, 16) == [32] * 16
+    sl = _fake_serving(K=3, max_slots=7, capacity=100)
     reqs = _requests(20, est=10, steps=4, gap=0.001)
     done, st = sl.run(reqs)
     assert sorted(done) == list(range(20)) and all(d["steps"] == 4 for d in done.values())
     assert st["completed"] == 20 and st["traj_steps"] == 80 and st["preempted"] == 0
-    assert all(p.peak <= 2 and not p.slots for p in sl.rt.parts)
+    assert [p.peak <= c for p, c in zip(sl.rt.parts, [3, 2, 2])] == [True] * 3
+    assert not any(p.slots for p in sl.rt.parts)
     assert all(p.threads <= {"hops-serve-%d" % k} for k, p in enumerate(sl.rt.parts))
     sl._prep.shutdown()
     # bases that grow: two slots of a batch cannot both reach 68 bytes
-    sl = _fake_serving(K=2, per_slots=2, capacity=100)
+    sl = _fake_serving(K=2, max_slots=4, capacity=100)
     done, st = sl.run(_requests(6, est=20, grow=8, steps=6))
     assert sorted(done) == list(range(6)) and all(d["steps"] == 6 for d in done.values())
     assert st["preempted"] > 0 and st["traj_steps"] >= 36
@@ -359,12 +363,12 @@ def test_serving_refuses_requests_no_batch_can_hold(monkeypatch):
     from mesohops_amd import serving
     from mesohops_amd.errors import UnsupportedRequest
     monkeypatch.setattr(serving.torch.cuda, "device", lambda d: contextlib.nullcontext())
-    sl = _fake_serving(K=2, per_slots=2, capacity=100)
+    sl = _fake_serving(K=2, max_slots=4, capacity=100)
     with pytest.raises(UnsupportedRequest, match="needs more device memory"):
         sl.run(_requests(3, est=60))  # twice the footprint exceeds an empty batch
     sl._prep.shutdown()
     # alone in its batch and still outgrowing it: one restart, then refused
-    sl = _fake_serving(K=1, per_slots=1, capacity=100)
+    sl = _fake_serving(K=1, max_slots=1, capacity=100)
     with pytest.raises(UnsupportedRequest, match="while running alone"):
         sl.run(_requests(1, est=20, grow=30, steps=6))
     sl._prep.shutdown()
