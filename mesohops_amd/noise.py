@@ -208,7 +208,13 @@ def device_noise(noises, rows, device, stream_ptr, raws=None, c128=False):
                 check(filt(ctypes.c_void_p(z_d.data_ptr()), (b1 - b0) * nr, nt,
                            ctypes.c_void_p(s_w.data_ptr()), ctypes.c_void_p(nu.data_ptr()),
                            ctypes.c_void_p(row_lop.data_ptr()), ctypes.c_void_p(out[b0:b1].data_ptr()), stream_ptr))
-                torch.cuda.synchronize(device)  # the filter has read z_d before it is freed
+                # the filter has read z_d before it is freed: wait for the
+                # filter's stream alone when torch queues on it too
+                cur = torch.cuda.current_stream(device)
+                if cur.cuda_stream == (stream_ptr.value or 0):
+                    cur.synchronize()
+                else:
+                    torch.cuda.synchronize(device)
                 del z_d
     if nz.param["FLAG_REAL"]:
         torch.view_as_real(out)[..., 1] = 0

@@ -50,7 +50,7 @@ _EOM = {"LINEAR": _lib.HOPS_EOM_LINEAR, "NONLINEAR": _lib.HOPS_EOM_NONLINEAR,
 
 
 class ServingLoop:
-    def __init__(self, precision_mode="mixed", arena_bytes=None, max_slots=256, n_threads=None, prep_workers=4,
+    def __init__(self, precision_mode="mixed", arena_bytes=None, max_slots=256, n_threads=None, prep_workers=8,
                  eom="NORMALIZED NONLINEAR", record=False, device="cuda:0", system_fn=chain_system,
                  reserve_states=12, reserve_aux=4000, sub_batches=None):
         """``sub_batches``: runtime batches, each served by its own host
