@@ -170,6 +170,18 @@ class ServingLoop:
         def serve(k):
             part = self.rt.parts[k]
             mine = {}        # slot -> [req, steps_done, keep, t_admit, peak device bytes]
+            try:
+                serve_batch(k, part, mine)
+            finally:
+                # requests still running when the loop stops (time limit or
+                # error) leave the batch, which is then ready for another run
+                for s in list(mine):
+                    try:
+                        part.remove(s)
+                    except Exception:  # the first error is the one reported
+                        pass
+
+        def serve_batch(k, part, mine):
             while not stop.is_set():
                 now = clock()
                 if time_limit is not None and now > time_limit:

@@ -356,6 +356,12 @@ def test_serving_batches_run_independently_and_finish_every_request(monkeypatch)
     assert sorted(done) == list(range(6)) and all(d["steps"] == 6 for d in done.values())
     assert st["preempted"] > 0 and st["traj_steps"] >= 36
     sl._prep.shutdown()
+    # stopped by the time limit: unfinished requests leave their batches
+    sl = _fake_serving(K=2, max_slots=4, capacity=100)
+    done, st = sl.run(_requests(4, est=10, steps=10 ** 9), time_limit=0.2)
+    assert not done and st["admitted_unfinished"] == 4
+    assert not any(p.slots for p in sl.rt.parts)
+    sl._prep.shutdown()
 
 
 def test_serving_refuses_requests_no_batch_can_hold(monkeypatch):
