@@ -254,7 +254,11 @@ int hops_noise_embedding(const float *ang_d, const float *mag_d, int n_lop, int 
 
 /* Correlated noise for R rows: out[r] = c64( unrotate( IFFT( FFT(z[r]) *
    sqrt(s_w[lop(r)]/2) )[:n_t] ) ).  z_d: complex64 [R][2(n_t-1)];
-   row_lop_d: int32 [R] row -> embedding row; out_d: complex64 [R][n_t]. */
+   row_lop_d: int32 [R] row -> embedding row; out_d: complex64 [R][n_t].
+   The embedding and both filters may be called from several host threads
+   on different streams at once: transforms longer than the LDS holds work
+   in library-owned scratch rows kept per stream, and the shared Bluestein
+   plans are published only once written. */
 int hops_noise_filter(const void *z_d, int R, int n_t, const double *s_w_d,
                       const double *nu_d, const int32_t *row_lop_d, void *out_d,
                       void *stream);
