@@ -60,6 +60,7 @@ class ServingLoop:
         K = max(1, min(16, max_slots // 32)) if sub_batches is None else int(sub_batches)
         if K < 1:
             raise UnsupportedRequest("sub_batches < 1", "ServingLoop")
+        K = min(K, max(1, max_slots))  # no batch without a slot
         nthr = max(1, 16 // K) if n_threads is None else int(n_threads)
         self.K = K
         self.slot_cap = self.split_slots(max_slots, K)

@@ -22,15 +22,17 @@ def _requests():
             Request(rid=14, arrival=0.05, nsite=12, maxhier=3, delta=8e-3, t_max=15.0)]
 
 
-@pytest.mark.parametrize("prec,sub", [("fp64", 1), ("mixed", 1), ("mixed", 3)])
+@pytest.mark.parametrize("prec,sub", [("fp64", 1), ("mixed", 1), ("mixed", 3), ("mixed", 16)])
 def test_alone_equals_in_mix(prec, sub):
-    """sub: runtime batches stepped concurrently (3: the four requests spread
-    over three batches, each with its own stream and worker threads)."""
+    """sub: runtime batches, each served by its own host thread (3: the four
+    requests spread over three batches, each with its own stream and worker
+    threads; 16: clamped to the 9 slots, one slot and one worker thread per
+    batch)."""
     _need_gpu()
     from mesohops_amd.serving import ServingLoop
     reqs = _requests()
     loop = ServingLoop(precision_mode=prec, arena_bytes=1 << 30, max_slots=9, record=True, sub_batches=sub)
-    assert loop.K == sub
+    assert loop.K == min(sub, 9) and loop.slot_cap == ServingLoop.split_slots(9, loop.K)
     mix, stats = loop.run(reqs)
     loop.close()
     print(prec, "sub-batches", sub, "mix stats:", stats)
