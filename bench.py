@@ -791,6 +791,7 @@ def main():
             "traj_steps_per_s": st["traj_steps_per_s"], "arrived": st["arrived"], "completed": st["completed"],
             "running_at_end": st["admitted_unfinished"], "waiting_at_end": st["waiting_at_end"],
             "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
+            "runtime_batches": st["runtime_batches"], "threads_per_batch": st["threads_per_batch"],
             "timeline (t, arrived, waiting, running, done)": st["timeline"],
             "note": "200/s exceeds what one GPU completes (the rate sweep of --workload c4 --c4-window 60 "
                     "finds the largest rate it keeps up with); latencies are those of the requests completed "

@@ -63,6 +63,7 @@ class ServingLoop:
         K = min(K, max(1, max_slots))  # no batch without a slot
         nthr = max(1, 16 // K) if n_threads is None else int(n_threads)
         self.K = K
+        self.n_threads = nthr
         self.slot_cap = self.split_slots(max_slots, K)
         self.rt = RaggedGroup(K, precision_mode, arena_bytes, max_slots=max(1, max(self.slot_cap)), n_threads=nthr,
                               device=device)
@@ -307,6 +308,7 @@ class ServingLoop:
                  "wall_s": wall, "traj_steps": cnt["steps"],
                  "traj_steps_per_s": cnt["steps"] / wall if wall else 0.0,
                  "iterations": cnt["iter"], "preempted": cnt["preempt"],
+                 "runtime_batches": self.K, "threads_per_batch": self.n_threads,
                  "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
                  "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None}
         return done, stats

@@ -314,7 +314,7 @@ def _fake_serving(K, max_slots, capacity):
             del self.slots[s]
 
     sl = serving.ServingLoop.__new__(serving.ServingLoop)
-    sl.K, sl.max_slots, sl.record, sl.reserve = K, max_slots, False, (12, 4000)
+    sl.K, sl.n_threads, sl.max_slots, sl.record, sl.reserve = K, 1, max_slots, False, (12, 4000)
     sl.slot_cap = serving.ServingLoop.split_slots(max_slots, K)
     sl.device = "cpu"
     sl.rt = types.SimpleNamespace(parts=[Part() for _ in range(K)])
